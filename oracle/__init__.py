@@ -1,0 +1,31 @@
+"""CPU oracle for the WarpGBM training / inference hot path.
+
+TEST INFRASTRUCTURE ONLY.  Only ``tests/``, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of ``bench.py`` may import this package, and only as the
+checker (or as the timed CPU baseline).  The product path (``warpgbm_amd``)
+never imports it and fails loudly when its HIP library is missing.
+
+``gbm_oracle`` is an FP64 NumPy restatement of the reference's kernel and
+host semantics (each function cites the reference file:line it follows).
+Bin edges and bin codes are defined in float32, bit for bit; everything
+else (histograms, gains, leaf values, predictions) is carried in float64.
+
+Pinning status of each function (what checks it besides itself):
+  linspace_f32 / quantile_edges_f32 -- torch.linspace / torch.quantile (CPU,
+      float32) over every num_bins 2..127 and several N, NaN and ties.
+  bin_codes                         -- np.searchsorted(edges, x, 'right').
+  histogram                         -- np.bincount per feature/era, and the
+      invariants sum_b G = sum_rows g, parent = left + right.
+  split_gains                       -- brute force: gain recomputed from row
+      subsets by direct summation on tiny inputs.
+  select_split                      -- brute-force enumeration of all (f, b)
+      with the selection rule written as a plain Python loop; hand-built
+      special cases (conflicting eras -> leaf, mcw > n -> leaf).
+  grow_tree / partition / leaf      -- row->leaf assignment cross-checked by
+      traversing the produced tree with predict_tree; leaf = closed form
+      mean of g over h on each leaf's rows.
+  flatten_tree / predict_forest     -- traversal of the nested dict vs the
+      flattened array; chunk invariance.
+  softmax_grad_hess                 -- closed form on 2-class inputs
+      (sigmoid) and rows of the softmax summing to 1.
+"""

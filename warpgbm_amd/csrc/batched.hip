@@ -1,0 +1,439 @@
+// Batched level-wise boosting on one stream (gfx950).
+//
+// One call trains n_rounds boosting rounds with no host synchronisation:
+// per round the objective kernel writes (g, h) in root order, then for every
+// tree level d = 0..D the level's kernels run for ALL frontier nodes of ALL
+// K class trees at once:
+//   histogram of the smaller children (one launch; root: every era of every
+//   class) -> sibling = parent - built -> fused gain scan + DES selection ->
+//   leaf values and score updates of the level's leaves -> partition of the
+//   split nodes into the next level.
+// The per-node decisions and leaf values are recorded into level-order
+// (heap id) node tables, from which the host builds the reference's nested
+// tree dicts once training returns.
+//
+// Trees are identical to the plain per-node path: both use the same
+// histogram kernel (same chunking), the same gain/selection arithmetic, the
+// same stable partition and the same leaf reduction.  The reference's DFS
+// growth (warpgbm/core.py:1364-1470) freezes g/h per round (:1545-1547), so
+// level order and DFS order produce the same tree.
+#include "common.h"
+#include "wgbm_internal.h"
+#include "wgbm_level.h"
+#include "../../include/warpgbm_amd.h"
+
+namespace wgbm {
+
+struct FitLayout {
+  size_t total = 0;
+  // offsets
+  size_t perm_root, gh_root, perm_pp[2], gh_pp[2];
+  size_t hg[2], hh[2];
+  struct Tables { size_t seg_start, seg_count, node_active, node_start, node_count; } tab[2];
+  size_t dec_feat, dec_bin, era_gain, cand_agree, cand_crit, cand_idx;
+  size_t b_seg_start, b_seg_count, b_seg_slot, n_build, sib, n_split;
+  size_t h_seg_first, h_seg_pfirst, h_nseg, h_multi, h_n_multi, h_part;
+  size_t p_pcount, p_seg_first, p_nseg, p_multi, p_n_multi, p_lbase, p_rbase, p_segL;
+  size_t l_lcount, l_seg_first, l_nseg, l_multi, l_n_multi, l_part, l_val;
+  size_t root_seg_start, root_seg_count, root_seg_slot, root_nseg, era_start;
+  // sizes
+  int64_t KN, max_nodes, node_elems, hist_nodes, part_chunks, leaf_chunks, hist_part_floats;
+  HistConfig hc;
+};
+
+static HistConfig fit_hist_config(int n_feat, int n_bins, int64_t n_cols) {
+  return pick_hist_config(n_feat, n_bins, n_cols);
+}
+
+static FitLayout fit_layout(const wgbm_fit_config& c) {
+  FitLayout L;
+  auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+  size_t o = 0;
+  auto take = [&](size_t bytes) { const size_t r = o; o = al(o + bytes); return r; };
+  const int64_t K = c.n_classes, N = c.n_rows, E = c.n_eras, D = c.max_depth;
+  L.KN = K * N;
+  L.max_nodes = K << D;                       // nodes of the deepest level
+  L.node_elems = E * (int64_t)c.n_feat * c.n_bins;
+  L.hist_nodes = D >= 1 ? (K << (D - 1)) : K;  // deepest level with histograms
+  L.hc = fit_hist_config(c.n_feat, c.n_bins, c.n_cols);
+  L.perm_root = take(4 * L.KN);
+  L.gh_root = take(8 * L.KN);
+  for (int i = 0; i < 2; ++i) { L.perm_pp[i] = take(4 * L.KN); L.gh_pp[i] = take(8 * L.KN); }
+  for (int i = 0; i < 2; ++i) {
+    L.hg[i] = take(4 * L.hist_nodes * L.node_elems);
+    L.hh[i] = take(4 * L.hist_nodes * L.node_elems);
+  }
+  const int64_t MS = L.max_nodes * E;
+  for (int i = 0; i < 2; ++i) {
+    L.tab[i].seg_start = take(4 * MS);
+    L.tab[i].seg_count = take(4 * MS);
+    L.tab[i].node_active = take(4 * L.max_nodes);
+    L.tab[i].node_start = take(4 * L.max_nodes);
+    L.tab[i].node_count = take(4 * L.max_nodes);
+  }
+  const int64_t wgpn = ceil_div(c.n_feat, 256) > 0 ? ceil_div(c.n_feat, 256) : 1;
+  L.dec_feat = take(4 * L.max_nodes);
+  L.dec_bin = take(4 * L.max_nodes);
+  L.era_gain = take(4 * MS);
+  L.cand_agree = take(4 * L.max_nodes * wgpn);
+  L.cand_crit = take(4 * L.max_nodes * wgpn);
+  L.cand_idx = take(4 * L.max_nodes * wgpn);
+  L.b_seg_start = take(4 * MS);
+  L.b_seg_count = take(4 * MS);
+  L.b_seg_slot = take(4 * MS);
+  L.n_build = take(4);
+  L.sib = take(4 * 3 * L.max_nodes);
+  L.n_split = take(4);
+  L.h_seg_first = take(4 * (MS + 1));
+  L.h_seg_pfirst = take(4 * (MS + 1));
+  L.h_nseg = take(4);
+  L.h_multi = take(4 * (L.KN / (L.hc.chunk_rows + 1) + 2));
+  L.h_n_multi = take(4);
+  L.hist_part_floats = (int64_t)hist_partial_floats(L.hc, c.n_feat, c.n_bins, L.KN);
+  L.h_part = take(2 * 4 * L.hist_part_floats);
+  L.part_chunks = ceil_div(L.KN, kPartChunk) + MS;
+  L.p_pcount = take(4 * MS);
+  L.p_seg_first = take(4 * (MS + 1));
+  L.p_nseg = take(4);
+  L.p_multi = take(4 * (L.KN / (kPartChunk + 1) + 2));
+  L.p_n_multi = take(4);
+  L.p_lbase = take(4 * L.part_chunks);
+  L.p_rbase = take(4 * L.part_chunks);
+  L.p_segL = take(4 * MS);
+  L.leaf_chunks = leaf_max_chunks(L.KN, L.max_nodes);
+  L.l_lcount = take(4 * L.max_nodes);
+  L.l_seg_first = take(4 * (L.max_nodes + 1));
+  L.l_nseg = take(4);
+  L.l_multi = take(4 * (L.leaf_chunks + 1));
+  L.l_n_multi = take(4);
+  L.l_part = take(8 * 2 * L.leaf_chunks);
+  L.l_val = take(4 * L.max_nodes);
+  L.root_seg_start = take(4 * K * E);
+  L.root_seg_count = take(4 * K * E);
+  L.root_seg_slot = take(4 * K * E);
+  L.root_nseg = take(4);
+  L.era_start = take(4 * E);
+  L.total = o;
+  return L;
+}
+
+// Root tables: class c occupies positions [c*N, (c+1)*N) in root order.
+__global__ void k_root_tables(const int32_t* __restrict__ era_count, int E, int K, int N,
+                              int32_t* era_start, int32_t* seg_start, int32_t* seg_count, int32_t* seg_slot,
+                              int32_t* nseg, int32_t* t_seg_start, int32_t* t_seg_count, int32_t* t_active,
+                              int32_t* t_node_start, int32_t* t_node_count) {
+  if (threadIdx.x != 0) return;
+  int run = 0;
+  for (int e = 0; e < E; ++e) { era_start[e] = run; run += era_count[e]; }
+  for (int c = 0; c < K; ++c) {
+    for (int e = 0; e < E; ++e) {
+      const int s = c * E + e;
+      seg_start[s] = c * N + era_start[e];
+      seg_count[s] = era_count[e];
+      seg_slot[s] = s;
+      t_seg_start[s] = seg_start[s];
+      t_seg_count[s] = seg_count[s];
+    }
+    t_active[c] = 1;
+    t_node_start[c] = c * N;
+    t_node_count[c] = run;
+  }
+  *nseg = K * E;
+}
+
+__global__ void __launch_bounds__(256)
+k_root_perm(const int32_t* __restrict__ order, int64_t N, int K, int32_t* __restrict__ perm) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N * K) return;
+  const int64_t p = i % N;
+  perm[i] = order ? order[p] : (int32_t)p;
+}
+
+__global__ void __launch_bounds__(256)
+k_grad_regression_pos(const float* __restrict__ y, const float* __restrict__ pred, const float* __restrict__ w,
+                      int64_t n, const int32_t* __restrict__ perm0, float2* __restrict__ gh) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int64_t r = perm0[p];
+  const float wr = w[r];
+  gh[p] = make_float2((y[r] - pred[r]) * wr, wr);
+}
+
+__global__ void __launch_bounds__(256)
+k_grad_softmax_pos(const float* __restrict__ F, const int32_t* __restrict__ y, const float* __restrict__ w,
+                   int64_t n, int K, const int32_t* __restrict__ perm0, float2* __restrict__ gh) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int64_t r = perm0[p];
+  const float* f = F + r * K;
+  float mx = f[0];
+  for (int k = 1; k < K; ++k) mx = fmaxf(mx, f[k]);
+  float den = 0.0f;
+  for (int k = 0; k < K; ++k) den += expf(f[k] - mx);
+  const float wr = w[r];
+  const int yr = y[r];
+  for (int k = 0; k < K; ++k) {
+    const float pk = expf(f[k] - mx) / den;
+    gh[(int64_t)k * n + p] = make_float2(-(pk - (yr == k ? 1.0f : 0.0f)) * wr, fmaxf(pk * (1.0f - pk), 1e-6f) * wr);
+  }
+}
+
+// Record level d of every class tree into the round's node tables.
+__global__ void __launch_bounds__(256)
+k_record_level(int d, int D, int K, int E, int n_nodes, const int32_t* __restrict__ active,
+               const int32_t* __restrict__ dec_feat, const int32_t* __restrict__ dec_bin,
+               const int32_t* __restrict__ feat_idx, const float* __restrict__ leaf_val,
+               const int32_t* __restrict__ node_count, const float* __restrict__ era_gain, int32_t* status,
+               int32_t* feat, int32_t* bin, float* value, int32_t* count, float* out_era_gain) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n_nodes) return;
+  const int NH = (2 << D) - 1;
+  const int c = j >> d, jj = j & ((1 << d) - 1);
+  const int64_t o = (int64_t)c * NH + ((1 << d) - 1) + jj;
+  int st = 0, f = -1, b = -1;
+  float v = 0.0f;
+  if (active[j]) {
+    if (d < D && dec_feat[j] >= 0) {
+      st = 1;
+      f = feat_idx ? feat_idx[dec_feat[j]] : dec_feat[j];
+      b = dec_bin[j];
+    } else {
+      st = 2;
+      v = leaf_val[j];
+    }
+  }
+  status[o] = st;
+  feat[o] = f;
+  bin[o] = b;
+  value[o] = v;
+  count[o] = active[j] ? node_count[j] : 0;
+  if (out_era_gain)
+    for (int e = 0; e < E; ++e)
+      out_era_gain[o * E + e] = (st == 1) ? era_gain[(int64_t)j * E + e] : 0.0f;
+}
+
+}  // namespace wgbm
+
+using namespace wgbm;
+
+extern "C" int wgbm_set_hist_config(int32_t chunk_rows, int32_t waves, int32_t features_per_lane) {
+  WGBM_REQUIRE(chunk_rows >= 0 && (waves == 0 || waves == 1 || waves == 2 || waves == 4),
+               "bad histogram configuration");
+  WGBM_REQUIRE(features_per_lane == 0 || features_per_lane == 1 || features_per_lane == 2 ||
+                   features_per_lane == 4 || features_per_lane == 8 || features_per_lane == 16,
+               "features_per_lane must be 0 (auto), 1, 2, 4, 8 or 16");
+  HistConfig c;
+  c.chunk_rows = chunk_rows;
+  c.waves = waves;
+  c.fpl = features_per_lane;
+  set_hist_override(c);
+  return kOk;
+}
+
+extern "C" int wgbm_get_hist_config(int32_t n_feat, int32_t n_bins, int64_t n_cols, int32_t* out3) {
+  const HistConfig c = fit_hist_config(n_feat, n_bins, n_cols);
+  out3[0] = c.chunk_rows;
+  out3[1] = c.waves;
+  out3[2] = c.fpl;
+  return kOk;
+}
+
+static int check_config(const wgbm_fit_config* c) {
+  WGBM_REQUIRE(c != nullptr, "null config");
+  WGBM_REQUIRE(c->n_rows >= 1 && c->n_rows * (int64_t)c->n_classes < 0x7fffffff, "row count out of range");
+  WGBM_REQUIRE(c->n_feat >= 1 && c->n_cols >= 1, "feature count out of range");
+  WGBM_REQUIRE(c->n_bins >= 2 && c->n_bins <= 127, "num_bins must be in [2, 127]");
+  WGBM_REQUIRE(c->n_eras >= 1, "era count must be positive");
+  WGBM_REQUIRE(c->n_classes >= 1 && c->n_classes <= 256, "class count out of range");
+  WGBM_REQUIRE(c->max_depth >= 1 && c->max_depth <= 20, "max_depth out of range");
+  WGBM_REQUIRE((int64_t)c->n_classes << c->max_depth < (1 << 24), "tree too large for level tables");
+  WGBM_REQUIRE(c->objective == 0 || c->objective == 1, "unknown objective %d", c->objective);
+  WGBM_REQUIRE(c->objective != 0 || c->n_classes == 1, "regression uses one channel");
+  return kOk;
+}
+
+extern "C" size_t wgbm_fit_workspace_bytes(const wgbm_fit_config* c) {
+  if (check_config(c) != kOk) return 0;
+  return fit_layout(*c).total;
+}
+
+extern "C" int wgbm_fit_rounds(const wgbm_fit_config* cfg, const wgbm_fit_io* io, int32_t n_rounds,
+                               void* workspace, size_t workspace_bytes, hipStream_t st) {
+  int rc = check_config(cfg);
+  if (rc != kOk) return rc;
+  const wgbm_fit_config& c = *cfg;
+  const FitLayout L = fit_layout(c);
+  WGBM_REQUIRE(workspace_bytes >= L.total, "fit workspace too small: need %zu bytes", L.total);
+  WGBM_REQUIRE(io && io->codes && io->weight && io->scores && io->era_count, "missing fit inputs");
+  WGBM_REQUIRE(c.objective == 1 ? io->labels != nullptr : io->y != nullptr, "missing targets");
+  WGBM_REQUIRE(io->feat_idx != nullptr || c.n_feat <= c.n_cols, "feature subset exceeds the bin matrix");
+  char* ws = static_cast<char*>(workspace);
+  auto I = [&](size_t off) { return reinterpret_cast<int32_t*>(ws + off); };
+  auto Fp = [&](size_t off) { return reinterpret_cast<float*>(ws + off); };
+  auto G2 = [&](size_t off) { return reinterpret_cast<float2*>(ws + off); };
+  const int K = c.n_classes, E = c.n_eras, D = c.max_depth;
+  const int64_t N = c.n_rows;
+  const int NH = (2 << D) - 1;
+
+  hipLaunchKernelGGL(k_root_perm, dim3((unsigned)ceil_div(L.KN, 256)), dim3(256), 0, st, io->root_order, N, K,
+                     I(L.perm_root));
+  WGBM_LAUNCHED("k_root_perm");
+
+  for (int r = 0; r < n_rounds; ++r) {
+    const int32_t* fidx = io->feat_idx ? io->feat_idx + (int64_t)r * c.n_feat : nullptr;
+    // level-0 tables (rewritten every round: partition overwrites table 0 at level 1)
+    hipLaunchKernelGGL(k_root_tables, dim3(1), dim3(64), 0, st, io->era_count, E, K, (int)N, I(L.era_start),
+                       I(L.root_seg_start), I(L.root_seg_count), I(L.root_seg_slot), I(L.root_nseg),
+                       I(L.tab[0].seg_start), I(L.tab[0].seg_count), I(L.tab[0].node_active),
+                       I(L.tab[0].node_start), I(L.tab[0].node_count));
+    WGBM_LAUNCHED("k_root_tables");
+    if (c.objective == 0) {
+      hipLaunchKernelGGL(k_grad_regression_pos, dim3((unsigned)ceil_div(N, 256)), dim3(256), 0, st, io->y,
+                         io->scores, io->weight, N, I(L.perm_root), G2(L.gh_root));
+      WGBM_LAUNCHED("k_grad_regression_pos");
+    } else {
+      hipLaunchKernelGGL(k_grad_softmax_pos, dim3((unsigned)ceil_div(N, 256)), dim3(256), 0, st, io->scores,
+                         io->labels, io->weight, N, K, I(L.perm_root), G2(L.gh_root));
+      WGBM_LAUNCHED("k_grad_softmax_pos");
+    }
+    const int64_t out_base = (int64_t)r * K * NH;
+    for (int d = 0; d <= D; ++d) {
+      const int n_nodes = K << d;
+      const FitLayout::Tables& cur = L.tab[d & 1];
+      const FitLayout::Tables& nxt = L.tab[(d + 1) & 1];
+      const int32_t* perm_d = d == 0 ? I(L.perm_root) : I(L.perm_pp[d & 1]);
+      const float2* gh_d = d == 0 ? G2(L.gh_root) : G2(L.gh_pp[d & 1]);
+      float* hg = Fp(L.hg[d & 1]);
+      float* hh = Fp(L.hh[d & 1]);
+      if (d < D) {
+        HistArgs a{};
+        a.codes = io->codes;
+        a.row_stride = c.n_cols;
+        a.n_rows = N;
+        a.feat_idx = fidx;
+        a.n_feat = c.n_feat;
+        a.n_bins = c.n_bins;
+        a.perm = perm_d;
+        a.gh = gh_d;
+        a.seg_first = I(L.h_seg_first);
+        a.seg_pfirst = I(L.h_seg_pfirst);
+        a.out_g = hg;
+        a.out_h = hh;
+        a.part_g = Fp(L.h_part);
+        a.part_h = Fp(L.h_part) + L.hist_part_floats;
+        a.accumulate = 0;
+        if (d == 0) {
+          a.seg_start = I(L.root_seg_start);
+          a.seg_count = I(L.root_seg_count);
+          a.seg_slot = I(L.root_seg_slot);
+          a.nseg_dev = I(L.root_nseg);
+          rc = hist_launch(a, L.hc, L.KN, (int64_t)K * E, I(L.h_multi), I(L.h_n_multi), st);
+        } else {
+          a.seg_start = I(L.b_seg_start);
+          a.seg_count = I(L.b_seg_count);
+          a.seg_slot = I(L.b_seg_slot);
+          a.nseg_dev = I(L.n_build);
+          rc = hist_launch(a, L.hc, L.KN / 2 + 1, ((int64_t)K << (d - 1)) * E, I(L.h_multi), I(L.h_n_multi), st);
+          if (rc != kOk) return rc;
+          rc = subtract_launch(Fp(L.hg[(d - 1) & 1]), Fp(L.hh[(d - 1) & 1]), hg, hh, I(L.sib), I(L.n_split),
+                               (int64_t)K << (d - 1), L.node_elems, st);
+        }
+        if (rc != kOk) return rc;
+        LevelScanArgs s{};
+        s.G = hg;
+        s.H = hh;
+        s.n_nodes = n_nodes;
+        s.n_eras = E;
+        s.n_feat = c.n_feat;
+        s.n_bins = c.n_bins;
+        s.node_active = I(cur.node_active);
+        s.feat_idx = fidx;
+        s.mono_req = io->mono_required;
+        s.msg = c.min_split_gain;
+        s.mcw = c.min_child_weight;
+        s.lam = c.lambda;
+        s.cand_agree = I(L.cand_agree);
+        s.cand_crit = Fp(L.cand_crit);
+        s.cand_idx = I(L.cand_idx);
+        s.dec_feat = I(L.dec_feat);
+        s.dec_bin = I(L.dec_bin);
+        s.era_gain = Fp(L.era_gain);
+        rc = level_scan_launch(s, st);
+        if (rc != kOk) return rc;
+      }
+      LeafArgs la{};
+      la.n_nodes = n_nodes;
+      la.node_active = I(cur.node_active);
+      la.dec_feat = d < D ? I(L.dec_feat) : nullptr;
+      la.node_start = I(cur.node_start);
+      la.node_count = I(cur.node_count);
+      la.lcount = I(L.l_lcount);
+      la.seg_first = I(L.l_seg_first);
+      la.nseg_dev = I(L.l_nseg);
+      la.multi = I(L.l_multi);
+      la.n_multi = I(L.l_n_multi);
+      la.perm = perm_d;
+      la.gh = gh_d;
+      la.n_rows = N;
+      la.lam = c.lambda;
+      la.lr = c.learning_rate;
+      la.pred = io->scores;
+      la.pred_stride = K;
+      la.nodes_per_class = 1 << d;
+      la.leaf_val = Fp(L.l_val);
+      la.node_sum = nullptr;
+      la.part = reinterpret_cast<double*>(ws + L.l_part);
+      rc = leaf_launch(la, L.KN, st);
+      if (rc != kOk) return rc;
+      hipLaunchKernelGGL(k_record_level, dim3((unsigned)ceil_div(n_nodes, 256)), dim3(256), 0, st, d, D, K, E,
+                         n_nodes, I(cur.node_active), I(L.dec_feat), I(L.dec_bin), fidx, Fp(L.l_val),
+                         I(cur.node_count), Fp(L.era_gain), io->node_status + out_base, io->node_feat + out_base,
+                         io->node_bin + out_base, io->node_value + out_base, io->node_count + out_base,
+                         io->node_era_gain ? io->node_era_gain + out_base * E : nullptr);
+      WGBM_LAUNCHED("k_record_level");
+      if (d < D) {
+        PartArgs p{};
+        p.codes = io->codes;
+        p.row_stride = c.n_cols;
+        p.n_rows = N;
+        p.feat_idx = fidx;
+        p.n_nodes = n_nodes;
+        p.n_eras = E;
+        p.node_active = I(cur.node_active);
+        p.dec_feat = I(L.dec_feat);
+        p.dec_bin = I(L.dec_bin);
+        p.seg_start = I(cur.seg_start);
+        p.seg_count = I(cur.seg_count);
+        p.node_start = I(cur.node_start);
+        p.node_count = I(cur.node_count);
+        p.perm_in = perm_d;
+        p.gh_in = gh_d;
+        p.perm_out = I(L.perm_pp[(d + 1) & 1]);
+        p.gh_out = G2(L.gh_pp[(d + 1) & 1]);
+        p.pcount = I(L.p_pcount);
+        p.seg_first = I(L.p_seg_first);
+        p.nseg_dev = I(L.p_nseg);
+        p.multi = I(L.p_multi);
+        p.n_multi = I(L.p_n_multi);
+        p.lbase = I(L.p_lbase);
+        p.rbase = I(L.p_rbase);
+        p.segL = I(L.p_segL);
+        p.c_seg_start = I(nxt.seg_start);
+        p.c_seg_count = I(nxt.seg_count);
+        p.c_active = I(nxt.node_active);
+        p.c_node_start = I(nxt.node_start);
+        p.c_node_count = I(nxt.node_count);
+        p.b_seg_start = I(L.b_seg_start);
+        p.b_seg_count = I(L.b_seg_count);
+        p.b_seg_slot = I(L.b_seg_slot);
+        p.n_build = I(L.n_build);
+        p.sib = I(L.sib);
+        p.n_split = I(L.n_split);
+        p.chunk_rows = kPartChunk;
+        rc = partition_launch(p, L.KN, st);
+        if (rc != kOk) return rc;
+      }
+    }
+  }
+  return kOk;
+}

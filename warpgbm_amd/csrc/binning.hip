@@ -1,0 +1,159 @@
+// Feature binning to int8 codes (gfx950).
+//
+// code(x) = number of leading edges e_k with x >= e_k, stopping at the first
+// edge greater than x; NaN compares false and yields 0
+// (warpgbm/cuda/binner.cu:16-25).  Two entry points: the reference's
+// per-column op, and a whole-matrix op that bins an [N, F] row-major float32
+// matrix against edges [F, B-1] in one launch, reading 16 bytes of X and
+// writing 4 codes per thread (no per-feature launches, no transposes).
+#include "common.h"
+
+namespace wgbm {
+
+__device__ __forceinline__ int8_t bin_one(float x, const float* __restrict__ e, int nb, int stride) {
+  int b = 0;
+  while (b < nb && x >= e[(int64_t)b * stride]) ++b;
+  return (int8_t)b;
+}
+
+__global__ void __launch_bounds__(256)
+k_bin_column(const float* __restrict__ x, const float* __restrict__ edges, int8_t* __restrict__ out,
+             int64_t n, int nb) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  out[i] = bin_one(x[i], edges, nb, 1);
+}
+
+// Edges transposed to [B-1][F] so that threads of consecutive features read
+// consecutive edge words.
+__global__ void __launch_bounds__(256)
+k_transpose_edges(const float* __restrict__ edges, float* __restrict__ et, int F, int nb) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)F * nb) return;
+  const int f = (int)(i / nb), k = (int)(i - (int64_t)f * nb);
+  et[(int64_t)k * F + f] = edges[i];
+}
+
+// Four consecutive elements of a row per thread (F % 4 == 0).
+__global__ void __launch_bounds__(256)
+k_bin_matrix4(const float4* __restrict__ X, const float* __restrict__ et, uint32_t* __restrict__ out,
+              int64_t n4, int F, int nb) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n4) return;
+  const float4 v = X[i];
+  const int f = (int)((i * 4) % F);
+  const uint32_t c0 = (uint8_t)bin_one(v.x, et + f, nb, F);
+  const uint32_t c1 = (uint8_t)bin_one(v.y, et + f + 1, nb, F);
+  const uint32_t c2 = (uint8_t)bin_one(v.z, et + f + 2, nb, F);
+  const uint32_t c3 = (uint8_t)bin_one(v.w, et + f + 3, nb, F);
+  out[i] = c0 | (c1 << 8) | (c2 << 16) | (c3 << 24);
+}
+
+__global__ void __launch_bounds__(256)
+k_bin_matrix1(const float* __restrict__ X, const float* __restrict__ et, int8_t* __restrict__ out, int64_t n,
+              int F, int nb) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int f = (int)(i % F);
+  out[i] = bin_one(X[i], et + f, nb, F);
+}
+
+// Quantile edges of column-sorted data: sorted [n, ncol] (each column
+// ascending, NaN last) -> edges [ncol, B-1] float32.  Same float32 steps as
+// torch.quantile(col, torch.linspace(0, 1, B+1)[1:-1]) with linear
+// interpolation (warpgbm/core.py:1271-1278): level q_j by one fused
+// multiply-add as torch.linspace forms it, rank = q*(n-1), lo/hi = floor/
+// ceil(rank), w = rank - lo, lerp = w < 0.5 ? a + w*(b-a) : b - (b-a)*(1-w),
+// every other operation rounded separately (built with -ffp-contract=off).
+__global__ void __launch_bounds__(256)
+k_edges_from_sorted(const float* __restrict__ sorted, int64_t n, int ncol, int num_bins,
+                    float* __restrict__ edges, int edge_row_stride) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int ne = num_bins - 1;
+  if (t >= (int64_t)ncol * ne) return;
+  const int c = (int)(t / ne);
+  const int j = (int)(t - (int64_t)c * ne) + 1;  // interior level index 1..B-1
+  const int steps = num_bins + 1;
+  const float step = 1.0f / (float)(steps - 1);
+  const int half = steps / 2;
+  const float q = (j < half) ? fmaf(step, (float)j, 0.0f) : fmaf(-step, (float)(steps - 1 - j), 1.0f);
+  float e;
+  if (isnan(sorted[(n - 1) * ncol + c])) {
+    e = __builtin_nanf("");
+  } else {
+    const float rank = q * (float)(n - 1);
+    const float lo_f = floorf(rank), hi_f = ceilf(rank);
+    const int64_t lo = (int64_t)lo_f, hi = (int64_t)hi_f;
+    const float w = rank - lo_f;
+    const float a = sorted[lo * ncol + c];
+    const float b = sorted[hi * ncol + c];
+    const float d = b - a;
+    if (w < 0.5f) {
+      const float wd = w * d;
+      e = a + wd;
+    } else {
+      const float om = 1.0f - w;
+      const float dm = d * om;
+      e = b - dm;
+    }
+  }
+  edges[(int64_t)c * edge_row_stride + (j - 1)] = e;
+}
+
+}  // namespace wgbm
+
+using namespace wgbm;
+
+extern "C" int wgbm_edges_from_sorted(const float* sorted, int64_t n, int32_t ncol, int32_t num_bins, float* edges,
+                                      int32_t edge_row_stride, hipStream_t stream) {
+  WGBM_REQUIRE(n >= 1 && ncol >= 0 && num_bins >= 2 && num_bins <= 127 && edge_row_stride >= num_bins - 1,
+               "bad quantile shape");
+  const int64_t total = (int64_t)ncol * (num_bins - 1);
+  if (total == 0) return kOk;
+  hipLaunchKernelGGL(k_edges_from_sorted, dim3((unsigned)ceil_div(total, 256)), dim3(256), 0, stream, sorted, n,
+                     (int)ncol, (int)num_bins, edges, (int)edge_row_stride);
+  WGBM_LAUNCHED("k_edges_from_sorted");
+  return kOk;
+}
+
+extern "C" int wgbm_bin_column(const float* x, int64_t n, const float* bin_edges, int32_t n_edges, int8_t* out,
+                               hipStream_t stream) {
+  WGBM_REQUIRE(n >= 0 && n_edges >= 0 && n_edges <= 127, "bad binning shape");
+  if (n == 0) return kOk;
+  hipLaunchKernelGGL(k_bin_column, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, stream, x, bin_edges, out, n,
+                     (int)n_edges);
+  WGBM_LAUNCHED("k_bin_column");
+  return kOk;
+}
+
+extern "C" size_t wgbm_bin_matrix_workspace_bytes(int32_t n_feat, int32_t n_edges) {
+  return (size_t)n_feat * (size_t)(n_edges > 0 ? n_edges : 1) * sizeof(float);
+}
+
+// X [n, F] row-major float32, edges [F, n_edges] float32 -> out [n, F] int8.
+extern "C" int wgbm_bin_matrix(const float* X, int64_t n, int32_t F, const float* edges, int32_t n_edges,
+                               int8_t* out, void* workspace, size_t workspace_bytes, hipStream_t stream) {
+  WGBM_REQUIRE(n >= 0 && F >= 0 && n_edges >= 0 && n_edges <= 127, "bad binning shape");
+  WGBM_REQUIRE(workspace_bytes >= wgbm_bin_matrix_workspace_bytes(F, n_edges), "binning workspace too small");
+  const int64_t total = n * (int64_t)F;
+  if (total == 0) return kOk;
+  float* et = static_cast<float*>(workspace);
+  if (n_edges > 0) {
+    hipLaunchKernelGGL(k_transpose_edges, dim3((unsigned)ceil_div((int64_t)F * n_edges, 256)), dim3(256), 0,
+                       stream, edges, et, (int)F, (int)n_edges);
+    WGBM_LAUNCHED("k_transpose_edges");
+  }
+  const bool vec = (F % 4 == 0) && ((uintptr_t)X % 16 == 0) && ((uintptr_t)out % 4 == 0);
+  if (vec) {
+    const int64_t n4 = total / 4;
+    hipLaunchKernelGGL(k_bin_matrix4, dim3((unsigned)ceil_div(n4, 256)), dim3(256), 0, stream,
+                       reinterpret_cast<const float4*>(X), et, reinterpret_cast<uint32_t*>(out), n4, (int)F,
+                       (int)n_edges);
+    WGBM_LAUNCHED("k_bin_matrix4");
+  } else {
+    hipLaunchKernelGGL(k_bin_matrix1, dim3((unsigned)ceil_div(total, 256)), dim3(256), 0, stream, X, et, out,
+                       total, (int)F, (int)n_edges);
+    WGBM_LAUNCHED("k_bin_matrix1");
+  }
+  return kOk;
+}

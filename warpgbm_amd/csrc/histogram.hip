@@ -1,0 +1,608 @@
+// Gradient / hessian histograms over segmented row lists (gfx950).
+//
+// A histogram build is described by a position list (perm: position -> row
+// id) cut into SEGMENTS (start, count, output slot).  A segment is one
+// (node, era) run of positions; every row of a segment lands in histogram
+// slot `slot`, laid out [slot][n_feat][n_bins] for G and for H (the
+// reference's [E, F, B] per node, warpgbm/cuda/histogram_kernel.cu:51-60).
+//
+// Work decomposition (one wave = 64 lanes):
+//   * a FEATURE SLAB is 64*FPL consecutive local features; lane l owns
+//     features [l*FPL, l*FPL+FPL) of the slab and loads their FPL code bytes
+//     of a row with one load (1..16 bytes);
+//   * a CHUNK is up to C consecutive positions of one segment; a workgroup
+//     of W waves takes (chunk, slab) and splits the chunk into W contiguous
+//     sub-ranges, one per wave;
+//   * every wave accumulates its rows IN ORDER into its own LDS
+//     sub-histogram (lane-private addresses, so no two lanes ever touch the
+//     same word: ds_add_f32 without contention, deterministic);
+//   * the workgroup adds its W sub-histograms in wave order, and a
+//     segment's chunks are added in chunk order.
+// The float32 result therefore depends only on (C, W) and on the order of
+// positions, never on scheduling: plain per-node builds and batched level
+// builds of the same rows are bit-identical.
+//
+// LDS layout per wave: G plane then H plane, each [FPL][B][65] floats
+// (index (j*B + bin)*65 + lane).  Lanes hit consecutive words within one
+// ds_add, and the final cross-lane read walks bins with stride 65, so both
+// phases are free of bank conflicts.
+#include "common.h"
+#include "wgbm_internal.h"
+
+namespace wgbm {
+
+__device__ __forceinline__ void lds_add(float* p, float v) {
+  __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+template <int FPL> struct CodeVec;
+template <> struct CodeVec<1>  { using T = uint8_t; };
+template <> struct CodeVec<2>  { using T = uint16_t; };
+template <> struct CodeVec<4>  { using T = uint32_t; };
+template <> struct CodeVec<8>  { using T = uint64_t; };
+template <> struct CodeVec<16> { using T = uint4; };
+
+template <int FPL>
+__device__ __forceinline__ int code_byte(const typename CodeVec<FPL>::T& v, int j) {
+  if constexpr (FPL == 16) {
+    uint32_t w = j < 4 ? v.x : j < 8 ? v.y : j < 12 ? v.z : v.w;
+    return (int)(int8_t)(uint8_t)(w >> (8 * (j & 3)));
+  } else {
+    return (int)(int8_t)(uint8_t)((uint64_t)v >> (8 * j));
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Plan: chunk counts per segment (exclusive scan) for the histogram grid.
+// seg_first[s] = first global chunk of segment s, seg_first[nseg] = total.
+// A segment with count 0 still owns one (empty) chunk so that its slot is
+// written; count < 0 marks a padding segment that owns no chunk.
+// Segments with more than one chunk are listed in `multi` for the merge.
+__global__ void __launch_bounds__(1024)
+k_hist_plan(const int32_t* __restrict__ seg_count, const int32_t* __restrict__ nseg_dev,
+            int32_t chunk_rows, int32_t* __restrict__ seg_first, int32_t* __restrict__ seg_pfirst,
+            int32_t* __restrict__ multi, int32_t* __restrict__ n_multi) {
+  __shared__ int32_t s_sum[3][1024 / kWave];
+  __shared__ int32_t s_carry[3];
+  const int nseg = *nseg_dev;
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  if (t < 3) s_carry[t] = 0;
+  __syncthreads();
+  for (int base = 0; base < nseg; base += 1024) {
+    const int s = base + t;
+    int v[3] = {0, 0, 0};  // chunks, is-multi, partial chunks
+    if (s < nseg) {
+      const int c = seg_count[s];
+      v[0] = c < 0 ? 0 : (c == 0 ? 1 : (c + chunk_rows - 1) / chunk_rows);
+      v[1] = v[0] > 1;
+      v[2] = v[1] ? v[0] : 0;
+    }
+    int x[3] = {v[0], v[1], v[2]};
+    for (int o = 1; o < 64; o <<= 1) {
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        const int y = __shfl_up(x[q], o);
+        if (lane >= o) x[q] += y;
+      }
+    }
+    if (lane == 63) { s_sum[0][wv] = x[0]; s_sum[1][wv] = x[1]; s_sum[2][wv] = x[2]; }
+    __syncthreads();
+    int pre[3] = {0, 0, 0};
+    for (int k = 0; k < wv; ++k) { pre[0] += s_sum[0][k]; pre[1] += s_sum[1][k]; pre[2] += s_sum[2][k]; }
+    const int c0 = s_carry[0], c1 = s_carry[1], c2 = s_carry[2];
+    if (s < nseg) {
+      seg_first[s] = c0 + pre[0] + x[0] - v[0];
+      if (seg_pfirst) seg_pfirst[s] = c2 + pre[2] + x[2] - v[2];
+      if (v[1] && multi) multi[c1 + pre[1] + x[1] - 1] = s;
+    }
+    __syncthreads();
+    if (t == 1023) {
+      int tot[3] = {0, 0, 0};
+      for (int k = 0; k < 16; ++k) { tot[0] += s_sum[0][k]; tot[1] += s_sum[1][k]; tot[2] += s_sum[2][k]; }
+      s_carry[0] = c0 + tot[0];
+      s_carry[1] = c1 + tot[1];
+      s_carry[2] = c2 + tot[2];
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    seg_first[nseg] = s_carry[0];
+    if (n_multi) *n_multi = s_carry[1];
+  }
+}
+
+// ---------------------------------------------------------------------------
+template <int FPL, bool GH_POS>
+__global__ void __launch_bounds__(256)
+k_hist_build(const HistArgs a) {
+  extern __shared__ float smem[];
+  const int lane = lane_id();
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int W = blockDim.x >> 6;
+  // XCD-aware mapping: dispatch sends block b to XCD b % 8.  All slabs of a
+  // chunk go to the same XCD, back to back, so the rows of a chunk are read
+  // from HBM once into that XCD's L2 and shared by its slabs.
+  const int b = blockIdx.x;
+  const int i = b >> 3;
+  const int slab = i % a.nslab;
+  const int chunk = ((i / a.nslab) << 3) + (b & 7);
+  const int nseg = *a.nseg_dev;
+  const int total = a.seg_first[nseg];
+  if (chunk >= total) return;
+  const int s = upper_index(a.seg_first, nseg, chunk);
+  const int k = chunk - a.seg_first[s];
+  const int nch = a.seg_first[s + 1] - a.seg_first[s];
+  const int seg_start = a.seg_start[s];
+  const int seg_end = seg_start + max(a.seg_count[s], 0);
+  const int c0 = seg_start + k * a.chunk_rows;
+  const int c1 = min(c0 + a.chunk_rows, seg_end);
+  const int len = max(c1 - c0, 0);
+  const int per = (len + W - 1) / W;
+  const int w0 = min(c0 + wave * per, c1);
+  const int w1 = min(w0 + per, c1);
+
+  const int B = a.n_bins;
+  const int plane = FPL * B * 65;
+  float* lg = smem + wave * 2 * plane;
+  float* lh = lg + plane;
+  for (int q = lane; q < 2 * plane; q += kWave) lg[q] = 0.0f;
+
+  const int fl0 = slab * kWave * FPL + lane * FPL;  // local feature of byte 0
+  const bool lane_on = fl0 < a.n_feat;
+  // Wide loads need the slab's features to be the identity columns.
+  bool ident = a.wide_ok;
+  int cols[FPL];
+  bool jon_init[FPL];
+#pragma unroll
+  for (int j = 0; j < FPL; ++j) {
+    const int fl = fl0 + j;
+    int c = (fl < a.n_feat) ? (a.feat_idx ? a.feat_idx[fl] : fl) : 0;
+    // A column outside the matrix contributes nothing (and is never read).
+    jon_init[j] = fl < a.n_feat && (uint64_t)c < (uint64_t)a.row_stride;
+    if (!jon_init[j]) c = 0;
+    cols[j] = c;
+    if (fl < a.n_feat && c != fl) ident = false;
+  }
+  ident = __all(ident || !lane_on);
+  bool jon[FPL];
+#pragma unroll
+  for (int j = 0; j < FPL; ++j) jon[j] = jon_init[j];
+
+  using T = typename CodeVec<FPL>::T;
+  const int8_t* __restrict__ codes = a.codes;
+  const int64_t stride = a.row_stride;
+  // rows whose code loads are in flight at once per wave
+  constexpr int U = (FPL == 1) ? 32 : (FPL <= 4) ? 16 : (FPL == 8) ? 8 : 4;
+
+  auto accum = [&](const T& v, float gg, float hh) {
+#pragma unroll
+    for (int j = 0; j < FPL; ++j) {
+      const int bin = code_byte<FPL>(v, j);
+      if (jon[j] && (unsigned)bin < (unsigned)B) {
+        const int o = (j * B + bin) * 65 + lane;
+        lds_add(lg + o, gg);
+        lds_add(lh + o, hh);
+      }
+    }
+  };
+  auto load_codes = [&](int row) -> T {
+    T v;
+    if (!lane_on) {
+      v = T{};
+    } else if (ident) {
+      v = *reinterpret_cast<const T*>(codes + (int64_t)row * stride + fl0);
+    } else {
+      uint8_t tmp[FPL];
+#pragma unroll
+      for (int j = 0; j < FPL; ++j) tmp[j] = (uint8_t)codes[(int64_t)row * stride + cols[j]];
+      __builtin_memcpy(&v, tmp, sizeof(T));
+    }
+    return v;
+  };
+
+  for (int base = w0; base < w1; base += kWave) {
+    const int pos = base + lane;
+    const bool okp = pos < w1;
+    int myrow = okp ? a.perm[pos] : 0;
+    // Rows outside the matrix are skipped: weight 0 and a safe address.
+    const bool okr = okp && (uint64_t)myrow < (uint64_t)a.n_rows;
+    if (!okr) myrow = 0;
+    float mg = 0.0f, mh = 0.0f;
+    if (okr) {
+      if constexpr (GH_POS) {
+        const float2 v = a.gh[pos];
+        mg = v.x; mh = v.y;
+      } else {
+        mg = a.g[myrow]; mh = a.h[myrow];
+      }
+    }
+    const int cnt = min(kWave, w1 - base);
+    int u = 0;
+    for (; u + U <= cnt; u += U) {
+      T cv[U];
+#pragma unroll
+      for (int q = 0; q < U; ++q) cv[q] = load_codes(__builtin_amdgcn_readlane(myrow, u + q));
+#pragma unroll
+      for (int q = 0; q < U; ++q) {
+        const float gg = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, mg), u + q));
+        const float hh = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, mh), u + q));
+        accum(cv[q], gg, hh);
+      }
+    }
+    for (; u < cnt; ++u) {
+      const T v = load_codes(__builtin_amdgcn_readlane(myrow, u));
+      const float gg = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, mg), u));
+      const float hh = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, mh), u));
+      accum(v, gg, hh);
+    }
+  }
+  __syncthreads();
+
+  // Add the W sub-histograms in wave order and write the chunk result.
+  const int SL = kWave * FPL * B;  // outputs per plane of this slab
+  const int f_base = slab * kWave * FPL;
+  const bool direct = (nch == 1);
+  const int slot = a.seg_slot[s];
+  for (int e = threadIdx.x; e < SL; e += blockDim.x) {
+    const int fl = e / B;
+    const int bin = e - fl * B;
+    if (f_base + fl >= a.n_feat) continue;
+    const int o = ((fl % FPL) * B + bin) * 65 + fl / FPL;
+    float sg = smem[o], sh = smem[plane + o];
+    for (int w = 1; w < W; ++w) {
+      sg += smem[w * 2 * plane + o];
+      sh += smem[w * 2 * plane + plane + o];
+    }
+    if (direct) {
+      const int64_t d = ((int64_t)slot * a.n_feat + f_base + fl) * B + bin;
+      if (a.accumulate) { a.out_g[d] += sg; a.out_h[d] += sh; }
+      else { a.out_g[d] = sg; a.out_h[d] = sh; }
+    } else {
+      const int64_t d = ((int64_t)(a.seg_pfirst[s] + k) * a.nslab + slab) * SL + e;
+      a.part_g[d] = sg;
+      a.part_h[d] = sh;
+    }
+  }
+}
+
+// Sum the chunk partials of every multi-chunk segment in chunk order.
+template <int FPL>
+__global__ void __launch_bounds__(256)
+k_hist_merge(const HistArgs a, const int32_t* __restrict__ multi,
+             const int32_t* __restrict__ n_multi, int nblk) {
+  const int B = a.n_bins;
+  const int SL = kWave * FPL * B;
+  const int m = blockIdx.x / (a.nslab * nblk);
+  const int r = blockIdx.x - m * (a.nslab * nblk);
+  const int slab = r / nblk;
+  const int blk = r - slab * nblk;
+  if (m >= *n_multi) return;
+  const int s = multi[m];
+  const int first = a.seg_pfirst[s];
+  const int nch = a.seg_first[s + 1] - a.seg_first[s];
+  const int slot = a.seg_slot[s];
+  const int f_base = slab * kWave * FPL;
+  const int e = blk * blockDim.x + threadIdx.x;
+  if (e >= SL) return;
+  const int fl = e / B;
+  const int bin = e - fl * B;
+  if (f_base + fl >= a.n_feat) return;
+  float sg = 0.0f, sh = 0.0f;
+  const float* pg = a.part_g + ((int64_t)first * a.nslab + slab) * SL + e;
+  const float* ph = a.part_h + ((int64_t)first * a.nslab + slab) * SL + e;
+  const int64_t step = (int64_t)a.nslab * SL;
+  int k = 0;
+  for (; k + 4 <= nch; k += 4) {
+    const float g0 = pg[0], g1 = pg[step], g2 = pg[2 * step], g3 = pg[3 * step];
+    const float h0 = ph[0], h1 = ph[step], h2 = ph[2 * step], h3 = ph[3 * step];
+    sg += g0; sg += g1; sg += g2; sg += g3;
+    sh += h0; sh += h1; sh += h2; sh += h3;
+    pg += 4 * step; ph += 4 * step;
+  }
+  for (; k < nch; ++k) { sg += pg[0]; sh += ph[0]; pg += step; ph += step; }
+  const int64_t d = ((int64_t)slot * a.n_feat + f_base + fl) * B + bin;
+  if (a.accumulate) { a.out_g[d] += sg; a.out_h[d] += sh; }
+  else { a.out_g[d] = sg; a.out_h[d] = sh; }
+}
+
+// ---------------------------------------------------------------------------
+// Stable counting sort of a row list by era (reference op with E > 1).
+// Positions are cut into blocks of kSortBlock; era counts per block, an
+// era-major exclusive scan, then an in-order scatter per block.  Rows whose
+// era lies outside [0, E) are dropped (they contribute to no histogram).
+constexpr int kSortBlock = 4096;
+
+__global__ void __launch_bounds__(256)
+k_era_count(const int32_t* __restrict__ rows, int n, int64_t n_rows,
+            const int32_t* __restrict__ era, int E, int32_t* __restrict__ cnt) {
+  extern __shared__ int32_t sc[];
+  for (int e = threadIdx.x; e < E; e += blockDim.x) sc[e] = 0;
+  __syncthreads();
+  const int p0 = blockIdx.x * kSortBlock;
+  const int p1 = min(p0 + kSortBlock, n);
+  for (int p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
+    const int r = rows[p];
+    if ((uint64_t)r >= (uint64_t)n_rows) continue;
+    const int e = era[r];
+    if ((unsigned)e < (unsigned)E) atomicAdd(&sc[e], 1);
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < E; e += blockDim.x) cnt[(int64_t)blockIdx.x * E + e] = sc[e];
+}
+
+__global__ void __launch_bounds__(1024)
+k_era_scan(int32_t* __restrict__ cnt, int nblocks, int E, int32_t* __restrict__ seg_start,
+           int32_t* __restrict__ seg_count, int32_t* __restrict__ seg_slot) {
+  __shared__ int32_t s_tot[1024];
+  __shared__ int32_t s_carry;
+  if (threadIdx.x == 0) s_carry = 0;
+  __syncthreads();
+  for (int base = 0; base < E; base += 1024) {
+    const int e = base + threadIdx.x;
+    int tot = 0;
+    if (e < E)
+      for (int c = 0; c < nblocks; ++c) tot += cnt[(int64_t)c * E + e];
+    s_tot[threadIdx.x] = tot;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int run = s_carry;
+      for (int k = 0; k < 1024; ++k) { const int v = s_tot[k]; s_tot[k] = run; run += v; }
+      s_carry = run;
+    }
+    __syncthreads();
+    if (e < E) {
+      int run = s_tot[threadIdx.x];
+      seg_start[e] = run;
+      seg_count[e] = tot;
+      seg_slot[e] = e;
+      for (int c = 0; c < nblocks; ++c) {
+        const int v = cnt[(int64_t)c * E + e];
+        cnt[(int64_t)c * E + e] = run;
+        run += v;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void __launch_bounds__(64)
+k_era_scatter(const int32_t* __restrict__ rows, int n, int64_t n_rows, const int32_t* __restrict__ era, int E,
+              const int32_t* __restrict__ off, int32_t* __restrict__ out) {
+  extern __shared__ int32_t run[];
+  for (int e = threadIdx.x; e < E; e += 64) run[e] = off[(int64_t)blockIdx.x * E + e];
+  __syncthreads();
+  const int lane = threadIdx.x;
+  const int p0 = blockIdx.x * kSortBlock;
+  const int p1 = min(p0 + kSortBlock, n);
+  for (int base = p0; base < p1; base += 64) {
+    const int p = base + lane;
+    int row = 0, e = -1;
+    if (p < p1) {
+      row = rows[p];
+      e = (uint64_t)row < (uint64_t)n_rows ? era[row] : -1;
+      if ((unsigned)e >= (unsigned)E) e = -1;
+    }
+    bool pending = e >= 0;
+    while (true) {
+      const uint64_t act = __ballot(pending);
+      if (act == 0) break;
+      const int leader = __builtin_ctzll(act);
+      const int le = __shfl(e, leader);
+      const uint64_t mask = __ballot(pending && e == le);
+      if (pending && e == le) {
+        out[run[le] + lanes_below(mask)] = row;
+        pending = false;
+      }
+      __syncthreads();
+      if (lane == leader) run[le] += __popcll(mask);
+      __syncthreads();
+    }
+  }
+}
+
+__global__ void k_single_segment(int32_t* seg_start, int32_t* seg_count, int32_t* seg_slot,
+                                 int32_t* nseg, int n) {
+  if (threadIdx.x == 0) { seg_start[0] = 0; seg_count[0] = n; seg_slot[0] = 0; *nseg = 1; }
+}
+
+__global__ void k_set_scalar(int32_t* p, int v) {
+  if (threadIdx.x == 0) *p = v;
+}
+
+void plan_launch(const int32_t* seg_count, const int32_t* nseg_dev, int chunk_rows,
+                 int32_t* seg_first, int32_t* multi, int32_t* n_multi, hipStream_t st) {
+  hipLaunchKernelGGL(k_hist_plan, dim3(1), dim3(1024), 0, st, seg_count, nseg_dev, chunk_rows, seg_first,
+                     (int32_t*)nullptr, multi, n_multi);
+}
+
+// ---------------------------------------------------------------------------
+size_t lds_bytes(const HistConfig& c, int n_bins) {
+  return (size_t)c.waves * 2 * c.fpl * n_bins * 65 * sizeof(float);
+}
+
+static HistConfig g_override;   // all-zero: automatic
+
+void set_hist_override(const HistConfig& c) { g_override = c; }
+
+HistConfig pick_hist_config(int n_feat, int n_bins, int64_t row_stride) {
+  HistConfig c = auto_hist_config(n_feat, n_bins, row_stride);
+  if (g_override.chunk_rows > 0) c.chunk_rows = g_override.chunk_rows;
+  if (g_override.waves > 0) c.waves = g_override.waves;
+  if (g_override.fpl > 0 && row_stride % g_override.fpl == 0) c.fpl = g_override.fpl;
+  return c;
+}
+
+HistConfig auto_hist_config(int n_feat, int n_bins, int64_t row_stride) {
+  (void)n_feat;
+  HistConfig c;
+  c.chunk_rows = 8192;
+  // Widest per-lane load whose sub-histograms still leave room for two
+  // workgroups per CU (160 KB LDS); W = 4 waves unless bins are large.
+  c.waves = 4;
+  while (c.waves > 1 && (size_t)c.waves * 2 * 1 * n_bins * 65 * 4 > 80 * 1024) c.waves >>= 1;
+  int fpl = 16;
+  while (fpl > 1 && ((size_t)c.waves * 2 * fpl * n_bins * 65 * 4 > 80 * 1024 || row_stride % fpl != 0))
+    fpl >>= 1;
+  c.fpl = fpl;
+  return c;
+}
+
+int64_t hist_max_chunks(int64_t rows_max, int64_t nseg_max, int chunk_rows) {
+  return ceil_div(rows_max, chunk_rows) + nseg_max;
+}
+
+size_t hist_partial_floats(const HistConfig& c, int n_feat, int n_bins, int64_t rows_max) {
+  const int nslab = (int)ceil_div(n_feat, kWave * c.fpl);
+  // Only multi-chunk segments write partials; each holds more than C rows,
+  // so their chunks number at most 2 * rows / C.
+  const int64_t chunks = 2 * ceil_div(rows_max, c.chunk_rows) + 2;
+  return (size_t)chunks * nslab * kWave * c.fpl * n_bins;
+}
+
+template <int FPL>
+static int launch_fpl(const HistArgs& a, const HistConfig& c, int64_t max_chunks, int64_t max_multi,
+                      int32_t* multi, int32_t* n_multi, hipStream_t st) {
+  const size_t lds = lds_bytes(c, a.n_bins);
+  const int64_t groups = ceil_div(max_chunks, 8) * 8 * a.nslab;
+  if (groups > 0x7fffffff) { set_error("histogram grid too large"); return kTooBig; }
+  if (a.gh)
+    hipLaunchKernelGGL((k_hist_build<FPL, true>), dim3((unsigned)groups), dim3(64 * c.waves), lds, st, a);
+  else
+    hipLaunchKernelGGL((k_hist_build<FPL, false>), dim3((unsigned)groups), dim3(64 * c.waves), lds, st, a);
+  WGBM_LAUNCHED("k_hist_build");
+  if (max_multi > 0) {
+    const int SL = kWave * FPL * a.n_bins;
+    const int nblk = (int)ceil_div(SL, 256);
+    const int64_t mg = max_multi * a.nslab * nblk;
+    hipLaunchKernelGGL((k_hist_merge<FPL>), dim3((unsigned)mg), dim3(256), 0, st, a, multi, n_multi, nblk);
+    WGBM_LAUNCHED("k_hist_merge");
+  }
+  return kOk;
+}
+
+int hist_launch(HistArgs a, const HistConfig& c, int64_t rows_max, int64_t nseg_max,
+                int32_t* multi, int32_t* n_multi, hipStream_t st) {
+  a.chunk_rows = c.chunk_rows;
+  a.nslab = (int)ceil_div(a.n_feat, kWave * c.fpl);
+  a.wide_ok = (a.row_stride % c.fpl == 0) && ((uintptr_t)a.codes % (uintptr_t)c.fpl == 0);
+  if (lds_bytes(c, a.n_bins) > 160 * 1024) { set_error("histogram LDS budget exceeded"); return kTooBig; }
+  hipLaunchKernelGGL(k_hist_plan, dim3(1), dim3(1024), 0, st, a.seg_count, a.nseg_dev, c.chunk_rows,
+                     a.seg_first, a.seg_pfirst, multi, n_multi);
+  WGBM_LAUNCHED("k_hist_plan");
+  const int64_t max_chunks = hist_max_chunks(rows_max, nseg_max, c.chunk_rows);
+  const int64_t max_multi = rows_max / (c.chunk_rows + 1) + 1;
+  if (a.n_feat == 0) return kOk;
+  switch (c.fpl) {
+    case 1: return launch_fpl<1>(a, c, max_chunks, max_multi, multi, n_multi, st);
+    case 2: return launch_fpl<2>(a, c, max_chunks, max_multi, multi, n_multi, st);
+    case 4: return launch_fpl<4>(a, c, max_chunks, max_multi, multi, n_multi, st);
+    case 8: return launch_fpl<8>(a, c, max_chunks, max_multi, multi, n_multi, st);
+    case 16: return launch_fpl<16>(a, c, max_chunks, max_multi, multi, n_multi, st);
+  }
+  set_error("unsupported features-per-lane %d", c.fpl);
+  return kBadArg;
+}
+
+}  // namespace wgbm
+
+using namespace wgbm;
+
+// Workspace carve-up for the reference-style histogram op.
+struct Hist3Layout {
+  size_t seg_start, seg_count, seg_slot, seg_first, seg_pfirst, nseg, multi, n_multi, sorted, cnt, part, total;
+};
+
+static Hist3Layout hist3_layout(int64_t n, int n_feat, int n_bins, int n_eras, const HistConfig& c) {
+  Hist3Layout L{};
+  auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+  size_t o = 0;
+  const int64_t E = n_eras;
+  L.seg_start = o; o = al(o + 4 * E);
+  L.seg_count = o; o = al(o + 4 * E);
+  L.seg_slot = o;  o = al(o + 4 * E);
+  L.seg_first = o; o = al(o + 4 * (E + 1));
+  L.seg_pfirst = o; o = al(o + 4 * E);
+  L.nseg = o;      o = al(o + 4);
+  const int64_t max_multi = n / (c.chunk_rows + 1) + 1;
+  L.multi = o;     o = al(o + 4 * max_multi);
+  L.n_multi = o;   o = al(o + 4);
+  const int64_t nb = ceil_div(n, kSortBlock);
+  L.sorted = o;    o = al(o + (E > 1 ? 4 * n : 0));
+  L.cnt = o;       o = al(o + (E > 1 ? 4 * nb * E : 0));
+  L.part = o;      o = al(o + 2 * 4 * hist_partial_floats(c, n_feat, n_bins, n));
+  L.total = o;
+  return L;
+}
+
+extern "C" size_t wgbm_histogram3_workspace_bytes(int64_t n_rows, int32_t n_feat, int32_t n_bins,
+                                                  int32_t n_eras) {
+  if (n_rows < 0 || n_feat < 0 || n_bins < 1 || n_eras < 1) return 0;
+  const HistConfig c = pick_hist_config(n_feat, n_bins, n_feat);
+  return hist3_layout(n_rows, n_feat, n_bins, n_eras, c).total;
+}
+
+extern "C" int wgbm_compute_histogram3(const int8_t* bin_indices, int64_t n_total_rows,
+                                       int64_t n_master_feat, const float* grads, const float* hess,
+                                       const int32_t* sample_indices, int64_t n_samples,
+                                       const int32_t* feature_indices, int32_t n_feat,
+                                       const int32_t* era_indices, float* grad_hist, float* hess_hist,
+                                       int32_t n_eras, int32_t num_bins, void* workspace,
+                                       size_t workspace_bytes, hipStream_t stream) {
+  WGBM_REQUIRE(n_samples >= 0 && n_samples < 0x7fffffff, "sample count out of range: %lld",
+               (long long)n_samples);
+  WGBM_REQUIRE(num_bins >= 1 && num_bins <= 127, "num_bins must be in [1, 127], got %d", num_bins);
+  WGBM_REQUIRE(n_eras >= 1 && n_eras <= 32768, "era count must be in [1, 32768], got %d", n_eras);
+  WGBM_REQUIRE(n_feat >= 0, "feature count must be non-negative");
+  WGBM_REQUIRE(n_total_rows >= 0 && n_master_feat >= 0, "bad bin matrix shape");
+  const HistConfig c = pick_hist_config(n_feat, num_bins, n_master_feat);
+  const Hist3Layout L = hist3_layout(n_samples, n_feat, num_bins, n_eras, c);
+  WGBM_REQUIRE(workspace_bytes >= L.total, "workspace too small: need %zu bytes, got %zu", L.total,
+               workspace_bytes);
+  if (n_samples == 0 || n_feat == 0) return kOk;  // histograms accumulate nothing
+  char* ws = static_cast<char*>(workspace);
+  auto P = [&](size_t off) { return reinterpret_cast<int32_t*>(ws + off); };
+  HistArgs a{};
+  a.codes = bin_indices;
+  a.row_stride = n_master_feat;
+  a.n_rows = n_total_rows;
+  a.feat_idx = feature_indices;
+  a.n_feat = n_feat;
+  a.n_bins = num_bins;
+  a.g = grads;
+  a.h = hess;
+  a.gh = nullptr;
+  a.seg_start = P(L.seg_start);
+  a.seg_count = P(L.seg_count);
+  a.seg_slot = P(L.seg_slot);
+  a.seg_first = P(L.seg_first);
+  a.seg_pfirst = P(L.seg_pfirst);
+  a.nseg_dev = P(L.nseg);
+  a.out_g = grad_hist;
+  a.out_h = hess_hist;
+  a.part_g = reinterpret_cast<float*>(ws + L.part);
+  a.part_h = a.part_g + hist_partial_floats(c, n_feat, num_bins, n_samples);
+  a.accumulate = 1;
+  const int n = (int)n_samples;
+  if (n_eras == 1) {
+    a.perm = sample_indices;
+    hipLaunchKernelGGL(k_single_segment, dim3(1), dim3(64), 0, stream, P(L.seg_start), P(L.seg_count),
+                       P(L.seg_slot), P(L.nseg), n);
+    WGBM_LAUNCHED("k_single_segment");
+  } else {
+    const int nb = (int)ceil_div(n, kSortBlock);
+    const size_t lds = (size_t)n_eras * 4;
+    hipLaunchKernelGGL(k_era_count, dim3(nb), dim3(256), lds, stream, sample_indices, n, n_total_rows, era_indices,
+                       (int)n_eras, P(L.cnt));
+    WGBM_LAUNCHED("k_era_count");
+    hipLaunchKernelGGL(k_era_scan, dim3(1), dim3(1024), 0, stream, P(L.cnt), nb, (int)n_eras,
+                       P(L.seg_start), P(L.seg_count), P(L.seg_slot));
+    WGBM_LAUNCHED("k_era_scan");
+    hipLaunchKernelGGL(k_era_scatter, dim3(nb), dim3(64), lds, stream, sample_indices, n, n_total_rows, era_indices,
+                       (int)n_eras, P(L.cnt), P(L.sorted));
+    WGBM_LAUNCHED("k_era_scatter");
+    hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(64), 0, stream, P(L.nseg), (int)n_eras);
+    WGBM_LAUNCHED("k_set_scalar");
+    a.perm = P(L.sorted);
+  }
+  return hist_launch(a, c, n, n_eras, P(L.multi), P(L.n_multi), stream);
+}

@@ -1,0 +1,543 @@
+// Row partitioning, leaf values, score updates, histogram subtraction and
+// gradient/hessian kernels (gfx950).
+//
+// Positions of a tree level are grouped by node, and within a node by era
+// (segment (node, era)).  A split moves every segment's rows into the left
+// child (code <= bin) or the right child, each side keeping the parent's
+// order (stable; warpgbm/core.py:1425-1427 compacts with a boolean mask),
+// so children stay era-grouped.  Children occupy the parent's range: left
+// child first, then right.
+#include "common.h"
+#include "wgbm_internal.h"
+#include "wgbm_level.h"
+
+namespace wgbm {
+
+// ----------------------------------------------------------------- partition
+__global__ void k_part_prep(PartArgs a) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nseg = a.n_nodes * a.n_eras;
+  if (s == 0) *a.nseg_dev = nseg;
+  if (s >= nseg) return;
+  const int j = s / a.n_eras;
+  const bool split = a.node_active[j] && a.dec_feat[j] >= 0;
+  a.pcount[s] = split ? a.seg_count[s] : -1;
+}
+
+__device__ __forceinline__ int split_col(const PartArgs& a, int j) {
+  const int fl = a.dec_feat[j];
+  return a.feat_idx ? a.feat_idx[fl] : fl;
+}
+
+__device__ __forceinline__ bool goes_left(const PartArgs& a, int row, int col, int bin) {
+  if ((uint64_t)row >= (uint64_t)a.n_rows) return true;
+  return (int)a.codes[(int64_t)row * a.row_stride + col] <= bin;
+}
+
+__global__ void __launch_bounds__(256)
+k_part_count(PartArgs a) {
+  const int chunk = blockIdx.x;
+  const int nseg = *a.nseg_dev;
+  if (chunk >= a.seg_first[nseg]) return;
+  const int s = upper_index(a.seg_first, nseg, chunk);
+  const int k = chunk - a.seg_first[s];
+  const int j = s / a.n_eras;
+  const int col = split_col(a, j);
+  const int bin = a.dec_bin[j];
+  const int c0 = a.seg_start[s] + k * a.chunk_rows;
+  const int c1 = min(c0 + a.chunk_rows, a.seg_start[s] + max(a.seg_count[s], 0));
+  int cnt = 0;
+  for (int p = c0 + threadIdx.x; p < c1; p += blockDim.x) cnt += goes_left(a, a.perm_in[p], col, bin);
+  for (int o = 32; o >= 1; o >>= 1) cnt += __shfl_xor(cnt, o);
+  __shared__ int s_c[4];
+  if ((threadIdx.x & 63) == 0) s_c[threadIdx.x >> 6] = cnt;
+  __syncthreads();
+  if (threadIdx.x == 0) a.lbase[chunk] = s_c[0] + s_c[1] + s_c[2] + s_c[3];
+}
+
+// One workgroup: per-segment left totals and chunk offsets, child segment
+// tables, smaller-child build list and sibling map.
+__global__ void __launch_bounds__(1024)
+k_part_scan(PartArgs a) {
+  const int E = a.n_eras;
+  const int nseg = a.n_nodes * E;
+  // A: left totals per segment; chunk-local left/right offsets.
+  for (int s = threadIdx.x; s < nseg; s += blockDim.x) {
+    if (a.pcount[s] < 0) { a.segL[s] = 0; continue; }
+    const int first = a.seg_first[s], last = a.seg_first[s + 1];
+    int run = 0;
+    for (int c = first; c < last; ++c) {
+      const int v = a.lbase[c];
+      a.lbase[c] = run;                                // lefts before chunk (in segment)
+      a.rbase[c] = (c - first) * a.chunk_rows - run;   // rights before chunk
+      run += v;
+    }
+    a.segL[s] = run;
+  }
+  __syncthreads();
+  // B: child tables per node.
+  for (int j = threadIdx.x; j < a.n_nodes; j += blockDim.x) {
+    const bool split = a.node_active[j] && a.dec_feat[j] >= 0;
+    const int l = 2 * j, r = 2 * j + 1;
+    if (!split) {
+      a.c_active[l] = 0; a.c_active[r] = 0;
+      a.c_node_start[l] = 0; a.c_node_start[r] = 0;
+      a.c_node_count[l] = 0; a.c_node_count[r] = 0;
+      for (int e = 0; e < E; ++e) {
+        a.c_seg_start[l * E + e] = 0; a.c_seg_count[l * E + e] = 0;
+        a.c_seg_start[r * E + e] = 0; a.c_seg_count[r * E + e] = 0;
+      }
+      continue;
+    }
+    const int start = a.node_start[j];
+    int Lt = 0;
+    for (int e = 0; e < E; ++e) Lt += a.segL[j * E + e];
+    const int Rt = a.node_count[j] - Lt;
+    int runl = start, runr = start + Lt;
+    for (int e = 0; e < E; ++e) {
+      const int L = a.segL[j * E + e];
+      const int R = max(a.seg_count[j * E + e], 0) - L;
+      a.c_seg_start[l * E + e] = runl; a.c_seg_count[l * E + e] = L;
+      a.c_seg_start[r * E + e] = runr; a.c_seg_count[r * E + e] = R;
+      runl += L; runr += R;
+    }
+    a.c_active[l] = 1; a.c_active[r] = 1;
+    a.c_node_start[l] = start; a.c_node_count[l] = Lt;
+    a.c_node_start[r] = start + Lt; a.c_node_count[r] = Rt;
+  }
+  __syncthreads();
+  // C: absolute destination bases per chunk.
+  for (int s = threadIdx.x; s < nseg; s += blockDim.x) {
+    if (a.pcount[s] < 0) continue;
+    const int j = s / E, e = s - j * E;
+    const int ls = a.c_seg_start[(2 * j) * E + e];
+    const int rs = a.c_seg_start[(2 * j + 1) * E + e];
+    for (int c = a.seg_first[s]; c < a.seg_first[s + 1]; ++c) {
+      a.lbase[c] += ls;
+      a.rbase[c] += rs;
+    }
+  }
+  // D: build list (smaller child, left on ties: warpgbm/core.py:1438) in node order.
+  if (threadIdx.x == 0) {
+    int rank = 0;
+    for (int j = 0; j < a.n_nodes; ++j) {
+      const bool split = a.node_active[j] && a.dec_feat[j] >= 0;
+      if (!split) continue;
+      const int l = 2 * j, r = 2 * j + 1;
+      const int built = (a.c_node_count[l] <= a.c_node_count[r]) ? l : r;
+      const int other = built ^ 1;
+      a.sib[3 * rank + 0] = j;
+      a.sib[3 * rank + 1] = built;
+      a.sib[3 * rank + 2] = other;
+      ++rank;
+    }
+    *a.n_split = rank;
+    *a.n_build = rank * E;
+  }
+  __syncthreads();
+  const int ns = *a.n_split;
+  for (int t = threadIdx.x; t < ns * E; t += blockDim.x) {
+    const int q = t / E, e = t - q * E;
+    const int c = a.sib[3 * q + 1];
+    a.b_seg_start[t] = a.c_seg_start[c * E + e];
+    a.b_seg_count[t] = a.c_seg_count[c * E + e];
+    a.b_seg_slot[t] = c * E + e;
+  }
+}
+
+__global__ void __launch_bounds__(256)
+k_part_scatter(PartArgs a) {
+  const int chunk = blockIdx.x;
+  const int nseg = *a.nseg_dev;
+  if (chunk >= a.seg_first[nseg]) return;
+  const int s = upper_index(a.seg_first, nseg, chunk);
+  const int k = chunk - a.seg_first[s];
+  const int j = s / a.n_eras;
+  const int col = split_col(a, j);
+  const int bin = a.dec_bin[j];
+  const int c0 = a.seg_start[s] + k * a.chunk_rows;
+  const int c1 = min(c0 + a.chunk_rows, a.seg_start[s] + max(a.seg_count[s], 0));
+  __shared__ int s_l[4];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  int lrun = a.lbase[chunk], rrun = a.rbase[chunk];
+  for (int base = c0; base < c1; base += 256) {
+    const int p = base + threadIdx.x;
+    const bool ok = p < c1;
+    int row = 0;
+    float2 v = make_float2(0.0f, 0.0f);
+    bool left = false;
+    if (ok) {
+      row = a.perm_in[p];
+      if (a.gh_in) v = a.gh_in[p];
+      left = goes_left(a, row, col, bin);
+    }
+    const uint64_t lm = __ballot(ok && left);
+    const uint64_t rm = __ballot(ok && !left);
+    if (lane == 0) s_l[wv] = __popcll(lm);
+    __syncthreads();
+    int lpre = 0, ltot = 0;
+    for (int w = 0; w < 4; ++w) { const int c = s_l[w]; if (w < wv) lpre += c; ltot += c; }
+    const int n_here = min(256, c1 - base);
+    const int wave_lo = min(64 * wv, n_here);           // positions before this wave
+    const int rpre = wave_lo - lpre;
+    if (ok) {
+      const int dst = left ? (lrun + lpre + lanes_below(lm)) : (rrun + rpre + lanes_below(rm));
+      a.perm_out[dst] = row;
+      if (a.gh_out) a.gh_out[dst] = v;
+    }
+    lrun += ltot;
+    rrun += n_here - ltot;
+    __syncthreads();
+  }
+}
+
+int partition_launch(const PartArgs& a, int64_t rows_max, hipStream_t st) {
+  const int nseg = a.n_nodes * a.n_eras;
+  if (nseg == 0) return kOk;
+  hipLaunchKernelGGL(k_part_prep, dim3((unsigned)ceil_div(nseg, 256)), dim3(256), 0, st, a);
+  WGBM_LAUNCHED("k_part_prep");
+  plan_launch(a.pcount, a.nseg_dev, a.chunk_rows, a.seg_first, a.multi, a.n_multi, st);
+  WGBM_LAUNCHED("k_hist_plan(partition)");
+  const int64_t max_chunks = ceil_div(rows_max, a.chunk_rows) + nseg;
+  hipLaunchKernelGGL(k_part_count, dim3((unsigned)max_chunks), dim3(256), 0, st, a);
+  WGBM_LAUNCHED("k_part_count");
+  hipLaunchKernelGGL(k_part_scan, dim3(1), dim3(1024), 0, st, a);
+  WGBM_LAUNCHED("k_part_scan");
+  hipLaunchKernelGGL(k_part_scatter, dim3((unsigned)max_chunks), dim3(256), 0, st, a);
+  WGBM_LAUNCHED("k_part_scatter");
+  return kOk;
+}
+
+// ----------------------------------------------------------------- leaves
+// Leaf sums over a node's positions: each workgroup of 256 threads takes
+// kLeafChunk consecutive positions; thread t adds positions t, t+256, ...
+// in float64, the workgroup combines thread sums by a fixed tree, and the
+// node adds its chunk sums in chunk order.
+constexpr int kLeafChunk = 4096;
+
+__global__ void k_leaf_prep(LeafArgs a) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j == 0) *a.nseg_dev = a.n_nodes;
+  if (j >= a.n_nodes) return;
+  const bool leaf = a.node_active[j] && (a.dec_feat == nullptr || a.dec_feat[j] < 0);
+  a.lcount[j] = leaf ? a.node_count[j] : -1;
+}
+
+__device__ __forceinline__ void leaf_gh(const LeafArgs& a, int p, double& g, double& h) {
+  if (a.gh) {
+    const float2 v = a.gh[p];
+    g = v.x; h = v.y;
+  } else {
+    const int r = a.perm[p];
+    g = a.g[r]; h = a.h[r];
+  }
+}
+
+__global__ void __launch_bounds__(256)
+k_leaf_partial(LeafArgs a) {
+  const int chunk = blockIdx.x;
+  const int nseg = *a.nseg_dev;
+  if (chunk >= a.seg_first[nseg]) return;
+  const int j = upper_index(a.seg_first, nseg, chunk);
+  const int k = chunk - a.seg_first[j];
+  const int c0 = a.node_start[j] + k * kLeafChunk;
+  const int c1 = min(c0 + kLeafChunk, a.node_start[j] + a.node_count[j]);
+  double sg = 0.0, sh = 0.0;
+  for (int p = c0 + threadIdx.x; p < c1; p += 256) {
+    double g, h;
+    leaf_gh(a, p, g, h);
+    sg += g; sh += h;
+  }
+  __shared__ double s_g[256], s_h[256];
+  s_g[threadIdx.x] = sg; s_h[threadIdx.x] = sh;
+  __syncthreads();
+  for (int w = 128; w >= 1; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      s_g[threadIdx.x] += s_g[threadIdx.x + w];
+      s_h[threadIdx.x] += s_h[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) { a.part[2 * (int64_t)chunk] = s_g[0]; a.part[2 * (int64_t)chunk + 1] = s_h[0]; }
+}
+
+__global__ void k_leaf_final(LeafArgs a) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= a.n_nodes) return;
+  if (a.lcount[j] < 0) return;
+  double sg = 0.0, sh = 0.0;
+  for (int c = a.seg_first[j]; c < a.seg_first[j + 1]; ++c) {
+    sg += a.part[2 * (int64_t)c];
+    sh += a.part[2 * (int64_t)c + 1];
+  }
+  const float v = (float)(sg / (sh + a.lam));
+  a.leaf_val[j] = v;
+  if (a.node_sum) { a.node_sum[2 * j] = sg; a.node_sum[2 * j + 1] = sh; }
+}
+
+// pred[row*K + class] += f32(lr) * leaf, class = node / nodes_per_class.
+__global__ void __launch_bounds__(256)
+k_leaf_apply(LeafArgs a) {
+  const int chunk = blockIdx.x;
+  const int nseg = *a.nseg_dev;
+  if (chunk >= a.seg_first[nseg]) return;
+  const int j = upper_index(a.seg_first, nseg, chunk);
+  const int k = chunk - a.seg_first[j];
+  const int c0 = a.node_start[j] + k * kLeafChunk;
+  const int c1 = min(c0 + kLeafChunk, a.node_start[j] + a.node_count[j]);
+  const float step = a.lr * a.leaf_val[j];
+  const int cls = j / a.nodes_per_class;
+  for (int p = c0 + threadIdx.x; p < c1; p += 256) {
+    const int r = a.perm[p];
+    if ((uint64_t)r >= (uint64_t)a.n_rows) continue;
+    float* d = a.pred + (int64_t)r * a.pred_stride + cls;
+    *d = *d + step;
+  }
+}
+
+int leaf_launch(const LeafArgs& a, int64_t rows_max, hipStream_t st) {
+  if (a.n_nodes == 0) return kOk;
+  hipLaunchKernelGGL(k_leaf_prep, dim3((unsigned)ceil_div(a.n_nodes, 256)), dim3(256), 0, st, a);
+  WGBM_LAUNCHED("k_leaf_prep");
+  plan_launch(a.lcount, a.nseg_dev, kLeafChunk, a.seg_first, a.multi, a.n_multi, st);
+  WGBM_LAUNCHED("k_hist_plan(leaf)");
+  const int64_t max_chunks = ceil_div(rows_max, kLeafChunk) + a.n_nodes;
+  hipLaunchKernelGGL(k_leaf_partial, dim3((unsigned)max_chunks), dim3(256), 0, st, a);
+  WGBM_LAUNCHED("k_leaf_partial");
+  hipLaunchKernelGGL(k_leaf_final, dim3((unsigned)ceil_div(a.n_nodes, 256)), dim3(256), 0, st, a);
+  WGBM_LAUNCHED("k_leaf_final");
+  if (a.pred) {
+    hipLaunchKernelGGL(k_leaf_apply, dim3((unsigned)max_chunks), dim3(256), 0, st, a);
+    WGBM_LAUNCHED("k_leaf_apply");
+  }
+  return kOk;
+}
+
+int64_t leaf_max_chunks(int64_t rows_max, int64_t n_nodes) { return ceil_div(rows_max, kLeafChunk) + n_nodes; }
+
+// ----------------------------------------------------------------- subtract
+// other = parent - built for every split of the previous level
+// (warpgbm/core.py:1445-1455), float32 elementwise over [E][F][B].
+__global__ void __launch_bounds__(256)
+k_hist_subtract(const float* __restrict__ parent_g, const float* __restrict__ parent_h,
+                float* __restrict__ child_g, float* __restrict__ child_h, const int32_t* __restrict__ sib,
+                const int32_t* __restrict__ n_split, int64_t node_elems, int blocks_per_node) {
+  const int q = blockIdx.x / blocks_per_node;
+  if (q >= *n_split) return;
+  const int blk = blockIdx.x - q * blocks_per_node;
+  const int64_t pj = sib[3 * q], bj = sib[3 * q + 1], oj = sib[3 * q + 2];
+  for (int64_t i = (int64_t)blk * 256 * 4 + threadIdx.x; i < node_elems && i < (int64_t)(blk + 1) * 1024;
+       i += 256) {
+    child_g[oj * node_elems + i] = parent_g[pj * node_elems + i] - child_g[bj * node_elems + i];
+    child_h[oj * node_elems + i] = parent_h[pj * node_elems + i] - child_h[bj * node_elems + i];
+  }
+}
+
+int subtract_launch(const float* pg, const float* ph, float* cg, float* ch, const int32_t* sib,
+                    const int32_t* n_split, int64_t max_split, int64_t node_elems, hipStream_t st) {
+  if (max_split == 0 || node_elems == 0) return kOk;
+  const int bpn = (int)ceil_div(node_elems, 1024);
+  hipLaunchKernelGGL(k_hist_subtract, dim3((unsigned)(max_split * bpn)), dim3(256), 0, st, pg, ph, cg, ch, sib,
+                     n_split, node_elems, bpn);
+  WGBM_LAUNCHED("k_hist_subtract");
+  return kOk;
+}
+
+__global__ void k_part_single_init(int32_t* sc, int feature, int bin, int n) {
+  if (threadIdx.x == 0) { sc[0] = 1; sc[1] = feature; sc[2] = bin; sc[3] = 0; sc[4] = n; }
+}
+
+__global__ void k_leaf_single_init(int32_t* sc, int n) {
+  if (threadIdx.x == 0) { sc[0] = 1; sc[1] = 0; sc[2] = n; }
+}
+
+__global__ void k_sub_plain(const float* pg, const float* ph, const float* cg, const float* ch, float* sg,
+                            float* sh, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  sg[i] = pg[i] - cg[i];
+  sh[i] = ph[i] - ch[i];
+}
+
+// ----------------------------------------------------------------- objectives
+// Regression: g = (y - F) * w, h = w (warpgbm/core.py:1545-1547).
+// Written per row (g, h) and/or per position of order `perm0` (gh).
+__global__ void __launch_bounds__(256)
+k_grad_regression(const float* __restrict__ y, const float* __restrict__ pred, const float* __restrict__ w,
+                  int64_t n, const int32_t* __restrict__ perm0, float* __restrict__ g, float* __restrict__ h,
+                  float2* __restrict__ gh) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int64_t r = perm0 ? perm0[p] : p;
+  const float wr = w[r];
+  const float gr = (y[r] - pred[r]) * wr;
+  if (g) { g[r] = gr; h[r] = wr; }
+  if (gh) gh[p] = make_float2(gr, wr);
+}
+
+// Softmax cross-entropy, one boosting round (warpgbm/core.py:339-367,
+// :1755-1756; softmax of warpgbm/metrics.py:12-15): p = softmax(F_row),
+// g_k = -(p_k - [y == k]) * w, h_k = max(p_k (1 - p_k), 1e-6) * w.
+// Output class-major: index k*n + position.
+__global__ void __launch_bounds__(256)
+k_grad_softmax(const float* __restrict__ F, const int32_t* __restrict__ y, const float* __restrict__ w,
+               int64_t n, int K, const int32_t* __restrict__ perm0, float* __restrict__ g,
+               float* __restrict__ h, float2* __restrict__ gh) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int64_t r = perm0 ? perm0[p] : p;
+  const float* f = F + r * K;
+  float mx = f[0];
+  for (int k = 1; k < K; ++k) mx = fmaxf(mx, f[k]);
+  float den = 0.0f;
+  for (int k = 0; k < K; ++k) den += expf(f[k] - mx);
+  const float wr = w[r];
+  const int yr = y[r];
+  for (int k = 0; k < K; ++k) {
+    const float pk = expf(f[k] - mx) / den;
+    const float gk = -(pk - (yr == k ? 1.0f : 0.0f)) * wr;
+    const float hk = fmaxf(pk * (1.0f - pk), 1e-6f) * wr;
+    if (g) { g[(int64_t)k * n + r] = gk; h[(int64_t)k * n + r] = hk; }
+    if (gh) gh[(int64_t)k * n + p] = make_float2(gk, hk);
+  }
+}
+
+}  // namespace wgbm
+
+using namespace wgbm;
+
+// ----------------------------------------------------------------- C ABI
+extern "C" size_t wgbm_partition_workspace_bytes(int64_t n_samples) {
+  if (n_samples < 0) return 0;
+  const int64_t mc = ceil_div(n_samples, kPartChunk) + 1;
+  return (size_t)(4 * (16 + 3 * mc) + 64 * 4) + 1024;
+}
+
+// Stable split of one node's row list (plain path): rows with
+// code[row, feature] <= bin first, then the others, each in input order.
+// out_rows[0:n_left] / out_rows[n_left:n]; n_left written to *n_left_out
+// (device int32).
+extern "C" int wgbm_partition_rows(const int8_t* bin_indices, int64_t n_total_rows, int64_t n_master_feat,
+                                   const int32_t* rows, int64_t n, int32_t feature, int32_t bin,
+                                   int32_t* out_rows, int32_t* n_left_out, void* workspace,
+                                   size_t workspace_bytes, hipStream_t stream) {
+  WGBM_REQUIRE(n >= 0 && n < 0x7fffffff, "row count out of range");
+  WGBM_REQUIRE(feature >= 0 && feature < n_master_feat, "split feature %d out of range [0, %lld)", feature,
+               (long long)n_master_feat);
+  WGBM_REQUIRE(workspace_bytes >= wgbm_partition_workspace_bytes(n), "partition workspace too small");
+  const int64_t mc = ceil_div(n, kPartChunk) + 1;
+  int32_t* w = static_cast<int32_t*>(workspace);
+  // scalars / single-node tables at the front, chunk arrays after.
+  int32_t* sc = w;            // 64 ints of scalars and tiny tables
+  int32_t* lbase = w + 64;
+  int32_t* rbase = lbase + mc;
+  int32_t* multi = rbase + mc;
+  PartArgs a{};
+  a.codes = bin_indices; a.row_stride = n_master_feat; a.n_rows = n_total_rows;
+  a.feat_idx = nullptr;
+  a.n_nodes = 1; a.n_eras = 1;
+  a.node_active = sc + 0;    // 1
+  a.dec_feat = sc + 1;       // feature
+  a.dec_bin = sc + 2;        // bin
+  a.seg_start = sc + 3;      // 0
+  a.seg_count = sc + 4;      // n
+  a.node_start = sc + 3;
+  a.node_count = sc + 4;
+  a.pcount = sc + 5;
+  a.seg_first = sc + 6;      // 2 ints
+  a.nseg_dev = sc + 8;
+  a.n_multi = sc + 9;
+  a.segL = sc + 10;
+  a.c_seg_start = sc + 12; a.c_seg_count = sc + 14; a.c_active = sc + 16;
+  a.c_node_start = sc + 18; a.c_node_count = sc + 20;
+  a.b_seg_start = sc + 22; a.b_seg_count = sc + 23; a.b_seg_slot = sc + 24;
+  a.n_build = sc + 25; a.n_split = sc + 26; a.sib = sc + 27;  // 3 ints
+  a.multi = multi;
+  a.lbase = lbase; a.rbase = rbase;
+  a.perm_in = rows; a.gh_in = nullptr;
+  a.perm_out = out_rows; a.gh_out = nullptr;
+  a.chunk_rows = kPartChunk;
+  hipLaunchKernelGGL(k_part_single_init, dim3(1), dim3(64), 0, stream, sc, feature, bin, (int)n);
+  WGBM_LAUNCHED("k_part_single_init");
+  const int rc = partition_launch(a, n, stream);
+  if (rc != kOk) return rc;
+  WGBM_HIP(hipMemcpyAsync(n_left_out, a.c_node_count, sizeof(int32_t), hipMemcpyDeviceToDevice, stream));
+  return kOk;
+}
+
+extern "C" size_t wgbm_leaf_workspace_bytes(int64_t n_samples) {
+  if (n_samples < 0) return 0;
+  const int64_t mc = leaf_max_chunks(n_samples, 1);
+  return (size_t)(64 * 4 + 8 * 2 * mc + 4 * mc) + 1024;
+}
+
+// Leaf value of one node (plain path): sum(g)/(sum(h)+lambda) over its rows
+// in float64 (warpgbm/core.py:1377-1380), written as float32 to *leaf_out,
+// and pred[row*stride + col] += f32(lr) * leaf for every row when pred is set.
+extern "C" int wgbm_leaf_value(const float* grads, const float* hess, const int32_t* rows, int64_t n,
+                               int64_t n_total_rows, double lambda, float learning_rate, float* pred,
+                               int32_t pred_stride, int32_t pred_col, float* leaf_out, void* workspace,
+                               size_t workspace_bytes, hipStream_t stream) {
+  WGBM_REQUIRE(n >= 0 && n < 0x7fffffff, "row count out of range");
+  WGBM_REQUIRE(workspace_bytes >= wgbm_leaf_workspace_bytes(n), "leaf workspace too small");
+  WGBM_REQUIRE(pred == nullptr || (pred_stride >= 1 && pred_col >= 0 && pred_col < pred_stride),
+               "bad prediction column");
+  int32_t* sc = static_cast<int32_t*>(workspace);
+  const int64_t mc = leaf_max_chunks(n, 1);
+  LeafArgs a{};
+  a.n_nodes = 1;
+  a.node_active = sc + 0;
+  a.dec_feat = nullptr;
+  a.node_start = sc + 1;
+  a.node_count = sc + 2;
+  a.lcount = sc + 3;
+  a.seg_first = sc + 4;  // 2
+  a.nseg_dev = sc + 6;
+  a.n_multi = sc + 7;
+  a.perm = rows; a.gh = nullptr; a.g = grads; a.h = hess;
+  a.n_rows = n_total_rows;
+  a.lam = lambda; a.lr = learning_rate;
+  a.pred = pred;
+  a.pred_stride = pred_stride;
+  a.nodes_per_class = 1;
+  a.leaf_val = leaf_out;
+  a.node_sum = nullptr;
+  double* part = reinterpret_cast<double*>(reinterpret_cast<char*>(workspace) + 256);
+  a.part = part;
+  a.multi = reinterpret_cast<int32_t*>(part + 2 * mc);
+  if (pred) a.pred = pred + pred_col;
+  hipLaunchKernelGGL(k_leaf_single_init, dim3(1), dim3(64), 0, stream, sc, (int)n);
+  WGBM_LAUNCHED("k_leaf_single_init");
+  return leaf_launch(a, n, stream);
+}
+
+extern "C" int wgbm_grad_regression(const float* y, const float* pred, const float* w, int64_t n,
+                                    float* g_out, float* h_out, hipStream_t stream) {
+  WGBM_REQUIRE(n >= 0, "row count out of range");
+  if (n == 0) return kOk;
+  hipLaunchKernelGGL(k_grad_regression, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, stream, y, pred, w, n,
+                     (const int32_t*)nullptr, g_out, h_out, (float2*)nullptr);
+  WGBM_LAUNCHED("k_grad_regression");
+  return kOk;
+}
+
+extern "C" int wgbm_grad_softmax(const float* scores, const int32_t* labels, const float* w, int64_t n,
+                                 int32_t n_classes, float* g_out, float* h_out, hipStream_t stream) {
+  WGBM_REQUIRE(n >= 0 && n_classes >= 2, "bad softmax shape");
+  if (n == 0) return kOk;
+  hipLaunchKernelGGL(k_grad_softmax, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, stream, scores, labels, w,
+                     n, (int)n_classes, (const int32_t*)nullptr, g_out, h_out, (float2*)nullptr);
+  WGBM_LAUNCHED("k_grad_softmax");
+  return kOk;
+}
+
+extern "C" int wgbm_hist_subtract(const float* parent_g, const float* parent_h, const float* child_g,
+                                  const float* child_h, float* sib_g, float* sib_h, int64_t n,
+                                  hipStream_t stream) {
+  WGBM_REQUIRE(n >= 0, "element count out of range");
+  if (n == 0) return kOk;
+  hipLaunchKernelGGL(k_sub_plain, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, stream, parent_g, parent_h,
+                     child_g, child_h, sib_g, sib_h, n);
+  WGBM_LAUNCHED("k_sub_plain");
+  return kOk;
+}

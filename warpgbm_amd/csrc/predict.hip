@@ -1,0 +1,224 @@
+// Forest traversal (gfx950).
+//
+// 1. wgbm_predict_forest: the reference op on the reference's flattened
+//    forest [T, max_nodes, 6] = (feature, bin, left, right, is_leaf, value)
+//    (warpgbm/cuda/predict.cu:18-44, layout warpgbm/core.py:2283-2328).
+//    One thread per row adds lr*leaf of every tree IN TREE ORDER (no
+//    atomics), so results are deterministic and equal, bit for bit, to the
+//    scores training accumulates.
+// 2. A compiled "heap" forest for serving: every tree padded to a complete
+//    binary tree of depth D (children of heap node i are 2i+1, 2i+2; a leaf
+//    above depth D becomes pass-through nodes that always go left), internal
+//    nodes packed as (feature << 8 | bin) in 32 bits and leaves stored
+//    already multiplied by the learning rate.  Traversal is D branch-free
+//    steps.  A workgroup stages the codes of 64 rows in LDS; wave q walks
+//    trees q, q+4, ... of its tree block for those 64 rows; partial sums
+//    are combined in a fixed order (tree block, then wave), so a row's
+//    result does not depend on which other rows share the launch.
+#include "common.h"
+
+namespace wgbm {
+
+__global__ void __launch_bounds__(256)
+k_predict_ref(const int8_t* __restrict__ codes, int64_t N, int64_t F, const float* __restrict__ trees,
+              int64_t T, int64_t max_nodes, float lr, float* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const int8_t* row = codes + i * F;
+  float acc = out[i];
+  for (int64_t t = 0; t < T; ++t) {
+    const float* tr = trees + t * max_nodes * 6;
+    int64_t node = 0;
+    for (int64_t step = 0; step <= max_nodes; ++step) {
+      const float* nd = tr + node * 6;
+      if (nd[4] > 0.5f) {
+        acc = acc + lr * nd[5];
+        break;
+      }
+      const int64_t feat = (int64_t)nd[0];
+      const int bin = (int)nd[1];
+      const int64_t nxt = (feat >= 0 && feat < F && (int)row[feat] <= bin) ? (int64_t)nd[2] : (int64_t)nd[3];
+      if (!(nxt > 0 && nxt < max_nodes)) break;  // malformed tree: contributes nothing
+      node = nxt;
+    }
+  }
+  out[i] = acc;
+}
+
+// --------------------------------------------------------------- heap forest
+struct HeapForest {
+  const uint32_t* nodes;   // [T][2^D - 1]
+  const float* leaves;     // [T][2^D], lr already applied
+  int32_t T;
+  int32_t depth;
+  int32_t tree_block;      // trees per block (fixed per forest)
+};
+
+constexpr int kRowsPerWG = 64;
+
+// Partial sums for (row tile, class, tree block): out_part[(cls*n_blocks +
+// blk) * n + row] = sum over the block's trees (wave order, then tree order).
+__global__ void __launch_bounds__(256)
+k_predict_heap(const int8_t* __restrict__ codes, int64_t n, int32_t F, HeapForest hf,
+               const int32_t* __restrict__ class_first, int32_t n_classes, int32_t n_blocks,
+               float* __restrict__ part) {
+  extern __shared__ int8_t s_codes[];  // [64][F]
+  __shared__ float s_sum[4][kRowsPerWG];
+  const int64_t row0 = (int64_t)blockIdx.x * kRowsPerWG;
+  const int cls = blockIdx.y / n_blocks;
+  const int blk = blockIdx.y - cls * n_blocks;
+  const int nrows = (int)min((int64_t)kRowsPerWG, n - row0);
+  // stage codes (row-major, contiguous rows) with 4-byte loads where aligned
+  const int64_t bytes = (int64_t)nrows * F;
+  const int8_t* src = codes + row0 * F;
+  if (((uintptr_t)src & 3) == 0) {
+    const int64_t w4 = bytes >> 2;
+    for (int64_t q = threadIdx.x; q < w4; q += 256)
+      reinterpret_cast<uint32_t*>(s_codes)[q] = reinterpret_cast<const uint32_t*>(src)[q];
+    for (int64_t q = (w4 << 2) + threadIdx.x; q < bytes; q += 256) s_codes[q] = src[q];
+  } else {
+    for (int64_t q = threadIdx.x; q < bytes; q += 256) s_codes[q] = src[q];
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int t_lo = class_first[cls] + blk * hf.tree_block;
+  const int t_hi = min(t_lo + hf.tree_block, class_first[cls + 1]);
+  const int internal = (1 << hf.depth) - 1;
+  const int8_t* myrow = s_codes + (int64_t)(lane < nrows ? lane : 0) * F;
+  float acc = 0.0f;
+  int t = t_lo + wv;
+  // two independent trees in flight per lane
+  for (; t + 4 < t_hi; t += 8) {
+    const uint32_t* n0 = hf.nodes + (int64_t)t * internal;
+    const uint32_t* n1 = hf.nodes + (int64_t)(t + 4) * internal;
+    int a = 0, b = 0;
+    for (int d = 0; d < hf.depth; ++d) {
+      const uint32_t x = n0[a], y = n1[b];
+      const int ca = myrow[x >> 8], cb = myrow[y >> 8];
+      a = 2 * a + 1 + (ca > (int)(x & 0xff));
+      b = 2 * b + 1 + (cb > (int)(y & 0xff));
+    }
+    const float va = hf.leaves[(int64_t)t * (internal + 1) + (a - internal)];
+    const float vb = hf.leaves[(int64_t)(t + 4) * (internal + 1) + (b - internal)];
+    acc = acc + va;
+    acc = acc + vb;
+  }
+  for (; t < t_hi; t += 4) {
+    const uint32_t* n0 = hf.nodes + (int64_t)t * internal;
+    int a = 0;
+    for (int d = 0; d < hf.depth; ++d) {
+      const uint32_t x = n0[a];
+      a = 2 * a + 1 + ((int)myrow[x >> 8] > (int)(x & 0xff));
+    }
+    acc = acc + hf.leaves[(int64_t)t * (internal + 1) + (a - internal)];
+  }
+  s_sum[wv][lane] = acc;
+  __syncthreads();
+  if (threadIdx.x < nrows) {
+    const int r = threadIdx.x;
+    const float v = ((s_sum[0][r] + s_sum[1][r]) + s_sum[2][r]) + s_sum[3][r];
+    part[((int64_t)cls * n_blocks + blk) * n + row0 + r] = v;
+  }
+}
+
+// out[row*K + cls] = base[cls] + sum of block partials in block order.
+__global__ void __launch_bounds__(256)
+k_predict_heap_final(const float* __restrict__ part, int64_t n, int32_t n_classes, int32_t n_blocks,
+                     const int32_t* __restrict__ blocks_of_class, const float* __restrict__ base,
+                     float* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n * n_classes) return;
+  const int64_t r = i / n_classes;
+  const int c = (int)(i - r * n_classes);
+  float acc = base[c];
+  const int nb = blocks_of_class[c];
+  for (int b = 0; b < nb; ++b) acc = acc + part[((int64_t)c * n_blocks + b) * n + r];
+  out[i] = acc;
+}
+
+// Build the heap forest from level-order node tables (heap ids): status
+// 1 = split (feature, bin), 2 = leaf (value), 0 = absent.  Thread per (tree,
+// heap slot of the padded tree).
+__global__ void __launch_bounds__(256)
+k_heap_compile(const int32_t* __restrict__ status, const int32_t* __restrict__ feat,
+               const int32_t* __restrict__ bin, const float* __restrict__ value, int32_t src_nodes,
+               int32_t T, int32_t depth, float lr, uint32_t* __restrict__ nodes, float* __restrict__ leaves) {
+  const int internal = (1 << depth) - 1;
+  const int total = 2 * internal + 1;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)T * total) return;
+  const int t = (int)(i / total);
+  const int h = (int)(i - (int64_t)t * total);
+  const int32_t* st = status + (int64_t)t * src_nodes;
+  // nearest ancestor-or-self that exists in the source table
+  int a = h;
+  while (a > 0 && (a >= src_nodes || st[a] == 0)) a = (a - 1) >> 1;
+  if (h < internal) {
+    uint32_t v = 127u;  // pass-through: feature 0, bin 127 -> always left
+    if (a == h && st[a] == 1) v = ((uint32_t)feat[(int64_t)t * src_nodes + a] << 8) | (uint32_t)(bin[(int64_t)t * src_nodes + a] & 0xff);
+    nodes[(int64_t)t * internal + h] = v;
+  } else {
+    float v = 0.0f;
+    if (a < src_nodes && st[a] == 2) v = lr * value[(int64_t)t * src_nodes + a];
+    leaves[(int64_t)t * (internal + 1) + (h - internal)] = v;
+  }
+}
+
+}  // namespace wgbm
+
+using namespace wgbm;
+
+extern "C" int wgbm_predict_forest(const int8_t* bin_indices, int64_t n_rows, int64_t n_feat, const float* trees,
+                                   int64_t n_trees, int64_t max_nodes, float learning_rate, float* out,
+                                   hipStream_t stream) {
+  WGBM_REQUIRE(n_rows >= 0 && n_feat >= 0 && n_trees >= 0 && max_nodes >= 1, "bad forest shape");
+  if (n_rows == 0 || n_trees == 0) return kOk;
+  hipLaunchKernelGGL(k_predict_ref, dim3((unsigned)ceil_div(n_rows, 256)), dim3(256), 0, stream, bin_indices,
+                     n_rows, n_feat, trees, n_trees, max_nodes, learning_rate, out);
+  WGBM_LAUNCHED("k_predict_ref");
+  return kOk;
+}
+
+extern "C" int wgbm_heap_compile(const int32_t* status, const int32_t* feat, const int32_t* bin, const float* value,
+                                 int32_t src_nodes, int32_t n_trees, int32_t depth, float learning_rate,
+                                 uint32_t* nodes, float* leaves, hipStream_t stream) {
+  WGBM_REQUIRE(depth >= 1 && depth <= 16 && n_trees >= 0 && src_nodes >= 1, "bad heap forest shape");
+  const int64_t total = (int64_t)n_trees * ((2 << depth) - 1);
+  if (total == 0) return kOk;
+  hipLaunchKernelGGL(k_heap_compile, dim3((unsigned)ceil_div(total, 256)), dim3(256), 0, stream, status, feat, bin,
+                     value, src_nodes, n_trees, depth, learning_rate, nodes, leaves);
+  WGBM_LAUNCHED("k_heap_compile");
+  return kOk;
+}
+
+extern "C" size_t wgbm_heap_predict_workspace_bytes(int64_t n_rows, int32_t n_classes, int32_t n_blocks) {
+  return (size_t)n_rows * n_classes * n_blocks * sizeof(float);
+}
+
+// Serve one batch of rows: out[row*K + k] = base[k] + sum over class-k trees.
+// class_first [K+1]: tree ranges per class (trees of a class contiguous);
+// blocks_of_class [K]: ceil(trees of class / tree_block).
+extern "C" int wgbm_heap_predict(const int8_t* codes, int64_t n_rows, int32_t n_feat, const uint32_t* nodes,
+                                 const float* leaves, int32_t n_trees, int32_t depth, int32_t tree_block,
+                                 const int32_t* class_first, const int32_t* blocks_of_class, int32_t n_classes,
+                                 int32_t n_blocks, const float* base, float* out, void* workspace,
+                                 size_t workspace_bytes, hipStream_t stream) {
+  WGBM_REQUIRE(n_rows >= 0 && n_feat >= 1 && depth >= 1 && depth <= 16 && tree_block >= 1 && n_classes >= 1 &&
+                   n_blocks >= 1,
+               "bad heap predict arguments");
+  WGBM_REQUIRE((int64_t)n_feat * kRowsPerWG <= 96 * 1024, "too many features for the LDS row tile (%d)", n_feat);
+  WGBM_REQUIRE(workspace_bytes >= wgbm_heap_predict_workspace_bytes(n_rows, n_classes, n_blocks),
+               "predict workspace too small");
+  if (n_rows == 0) return kOk;
+  HeapForest hf{nodes, leaves, n_trees, depth, tree_block};
+  float* part = static_cast<float*>(workspace);
+  const dim3 grid((unsigned)ceil_div(n_rows, kRowsPerWG), (unsigned)(n_classes * n_blocks));
+  const size_t lds = (size_t)n_feat * kRowsPerWG;
+  hipLaunchKernelGGL(k_predict_heap, grid, dim3(256), lds, stream, codes, n_rows, n_feat, hf, class_first,
+                     n_classes, n_blocks, part);
+  WGBM_LAUNCHED("k_predict_heap");
+  hipLaunchKernelGGL(k_predict_heap_final, dim3((unsigned)ceil_div(n_rows * n_classes, 256)), dim3(256), 0, stream,
+                     part, n_rows, n_classes, n_blocks, blocks_of_class, base, out);
+  WGBM_LAUNCHED("k_predict_heap_final");
+  return kOk;
+}

@@ -1,0 +1,321 @@
+// Directional Era-Splitting gain scan and split selection (gfx950).
+//
+// Per (era, feature) the scan walks bins in order: prefix G_L, H_L, then for
+// every threshold b < B-1 the reference's gain and direction
+// (warpgbm/cuda/best_split_kernel.cu:25-54).  Prefix sums and the gain are
+// evaluated in float64 from the float32 histogram and rounded once to
+// float32; all arithmetic here is built with -ffp-contract=off so that the
+// per-node op and the batched level scan share bit-identical results.
+//
+// Selection (warpgbm/core.py:1322-1362): E == 1 -> crit = gain, valid iff
+// crit > min_split_gain; E > 1 -> crit = (sum_e gain_e)/E in float32 (era
+// order), agreement = |sum_e dir_e|, only candidates at the node's maximum
+// agreement may win.  Monotone constraints require dir == required in every
+// era.  Winner = lexicographic max of (agreement, crit, -flat index); a
+// winner whose crit is -inf (no valid candidate) means "leaf".
+#include "common.h"
+#include "wgbm_internal.h"
+#include <math.h>
+
+namespace wgbm {
+
+struct Cand {
+  int agree;    // |sum over eras of direction| (0 when E == 1)
+  float crit;   // -inf when the candidate is masked out
+  int idx;      // flat index f_local*(B-1) + b, INT_MAX for "none"
+};
+
+__device__ __forceinline__ bool cand_better(const Cand& a, const Cand& b) {
+  if (a.agree != b.agree) return a.agree > b.agree;
+  if (a.crit != b.crit) return a.crit > b.crit;
+  return a.idx < b.idx;
+}
+
+__device__ __forceinline__ Cand cand_none() { return Cand{-1, -INFINITY, 0x7fffffff}; }
+
+__device__ __forceinline__ void gain_dir(double GL, double HL, double Gt, double Ht, double mcw,
+                                         double lam, float& gain, float& dir) {
+  const double GR = Gt - GL;
+  const double HR = Ht - HL;
+  if (HL >= mcw && HR >= mcw) {
+    const double a = GL * GL / (HL + lam);
+    const double b = GR * GR / (HR + lam);
+    const double c = Gt * Gt / (Ht + lam);
+    gain = (float)((a + b) - c);
+    dir = (GL / (HL + lam) > GR / (HR + lam)) ? 1.0f : -1.0f;
+  } else {
+    gain = 0.0f;
+    dir = 0.0f;
+  }
+}
+
+__device__ __forceinline__ Cand wave_best(Cand c) {
+  for (int o = 32; o >= 1; o >>= 1) {
+    Cand d;
+    d.agree = __shfl_xor(c.agree, o);
+    d.crit = __shfl_xor(c.crit, o);
+    d.idx = __shfl_xor(c.idx, o);
+    if (cand_better(d, c)) c = d;
+  }
+  return c;
+}
+
+// Reference op: per-era gain and direction, thread per (era, feature).
+__global__ void __launch_bounds__(256)
+k_compute_split(const float* __restrict__ G, const float* __restrict__ H, int E, int F, int B,
+                float mcw, float lam, float* __restrict__ gain, float* __restrict__ dir) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)E * F) return;
+  const float* g = G + t * B;
+  const float* h = H + t * B;
+  double Gt = 0.0, Ht = 0.0;
+  for (int b = 0; b < B; ++b) { Gt += (double)g[b]; Ht += (double)h[b]; }
+  double GL = 0.0, HL = 0.0;
+  float* go = gain + t * (B - 1);
+  float* dO = dir + t * (B - 1);
+  for (int b = 0; b < B - 1; ++b) {
+    GL += (double)g[b];
+    HL += (double)h[b];
+    float ga, di;
+    gain_dir(GL, HL, Gt, Ht, (double)mcw, (double)lam, ga, di);
+    go[b] = ga;
+    dO[b] = di;
+  }
+}
+
+// Candidate of flat index c = f*(B-1)+b from per-era gain/dir arrays.
+__device__ __forceinline__ Cand eval_cand(const float* __restrict__ gain, const float* __restrict__ dir,
+                                          int E, int FB, int c, float msg, int req) {
+  Cand k;
+  k.idx = c;
+  if (E == 1) {
+    const float cr = gain[c];
+    const float d = dir[c];
+    const bool ok = (cr > msg) && (req == 0 || d == (float)req);
+    k.agree = 0;
+    k.crit = ok ? cr : -INFINITY;
+  } else {
+    float sg = 0.0f;
+    int sd = 0;
+    bool mono = true;
+    for (int e = 0; e < E; ++e) {
+      const float d = dir[(int64_t)e * FB + c];
+      sg += gain[(int64_t)e * FB + c];
+      sd += (int)d;
+      mono = mono && (req == 0 || d == (float)req);
+    }
+    const float cr = sg / (float)E;
+    k.agree = sd < 0 ? -sd : sd;
+    k.crit = (cr > msg && mono) ? cr : -INFINITY;
+  }
+  return k;
+}
+
+// Plain-path selection for one node: out[0] = local feature, out[1] = bin,
+// or (-1, -1) when no candidate is valid.
+__global__ void __launch_bounds__(1024)
+k_select_split(const float* __restrict__ gain, const float* __restrict__ dir, int E, int F, int B,
+               float msg, const int8_t* __restrict__ mono_req, const int32_t* __restrict__ feat_idx,
+               int32_t* __restrict__ out) {
+  __shared__ Cand s_c[16];
+  const int FB = F * (B - 1);
+  Cand best = cand_none();
+  for (int c = threadIdx.x; c < FB; c += blockDim.x) {
+    const int f = c / (B - 1);
+    const int req = mono_req ? (int)mono_req[feat_idx ? feat_idx[f] : f] : 0;
+    const Cand k = eval_cand(gain, dir, E, FB, c, msg, req);
+    if (cand_better(k, best)) best = k;
+  }
+  best = wave_best(best);
+  if ((threadIdx.x & 63) == 0) s_c[threadIdx.x >> 6] = best;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    Cand b = s_c[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w)
+      if (cand_better(s_c[w], b)) b = s_c[w];
+    if (b.idx == 0x7fffffff || b.crit == -INFINITY) { out[0] = -1; out[1] = -1; }
+    else { out[0] = b.idx / (B - 1); out[1] = b.idx % (B - 1); }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Batched level scan.  Thread per (node, local feature); the per-era gains
+// are the same float32 values k_compute_split would produce, and the era
+// sums run in era order, so the candidate is bit-identical to the plain op
+// followed by k_select_split.  One candidate per (node, workgroup) goes to
+// `cand`; k_level_pick reduces them in workgroup order.
+constexpr int kScanBlk = 32;   // thresholds per register block (E > 1)
+
+__global__ void __launch_bounds__(256)
+k_level_scan(const LevelScanArgs a) {
+  const int wg_per_node = (a.n_feat + blockDim.x - 1) / blockDim.x;
+  const int node = blockIdx.x / wg_per_node;
+  const int wg = blockIdx.x - node * wg_per_node;
+  const int f = wg * blockDim.x + threadIdx.x;
+  Cand best = cand_none();
+  const bool active = node < a.n_nodes && a.node_active[node] != 0;
+  if (active && f < a.n_feat) {
+    const int B = a.n_bins, E = a.n_eras;
+    const int req = a.mono_req ? (int)a.mono_req[a.feat_idx ? a.feat_idx[f] : f] : 0;
+    const int64_t node_off = (int64_t)node * E * a.n_feat * B;
+    if (E == 1) {
+      const float* g = a.G + node_off + (int64_t)f * B;
+      const float* h = a.H + node_off + (int64_t)f * B;
+      double Gt = 0.0, Ht = 0.0;
+      for (int b = 0; b < B; ++b) { Gt += (double)g[b]; Ht += (double)h[b]; }
+      double GL = 0.0, HL = 0.0;
+      for (int b = 0; b < B - 1; ++b) {
+        GL += (double)g[b];
+        HL += (double)h[b];
+        float ga, di;
+        gain_dir(GL, HL, Gt, Ht, (double)a.mcw, (double)a.lam, ga, di);
+        const bool ok = (ga > a.msg) && (req == 0 || di == (float)req);
+        Cand k{0, ok ? ga : -INFINITY, f * (B - 1) + b};
+        if (cand_better(k, best)) best = k;
+      }
+    } else {
+      for (int b0 = 0; b0 < B - 1; b0 += kScanBlk) {
+        const int nb = min(kScanBlk, B - 1 - b0);
+        float sg[kScanBlk];
+        int sd[kScanBlk];
+        bool mono[kScanBlk];
+#pragma unroll
+        for (int q = 0; q < kScanBlk; ++q) { sg[q] = 0.0f; sd[q] = 0; mono[q] = true; }
+        for (int e = 0; e < E; ++e) {
+          const float* g = a.G + node_off + ((int64_t)e * a.n_feat + f) * B;
+          const float* h = a.H + node_off + ((int64_t)e * a.n_feat + f) * B;
+          double Gt = 0.0, Ht = 0.0;
+          for (int b = 0; b < B; ++b) { Gt += (double)g[b]; Ht += (double)h[b]; }
+          double GL = 0.0, HL = 0.0;
+          for (int b = 0; b < b0; ++b) { GL += (double)g[b]; HL += (double)h[b]; }
+#pragma unroll
+          for (int q = 0; q < kScanBlk; ++q) {
+            if (q < nb) {
+              GL += (double)g[b0 + q];
+              HL += (double)h[b0 + q];
+              float ga, di;
+              gain_dir(GL, HL, Gt, Ht, (double)a.mcw, (double)a.lam, ga, di);
+              sg[q] += ga;
+              sd[q] += (int)di;
+              mono[q] = mono[q] && (req == 0 || di == (float)req);
+            }
+          }
+        }
+#pragma unroll
+        for (int q = 0; q < kScanBlk; ++q) {
+          if (q < nb) {
+            const float cr = sg[q] / (float)E;
+            Cand k{sd[q] < 0 ? -sd[q] : sd[q], (cr > a.msg && mono[q]) ? cr : -INFINITY,
+                   f * (B - 1) + b0 + q};
+            if (cand_better(k, best)) best = k;
+          }
+        }
+      }
+    }
+  }
+  __shared__ Cand s_c[4];
+  best = wave_best(best);
+  if ((threadIdx.x & 63) == 0) s_c[threadIdx.x >> 6] = best;
+  __syncthreads();
+  if (threadIdx.x == 0 && node < a.n_nodes) {
+    Cand b = s_c[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w)
+      if (cand_better(s_c[w], b)) b = s_c[w];
+    const int64_t o = (int64_t)node * wg_per_node + wg;
+    a.cand_agree[o] = b.agree;
+    a.cand_crit[o] = b.crit;
+    a.cand_idx[o] = b.idx;
+  }
+}
+
+// Per node: reduce the workgroup candidates in order, write the decision
+// (local feature, bin) or (-1, -1), and the per-era gain of the winner
+// (feature importance, warpgbm/core.py:1418-1423).
+__global__ void __launch_bounds__(64)
+k_level_pick(const LevelScanArgs a) {
+  const int node = blockIdx.x;
+  if (node >= a.n_nodes) return;
+  const int wg_per_node = (a.n_feat + 255) / 256;
+  __shared__ int s_f, s_b;
+  if (threadIdx.x == 0) {
+    int fl = -1, bin = -1;
+    if (a.node_active[node]) {
+      Cand b = cand_none();
+      for (int w = 0; w < wg_per_node; ++w) {
+        const int64_t o = (int64_t)node * wg_per_node + w;
+        Cand k{a.cand_agree[o], a.cand_crit[o], a.cand_idx[o]};
+        if (cand_better(k, b)) b = k;
+      }
+      if (b.idx != 0x7fffffff && b.crit != -INFINITY) {
+        fl = b.idx / (a.n_bins - 1);
+        bin = b.idx % (a.n_bins - 1);
+      }
+    }
+    a.dec_feat[node] = fl;
+    a.dec_bin[node] = bin;
+    s_f = fl;
+    s_b = bin;
+  }
+  __syncthreads();
+  const int fl = s_f, bin = s_b;
+  if (a.era_gain == nullptr) return;
+  const int B = a.n_bins, E = a.n_eras;
+  for (int e = threadIdx.x; e < E; e += blockDim.x) {
+    float ga = 0.0f;
+    if (fl >= 0) {
+      const int64_t off = (((int64_t)node * E + e) * a.n_feat + fl) * B;
+      const float* g = a.G + off;
+      const float* h = a.H + off;
+      double Gt = 0.0, Ht = 0.0;
+      for (int b = 0; b < B; ++b) { Gt += (double)g[b]; Ht += (double)h[b]; }
+      double GL = 0.0, HL = 0.0;
+      for (int b = 0; b <= bin; ++b) { GL += (double)g[b]; HL += (double)h[b]; }
+      float di;
+      gain_dir(GL, HL, Gt, Ht, (double)a.mcw, (double)a.lam, ga, di);
+    }
+    a.era_gain[(int64_t)node * E + e] = ga;
+  }
+}
+
+int level_scan_launch(const LevelScanArgs& a, hipStream_t st) {
+  if (a.n_nodes == 0) return kOk;
+  const int wg_per_node = (int)ceil_div(a.n_feat, 256);
+  hipLaunchKernelGGL(k_level_scan, dim3((unsigned)(a.n_nodes * (int64_t)wg_per_node)), dim3(256), 0, st, a);
+  WGBM_LAUNCHED("k_level_scan");
+  hipLaunchKernelGGL(k_level_pick, dim3((unsigned)a.n_nodes), dim3(64), 0, st, a);
+  WGBM_LAUNCHED("k_level_pick");
+  return kOk;
+}
+
+}  // namespace wgbm
+
+using namespace wgbm;
+
+extern "C" int wgbm_compute_split(const float* grad_hist, const float* hess_hist, int32_t n_eras,
+                                  int32_t n_feat, int32_t n_bins, float min_split_gain,
+                                  float min_child_weight, float lambda, float* per_era_gain,
+                                  float* per_era_direction, hipStream_t stream) {
+  (void)min_split_gain;  // the reference kernel receives it but does not use it
+  WGBM_REQUIRE(n_eras >= 1 && n_feat >= 0 && n_bins >= 2, "bad histogram shape [%d, %d, %d]", n_eras,
+               n_feat, n_bins);
+  const int64_t n = (int64_t)n_eras * n_feat;
+  if (n == 0) return kOk;
+  hipLaunchKernelGGL(k_compute_split, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, stream, grad_hist,
+                     hess_hist, n_eras, n_feat, n_bins, min_child_weight, lambda, per_era_gain,
+                     per_era_direction);
+  WGBM_LAUNCHED("k_compute_split");
+  return kOk;
+}
+
+extern "C" int wgbm_select_split(const float* per_era_gain, const float* per_era_direction, int32_t n_eras,
+                                 int32_t n_feat, int32_t n_bins, float min_split_gain,
+                                 const int8_t* mono_required, const int32_t* feature_indices,
+                                 int32_t* out_feat_bin, hipStream_t stream) {
+  WGBM_REQUIRE(n_eras >= 1 && n_feat >= 0 && n_bins >= 2, "bad gain shape [%d, %d, %d]", n_eras, n_feat,
+               n_bins);
+  WGBM_REQUIRE((int64_t)n_feat * (n_bins - 1) < 0x7fffffff, "too many split candidates");
+  hipLaunchKernelGGL(k_select_split, dim3(1), dim3(1024), 0, stream, per_era_gain, per_era_direction,
+                     n_eras, n_feat, n_bins, min_split_gain, mono_required, feature_indices, out_feat_bin);
+  WGBM_LAUNCHED("k_select_split");
+  return kOk;
+}

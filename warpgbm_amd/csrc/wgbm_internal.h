@@ -1,0 +1,79 @@
+// Internal (C++) interfaces shared between the kernel translation units.
+#pragma once
+#include "common.h"
+
+namespace wgbm {
+
+// Order-defining histogram configuration: chunk_rows (C) and waves (W) fix
+// the float32 summation order; fpl (features per lane) only changes the
+// load width.  Plain and batched builds must use the same C and W.
+struct HistConfig {
+  int chunk_rows = 0;
+  int waves = 0;
+  int fpl = 0;
+};
+
+struct HistArgs {
+  const int8_t* codes;     // [n_rows, row_stride] row-major bin codes
+  int64_t row_stride;      // columns of the bin matrix
+  int64_t n_rows;          // rows of the bin matrix
+  const int32_t* feat_idx; // [n_feat] column of each local feature (nullptr: identity)
+  int32_t n_feat;
+  int32_t n_bins;
+  const int32_t* perm;     // position -> row id
+  const float* g;          // per row (gather mode)
+  const float* h;
+  const float2* gh;        // per position (positional mode), or nullptr
+  const int32_t* seg_start;
+  const int32_t* seg_count;
+  const int32_t* seg_slot;
+  int32_t* seg_first;      // [nseg+1], written by the plan kernel
+  int32_t* seg_pfirst;     // [nseg] first partial slot of multi-chunk segments
+  const int32_t* nseg_dev; // device scalar: number of segments
+  int32_t chunk_rows;
+  int32_t nslab;
+  int32_t wide_ok;
+  float* out_g;            // [slot][n_feat][n_bins]
+  float* out_h;
+  float* part_g;           // chunk partials
+  float* part_h;
+  int32_t accumulate;      // 1: out += result (reference op), 0: out = result
+};
+
+// Configuration used by every histogram build: the automatic choice with
+// the process-wide override (wgbm_set_hist_config) applied.
+HistConfig pick_hist_config(int n_feat, int n_bins, int64_t row_stride);
+HistConfig auto_hist_config(int n_feat, int n_bins, int64_t row_stride);
+void set_hist_override(const HistConfig& c);
+size_t lds_bytes(const HistConfig& c, int n_bins);
+size_t hist_partial_floats(const HistConfig& c, int n_feat, int n_bins, int64_t rows_max);
+// Launch plan + build + merge.  rows_max / nseg_max bound the device-side
+// row and segment counts (they size the grid).  multi/n_multi: scratch of
+// rows_max/(C+1)+1 ints and one int.
+int hist_launch(HistArgs a, const HistConfig& c, int64_t rows_max, int64_t nseg_max,
+                int32_t* multi, int32_t* n_multi, hipStream_t st);
+
+// Batched level scan: candidates per (node, workgroup of 256 features),
+// then a per-node pick.  Histograms are [n_nodes][E][n_feat][B].
+struct LevelScanArgs {
+  const float* G;
+  const float* H;
+  int32_t n_nodes, n_eras, n_feat, n_bins;
+  const int32_t* node_active;   // [n_nodes]
+  const int32_t* feat_idx;      // local -> global feature (nullptr: identity)
+  const int8_t* mono_req;       // [global features] required direction, or nullptr
+  float msg, mcw, lam;
+  int32_t* cand_agree;          // [n_nodes * ceil(n_feat/256)]
+  float* cand_crit;
+  int32_t* cand_idx;
+  int32_t* dec_feat;            // [n_nodes] local feature or -1
+  int32_t* dec_bin;             // [n_nodes] bin or -1
+  float* era_gain;              // [n_nodes][E] gain of the winner, or nullptr
+};
+int level_scan_launch(const LevelScanArgs& a, hipStream_t st);
+
+// Chunk plan shared by histogram, partition and leaf kernels.
+void plan_launch(const int32_t* seg_count, const int32_t* nseg_dev, int chunk_rows,
+                 int32_t* seg_first, int32_t* multi, int32_t* n_multi, hipStream_t st);
+
+}  // namespace wgbm

@@ -1,0 +1,89 @@
+// Level-wise tree growth: partition and leaf argument blocks.
+#pragma once
+#include "common.h"
+
+namespace wgbm {
+
+constexpr int kPartChunk = 4096;
+
+struct PartArgs {
+  const int8_t* codes;
+  int64_t row_stride;
+  int64_t n_rows;
+  const int32_t* feat_idx;      // local -> global feature (nullptr: dec_feat is global)
+  int32_t n_nodes;              // nodes of this level
+  int32_t n_eras;
+  const int32_t* node_active;   // [n_nodes]
+  const int32_t* dec_feat;      // [n_nodes] local feature or -1
+  const int32_t* dec_bin;       // [n_nodes]
+  const int32_t* seg_start;     // [n_nodes*E]
+  const int32_t* seg_count;
+  const int32_t* node_start;    // [n_nodes]
+  const int32_t* node_count;
+  const int32_t* perm_in;       // positions of this level
+  const float2* gh_in;          // or nullptr
+  int32_t* perm_out;            // positions of the next level
+  float2* gh_out;               // or nullptr
+  // scratch
+  int32_t* pcount;              // [n_nodes*E]
+  int32_t* seg_first;           // [n_nodes*E+1]
+  int32_t* nseg_dev;            // scalar
+  int32_t* multi;               // plan scratch
+  int32_t* n_multi;
+  int32_t* lbase;               // [max chunks]
+  int32_t* rbase;
+  int32_t* segL;                // [n_nodes*E]
+  // next level tables ([2*n_nodes(*E)])
+  int32_t* c_seg_start;
+  int32_t* c_seg_count;
+  int32_t* c_active;
+  int32_t* c_node_start;
+  int32_t* c_node_count;
+  // smaller-child build list ([n_nodes*E]) and sibling map ([3*n_nodes])
+  int32_t* b_seg_start;
+  int32_t* b_seg_count;
+  int32_t* b_seg_slot;
+  int32_t* n_build;
+  int32_t* sib;
+  int32_t* n_split;
+  int32_t chunk_rows;
+};
+
+struct LeafArgs {
+  int32_t n_nodes;
+  const int32_t* node_active;
+  const int32_t* dec_feat;      // nullptr: every active node is a leaf
+  const int32_t* node_start;
+  const int32_t* node_count;
+  int32_t* lcount;              // scratch [n_nodes]
+  int32_t* seg_first;           // scratch [n_nodes+1]
+  int32_t* nseg_dev;
+  int32_t* multi;
+  int32_t* n_multi;
+  const int32_t* perm;
+  const float2* gh;             // positional (g, h), or nullptr for gather mode
+  const float* g;
+  const float* h;
+  int64_t n_rows;
+  double lam;
+  float lr;
+  float* pred;                  // nullable
+  int32_t pred_stride;
+  int32_t nodes_per_class;
+  float* leaf_val;              // [n_nodes]
+  double* node_sum;             // [n_nodes*2] or nullptr
+  double* part;                 // [max chunks * 2]
+};
+
+int partition_launch(const PartArgs& a, int64_t rows_max, hipStream_t st);
+int leaf_launch(const LeafArgs& a, int64_t rows_max, hipStream_t st);
+int64_t leaf_max_chunks(int64_t rows_max, int64_t n_nodes);
+int subtract_launch(const float* pg, const float* ph, float* cg, float* ch, const int32_t* sib,
+                    const int32_t* n_split, int64_t max_split, int64_t node_elems, hipStream_t st);
+
+__global__ void k_part_single_init(int32_t* sc, int feature, int bin, int n);
+__global__ void k_leaf_single_init(int32_t* sc, int n);
+__global__ void k_sub_plain(const float* pg, const float* ph, const float* cg, const float* ch, float* sg,
+                            float* sh, int64_t n);
+
+}  // namespace wgbm
