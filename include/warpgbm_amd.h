@@ -115,6 +115,13 @@ int wgbm_grad_softmax(const float* scores, const int32_t* labels, const float* w
 
 /* ---------------------------------------------------------------- binning */
 
+/* Quantile edges from column-sorted data: sorted [n, ncol] float32 (each
+ * column ascending, NaN last) -> edges[c*edge_row_stride + j-1], j = 1..B-1,
+ * with torch.quantile's float32 linear interpolation at
+ * torch.linspace(0, 1, B+1)[1:-1] (warpgbm/core.py:1271-1278). */
+int wgbm_edges_from_sorted(const float* sorted, int64_t n, int32_t ncol, int32_t num_bins, float* edges,
+                           int32_t edge_row_stride, hipStream_t stream);
+
 /* Whole-matrix binning: X [n, n_feat] row-major float32 with edges
  * [n_feat, n_edges] -> codes [n, n_feat] int8, one launch
  * (replaces the per-column loops of warpgbm/core.py:1271-1282, :1893-1898). */

@@ -33,15 +33,49 @@ def linspace_f32(num_bins: int) -> np.ndarray:
     return out[1:-1]
 
 
+def sort_total_order(col: np.ndarray) -> np.ndarray:
+    """Ascending float32 sort with -0.0 ordered before +0.0 (the order of a
+    radix sort on the float bits).  Plain comparison sorts leave the two
+    zeros in arbitrary relative order, which can flip the sign of a zero
+    edge (never its value, so never a code)."""
+    bits = col.view(np.int32)
+    key = np.where(bits < 0, ~bits ^ np.int32(-0x80000000), bits)
+    return col[np.argsort(key, kind="stable")]
+
+
+def fma_f32(a, b, c) -> np.float32:
+    """Correctly rounded float32 fused multiply-add a*b + c (exact rational
+    arithmetic, then one round-to-nearest-even to float32)."""
+    from fractions import Fraction
+    a, b, c = F32(a), F32(b), F32(c)
+    if not (np.isfinite(a) and np.isfinite(b) and np.isfinite(c)):
+        return F32(np.float64(a) * np.float64(b) + np.float64(c))
+    x = Fraction(float(a)) * Fraction(float(b)) + Fraction(float(c))
+    y = F32(float(x))  # within one float32 ulp of x
+    best = None
+    for cand in (np.nextafter(y, F32(-np.inf)), y, np.nextafter(y, F32(np.inf))):
+        err = abs(Fraction(float(cand)) - x)
+        if best is None or err < best[0] or (err == best[0] and (int(cand.view(np.int32)) & 1) == 0):
+            best = (err, cand)
+    r = F32(best[1])
+    if r == 0:
+        if x == 0:  # exact zero: -0 only when a*b and c are both negative zeros
+            neg = (bool(np.signbit(a)) != bool(np.signbit(b))) and bool(np.signbit(c))
+        else:       # underflow to zero keeps the sign of the exact result
+            neg = x < 0
+        r = F32(-0.0) if neg else F32(0.0)
+    return r
+
+
 def quantile_edges_f32(col: np.ndarray, num_bins: int) -> np.ndarray:
     """Bin edges of one feature column: ``torch.quantile(col, q)`` in float32.
 
     Follows warpgbm/core.py:1271-1278 (linear interpolation).  Definition:
-    sort the column; ``rank = f32(q*(N-1))``; ``lo = floor(rank)``,
-    ``hi = ceil(rank)``; ``w = f32(rank - lo)``; lerp
-    ``w < 0.5 ? a + w*(b-a) : b - (b-a)*(1-w)`` with every operation rounded
-    to float32 (no fused multiply-add).  A column holding a NaN yields NaN
-    edges (torch propagates NaN through quantile).
+    sort the column (sort_total_order); ``rank = f32(q*(N-1))``;
+    ``lo = floor(rank)``, ``hi = ceil(rank)``; ``w = f32(rank - lo)``;
+    lerp with ONE rounding per branch, as torch's lerp evaluates it:
+    ``w < 0.5 ? fma(w, b-a, a) : fma(-(b-a), 1-w, b)``.  A column holding a
+    NaN yields NaN edges (torch propagates NaN through quantile).
     """
     col = np.asarray(col, dtype=F32)
     q = linspace_f32(num_bins)
@@ -50,17 +84,20 @@ def quantile_edges_f32(col: np.ndarray, num_bins: int) -> np.ndarray:
         raise ValueError("quantile of an empty column")
     if np.isnan(col).any():
         return np.full(num_bins - 1, np.nan, dtype=F32)
-    s = np.sort(col)
-    rank = (q * F32(n - 1)).astype(F32)
-    lo = np.floor(rank).astype(np.int64)
-    hi = np.ceil(rank).astype(np.int64)
-    w = (rank - lo.astype(F32)).astype(F32)
-    a = s[lo]
-    b = s[hi]
-    d = (b - a).astype(F32)
-    low = (a + (w * d).astype(F32)).astype(F32)
-    high = (b - (d * (F32(1.0) - w).astype(F32)).astype(F32)).astype(F32)
-    return np.where(w < F32(0.5), low, high).astype(F32)
+    s = sort_total_order(col)
+    out = np.empty(num_bins - 1, dtype=F32)
+    for j, qq in enumerate(q):
+        rank = F32(qq * F32(n - 1))
+        lo = int(np.floor(rank))
+        hi = int(np.ceil(rank))
+        w = F32(rank - F32(lo))
+        a, b = s[lo], s[hi]
+        d = F32(b - a)
+        if w < F32(0.5):
+            out[j] = fma_f32(w, d, a)
+        else:
+            out[j] = fma_f32(-d, F32(F32(1.0) - w), b)
+    return out
 
 
 def bin_codes(col: np.ndarray, edges: np.ndarray) -> np.ndarray:

@@ -1,0 +1,130 @@
+"""CPU checks of the C ABI library and the host-side logic (no GPU calls).
+
+* every function declared in include/warpgbm_amd.h is exported by the built
+  library and bound in warpgbm_amd._lib;
+* argument validation in the library returns an error code and message
+  before touching the device;
+* host logic of the estimator: hyper-parameter validation, monotone
+  constraint normalisation, flattening, and the level-order node tables ->
+  nested tree dicts conversion.
+"""
+import ctypes
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def _declared():
+    text = (ROOT / "include" / "warpgbm_amd.h").read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(wgbm_\w+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    from warpgbm_amd import _lib
+    lib = _lib.load()
+    names = _declared()
+    assert len(names) >= 25
+    for n in names:
+        assert hasattr(lib, n), n
+        assert n in _lib.SIGNATURES, n
+    assert lib.wgbm_build_arch() == b"gfx950"
+
+
+def test_library_rejects_bad_arguments_without_a_device():
+    from warpgbm_amd import _lib
+    lib = _lib.load()
+    rc = lib.wgbm_compute_split(None, None, 1, 4, 1, 0.0, 1.0, 0.0, None, None, None)
+    assert rc != 0 and b"shape" in lib.wgbm_last_error()
+    rc = lib.wgbm_set_hist_config(0, 3, 0)
+    assert rc != 0
+    rc = lib.wgbm_set_hist_config(0, 0, 5)
+    assert rc != 0 and b"features_per_lane" in lib.wgbm_last_error()
+    with pytest.raises(RuntimeError, match="num_bins"):
+        _lib.call("wgbm_compute_histogram3", None, 10, 4, None, None, None, 5, None, 4, None, None, None, 1, 500,
+                  None, 0, None)
+
+
+def test_workspace_queries():
+    from warpgbm_amd import _lib
+    lib = _lib.load()
+    assert lib.wgbm_histogram3_workspace_bytes(1000, 50, 32, 1) > 0
+    assert lib.wgbm_histogram3_workspace_bytes(1000, 50, 32, 7) > lib.wgbm_histogram3_workspace_bytes(1000, 50, 32, 1)
+    c = _lib.FitConfig(n_rows=1_000_000, n_cols=1000, n_feat=1000, n_bins=32, n_eras=1, n_classes=1, max_depth=5,
+                       objective=0, learning_rate=0.01, min_split_gain=0.0, min_child_weight=20.0, lambda_=1e-6)
+    ws = lib.wgbm_fit_workspace_bytes(ctypes.byref(c))
+    assert 40e6 < ws < 2e9  # perm/gh buffers, histograms, partials
+    c.n_bins = 1
+    assert lib.wgbm_fit_workspace_bytes(ctypes.byref(c)) == 0
+    out = (ctypes.c_int32 * 3)()
+    lib.wgbm_get_hist_config(1000, 32, 1000, out)
+    assert out[0] > 0 and out[1] in (1, 2, 4) and out[2] in (1, 2, 4, 8, 16)
+
+
+def test_hyperparameter_validation_messages():
+    from warpgbm_amd import WarpGBM
+    with pytest.raises(ValueError, match="num_bins must be between 2 and 127"):
+        WarpGBM(num_bins=200)
+    with pytest.raises(TypeError, match="max_depth must be an integer"):
+        WarpGBM(max_depth=2.5)
+    with pytest.raises(ValueError, match="objective must be"):
+        WarpGBM(objective="poisson")
+    with pytest.raises(ValueError, match="engine"):
+        WarpGBM(engine="cpu")
+    with pytest.raises(TypeError, match="monotonic_constraints must be None"):
+        WarpGBM(monotonic_constraints="up")
+
+
+def test_monotone_normalisation():
+    from warpgbm_amd import WarpGBM
+    m = WarpGBM(monotonic_constraints={2: 1, 0: -1})
+    assert m._normalize_monotonic_constraints(4).tolist() == [-1, 0, 1, 0]
+    m = WarpGBM(monotonic_constraints=[1, 0])
+    with pytest.raises(ValueError, match="length must equal n_features"):
+        m._normalize_monotonic_constraints(3)
+    m = WarpGBM(monotonic_constraints={5: 1})
+    with pytest.raises(ValueError, match="out of range"):
+        m._normalize_monotonic_constraints(3)
+
+
+def test_tables_to_dicts_and_flatten_roundtrip():
+    """Level-order node tables (as the batched trainer writes them) become
+    the reference's nested dicts, whose flattening traverses identically."""
+    import torch
+    from oracle import gbm_oracle as O
+    from warpgbm_amd import WarpGBM
+    rng = np.random.default_rng(0)
+    N, F, B, D = 500, 6, 8, 3
+    codes = rng.integers(0, B, size=(N, F)).astype(np.int8)
+    rec = O.grow_tree(codes, rng.standard_normal(N) + (codes[:, 2] > 3), np.ones(N), np.zeros(N, int), 1, B, D,
+                      20, 0.0, 1e-6)
+    NH = 2 ** (D + 1) - 1
+    st = np.zeros((1, 1, NH), np.int32)
+    ft = np.full((1, 1, NH), -1, np.int32)
+    bt = np.full((1, 1, NH), -1, np.int32)
+    vt = np.zeros((1, 1, NH), np.float32)
+    ct = np.zeros((1, 1, NH), np.int32)
+    eg = np.zeros((1, 1, NH, 1), np.float32)
+    for h, node in rec.nodes.items():
+        if node[0] == "split":
+            st[0, 0, h], ft[0, 0, h], bt[0, 0, h], eg[0, 0, h, 0] = 1, node[1], node[2], node[3][0]
+        else:
+            st[0, 0, h], vt[0, 0, h], ct[0, 0, h] = 2, node[1], node[2]
+    m = WarpGBM(max_depth=D, n_estimators=1)
+    m.objective = "regression"
+    m._feature_ids = torch.arange(F, dtype=torch.int32)
+    m.per_era_feature_importance_ = np.zeros((1, F), np.float32)
+    m.forest = [{}]
+    m._trees_trained = 0
+    m._append_rounds({"status": st, "feat": ft, "bin": bt, "value": vt, "count": ct, "era_gain": eg}, 1, 1, None)
+    t = m.forest[0]
+    flat = m.flatten_tree(t, 2 ** (D + 1)).numpy()
+    ref = O.predict_tree(rec.tree, codes).astype(np.float32)
+    got = O.predict_forest_flat(codes, [flat], 1.0)
+    np.testing.assert_allclose(got, ref, rtol=1e-6)
+    assert m._feature_of(t["feature"]) == rec.tree["feature"]
+    assert m.per_era_feature_importance_[0, rec.tree["feature"]] > 0

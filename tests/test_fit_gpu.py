@@ -1,0 +1,158 @@
+"""End-to-end GPU parity of WarpGBM.fit/predict against the oracle.
+
+* plain per-node engine and batched level-wise engine give identical trees
+  (same split at every node, same leaf values bit for bit);
+* each tree matches the FP64 oracle's tree grown from the same float32
+  gradients (teacher forcing): same (feature, bin) at every node, same
+  row -> leaf assignment, leaf values within 1e-5 relative;
+* predictions within 1e-5 of the oracle's float64 forest evaluation.
+"""
+import numpy as np
+import pytest
+
+from oracle import gbm_oracle as O
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+
+def _data_cfg1(n=20000, f=50, seed=0):
+    """Plain regression, small (20k x 50, y = X[:, :10] w + noise)."""
+    rng = np.random.default_rng(seed)
+    X = rng.standard_normal((n, f)).astype(np.float32)
+    w = rng.standard_normal(10)
+    y = (X[:, :10] @ w + rng.normal(0, 0.1, n)).astype(np.float32)
+    return X, y
+
+
+def _trees_equal(a, b, path="root"):
+    if ("leaf_value" in a) != ("leaf_value" in b):
+        return f"{path}: leaf/split mismatch"
+    if "leaf_value" in a:
+        if a["leaf_value"] != b["leaf_value"] or a["samples"] != b["samples"]:
+            return f"{path}: leaf {a} vs {b}"
+        return None
+    fa = int(a["feature"].item()) if torch.is_tensor(a["feature"]) else int(a["feature"])
+    fb = int(b["feature"].item()) if torch.is_tensor(b["feature"]) else int(b["feature"])
+    if fa != fb or a["bin"] != b["bin"]:
+        return f"{path}: split ({fa},{a['bin']}) vs ({fb},{b['bin']})"
+    return _trees_equal(a["left"], b["left"], path + "L") or _trees_equal(a["right"], b["right"], path + "R")
+
+
+def _oracle_check_tree(tree, codes, g, h, eras, E, B, D, lam):
+    """Grow the oracle tree from the same g/h and compare node by node."""
+    rec = O.grow_tree(codes, g, h, eras, E, B, D, 20, 0.0, lam)
+    msg = _compare_dict_to_oracle(tree, rec.tree, "root")
+    assert msg is None, msg
+
+
+def _compare_dict_to_oracle(t, o, path):
+    if "leaf_value" in o:
+        assert "leaf_value" in t, f"{path}: engine split where oracle has a leaf"
+        assert t["samples"] == o["samples"], f"{path}: sample count {t['samples']} vs {o['samples']}"
+        ref = o["leaf_value"]
+        assert abs(t["leaf_value"] - ref) <= 1e-5 * max(abs(ref), 1e-3), f"{path}: leaf {t['leaf_value']} vs {ref}"
+        return None
+    assert "feature" in t, f"{path}: engine leaf where oracle splits"
+    f = int(t["feature"].item())
+    assert (f, t["bin"]) == (o["feature"], o["bin"]), f"{path}: split ({f},{t['bin']}) vs ({o['feature']},{o['bin']})"
+    _compare_dict_to_oracle(t["left"], o["left"], path + "L")
+    _compare_dict_to_oracle(t["right"], o["right"], path + "R")
+    return None
+
+
+def test_cfg1_plain_equals_batched_and_oracle():
+    from warpgbm_amd import WarpGBM
+    X, y = _data_cfg1()
+    kw = dict(objective="regression", num_bins=32, max_depth=4, n_estimators=20, learning_rate=0.1, random_state=0)
+    mb = WarpGBM(engine="batched", **kw).fit(X, y)
+    mp = WarpGBM(engine="plain", **kw).fit(X, y)
+    for i in range(20):
+        msg = _trees_equal(mb.forest[i], mp.forest[i], f"tree{i}:")
+        assert msg is None, msg
+    pb, pp = mb.predict(X), mp.predict(X)
+    assert np.array_equal(pb, pp)
+    np.testing.assert_array_equal(mb.feature_importance_, mp.feature_importance_)
+
+    # oracle with teacher forcing: replay the engine's float32 scores
+    codes, _ = O.bin_matrix(X, 32)
+    eras = np.zeros(len(y), np.int32)
+    scores = np.full(len(y), np.float32(mb.base_prediction), np.float32)
+    lr = np.float32(0.1)
+    flats = []
+    for i in range(20):
+        g = ((y - scores) * np.float32(1.0)).astype(np.float32)
+        h = np.ones_like(g)
+        _oracle_check_tree(mb.forest[i], codes, g, h, eras, 1, 32, 4, 1e-6)
+        vals = O.predict_tree(mb.forest[i] if False else _to_plain(mb.forest[i]), codes)
+        scores = (scores + (lr * vals.astype(np.float32)).astype(np.float32)).astype(np.float32)
+        flats.append(O.flatten_tree(_to_plain(mb.forest[i]), 2 ** 5))
+    ref = O.predict_forest_flat(codes, flats, 0.1, base=mb.base_prediction)
+    np.testing.assert_allclose(pb, ref, rtol=1e-5, atol=1e-5)
+    # training scores and predictions of the training matrix coincide bit for bit
+    assert np.array_equal(pb, scores)
+
+
+def _to_plain(t):
+    if "leaf_value" in t:
+        return dict(t)
+    return {"feature": int(t["feature"].item()), "bin": t["bin"], "left": _to_plain(t["left"]),
+            "right": _to_plain(t["right"])}
+
+
+def test_eras_and_monotone_batched_equals_plain():
+    from warpgbm_amd import WarpGBM
+    rng = np.random.default_rng(5)
+    n, f = 12000, 12
+    X = rng.standard_normal((n, f)).astype(np.float32)
+    era = np.repeat(np.arange(6), n // 6)
+    rng.shuffle(era)
+    sign = np.where(era % 3 == 0, -1.0, 1.0)
+    y = (X[:, 0] * sign + 0.5 * X[:, 1] + 0.1 * rng.standard_normal(n)).astype(np.float32)
+    kw = dict(objective="regression", num_bins=16, max_depth=3, n_estimators=8, learning_rate=0.2,
+              monotonic_constraints={1: 1})
+    mb = WarpGBM(engine="batched", **kw).fit(X, y, era_id=era)
+    mp = WarpGBM(engine="plain", **kw).fit(X, y, era_id=era)
+    for i in range(8):
+        msg = _trees_equal(mb.forest[i], mp.forest[i], f"tree{i}:")
+        assert msg is None, msg
+    np.testing.assert_array_equal(mb.per_era_feature_importance_, mp.per_era_feature_importance_)
+    assert np.array_equal(mb.predict(X), mp.predict(X))
+
+
+def test_conflicting_eras_root_is_leaf():
+    """warpgbm tests/test_monotonic_constraints.py:124-149."""
+    from warpgbm_amd import WarpGBM
+    rng = np.random.default_rng(90)
+    n = 800
+    x0 = rng.uniform(-2.0, 2.0, size=n).astype(np.float32)
+    era = np.zeros(n, dtype=np.int32)
+    era[n // 2:] = 1
+    y = np.empty(n, dtype=np.float32)
+    y[: n // 2] = 2.0 * x0[: n // 2]
+    y[n // 2:] = -2.0 * x0[n // 2:]
+    y += 0.01 * rng.normal(size=n).astype(np.float32)
+    for engine in ("batched", "plain"):
+        m = WarpGBM(objective="regression", n_estimators=10, max_depth=3, learning_rate=0.1, num_bins=32,
+                    monotonic_constraints=[1], random_state=90, engine=engine)
+        m.fit(x0.reshape(-1, 1), y, era_id=era)
+        assert "leaf_value" in m.forest[0]
+
+
+def test_multiclass_batched_equals_plain():
+    from warpgbm_amd import WarpGBM
+    rng = np.random.default_rng(7)
+    n, f, K = 6000, 20, 4
+    X = rng.standard_normal((n, f)).astype(np.float32)
+    W = rng.standard_normal((8, K))
+    lab = np.argmax(X[:, :8] @ W + rng.gumbel(size=(n, K)), axis=1)
+    kw = dict(objective="multiclass", num_bins=32, max_depth=3, n_estimators=6, learning_rate=0.1)
+    mb = WarpGBM(engine="batched", **kw).fit(X, lab)
+    mp = WarpGBM(engine="plain", **kw).fit(X, lab)
+    for i in range(6):
+        for k in range(K):
+            msg = _trees_equal(mb.forest[i][k], mp.forest[i][k], f"round{i} class{k}:")
+            assert msg is None, msg
+    pb = mb.predict_proba(X)
+    assert np.array_equal(pb, mp.predict_proba(X))
+    np.testing.assert_allclose(pb.sum(axis=1), 1.0, atol=1e-5)

@@ -1,0 +1,252 @@
+"""GPU parity of the reference ops and per-node helpers against the oracle.
+
+Every test draws seeded inputs, runs the HIP library through the C ABI (via
+the reference-compatible op module where one exists) and compares with the
+FP64 oracle: integer/byte/index results bit-exact, floating results within
+the tolerance stated in each test.
+"""
+import numpy as np
+import pytest
+
+from oracle import gbm_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def _t(x, dev):
+    return torch.from_numpy(np.ascontiguousarray(x)).to(dev)
+
+
+def _hist_gpu(dev, codes, g, h, rows, feats, eras, E, B):
+    from warpgbm_amd.cuda import node_kernel as nk
+    G = torch.zeros((E, len(feats), B), dtype=torch.float32, device=dev)
+    H = torch.zeros_like(G)
+    nk.compute_histogram3(_t(codes, dev), _t(g.astype(np.float32), dev), _t(h.astype(np.float32), dev),
+                          _t(rows.astype(np.int32), dev), _t(np.asarray(feats, np.int32), dev),
+                          _t(eras.astype(np.int32), dev), G, H, B, 64, 4)
+    torch.cuda.synchronize()
+    return G.cpu().numpy(), H.cpu().numpy()
+
+
+def _rel_err(got, ref, scale):
+    return float(np.max(np.abs(got.astype(np.float64) - ref)) / max(scale, 1e-30))
+
+
+@pytest.mark.parametrize("N,F,B,E,subset", [
+    (5000, 50, 32, 1, False),
+    (20000, 37, 7, 1, True),
+    (30000, 100, 5, 4, False),
+    (3000, 1000, 32, 1, False),
+    (40000, 64, 127, 1, False),
+])
+def test_histogram_matches_oracle(dev, N, F, B, E, subset):
+    rng = np.random.default_rng(N + F)
+    codes = rng.integers(0, B, size=(N, F), dtype=np.int8)
+    g = rng.standard_normal(N).astype(np.float32)
+    h = rng.uniform(0.5, 1.5, N).astype(np.float32)
+    eras = rng.integers(0, E, N).astype(np.int32)
+    rows = np.sort(rng.choice(N, size=N * 2 // 3, replace=False)).astype(np.int32)
+    feats = rng.permutation(F)[: F // 2] if subset else np.arange(F)
+    Gg, Hg = _hist_gpu(dev, codes, g, h, rows, feats, eras, E, B)
+    Go, Ho = O.histogram(codes, g, h, rows, feats, eras, E, B)
+    # FP32 accumulation vs FP64: error relative to the node's sum of |g| (|h|).
+    assert _rel_err(Gg, Go, np.abs(g[rows]).sum()) < 1e-5
+    assert _rel_err(Hg, Ho, np.abs(h[rows]).sum()) < 1e-5
+    # invariants: every row lands in exactly one bin per feature
+    np.testing.assert_allclose(Gg.sum(axis=(0, 2)), g[rows].astype(np.float64).sum(), rtol=1e-4, atol=1e-2)
+
+
+def test_histogram_counts_exact_and_drops_out_of_range(dev):
+    rng = np.random.default_rng(3)
+    N, F, B = 10000, 20, 16
+    codes = rng.integers(-3, B + 3, size=(N, F)).astype(np.int8)
+    g = np.ones(N, np.float32)
+    h = np.ones(N, np.float32)
+    rows = np.arange(N, dtype=np.int32)
+    eras = np.zeros(N, np.int32)
+    Gg, Hg = _hist_gpu(dev, codes, g, h, rows, np.arange(F), eras, 1, B)
+    Go, Ho = O.histogram(codes, g, h, rows, np.arange(F), eras, 1, B)
+    assert np.array_equal(Hg, Ho.astype(np.float32))  # integer counts are exact in FP32
+    assert np.array_equal(Gg, Go.astype(np.float32))
+
+
+def test_histogram_deterministic_and_era_order_independent(dev):
+    rng = np.random.default_rng(11)
+    N, F, B, E = 50000, 40, 32, 5
+    codes = rng.integers(0, B, size=(N, F), dtype=np.int8)
+    g = rng.standard_normal(N).astype(np.float32)
+    h = np.ones(N, np.float32)
+    eras = rng.integers(0, E, N).astype(np.int32)
+    rows = np.arange(N, dtype=np.int32)
+    a = _hist_gpu(dev, codes, g, h, rows, np.arange(F), eras, E, B)
+    b = _hist_gpu(dev, codes, g, h, rows, np.arange(F), eras, E, B)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    # the op sorts rows by era stably: an era-grouped list gives the same bits
+    grouped = np.argsort(eras, kind="stable").astype(np.int32)
+    c = _hist_gpu(dev, codes, g, h, grouped, np.arange(F), eras, E, B)
+    assert np.array_equal(a[0], c[0]) and np.array_equal(a[1], c[1])
+
+
+def test_histogram_empty_rows(dev):
+    codes = np.zeros((10, 4), np.int8)
+    G, H = _hist_gpu(dev, codes, np.ones(10, np.float32), np.ones(10, np.float32), np.zeros(0, np.int32),
+                     np.arange(4), np.zeros(10, np.int32), 1, 8)
+    assert not G.any() and not H.any()
+
+
+@pytest.mark.parametrize("E", [1, 3])
+def test_compute_split_and_select_match_oracle(dev, E):
+    from warpgbm_amd.cuda import node_kernel as nk
+    from warpgbm_amd import _lib
+    rng = np.random.default_rng(5 + E)
+    F, B = 30, 16
+    N = 4000
+    codes = rng.integers(0, B, size=(N, F), dtype=np.int8)
+    y = (codes[:, 3] > 7).astype(np.float32) * 2.0 - (codes[:, 5] > 3) + 0.1 * rng.standard_normal(N)
+    g = (y - y.mean()).astype(np.float32)
+    h = np.ones(N, np.float32)
+    eras = (np.arange(N) * E // N).astype(np.int32)
+    rows = np.arange(N, dtype=np.int32)
+    Gg, Hg = _hist_gpu(dev, codes, g, h, rows, np.arange(F), eras, E, B)
+    Gt, Ht = _t(Gg, dev), _t(Hg, dev)
+    gain = torch.empty((E, F, B - 1), dtype=torch.float32, device=dev)
+    d = torch.empty_like(gain)
+    nk.compute_split(Gt, Ht, 0.0, 20.0, 1e-6, gain, d, 64)
+    out = torch.empty(2, dtype=torch.int32, device=dev)
+    _lib.call("wgbm_select_split", gain.data_ptr(), d.data_ptr(), E, F, B, 0.0, None, None, out.data_ptr(),
+              torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    gain_o, dir_o = O.split_gains(*O.histogram(codes, g, h, rows, np.arange(F), eras, E, B), 20.0, 1e-6)
+    gg = gain.cpu().numpy()
+    scale = np.abs(gain_o).max()
+    assert np.max(np.abs(gg - gain_o)) <= 1e-5 * scale
+    dd = d.cpu().numpy()
+    clear = np.abs(gain_o) > 1e-3 * scale
+    assert np.array_equal(dd[clear], dir_o[clear])
+    f, b = O.select_split(gain_o, dir_o, 0.0)
+    assert tuple(out.cpu().numpy().tolist()) == (f, b)
+
+
+def test_select_conflicting_eras_is_leaf(dev):
+    """Two eras with opposite monotone signal and a +1 constraint -> no split
+    (mirrors warpgbm tests/test_monotonic_constraints.py:124-149)."""
+    from warpgbm_amd import _lib
+    E, F, B = 2, 1, 8
+    gain = np.zeros((E, F, B - 1), np.float32) + 1.0
+    d = np.zeros((E, F, B - 1), np.float32)
+    d[0] = 1.0
+    d[1] = -1.0
+    req = np.array([-1], np.int8)  # monotone increasing -> required direction -1
+    out = torch.empty(2, dtype=torch.int32, device=dev)
+    _lib.call("wgbm_select_split", _t(gain, dev).data_ptr(), _t(d, dev).data_ptr(), E, F, B, 0.0,
+              _t(req, dev).data_ptr(), None, out.data_ptr(), torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    assert out.cpu().tolist() == [-1, -1]
+    assert O.select_split(gain, d, 0.0, req) == (-1, -1)
+
+
+def test_partition_rows_is_stable_and_exact(dev):
+    from warpgbm_amd import _lib
+    rng = np.random.default_rng(9)
+    N, F = 100000, 12
+    codes = rng.integers(0, 32, size=(N, F), dtype=np.int8)
+    rows = np.sort(rng.choice(N, 70001, replace=False)).astype(np.int32)
+    ws = torch.empty(_lib.query("wgbm_partition_workspace_bytes", rows.size), dtype=torch.uint8, device=dev)
+    out = torch.empty(rows.size, dtype=torch.int32, device=dev)
+    nl = torch.empty(1, dtype=torch.int32, device=dev)
+    ct, rt = _t(codes, dev), _t(rows, dev)
+    _lib.call("wgbm_partition_rows", ct.data_ptr(), N, F, rt.data_ptr(), rows.size, 7, 13, out.data_ptr(),
+              nl.data_ptr(), ws.data_ptr(), ws.numel(), torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    m = codes[rows, 7] <= 13
+    ref = np.concatenate([rows[m], rows[~m]])
+    assert int(nl.item()) == int(m.sum())
+    assert np.array_equal(out.cpu().numpy(), ref)
+
+
+def test_leaf_value_and_score_update(dev):
+    from warpgbm_amd import _lib
+    rng = np.random.default_rng(2)
+    N = 30000
+    g = rng.standard_normal(N).astype(np.float32)
+    h = rng.uniform(0.1, 2, N).astype(np.float32)
+    rows = np.sort(rng.choice(N, 12345, replace=False)).astype(np.int32)
+    pred = rng.standard_normal(N).astype(np.float32)
+    ws = torch.empty(_lib.query("wgbm_leaf_workspace_bytes", rows.size), dtype=torch.uint8, device=dev)
+    out = torch.empty(1, dtype=torch.float32, device=dev)
+    pt = _t(pred, dev)
+    gt, ht, rt = _t(g, dev), _t(h, dev), _t(rows, dev)
+    _lib.call("wgbm_leaf_value", gt.data_ptr(), ht.data_ptr(), rt.data_ptr(), rows.size, N, 1e-6, 0.1,
+              pt.data_ptr(), 1, 0, out.data_ptr(), ws.data_ptr(), ws.numel(), torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    ref = O.leaf_value(g, h, rows, 1e-6)
+    v = float(out.item())
+    assert abs(v - ref) <= 1e-6 * max(abs(ref), 1e-6)
+    exp = pred.copy()
+    exp[rows] = exp[rows] + np.float32(np.float32(0.1) * np.float32(v))
+    assert np.array_equal(pt.cpu().numpy(), exp)
+
+
+@pytest.mark.parametrize("N,B", [(20000, 32), (12345, 10), (1000, 127), (7, 5)])
+def test_quantile_edges_and_binning_bit_exact(dev, N, B):
+    from warpgbm_amd import _lib
+    rng = np.random.default_rng(N)
+    F = 8
+    X = rng.standard_normal((N, F)).astype(np.float32)
+    X[:, 3] = np.round(X[:, 3])  # heavy ties
+    X[5 % N, 6] = np.nan         # NaN column -> NaN edges, codes 0
+    Xt = _t(X, dev)
+    srt = torch.sort(Xt, dim=0).values.contiguous()
+    edges = torch.empty((F, B - 1), dtype=torch.float32, device=dev)
+    st = torch.cuda.current_stream().cuda_stream
+    _lib.call("wgbm_edges_from_sorted", srt.data_ptr(), N, F, B, edges.data_ptr(), B - 1, st)
+    codes = torch.empty((N, F), dtype=torch.int8, device=dev)
+    ws = torch.empty(_lib.query("wgbm_bin_matrix_workspace_bytes", F, B - 1), dtype=torch.uint8, device=dev)
+    _lib.call("wgbm_bin_matrix", Xt.data_ptr(), N, F, edges.data_ptr(), B - 1, codes.data_ptr(), ws.data_ptr(),
+              ws.numel(), st)
+    torch.cuda.synchronize()
+    ref_codes, ref_edges = O.bin_matrix(X, B)
+    e = edges.cpu().numpy()
+    assert np.array_equal(np.isnan(e), np.isnan(ref_edges))
+    assert np.array_equal(e[~np.isnan(e)].view(np.int32), ref_edges[~np.isnan(ref_edges)].view(np.int32))
+    assert np.array_equal(codes.cpu().numpy(), ref_codes)
+
+
+def test_bin_column_reference_op(dev):
+    from warpgbm_amd.cuda import node_kernel as nk
+    rng = np.random.default_rng(1)
+    x = rng.standard_normal(5001).astype(np.float32)
+    x[7] = np.nan
+    edges = np.sort(rng.standard_normal(9)).astype(np.float32)
+    out = torch.empty(x.size, dtype=torch.int8, device=dev)
+    nk.custom_cuda_binner(_t(x, dev), _t(edges, dev), out)
+    torch.cuda.synchronize()
+    ref = O.bin_codes(x, edges)
+    assert np.array_equal(out.cpu().numpy(), ref)
+    assert np.array_equal(ref[~np.isnan(x)], np.searchsorted(edges, x[~np.isnan(x)], side="right"))
+
+
+def test_predict_forest_reference_op(dev):
+    from warpgbm_amd.cuda import node_kernel as nk
+    rng = np.random.default_rng(4)
+    N, F, B = 3000, 10, 16
+    codes = rng.integers(0, B, size=(N, F), dtype=np.int8)
+    g = rng.standard_normal(N)
+    h = np.ones(N)
+    trees = []
+    for t in range(7):
+        rec = O.grow_tree(codes, g * (t + 1), h, np.zeros(N, np.int32), 1, B, 4, 20, 0.0, 1e-6)
+        trees.append(O.flatten_tree(rec.tree, 2 ** 5))
+    flat = np.stack(trees).astype(np.float32)
+    out = torch.full((N,), 0.5, dtype=torch.float32, device=dev)
+    nk.predict_forest(_t(codes, dev), _t(flat, dev), 0.1, out)
+    torch.cuda.synchronize()
+    ref = O.predict_forest_flat(codes, flat, 0.1, base=0.5)
+    np.testing.assert_allclose(out.cpu().numpy(), ref, rtol=1e-5, atol=1e-6)
+    # chunk invariance (warpgbm tests/test_predict_chunking.py:19-40), bitwise here
+    o1 = torch.full((N // 2,), 0.5, dtype=torch.float32, device=dev)
+    nk.predict_forest(_t(codes[: N // 2], dev), _t(flat, dev), 0.1, o1)
+    torch.cuda.synchronize()
+    assert np.array_equal(o1.cpu().numpy(), out.cpu().numpy()[: N // 2])

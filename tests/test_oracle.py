@@ -1,0 +1,261 @@
+"""Pins the FP64 oracle (CPU only).
+
+The reference ships no golden vectors for this path (its tests are
+end-to-end), so each oracle function is checked against an independent
+definition: torch's own CPU float32 linspace/quantile (the library routine
+the reference calls), np.searchsorted, brute-force loops on tiny inputs,
+closed forms and invariants.  test_planted_mistakes_are_caught shows that
+plausible slips in the oracle fail these checks.
+"""
+import itertools
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import gbm_oracle as O
+
+
+def test_linspace_matches_torch_for_every_bin_count():
+    for B in range(2, 128):
+        ref = torch.linspace(0, 1, B + 1, dtype=torch.float32).numpy()[1:-1]
+        assert np.array_equal(O.linspace_f32(B), ref), B
+
+
+@pytest.mark.parametrize("N", [2, 7, 1000, 12345, 20000])
+@pytest.mark.parametrize("B", [2, 5, 10, 32, 127])
+def test_quantile_edges_match_torch_quantile(N, B):
+    rng = np.random.default_rng(N * 131 + B)
+    for col in (rng.standard_normal(N).astype(np.float32),
+                np.round(rng.standard_normal(N) * 2).astype(np.float32),   # ties
+                (rng.standard_normal(N) * 1e-30).astype(np.float32)):
+        q = torch.linspace(0, 1, B + 1, dtype=torch.float32)[1:-1]
+        ref = torch.quantile(torch.from_numpy(col), q).numpy()
+        got = O.quantile_edges_f32(col, B)
+        # equal values; equal bits except possibly the sign of a zero edge,
+        # which torch's comparison sort leaves to chance (codes unaffected)
+        assert np.array_equal(got, ref)
+        nz = got != 0
+        assert np.array_equal(got[nz].view(np.int32), ref[nz].view(np.int32))
+
+
+def test_quantile_edges_nan_column():
+    col = np.array([1.0, np.nan, 3.0, 2.0], np.float32)
+    q = torch.linspace(0, 1, 5, dtype=torch.float32)[1:-1]
+    ref = torch.quantile(torch.from_numpy(col), q).numpy()
+    assert np.all(np.isnan(ref)) and np.all(np.isnan(O.quantile_edges_f32(col, 4)))
+
+
+def test_quantile_edges_large_n_float32_rank():
+    """At N = 1M the float32 rank q*(N-1) rounds (e.g. 968749.03 -> 968749):
+    the oracle must follow torch's float32 arithmetic, not float64."""
+    rng = np.random.default_rng(0)
+    col = rng.standard_normal(1_000_000).astype(np.float32)
+    q = torch.linspace(0, 1, 33, dtype=torch.float32)[1:-1]
+    ref = torch.quantile(torch.from_numpy(col), q).numpy()
+    assert np.array_equal(O.quantile_edges_f32(col, 32), ref)
+    f64 = np.quantile(col.astype(np.float64), q.numpy().astype(np.float64)).astype(np.float32)
+    assert not np.array_equal(f64, ref)  # the naive float64 definition differs here
+
+
+def test_bin_codes_match_searchsorted_and_nan():
+    rng = np.random.default_rng(1)
+    x = rng.standard_normal(10000).astype(np.float32)
+    x[::97] = np.nan
+    edges = np.sort(rng.standard_normal(31)).astype(np.float32)
+    edges[5] = edges[6]  # duplicate edge
+    codes = O.bin_codes(x, edges)
+    ok = ~np.isnan(x)
+    assert np.array_equal(codes[ok], np.searchsorted(edges, x[ok], side="right"))
+    assert np.all(codes[~ok] == 0)
+
+
+def test_prebinned_detection():
+    assert O.is_prebinned(np.array([[0, 1], [4, 2]], np.int64), 5)
+    assert not O.is_prebinned(np.array([[0, 1], [5, 2]], np.int64), 5)
+    assert not O.is_prebinned(np.array([[0.0, 1.0]], np.float32), 5)
+
+
+def _brute_hist(codes, g, h, rows, feats, eras, E, B):
+    G = np.zeros((E, len(feats), B))
+    H = np.zeros((E, len(feats), B))
+    for r in rows:
+        for j, f in enumerate(feats):
+            c = int(codes[r, f])
+            if 0 <= c < B:
+                G[eras[r], j, c] += g[r]
+                H[eras[r], j, c] += h[r]
+    return G, H
+
+
+def test_histogram_brute_force_and_invariants():
+    rng = np.random.default_rng(2)
+    N, F, B, E = 300, 6, 5, 3
+    codes = rng.integers(-1, B + 1, size=(N, F)).astype(np.int8)
+    g = rng.standard_normal(N)
+    h = rng.uniform(0.5, 2, N)
+    eras = rng.integers(0, E, N)
+    rows = rng.choice(N, 200, replace=False)
+    feats = np.array([4, 1, 3])
+    G, H = O.histogram(codes, g, h, rows, feats, eras, E, B)
+    Gb, Hb = _brute_hist(codes, g, h, rows, feats, eras, E, B)
+    np.testing.assert_allclose(G, Gb, rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(H, Hb, rtol=1e-12, atol=1e-12)
+    # parent = left + right
+    m = codes[rows, 2] <= 2
+    Gl, _ = O.histogram(codes, g, h, rows[m], feats, eras, E, B)
+    Gr, _ = O.histogram(codes, g, h, rows[~m], feats, eras, E, B)
+    np.testing.assert_allclose(G, Gl + Gr, atol=1e-12)
+
+
+def _brute_gain(codes, g, h, rows, f, b, mcw, lam):
+    """Gain of threshold b on feature f from the row subsets directly."""
+    c = codes[rows, f].astype(int)
+    ok = (c >= 0)
+    left = rows[ok & (c <= b)]
+    right = rows[ok & (c > b)]
+    allr = rows[ok]
+    GL, HL = g[left].sum(), h[left].sum()
+    GR, HR = g[right].sum(), h[right].sum()
+    if HL < mcw or HR < mcw:
+        return 0.0, 0.0
+    G, H = g[allr].sum(), h[allr].sum()
+    gain = GL * GL / (HL + lam) + GR * GR / (HR + lam) - G * G / (H + lam)
+    return gain, (1.0 if GL / (HL + lam) > GR / (HR + lam) else -1.0)
+
+
+def test_split_gains_brute_force():
+    rng = np.random.default_rng(3)
+    N, F, B = 400, 4, 6
+    codes = rng.integers(0, B, size=(N, F)).astype(np.int8)
+    g = rng.standard_normal(N) + (codes[:, 1] > 2)
+    h = rng.uniform(0.5, 2, N)
+    rows = np.arange(N)
+    G, H = O.histogram(codes, g, h, rows, np.arange(F), np.zeros(N, int), 1, B)
+    gain, d = O.split_gains(G, H, 20.0, 1e-6)
+    for f, b in itertools.product(range(F), range(B - 1)):
+        bg, bd = _brute_gain(codes, g, h, rows, f, b, 20.0, 1e-6)
+        assert abs(gain[0, f, b] - bg) < 1e-9 * max(1, abs(bg))
+        assert d[0, f, b] == bd
+
+
+def _brute_select(gain, d, msg, req=None):
+    """Reference selection written as a plain loop over every (f, b)."""
+    E, F, Bm1 = gain.shape
+    cands = []
+    for f in range(F):
+        for b in range(Bm1):
+            if E == 1:
+                crit, agree = gain[0, f, b], 0
+            else:
+                crit = sum(gain[e, f, b] for e in range(E)) / E
+                agree = abs(sum(d[e, f, b] for e in range(E)))
+            mono = True
+            if req is not None and req[f] != 0:
+                mono = all(d[e, f, b] == req[f] for e in range(E))
+            cands.append((agree, crit, f, b, mono))
+    top = max(c[0] for c in cands)
+    best = None
+    for agree, crit, f, b, mono in cands:
+        if agree == top and crit > msg and mono and (best is None or crit > best[0]):
+            best = (crit, f, b)
+    return (-1, -1) if best is None else (best[1], best[2])
+
+
+def test_select_split_brute_force():
+    rng = np.random.default_rng(4)
+    for trial in range(200):
+        E = int(rng.integers(1, 5))
+        F, Bm1 = int(rng.integers(1, 5)), int(rng.integers(1, 6))
+        gain = np.round(rng.standard_normal((E, F, Bm1)), 1)  # rounding creates ties
+        d = rng.choice([-1.0, 0.0, 1.0], size=(E, F, Bm1))
+        req = rng.choice([-1, 0, 1], size=F) if trial % 2 else None
+        msg = float(rng.choice([0.0, 0.3]))
+        assert O.select_split(gain, d, msg, req) == _brute_select(gain, d, msg, req), trial
+
+
+def test_select_split_special_cases():
+    # conflicting eras under a monotone constraint -> leaf
+    gain = np.ones((2, 1, 4))
+    d = np.stack([np.ones((1, 4)), -np.ones((1, 4))])
+    assert O.select_split(gain, d, 0.0, np.array([-1])) == (-1, -1)
+    # a higher agreement wins even when it has no positive gain -> leaf
+    gain = np.array([[[5.0, -1.0]], [[5.0, -1.0]]])
+    d = np.array([[[1.0, 1.0]], [[-1.0, 1.0]]])
+    assert O.select_split(gain, d, 0.0) == (-1, -1)
+    # mcw larger than the node -> every gain 0 -> leaf
+    G = np.ones((1, 3, 4))
+    gain0, d0 = O.split_gains(G, G, 100.0, 1e-6)
+    assert O.select_split(gain0, d0, 0.0) == (-1, -1)
+
+
+def test_grow_tree_assignment_and_leaves():
+    rng = np.random.default_rng(5)
+    N, F, B = 3000, 5, 8
+    codes = rng.integers(0, B, size=(N, F)).astype(np.int8)
+    g = rng.standard_normal(N) + 2 * (codes[:, 0] > 3)
+    h = np.ones(N)
+    rec = O.grow_tree(codes, g, h, np.zeros(N, int), 1, B, 3, 20, 0.0, 1e-6)
+    vals = O.predict_tree(rec.tree, codes)
+    for heap, node in rec.nodes.items():
+        if node[0] == "leaf":
+            rows = np.where(rec.leaf_of_row == heap)[0]
+            assert rows.size == node[2]
+            assert abs(node[1] - g[rows].sum() / (h[rows].sum() + 1e-6)) < 1e-12
+            assert np.all(vals[rows] == node[1])
+    assert np.all(rec.leaf_of_row >= 0)
+    # the root split is the brute-force best split
+    root = rec.nodes[0]
+    assert root[0] == "split"
+    best = max(((f, b) for f in range(F) for b in range(B - 1)),
+               key=lambda fb: (_brute_gain(codes, g, h, np.arange(N), fb[0], fb[1], 20, 1e-6)[0], -fb[0], -fb[1]))
+    assert (root[1], root[2]) == best
+
+
+def test_flatten_and_flat_predict_match_dict_traversal():
+    rng = np.random.default_rng(6)
+    N, F, B = 2000, 6, 8
+    codes = rng.integers(0, B, size=(N, F)).astype(np.int8)
+    trees = [O.grow_tree(codes, rng.standard_normal(N) + (codes[:, k] > 4), np.ones(N), np.zeros(N, int), 1, B,
+                         4, 20, 0.0, 1e-6).tree for k in range(4)]
+    flats = [O.flatten_tree(t, 32) for t in trees]
+    ref = 0.25 + 0.1 * sum(O.predict_tree(t, codes) for t in trees)
+    np.testing.assert_allclose(O.predict_forest_flat(codes, flats, 0.1, base=0.25), ref, rtol=1e-6, atol=1e-7)
+    half = O.predict_forest_flat(codes[:1000], flats, 0.1, base=0.25)
+    assert np.array_equal(half, O.predict_forest_flat(codes, flats, 0.1, base=0.25)[:1000])
+
+
+def test_softmax_grad_hess_closed_forms():
+    rng = np.random.default_rng(7)
+    s = rng.standard_normal((100, 2))
+    y = rng.integers(0, 2, 100)
+    w = rng.uniform(0.5, 1.5, 100)
+    g, h = O.softmax_grad_hess(s, y, w)
+    p1 = 1.0 / (1.0 + np.exp(-(s[:, 1] - s[:, 0])))
+    np.testing.assert_allclose(g[:, 1], -(p1 - (y == 1)) * w, rtol=1e-12)
+    np.testing.assert_allclose(h[:, 1], np.maximum(p1 * (1 - p1), 1e-6) * w, rtol=1e-12)
+    np.testing.assert_allclose(O.softmax(rng.standard_normal((50, 5))).sum(axis=1), 1.0, rtol=1e-12)
+    np.testing.assert_allclose(g.sum(axis=1), 0.0, atol=1e-12)
+
+
+def test_planted_mistakes_are_caught(monkeypatch):
+    """A dropped parent term in the gain, or a flipped direction, fails the
+    brute-force pins above."""
+    rng = np.random.default_rng(3)
+    N, F, B = 400, 4, 6
+    codes = rng.integers(0, B, size=(N, F)).astype(np.int8)
+    g = rng.standard_normal(N) + (codes[:, 1] > 2)
+    h = rng.uniform(0.5, 2, N)
+    G, H = O.histogram(codes, g, h, np.arange(N), np.arange(F), np.zeros(N, int), 1, B)
+
+    def dropped_parent(G, H, mcw, eps):
+        GL = np.cumsum(G, -1)[..., :-1]
+        HL = np.cumsum(H, -1)[..., :-1]
+        GR = G.sum(-1, keepdims=True) - GL
+        HR = H.sum(-1, keepdims=True) - HL
+        return GL * GL / (HL + eps) + GR * GR / (HR + eps), np.where(GL / HL > GR / HR, -1.0, 1.0)
+
+    bad_gain, bad_dir = dropped_parent(G, H, 20.0, 1e-6)
+    bg, bd = _brute_gain(codes, g, h, np.arange(N), 1, 2, 20.0, 1e-6)
+    assert abs(bad_gain[0, 1, 2] - bg) > 1e-3
+    assert bad_dir[0, 1, 2] != bd

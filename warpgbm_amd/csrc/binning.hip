@@ -63,8 +63,9 @@ k_bin_matrix1(const float* __restrict__ X, const float* __restrict__ et, int8_t*
 // torch.quantile(col, torch.linspace(0, 1, B+1)[1:-1]) with linear
 // interpolation (warpgbm/core.py:1271-1278): level q_j by one fused
 // multiply-add as torch.linspace forms it, rank = q*(n-1), lo/hi = floor/
-// ceil(rank), w = rank - lo, lerp = w < 0.5 ? a + w*(b-a) : b - (b-a)*(1-w),
-// every other operation rounded separately (built with -ffp-contract=off).
+// ceil(rank), w = rank - lo, and the lerp with one rounding per branch as
+// torch's lerp evaluates it: w < 0.5 ? fma(w, b-a, a) : fma(-(b-a), 1-w, b).
+// Every other operation is rounded separately (built with -ffp-contract=off).
 __global__ void __launch_bounds__(256)
 k_edges_from_sorted(const float* __restrict__ sorted, int64_t n, int ncol, int num_bins,
                     float* __restrict__ edges, int edge_row_stride) {
@@ -88,14 +89,7 @@ k_edges_from_sorted(const float* __restrict__ sorted, int64_t n, int ncol, int n
     const float a = sorted[lo * ncol + c];
     const float b = sorted[hi * ncol + c];
     const float d = b - a;
-    if (w < 0.5f) {
-      const float wd = w * d;
-      e = a + wd;
-    } else {
-      const float om = 1.0f - w;
-      const float dm = d * om;
-      e = b - dm;
-    }
+    e = (w < 0.5f) ? fmaf(w, d, a) : fmaf(-d, 1.0f - w, b);
   }
   edges[(int64_t)c * edge_row_stride + (j - 1)] = e;
 }

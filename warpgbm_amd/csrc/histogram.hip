@@ -22,10 +22,12 @@
 // positions, never on scheduling: plain per-node builds and batched level
 // builds of the same rows are bit-identical.
 //
-// LDS layout per wave: G plane then H plane, each [FPL][B][65] floats
-// (index (j*B + bin)*65 + lane).  Lanes hit consecutive words within one
-// ds_add, and the final cross-lane read walks bins with stride 65, so both
-// phases are free of bank conflicts.
+// LDS layout per wave: G plane then H plane, each [FPL][B][P] floats
+// (index (j*B + bin)*P + lane) with P = 65.  Lanes hit consecutive words
+// within one ds_add, and the final cross-lane read walks bins with stride
+// 65, so both phases are free of bank conflicts.  When 65 does not fit the
+// 64 KB budget (127 bins) P = 64: accumulation stays conflict-free and only
+// the once-per-chunk read-out conflicts.
 #include "common.h"
 #include "wgbm_internal.h"
 
@@ -142,7 +144,8 @@ k_hist_build(const HistArgs a) {
   const int w1 = min(w0 + per, c1);
 
   const int B = a.n_bins;
-  const int plane = FPL * B * 65;
+  const int P = a.lds_pitch;
+  const int plane = FPL * B * P;
   float* lg = smem + wave * 2 * plane;
   float* lh = lg + plane;
   for (int q = lane; q < 2 * plane; q += kWave) lg[q] = 0.0f;
@@ -179,7 +182,7 @@ k_hist_build(const HistArgs a) {
     for (int j = 0; j < FPL; ++j) {
       const int bin = code_byte<FPL>(v, j);
       if (jon[j] && (unsigned)bin < (unsigned)B) {
-        const int o = (j * B + bin) * 65 + lane;
+        const int o = (j * B + bin) * P + lane;
         lds_add(lg + o, gg);
         lds_add(lh + o, hh);
       }
@@ -247,7 +250,7 @@ k_hist_build(const HistArgs a) {
     const int fl = e / B;
     const int bin = e - fl * B;
     if (f_base + fl >= a.n_feat) continue;
-    const int o = ((fl % FPL) * B + bin) * 65 + fl / FPL;
+    const int o = ((fl % FPL) * B + bin) * P + fl / FPL;
     float sg = smem[o], sh = smem[plane + o];
     for (int w = 1; w < W; ++w) {
       sg += smem[w * 2 * plane + o];
@@ -416,8 +419,14 @@ void plan_launch(const int32_t* seg_count, const int32_t* nseg_dev, int chunk_ro
 }
 
 // ---------------------------------------------------------------------------
+constexpr size_t kLdsBudget = 64 * 1024;
+
+int lds_pitch(const HistConfig& c, int n_bins) {
+  return (size_t)c.waves * 2 * c.fpl * n_bins * 65 * 4 <= kLdsBudget ? 65 : 64;
+}
+
 size_t lds_bytes(const HistConfig& c, int n_bins) {
-  return (size_t)c.waves * 2 * c.fpl * n_bins * 65 * sizeof(float);
+  return (size_t)c.waves * 2 * c.fpl * n_bins * lds_pitch(c, n_bins) * sizeof(float);
 }
 
 static HistConfig g_override;   // all-zero: automatic
@@ -436,12 +445,12 @@ HistConfig auto_hist_config(int n_feat, int n_bins, int64_t row_stride) {
   (void)n_feat;
   HistConfig c;
   c.chunk_rows = 8192;
-  // Widest per-lane load whose sub-histograms still leave room for two
-  // workgroups per CU (160 KB LDS); W = 4 waves unless bins are large.
+  // At most 64 KB of sub-histograms per workgroup: W = 4 waves unless bins
+  // are many, then the widest per-lane load that still fits.
   c.waves = 4;
-  while (c.waves > 1 && (size_t)c.waves * 2 * 1 * n_bins * 65 * 4 > 80 * 1024) c.waves >>= 1;
+  while (c.waves > 1 && (size_t)c.waves * 2 * 1 * n_bins * 65 * 4 > kLdsBudget) c.waves >>= 1;
   int fpl = 16;
-  while (fpl > 1 && ((size_t)c.waves * 2 * fpl * n_bins * 65 * 4 > 80 * 1024 || row_stride % fpl != 0))
+  while (fpl > 1 && ((size_t)c.waves * 2 * fpl * n_bins * 65 * 4 > kLdsBudget || row_stride % fpl != 0))
     fpl >>= 1;
   c.fpl = fpl;
   return c;
@@ -485,7 +494,11 @@ int hist_launch(HistArgs a, const HistConfig& c, int64_t rows_max, int64_t nseg_
   a.chunk_rows = c.chunk_rows;
   a.nslab = (int)ceil_div(a.n_feat, kWave * c.fpl);
   a.wide_ok = (a.row_stride % c.fpl == 0) && ((uintptr_t)a.codes % (uintptr_t)c.fpl == 0);
-  if (lds_bytes(c, a.n_bins) > 160 * 1024) { set_error("histogram LDS budget exceeded"); return kTooBig; }
+  a.lds_pitch = lds_pitch(c, a.n_bins);
+  if (lds_bytes(c, a.n_bins) > kLdsBudget) {
+    set_error("histogram LDS budget exceeded (%d bins, %d waves, %d features per lane)", a.n_bins, c.waves, c.fpl);
+    return kTooBig;
+  }
   hipLaunchKernelGGL(k_hist_plan, dim3(1), dim3(1024), 0, st, a.seg_count, a.nseg_dev, c.chunk_rows,
                      a.seg_first, a.seg_pfirst, multi, n_multi);
   WGBM_LAUNCHED("k_hist_plan");

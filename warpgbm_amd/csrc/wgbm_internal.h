@@ -38,6 +38,7 @@ struct HistArgs {
   float* part_g;           // chunk partials
   float* part_h;
   int32_t accumulate;      // 1: out += result (reference op), 0: out = result
+  int32_t lds_pitch;       // floats between consecutive (j, bin) rows of a plane
 };
 
 // Configuration used by every histogram build: the automatic choice with
