@@ -1,0 +1,266 @@
+"""Benchmark: WarpGBM fit on 1M x 1000 (100 trees), batched vs plain, on one MI355X.
+
+Workload (one "step" = one complete 100-tree fit of the batched level-wise
+engine on bin codes already resident in HBM):
+  X ~ N(0,1) float32 [1_000_000, 1000] generated on the device (seeded),
+  y = tanh(X[:, :20] w) + N(0, 0.1), w ~ N(0, 1); 32 quantile bins computed
+  on the device; depth 5, 100 trees, learning rate 0.01, colsample 1.0.
+
+Prints ONE JSON line: value = seconds per batched fit (lower is better),
+beside the plain per-node engine's fit time on the same data (and whether
+the two produce identical trees), the histogram kernel's achieved HBM
+bandwidth (HIP events around every histogram launch of one fit), the
+predict latency of 10k-row requests on the resident compiled forest, and
+the oracle timed on the host cores for a bounded sample.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=3)
+    p.add_argument("--warmup", type=int, default=1)
+    p.add_argument("--rows", type=int, default=1_000_000)
+    p.add_argument("--features", type=int, default=1000)
+    p.add_argument("--trees", type=int, default=100)
+    p.add_argument("--depth", type=int, default=5)
+    p.add_argument("--bins", type=int, default=32)
+    p.add_argument("--plain-trees", type=int, default=100,
+                   help="trees of the plain per-node fit (its time is scaled to --trees if fewer)")
+    p.add_argument("--cpu-rows", type=int, default=100_000, help="rows of the CPU oracle sample")
+    p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--no-plain", action="store_true")
+    p.add_argument("--profile-only", action="store_true", help="one fit, no extras (for rocprof)")
+    return p.parse_args()
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def make_data(args, dev):
+    import torch
+    g = torch.Generator(device=dev)
+    g.manual_seed(0)
+    N, F = args.rows, args.features
+    X = torch.randn((N, F), generator=g, device=dev, dtype=torch.float32)
+    w = torch.randn((20,), generator=g, device=dev, dtype=torch.float32)
+    y = torch.tanh(X[:, :20] @ w) + 0.1 * torch.randn((N,), generator=g, device=dev, dtype=torch.float32)
+    return X, y.contiguous()
+
+
+def bin_on_device(X, bins):
+    from warpgbm_amd import WarpGBM
+    m = WarpGBM(num_bins=bins)
+    m.device = X.device
+    edges = m._quantile_edges(X)
+    codes = m._bin_matrix(X, edges)
+    return codes, edges
+
+
+def hist_algorithmic_bytes(tables, N, F, B, D, E=1):
+    """Bytes the histogram builds of one fit must move: every built row reads
+    F code bytes + 8 B (g, h) + 4 B row id; every built (node, era) writes
+    F*B*8 B.  Built nodes: each root, then the smaller child of every split
+    whose children are above depth D."""
+    st, ct = tables["status"], tables["count"]
+    rows = 0
+    nodes = 0
+    R, K, _ = st.shape
+    for r in range(R):
+        for k in range(K):
+            rows += N
+            nodes += 1
+            for d in range(1, D):
+                for j in range(1 << (d - 1)):
+                    p = (1 << (d - 1)) - 1 + j
+                    if st[r, k, p] == 1:
+                        cl, cr = ct[r, k, 2 * p + 1], ct[r, k, 2 * p + 2]
+                        rows += min(cl, cr)
+                        nodes += 1
+    return rows * (F + 8 + 4) + nodes * E * F * B * 8, rows
+
+
+def trees_identical(fa, fb):
+    import torch
+
+    def same(a, b):
+        if ("leaf_value" in a) != ("leaf_value" in b):
+            return False
+        if "leaf_value" in a:
+            return a["leaf_value"] == b["leaf_value"] and a["samples"] == b["samples"]
+        fa_ = int(a["feature"].item()) if torch.is_tensor(a["feature"]) else int(a["feature"])
+        fb_ = int(b["feature"].item()) if torch.is_tensor(b["feature"]) else int(b["feature"])
+        return fa_ == fb_ and a["bin"] == b["bin"] and same(a["left"], b["left"]) and same(a["right"], b["right"])
+
+    return all(same(a, b) for a, b in zip(fa, fb))
+
+
+def cpu_baseline(codes_host, y_host, args):
+    """FP64 oracle, one tree on the first cpu_rows rows, extrapolated."""
+    from oracle import gbm_oracle as O
+    n = codes_host.shape[0]
+    g = (y_host - np.float32(y_host.mean())).astype(np.float64)
+    t0 = time.perf_counter()
+    O.grow_tree(codes_host, g, np.ones(n), np.zeros(n, np.int32), 1, args.bins, args.depth, 20, 0.0, 1e-6)
+    dt = time.perf_counter() - t0
+    scale = (args.rows / n) * args.trees
+    return {
+        "value": dt * scale,
+        "unit": "s",
+        "cores": 1,
+        "kind": "port",
+        "sample": (f"FP64 NumPy oracle grow_tree, 1 tree (depth {args.depth}, {args.bins} bins) on the first "
+                   f"{n} rows x {args.features} features took {dt:.2f} s on 1 host core; value = that x "
+                   f"{args.rows // n} (rows) x {args.trees} (trees), linear extrapolation"),
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (this engine runs on a "
+              "single MI355X; see DESIGN.md for how the path would shard).", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from warpgbm_amd import WarpGBM, _lib
+    from warpgbm_amd.serving import HeapForest
+
+    dev = torch.device("cuda:0")
+    log(f"device {torch.cuda.get_device_name(0)}; generating {args.rows}x{args.features}")
+    X, y = make_data(args, dev)
+    t0 = time.perf_counter()
+    codes, edges = bin_on_device(X, args.bins)
+    torch.cuda.synchronize()
+    bin_s = time.perf_counter() - t0
+    del X
+    torch.cuda.empty_cache()
+    log(f"binned on device in {bin_s:.2f} s")
+
+    kw = dict(objective="regression", num_bins=args.bins, max_depth=args.depth, n_estimators=args.trees,
+              learning_rate=0.01, colsample_bytree=1.0, device=dev)
+
+    def fit(engine, trees=None):
+        k = dict(kw)
+        if trees:
+            k["n_estimators"] = trees
+        m = WarpGBM(engine=engine, **k)
+        import contextlib, io
+        with contextlib.redirect_stdout(io.StringIO()):
+            m.fit_binned(codes, y, bin_edges=edges)
+        return m
+
+    if args.profile_only:
+        fit("batched")
+        torch.cuda.synchronize()
+        log("profile-only fit done")
+        return
+
+    for i in range(args.warmup):
+        fit("batched")
+        log(f"warmup {i} done")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        mb = fit("batched")
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    log(f"batched fit: {dt:.3f} s per {args.trees}-tree fit")
+
+    # histogram kernel: HIP events around every build/merge launch of one fit
+    _lib.call("wgbm_profile_begin")
+    mp_ = fit("batched")
+    out4 = (__import__("ctypes").c_double * 4)()
+    _lib.call("wgbm_profile_end", out4)
+    build_ms, n_build, merge_ms, n_merge = out4[0], int(out4[1]), out4[2], int(out4[3])
+    alg_bytes, built_rows = hist_algorithmic_bytes(mp_._last_tables, args.rows, args.features, args.bins, args.depth)
+    achieved = alg_bytes / (build_ms * 1e-3) / 1e9
+    log(f"hist build {build_ms:.1f} ms over {n_build} launches ({achieved:.0f} GB/s), merge {merge_ms:.1f} ms")
+
+    # plain per-node engine on the same data
+    plain = None
+    if not args.no_plain:
+        pt = min(args.plain_trees, args.trees)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        mpl = fit("plain", trees=pt)
+        torch.cuda.synchronize()
+        pdt = (time.perf_counter() - t0) * (args.trees / pt)
+        same = trees_identical(mb.forest[:pt], mpl.forest[:pt])
+        plain = {"fit_s": pdt, "trees_timed": pt, "trees_identical_to_batched": bool(same),
+                 "speedup_batched_vs_plain": pdt / dt}
+        log(f"plain fit: {pdt:.2f} s (x{pdt / dt:.1f}); identical trees: {same}")
+
+    # predict latency: 10k-row requests on the resident compiled forest
+    tb = mb._last_tables
+    hf = HeapForest.from_tables(tb["status"].reshape(-1, tb["status"].shape[-1]),
+                                tb["feat"].reshape(-1, tb["feat"].shape[-1]),
+                                tb["bin"].reshape(-1, tb["bin"].shape[-1]),
+                                tb["value"].reshape(-1, tb["value"].shape[-1]), args.depth, 0.01, args.features,
+                                [mb.base_prediction], device=dev)
+    req = codes[:10000].contiguous()
+    out = torch.empty((10000, 1), dtype=torch.float32, device=dev)
+    ws = hf.workspace(10000)
+    lat = []
+    for i in range(220):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        hf.predict_into(req, out, workspace=ws)
+        torch.cuda.synchronize()
+        lat.append((time.perf_counter() - t0) * 1e3)
+    lat = np.array(lat[20:])
+    ref = mb.predict_binned(req)
+    pred_err = float((out[:, 0] - ref).abs().max().item())
+    log(f"predict 10k rows: p50 {np.percentile(lat, 50):.3f} ms p99 {np.percentile(lat, 99):.3f} ms")
+
+    cpu = None
+    if not args.no_cpu:
+        n = min(args.cpu_rows, args.rows)
+        ch = codes[:n].cpu().numpy()
+        yh = y[:n].cpu().numpy()
+        cpu = cpu_baseline(ch, yh, args)
+        log(f"cpu oracle: {cpu['value']:.0f} s (extrapolated)")
+
+    res = {
+        "metric": "fit sec (1M×1K, 100 trees) batched vs plain; hist GB/s; predict p99 ms",
+        "value": dt,
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt * 1e3,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (device-generated, seeded), bin codes resident in HBM before the timed region",
+        "config": {"workload": f"regression {args.rows}x{args.features}, {args.bins} bins, depth {args.depth}, "
+                               f"{args.trees} trees, lr 0.01, batched level-wise engine",
+                   "rows": args.rows, "features": args.features, "trees": args.trees, "depth": args.depth,
+                   "bins": args.bins},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_hist_build", "launches": n_build, "kernel_ms_per_fit": build_ms,
+                     "algorithmic_bytes_per_fit": alg_bytes, "built_rows_per_fit": built_rows,
+                     "merge_ms_per_fit": merge_ms},
+        "plain": plain,
+        "predict": {"rows_per_request": 10000, "p50_ms": float(np.percentile(lat, 50)),
+                    "p99_ms": float(np.percentile(lat, 99)), "rows_per_s": 10000 / (np.mean(lat) * 1e-3),
+                    "max_abs_diff_vs_reference_op": pred_err, "mode": "one request per launch (unbatched)"},
+        "binning_s": bin_s,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(res))
+
+
+if __name__ == "__main__":
+    main()

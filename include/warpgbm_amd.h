@@ -137,6 +137,12 @@ int wgbm_bin_matrix(const float* X, int64_t n, int32_t n_feat, const float* edge
 int wgbm_set_hist_config(int32_t chunk_rows, int32_t waves, int32_t features_per_lane);
 int wgbm_get_hist_config(int32_t n_feat, int32_t num_bins, int64_t n_cols, int32_t* out3);
 
+/* Live timing of histogram launches with HIP events on their streams:
+ * begin clears and enables; end waits and returns
+ * out4 = {build ms, build launches, merge ms, merge launches}. */
+int wgbm_profile_begin(void);
+int wgbm_profile_end(double* out4);
+
 typedef struct wgbm_fit_config {
   int64_t n_rows;           /* N */
   int64_t n_cols;           /* columns of the bin matrix */

@@ -61,6 +61,8 @@ SIGNATURES = {
     "wgbm_bin_matrix": (C.c_int, [vp, i64, i32, vp, i32, vp, vp, sz, vp]),
     "wgbm_set_hist_config": (C.c_int, [i32, i32, i32]),
     "wgbm_get_hist_config": (C.c_int, [i32, i32, i64, C.POINTER(i32)]),
+    "wgbm_profile_begin": (C.c_int, []),
+    "wgbm_profile_end": (C.c_int, [C.POINTER(f64)]),
     "wgbm_fit_workspace_bytes": (sz, [C.POINTER(FitConfig)]),
     "wgbm_fit_rounds": (C.c_int, [C.POINTER(FitConfig), C.POINTER(FitIO), i32, vp, sz, vp]),
     "wgbm_heap_compile": (C.c_int, [vp, vp, vp, vp, i32, i32, i32, f32, vp, vp, vp]),

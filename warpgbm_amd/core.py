@@ -443,6 +443,58 @@ class WarpGBM(BaseEstimator, RegressorMixin):
         self.root_node_indices = (self._root_order if self._root_order is not None else
                                   torch.arange(self.num_samples, dtype=torch.int32, device=self.device))
 
+    def fit_binned(self, bin_indices, y, era_id=None, sample_weight=None, bin_edges=None):
+        """Regression fit from int8 bin codes already resident on the device.
+
+        ``bin_indices``: int8 [N, F] device tensor; ``y``: float32 [N] device
+        tensor.  No host copies and no binning: this is the training step
+        of ``fit`` on data that already lives in HBM (benchmarks, pipelines
+        that bin once and train many models).  ``bin_edges`` (float32
+        [F, B-1]) is stored for later float predictions; default: the
+        pre-binned edges 1..B-1.
+        """
+        if self.objective != "regression":
+            raise ValueError("fit_binned supports objective='regression'")
+        N, F = bin_indices.shape
+        self._is_fitted = False
+        self._trees_trained = 0
+        self.forest = [{} for _ in range(self.n_estimators)]
+        self.training_loss, self.eval_loss = [], []
+        self._invalidate_prediction_cache()
+        self.num_samples, self.num_features = N, F
+        self.num_outputs = 1
+        era = np.zeros(N, np.int32) if era_id is None else np.unique(np.asarray(era_id).astype(np.int32),
+                                                                      return_inverse=True)[1].astype(np.int32)
+        self._era_np = era.reshape(-1)
+        self.unique_eras = torch.arange(int(self._era_np.max()) + 1, device=self.device)
+        self.num_eras = len(self.unique_eras)
+        self.bin_indices = bin_indices.contiguous()
+        self.era_indices = torch.from_numpy(self._era_np).to(self.device)
+        self.Y_gpu = y.to(torch.float32).contiguous()
+        w = torch.ones(N, dtype=torch.float32, device=self.device) if sample_weight is None else sample_weight
+        self.sample_weight_gpu = w.to(torch.float32).contiguous()
+        self.bin_edges = (bin_edges if bin_edges is not None else
+                          torch.arange(1, self.num_bins, dtype=torch.float32, device=self.device).repeat(F, 1))
+        self._prepare_monotonic_constraints(F)
+        self._feature_ids = torch.arange(F, dtype=torch.int32, device=self.device)
+        self.feature_indices = self._feature_ids
+        if self.num_eras > 1:
+            self._root_order = torch.from_numpy(np.argsort(self._era_np, kind="stable").astype(np.int32)).to(self.device)
+        else:
+            self._root_order = None
+        self._era_count = torch.from_numpy(np.bincount(self._era_np, minlength=self.num_eras).astype(np.int32)).to(
+            self.device)
+        self.root_node_indices = (self._root_order if self._root_order is not None else
+                                  torch.arange(N, dtype=torch.int32, device=self.device))
+        self.base_prediction = float(self.Y_gpu.double().mean().item())
+        self.gradients = torch.full((N,), np.float32(self.base_prediction), dtype=torch.float32, device=self.device)
+        self._setup_eval(None, None, None, None, None, "mse", classification=False)
+        with torch.no_grad():
+            self.grow_forest()
+        del self.bin_indices
+        del self.Y_gpu
+        return self
+
     def _setup_eval(self, X_eval, y_eval, sample_weight_eval, eval_every_n_trees, early_stopping_rounds,
                     eval_metric, classification):
         if X_eval is not None and y_eval is not None:
@@ -814,7 +866,9 @@ class WarpGBM(BaseEstimator, RegressorMixin):
         io.node_era_gain = t["era_gain"].data_ptr()
         _lib.call("wgbm_fit_rounds", ctypes.byref(cfg), ctypes.byref(io), n, ws.data_ptr(), ws.numel(),
                   _stream(dev))
-        return {key: v.cpu().numpy() for key, v in t.items()}
+        tables = {key: v.cpu().numpy() for key, v in t.items()}
+        self._last_tables = tables
+        return tables
 
     def _append_rounds(self, tables, n, K, feats):
         st, ft, bt, vt, ct, eg = (tables[k] for k in ("status", "feat", "bin", "value", "count", "era_gain"))

@@ -30,6 +30,8 @@
 // the once-per-chunk read-out conflicts.
 #include "common.h"
 #include "wgbm_internal.h"
+#include <utility>
+#include <vector>
 
 namespace wgbm {
 
@@ -468,23 +470,56 @@ size_t hist_partial_floats(const HistConfig& c, int n_feat, int n_bins, int64_t 
   return (size_t)chunks * nslab * kWave * c.fpl * n_bins;
 }
 
+// ---------------------------------------------------------------------------
+// Live kernel timing: when enabled, every histogram build and merge launch
+// is bracketed by HIP events on its own stream.
+struct KernelTimer {
+  bool on = false;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> build, merge;
+};
+static KernelTimer g_timer;
+
+static void timer_mark(std::vector<std::pair<hipEvent_t, hipEvent_t>>& v, bool start, hipStream_t st) {
+  if (!g_timer.on) return;
+  if (start) {
+    hipEvent_t a, b;
+    hipEventCreate(&a);
+    hipEventCreate(&b);
+    v.emplace_back(a, b);
+    hipEventRecord(a, st);
+  } else {
+    hipEventRecord(v.back().second, st);
+  }
+}
+
+static void timer_clear() {
+  for (auto* v : {&g_timer.build, &g_timer.merge})
+    for (auto& p : *v) { hipEventDestroy(p.first); hipEventDestroy(p.second); }
+  g_timer.build.clear();
+  g_timer.merge.clear();
+}
+
 template <int FPL>
 static int launch_fpl(const HistArgs& a, const HistConfig& c, int64_t max_chunks, int64_t max_multi,
                       int32_t* multi, int32_t* n_multi, hipStream_t st) {
   const size_t lds = lds_bytes(c, a.n_bins);
   const int64_t groups = ceil_div(max_chunks, 8) * 8 * a.nslab;
   if (groups > 0x7fffffff) { set_error("histogram grid too large"); return kTooBig; }
+  timer_mark(g_timer.build, true, st);
   if (a.gh)
     hipLaunchKernelGGL((k_hist_build<FPL, true>), dim3((unsigned)groups), dim3(64 * c.waves), lds, st, a);
   else
     hipLaunchKernelGGL((k_hist_build<FPL, false>), dim3((unsigned)groups), dim3(64 * c.waves), lds, st, a);
   WGBM_LAUNCHED("k_hist_build");
+  timer_mark(g_timer.build, false, st);
   if (max_multi > 0) {
     const int SL = kWave * FPL * a.n_bins;
     const int nblk = (int)ceil_div(SL, 256);
     const int64_t mg = max_multi * a.nslab * nblk;
+    timer_mark(g_timer.merge, true, st);
     hipLaunchKernelGGL((k_hist_merge<FPL>), dim3((unsigned)mg), dim3(256), 0, st, a, multi, n_multi, nblk);
     WGBM_LAUNCHED("k_hist_merge");
+    timer_mark(g_timer.merge, false, st);
   }
   return kOk;
 }
@@ -519,6 +554,36 @@ int hist_launch(HistArgs a, const HistConfig& c, int64_t rows_max, int64_t nseg_
 }  // namespace wgbm
 
 using namespace wgbm;
+
+// Start timing histogram launches (clears earlier records).
+extern "C" int wgbm_profile_begin(void) {
+  timer_clear();
+  g_timer.on = true;
+  return kOk;
+}
+
+// Stop timing; out4 = {build ms, build launches, merge ms, merge launches}.
+// Waits for the recorded events.
+extern "C" int wgbm_profile_end(double* out4) {
+  g_timer.on = false;
+  double t[2] = {0.0, 0.0};
+  int k = 0;
+  for (auto* v : {&g_timer.build, &g_timer.merge}) {
+    for (auto& p : *v) {
+      WGBM_HIP(hipEventSynchronize(p.second));
+      float ms = 0.0f;
+      WGBM_HIP(hipEventElapsedTime(&ms, p.first, p.second));
+      t[k] += ms;
+    }
+    ++k;
+  }
+  out4[0] = t[0];
+  out4[1] = (double)g_timer.build.size();
+  out4[2] = t[1];
+  out4[3] = (double)g_timer.merge.size();
+  timer_clear();
+  return kOk;
+}
 
 // Workspace carve-up for the reference-style histogram op.
 struct Hist3Layout {

@@ -1,0 +1,169 @@
+"""Continuous-batching predict server over a resident heap forest.
+
+Requests (int8 bin-code matrices of any row count) are queued; a worker
+thread admits them in arrival order while the device memory they need fits
+a budget, packs every admitted request into one traversal launch (a
+"batch"), and keeps up to ``slots`` batches in flight on separate HIP
+streams, so the host-to-device copy of one batch overlaps the traversal of
+the previous one.  Requests that arrive while batches are running join the
+next batch as soon as a slot and budget free up.
+
+Each row's score depends only on its own codes and the forest (the serving
+kernel sums trees in a fixed order), so a request's result is bit-identical
+whether it is served alone or mixed with others.
+"""
+from __future__ import annotations
+
+import collections
+import threading
+import time
+from concurrent.futures import Future
+
+import numpy as np
+import torch
+
+
+class _Request:
+    __slots__ = ("codes", "future", "t_submit", "rows")
+
+    def __init__(self, codes, future):
+        self.codes = codes
+        self.future = future
+        self.t_submit = time.perf_counter()
+        self.rows = codes.shape[0]
+
+
+class _Slot:
+    def __init__(self, forest, max_rows, device):
+        F, K = forest.n_feat, forest.n_classes
+        self.stream = torch.cuda.Stream(device=device)
+        self.h_codes = torch.empty((max_rows, F), dtype=torch.int8, pin_memory=True)
+        self.d_codes = torch.empty((max_rows, F), dtype=torch.int8, device=device)
+        self.d_out = torch.empty((max_rows, K), dtype=torch.float32, device=device)
+        self.h_out = torch.empty((max_rows, K), dtype=torch.float32, pin_memory=True)
+        need = forest.workspace(max_rows).numel()
+        self.ws = torch.empty(need, dtype=torch.uint8, device=device)
+        self.event = None
+        self.reqs = []
+        self.rows = 0
+        self.bytes = 0
+
+
+class PredictServer:
+    """Serve predictions of a compiled ``HeapForest``.
+
+    memory_budget: device bytes the in-flight requests may hold (codes,
+    outputs and reduction workspace per row); a request needing more than
+    the whole budget is rejected.  max_batch_rows caps one launch.
+    """
+
+    def __init__(self, forest, memory_budget=1 << 30, max_batch_rows=200_000, slots=3):
+        self.forest = forest
+        self.device = forest.device
+        self.max_batch_rows = int(max_batch_rows)
+        self.budget = int(memory_budget)
+        K = forest.n_classes
+        self.bytes_per_row = forest.n_feat + 8 * K + 4 * K * forest.n_blocks
+        self._slots = [_Slot(forest, self.max_batch_rows, self.device) for _ in range(slots)]
+        self._free = collections.deque(self._slots)
+        self._busy = collections.deque()
+        self._queue = collections.deque()
+        self._cv = threading.Condition()
+        self._in_flight_bytes = 0
+        self._stop = False
+        self._thread = threading.Thread(target=self._loop, name="wgbm-predict-server", daemon=True)
+        self.latencies_ms = []
+        self.batches = 0
+        self._thread.start()
+
+    # ---------------------------------------------------------------- client
+    def submit(self, codes: np.ndarray) -> Future:
+        """Queue one request (int8 [rows, n_feat]); the future yields float32
+        [rows, n_classes] scores."""
+        codes = np.ascontiguousarray(codes, dtype=np.int8)
+        if codes.ndim != 2 or codes.shape[1] != self.forest.n_feat:
+            raise ValueError(f"request must be int8 [rows, {self.forest.n_feat}]")
+        if codes.shape[0] > self.max_batch_rows or codes.shape[0] * self.bytes_per_row > self.budget:
+            raise ValueError("request exceeds the server's batch or memory budget")
+        fut = Future()
+        with self._cv:
+            self._queue.append(_Request(codes, fut))
+            self._cv.notify()
+        return fut
+
+    def close(self):
+        with self._cv:
+            self._stop = True
+            self._cv.notify()
+        self._thread.join()
+
+    # ---------------------------------------------------------------- worker
+    def _loop(self):
+        torch.cuda.set_device(self.device)
+        while True:
+            with self._cv:
+                if self._stop and not self._busy and not self._queue:
+                    return
+                if not self._busy and not (self._queue and self._free):
+                    self._cv.wait(timeout=0.05)
+                    continue
+                batch = self._admit_locked()
+            if batch is not None:
+                self._launch(*batch)
+            done = self._complete_ready()
+            if batch is None and not done:
+                time.sleep(20e-6)
+
+    def _admit_locked(self):
+        """Take requests in arrival order while rows and budget allow."""
+        if not self._queue or not self._free:
+            return None
+        reqs, rows, nbytes = [], 0, 0
+        while self._queue:
+            r = self._queue[0]
+            b = r.rows * self.bytes_per_row
+            if rows + r.rows > self.max_batch_rows or self._in_flight_bytes + nbytes + b > self.budget:
+                break
+            reqs.append(self._queue.popleft())
+            rows += r.rows
+            nbytes += b
+        if not reqs:
+            return None
+        self._in_flight_bytes += nbytes
+        return self._free.popleft(), reqs, rows, nbytes
+
+    def _launch(self, slot, reqs, rows, nbytes):
+        o = 0
+        hc = slot.h_codes.numpy()
+        for r in reqs:
+            hc[o:o + r.rows] = r.codes
+            o += r.rows
+        with torch.cuda.stream(slot.stream):
+            slot.d_codes[:rows].copy_(slot.h_codes[:rows], non_blocking=True)
+            self.forest.predict_into(slot.d_codes[:rows], slot.d_out[:rows], stream=slot.stream.cuda_stream,
+                                     workspace=slot.ws)
+            slot.h_out[:rows].copy_(slot.d_out[:rows], non_blocking=True)
+            ev = torch.cuda.Event()
+            ev.record(slot.stream)
+        slot.event, slot.reqs, slot.rows, slot.bytes = ev, reqs, rows, nbytes
+        self.batches += 1
+        self._busy.append(slot)
+
+    def _complete_ready(self):
+        """Deliver every finished batch (in launch order); True if any."""
+        done = False
+        while self._busy and self._busy[0].event.query():
+            slot = self._busy.popleft()
+            out = slot.h_out.numpy()
+            o = 0
+            now = time.perf_counter()
+            for r in slot.reqs:
+                r.future.set_result(out[o:o + r.rows].copy())
+                self.latencies_ms.append((now - r.t_submit) * 1e3)
+                o += r.rows
+            with self._cv:
+                self._in_flight_bytes -= slot.bytes
+                slot.reqs = []
+                self._free.append(slot)
+            done = True
+        return done
