@@ -160,6 +160,13 @@ def main():
         return m
 
     if args.profile_only:
+        # a plain copy of known size (256 MiB read + 256 MiB written) to
+        # calibrate the FETCH_SIZE / WRITE_SIZE counters
+        src = torch.ones(64 << 20, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
         fit("batched")
         torch.cuda.synchronize()
         log("profile-only fit done")
