@@ -84,10 +84,10 @@ def hist_algorithmic_bytes(tables, N, F, B, D, E=1):
                 for j in range(1 << (d - 1)):
                     p = (1 << (d - 1)) - 1 + j
                     if st[r, k, p] == 1:
-                        cl, cr = ct[r, k, 2 * p + 1], ct[r, k, 2 * p + 2]
+                        cl, cr = int(ct[r, k, 2 * p + 1]), int(ct[r, k, 2 * p + 2])
                         rows += min(cl, cr)
                         nodes += 1
-    return rows * (F + 8 + 4) + nodes * E * F * B * 8, rows
+    return int(rows * (F + 8 + 4) + nodes * E * F * B * 8), int(rows)
 
 
 def trees_identical(fa, fb):

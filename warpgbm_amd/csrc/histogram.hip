@@ -22,12 +22,10 @@
 // positions, never on scheduling: plain per-node builds and batched level
 // builds of the same rows are bit-identical.
 //
-// LDS layout per wave: G plane then H plane, each [FPL][B][P] floats
-// (index (j*B + bin)*P + lane) with P = 65.  Lanes hit consecutive words
-// within one ds_add, and the final cross-lane read walks bins with stride
-// 65, so both phases are free of bank conflicts.  When 65 does not fit the
-// 64 KB budget (127 bins) P = 64: accumulation stays conflict-free and only
-// the once-per-chunk read-out conflicts.
+// LDS layout per wave: G plane then H plane, each [FPL][B][64] floats
+// (index (j*B + bin)*64 + lane).  Whatever bins the lanes of one ds_add
+// hit, lane l's word sits in bank l (64 banks), so accumulation is free of
+// bank conflicts; the read-out walks the words in lane order.
 #include "common.h"
 #include "wgbm_internal.h"
 #include <utility>
@@ -248,11 +246,17 @@ k_hist_build(const HistArgs a) {
   const int f_base = slab * kWave * FPL;
   const bool direct = (nch == 1);
   const int slot = a.seg_slot[s];
-  for (int e = threadIdx.x; e < SL; e += blockDim.x) {
-    const int fl = e / B;
-    const int bin = e - fl * B;
+  // Thread t reads LDS word t (lane-fastest: conflict free) = feature
+  // lane*FPL + j, bin, with (j, bin) = divmod(t / 64, B).
+  for (int t = threadIdx.x; t < SL; t += blockDim.x) {
+    const int r = t >> 6;
+    const int ln = t & 63;
+    const int j = r / B;
+    const int bin = r - j * B;
+    const int fl = ln * FPL + j;
     if (f_base + fl >= a.n_feat) continue;
-    const int o = ((fl % FPL) * B + bin) * P + fl / FPL;
+    const int e = fl * B + bin;
+    const int o = r * P + ln;
     float sg = smem[o], sh = smem[plane + o];
     for (int w = 1; w < W; ++w) {
       sg += smem[w * 2 * plane + o];
@@ -421,11 +425,12 @@ void plan_launch(const int32_t* seg_count, const int32_t* nseg_dev, int chunk_ro
 }
 
 // ---------------------------------------------------------------------------
+// Automatic configurations keep two workgroups' sub-histograms per CU
+// (64 KB each); explicit ones may use a workgroup's full 160 KB.
 constexpr size_t kLdsBudget = 64 * 1024;
+constexpr size_t kLdsMax = 160 * 1024;
 
-int lds_pitch(const HistConfig& c, int n_bins) {
-  return (size_t)c.waves * 2 * c.fpl * n_bins * 65 * 4 <= kLdsBudget ? 65 : 64;
-}
+int lds_pitch(const HistConfig&, int) { return 64; }
 
 size_t lds_bytes(const HistConfig& c, int n_bins) {
   return (size_t)c.waves * 2 * c.fpl * n_bins * lds_pitch(c, n_bins) * sizeof(float);
@@ -450,9 +455,9 @@ HistConfig auto_hist_config(int n_feat, int n_bins, int64_t row_stride) {
   // At most 64 KB of sub-histograms per workgroup: W = 4 waves unless bins
   // are many, then the widest per-lane load that still fits.
   c.waves = 4;
-  while (c.waves > 1 && (size_t)c.waves * 2 * 1 * n_bins * 65 * 4 > kLdsBudget) c.waves >>= 1;
+  while (c.waves > 1 && (size_t)c.waves * 2 * 1 * n_bins * 64 * 4 > kLdsBudget) c.waves >>= 1;
   int fpl = 16;
-  while (fpl > 1 && ((size_t)c.waves * 2 * fpl * n_bins * 65 * 4 > kLdsBudget || row_stride % fpl != 0))
+  while (fpl > 1 && ((size_t)c.waves * 2 * fpl * n_bins * 64 * 4 > kLdsBudget || row_stride % fpl != 0))
     fpl >>= 1;
   c.fpl = fpl;
   return c;
@@ -505,6 +510,12 @@ static int launch_fpl(const HistArgs& a, const HistConfig& c, int64_t max_chunks
   const size_t lds = lds_bytes(c, a.n_bins);
   const int64_t groups = ceil_div(max_chunks, 8) * 8 * a.nslab;
   if (groups > 0x7fffffff) { set_error("histogram grid too large"); return kTooBig; }
+  if (lds > 64 * 1024) {  // opt in to more than 64 KB of dynamic LDS
+    WGBM_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_hist_build<FPL, true>),
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    WGBM_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_hist_build<FPL, false>),
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  }
   timer_mark(g_timer.build, true, st);
   if (a.gh)
     hipLaunchKernelGGL((k_hist_build<FPL, true>), dim3((unsigned)groups), dim3(64 * c.waves), lds, st, a);
@@ -530,7 +541,7 @@ int hist_launch(HistArgs a, const HistConfig& c, int64_t rows_max, int64_t nseg_
   a.nslab = (int)ceil_div(a.n_feat, kWave * c.fpl);
   a.wide_ok = (a.row_stride % c.fpl == 0) && ((uintptr_t)a.codes % (uintptr_t)c.fpl == 0);
   a.lds_pitch = lds_pitch(c, a.n_bins);
-  if (lds_bytes(c, a.n_bins) > kLdsBudget) {
+  if (lds_bytes(c, a.n_bins) > kLdsMax) {
     set_error("histogram LDS budget exceeded (%d bins, %d waves, %d features per lane)", a.n_bins, c.waves, c.fpl);
     return kTooBig;
   }
