@@ -133,9 +133,11 @@ int wgbm_bin_matrix(const float* X, int64_t n, int32_t n_feat, const float* edge
 
 /* Histogram tiling shared by every histogram build (plain and batched):
  * rows per chunk, waves per workgroup (these two fix the float32 summation
- * order) and features per lane (load width).  0 = automatic. */
-int wgbm_set_hist_config(int32_t chunk_rows, int32_t waves, int32_t features_per_lane);
-int wgbm_get_hist_config(int32_t n_feat, int32_t num_bins, int64_t n_cols, int32_t* out3);
+ * order), features per lane (load width) and rows whose code loads are in
+ * flight per wave.  0 = automatic.  wgbm_get_hist_config returns the
+ * configuration used for a shape: out4 = {chunk_rows, waves, fpl, depth}. */
+int wgbm_set_hist_config(int32_t chunk_rows, int32_t waves, int32_t features_per_lane, int32_t rows_in_flight);
+int wgbm_get_hist_config(int32_t n_feat, int32_t num_bins, int64_t n_cols, int32_t* out4);
 
 /* Live timing of histogram launches with HIP events on their streams:
  * begin clears and enables; end waits and returns

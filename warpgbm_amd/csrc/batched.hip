@@ -33,8 +33,8 @@ struct FitLayout {
   size_t dec_feat, dec_bin, era_gain, cand_agree, cand_crit, cand_idx;
   size_t b_seg_start, b_seg_count, b_seg_slot, n_build, sib, n_split;
   size_t h_seg_first, h_seg_pfirst, h_nseg, h_multi, h_n_multi, h_part;
-  size_t p_pcount, p_seg_first, p_nseg, p_multi, p_n_multi, p_lbase, p_rbase, p_segL;
-  size_t l_lcount, l_seg_first, l_nseg, l_multi, l_n_multi, l_part, l_val;
+  size_t p_flag, p_seg_first, p_nseg, p_multi, p_n_multi, p_lbase, p_rbase, p_segL;
+  size_t l_seg_first, l_nseg, l_multi, l_n_multi, l_part, l_val;
   size_t root_seg_start, root_seg_count, root_seg_slot, root_nseg, era_start;
   // sizes
   int64_t KN, max_nodes, node_elems, hist_nodes, part_chunks, leaf_chunks, hist_part_floats;
@@ -92,7 +92,7 @@ static FitLayout fit_layout(const wgbm_fit_config& c) {
   L.hist_part_floats = (int64_t)hist_partial_floats(L.hc, c.n_feat, c.n_bins, L.KN);
   L.h_part = take(2 * 4 * L.hist_part_floats);
   L.part_chunks = ceil_div(L.KN, kPartChunk) + MS;
-  L.p_pcount = take(4 * MS);
+  L.p_flag = take(L.KN);
   L.p_seg_first = take(4 * (MS + 1));
   L.p_nseg = take(4);
   L.p_multi = take(4 * (L.KN / (kPartChunk + 1) + 2));
@@ -101,7 +101,6 @@ static FitLayout fit_layout(const wgbm_fit_config& c) {
   L.p_rbase = take(4 * L.part_chunks);
   L.p_segL = take(4 * MS);
   L.leaf_chunks = leaf_max_chunks(L.KN, L.max_nodes);
-  L.l_lcount = take(4 * L.max_nodes);
   L.l_seg_first = take(4 * (L.max_nodes + 1));
   L.l_nseg = take(4);
   L.l_multi = take(4 * (L.leaf_chunks + 1));
@@ -216,25 +215,29 @@ k_record_level(int d, int D, int K, int E, int n_nodes, const int32_t* __restric
 
 using namespace wgbm;
 
-extern "C" int wgbm_set_hist_config(int32_t chunk_rows, int32_t waves, int32_t features_per_lane) {
+extern "C" int wgbm_set_hist_config(int32_t chunk_rows, int32_t waves, int32_t features_per_lane,
+                                    int32_t rows_in_flight) {
   WGBM_REQUIRE(chunk_rows >= 0 && (waves == 0 || waves == 1 || waves == 2 || waves == 4),
                "bad histogram configuration");
   WGBM_REQUIRE(features_per_lane == 0 || features_per_lane == 1 || features_per_lane == 2 ||
                    features_per_lane == 4 || features_per_lane == 8 || features_per_lane == 16,
                "features_per_lane must be 0 (auto), 1, 2, 4, 8 or 16");
+  WGBM_REQUIRE(rows_in_flight >= 0 && rows_in_flight <= 64, "rows_in_flight must be in [0, 64]");
   HistConfig c;
   c.chunk_rows = chunk_rows;
   c.waves = waves;
   c.fpl = features_per_lane;
+  c.depth = rows_in_flight;
   set_hist_override(c);
   return kOk;
 }
 
-extern "C" int wgbm_get_hist_config(int32_t n_feat, int32_t n_bins, int64_t n_cols, int32_t* out3) {
+extern "C" int wgbm_get_hist_config(int32_t n_feat, int32_t n_bins, int64_t n_cols, int32_t* out4) {
   const HistConfig c = fit_hist_config(n_feat, n_bins, n_cols);
-  out3[0] = c.chunk_rows;
-  out3[1] = c.waves;
-  out3[2] = c.fpl;
+  out4[0] = c.chunk_rows;
+  out4[1] = c.waves;
+  out4[2] = c.fpl;
+  out4[3] = c.depth;
   return kOk;
 }
 
@@ -367,7 +370,6 @@ extern "C" int wgbm_fit_rounds(const wgbm_fit_config* cfg, const wgbm_fit_io* io
       la.dec_feat = d < D ? I(L.dec_feat) : nullptr;
       la.node_start = I(cur.node_start);
       la.node_count = I(cur.node_count);
-      la.lcount = I(L.l_lcount);
       la.seg_first = I(L.l_seg_first);
       la.nseg_dev = I(L.l_nseg);
       la.multi = I(L.l_multi);
@@ -410,7 +412,7 @@ extern "C" int wgbm_fit_rounds(const wgbm_fit_config* cfg, const wgbm_fit_io* io
         p.gh_in = gh_d;
         p.perm_out = I(L.perm_pp[(d + 1) & 1]);
         p.gh_out = G2(L.gh_pp[(d + 1) & 1]);
-        p.pcount = I(L.p_pcount);
+        p.flag = reinterpret_cast<uint8_t*>(ws + L.p_flag);
         p.seg_first = I(L.p_seg_first);
         p.nseg_dev = I(L.p_nseg);
         p.multi = I(L.p_multi);

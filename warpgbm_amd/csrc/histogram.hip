@@ -22,20 +22,19 @@
 // positions, never on scheduling: plain per-node builds and batched level
 // builds of the same rows are bit-identical.
 //
-// LDS layout per wave: G plane then H plane, each [FPL][B][64] floats
-// (index (j*B + bin)*64 + lane).  Whatever bins the lanes of one ds_add
-// hit, lane l's word sits in bank l (64 banks), so accumulation is free of
-// bank conflicts; the read-out walks the words in lane order.
+// LDS layout per wave: [FPL][B][64] float2 (g, h), word (j*B + bin)*64 +
+// lane.  Lane l only ever touches its own words, so updates are plain
+// read-modify-write (ds_read_b64 / ds_write_b64, no atomics: LDS float
+// atomics serialise lanes), and whatever bins the 64 lanes hit, their 8-byte
+// words fall in distinct bank pairs.  A lane reads U rows' words at once and
+// forwards in registers when two of those rows share a bin, so the sum is
+// still the rows' sequential sum.  The read-out walks words in lane order.
 #include "common.h"
 #include "wgbm_internal.h"
 #include <utility>
 #include <vector>
 
 namespace wgbm {
-
-__device__ __forceinline__ void lds_add(float* p, float v) {
-  __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-}
 
 template <int FPL> struct CodeVec;
 template <> struct CodeVec<1>  { using T = uint8_t; };
@@ -60,13 +59,25 @@ __device__ __forceinline__ int code_byte(const typename CodeVec<FPL>::T& v, int 
 // A segment with count 0 still owns one (empty) chunk so that its slot is
 // written; count < 0 marks a padding segment that owns no chunk.
 // Segments with more than one chunk are listed in `multi` for the merge.
+// Row count of segment s for the three kinds of plan.
+__device__ __forceinline__ int plan_count(const PlanSrc& p, int s) {
+  if (p.mode == kPlanSegments) return p.seg_count[s];
+  if (p.mode == kPlanLeaves) {
+    const bool leaf = p.node_active[s] && (p.dec_feat == nullptr || p.dec_feat[s] < 0);
+    return leaf ? p.node_count[s] : -1;
+  }
+  const int j = s / p.n_eras;  // kPlanSplits: segments of split nodes
+  const bool split = p.node_active[j] && p.dec_feat[j] >= 0;
+  return split ? p.seg_count[s] : -1;
+}
+
 __global__ void __launch_bounds__(1024)
-k_hist_plan(const int32_t* __restrict__ seg_count, const int32_t* __restrict__ nseg_dev,
-            int32_t chunk_rows, int32_t* __restrict__ seg_first, int32_t* __restrict__ seg_pfirst,
-            int32_t* __restrict__ multi, int32_t* __restrict__ n_multi) {
+k_hist_plan(const PlanSrc src, int32_t chunk_rows, int32_t* __restrict__ seg_first,
+            int32_t* __restrict__ seg_pfirst, int32_t* __restrict__ multi, int32_t* __restrict__ n_multi) {
   __shared__ int32_t s_sum[3][1024 / kWave];
   __shared__ int32_t s_carry[3];
-  const int nseg = *nseg_dev;
+  const int nseg = src.nseg_dev ? *src.nseg_dev : src.nseg;
+  if (src.nseg_out && threadIdx.x == 0) *src.nseg_out = nseg;
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
   if (t < 3) s_carry[t] = 0;
   __syncthreads();
@@ -74,7 +85,7 @@ k_hist_plan(const int32_t* __restrict__ seg_count, const int32_t* __restrict__ n
     const int s = base + t;
     int v[3] = {0, 0, 0};  // chunks, is-multi, partial chunks
     if (s < nseg) {
-      const int c = seg_count[s];
+      const int c = plan_count(src, s);
       v[0] = c < 0 ? 0 : (c == 0 ? 1 : (c + chunk_rows - 1) / chunk_rows);
       v[1] = v[0] > 1;
       v[2] = v[1] ? v[0] : 0;
@@ -114,7 +125,7 @@ k_hist_plan(const int32_t* __restrict__ seg_count, const int32_t* __restrict__ n
 }
 
 // ---------------------------------------------------------------------------
-template <int FPL, bool GH_POS>
+template <int FPL, int L, bool GH_POS>
 __global__ void __launch_bounds__(256)
 k_hist_build(const HistArgs a) {
   extern __shared__ float smem[];
@@ -144,50 +155,35 @@ k_hist_build(const HistArgs a) {
   const int w1 = min(w0 + per, c1);
 
   const int B = a.n_bins;
-  const int P = a.lds_pitch;
-  const int plane = FPL * B * P;
-  float* lg = smem + wave * 2 * plane;
-  float* lh = lg + plane;
-  for (int q = lane; q < 2 * plane; q += kWave) lg[q] = 0.0f;
+  const int plane = FPL * B * kWave;  // float2 (g, h) words per wave
+  float2* lgh = reinterpret_cast<float2*>(smem) + wave * plane;
+  for (int q = lane; q < plane; q += kWave) lgh[q] = make_float2(0.0f, 0.0f);
 
   const int fl0 = slab * kWave * FPL + lane * FPL;  // local feature of byte 0
   const bool lane_on = fl0 < a.n_feat;
   // Wide loads need the slab's features to be the identity columns.
   bool ident = a.wide_ok;
   int cols[FPL];
-  bool jon_init[FPL];
+  bool jon[FPL];
 #pragma unroll
   for (int j = 0; j < FPL; ++j) {
     const int fl = fl0 + j;
     int c = (fl < a.n_feat) ? (a.feat_idx ? a.feat_idx[fl] : fl) : 0;
     // A column outside the matrix contributes nothing (and is never read).
-    jon_init[j] = fl < a.n_feat && (uint64_t)c < (uint64_t)a.row_stride;
-    if (!jon_init[j]) c = 0;
+    jon[j] = fl < a.n_feat && (uint64_t)c < (uint64_t)a.row_stride;
+    if (!jon[j]) c = 0;
     cols[j] = c;
     if (fl < a.n_feat && c != fl) ident = false;
   }
   ident = __all(ident || !lane_on);
-  bool jon[FPL];
-#pragma unroll
-  for (int j = 0; j < FPL; ++j) jon[j] = jon_init[j];
 
   using T = typename CodeVec<FPL>::T;
   const int8_t* __restrict__ codes = a.codes;
   const int64_t stride = a.row_stride;
-  // rows whose code loads are in flight at once per wave
-  constexpr int U = (FPL == 1) ? 32 : (FPL <= 4) ? 16 : (FPL == 8) ? 8 : 4;
+  // RMW groups of U rows (L rows per load block, a power of two <= 64).
+  constexpr int U = (FPL == 1) ? 8 : (FPL <= 4) ? 4 : 2;
+  static_assert(L % U == 0 && L <= 64 && (64 % L) == 0, "load block");
 
-  auto accum = [&](const T& v, float gg, float hh) {
-#pragma unroll
-    for (int j = 0; j < FPL; ++j) {
-      const int bin = code_byte<FPL>(v, j);
-      if (jon[j] && (unsigned)bin < (unsigned)B) {
-        const int o = (j * B + bin) * P + lane;
-        lds_add(lg + o, gg);
-        lds_add(lh + o, hh);
-      }
-    }
-  };
   auto load_codes = [&](int row) -> T {
     T v;
     if (!lane_on) {
@@ -207,7 +203,7 @@ k_hist_build(const HistArgs a) {
     const int pos = base + lane;
     const bool okp = pos < w1;
     int myrow = okp ? a.perm[pos] : 0;
-    // Rows outside the matrix are skipped: weight 0 and a safe address.
+    // Rows outside the matrix are skipped: never accumulated, safe address.
     const bool okr = okp && (uint64_t)myrow < (uint64_t)a.n_rows;
     if (!okr) myrow = 0;
     float mg = 0.0f, mh = 0.0f;
@@ -219,24 +215,59 @@ k_hist_build(const HistArgs a) {
         mg = a.g[myrow]; mh = a.h[myrow];
       }
     }
+    const uint64_t vmask = __ballot(okr);  // row r of this batch is valid
     const int cnt = min(kWave, w1 - base);
-    int u = 0;
-    for (; u + U <= cnt; u += U) {
-      T cv[U];
+    // Software pipeline: the next L rows' code loads are issued before the
+    // current L rows go through LDS, so every wave keeps loads in flight.
+    T cur[L], nxt[L];
 #pragma unroll
-      for (int q = 0; q < U; ++q) cv[q] = load_codes(__builtin_amdgcn_readlane(myrow, u + q));
+    for (int q = 0; q < L; ++q) cur[q] = load_codes(__builtin_amdgcn_readlane(myrow, q));
+    for (int u = 0; u < cnt; u += L) {
+      if (u + L < cnt) {
 #pragma unroll
-      for (int q = 0; q < U; ++q) {
-        const float gg = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, mg), u + q));
-        const float hh = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, mh), u + q));
-        accum(cv[q], gg, hh);
+        for (int q = 0; q < L; ++q) nxt[q] = load_codes(__builtin_amdgcn_readlane(myrow, (u + L + q) & 63));
       }
-    }
-    for (; u < cnt; ++u) {
-      const T v = load_codes(__builtin_amdgcn_readlane(myrow, u));
-      const float gg = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, mg), u));
-      const float hh = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, mh), u));
-      accum(v, gg, hh);
+#pragma unroll
+      for (int g0 = 0; g0 < L; g0 += U) {
+        // Lane-private read-modify-write of U rows: read the U target words,
+        // forward within the group when two rows hit the same bin, write
+        // back in row order.  The sum stays the sequential sum of the rows.
+        int addr[FPL][U];
+        bool ok[FPL][U];
+        float2 val[FPL][U];
+#pragma unroll
+        for (int k = 0; k < U; ++k) {
+          const bool vr = (vmask >> ((u + g0 + k) & 63)) & 1;
+#pragma unroll
+          for (int j = 0; j < FPL; ++j) {
+            const int bin = code_byte<FPL>(cur[g0 + k], j);
+            ok[j][k] = vr && jon[j] && (unsigned)bin < (unsigned)B;
+            addr[j][k] = (j * B + (ok[j][k] ? bin : 0)) * kWave + lane;
+            val[j][k] = lgh[addr[j][k]];
+          }
+        }
+#pragma unroll
+        for (int k = 0; k < U; ++k) {
+          const int r = (u + g0 + k) & 63;
+          const float gg = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, mg), r));
+          const float hh = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, mh), r));
+#pragma unroll
+          for (int j = 0; j < FPL; ++j) {
+            float2 b0 = val[j][k];
+#pragma unroll
+            for (int q = 0; q < k; ++q)
+              if (ok[j][q] && addr[j][q] == addr[j][k]) b0 = val[j][q];
+            val[j][k] = make_float2(b0.x + gg, b0.y + hh);
+          }
+        }
+#pragma unroll
+        for (int k = 0; k < U; ++k)
+#pragma unroll
+          for (int j = 0; j < FPL; ++j)
+            if (ok[j][k]) lgh[addr[j][k]] = val[j][k];
+      }
+#pragma unroll
+      for (int q = 0; q < L; ++q) cur[q] = nxt[q];
     }
   }
   __syncthreads();
@@ -246,8 +277,9 @@ k_hist_build(const HistArgs a) {
   const int f_base = slab * kWave * FPL;
   const bool direct = (nch == 1);
   const int slot = a.seg_slot[s];
-  // Thread t reads LDS word t (lane-fastest: conflict free) = feature
-  // lane*FPL + j, bin, with (j, bin) = divmod(t / 64, B).
+  const float2* all = reinterpret_cast<const float2*>(smem);
+  // Thread t reads word t of each wave's plane (lane-fastest: conflict free)
+  // = feature lane*FPL + j, bin, with (j, bin) = divmod(t / 64, B).
   for (int t = threadIdx.x; t < SL; t += blockDim.x) {
     const int r = t >> 6;
     const int ln = t & 63;
@@ -256,12 +288,13 @@ k_hist_build(const HistArgs a) {
     const int fl = ln * FPL + j;
     if (f_base + fl >= a.n_feat) continue;
     const int e = fl * B + bin;
-    const int o = r * P + ln;
-    float sg = smem[o], sh = smem[plane + o];
+    float2 v = all[t];
     for (int w = 1; w < W; ++w) {
-      sg += smem[w * 2 * plane + o];
-      sh += smem[w * 2 * plane + plane + o];
+      const float2 x = all[w * plane + t];
+      v.x += x.x;
+      v.y += x.y;
     }
+    const float sg = v.x, sh = v.y;
     if (direct) {
       const int64_t d = ((int64_t)slot * a.n_feat + f_base + fl) * B + bin;
       if (a.accumulate) { a.out_g[d] += sg; a.out_h[d] += sh; }
@@ -418,10 +451,10 @@ __global__ void k_set_scalar(int32_t* p, int v) {
   if (threadIdx.x == 0) *p = v;
 }
 
-void plan_launch(const int32_t* seg_count, const int32_t* nseg_dev, int chunk_rows,
-                 int32_t* seg_first, int32_t* multi, int32_t* n_multi, hipStream_t st) {
-  hipLaunchKernelGGL(k_hist_plan, dim3(1), dim3(1024), 0, st, seg_count, nseg_dev, chunk_rows, seg_first,
-                     (int32_t*)nullptr, multi, n_multi);
+void plan_launch(const PlanSrc& src, int chunk_rows, int32_t* seg_first, int32_t* multi, int32_t* n_multi,
+                 hipStream_t st) {
+  hipLaunchKernelGGL(k_hist_plan, dim3(1), dim3(1024), 0, st, src, chunk_rows, seg_first, (int32_t*)nullptr, multi,
+                     n_multi);
 }
 
 // ---------------------------------------------------------------------------
@@ -433,10 +466,11 @@ constexpr size_t kLdsMax = 160 * 1024;
 int lds_pitch(const HistConfig&, int) { return 64; }
 
 size_t lds_bytes(const HistConfig& c, int n_bins) {
-  return (size_t)c.waves * 2 * c.fpl * n_bins * lds_pitch(c, n_bins) * sizeof(float);
+  return (size_t)c.waves * c.fpl * n_bins * kWave * sizeof(float2);
 }
 
 static HistConfig g_override;   // all-zero: automatic
+int clamp_depth(int fpl, int depth);
 
 void set_hist_override(const HistConfig& c) { g_override = c; }
 
@@ -445,6 +479,7 @@ HistConfig pick_hist_config(int n_feat, int n_bins, int64_t row_stride) {
   if (g_override.chunk_rows > 0) c.chunk_rows = g_override.chunk_rows;
   if (g_override.waves > 0) c.waves = g_override.waves;
   if (g_override.fpl > 0 && row_stride % g_override.fpl == 0) c.fpl = g_override.fpl;
+  c.depth = clamp_depth(c.fpl, g_override.depth);
   return c;
 }
 
@@ -504,23 +539,23 @@ static void timer_clear() {
   g_timer.merge.clear();
 }
 
-template <int FPL>
+template <int FPL, int L>
 static int launch_fpl(const HistArgs& a, const HistConfig& c, int64_t max_chunks, int64_t max_multi,
                       int32_t* multi, int32_t* n_multi, hipStream_t st) {
   const size_t lds = lds_bytes(c, a.n_bins);
   const int64_t groups = ceil_div(max_chunks, 8) * 8 * a.nslab;
   if (groups > 0x7fffffff) { set_error("histogram grid too large"); return kTooBig; }
   if (lds > 64 * 1024) {  // opt in to more than 64 KB of dynamic LDS
-    WGBM_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_hist_build<FPL, true>),
+    WGBM_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_hist_build<FPL, L, true>),
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    WGBM_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_hist_build<FPL, false>),
+    WGBM_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_hist_build<FPL, L, false>),
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   }
   timer_mark(g_timer.build, true, st);
   if (a.gh)
-    hipLaunchKernelGGL((k_hist_build<FPL, true>), dim3((unsigned)groups), dim3(64 * c.waves), lds, st, a);
+    hipLaunchKernelGGL((k_hist_build<FPL, L, true>), dim3((unsigned)groups), dim3(64 * c.waves), lds, st, a);
   else
-    hipLaunchKernelGGL((k_hist_build<FPL, false>), dim3((unsigned)groups), dim3(64 * c.waves), lds, st, a);
+    hipLaunchKernelGGL((k_hist_build<FPL, L, false>), dim3((unsigned)groups), dim3(64 * c.waves), lds, st, a);
   WGBM_LAUNCHED("k_hist_build");
   timer_mark(g_timer.build, false, st);
   if (max_multi > 0) {
@@ -535,6 +570,26 @@ static int launch_fpl(const HistArgs& a, const HistConfig& c, int64_t max_chunks
   return kOk;
 }
 
+// Load depth (rows in flight per wave) allowed per features-per-lane.
+int clamp_depth(int fpl, int depth) {
+  const int hi = fpl == 1 ? 64 : fpl == 2 ? 32 : fpl == 4 ? 16 : fpl == 8 ? 8 : 4;
+  const int lo = hi / 4 < 2 ? 2 : hi / 4;
+  const int def = fpl <= 2 ? 32 : fpl == 4 ? 16 : fpl == 8 ? 8 : 4;
+  int d = depth > 0 ? depth : def;
+  if (d > hi) d = hi;
+  if (d < lo) d = lo;
+  int p = 1;
+  while (p * 2 <= d) p *= 2;
+  return p;
+}
+
+#define WGBM_DEPTHS(FPL, D0, D1, D2)                                                   \
+  switch (c.depth) {                                                                  \
+    case D0: return launch_fpl<FPL, D0>(a, c, max_chunks, max_multi, multi, n_multi, st); \
+    case D1: return launch_fpl<FPL, D1>(a, c, max_chunks, max_multi, multi, n_multi, st); \
+    default: return launch_fpl<FPL, D2>(a, c, max_chunks, max_multi, multi, n_multi, st); \
+  }
+
 int hist_launch(HistArgs a, const HistConfig& c, int64_t rows_max, int64_t nseg_max,
                 int32_t* multi, int32_t* n_multi, hipStream_t st) {
   a.chunk_rows = c.chunk_rows;
@@ -545,18 +600,24 @@ int hist_launch(HistArgs a, const HistConfig& c, int64_t rows_max, int64_t nseg_
     set_error("histogram LDS budget exceeded (%d bins, %d waves, %d features per lane)", a.n_bins, c.waves, c.fpl);
     return kTooBig;
   }
-  hipLaunchKernelGGL(k_hist_plan, dim3(1), dim3(1024), 0, st, a.seg_count, a.nseg_dev, c.chunk_rows,
-                     a.seg_first, a.seg_pfirst, multi, n_multi);
+  PlanSrc src{};
+  src.mode = kPlanSegments;
+  src.seg_count = a.seg_count;
+  src.nseg_dev = a.nseg_dev;
+  hipLaunchKernelGGL(k_hist_plan, dim3(1), dim3(1024), 0, st, src, c.chunk_rows, a.seg_first, a.seg_pfirst, multi,
+                     n_multi);
   WGBM_LAUNCHED("k_hist_plan");
   const int64_t max_chunks = hist_max_chunks(rows_max, nseg_max, c.chunk_rows);
   const int64_t max_multi = rows_max / (c.chunk_rows + 1) + 1;
   if (a.n_feat == 0) return kOk;
   switch (c.fpl) {
-    case 1: return launch_fpl<1>(a, c, max_chunks, max_multi, multi, n_multi, st);
-    case 2: return launch_fpl<2>(a, c, max_chunks, max_multi, multi, n_multi, st);
-    case 4: return launch_fpl<4>(a, c, max_chunks, max_multi, multi, n_multi, st);
-    case 8: return launch_fpl<8>(a, c, max_chunks, max_multi, multi, n_multi, st);
-    case 16: return launch_fpl<16>(a, c, max_chunks, max_multi, multi, n_multi, st);
+    case 1: WGBM_DEPTHS(1, 16, 32, 64)
+    case 2: WGBM_DEPTHS(2, 8, 16, 32)
+    case 4: WGBM_DEPTHS(4, 4, 8, 16)
+    case 8: WGBM_DEPTHS(8, 2, 4, 8)
+    case 16:
+      return c.depth <= 2 ? launch_fpl<16, 2>(a, c, max_chunks, max_multi, multi, n_multi, st)
+                          : launch_fpl<16, 4>(a, c, max_chunks, max_multi, multi, n_multi, st);
   }
   set_error("unsupported features-per-lane %d", c.fpl);
   return kBadArg;

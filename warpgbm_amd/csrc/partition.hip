@@ -14,14 +14,8 @@
 namespace wgbm {
 
 // ----------------------------------------------------------------- partition
-__global__ void k_part_prep(PartArgs a) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  const int nseg = a.n_nodes * a.n_eras;
-  if (s == 0) *a.nseg_dev = nseg;
-  if (s >= nseg) return;
-  const int j = s / a.n_eras;
-  const bool split = a.node_active[j] && a.dec_feat[j] >= 0;
-  a.pcount[s] = split ? a.seg_count[s] : -1;
+__device__ __forceinline__ bool is_split(const PartArgs& a, int j) {
+  return a.node_active[j] && a.dec_feat[j] >= 0;
 }
 
 __device__ __forceinline__ int split_col(const PartArgs& a, int j) {
@@ -47,7 +41,11 @@ k_part_count(PartArgs a) {
   const int c0 = a.seg_start[s] + k * a.chunk_rows;
   const int c1 = min(c0 + a.chunk_rows, a.seg_start[s] + max(a.seg_count[s], 0));
   int cnt = 0;
-  for (int p = c0 + threadIdx.x; p < c1; p += blockDim.x) cnt += goes_left(a, a.perm_in[p], col, bin);
+  for (int p = c0 + threadIdx.x; p < c1; p += blockDim.x) {
+    const bool l = goes_left(a, a.perm_in[p], col, bin);
+    a.flag[p] = (uint8_t)l;  // the scatter reads this instead of the code
+    cnt += l;
+  }
   for (int o = 32; o >= 1; o >>= 1) cnt += __shfl_xor(cnt, o);
   __shared__ int s_c[4];
   if ((threadIdx.x & 63) == 0) s_c[threadIdx.x >> 6] = cnt;
@@ -63,7 +61,7 @@ k_part_scan(PartArgs a) {
   const int nseg = a.n_nodes * E;
   // A: left totals per segment; chunk-local left/right offsets.
   for (int s = threadIdx.x; s < nseg; s += blockDim.x) {
-    if (a.pcount[s] < 0) { a.segL[s] = 0; continue; }
+    if (!is_split(a, s / E)) { a.segL[s] = 0; continue; }
     const int first = a.seg_first[s], last = a.seg_first[s + 1];
     int run = 0;
     for (int c = first; c < last; ++c) {
@@ -108,7 +106,7 @@ k_part_scan(PartArgs a) {
   __syncthreads();
   // C: absolute destination bases per chunk.
   for (int s = threadIdx.x; s < nseg; s += blockDim.x) {
-    if (a.pcount[s] < 0) continue;
+    if (!is_split(a, s / E)) continue;
     const int j = s / E, e = s - j * E;
     const int ls = a.c_seg_start[(2 * j) * E + e];
     const int rs = a.c_seg_start[(2 * j + 1) * E + e];
@@ -152,9 +150,6 @@ k_part_scatter(PartArgs a) {
   if (chunk >= a.seg_first[nseg]) return;
   const int s = upper_index(a.seg_first, nseg, chunk);
   const int k = chunk - a.seg_first[s];
-  const int j = s / a.n_eras;
-  const int col = split_col(a, j);
-  const int bin = a.dec_bin[j];
   const int c0 = a.seg_start[s] + k * a.chunk_rows;
   const int c1 = min(c0 + a.chunk_rows, a.seg_start[s] + max(a.seg_count[s], 0));
   __shared__ int s_l[4];
@@ -169,7 +164,7 @@ k_part_scatter(PartArgs a) {
     if (ok) {
       row = a.perm_in[p];
       if (a.gh_in) v = a.gh_in[p];
-      left = goes_left(a, row, col, bin);
+      left = a.flag[p] != 0;
     }
     const uint64_t lm = __ballot(ok && left);
     const uint64_t rm = __ballot(ok && !left);
@@ -194,9 +189,15 @@ k_part_scatter(PartArgs a) {
 int partition_launch(const PartArgs& a, int64_t rows_max, hipStream_t st) {
   const int nseg = a.n_nodes * a.n_eras;
   if (nseg == 0) return kOk;
-  hipLaunchKernelGGL(k_part_prep, dim3((unsigned)ceil_div(nseg, 256)), dim3(256), 0, st, a);
-  WGBM_LAUNCHED("k_part_prep");
-  plan_launch(a.pcount, a.nseg_dev, a.chunk_rows, a.seg_first, a.multi, a.n_multi, st);
+  PlanSrc src{};
+  src.mode = kPlanSplits;
+  src.nseg = nseg;
+  src.nseg_out = a.nseg_dev;
+  src.seg_count = a.seg_count;
+  src.node_active = a.node_active;
+  src.dec_feat = a.dec_feat;
+  src.n_eras = a.n_eras;
+  plan_launch(src, a.chunk_rows, a.seg_first, a.multi, a.n_multi, st);
   WGBM_LAUNCHED("k_hist_plan(partition)");
   const int64_t max_chunks = ceil_div(rows_max, a.chunk_rows) + nseg;
   hipLaunchKernelGGL(k_part_count, dim3((unsigned)max_chunks), dim3(256), 0, st, a);
@@ -214,14 +215,6 @@ int partition_launch(const PartArgs& a, int64_t rows_max, hipStream_t st) {
 // in float64, the workgroup combines thread sums by a fixed tree, and the
 // node adds its chunk sums in chunk order.
 constexpr int kLeafChunk = 4096;
-
-__global__ void k_leaf_prep(LeafArgs a) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j == 0) *a.nseg_dev = a.n_nodes;
-  if (j >= a.n_nodes) return;
-  const bool leaf = a.node_active[j] && (a.dec_feat == nullptr || a.dec_feat[j] < 0);
-  a.lcount[j] = leaf ? a.node_count[j] : -1;
-}
 
 __device__ __forceinline__ void leaf_gh(const LeafArgs& a, int p, double& g, double& h) {
   if (a.gh) {
@@ -261,21 +254,9 @@ k_leaf_partial(LeafArgs a) {
   if (threadIdx.x == 0) { a.part[2 * (int64_t)chunk] = s_g[0]; a.part[2 * (int64_t)chunk + 1] = s_h[0]; }
 }
 
-__global__ void k_leaf_final(LeafArgs a) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= a.n_nodes) return;
-  if (a.lcount[j] < 0) return;
-  double sg = 0.0, sh = 0.0;
-  for (int c = a.seg_first[j]; c < a.seg_first[j + 1]; ++c) {
-    sg += a.part[2 * (int64_t)c];
-    sh += a.part[2 * (int64_t)c + 1];
-  }
-  const float v = (float)(sg / (sh + a.lam));
-  a.leaf_val[j] = v;
-  if (a.node_sum) { a.node_sum[2 * j] = sg; a.node_sum[2 * j + 1] = sh; }
-}
-
-// pred[row*K + class] += f32(lr) * leaf, class = node / nodes_per_class.
+// Per chunk: the node's total from its chunk sums (chunk order, float64),
+// leaf = f32(sum g / (sum h + lambda)) written once per node (chunk 0), and
+// pred[row*K + class] += f32(lr) * leaf for the chunk's rows.
 __global__ void __launch_bounds__(256)
 k_leaf_apply(LeafArgs a) {
   const int chunk = blockIdx.x;
@@ -283,9 +264,25 @@ k_leaf_apply(LeafArgs a) {
   if (chunk >= a.seg_first[nseg]) return;
   const int j = upper_index(a.seg_first, nseg, chunk);
   const int k = chunk - a.seg_first[j];
+  __shared__ float s_v;
+  if (threadIdx.x == 0) {
+    double sg = 0.0, sh = 0.0;
+    for (int c = a.seg_first[j]; c < a.seg_first[j + 1]; ++c) {
+      sg += a.part[2 * (int64_t)c];
+      sh += a.part[2 * (int64_t)c + 1];
+    }
+    const float v = (float)(sg / (sh + a.lam));
+    s_v = v;
+    if (k == 0) {
+      a.leaf_val[j] = v;
+      if (a.node_sum) { a.node_sum[2 * j] = sg; a.node_sum[2 * j + 1] = sh; }
+    }
+  }
+  __syncthreads();
+  if (a.pred == nullptr) return;
   const int c0 = a.node_start[j] + k * kLeafChunk;
   const int c1 = min(c0 + kLeafChunk, a.node_start[j] + a.node_count[j]);
-  const float step = a.lr * a.leaf_val[j];
+  const float step = a.lr * s_v;
   const int cls = j / a.nodes_per_class;
   for (int p = c0 + threadIdx.x; p < c1; p += 256) {
     const int r = a.perm[p];
@@ -297,19 +294,20 @@ k_leaf_apply(LeafArgs a) {
 
 int leaf_launch(const LeafArgs& a, int64_t rows_max, hipStream_t st) {
   if (a.n_nodes == 0) return kOk;
-  hipLaunchKernelGGL(k_leaf_prep, dim3((unsigned)ceil_div(a.n_nodes, 256)), dim3(256), 0, st, a);
-  WGBM_LAUNCHED("k_leaf_prep");
-  plan_launch(a.lcount, a.nseg_dev, kLeafChunk, a.seg_first, a.multi, a.n_multi, st);
+  PlanSrc src{};
+  src.mode = kPlanLeaves;
+  src.nseg = a.n_nodes;
+  src.nseg_out = a.nseg_dev;
+  src.node_active = a.node_active;
+  src.dec_feat = a.dec_feat;
+  src.node_count = a.node_count;
+  plan_launch(src, kLeafChunk, a.seg_first, a.multi, a.n_multi, st);
   WGBM_LAUNCHED("k_hist_plan(leaf)");
   const int64_t max_chunks = ceil_div(rows_max, kLeafChunk) + a.n_nodes;
   hipLaunchKernelGGL(k_leaf_partial, dim3((unsigned)max_chunks), dim3(256), 0, st, a);
   WGBM_LAUNCHED("k_leaf_partial");
-  hipLaunchKernelGGL(k_leaf_final, dim3((unsigned)ceil_div(a.n_nodes, 256)), dim3(256), 0, st, a);
-  WGBM_LAUNCHED("k_leaf_final");
-  if (a.pred) {
-    hipLaunchKernelGGL(k_leaf_apply, dim3((unsigned)max_chunks), dim3(256), 0, st, a);
-    WGBM_LAUNCHED("k_leaf_apply");
-  }
+  hipLaunchKernelGGL(k_leaf_apply, dim3((unsigned)max_chunks), dim3(256), 0, st, a);
+  WGBM_LAUNCHED("k_leaf_apply");
   return kOk;
 }
 
@@ -410,7 +408,7 @@ using namespace wgbm;
 extern "C" size_t wgbm_partition_workspace_bytes(int64_t n_samples) {
   if (n_samples < 0) return 0;
   const int64_t mc = ceil_div(n_samples, kPartChunk) + 1;
-  return (size_t)(4 * (16 + 3 * mc) + 64 * 4) + 1024;
+  return (size_t)(4 * (16 + 3 * mc) + 64 * 4) + (size_t)n_samples + 1024;
 }
 
 // Stable split of one node's row list (plain path): rows with
@@ -443,7 +441,6 @@ extern "C" int wgbm_partition_rows(const int8_t* bin_indices, int64_t n_total_ro
   a.seg_count = sc + 4;      // n
   a.node_start = sc + 3;
   a.node_count = sc + 4;
-  a.pcount = sc + 5;
   a.seg_first = sc + 6;      // 2 ints
   a.nseg_dev = sc + 8;
   a.n_multi = sc + 9;
@@ -453,6 +450,7 @@ extern "C" int wgbm_partition_rows(const int8_t* bin_indices, int64_t n_total_ro
   a.b_seg_start = sc + 22; a.b_seg_count = sc + 23; a.b_seg_slot = sc + 24;
   a.n_build = sc + 25; a.n_split = sc + 26; a.sib = sc + 27;  // 3 ints
   a.multi = multi;
+  a.flag = reinterpret_cast<uint8_t*>(multi + mc);
   a.lbase = lbase; a.rbase = rbase;
   a.perm_in = rows; a.gh_in = nullptr;
   a.perm_out = out_rows; a.gh_out = nullptr;
@@ -490,7 +488,6 @@ extern "C" int wgbm_leaf_value(const float* grads, const float* hess, const int3
   a.dec_feat = nullptr;
   a.node_start = sc + 1;
   a.node_count = sc + 2;
-  a.lcount = sc + 3;
   a.seg_first = sc + 4;  // 2
   a.nseg_dev = sc + 6;
   a.n_multi = sc + 7;

@@ -11,6 +11,7 @@ struct HistConfig {
   int chunk_rows = 0;
   int waves = 0;
   int fpl = 0;
+  int depth = 0;  // rows whose code loads are in flight per wave (load block)
 };
 
 struct HistArgs {
@@ -73,8 +74,24 @@ struct LevelScanArgs {
 };
 int level_scan_launch(const LevelScanArgs& a, hipStream_t st);
 
-// Chunk plan shared by histogram, partition and leaf kernels.
-void plan_launch(const int32_t* seg_count, const int32_t* nseg_dev, int chunk_rows,
-                 int32_t* seg_first, int32_t* multi, int32_t* n_multi, hipStream_t st);
+// Chunk plan shared by histogram, partition and leaf kernels: the row
+// count of segment s is seg_count[s] (segments), node_count[s] for leaf
+// nodes else -1 (leaves), or seg_count[s] for segments of split nodes else
+// -1 (splits).  nseg comes from nseg_dev or, when that is null, nseg; it is
+// copied to nseg_out for the kernels that follow.
+enum PlanMode : int { kPlanSegments = 0, kPlanLeaves = 1, kPlanSplits = 2 };
+struct PlanSrc {
+  int32_t mode;
+  int32_t nseg;
+  const int32_t* nseg_dev;
+  int32_t* nseg_out;
+  const int32_t* seg_count;
+  const int32_t* node_active;
+  const int32_t* dec_feat;
+  const int32_t* node_count;
+  int32_t n_eras;
+};
+void plan_launch(const PlanSrc& src, int chunk_rows, int32_t* seg_first, int32_t* multi, int32_t* n_multi,
+                 hipStream_t st);
 
 }  // namespace wgbm

@@ -25,7 +25,7 @@ struct PartArgs {
   int32_t* perm_out;            // positions of the next level
   float2* gh_out;               // or nullptr
   // scratch
-  int32_t* pcount;              // [n_nodes*E]
+  uint8_t* flag;                // [positions] goes-left flag per position
   int32_t* seg_first;           // [n_nodes*E+1]
   int32_t* nseg_dev;            // scalar
   int32_t* multi;               // plan scratch
@@ -55,7 +55,6 @@ struct LeafArgs {
   const int32_t* dec_feat;      // nullptr: every active node is a leaf
   const int32_t* node_start;
   const int32_t* node_count;
-  int32_t* lcount;              // scratch [n_nodes]
   int32_t* seg_first;           // scratch [n_nodes+1]
   int32_t* nseg_dev;
   int32_t* multi;
