@@ -20,6 +20,7 @@
 #include "common.h"
 #include "wgbm_internal.h"
 #include "wgbm_level.h"
+#include "wgbm_stats.h"
 #include "../../include/warpgbm_amd.h"
 
 namespace wgbm {
@@ -29,7 +30,7 @@ struct FitLayout {
   // offsets
   size_t perm_root, gh_root, perm_pp[2], gh_pp[2];
   size_t hg[2], hh[2];
-  struct Tables { size_t seg_start, seg_count, node_active, node_start, node_count; } tab[2];
+  struct Tables { size_t seg_start, seg_count, node_active, node_start, node_count, stats; } tab[2];
   size_t dec_feat, dec_bin, era_gain, cand_agree, cand_crit, cand_idx;
   size_t b_seg_start, b_seg_count, b_seg_slot, n_build, sib, n_split;
   size_t h_seg_first, h_seg_pfirst, h_nseg, h_multi, h_n_multi, h_part;
@@ -70,6 +71,7 @@ static FitLayout fit_layout(const wgbm_fit_config& c) {
     L.tab[i].node_active = take(4 * L.max_nodes);
     L.tab[i].node_start = take(4 * L.max_nodes);
     L.tab[i].node_count = take(4 * L.max_nodes);
+    L.tab[i].stats = take(4 * kStatWords * L.max_nodes);
   }
   const int64_t wgpn = ceil_div(c.n_feat, 256) > 0 ? ceil_div(c.n_feat, 256) : 1;
   L.dec_feat = take(4 * L.max_nodes);
@@ -89,8 +91,8 @@ static FitLayout fit_layout(const wgbm_fit_config& c) {
   L.h_nseg = take(4);
   L.h_multi = take(4 * (L.KN / (L.hc.chunk_rows + 1) + 2));
   L.h_n_multi = take(4);
-  L.hist_part_floats = (int64_t)hist_partial_floats(L.hc, c.n_feat, c.n_bins, L.KN);
-  L.h_part = take(2 * 4 * L.hist_part_floats);
+  L.hist_part_floats = (int64_t)hist_partial_words(L.hc, c.n_feat, c.n_bins, L.KN);
+  L.h_part = take(sizeof(longlong2) * L.hist_part_floats);
   L.part_chunks = ceil_div(L.KN, kPartChunk) + MS;
   L.p_flag = take(L.KN);
   L.p_seg_first = take(4 * (MS + 1));
@@ -120,8 +122,9 @@ static FitLayout fit_layout(const wgbm_fit_config& c) {
 __global__ void k_root_tables(const int32_t* __restrict__ era_count, int E, int K, int N,
                               int32_t* era_start, int32_t* seg_start, int32_t* seg_count, int32_t* seg_slot,
                               int32_t* nseg, int32_t* t_seg_start, int32_t* t_seg_count, int32_t* t_active,
-                              int32_t* t_node_start, int32_t* t_node_count) {
+                              int32_t* t_node_start, int32_t* t_node_count, int32_t* t_stats) {
   if (threadIdx.x != 0) return;
+  for (int c = 0; c < K; ++c) stats_init(t_stats + c * kStatWords);
   int run = 0;
   for (int e = 0; e < E; ++e) { era_start[e] = run; run += era_count[e]; }
   for (int c = 0; c < K; ++c) {
@@ -150,19 +153,26 @@ k_root_perm(const int32_t* __restrict__ order, int64_t N, int K, int32_t* __rest
 
 __global__ void __launch_bounds__(256)
 k_grad_regression_pos(const float* __restrict__ y, const float* __restrict__ pred, const float* __restrict__ w,
-                      int64_t n, const int32_t* __restrict__ perm0, float2* __restrict__ gh) {
+                      int64_t n, const int32_t* __restrict__ perm0, float2* __restrict__ gh,
+                      int32_t* __restrict__ stats) {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n) return;
-  const int64_t r = perm0[p];
-  const float wr = w[r];
-  gh[p] = make_float2((y[r] - pred[r]) * wr, wr);
+  float g = 0.0f, h = 0.0f;
+  if (p < n) {
+    const int64_t r = perm0[p];
+    h = w[r];
+    g = (y[r] - pred[r]) * h;
+    gh[p] = make_float2(g, h);
+  }
+  stats_add_wave(stats, p < n, g, h);
 }
 
 __global__ void __launch_bounds__(256)
 k_grad_softmax_pos(const float* __restrict__ F, const int32_t* __restrict__ y, const float* __restrict__ w,
-                   int64_t n, int K, const int32_t* __restrict__ perm0, float2* __restrict__ gh) {
-  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n) return;
+                   int64_t n, int K, const int32_t* __restrict__ perm0, float2* __restrict__ gh,
+                   int32_t* __restrict__ stats) {
+  const int64_t p0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool ok = p0 < n;
+  const int64_t p = ok ? p0 : n - 1;
   const int64_t r = perm0[p];
   const float* f = F + r * K;
   float mx = f[0];
@@ -173,7 +183,10 @@ k_grad_softmax_pos(const float* __restrict__ F, const int32_t* __restrict__ y, c
   const int yr = y[r];
   for (int k = 0; k < K; ++k) {
     const float pk = expf(f[k] - mx) / den;
-    gh[(int64_t)k * n + p] = make_float2(-(pk - (yr == k ? 1.0f : 0.0f)) * wr, fmaxf(pk * (1.0f - pk), 1e-6f) * wr);
+    const float gk = -(pk - (yr == k ? 1.0f : 0.0f)) * wr;
+    const float hk = fmaxf(pk * (1.0f - pk), 1e-6f) * wr;
+    if (ok) gh[(int64_t)k * n + p] = make_float2(gk, hk);
+    stats_add_wave(stats + k * kStatWords, ok, gk, hk);
   }
 }
 
@@ -288,15 +301,15 @@ extern "C" int wgbm_fit_rounds(const wgbm_fit_config* cfg, const wgbm_fit_io* io
     hipLaunchKernelGGL(k_root_tables, dim3(1), dim3(64), 0, st, io->era_count, E, K, (int)N, I(L.era_start),
                        I(L.root_seg_start), I(L.root_seg_count), I(L.root_seg_slot), I(L.root_nseg),
                        I(L.tab[0].seg_start), I(L.tab[0].seg_count), I(L.tab[0].node_active),
-                       I(L.tab[0].node_start), I(L.tab[0].node_count));
+                       I(L.tab[0].node_start), I(L.tab[0].node_count), I(L.tab[0].stats));
     WGBM_LAUNCHED("k_root_tables");
     if (c.objective == 0) {
       hipLaunchKernelGGL(k_grad_regression_pos, dim3((unsigned)ceil_div(N, 256)), dim3(256), 0, st, io->y,
-                         io->scores, io->weight, N, I(L.perm_root), G2(L.gh_root));
+                         io->scores, io->weight, N, I(L.perm_root), G2(L.gh_root), I(L.tab[0].stats));
       WGBM_LAUNCHED("k_grad_regression_pos");
     } else {
       hipLaunchKernelGGL(k_grad_softmax_pos, dim3((unsigned)ceil_div(N, 256)), dim3(256), 0, st, io->scores,
-                         io->labels, io->weight, N, K, I(L.perm_root), G2(L.gh_root));
+                         io->labels, io->weight, N, K, I(L.perm_root), G2(L.gh_root), I(L.tab[0].stats));
       WGBM_LAUNCHED("k_grad_softmax_pos");
     }
     const int64_t out_base = (int64_t)r * K * NH;
@@ -322,8 +335,9 @@ extern "C" int wgbm_fit_rounds(const wgbm_fit_config* cfg, const wgbm_fit_io* io
         a.seg_pfirst = I(L.h_seg_pfirst);
         a.out_g = hg;
         a.out_h = hh;
-        a.part_g = Fp(L.h_part);
-        a.part_h = Fp(L.h_part) + L.hist_part_floats;
+        a.part = reinterpret_cast<longlong2*>(ws + L.h_part);
+        a.slots_per_node = E;
+        a.stats = I(cur.stats);
         a.accumulate = 0;
         if (d == 0) {
           a.seg_start = I(L.root_seg_start);
@@ -425,6 +439,7 @@ extern "C" int wgbm_fit_rounds(const wgbm_fit_config* cfg, const wgbm_fit_io* io
         p.c_active = I(nxt.node_active);
         p.c_node_start = I(nxt.node_start);
         p.c_node_count = I(nxt.node_count);
+        p.c_stats = I(nxt.stats);
         p.b_seg_start = I(L.b_seg_start);
         p.b_seg_count = I(L.b_seg_count);
         p.b_seg_slot = I(L.b_seg_slot);

@@ -31,6 +31,7 @@
 // still the rows' sequential sum.  The read-out walks words in lane order.
 #include "common.h"
 #include "wgbm_internal.h"
+#include "wgbm_stats.h"
 #include <utility>
 #include <vector>
 
@@ -125,10 +126,31 @@ k_hist_plan(const PlanSrc src, int32_t chunk_rows, int32_t* __restrict__ seg_fir
 }
 
 // ---------------------------------------------------------------------------
+// Shifts of a segment's node and the partial word of one bin.
+struct NodeShift {
+  int sg, sh;
+  bool count_mode;    // every row of the node has the same hessian
+  long long qh_const; // its fixed-point value in count mode
+};
+
+__device__ __forceinline__ NodeShift node_shift(const int32_t* st) {
+  NodeShift n;
+  const float gmax = __builtin_bit_cast(float, st[0]);
+  const float hmin = ord_float(st[1]);
+  const float hmax = ord_float(st[2]);
+  const float habs = __builtin_bit_cast(float, st[3]);
+  n.sg = fixed_shift(gmax, kGBits);
+  n.sh = fixed_shift(habs, kHBits);
+  n.count_mode = st[1] == st[2];
+  n.qh_const = n.count_mode ? to_fixed(hmin, n.sh) : 0;
+  (void)hmax;
+  return n;
+}
+
 template <int FPL, int L, bool GH_POS>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(512)
 k_hist_build(const HistArgs a) {
-  extern __shared__ float smem[];
+  extern __shared__ unsigned long long s64[];
   const int lane = lane_id();
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int W = blockDim.x >> 6;
@@ -153,11 +175,17 @@ k_hist_build(const HistArgs a) {
   const int per = (len + W - 1) / W;
   const int w0 = min(c0 + wave * per, c1);
   const int w1 = min(w0 + per, c1);
+  const int slot = a.seg_slot[s];
+  const NodeShift ns = node_shift(a.stats + (int64_t)(slot / a.slots_per_node) * kStatWords);
 
+  // One shared histogram per workgroup: T plane (count << 44 | sum qg) and
+  // H plane (sum qh, unused in count mode), each [FPL][B][64] words.
   const int B = a.n_bins;
-  const int plane = FPL * B * kWave;  // float2 (g, h) words per wave
-  float2* lgh = reinterpret_cast<float2*>(smem) + wave * plane;
-  for (int q = lane; q < plane; q += kWave) lgh[q] = make_float2(0.0f, 0.0f);
+  const int plane = FPL * B * kWave;
+  unsigned long long* T = s64;
+  unsigned long long* Hq = s64 + plane;
+  for (int q = threadIdx.x; q < 2 * plane; q += blockDim.x) s64[q] = 0ull;
+  __syncthreads();
 
   const int fl0 = slab * kWave * FPL + lane * FPL;  // local feature of byte 0
   const bool lane_on = fl0 < a.n_feat;
@@ -177,49 +205,60 @@ k_hist_build(const HistArgs a) {
   }
   ident = __all(ident || !lane_on);
 
-  using T = typename CodeVec<FPL>::T;
+  using TV = typename CodeVec<FPL>::T;
   const int8_t* __restrict__ codes = a.codes;
   const int64_t stride = a.row_stride;
-  // RMW groups of U rows (L rows per load block, a power of two <= 64).
-  constexpr int U = (FPL == 1) ? 8 : (FPL <= 4) ? 4 : 2;
-  static_assert(L % U == 0 && L <= 64 && (64 % L) == 0, "load block");
+  static_assert(L <= 64 && (64 % L) == 0, "load block");
 
-  auto load_codes = [&](int row) -> T {
-    T v;
+  auto load_codes = [&](int row) -> TV {
+    TV v;
     if (!lane_on) {
-      v = T{};
+      v = TV{};
     } else if (ident) {
-      v = *reinterpret_cast<const T*>(codes + (int64_t)row * stride + fl0);
+      v = *reinterpret_cast<const TV*>(codes + (int64_t)row * stride + fl0);
     } else {
       uint8_t tmp[FPL];
 #pragma unroll
       for (int j = 0; j < FPL; ++j) tmp[j] = (uint8_t)codes[(int64_t)row * stride + cols[j]];
-      __builtin_memcpy(&v, tmp, sizeof(T));
+      __builtin_memcpy(&v, tmp, sizeof(TV));
     }
     return v;
+  };
+  auto add_row = [&](const TV& v, unsigned long long qp, unsigned long long qh) {
+#pragma unroll
+    for (int j = 0; j < FPL; ++j) {
+      const int bin = code_byte<FPL>(v, j);
+      if (jon[j] && (unsigned)bin < (unsigned)B) {
+        const int o = (j * B + bin) * kWave + lane;
+        __hip_atomic_fetch_add(T + o, qp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (!ns.count_mode) __hip_atomic_fetch_add(Hq + o, qh, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      }
+    }
   };
 
   for (int base = w0; base < w1; base += kWave) {
     const int pos = base + lane;
     const bool okp = pos < w1;
     int myrow = okp ? a.perm[pos] : 0;
-    // Rows outside the matrix are skipped: never accumulated, safe address.
+    // Rows outside the matrix are skipped (they add zero words).
     const bool okr = okp && (uint64_t)myrow < (uint64_t)a.n_rows;
     if (!okr) myrow = 0;
-    float mg = 0.0f, mh = 0.0f;
+    float g = 0.0f, h = 0.0f;
     if (okr) {
       if constexpr (GH_POS) {
         const float2 v = a.gh[pos];
-        mg = v.x; mh = v.y;
+        g = v.x; h = v.y;
       } else {
-        mg = a.g[myrow]; mh = a.h[myrow];
+        g = a.g[myrow]; h = a.h[myrow];
       }
     }
-    const uint64_t vmask = __ballot(okr);  // row r of this batch is valid
+    const unsigned long long myqp =
+        okr ? (1ull << kCountShift) + (unsigned long long)to_fixed(g, ns.sg) : 0ull;
+    const unsigned long long myqh = okr ? (unsigned long long)to_fixed(h, ns.sh) : 0ull;
     const int cnt = min(kWave, w1 - base);
     // Software pipeline: the next L rows' code loads are issued before the
-    // current L rows go through LDS, so every wave keeps loads in flight.
-    T cur[L], nxt[L];
+    // current L rows are added, so every wave keeps loads in flight.
+    TV cur[L], nxt[L];
 #pragma unroll
     for (int q = 0; q < L; ++q) cur[q] = load_codes(__builtin_amdgcn_readlane(myrow, q));
     for (int u = 0; u < cnt; u += L) {
@@ -228,43 +267,16 @@ k_hist_build(const HistArgs a) {
         for (int q = 0; q < L; ++q) nxt[q] = load_codes(__builtin_amdgcn_readlane(myrow, (u + L + q) & 63));
       }
 #pragma unroll
-      for (int g0 = 0; g0 < L; g0 += U) {
-        // Lane-private read-modify-write of U rows: read the U target words,
-        // forward within the group when two rows hit the same bin, write
-        // back in row order.  The sum stays the sequential sum of the rows.
-        int addr[FPL][U];
-        bool ok[FPL][U];
-        float2 val[FPL][U];
-#pragma unroll
-        for (int k = 0; k < U; ++k) {
-          const bool vr = (vmask >> ((u + g0 + k) & 63)) & 1;
-#pragma unroll
-          for (int j = 0; j < FPL; ++j) {
-            const int bin = code_byte<FPL>(cur[g0 + k], j);
-            ok[j][k] = vr && jon[j] && (unsigned)bin < (unsigned)B;
-            addr[j][k] = (j * B + (ok[j][k] ? bin : 0)) * kWave + lane;
-            val[j][k] = lgh[addr[j][k]];
-          }
-        }
-#pragma unroll
-        for (int k = 0; k < U; ++k) {
-          const int r = (u + g0 + k) & 63;
-          const float gg = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, mg), r));
-          const float hh = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, mh), r));
-#pragma unroll
-          for (int j = 0; j < FPL; ++j) {
-            float2 b0 = val[j][k];
-#pragma unroll
-            for (int q = 0; q < k; ++q)
-              if (ok[j][q] && addr[j][q] == addr[j][k]) b0 = val[j][q];
-            val[j][k] = make_float2(b0.x + gg, b0.y + hh);
-          }
-        }
-#pragma unroll
-        for (int k = 0; k < U; ++k)
-#pragma unroll
-          for (int j = 0; j < FPL; ++j)
-            if (ok[j][k]) lgh[addr[j][k]] = val[j][k];
+      for (int q = 0; q < L; ++q) {
+        const int r = (u + q) & 63;
+        const unsigned long long qp =
+            ((unsigned long long)(uint32_t)__builtin_amdgcn_readlane((int)(myqp >> 32), r) << 32) |
+            (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)myqp, r);
+        unsigned long long qh = 0ull;
+        if (!ns.count_mode)
+          qh = ((unsigned long long)(uint32_t)__builtin_amdgcn_readlane((int)(myqh >> 32), r) << 32) |
+               (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)myqh, r);
+        add_row(cur[q], qp, qh);
       }
 #pragma unroll
       for (int q = 0; q < L; ++q) cur[q] = nxt[q];
@@ -272,14 +284,11 @@ k_hist_build(const HistArgs a) {
   }
   __syncthreads();
 
-  // Add the W sub-histograms in wave order and write the chunk result.
-  const int SL = kWave * FPL * B;  // outputs per plane of this slab
+  // Decode and write the chunk result.  Thread t reads word t (lane-fastest:
+  // conflict free) = feature lane*FPL + j, bin, with (j, bin) = divmod(t/64, B).
+  const int SL = kWave * FPL * B;
   const int f_base = slab * kWave * FPL;
   const bool direct = (nch == 1);
-  const int slot = a.seg_slot[s];
-  const float2* all = reinterpret_cast<const float2*>(smem);
-  // Thread t reads word t of each wave's plane (lane-fastest: conflict free)
-  // = feature lane*FPL + j, bin, with (j, bin) = divmod(t / 64, B).
   for (int t = threadIdx.x; t < SL; t += blockDim.x) {
     const int r = t >> 6;
     const int ln = t & 63;
@@ -287,27 +296,24 @@ k_hist_build(const HistArgs a) {
     const int bin = r - j * B;
     const int fl = ln * FPL + j;
     if (f_base + fl >= a.n_feat) continue;
-    const int e = fl * B + bin;
-    float2 v = all[t];
-    for (int w = 1; w < W; ++w) {
-      const float2 x = all[w * plane + t];
-      v.x += x.x;
-      v.y += x.y;
-    }
-    const float sg = v.x, sh = v.y;
+    const unsigned long long w = T[t];
+    const long long gsum = ((long long)(w << (64 - kCountShift))) >> (64 - kCountShift);
+    const long long count = (long long)((w - (unsigned long long)gsum) >> kCountShift);
+    const long long hsum = ns.count_mode ? count * ns.qh_const : (long long)Hq[t];
     if (direct) {
       const int64_t d = ((int64_t)slot * a.n_feat + f_base + fl) * B + bin;
-      if (a.accumulate) { a.out_g[d] += sg; a.out_h[d] += sh; }
-      else { a.out_g[d] = sg; a.out_h[d] = sh; }
+      const float fg = from_fixed(gsum, ns.sg), fh = from_fixed(hsum, ns.sh);
+      if (a.accumulate) { a.out_g[d] += fg; a.out_h[d] += fh; }
+      else { a.out_g[d] = fg; a.out_h[d] = fh; }
     } else {
-      const int64_t d = ((int64_t)(a.seg_pfirst[s] + k) * a.nslab + slab) * SL + e;
-      a.part_g[d] = sg;
-      a.part_h[d] = sh;
+      const int64_t d = ((int64_t)(a.seg_pfirst[s] + k) * a.nslab + slab) * SL + fl * B + bin;
+      a.part[d] = make_longlong2(gsum, hsum);
     }
   }
 }
 
-// Sum the chunk partials of every multi-chunk segment in chunk order.
+// Sum the chunk partials of every multi-chunk segment (exact integers) and
+// write the float32 histogram.
 template <int FPL>
 __global__ void __launch_bounds__(256)
 k_hist_merge(const HistArgs a, const int32_t* __restrict__ multi,
@@ -329,22 +335,38 @@ k_hist_merge(const HistArgs a, const int32_t* __restrict__ multi,
   const int fl = e / B;
   const int bin = e - fl * B;
   if (f_base + fl >= a.n_feat) return;
-  float sg = 0.0f, sh = 0.0f;
-  const float* pg = a.part_g + ((int64_t)first * a.nslab + slab) * SL + e;
-  const float* ph = a.part_h + ((int64_t)first * a.nslab + slab) * SL + e;
+  const NodeShift ns = node_shift(a.stats + (int64_t)(slot / a.slots_per_node) * kStatWords);
+  long long sg = 0, sh = 0;
+  const longlong2* p = a.part + ((int64_t)first * a.nslab + slab) * SL + e;
   const int64_t step = (int64_t)a.nslab * SL;
-  int k = 0;
-  for (; k + 4 <= nch; k += 4) {
-    const float g0 = pg[0], g1 = pg[step], g2 = pg[2 * step], g3 = pg[3 * step];
-    const float h0 = ph[0], h1 = ph[step], h2 = ph[2 * step], h3 = ph[3 * step];
-    sg += g0; sg += g1; sg += g2; sg += g3;
-    sh += h0; sh += h1; sh += h2; sh += h3;
-    pg += 4 * step; ph += 4 * step;
+  for (int k = 0; k < nch; ++k) {
+    const longlong2 v = p[(int64_t)k * step];
+    sg += v.x;
+    sh += v.y;
   }
-  for (; k < nch; ++k) { sg += pg[0]; sh += ph[0]; pg += step; ph += step; }
   const int64_t d = ((int64_t)slot * a.n_feat + f_base + fl) * B + bin;
-  if (a.accumulate) { a.out_g[d] += sg; a.out_h[d] += sh; }
-  else { a.out_g[d] = sg; a.out_h[d] = sh; }
+  const float fg = from_fixed(sg, ns.sg), fh = from_fixed(sh, ns.sh);
+  if (a.accumulate) { a.out_g[d] += fg; a.out_h[d] += fh; }
+  else { a.out_g[d] = fg; a.out_h[d] = fh; }
+}
+
+// Statistics of one row list (reference op): stats[0..3] over g[rows], h[rows].
+__global__ void k_stats_init(int32_t* st, int n_nodes) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < n_nodes) stats_init(st + (int64_t)j * kStatWords);
+}
+
+__global__ void __launch_bounds__(256)
+k_list_stats(const int32_t* __restrict__ rows, int n, int64_t n_rows, const float* __restrict__ g,
+             const float* __restrict__ h, int32_t* __restrict__ st) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  bool ok = false;
+  float gv = 0.0f, hv = 0.0f;
+  if (p < n) {
+    const int r = rows[p];
+    if ((uint64_t)r < (uint64_t)n_rows) { ok = true; gv = g[r]; hv = h[r]; }
+  }
+  stats_add_wave(st, ok, gv, hv);
 }
 
 // ---------------------------------------------------------------------------
@@ -466,7 +488,7 @@ constexpr size_t kLdsMax = 160 * 1024;
 int lds_pitch(const HistConfig&, int) { return 64; }
 
 size_t lds_bytes(const HistConfig& c, int n_bins) {
-  return (size_t)c.waves * c.fpl * n_bins * kWave * sizeof(float2);
+  return (size_t)2 * c.fpl * n_bins * kWave * sizeof(unsigned long long);
 }
 
 static HistConfig g_override;   // all-zero: automatic
@@ -476,7 +498,8 @@ void set_hist_override(const HistConfig& c) { g_override = c; }
 
 HistConfig pick_hist_config(int n_feat, int n_bins, int64_t row_stride) {
   HistConfig c = auto_hist_config(n_feat, n_bins, row_stride);
-  if (g_override.chunk_rows > 0) c.chunk_rows = g_override.chunk_rows;
+  if (g_override.chunk_rows > 0) c.chunk_rows = g_override.chunk_rows < kMaxChunkRows ? g_override.chunk_rows
+                                                                                     : kMaxChunkRows;
   if (g_override.waves > 0) c.waves = g_override.waves;
   if (g_override.fpl > 0 && row_stride % g_override.fpl == 0) c.fpl = g_override.fpl;
   c.depth = clamp_depth(c.fpl, g_override.depth);
@@ -487,13 +510,11 @@ HistConfig auto_hist_config(int n_feat, int n_bins, int64_t row_stride) {
   (void)n_feat;
   HistConfig c;
   c.chunk_rows = 8192;
-  // At most 64 KB of sub-histograms per workgroup: W = 4 waves unless bins
-  // are many, then the widest per-lane load that still fits.
-  c.waves = 4;
-  while (c.waves > 1 && (size_t)c.waves * 2 * 1 * n_bins * 64 * 4 > kLdsBudget) c.waves >>= 1;
+  c.waves = 8;
+  // Widest per-lane load whose shared histogram (FPL * B KB) still lets four
+  // workgroups share a CU's 160 KB of LDS.
   int fpl = 16;
-  while (fpl > 1 && ((size_t)c.waves * 2 * fpl * n_bins * 64 * 4 > kLdsBudget || row_stride % fpl != 0))
-    fpl >>= 1;
+  while (fpl > 1 && ((size_t)fpl * n_bins * 1024 > 40 * 1024 || row_stride % fpl != 0)) fpl >>= 1;
   c.fpl = fpl;
   return c;
 }
@@ -502,7 +523,7 @@ int64_t hist_max_chunks(int64_t rows_max, int64_t nseg_max, int chunk_rows) {
   return ceil_div(rows_max, chunk_rows) + nseg_max;
 }
 
-size_t hist_partial_floats(const HistConfig& c, int n_feat, int n_bins, int64_t rows_max) {
+size_t hist_partial_words(const HistConfig& c, int n_feat, int n_bins, int64_t rows_max) {
   const int nslab = (int)ceil_div(n_feat, kWave * c.fpl);
   // Only multi-chunk segments write partials; each holds more than C rows,
   // so their chunks number at most 2 * rows / C.
@@ -659,7 +680,8 @@ extern "C" int wgbm_profile_end(double* out4) {
 
 // Workspace carve-up for the reference-style histogram op.
 struct Hist3Layout {
-  size_t seg_start, seg_count, seg_slot, seg_first, seg_pfirst, nseg, multi, n_multi, sorted, cnt, part, total;
+  size_t seg_start, seg_count, seg_slot, seg_first, seg_pfirst, nseg, multi, n_multi, sorted, cnt, stats, part,
+      total;
 };
 
 static Hist3Layout hist3_layout(int64_t n, int n_feat, int n_bins, int n_eras, const HistConfig& c) {
@@ -679,7 +701,8 @@ static Hist3Layout hist3_layout(int64_t n, int n_feat, int n_bins, int n_eras, c
   const int64_t nb = ceil_div(n, kSortBlock);
   L.sorted = o;    o = al(o + (E > 1 ? 4 * n : 0));
   L.cnt = o;       o = al(o + (E > 1 ? 4 * nb * E : 0));
-  L.part = o;      o = al(o + 2 * 4 * hist_partial_floats(c, n_feat, n_bins, n));
+  L.stats = o;     o = al(o + 4 * kStatWords);
+  L.part = o;      o = al(o + sizeof(longlong2) * hist_partial_words(c, n_feat, n_bins, n));
   L.total = o;
   return L;
 }
@@ -729,10 +752,16 @@ extern "C" int wgbm_compute_histogram3(const int8_t* bin_indices, int64_t n_tota
   a.nseg_dev = P(L.nseg);
   a.out_g = grad_hist;
   a.out_h = hess_hist;
-  a.part_g = reinterpret_cast<float*>(ws + L.part);
-  a.part_h = a.part_g + hist_partial_floats(c, n_feat, num_bins, n_samples);
+  a.part = reinterpret_cast<longlong2*>(ws + L.part);
+  a.stats = P(L.stats);
+  a.slots_per_node = n_eras;
   a.accumulate = 1;
   const int n = (int)n_samples;
+  hipLaunchKernelGGL(k_stats_init, dim3(1), dim3(64), 0, stream, P(L.stats), 1);
+  WGBM_LAUNCHED("k_stats_init");
+  hipLaunchKernelGGL(k_list_stats, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, stream, sample_indices, n,
+                     n_total_rows, grads, hess, P(L.stats));
+  WGBM_LAUNCHED("k_list_stats");
   if (n_eras == 1) {
     a.perm = sample_indices;
     hipLaunchKernelGGL(k_single_segment, dim3(1), dim3(64), 0, stream, P(L.seg_start), P(L.seg_count),

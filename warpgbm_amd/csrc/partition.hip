@@ -10,6 +10,7 @@
 #include "common.h"
 #include "wgbm_internal.h"
 #include "wgbm_level.h"
+#include "wgbm_stats.h"
 
 namespace wgbm {
 
@@ -74,6 +75,8 @@ k_part_scan(PartArgs a) {
   }
   __syncthreads();
   // B: child tables per node.
+  if (a.c_stats)
+    for (int c = threadIdx.x; c < 2 * a.n_nodes; c += blockDim.x) stats_init(a.c_stats + (int64_t)c * kStatWords);
   for (int j = threadIdx.x; j < a.n_nodes; j += blockDim.x) {
     const bool split = a.node_active[j] && a.dec_feat[j] >= 0;
     const int l = 2 * j, r = 2 * j + 1;
@@ -179,6 +182,11 @@ k_part_scatter(PartArgs a) {
       const int dst = left ? (lrun + lpre + lanes_below(lm)) : (rrun + rpre + lanes_below(rm));
       a.perm_out[dst] = row;
       if (a.gh_out) a.gh_out[dst] = v;
+    }
+    if (a.c_stats) {  // statistics of each child's rows (for its histogram)
+      const int j = s / a.n_eras;
+      stats_add_wave(a.c_stats + (int64_t)(2 * j) * kStatWords, ok && left, v.x, v.y);
+      stats_add_wave(a.c_stats + (int64_t)(2 * j + 1) * kStatWords, ok && !left, v.x, v.y);
     }
     lrun += ltot;
     rrun += n_here - ltot;

@@ -36,8 +36,9 @@ struct HistArgs {
   int32_t wide_ok;
   float* out_g;            // [slot][n_feat][n_bins]
   float* out_h;
-  float* part_g;           // chunk partials
-  float* part_h;
+  longlong2* part;         // chunk partials (exact fixed-point sums of g, h)
+  const int32_t* stats;    // per node: max|g|, min h, max h, max|h| (see wgbm_stats.h)
+  int32_t slots_per_node;  // histogram slots of one node (eras); node = slot / this
   int32_t accumulate;      // 1: out += result (reference op), 0: out = result
   int32_t lds_pitch;       // floats between consecutive (j, bin) rows of a plane
 };
@@ -48,7 +49,8 @@ HistConfig pick_hist_config(int n_feat, int n_bins, int64_t row_stride);
 HistConfig auto_hist_config(int n_feat, int n_bins, int64_t row_stride);
 void set_hist_override(const HistConfig& c);
 size_t lds_bytes(const HistConfig& c, int n_bins);
-size_t hist_partial_floats(const HistConfig& c, int n_feat, int n_bins, int64_t rows_max);
+// Partial words (one longlong2 each) needed for rows_max rows.
+size_t hist_partial_words(const HistConfig& c, int n_feat, int n_bins, int64_t rows_max);
 // Launch plan + build + merge.  rows_max / nseg_max bound the device-side
 // row and segment counts (they size the grid).  multi/n_multi: scratch of
 // rows_max/(C+1)+1 ints and one int.
