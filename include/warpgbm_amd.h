@@ -132,12 +132,14 @@ int wgbm_bin_matrix(const float* X, int64_t n, int32_t n_feat, const float* edge
 /* ---------------------------------------------------------------- batched level-wise training */
 
 /* Histogram tiling shared by every histogram build (plain and batched):
- * rows per chunk, waves per workgroup (these two fix the float32 summation
- * order), features per lane (load width) and rows whose code loads are in
- * flight per wave.  0 = automatic.  wgbm_get_hist_config returns the
- * configuration used for a shape: out4 = {chunk_rows, waves, fpl, depth}. */
+ * rows per chunk, waves per workgroup, features per lane (load width) and
+ * rows whose code loads are in flight per wave.  0 = automatic.  Sums are
+ * exact fixed-point integers, so the tiling changes speed, never a bit of a
+ * histogram.  wgbm_get_hist_config returns the configuration used for a
+ * shape (const_hessian: every row has the same hessian, as in a regression
+ * fit with constant weights): out4 = {chunk_rows, waves, fpl, depth}. */
 int wgbm_set_hist_config(int32_t chunk_rows, int32_t waves, int32_t features_per_lane, int32_t rows_in_flight);
-int wgbm_get_hist_config(int32_t n_feat, int32_t num_bins, int64_t n_cols, int32_t* out4);
+int wgbm_get_hist_config(int32_t n_feat, int32_t num_bins, int64_t n_cols, int32_t const_hessian, int32_t* out4);
 
 /* Live timing of histogram launches with HIP events on their streams:
  * begin clears and enables; end waits and returns
@@ -158,6 +160,9 @@ typedef struct wgbm_fit_config {
   float min_split_gain;
   float min_child_weight;
   float lambda;             /* L2_reg */
+  int32_t const_weight;     /* 1: every weight[] entry holds the same value (regression
+                               hessians are then constant and histograms need one LDS
+                               plane); 0: unknown */
 } wgbm_fit_config;
 
 typedef struct wgbm_fit_io {

@@ -25,7 +25,7 @@ class FitConfig(C.Structure):
     _fields_ = [
         ("n_rows", i64), ("n_cols", i64), ("n_feat", i32), ("n_bins", i32), ("n_eras", i32),
         ("n_classes", i32), ("max_depth", i32), ("objective", i32), ("learning_rate", f32),
-        ("min_split_gain", f32), ("min_child_weight", f32), ("lambda_", f32),
+        ("min_split_gain", f32), ("min_child_weight", f32), ("lambda_", f32), ("const_weight", i32),
     ]
 
 
@@ -60,7 +60,7 @@ SIGNATURES = {
     "wgbm_bin_matrix_workspace_bytes": (sz, [i32, i32]),
     "wgbm_bin_matrix": (C.c_int, [vp, i64, i32, vp, i32, vp, vp, sz, vp]),
     "wgbm_set_hist_config": (C.c_int, [i32, i32, i32, i32]),
-    "wgbm_get_hist_config": (C.c_int, [i32, i32, i64, C.POINTER(i32)]),
+    "wgbm_get_hist_config": (C.c_int, [i32, i32, i64, i32, C.POINTER(i32)]),
     "wgbm_profile_begin": (C.c_int, []),
     "wgbm_profile_end": (C.c_int, [C.POINTER(f64)]),
     "wgbm_fit_workspace_bytes": (sz, [C.POINTER(FitConfig)]),

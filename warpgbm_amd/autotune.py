@@ -48,11 +48,18 @@ def _save_cache(d: dict) -> None:
         pass
 
 
-def candidates(n_feat: int, n_bins: int, n_cols: int):
-    """Configurations whose shared histogram fits a workgroup's 160 KB LDS."""
+def lds_bytes(fpl: int, n_bins: int, planes: int) -> int:
+    """Shared histogram of one workgroup: planes x [fpl][bins + 1][64] u64."""
+    return planes * fpl * (n_bins + 1) * 64 * 8
+
+
+def candidates(n_feat: int, n_bins: int, n_cols: int, const_hessian: bool = False):
+    """Configurations whose shared histogram fits a workgroup's 160 KB LDS
+    (two planes unless every row has the same hessian)."""
     out = []
+    planes = 1 if const_hessian else 2
     for fpl, depths in ((1, (32, 64)), (2, (16, 32)), (4, (8, 16)), (8, (4, 8)), (16, (2, 4))):
-        if n_cols % fpl or fpl * n_bins * 1024 > 160 * 1024:
+        if n_cols % fpl or lds_bytes(fpl, n_bins, planes) > 160 * 1024:
             continue
         if fpl > 1 and 64 * fpl > 2 * max(n_feat, 1):
             continue  # slab far wider than the features
@@ -61,9 +68,9 @@ def candidates(n_feat: int, n_bins: int, n_cols: int):
     return out
 
 
-def current(n_feat: int, n_bins: int, n_cols: int):
+def current(n_feat: int, n_bins: int, n_cols: int, const_hessian: bool = False):
     out = (ctypes.c_int32 * 4)()
-    _lib.call("wgbm_get_hist_config", n_feat, n_bins, n_cols, out)
+    _lib.call("wgbm_get_hist_config", n_feat, n_bins, n_cols, int(const_hessian), out)
     return tuple(out)
 
 

@@ -429,6 +429,7 @@ class WarpGBM(BaseEstimator, RegressorMixin):
         self.num_samples, self.num_features = X.shape
         self.num_eras = len(self.unique_eras)
         self.sample_weight_gpu = torch.from_numpy(np.ascontiguousarray(sample_weight, np.float32)).to(self.device)
+        self._const_weight = None
         self._feature_ids = torch.arange(self.num_features, dtype=torch.int32, device=self.device)
         self.feature_indices = self._feature_ids
         # Root row order: rows grouped by era, stable (identity for one era).
@@ -473,6 +474,7 @@ class WarpGBM(BaseEstimator, RegressorMixin):
         self.Y_gpu = y.to(torch.float32).contiguous()
         w = torch.ones(N, dtype=torch.float32, device=self.device) if sample_weight is None else sample_weight
         self.sample_weight_gpu = w.to(torch.float32).contiguous()
+        self._const_weight = None
         self.bin_edges = (bin_edges if bin_edges is not None else
                           torch.arange(1, self.num_bins, dtype=torch.float32, device=self.device).repeat(F, 1))
         self._prepare_monotonic_constraints(F)
@@ -805,6 +807,12 @@ class WarpGBM(BaseEstimator, RegressorMixin):
         c.min_split_gain = float(self.min_split_gain)
         c.min_child_weight = float(self.min_child_weight)
         c.lambda_ = float(self.L2_reg)
+        # Equal weights make regression hessians constant: the histogram
+        # kernel then keeps one LDS plane (checked once per fit on device).
+        w = self.sample_weight_gpu
+        if getattr(self, "_const_weight", None) is None:
+            self._const_weight = bool(w.numel() == 0 or bool((w == w[0]).all()))
+        c.const_weight = int(self._const_weight)
         return c
 
     def _batched_forest(self, start, k, n_classes):

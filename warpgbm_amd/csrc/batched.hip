@@ -42,8 +42,10 @@ struct FitLayout {
   HistConfig hc;
 };
 
-static HistConfig fit_hist_config(int n_feat, int n_bins, int64_t n_cols) {
-  return pick_hist_config(n_feat, n_bins, n_cols);
+static bool fit_one_plane(const wgbm_fit_config& c) { return c.objective == 0 && c.const_weight != 0; }
+
+static HistConfig fit_hist_config(int n_feat, int n_bins, int64_t n_cols, bool one_plane) {
+  return pick_hist_config(n_feat, n_bins, n_cols, one_plane);
 }
 
 static FitLayout fit_layout(const wgbm_fit_config& c) {
@@ -56,7 +58,7 @@ static FitLayout fit_layout(const wgbm_fit_config& c) {
   L.max_nodes = K << D;                       // nodes of the deepest level
   L.node_elems = E * (int64_t)c.n_feat * c.n_bins;
   L.hist_nodes = D >= 1 ? (K << (D - 1)) : K;  // deepest level with histograms
-  L.hc = fit_hist_config(c.n_feat, c.n_bins, c.n_cols);
+  L.hc = fit_hist_config(c.n_feat, c.n_bins, c.n_cols, fit_one_plane(c));
   L.perm_root = take(4 * L.KN);
   L.gh_root = take(8 * L.KN);
   for (int i = 0; i < 2; ++i) { L.perm_pp[i] = take(4 * L.KN); L.gh_pp[i] = take(8 * L.KN); }
@@ -245,8 +247,9 @@ extern "C" int wgbm_set_hist_config(int32_t chunk_rows, int32_t waves, int32_t f
   return kOk;
 }
 
-extern "C" int wgbm_get_hist_config(int32_t n_feat, int32_t n_bins, int64_t n_cols, int32_t* out4) {
-  const HistConfig c = fit_hist_config(n_feat, n_bins, n_cols);
+extern "C" int wgbm_get_hist_config(int32_t n_feat, int32_t n_bins, int64_t n_cols, int32_t const_hessian,
+                                    int32_t* out4) {
+  const HistConfig c = fit_hist_config(n_feat, n_bins, n_cols, const_hessian != 0);
   out4[0] = c.chunk_rows;
   out4[1] = c.waves;
   out4[2] = c.fpl;
@@ -337,6 +340,7 @@ extern "C" int wgbm_fit_rounds(const wgbm_fit_config* cfg, const wgbm_fit_io* io
         a.out_h = hh;
         a.part = reinterpret_cast<longlong2*>(ws + L.h_part);
         a.slots_per_node = E;
+        a.hess_const = fit_one_plane(c) ? 1 : 0;
         a.stats = I(cur.stats);
         a.accumulate = 0;
         if (d == 0) {

@@ -45,6 +45,16 @@ template <> struct CodeVec<8>  { using T = uint64_t; };
 template <> struct CodeVec<16> { using T = uint4; };
 
 template <int FPL>
+__device__ __forceinline__ uint32_t code_u8(const typename CodeVec<FPL>::T& v, int j) {
+  if constexpr (FPL == 16) {
+    const uint32_t w = j < 4 ? v.x : j < 8 ? v.y : j < 12 ? v.z : v.w;
+    return (w >> (8 * (j & 3))) & 0xffu;
+  } else {
+    return (uint32_t)((uint64_t)v >> (8 * j)) & 0xffu;
+  }
+}
+
+template <int FPL>
 __device__ __forceinline__ int code_byte(const typename CodeVec<FPL>::T& v, int j) {
   if constexpr (FPL == 16) {
     uint32_t w = j < 4 ? v.x : j < 8 ? v.y : j < 12 ? v.z : v.w;
@@ -147,7 +157,12 @@ __device__ __forceinline__ NodeShift node_shift(const int32_t* st) {
   return n;
 }
 
-template <int FPL, int L, bool GH_POS>
+// ONE_PLANE: only the T plane is allocated and H = count * qh, valid for a
+// node whose rows all have one hessian value (min h == max h).  Unless the
+// caller guarantees that for every node (a.hess_const), both variants are
+// launched over the same grid (a.dual) and each workgroup of each variant
+// runs only if its node's statistics select that variant.
+template <int FPL, int L, bool GH_POS, bool ONE_PLANE>
 __global__ void __launch_bounds__(512)
 k_hist_build(const HistArgs a) {
   extern __shared__ unsigned long long s64[];
@@ -177,19 +192,26 @@ k_hist_build(const HistArgs a) {
   const int w1 = min(w0 + per, c1);
   const int slot = a.seg_slot[s];
   const NodeShift ns = node_shift(a.stats + (int64_t)(slot / a.slots_per_node) * kStatWords);
+  if (a.dual && ONE_PLANE != ns.count_mode) return;  // the other variant builds this node
 
-  // One shared histogram per workgroup: T plane (count << 44 | sum qg) and
-  // H plane (sum qh, unused in count mode), each [FPL][B][64] words.
+  // One shared histogram per workgroup: T plane (count << 44 | sum qg) and,
+  // unless the hessian is constant, an H plane (sum qh).  A plane holds
+  // [FPL][B + 1][64] words: feature lane*FPL + j, bin b, lane at
+  // ((j * (B + 1) + b) * 64 + lane); bin B is a spare row that absorbs codes
+  // outside [0, B), so the adds need no branch.  Each lane owns its own
+  // column, so the 64 lanes of an add hit 64 different banks.
   const int B = a.n_bins;
-  const int plane = FPL * B * kWave;
+  const int plane = FPL * (B + 1) * kWave;
   unsigned long long* T = s64;
   unsigned long long* Hq = s64 + plane;
-  for (int q = threadIdx.x; q < 2 * plane; q += blockDim.x) s64[q] = 0ull;
+  for (int q = threadIdx.x; q < (ONE_PLANE ? 1 : 2) * plane; q += blockDim.x) s64[q] = 0ull;
   __syncthreads();
 
   const int fl0 = slab * kWave * FPL + lane * FPL;  // local feature of byte 0
   const bool lane_on = fl0 < a.n_feat;
   // Wide loads need the slab's features to be the identity columns.
+  // Features past n_feat (and a lane with none) add into their own lane's
+  // columns, which the read-out skips.
   bool ident = a.wide_ok;
   int cols[FPL];
   bool jon[FPL];
@@ -202,6 +224,14 @@ k_hist_build(const HistArgs a) {
     if (!jon[j]) c = 0;
     cols[j] = c;
     if (fl < a.n_feat && c != fl) ident = false;
+  }
+  // Byte offset of (j, bin 0) in this lane's column; a column outside the
+  // matrix points at its spare row so that it adds nothing readable.
+  uint32_t jbase[FPL], jcap[FPL];
+#pragma unroll
+  for (int j = 0; j < FPL; ++j) {
+    jbase[j] = ((uint32_t)(j * (B + 1) + (jon[j] ? 0 : B)) * kWave + lane) * 8u;
+    jcap[j] = jon[j] ? (uint32_t)B : 0u;
   }
   ident = __all(ident || !lane_on);
 
@@ -227,11 +257,16 @@ k_hist_build(const HistArgs a) {
   auto add_row = [&](const TV& v, unsigned long long qp, unsigned long long qh) {
 #pragma unroll
     for (int j = 0; j < FPL; ++j) {
-      const int bin = code_byte<FPL>(v, j);
-      if (jon[j] && (unsigned)bin < (unsigned)B) {
-        const int o = (j * B + bin) * kWave + lane;
-        __hip_atomic_fetch_add(T + o, qp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        if (!ns.count_mode) __hip_atomic_fetch_add(Hq + o, qh, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      // Codes are bytes; anything outside [0, B) (negative included) lands in
+      // the spare row.  A column outside the matrix has jbase at its spare
+      // row and a cap of 0.
+      const uint32_t bin = min(code_u8<FPL>(v, j), jcap[j]);
+      const uint32_t off = jbase[j] + bin * (uint32_t)(kWave * 8);
+      unsigned long long* tp = reinterpret_cast<unsigned long long*>(reinterpret_cast<char*>(T) + off);
+      __hip_atomic_fetch_add(tp, qp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      if constexpr (!ONE_PLANE) {
+        unsigned long long* hp = reinterpret_cast<unsigned long long*>(reinterpret_cast<char*>(Hq) + off);
+        __hip_atomic_fetch_add(hp, qh, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
       }
     }
   };
@@ -273,7 +308,7 @@ k_hist_build(const HistArgs a) {
             ((unsigned long long)(uint32_t)__builtin_amdgcn_readlane((int)(myqp >> 32), r) << 32) |
             (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)myqp, r);
         unsigned long long qh = 0ull;
-        if (!ns.count_mode)
+        if constexpr (!ONE_PLANE)
           qh = ((unsigned long long)(uint32_t)__builtin_amdgcn_readlane((int)(myqh >> 32), r) << 32) |
                (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)myqh, r);
         add_row(cur[q], qp, qh);
@@ -296,10 +331,11 @@ k_hist_build(const HistArgs a) {
     const int bin = r - j * B;
     const int fl = ln * FPL + j;
     if (f_base + fl >= a.n_feat) continue;
-    const unsigned long long w = T[t];
+    const int word = t + j * kWave;  // skip the spare rows of features 0..j-1
+    const unsigned long long w = T[word];
     const long long gsum = ((long long)(w << (64 - kCountShift))) >> (64 - kCountShift);
     const long long count = (long long)((w - (unsigned long long)gsum) >> kCountShift);
-    const long long hsum = ns.count_mode ? count * ns.qh_const : (long long)Hq[t];
+    const long long hsum = ONE_PLANE ? count * ns.qh_const : (long long)Hq[word];
     if (direct) {
       const int64_t d = ((int64_t)slot * a.n_feat + f_base + fl) * B + bin;
       const float fg = from_fixed(gsum, ns.sg), fh = from_fixed(hsum, ns.sh);
@@ -487,8 +523,8 @@ constexpr size_t kLdsMax = 160 * 1024;
 
 int lds_pitch(const HistConfig&, int) { return 64; }
 
-size_t lds_bytes(const HistConfig& c, int n_bins) {
-  return (size_t)2 * c.fpl * n_bins * kWave * sizeof(unsigned long long);
+size_t lds_bytes(const HistConfig& c, int n_bins, bool count_mode) {
+  return (size_t)(count_mode ? 1 : 2) * c.fpl * (n_bins + 1) * kWave * sizeof(unsigned long long);
 }
 
 static HistConfig g_override;   // all-zero: automatic
@@ -496,8 +532,8 @@ int clamp_depth(int fpl, int depth);
 
 void set_hist_override(const HistConfig& c) { g_override = c; }
 
-HistConfig pick_hist_config(int n_feat, int n_bins, int64_t row_stride) {
-  HistConfig c = auto_hist_config(n_feat, n_bins, row_stride);
+HistConfig pick_hist_config(int n_feat, int n_bins, int64_t row_stride, bool one_plane) {
+  HistConfig c = auto_hist_config(n_feat, n_bins, row_stride, one_plane);
   if (g_override.chunk_rows > 0) c.chunk_rows = g_override.chunk_rows < kMaxChunkRows ? g_override.chunk_rows
                                                                                      : kMaxChunkRows;
   if (g_override.waves > 0) c.waves = g_override.waves;
@@ -506,15 +542,19 @@ HistConfig pick_hist_config(int n_feat, int n_bins, int64_t row_stride) {
   return c;
 }
 
-HistConfig auto_hist_config(int n_feat, int n_bins, int64_t row_stride) {
+HistConfig auto_hist_config(int n_feat, int n_bins, int64_t row_stride, bool one_plane) {
   (void)n_feat;
   HistConfig c;
+  (void)one_plane;
   c.chunk_rows = 8192;
   c.waves = 8;
-  // Widest per-lane load whose shared histogram (FPL * B KB) still lets four
-  // workgroups share a CU's 160 KB of LDS.
+  // Widest per-lane load whose one-plane histogram lets two workgroups of 8
+  // waves share a CU's 160 KB of LDS (the two-plane variant then fits one).
+  // Measured at 1M x 1000 x 32 bins: FPL 4 (67 KB) beats FPL 2 (34 KB,
+  // four workgroups) by 1.4x and FPL 8 (135 KB, one workgroup) by 1.1x.
   int fpl = 16;
-  while (fpl > 1 && ((size_t)fpl * n_bins * 1024 > 40 * 1024 || row_stride % fpl != 0)) fpl >>= 1;
+  c.fpl = fpl;
+  while (fpl > 1 && (lds_bytes(c, n_bins, true) > 80 * 1024 || row_stride % fpl != 0)) c.fpl = fpl >>= 1;
   c.fpl = fpl;
   return c;
 }
@@ -560,25 +600,40 @@ static void timer_clear() {
   g_timer.merge.clear();
 }
 
-template <int FPL, int L>
-static int launch_fpl(const HistArgs& a, const HistConfig& c, int64_t max_chunks, int64_t max_multi,
-                      int32_t* multi, int32_t* n_multi, hipStream_t st) {
-  const size_t lds = lds_bytes(c, a.n_bins);
-  const int64_t groups = ceil_div(max_chunks, 8) * 8 * a.nslab;
-  if (groups > 0x7fffffff) { set_error("histogram grid too large"); return kTooBig; }
+template <int FPL, int L, bool CNT>
+static int launch_mode(const HistArgs& a, const HistConfig& c, int64_t groups, hipStream_t st) {
+  const size_t lds = lds_bytes(c, a.n_bins, CNT);
   if (lds > 64 * 1024) {  // opt in to more than 64 KB of dynamic LDS
-    WGBM_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_hist_build<FPL, L, true>),
+    WGBM_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_hist_build<FPL, L, true, CNT>),
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    WGBM_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_hist_build<FPL, L, false>),
+    WGBM_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_hist_build<FPL, L, false, CNT>),
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   }
   timer_mark(g_timer.build, true, st);
   if (a.gh)
-    hipLaunchKernelGGL((k_hist_build<FPL, L, true>), dim3((unsigned)groups), dim3(64 * c.waves), lds, st, a);
+    hipLaunchKernelGGL((k_hist_build<FPL, L, true, CNT>), dim3((unsigned)groups), dim3(64 * c.waves), lds, st, a);
   else
-    hipLaunchKernelGGL((k_hist_build<FPL, L, false>), dim3((unsigned)groups), dim3(64 * c.waves), lds, st, a);
+    hipLaunchKernelGGL((k_hist_build<FPL, L, false, CNT>), dim3((unsigned)groups), dim3(64 * c.waves), lds, st, a);
   WGBM_LAUNCHED("k_hist_build");
   timer_mark(g_timer.build, false, st);
+  return kOk;
+}
+
+template <int FPL, int L>
+static int launch_fpl(const HistArgs& a, const HistConfig& c, int64_t max_chunks, int64_t max_multi,
+                      int32_t* multi, int32_t* n_multi, hipStream_t st) {
+  const int64_t groups = ceil_div(max_chunks, 8) * 8 * a.nslab;
+  if (groups > 0x7fffffff) { set_error("histogram grid too large"); return kTooBig; }
+  int rc;
+  if (a.hess_const) {
+    rc = launch_mode<FPL, L, true>(a, c, groups, st);
+  } else {
+    HistArgs d = a;
+    d.dual = 1;
+    rc = launch_mode<FPL, L, true>(d, c, groups, st);
+    if (rc == kOk) rc = launch_mode<FPL, L, false>(d, c, groups, st);
+  }
+  if (rc != kOk) return rc;
   if (max_multi > 0) {
     const int SL = kWave * FPL * a.n_bins;
     const int nblk = (int)ceil_div(SL, 256);
@@ -594,8 +649,8 @@ static int launch_fpl(const HistArgs& a, const HistConfig& c, int64_t max_chunks
 // Load depth (rows in flight per wave) allowed per features-per-lane.
 int clamp_depth(int fpl, int depth) {
   const int hi = fpl == 1 ? 64 : fpl == 2 ? 32 : fpl == 4 ? 16 : fpl == 8 ? 8 : 4;
-  const int lo = hi / 4 < 2 ? 2 : hi / 4;
-  const int def = fpl <= 2 ? 32 : fpl == 4 ? 16 : fpl == 8 ? 8 : 4;
+  const int lo = hi / 2;
+  const int def = fpl == 1 ? lo : hi;  // measured: FPL 1 at 64 rows in flight runs 3x slower than at 32
   int d = depth > 0 ? depth : def;
   if (d > hi) d = hi;
   if (d < lo) d = lo;
@@ -604,20 +659,13 @@ int clamp_depth(int fpl, int depth) {
   return p;
 }
 
-#define WGBM_DEPTHS(FPL, D0, D1, D2)                                                   \
-  switch (c.depth) {                                                                  \
-    case D0: return launch_fpl<FPL, D0>(a, c, max_chunks, max_multi, multi, n_multi, st); \
-    case D1: return launch_fpl<FPL, D1>(a, c, max_chunks, max_multi, multi, n_multi, st); \
-    default: return launch_fpl<FPL, D2>(a, c, max_chunks, max_multi, multi, n_multi, st); \
-  }
-
 int hist_launch(HistArgs a, const HistConfig& c, int64_t rows_max, int64_t nseg_max,
                 int32_t* multi, int32_t* n_multi, hipStream_t st) {
   a.chunk_rows = c.chunk_rows;
   a.nslab = (int)ceil_div(a.n_feat, kWave * c.fpl);
   a.wide_ok = (a.row_stride % c.fpl == 0) && ((uintptr_t)a.codes % (uintptr_t)c.fpl == 0);
   a.lds_pitch = lds_pitch(c, a.n_bins);
-  if (lds_bytes(c, a.n_bins) > kLdsMax) {
+  if (lds_bytes(c, a.n_bins, a.hess_const != 0) > kLdsMax) {
     set_error("histogram LDS budget exceeded (%d bins, %d waves, %d features per lane)", a.n_bins, c.waves, c.fpl);
     return kTooBig;
   }
@@ -631,15 +679,17 @@ int hist_launch(HistArgs a, const HistConfig& c, int64_t rows_max, int64_t nseg_
   const int64_t max_chunks = hist_max_chunks(rows_max, nseg_max, c.chunk_rows);
   const int64_t max_multi = rows_max / (c.chunk_rows + 1) + 1;
   if (a.n_feat == 0) return kOk;
+#define WGBM_D2(FPL, D0, D1)                                                         \
+  return c.depth <= D0 ? launch_fpl<FPL, D0>(a, c, max_chunks, max_multi, multi, n_multi, st) \
+                       : launch_fpl<FPL, D1>(a, c, max_chunks, max_multi, multi, n_multi, st);
   switch (c.fpl) {
-    case 1: WGBM_DEPTHS(1, 16, 32, 64)
-    case 2: WGBM_DEPTHS(2, 8, 16, 32)
-    case 4: WGBM_DEPTHS(4, 4, 8, 16)
-    case 8: WGBM_DEPTHS(8, 2, 4, 8)
-    case 16:
-      return c.depth <= 2 ? launch_fpl<16, 2>(a, c, max_chunks, max_multi, multi, n_multi, st)
-                          : launch_fpl<16, 4>(a, c, max_chunks, max_multi, multi, n_multi, st);
+    case 1: WGBM_D2(1, 32, 64)
+    case 2: WGBM_D2(2, 16, 32)
+    case 4: WGBM_D2(4, 8, 16)
+    case 8: WGBM_D2(8, 4, 8)
+    case 16: WGBM_D2(16, 2, 4)
   }
+#undef WGBM_D2
   set_error("unsupported features-per-lane %d", c.fpl);
   return kBadArg;
 }
@@ -710,7 +760,7 @@ static Hist3Layout hist3_layout(int64_t n, int n_feat, int n_bins, int n_eras, c
 extern "C" size_t wgbm_histogram3_workspace_bytes(int64_t n_rows, int32_t n_feat, int32_t n_bins,
                                                   int32_t n_eras) {
   if (n_rows < 0 || n_feat < 0 || n_bins < 1 || n_eras < 1) return 0;
-  const HistConfig c = pick_hist_config(n_feat, n_bins, n_feat);
+  const HistConfig c = pick_hist_config(n_feat, n_bins, n_feat, false);
   return hist3_layout(n_rows, n_feat, n_bins, n_eras, c).total;
 }
 
@@ -727,7 +777,7 @@ extern "C" int wgbm_compute_histogram3(const int8_t* bin_indices, int64_t n_tota
   WGBM_REQUIRE(n_eras >= 1 && n_eras <= 32768, "era count must be in [1, 32768], got %d", n_eras);
   WGBM_REQUIRE(n_feat >= 0, "feature count must be non-negative");
   WGBM_REQUIRE(n_total_rows >= 0 && n_master_feat >= 0, "bad bin matrix shape");
-  const HistConfig c = pick_hist_config(n_feat, num_bins, n_master_feat);
+  const HistConfig c = pick_hist_config(n_feat, num_bins, n_master_feat, false);
   const Hist3Layout L = hist3_layout(n_samples, n_feat, num_bins, n_eras, c);
   WGBM_REQUIRE(workspace_bytes >= L.total, "workspace too small: need %zu bytes, got %zu", L.total,
                workspace_bytes);

@@ -39,16 +39,19 @@ struct HistArgs {
   longlong2* part;         // chunk partials (exact fixed-point sums of g, h)
   const int32_t* stats;    // per node: max|g|, min h, max h, max|h| (see wgbm_stats.h)
   int32_t slots_per_node;  // histogram slots of one node (eras); node = slot / this
+  int32_t hess_const;      // 1: the caller guarantees one hessian value for every row
+  int32_t dual;            // set by hist_launch: one- and two-plane variants both launched
   int32_t accumulate;      // 1: out += result (reference op), 0: out = result
   int32_t lds_pitch;       // floats between consecutive (j, bin) rows of a plane
 };
 
 // Configuration used by every histogram build: the automatic choice with
 // the process-wide override (wgbm_set_hist_config) applied.
-HistConfig pick_hist_config(int n_feat, int n_bins, int64_t row_stride);
-HistConfig auto_hist_config(int n_feat, int n_bins, int64_t row_stride);
+// one_plane: the hessian is known to be constant (one LDS plane per histogram).
+HistConfig pick_hist_config(int n_feat, int n_bins, int64_t row_stride, bool one_plane);
+HistConfig auto_hist_config(int n_feat, int n_bins, int64_t row_stride, bool one_plane);
 void set_hist_override(const HistConfig& c);
-size_t lds_bytes(const HistConfig& c, int n_bins);
+size_t lds_bytes(const HistConfig& c, int n_bins, bool count_mode);
 // Partial words (one longlong2 each) needed for rows_max rows.
 size_t hist_partial_words(const HistConfig& c, int n_feat, int n_bins, int64_t rows_max);
 // Launch plan + build + merge.  rows_max / nseg_max bound the device-side
