@@ -36,10 +36,13 @@ def parse():
     p.add_argument("--bins", type=int, default=32)
     p.add_argument("--plain-trees", type=int, default=100,
                    help="trees of the plain per-node fit (its time is scaled to --trees if fewer)")
-    p.add_argument("--cpu-rows", type=int, default=100_000, help="rows of the CPU oracle sample")
+    p.add_argument("--cpu-rows", type=int, default=200_000, help="rows of the CPU oracle sample")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-plain", action="store_true")
     p.add_argument("--profile-only", action="store_true", help="one fit, no extras (for rocprof)")
+    p.add_argument("--normal-edges", action="store_true",
+                   help="bin with the exact N(0,1) quantiles instead of sorting the columns (no sort kernels: "
+                        "for counter-collection runs, where the profiler fails inside torch.sort)")
     return p.parse_args()
 
 
@@ -58,11 +61,17 @@ def make_data(args, dev):
     return X, y.contiguous()
 
 
-def bin_on_device(X, bins):
+def bin_on_device(X, bins, normal_edges=False):
+    import torch
     from warpgbm_amd import WarpGBM
     m = WarpGBM(num_bins=bins)
     m.device = X.device
-    edges = m._quantile_edges(X)
+    if normal_edges:
+        from scipy.special import ndtri
+        q = ndtri(np.arange(1, bins) / bins).astype(np.float32)
+        edges = torch.from_numpy(np.tile(q, (X.shape[1], 1))).to(X.device)
+    else:
+        edges = m._quantile_edges(X)
     codes = m._bin_matrix(X, edges)
     return codes, edges
 
@@ -88,6 +97,24 @@ def hist_algorithmic_bytes(tables, N, F, B, D, E=1):
                         rows += min(cl, cr)
                         nodes += 1
     return int(rows * (F + 8 + 4) + nodes * E * F * B * 8), int(rows)
+
+
+TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                            "k_hist_build_1Mx1000_d5_traffic.md")
+
+
+def measured_traffic(args):
+    """HBM bytes per k_hist_build launch from the committed FETCH_SIZE /
+    WRITE_SIZE counter summary of this workload (None for other shapes)."""
+    if (args.rows, args.features, args.bins, args.depth, args.trees) != (1_000_000, 1000, 32, 5, 100):
+        return None
+    try:
+        for line in open(TRAFFIC_FILE):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return int(line.split(":")[1])
+    except OSError:
+        return None
+    return None
 
 
 def trees_identical(fa, fb):
@@ -139,7 +166,7 @@ def main():
     log(f"device {torch.cuda.get_device_name(0)}; generating {args.rows}x{args.features}")
     X, y = make_data(args, dev)
     t0 = time.perf_counter()
-    codes, edges = bin_on_device(X, args.bins)
+    codes, edges = bin_on_device(X, args.bins, args.normal_edges)
     torch.cuda.synchronize()
     bin_s = time.perf_counter() - t0
     del X
@@ -255,9 +282,11 @@ def main():
                    "rows": args.rows, "features": args.features, "trees": args.trees, "depth": args.depth,
                    "bins": args.bins},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_hist_build", "launches": n_build, "kernel_ms_per_fit": build_ms,
-                     "algorithmic_bytes_per_fit": alg_bytes, "built_rows_per_fit": built_rows,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args),
+                     "kernel": "k_hist_build", "launches": n_build,
+                     "avg_launch_us": build_ms * 1e3 / max(n_build, 1),
+                     "algorithmic_bytes_per_launch": alg_bytes // max(n_build, 1),
+                     "kernel_ms_per_fit": build_ms, "built_rows_per_fit": built_rows,
                      "merge_ms_per_fit": merge_ms},
         "plain": plain,
         "predict": {"rows_per_request": 10000, "p50_ms": float(np.percentile(lat, 50)),

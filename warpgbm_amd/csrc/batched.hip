@@ -34,6 +34,7 @@ struct FitLayout {
   size_t dec_feat, dec_bin, era_gain, cand_agree, cand_crit, cand_idx;
   size_t b_seg_start, b_seg_count, b_seg_slot, n_build, sib, n_split;
   size_t h_seg_first, h_seg_pfirst, h_nseg, h_multi, h_n_multi, h_part;
+  size_t stat_part;
   size_t p_flag, p_seg_first, p_nseg, p_multi, p_n_multi, p_lbase, p_rbase, p_segL;
   size_t l_seg_first, l_nseg, l_multi, l_n_multi, l_part, l_val;
   size_t root_seg_start, root_seg_count, root_seg_slot, root_nseg, era_start;
@@ -104,6 +105,10 @@ static FitLayout fit_layout(const wgbm_fit_config& c) {
   L.p_lbase = take(4 * L.part_chunks);
   L.p_rbase = take(4 * L.part_chunks);
   L.p_segL = take(4 * MS);
+  // statistics partials: per block of the objective (K classes) or per
+  // partition chunk and child side
+  const int64_t stat_parts = (int64_t)K * kStatBlocks > 2 * L.part_chunks ? (int64_t)K * kStatBlocks : 2 * L.part_chunks;
+  L.stat_part = take(4 * kStatWords * stat_parts);
   L.leaf_chunks = leaf_max_chunks(L.KN, L.max_nodes);
   L.l_seg_first = take(4 * (L.max_nodes + 1));
   L.l_nseg = take(4);
@@ -124,9 +129,8 @@ static FitLayout fit_layout(const wgbm_fit_config& c) {
 __global__ void k_root_tables(const int32_t* __restrict__ era_count, int E, int K, int N,
                               int32_t* era_start, int32_t* seg_start, int32_t* seg_count, int32_t* seg_slot,
                               int32_t* nseg, int32_t* t_seg_start, int32_t* t_seg_count, int32_t* t_active,
-                              int32_t* t_node_start, int32_t* t_node_count, int32_t* t_stats) {
+                              int32_t* t_node_start, int32_t* t_node_count) {
   if (threadIdx.x != 0) return;
-  for (int c = 0; c < K; ++c) stats_init(t_stats + c * kStatWords);
   int run = 0;
   for (int e = 0; e < E; ++e) { era_start[e] = run; run += era_count[e]; }
   for (int c = 0; c < K; ++c) {
@@ -153,25 +157,27 @@ k_root_perm(const int32_t* __restrict__ order, int64_t N, int K, int32_t* __rest
   perm[i] = order ? order[p] : (int32_t)p;
 }
 
+// Regression gradients in root position order, with one statistics partial
+// per block (grid-stride over positions).
 __global__ void __launch_bounds__(256)
 k_grad_regression_pos(const float* __restrict__ y, const float* __restrict__ pred, const float* __restrict__ w,
                       int64_t n, const int32_t* __restrict__ perm0, float2* __restrict__ gh,
-                      int32_t* __restrict__ stats) {
-  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  float g = 0.0f, h = 0.0f;
-  if (p < n) {
+                      int32_t* __restrict__ part) {
+  StatAcc acc;
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
     const int64_t r = perm0[p];
-    h = w[r];
-    g = (y[r] - pred[r]) * h;
+    const float h = w[r];
+    const float g = (y[r] - pred[r]) * h;
     gh[p] = make_float2(g, h);
+    acc.add(g, h);
   }
-  stats_add_wave(stats, p < n, g, h);
+  acc.block_reduce();
+  if (threadIdx.x == 0) acc.store(part + (int64_t)blockIdx.x * kStatWords);
 }
 
 __global__ void __launch_bounds__(256)
 k_grad_softmax_pos(const float* __restrict__ F, const int32_t* __restrict__ y, const float* __restrict__ w,
-                   int64_t n, int K, const int32_t* __restrict__ perm0, float2* __restrict__ gh,
-                   int32_t* __restrict__ stats) {
+                   int64_t n, int K, const int32_t* __restrict__ perm0, float2* __restrict__ gh) {
   const int64_t p0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const bool ok = p0 < n;
   const int64_t p = ok ? p0 : n - 1;
@@ -188,7 +194,6 @@ k_grad_softmax_pos(const float* __restrict__ F, const int32_t* __restrict__ y, c
     const float gk = -(pk - (yr == k ? 1.0f : 0.0f)) * wr;
     const float hk = fmaxf(pk * (1.0f - pk), 1e-6f) * wr;
     if (ok) gh[(int64_t)k * n + p] = make_float2(gk, hk);
-    stats_add_wave(stats + k * kStatWords, ok, gk, hk);
   }
 }
 
@@ -304,17 +309,23 @@ extern "C" int wgbm_fit_rounds(const wgbm_fit_config* cfg, const wgbm_fit_io* io
     hipLaunchKernelGGL(k_root_tables, dim3(1), dim3(64), 0, st, io->era_count, E, K, (int)N, I(L.era_start),
                        I(L.root_seg_start), I(L.root_seg_count), I(L.root_seg_slot), I(L.root_nseg),
                        I(L.tab[0].seg_start), I(L.tab[0].seg_count), I(L.tab[0].node_active),
-                       I(L.tab[0].node_start), I(L.tab[0].node_count), I(L.tab[0].stats));
+                       I(L.tab[0].node_start), I(L.tab[0].node_count));
     WGBM_LAUNCHED("k_root_tables");
+    const int nsb = stats_blocks(N);
     if (c.objective == 0) {
-      hipLaunchKernelGGL(k_grad_regression_pos, dim3((unsigned)ceil_div(N, 256)), dim3(256), 0, st, io->y,
-                         io->scores, io->weight, N, I(L.perm_root), G2(L.gh_root), I(L.tab[0].stats));
+      hipLaunchKernelGGL(k_grad_regression_pos, dim3((unsigned)nsb), dim3(256), 0, st, io->y, io->scores, io->weight,
+                         N, I(L.perm_root), G2(L.gh_root), I(L.stat_part));
       WGBM_LAUNCHED("k_grad_regression_pos");
     } else {
       hipLaunchKernelGGL(k_grad_softmax_pos, dim3((unsigned)ceil_div(N, 256)), dim3(256), 0, st, io->scores,
-                         io->labels, io->weight, N, K, I(L.perm_root), G2(L.gh_root), I(L.tab[0].stats));
+                         io->labels, io->weight, N, K, I(L.perm_root), G2(L.gh_root));
       WGBM_LAUNCHED("k_grad_softmax_pos");
+      hipLaunchKernelGGL(k_gh_stats, dim3((unsigned)nsb, (unsigned)K), dim3(256), 0, st, G2(L.gh_root), N,
+                         I(L.stat_part));
+      WGBM_LAUNCHED("k_gh_stats");
     }
+    hipLaunchKernelGGL(k_stats_reduce, dim3((unsigned)K), dim3(256), 0, st, I(L.stat_part), nsb, I(L.tab[0].stats));
+    WGBM_LAUNCHED("k_stats_reduce");
     const int64_t out_base = (int64_t)r * K * NH;
     for (int d = 0; d <= D; ++d) {
       const int n_nodes = K << d;
@@ -444,6 +455,7 @@ extern "C" int wgbm_fit_rounds(const wgbm_fit_config* cfg, const wgbm_fit_io* io
         p.c_node_start = I(nxt.node_start);
         p.c_node_count = I(nxt.node_count);
         p.c_stats = I(nxt.stats);
+        p.stat_part = I(L.stat_part);
         p.b_seg_start = I(L.b_seg_start);
         p.b_seg_count = I(L.b_seg_count);
         p.b_seg_slot = I(L.b_seg_slot);

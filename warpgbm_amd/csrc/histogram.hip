@@ -387,22 +387,44 @@ k_hist_merge(const HistArgs a, const int32_t* __restrict__ multi,
 }
 
 // Statistics of one row list (reference op): stats[0..3] over g[rows], h[rows].
-__global__ void k_stats_init(int32_t* st, int n_nodes) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j < n_nodes) stats_init(st + (int64_t)j * kStatWords);
+int stats_blocks(int64_t n) {
+  const int64_t b = ceil_div(n, 1024);
+  return (int)(b < 1 ? 1 : b > kStatBlocks ? kStatBlocks : b);
+}
+
+// Partial statistics of a row list (gathered g, h): one per block.
+__global__ void __launch_bounds__(256)
+k_list_stats(const int32_t* __restrict__ rows, int n, int64_t n_rows, const float* __restrict__ g,
+             const float* __restrict__ h, int32_t* __restrict__ part) {
+  StatAcc acc;
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
+    const int r = rows[p];
+    if ((uint64_t)r < (uint64_t)n_rows) acc.add(g[r], h[r]);
+  }
+  acc.block_reduce();
+  if (threadIdx.x == 0) acc.store(part + (int64_t)blockIdx.x * kStatWords);
 }
 
 __global__ void __launch_bounds__(256)
-k_list_stats(const int32_t* __restrict__ rows, int n, int64_t n_rows, const float* __restrict__ g,
-             const float* __restrict__ h, int32_t* __restrict__ st) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  bool ok = false;
-  float gv = 0.0f, hv = 0.0f;
-  if (p < n) {
-    const int r = rows[p];
-    if ((uint64_t)r < (uint64_t)n_rows) { ok = true; gv = g[r]; hv = h[r]; }
+k_gh_stats(const float2* __restrict__ gh, int64_t n, int32_t* __restrict__ part) {
+  const int k = blockIdx.y;
+  const float2* v = gh + (int64_t)k * n;
+  StatAcc acc;
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
+    const float2 x = v[p];
+    acc.add(x.x, x.y);
   }
-  stats_add_wave(st, ok, gv, hv);
+  acc.block_reduce();
+  if (threadIdx.x == 0) acc.store(part + ((int64_t)k * gridDim.x + blockIdx.x) * kStatWords);
+}
+
+__global__ void __launch_bounds__(256)
+k_stats_reduce(const int32_t* __restrict__ part, int n_part, int32_t* __restrict__ out) {
+  const int32_t* p = part + (int64_t)blockIdx.x * n_part * kStatWords;
+  StatAcc acc;
+  for (int i = threadIdx.x; i < n_part; i += blockDim.x) acc.load_merge(p + (int64_t)i * kStatWords);
+  acc.block_reduce();
+  if (threadIdx.x == 0) acc.store(out + (int64_t)blockIdx.x * kStatWords);
 }
 
 // ---------------------------------------------------------------------------
@@ -730,8 +752,8 @@ extern "C" int wgbm_profile_end(double* out4) {
 
 // Workspace carve-up for the reference-style histogram op.
 struct Hist3Layout {
-  size_t seg_start, seg_count, seg_slot, seg_first, seg_pfirst, nseg, multi, n_multi, sorted, cnt, stats, part,
-      total;
+  size_t seg_start, seg_count, seg_slot, seg_first, seg_pfirst, nseg, multi, n_multi, sorted, cnt, stats, stat_part,
+      part, total;
 };
 
 static Hist3Layout hist3_layout(int64_t n, int n_feat, int n_bins, int n_eras, const HistConfig& c) {
@@ -752,6 +774,7 @@ static Hist3Layout hist3_layout(int64_t n, int n_feat, int n_bins, int n_eras, c
   L.sorted = o;    o = al(o + (E > 1 ? 4 * n : 0));
   L.cnt = o;       o = al(o + (E > 1 ? 4 * nb * E : 0));
   L.stats = o;     o = al(o + 4 * kStatWords);
+  L.stat_part = o; o = al(o + 4 * kStatWords * kStatBlocks);
   L.part = o;      o = al(o + sizeof(longlong2) * hist_partial_words(c, n_feat, n_bins, n));
   L.total = o;
   return L;
@@ -807,11 +830,12 @@ extern "C" int wgbm_compute_histogram3(const int8_t* bin_indices, int64_t n_tota
   a.slots_per_node = n_eras;
   a.accumulate = 1;
   const int n = (int)n_samples;
-  hipLaunchKernelGGL(k_stats_init, dim3(1), dim3(64), 0, stream, P(L.stats), 1);
-  WGBM_LAUNCHED("k_stats_init");
-  hipLaunchKernelGGL(k_list_stats, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, stream, sample_indices, n,
-                     n_total_rows, grads, hess, P(L.stats));
+  const int nsb = stats_blocks(n);
+  hipLaunchKernelGGL(k_list_stats, dim3((unsigned)nsb), dim3(256), 0, stream, sample_indices, n, n_total_rows, grads,
+                     hess, P(L.stat_part));
   WGBM_LAUNCHED("k_list_stats");
+  hipLaunchKernelGGL(k_stats_reduce, dim3(1), dim3(256), 0, stream, P(L.stat_part), nsb, P(L.stats));
+  WGBM_LAUNCHED("k_stats_reduce");
   if (n_eras == 1) {
     a.perm = sample_indices;
     hipLaunchKernelGGL(k_single_segment, dim3(1), dim3(64), 0, stream, P(L.seg_start), P(L.seg_count),

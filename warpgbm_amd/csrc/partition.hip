@@ -75,8 +75,6 @@ k_part_scan(PartArgs a) {
   }
   __syncthreads();
   // B: child tables per node.
-  if (a.c_stats)
-    for (int c = threadIdx.x; c < 2 * a.n_nodes; c += blockDim.x) stats_init(a.c_stats + (int64_t)c * kStatWords);
   for (int j = threadIdx.x; j < a.n_nodes; j += blockDim.x) {
     const bool split = a.node_active[j] && a.dec_feat[j] >= 0;
     const int l = 2 * j, r = 2 * j + 1;
@@ -158,6 +156,7 @@ k_part_scatter(PartArgs a) {
   __shared__ int s_l[4];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   int lrun = a.lbase[chunk], rrun = a.rbase[chunk];
+  StatAcc sl, sr;  // statistics of the chunk's left and right rows
   for (int base = c0; base < c1; base += 256) {
     const int p = base + threadIdx.x;
     const bool ok = p < c1;
@@ -183,15 +182,36 @@ k_part_scatter(PartArgs a) {
       a.perm_out[dst] = row;
       if (a.gh_out) a.gh_out[dst] = v;
     }
-    if (a.c_stats) {  // statistics of each child's rows (for its histogram)
-      const int j = s / a.n_eras;
-      stats_add_wave(a.c_stats + (int64_t)(2 * j) * kStatWords, ok && left, v.x, v.y);
-      stats_add_wave(a.c_stats + (int64_t)(2 * j + 1) * kStatWords, ok && !left, v.x, v.y);
+    if (ok) {
+      if (left) sl.add(v.x, v.y);
+      else sr.add(v.x, v.y);
     }
     lrun += ltot;
     rrun += n_here - ltot;
     __syncthreads();
   }
+  if (a.c_stats) {  // one partial per chunk and side, folded per child by k_child_stats
+    sl.block_reduce();
+    sr.block_reduce();
+    if (threadIdx.x == 0) {
+      sl.store(a.stat_part + ((int64_t)chunk * 2 + 0) * kStatWords);
+      sr.store(a.stat_part + ((int64_t)chunk * 2 + 1) * kStatWords);
+    }
+  }
+}
+
+// Child c of node j = c / 2 (side c % 2) folds the partials of j's chunks,
+// which are contiguous: [seg_first[j*E], seg_first[(j+1)*E]).
+__global__ void __launch_bounds__(256)
+k_child_stats(PartArgs a) {
+  const int c = blockIdx.x;
+  const int j = c >> 1, side = c & 1;
+  const int first = a.seg_first[j * a.n_eras], last = a.seg_first[(j + 1) * a.n_eras];
+  StatAcc acc;
+  for (int k = first + threadIdx.x; k < last; k += blockDim.x)
+    acc.load_merge(a.stat_part + ((int64_t)k * 2 + side) * kStatWords);
+  acc.block_reduce();
+  if (threadIdx.x == 0) acc.store(a.c_stats + (int64_t)c * kStatWords);
 }
 
 int partition_launch(const PartArgs& a, int64_t rows_max, hipStream_t st) {
@@ -214,6 +234,10 @@ int partition_launch(const PartArgs& a, int64_t rows_max, hipStream_t st) {
   WGBM_LAUNCHED("k_part_scan");
   hipLaunchKernelGGL(k_part_scatter, dim3((unsigned)max_chunks), dim3(256), 0, st, a);
   WGBM_LAUNCHED("k_part_scatter");
+  if (a.c_stats) {
+    hipLaunchKernelGGL(k_child_stats, dim3((unsigned)(2 * a.n_nodes)), dim3(256), 0, st, a);
+    WGBM_LAUNCHED("k_child_stats");
+  }
   return kOk;
 }
 

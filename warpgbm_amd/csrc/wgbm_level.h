@@ -40,6 +40,7 @@ struct PartArgs {
   int32_t* c_node_start;
   int32_t* c_node_count;
   int32_t* c_stats;             // per child: gradient statistics (nullable)
+  int32_t* stat_part;           // [max chunks][2][kStatWords] scratch when c_stats is set
   // smaller-child build list ([n_nodes*E]) and sibling map ([3*n_nodes])
   int32_t* b_seg_start;
   int32_t* b_seg_count;

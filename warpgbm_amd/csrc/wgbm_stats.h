@@ -32,25 +32,46 @@ __device__ __forceinline__ void stats_init(int32_t* s) {
   s[3] = 0;
 }
 
-// Fold one wave's values into a node's stats (lanes with ok == false skip).
-__device__ __forceinline__ void stats_add_wave(int32_t* s, bool ok, float g, float h) {
-  int ga = ok ? float_ord(fabsf(g)) : 0;
-  int hmn = ok ? float_ord(h) : 0x7fffffff;
-  int hmx = ok ? float_ord(h) : (int)0x80000000;
-  int ha = ok ? float_ord(fabsf(h)) : 0;
-  for (int o = 32; o >= 1; o >>= 1) {
-    ga = max(ga, __shfl_xor(ga, o));
-    hmn = min(hmn, __shfl_xor(hmn, o));
-    hmx = max(hmx, __shfl_xor(hmx, o));
-    ha = max(ha, __shfl_xor(ha, o));
+// A thread's running statistics.  Blocks reduce theirs and store one
+// partial per block (plain stores: thousands of same-address atomics from
+// every wave of a grid serialise at ~10 ns each); k_stats_reduce then folds
+// the partials of each node.
+struct StatAcc {
+  int ga = 0, hmn = 0x7fffffff, hmx = (int)0x80000000, ha = 0;
+
+  __device__ __forceinline__ void add(float g, float h) {
+    ga = max(ga, float_ord(fabsf(g)));
+    hmn = min(hmn, float_ord(h));
+    hmx = max(hmx, float_ord(h));
+    ha = max(ha, float_ord(fabsf(h)));
   }
-  if ((threadIdx.x & 63) == 0 && hmn <= hmx) {  // some lane had a row
-    atomicMax(s + 0, ga);
-    atomicMin(s + 1, hmn);
-    atomicMax(s + 2, hmx);
-    atomicMax(s + 3, ha);
+  __device__ __forceinline__ void merge(int a0, int a1, int a2, int a3) {
+    ga = max(ga, a0);
+    hmn = min(hmn, a1);
+    hmx = max(hmx, a2);
+    ha = max(ha, a3);
   }
-}
+  __device__ __forceinline__ void load_merge(const int32_t* s) { merge(s[0], s[1], s[2], s[3]); }
+  __device__ __forceinline__ void store(int32_t* s) const {
+    s[0] = ga; s[1] = hmn; s[2] = hmx; s[3] = ha;
+  }
+  __device__ __forceinline__ void wave_reduce() {
+    for (int o = 32; o >= 1; o >>= 1)
+      merge(__shfl_xor(ga, o), __shfl_xor(hmn, o), __shfl_xor(hmx, o), __shfl_xor(ha, o));
+  }
+  // Whole-block reduction (blockDim.x a multiple of 64, at most 1024; every
+  // thread calls it).  Thread 0 holds the result.
+  __device__ __forceinline__ void block_reduce() {
+    __shared__ int32_t sh[16][4];
+    wave_reduce();
+    const int w = threadIdx.x >> 6;
+    __syncthreads();  // sh may still be read by a previous call
+    if ((threadIdx.x & 63) == 0) store(sh[w]);
+    __syncthreads();
+    if (threadIdx.x == 0)
+      for (int i = 1; i < (int)(blockDim.x >> 6); ++i) load_merge(sh[i]);
+  }
+};
 
 // Shift for values bounded by 2^bits: largest s with max * 2^s < 2^bits.
 __device__ __forceinline__ int fixed_shift(float maxabs, int bits) {
@@ -73,5 +94,13 @@ constexpr int kHBits = 48;
 // complement in the low 44 bits; a chunk's |sum| < 2^(28+14) < 2^43).
 constexpr int kCountShift = 44;
 constexpr int kMaxChunkRows = 1 << 14;
+
+// Two-stage statistics: at most kStatBlocks per-block partials per node
+// (stats_blocks(rows) of them), folded by k_stats_reduce (one block per node;
+// part is [node][n_part][kStatWords]).
+constexpr int kStatBlocks = 1024;
+int stats_blocks(int64_t n);
+__global__ void k_gh_stats(const float2* __restrict__ gh, int64_t n, int32_t* __restrict__ part);
+__global__ void k_stats_reduce(const int32_t* __restrict__ part, int n_part, int32_t* __restrict__ out);
 
 }  // namespace wgbm
