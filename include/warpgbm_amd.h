@@ -191,6 +191,11 @@ size_t wgbm_fit_workspace_bytes(const wgbm_fit_config* config);
  * grow_tree (:1364-1470). */
 int wgbm_fit_rounds(const wgbm_fit_config* config, const wgbm_fit_io* io, int32_t n_rounds, void* workspace,
                     size_t workspace_bytes, hipStream_t stream);
+/* Rounds of the last wgbm_fit_rounds call on this thread that were replayed
+ * from a captured graph (one round is captured once, then relaunched; 0 when
+ * the call ran eagerly: fewer than 3 rounds, or histogram timing on).
+ * Diagnostic; no counterpart in the reference. */
+int32_t wgbm_fit_graph_rounds(void);
 
 /* ---------------------------------------------------------------- serving predictor */
 

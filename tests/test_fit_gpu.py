@@ -156,3 +156,28 @@ def test_multiclass_batched_equals_plain():
     pb = mb.predict_proba(X)
     assert np.array_equal(pb, mp.predict_proba(X))
     np.testing.assert_allclose(pb.sum(axis=1), 1.0, atol=1e-5)
+
+
+def test_graph_replay_equals_eager():
+    """Rounds replayed from one captured graph (device-side round counter,
+    per-round feature subsets) give the trees of the eager launch sequence;
+    histogram timing forces the eager path."""
+    import ctypes
+    from warpgbm_amd import WarpGBM, _lib
+    X, y = _data_cfg1(n=8000, f=30, seed=3)
+    kw = dict(objective="regression", num_bins=32, max_depth=4, n_estimators=7, learning_rate=0.1,
+              colsample_bytree=0.6)
+    torch.manual_seed(11)
+    mg = WarpGBM(engine="batched", **kw).fit(X, y)
+    assert _lib.query("wgbm_fit_graph_rounds") == 6
+    _lib.call("wgbm_profile_begin")
+    try:
+        torch.manual_seed(11)
+        me = WarpGBM(engine="batched", **kw).fit(X, y)
+        assert _lib.query("wgbm_fit_graph_rounds") == 0
+    finally:
+        _lib.call("wgbm_profile_end", (ctypes.c_double * 4)())
+    for i in range(7):
+        msg = _trees_equal(mg.forest[i], me.forest[i], f"tree{i}:")
+        assert msg is None, msg
+    assert np.array_equal(mg.predict(X), me.predict(X))

@@ -879,25 +879,42 @@ class WarpGBM(BaseEstimator, RegressorMixin):
         return tables
 
     def _append_rounds(self, tables, n, K, feats):
+        """Node tables of n rounds -> the reference's nested tree dicts, and
+        per-era importances (gain of every split added to its feature)."""
         st, ft, bt, vt, ct, eg = (tables[k] for k in ("status", "feat", "bin", "value", "count", "era_gain"))
-        imp = self.per_era_feature_importance_
+        order = []  # (round, class, node) of every split in growth (depth-first) order
         ids = self._feature_ids
+        views = {}  # 0-d views into the feature-id vector, one per feature
 
-        def build(r, c, h):
-            s = st[r, c, h]
-            if s == 1:
-                f = int(ft[r, c, h])
-                imp[:, f] += eg[r, c, h]
-                left = build(r, c, 2 * h + 1)
-                right = build(r, c, 2 * h + 2)
-                return {"feature": ids[f], "bin": int(bt[r, c, h]), "left": left, "right": right}
-            return {"leaf_value": float(vt[r, c, h]), "samples": int(ct[r, c, h])}
+        def view(f):
+            v = views.get(f)
+            if v is None:
+                v = views[f] = ids[f]
+            return v
 
         for r in range(n):
+            trees = []
+            for c in range(K):
+                s_, f_, b_ = st[r, c].tolist(), ft[r, c].tolist(), bt[r, c].tolist()
+                v_, n_ = vt[r, c].tolist(), ct[r, c].tolist()
+
+                def build(h):
+                    if s_[h] == 1:
+                        order.append((r, c, h))
+                        return {"feature": view(f_[h]), "bin": b_[h], "left": build(2 * h + 1),
+                                "right": build(2 * h + 2)}
+                    return {"leaf_value": v_[h], "samples": n_[h]}
+
+                trees.append(build(0))
             if self.objective == "regression":
-                self.forest[self._trees_trained + r] = build(r, 0, 0)
+                self.forest[self._trees_trained + r] = trees[0]
             else:
-                self.forest.append([build(r, c, 0) for c in range(K)])
+                self.forest.append(trees)
+        if order:
+            # gains added split by split in growth order, as the per-node path does
+            o = np.asarray(order)
+            np.add.at(self.per_era_feature_importance_.T, ft[o[:, 0], o[:, 1], o[:, 2]],
+                      eg[o[:, 0], o[:, 1], o[:, 2]])
 
     # ------------------------------------------------------------ evaluation
     def get_eval_metric(self, y_true, y_pred, sample_weight=None):

@@ -33,9 +33,9 @@ struct FitLayout {
   struct Tables { size_t seg_start, seg_count, node_active, node_start, node_count, stats; } tab[2];
   size_t dec_feat, dec_bin, era_gain, cand_agree, cand_crit, cand_idx;
   size_t b_seg_start, b_seg_count, b_seg_slot, n_build, sib, n_split;
-  size_t h_seg_first, h_seg_pfirst, h_nseg, h_multi, h_n_multi, h_part;
-  size_t stat_part;
-  size_t p_flag, p_seg_first, p_nseg, p_multi, p_n_multi, p_lbase, p_rbase, p_segL;
+  size_t h_seg_first, h_seg_pfirst, h_nseg, h_multi, h_n_multi, h_chunk, h_part;
+  size_t stat_part, round_ctr, fidx_cur;
+  size_t p_flag, p_seg_first, p_nseg, p_multi, p_n_multi, p_lbase, p_rbase, p_cscan, p_segL, codes_t;
   size_t l_seg_first, l_nseg, l_multi, l_n_multi, l_part, l_val;
   size_t root_seg_start, root_seg_count, root_seg_slot, root_nseg, era_start;
   // sizes
@@ -76,7 +76,7 @@ static FitLayout fit_layout(const wgbm_fit_config& c) {
     L.tab[i].node_count = take(4 * L.max_nodes);
     L.tab[i].stats = take(4 * kStatWords * L.max_nodes);
   }
-  const int64_t wgpn = ceil_div(c.n_feat, 256) > 0 ? ceil_div(c.n_feat, 256) : 1;
+  const int64_t wgpn = level_scan_groups(c.n_feat);
   L.dec_feat = take(4 * L.max_nodes);
   L.dec_bin = take(4 * L.max_nodes);
   L.era_gain = take(4 * MS);
@@ -92,8 +92,9 @@ static FitLayout fit_layout(const wgbm_fit_config& c) {
   L.h_seg_first = take(4 * (MS + 1));
   L.h_seg_pfirst = take(4 * (MS + 1));
   L.h_nseg = take(4);
-  L.h_multi = take(4 * (L.KN / (L.hc.chunk_rows + 1) + 2));
+  L.h_multi = take(4 * hist_max_multi(L.hc, c.n_feat, L.KN));
   L.h_n_multi = take(4);
+  L.h_chunk = take(4);
   L.hist_part_floats = (int64_t)hist_partial_words(L.hc, c.n_feat, c.n_bins, L.KN);
   L.h_part = take(sizeof(longlong2) * L.hist_part_floats);
   L.part_chunks = ceil_div(L.KN, kPartChunk) + MS;
@@ -104,11 +105,15 @@ static FitLayout fit_layout(const wgbm_fit_config& c) {
   L.p_n_multi = take(4);
   L.p_lbase = take(4 * L.part_chunks);
   L.p_rbase = take(4 * L.part_chunks);
+  L.p_cscan = take(4 * (L.part_chunks + 1));
+  L.codes_t = take((size_t)N * c.n_cols);  // column-major copy of the bin matrix
   L.p_segL = take(4 * MS);
   // statistics partials: per block of the objective (K classes) or per
   // partition chunk and child side
   const int64_t stat_parts = (int64_t)K * kStatBlocks > 2 * L.part_chunks ? (int64_t)K * kStatBlocks : 2 * L.part_chunks;
   L.stat_part = take(4 * kStatWords * stat_parts);
+  L.round_ctr = take(4 * 2);           // {round being built, next round}
+  L.fidx_cur = take(4 * (int64_t)c.n_feat);  // this round's feature subset
   L.leaf_chunks = leaf_max_chunks(L.KN, L.max_nodes);
   L.l_seg_first = take(4 * (L.max_nodes + 1));
   L.l_nseg = take(4);
@@ -125,12 +130,22 @@ static FitLayout fit_layout(const wgbm_fit_config& c) {
   return L;
 }
 
-// Root tables: class c occupies positions [c*N, (c+1)*N) in root order.
+// Start of a round: advance the round counter, copy the round's feature
+// subset, and write the root tables (class c occupies positions
+// [c*N, (c+1)*N) in root order).  The round index lives on the device so
+// that one captured round can be replayed for every round.
 __global__ void k_root_tables(const int32_t* __restrict__ era_count, int E, int K, int N,
                               int32_t* era_start, int32_t* seg_start, int32_t* seg_count, int32_t* seg_slot,
                               int32_t* nseg, int32_t* t_seg_start, int32_t* t_seg_count, int32_t* t_active,
-                              int32_t* t_node_start, int32_t* t_node_count) {
+                              int32_t* t_node_start, int32_t* t_node_count, int32_t* round_ctr,
+                              const int32_t* __restrict__ feat_all, int n_feat, int32_t* fidx_cur) {
+  const int r = round_ctr[1];
+  if (feat_all)
+    for (int i = threadIdx.x; i < n_feat; i += blockDim.x) fidx_cur[i] = feat_all[(int64_t)r * n_feat + i];
+  __syncthreads();  // every thread has read round_ctr[1]
   if (threadIdx.x != 0) return;
+  round_ctr[0] = r;
+  round_ctr[1] = r + 1;
   int run = 0;
   for (int e = 0; e < E; ++e) { era_start[e] = run; run += era_count[e]; }
   for (int c = 0; c < K; ++c) {
@@ -147,6 +162,32 @@ __global__ void k_root_tables(const int32_t* __restrict__ era_count, int E, int 
     t_node_count[c] = run;
   }
   *nseg = K * E;
+}
+
+// codes_t[c][r] = codes[r][c], through 64 x 64 LDS tiles (row-contiguous
+// reads, column-contiguous writes).
+__global__ void __launch_bounds__(256)
+k_transpose_codes(const int8_t* __restrict__ codes, int64_t n_rows, int64_t n_cols, int8_t* __restrict__ codes_t) {
+  __shared__ int8_t tile[64][65];
+  const int64_t r0 = (int64_t)blockIdx.x * 64, c0 = (int64_t)blockIdx.y * 64;
+  const int t = threadIdx.x;
+  {
+    const int r = t >> 2, q = (t & 3) * 16;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      const int64_t rr = r0 + r, cc = c0 + q + k;
+      tile[r][q + k] = (rr < n_rows && cc < n_cols) ? codes[rr * n_cols + cc] : (int8_t)0;
+    }
+  }
+  __syncthreads();
+  {
+    const int c = t >> 2, q = (t & 3) * 16;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      const int64_t rr = r0 + q + k, cc = c0 + c;
+      if (rr < n_rows && cc < n_cols) codes_t[cc * n_rows + rr] = tile[q + k][c];
+    }
+  }
 }
 
 __global__ void __launch_bounds__(256)
@@ -202,13 +243,14 @@ __global__ void __launch_bounds__(256)
 k_record_level(int d, int D, int K, int E, int n_nodes, const int32_t* __restrict__ active,
                const int32_t* __restrict__ dec_feat, const int32_t* __restrict__ dec_bin,
                const int32_t* __restrict__ feat_idx, const float* __restrict__ leaf_val,
-               const int32_t* __restrict__ node_count, const float* __restrict__ era_gain, int32_t* status,
-               int32_t* feat, int32_t* bin, float* value, int32_t* count, float* out_era_gain) {
+               const int32_t* __restrict__ node_count, const float* __restrict__ era_gain,
+               const int32_t* __restrict__ round_ctr, int32_t* status, int32_t* feat, int32_t* bin, float* value,
+               int32_t* count, float* out_era_gain) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n_nodes) return;
   const int NH = (2 << D) - 1;
   const int c = j >> d, jj = j & ((1 << d) - 1);
-  const int64_t o = (int64_t)c * NH + ((1 << d) - 1) + jj;
+  const int64_t o = ((int64_t)round_ctr[0] * K + c) * NH + ((1 << d) - 1) + jj;
   int st = 0, f = -1, b = -1;
   float v = 0.0f;
   if (active[j]) {
@@ -281,35 +323,25 @@ extern "C" size_t wgbm_fit_workspace_bytes(const wgbm_fit_config* c) {
   return fit_layout(*c).total;
 }
 
-extern "C" int wgbm_fit_rounds(const wgbm_fit_config* cfg, const wgbm_fit_io* io, int32_t n_rounds,
-                               void* workspace, size_t workspace_bytes, hipStream_t st) {
-  int rc = check_config(cfg);
-  if (rc != kOk) return rc;
-  const wgbm_fit_config& c = *cfg;
-  const FitLayout L = fit_layout(c);
-  WGBM_REQUIRE(workspace_bytes >= L.total, "fit workspace too small: need %zu bytes", L.total);
-  WGBM_REQUIRE(io && io->codes && io->weight && io->scores && io->era_count, "missing fit inputs");
-  WGBM_REQUIRE(c.objective == 1 ? io->labels != nullptr : io->y != nullptr, "missing targets");
-  WGBM_REQUIRE(io->feat_idx != nullptr || c.n_feat <= c.n_cols, "feature subset exceeds the bin matrix");
-  char* ws = static_cast<char*>(workspace);
+// One boosting round (K class trees), every launch on stream st.  Nothing
+// here depends on the round index on the host: the round counter, the
+// feature subset and the output offsets are read on the device, so the
+// launches of one round can be captured once and replayed.
+static int enqueue_round(const wgbm_fit_config& c, const FitLayout& L, const wgbm_fit_io* io, char* ws,
+                         hipStream_t st) {
+  int rc = kOk;
   auto I = [&](size_t off) { return reinterpret_cast<int32_t*>(ws + off); };
   auto Fp = [&](size_t off) { return reinterpret_cast<float*>(ws + off); };
   auto G2 = [&](size_t off) { return reinterpret_cast<float2*>(ws + off); };
   const int K = c.n_classes, E = c.n_eras, D = c.max_depth;
   const int64_t N = c.n_rows;
-  const int NH = (2 << D) - 1;
-
-  hipLaunchKernelGGL(k_root_perm, dim3((unsigned)ceil_div(L.KN, 256)), dim3(256), 0, st, io->root_order, N, K,
-                     I(L.perm_root));
-  WGBM_LAUNCHED("k_root_perm");
-
-  for (int r = 0; r < n_rounds; ++r) {
-    const int32_t* fidx = io->feat_idx ? io->feat_idx + (int64_t)r * c.n_feat : nullptr;
+  const int32_t* fidx = io->feat_idx ? I(L.fidx_cur) : nullptr;
     // level-0 tables (rewritten every round: partition overwrites table 0 at level 1)
-    hipLaunchKernelGGL(k_root_tables, dim3(1), dim3(64), 0, st, io->era_count, E, K, (int)N, I(L.era_start),
+    hipLaunchKernelGGL(k_root_tables, dim3(1), dim3(256), 0, st, io->era_count, E, K, (int)N, I(L.era_start),
                        I(L.root_seg_start), I(L.root_seg_count), I(L.root_seg_slot), I(L.root_nseg),
                        I(L.tab[0].seg_start), I(L.tab[0].seg_count), I(L.tab[0].node_active),
-                       I(L.tab[0].node_start), I(L.tab[0].node_count));
+                       I(L.tab[0].node_start), I(L.tab[0].node_count), I(L.round_ctr), io->feat_idx, c.n_feat,
+                       I(L.fidx_cur));
     WGBM_LAUNCHED("k_root_tables");
     const int nsb = stats_blocks(N);
     if (c.objective == 0) {
@@ -326,7 +358,6 @@ extern "C" int wgbm_fit_rounds(const wgbm_fit_config* cfg, const wgbm_fit_io* io
     }
     hipLaunchKernelGGL(k_stats_reduce, dim3((unsigned)K), dim3(256), 0, st, I(L.stat_part), nsb, I(L.tab[0].stats));
     WGBM_LAUNCHED("k_stats_reduce");
-    const int64_t out_base = (int64_t)r * K * NH;
     for (int d = 0; d <= D; ++d) {
       const int n_nodes = K << d;
       const FitLayout::Tables& cur = L.tab[d & 1];
@@ -347,6 +378,7 @@ extern "C" int wgbm_fit_rounds(const wgbm_fit_config* cfg, const wgbm_fit_io* io
         a.gh = gh_d;
         a.seg_first = I(L.h_seg_first);
         a.seg_pfirst = I(L.h_seg_pfirst);
+        a.chunk_dev = I(L.h_chunk);
         a.out_g = hg;
         a.out_h = hh;
         a.part = reinterpret_cast<longlong2*>(ws + L.h_part);
@@ -418,9 +450,8 @@ extern "C" int wgbm_fit_rounds(const wgbm_fit_config* cfg, const wgbm_fit_io* io
       if (rc != kOk) return rc;
       hipLaunchKernelGGL(k_record_level, dim3((unsigned)ceil_div(n_nodes, 256)), dim3(256), 0, st, d, D, K, E,
                          n_nodes, I(cur.node_active), I(L.dec_feat), I(L.dec_bin), fidx, Fp(L.l_val),
-                         I(cur.node_count), Fp(L.era_gain), io->node_status + out_base, io->node_feat + out_base,
-                         io->node_bin + out_base, io->node_value + out_base, io->node_count + out_base,
-                         io->node_era_gain ? io->node_era_gain + out_base * E : nullptr);
+                         I(cur.node_count), Fp(L.era_gain), I(L.round_ctr), io->node_status, io->node_feat,
+                         io->node_bin, io->node_value, io->node_count, io->node_era_gain);
       WGBM_LAUNCHED("k_record_level");
       if (d < D) {
         PartArgs p{};
@@ -448,6 +479,8 @@ extern "C" int wgbm_fit_rounds(const wgbm_fit_config* cfg, const wgbm_fit_io* io
         p.n_multi = I(L.p_n_multi);
         p.lbase = I(L.p_lbase);
         p.rbase = I(L.p_rbase);
+        p.cscan = I(L.p_cscan);
+        p.codes_t = reinterpret_cast<const int8_t*>(ws + L.codes_t);
         p.segL = I(L.p_segL);
         p.c_seg_start = I(nxt.seg_start);
         p.c_seg_count = I(nxt.seg_count);
@@ -467,6 +500,102 @@ extern "C" int wgbm_fit_rounds(const wgbm_fit_config* cfg, const wgbm_fit_io* io
         if (rc != kOk) return rc;
       }
     }
+  return kOk;
+}
+
+__global__ void k_round_init(int32_t* round_ctr, int r0) {
+  if (threadIdx.x == 0) { round_ctr[0] = r0; round_ctr[1] = r0; }
+}
+
+// Rounds replayed from one captured round need at least this many rounds.
+constexpr int kGraphMinRounds = 3;
+static thread_local int g_graph_rounds = 0;  // rounds of the last call replayed from a graph
+
+extern "C" int32_t wgbm_fit_graph_rounds(void) { return g_graph_rounds; }
+
+// Non-blocking stream (default priority) per device for graph replays.
+static hipStream_t graph_stream() {
+  static hipStream_t streams[64] = {};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return nullptr;
+  if (!streams[dev] && hipStreamCreateWithFlags(&streams[dev], hipStreamNonBlocking) != hipSuccess)
+    streams[dev] = nullptr;
+  return streams[dev];
+}
+
+extern "C" int wgbm_fit_rounds(const wgbm_fit_config* cfg, const wgbm_fit_io* io, int32_t n_rounds,
+                               void* workspace, size_t workspace_bytes, hipStream_t st) {
+  int rc = check_config(cfg);
+  if (rc != kOk) return rc;
+  const wgbm_fit_config& c = *cfg;
+  const FitLayout L = fit_layout(c);
+  WGBM_REQUIRE(workspace_bytes >= L.total, "fit workspace too small: need %zu bytes", L.total);
+  WGBM_REQUIRE(io && io->codes && io->weight && io->scores && io->era_count, "missing fit inputs");
+  WGBM_REQUIRE(c.objective == 1 ? io->labels != nullptr : io->y != nullptr, "missing targets");
+  WGBM_REQUIRE(io->feat_idx != nullptr || c.n_feat <= c.n_cols, "feature subset exceeds the bin matrix");
+  char* ws = static_cast<char*>(workspace);
+  auto I = [&](size_t off) { return reinterpret_cast<int32_t*>(ws + off); };
+  const int K = c.n_classes;
+  const int64_t N = c.n_rows;
+
+  hipLaunchKernelGGL(k_root_perm, dim3((unsigned)ceil_div(L.KN, 256)), dim3(256), 0, st, io->root_order, N, K,
+                     I(L.perm_root));
+  WGBM_LAUNCHED("k_root_perm");
+  hipLaunchKernelGGL(k_transpose_codes, dim3((unsigned)ceil_div(N, 64), (unsigned)ceil_div(c.n_cols, 64)), dim3(256),
+                     0, st, io->codes, N, c.n_cols, reinterpret_cast<int8_t*>(ws + L.codes_t));
+  WGBM_LAUNCHED("k_transpose_codes");
+
+  g_graph_rounds = 0;
+  hipLaunchKernelGGL(k_round_init, dim3(1), dim3(64), 0, st, I(L.round_ctr), 0);
+  WGBM_LAUNCHED("k_round_init");
+  if (n_rounds < kGraphMinRounds || hist_timer_on()) {
+    for (int r = 0; r < n_rounds; ++r) {
+      rc = enqueue_round(c, L, io, ws, st);
+      if (rc != kOk) return rc;
+    }
+    return kOk;
   }
+  // Round 0 runs eagerly (it also sets kernel attributes); round 1 is
+  // captured into a graph on a side stream and replayed for rounds 1..n-1.
+  rc = enqueue_round(c, L, io, ws, st);
+  if (rc != kOk) return rc;
+  hipStream_t gs = graph_stream();
+  WGBM_REQUIRE(gs != nullptr, "could not create the graph stream");
+  hipEvent_t ev_in = nullptr, ev_out = nullptr;
+  WGBM_HIP(hipEventCreateWithFlags(&ev_in, hipEventDisableTiming));
+  WGBM_HIP(hipEventCreateWithFlags(&ev_out, hipEventDisableTiming));
+  WGBM_HIP(hipEventRecord(ev_in, st));
+  WGBM_HIP(hipStreamWaitEvent(gs, ev_in, 0));
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  if (hipStreamBeginCapture(gs, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+    const int crc = enqueue_round(c, L, io, ws, gs);
+    const hipError_t ce = hipStreamEndCapture(gs, &graph);
+    if (crc == kOk && ce == hipSuccess && graph &&
+        hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess)
+      exec = nullptr;
+    if (graph) hipGraphDestroy(graph);
+  }
+  if (exec == nullptr) {
+    // Capture is an optimisation: on any failure run the rounds eagerly.
+    (void)hipGetLastError();
+    for (int r = 1; r < n_rounds; ++r) {
+      rc = enqueue_round(c, L, io, ws, gs);
+      if (rc != kOk) { hipEventDestroy(ev_in); hipEventDestroy(ev_out); return rc; }
+    }
+  }
+  for (int r = 1; exec && r < n_rounds; ++r) {
+    const hipError_t le = hipGraphLaunch(exec, gs);
+    if (le != hipSuccess) { hipGraphExecDestroy(exec); WGBM_HIP(le); }
+    ++g_graph_rounds;
+  }
+  WGBM_HIP(hipEventRecord(ev_out, gs));
+  WGBM_HIP(hipStreamWaitEvent(st, ev_out, 0));
+  // The executable graph must outlive its launches: wait for them here
+  // (the caller reads the node tables right after this call anyway).
+  WGBM_HIP(hipEventSynchronize(ev_out));
+  if (exec) hipGraphExecDestroy(exec);
+  hipEventDestroy(ev_in);
+  hipEventDestroy(ev_out);
   return kOk;
 }

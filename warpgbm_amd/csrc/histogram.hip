@@ -90,6 +90,21 @@ k_hist_plan(const PlanSrc src, int32_t chunk_rows, int32_t* __restrict__ seg_fir
   const int nseg = src.nseg_dev ? *src.nseg_dev : src.nseg;
   if (src.nseg_out && threadIdx.x == 0) *src.nseg_out = nseg;
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  if (src.chunk_out) {
+    // Rows per chunk for this launch: enough chunks to give every slot a
+    // work item (chunk_target of them), within [chunk_min, chunk_rows].
+    __shared__ int s_rows;
+    if (t == 0) s_rows = 0;
+    __syncthreads();
+    int mine = 0;
+    for (int s = t; s < nseg; s += blockDim.x) mine += max(plan_count(src, s), 0);
+    for (int o = 32; o >= 1; o >>= 1) mine += __shfl_xor(mine, o);
+    if (lane == 0) atomicAdd(&s_rows, mine);
+    __syncthreads();
+    const int want = (s_rows + src.chunk_target - 1) / src.chunk_target;
+    chunk_rows = min(max((want + 255) & ~255, src.chunk_min), chunk_rows);
+    if (t == 0) *src.chunk_out = chunk_rows;
+  }
   if (t < 3) s_carry[t] = 0;
   __syncthreads();
   for (int base = 0; base < nseg; base += 1024) {
@@ -169,30 +184,34 @@ k_hist_build(const HistArgs a) {
   const int lane = lane_id();
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int W = blockDim.x >> 6;
-  // XCD-aware mapping: dispatch sends block b to XCD b % 8.  All slabs of a
-  // chunk go to the same XCD, back to back, so the rows of a chunk are read
-  // from HBM once into that XCD's L2 and shared by its slabs.
-  const int b = blockIdx.x;
+  const int nseg = *a.nseg_dev;
+  const int total = a.seg_first[nseg];
+  const int C = a.chunk_dev ? *a.chunk_dev : a.chunk_rows;  // rows per chunk of this launch
+  // Work items b = (chunk, slab), taken round-robin by a grid of at most
+  // one workgroup per slot (gridDim.x a multiple of 8).  XCD-aware mapping:
+  // dispatch sends block b to XCD b % 8, and b % 8 is fixed per workgroup,
+  // so all slabs of a chunk run on the same XCD, back to back, and the rows
+  // of a chunk are read from HBM once into that XCD's L2.  chunk grows with
+  // b, so the first item past the last chunk ends the loop.
+  for (int b = blockIdx.x;; b += gridDim.x) {
   const int i = b >> 3;
   const int slab = i % a.nslab;
   const int chunk = ((i / a.nslab) << 3) + (b & 7);
-  const int nseg = *a.nseg_dev;
-  const int total = a.seg_first[nseg];
-  if (chunk >= total) return;
+  if (chunk >= total) break;
   const int s = upper_index(a.seg_first, nseg, chunk);
   const int k = chunk - a.seg_first[s];
   const int nch = a.seg_first[s + 1] - a.seg_first[s];
   const int seg_start = a.seg_start[s];
   const int seg_end = seg_start + max(a.seg_count[s], 0);
-  const int c0 = seg_start + k * a.chunk_rows;
-  const int c1 = min(c0 + a.chunk_rows, seg_end);
+  const int c0 = seg_start + k * C;
+  const int c1 = min(c0 + C, seg_end);
   const int len = max(c1 - c0, 0);
   const int per = (len + W - 1) / W;
   const int w0 = min(c0 + wave * per, c1);
   const int w1 = min(w0 + per, c1);
   const int slot = a.seg_slot[s];
   const NodeShift ns = node_shift(a.stats + (int64_t)(slot / a.slots_per_node) * kStatWords);
-  if (a.dual && ONE_PLANE != ns.count_mode) return;  // the other variant builds this node
+  if (a.dual && ONE_PLANE != ns.count_mode) continue;  // the other variant builds this node
 
   // One shared histogram per workgroup: T plane (count << 44 | sum qg) and,
   // unless the hessian is constant, an H plane (sum qh).  A plane holds
@@ -343,47 +362,84 @@ k_hist_build(const HistArgs a) {
       else { a.out_g[d] = fg; a.out_h[d] = fh; }
     } else {
       const int64_t d = ((int64_t)(a.seg_pfirst[s] + k) * a.nslab + slab) * SL + fl * B + bin;
-      a.part[d] = make_longlong2(gsum, hsum);
+      if constexpr (ONE_PLANE) reinterpret_cast<unsigned long long*>(a.part)[d] = w;  // packed count | qg sum
+      else a.part[d] = make_longlong2(gsum, hsum);
     }
+  }
+  __syncthreads();  // the read-out is done before the next item clears the LDS
   }
 }
 
 // Sum the chunk partials of every multi-chunk segment (exact integers) and
-// write the float32 histogram.
+// write the float32 histogram.  Work items (segment, slab, block of 32
+// elements) are taken round-robin; the 8 groups of 32 threads of a
+// workgroup sum every 8th chunk of the item and reduce through LDS, so a
+// segment of many chunks keeps many loads in flight.
+constexpr int kMergeElems = 32;
+constexpr int kMergeParts = 8;
+
 template <int FPL>
-__global__ void __launch_bounds__(256)
-k_hist_merge(const HistArgs a, const int32_t* __restrict__ multi,
-             const int32_t* __restrict__ n_multi, int nblk) {
+__global__ void __launch_bounds__(kMergeElems * kMergeParts)
+k_hist_merge(const HistArgs a, const int32_t* __restrict__ multi, const int32_t* __restrict__ n_multi) {
+  __shared__ long long s_g[kMergeParts][kMergeElems];
+  __shared__ long long s_h[kMergeParts][kMergeElems];
   const int B = a.n_bins;
   const int SL = kWave * FPL * B;
-  const int m = blockIdx.x / (a.nslab * nblk);
-  const int r = blockIdx.x - m * (a.nslab * nblk);
-  const int slab = r / nblk;
-  const int blk = r - slab * nblk;
-  if (m >= *n_multi) return;
-  const int s = multi[m];
-  const int first = a.seg_pfirst[s];
-  const int nch = a.seg_first[s + 1] - a.seg_first[s];
-  const int slot = a.seg_slot[s];
-  const int f_base = slab * kWave * FPL;
-  const int e = blk * blockDim.x + threadIdx.x;
-  if (e >= SL) return;
-  const int fl = e / B;
-  const int bin = e - fl * B;
-  if (f_base + fl >= a.n_feat) return;
-  const NodeShift ns = node_shift(a.stats + (int64_t)(slot / a.slots_per_node) * kStatWords);
-  long long sg = 0, sh = 0;
-  const longlong2* p = a.part + ((int64_t)first * a.nslab + slab) * SL + e;
-  const int64_t step = (int64_t)a.nslab * SL;
-  for (int k = 0; k < nch; ++k) {
-    const longlong2 v = p[(int64_t)k * step];
-    sg += v.x;
-    sh += v.y;
+  const int neb = (SL + kMergeElems - 1) / kMergeElems;
+  const int64_t n_items = (int64_t)(*n_multi) * a.nslab * neb;
+  const int el = threadIdx.x % kMergeElems;
+  const int part = threadIdx.x / kMergeElems;
+  for (int64_t it = blockIdx.x; it < n_items; it += gridDim.x) {
+    const int m = (int)(it / (a.nslab * neb));
+    const int r = (int)(it - (int64_t)m * a.nslab * neb);
+    const int slab = r / neb;
+    const int e = (r - slab * neb) * kMergeElems + el;
+    const int s = multi[m];
+    const int first = a.seg_pfirst[s];
+    const int nch = a.seg_first[s + 1] - a.seg_first[s];
+    const int slot = a.seg_slot[s];
+    const NodeShift ns = node_shift(a.stats + (int64_t)(slot / a.slots_per_node) * kStatWords);
+    const bool one_plane = a.hess_const || ns.count_mode;
+    long long sg = 0, sx = 0;  // sx: count (one plane) or hessian sum
+    if (e < SL) {
+      const int64_t off = ((int64_t)first * a.nslab + slab) * SL + e;
+      const int64_t step = (int64_t)a.nslab * SL;
+      if (one_plane) {
+        // one-plane partials: packed words (count << 44 | chunk qg sum)
+        const unsigned long long* p = reinterpret_cast<const unsigned long long*>(a.part) + off;
+        for (int k = part; k < nch; k += kMergeParts) {
+          const unsigned long long w = p[(int64_t)k * step];
+          const long long gs = ((long long)(w << (64 - kCountShift))) >> (64 - kCountShift);
+          sg += gs;
+          sx += (long long)((w - (unsigned long long)gs) >> kCountShift);
+        }
+      } else {
+        const longlong2* p = a.part + off;
+        for (int k = part; k < nch; k += kMergeParts) {
+          const longlong2 v = p[(int64_t)k * step];
+          sg += v.x;
+          sx += v.y;
+        }
+      }
+    }
+    s_g[part][el] = sg;
+    s_h[part][el] = sx;
+    __syncthreads();
+    if (part == 0 && e < SL) {
+      for (int q = 1; q < kMergeParts; ++q) { sg += s_g[q][el]; sx += s_h[q][el]; }
+      const int fl = e / B;
+      const int bin = e - fl * B;
+      const int f_base = slab * kWave * FPL;
+      if (f_base + fl < a.n_feat) {
+        const long long sh = one_plane ? sx * ns.qh_const : sx;
+        const int64_t d = ((int64_t)slot * a.n_feat + f_base + fl) * B + bin;
+        const float fg = from_fixed(sg, ns.sg), fh = from_fixed(sh, ns.sh);
+        if (a.accumulate) { a.out_g[d] += fg; a.out_h[d] += fh; }
+        else { a.out_g[d] = fg; a.out_h[d] = fh; }
+      }
+    }
+    __syncthreads();
   }
-  const int64_t d = ((int64_t)slot * a.n_feat + f_base + fl) * B + bin;
-  const float fg = from_fixed(sg, ns.sg), fh = from_fixed(sh, ns.sh);
-  if (a.accumulate) { a.out_g[d] += fg; a.out_h[d] += fh; }
-  else { a.out_g[d] = fg; a.out_h[d] = fh; }
 }
 
 // Statistics of one row list (reference op): stats[0..3] over g[rows], h[rows].
@@ -581,15 +637,31 @@ HistConfig auto_hist_config(int n_feat, int n_bins, int64_t row_stride, bool one
   return c;
 }
 
-int64_t hist_max_chunks(int64_t rows_max, int64_t nseg_max, int chunk_rows) {
-  return ceil_div(rows_max, chunk_rows) + nseg_max;
+static int64_t hist_chunk_target(const HistConfig& c, int n_feat) {
+  const int64_t nslab = ceil_div(n_feat > 0 ? n_feat : 1, kWave * c.fpl);
+  const int64_t t = kHistSlots / nslab;
+  return t > 0 ? t : 1;
+}
+
+// With C >= ceil(rows / target) (or C = chunk_rows), rows / C <= cap.
+static int64_t hist_chunk_cap(const HistConfig& c, int n_feat, int64_t rows_max) {
+  const int64_t a = hist_chunk_target(c, n_feat), b = ceil_div(rows_max, c.chunk_rows);
+  return a > b ? a : b;
+}
+
+int64_t hist_max_chunks(const HistConfig& c, int n_feat, int64_t rows_max, int64_t nseg_max) {
+  return hist_chunk_cap(c, n_feat, rows_max) + nseg_max + 1;
+}
+
+int64_t hist_max_multi(const HistConfig& c, int n_feat, int64_t rows_max) {
+  return hist_chunk_cap(c, n_feat, rows_max) + 1;  // each holds more than C rows
 }
 
 size_t hist_partial_words(const HistConfig& c, int n_feat, int n_bins, int64_t rows_max) {
   const int nslab = (int)ceil_div(n_feat, kWave * c.fpl);
-  // Only multi-chunk segments write partials; each holds more than C rows,
-  // so their chunks number at most 2 * rows / C.
-  const int64_t chunks = 2 * ceil_div(rows_max, c.chunk_rows) + 2;
+  // Multi-chunk segments hold more than C rows each, so their chunks number
+  // at most 2 * rows / C <= 2 * cap.
+  const int64_t chunks = 2 * hist_chunk_cap(c, n_feat, rows_max) + 2;
   return (size_t)chunks * nslab * kWave * c.fpl * n_bins;
 }
 
@@ -601,6 +673,8 @@ struct KernelTimer {
   std::vector<std::pair<hipEvent_t, hipEvent_t>> build, merge;
 };
 static KernelTimer g_timer;
+
+bool hist_timer_on() { return g_timer.on; }
 
 static void timer_mark(std::vector<std::pair<hipEvent_t, hipEvent_t>>& v, bool start, hipStream_t st) {
   if (!g_timer.on) return;
@@ -625,11 +699,17 @@ static void timer_clear() {
 template <int FPL, int L, bool CNT>
 static int launch_mode(const HistArgs& a, const HistConfig& c, int64_t groups, hipStream_t st) {
   const size_t lds = lds_bytes(c, a.n_bins, CNT);
-  if (lds > 64 * 1024) {  // opt in to more than 64 KB of dynamic LDS
+  // Opt in to more than 64 KB of dynamic LDS, once per size and device (a
+  // second call may come while the stream is being captured into a graph).
+  static size_t opted[64] = {};
+  int dev = 0;
+  WGBM_HIP(hipGetDevice(&dev));
+  if (lds > 64 * 1024 && (dev < 0 || dev >= 64 || opted[dev] < lds)) {
     WGBM_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_hist_build<FPL, L, true, CNT>),
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsMax));
     WGBM_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_hist_build<FPL, L, false, CNT>),
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsMax));
+    if (dev >= 0 && dev < 64) opted[dev] = kLdsMax;
   }
   timer_mark(g_timer.build, true, st);
   if (a.gh)
@@ -644,8 +724,11 @@ static int launch_mode(const HistArgs& a, const HistConfig& c, int64_t groups, h
 template <int FPL, int L>
 static int launch_fpl(const HistArgs& a, const HistConfig& c, int64_t max_chunks, int64_t max_multi,
                       int32_t* multi, int32_t* n_multi, hipStream_t st) {
-  const int64_t groups = ceil_div(max_chunks, 8) * 8 * a.nslab;
-  if (groups > 0x7fffffff) { set_error("histogram grid too large"); return kTooBig; }
+  // Work items (chunk, slab); a grid of at most kHistSlots workgroups loops
+  // over them.
+  const int64_t items = ceil_div(max_chunks, 8) * 8 * a.nslab;
+  if (items > 0x7fffffff) { set_error("histogram grid too large"); return kTooBig; }
+  const int64_t groups = items < kHistSlots ? items : kHistSlots;
   int rc;
   if (a.hess_const) {
     rc = launch_mode<FPL, L, true>(a, c, groups, st);
@@ -658,10 +741,11 @@ static int launch_fpl(const HistArgs& a, const HistConfig& c, int64_t max_chunks
   if (rc != kOk) return rc;
   if (max_multi > 0) {
     const int SL = kWave * FPL * a.n_bins;
-    const int nblk = (int)ceil_div(SL, 256);
-    const int64_t mg = max_multi * a.nslab * nblk;
+    const int64_t items = max_multi * a.nslab * ceil_div(SL, kMergeElems);
+    const int64_t mg = items < 2048 ? items : 2048;
     timer_mark(g_timer.merge, true, st);
-    hipLaunchKernelGGL((k_hist_merge<FPL>), dim3((unsigned)mg), dim3(256), 0, st, a, multi, n_multi, nblk);
+    hipLaunchKernelGGL((k_hist_merge<FPL>), dim3((unsigned)mg), dim3(kMergeElems * kMergeParts), 0, st, a, multi,
+                       n_multi);
     WGBM_LAUNCHED("k_hist_merge");
     timer_mark(g_timer.merge, false, st);
   }
@@ -695,11 +779,14 @@ int hist_launch(HistArgs a, const HistConfig& c, int64_t rows_max, int64_t nseg_
   src.mode = kPlanSegments;
   src.seg_count = a.seg_count;
   src.nseg_dev = a.nseg_dev;
+  src.chunk_out = a.chunk_dev;
+  src.chunk_min = kChunkMin < c.chunk_rows ? kChunkMin : c.chunk_rows;
+  src.chunk_target = (int32_t)hist_chunk_target(c, a.n_feat);
   hipLaunchKernelGGL(k_hist_plan, dim3(1), dim3(1024), 0, st, src, c.chunk_rows, a.seg_first, a.seg_pfirst, multi,
                      n_multi);
   WGBM_LAUNCHED("k_hist_plan");
-  const int64_t max_chunks = hist_max_chunks(rows_max, nseg_max, c.chunk_rows);
-  const int64_t max_multi = rows_max / (c.chunk_rows + 1) + 1;
+  const int64_t max_chunks = hist_max_chunks(c, a.n_feat, rows_max, nseg_max);
+  const int64_t max_multi = hist_max_multi(c, a.n_feat, rows_max);
   if (a.n_feat == 0) return kOk;
 #define WGBM_D2(FPL, D0, D1)                                                         \
   return c.depth <= D0 ? launch_fpl<FPL, D0>(a, c, max_chunks, max_multi, multi, n_multi, st) \
@@ -752,8 +839,8 @@ extern "C" int wgbm_profile_end(double* out4) {
 
 // Workspace carve-up for the reference-style histogram op.
 struct Hist3Layout {
-  size_t seg_start, seg_count, seg_slot, seg_first, seg_pfirst, nseg, multi, n_multi, sorted, cnt, stats, stat_part,
-      part, total;
+  size_t seg_start, seg_count, seg_slot, seg_first, seg_pfirst, nseg, multi, n_multi, chunk, sorted, cnt, stats,
+      stat_part, part, total;
 };
 
 static Hist3Layout hist3_layout(int64_t n, int n_feat, int n_bins, int n_eras, const HistConfig& c) {
@@ -767,8 +854,9 @@ static Hist3Layout hist3_layout(int64_t n, int n_feat, int n_bins, int n_eras, c
   L.seg_first = o; o = al(o + 4 * (E + 1));
   L.seg_pfirst = o; o = al(o + 4 * E);
   L.nseg = o;      o = al(o + 4);
-  const int64_t max_multi = n / (c.chunk_rows + 1) + 1;
+  const int64_t max_multi = hist_max_multi(c, n_feat, n);
   L.multi = o;     o = al(o + 4 * max_multi);
+  L.chunk = o;     o = al(o + 4);
   L.n_multi = o;   o = al(o + 4);
   const int64_t nb = ceil_div(n, kSortBlock);
   L.sorted = o;    o = al(o + (E > 1 ? 4 * n : 0));
@@ -823,6 +911,7 @@ extern "C" int wgbm_compute_histogram3(const int8_t* bin_indices, int64_t n_tota
   a.seg_first = P(L.seg_first);
   a.seg_pfirst = P(L.seg_pfirst);
   a.nseg_dev = P(L.nseg);
+  a.chunk_dev = P(L.chunk);
   a.out_g = grad_hist;
   a.out_h = hess_hist;
   a.part = reinterpret_cast<longlong2*>(ws + L.part);

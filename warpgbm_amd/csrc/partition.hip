@@ -26,7 +26,10 @@ __device__ __forceinline__ int split_col(const PartArgs& a, int j) {
 
 __device__ __forceinline__ bool goes_left(const PartArgs& a, int row, int col, int bin) {
   if ((uint64_t)row >= (uint64_t)a.n_rows) return true;
-  return (int)a.codes[(int64_t)row * a.row_stride + col] <= bin;
+  // Column-major copy when present: a split column is then n_rows contiguous
+  // bytes (L2-resident) instead of one 128-byte line per row.
+  const int8_t code = a.codes_t ? a.codes_t[(int64_t)col * a.n_rows + row] : a.codes[(int64_t)row * a.row_stride + col];
+  return (int)code <= bin;
 }
 
 __global__ void __launch_bounds__(256)
@@ -56,27 +59,54 @@ k_part_count(PartArgs a) {
 
 // One workgroup: per-segment left totals and chunk offsets, child segment
 // tables, smaller-child build list and sibling map.
+// Exclusive prefix sum of v over the block (blockDim.x a multiple of 64, at
+// most 1024; every thread calls it); *total receives the block's sum.
+__device__ __forceinline__ int block_exscan(int v, int* total) {
+  __shared__ int ws[16];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int x = v;
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(x, o);
+    if (lane >= o) x += y;
+  }
+  __syncthreads();  // ws may still be read by a previous call
+  if (lane == 63) ws[w] = x;
+  __syncthreads();
+  int pre = 0, tot = 0;
+  for (int k = 0; k < (int)(blockDim.x >> 6); ++k) {
+    const int c = ws[k];
+    if (k < w) pre += c;
+    tot += c;
+  }
+  *total = tot;
+  return pre + x - v;
+}
+
+// One workgroup: per-chunk destinations, child tables and the build list.
+// Chunks of a segment are contiguous, so one scan over all chunks gives
+// every chunk its left rows before it within its segment.
 __global__ void __launch_bounds__(1024)
 k_part_scan(PartArgs a) {
   const int E = a.n_eras;
   const int nseg = a.n_nodes * E;
-  // A: left totals per segment; chunk-local left/right offsets.
-  for (int s = threadIdx.x; s < nseg; s += blockDim.x) {
-    if (!is_split(a, s / E)) { a.segL[s] = 0; continue; }
-    const int first = a.seg_first[s], last = a.seg_first[s + 1];
-    int run = 0;
-    for (int c = first; c < last; ++c) {
-      const int v = a.lbase[c];
-      a.lbase[c] = run;                                // lefts before chunk (in segment)
-      a.rbase[c] = (c - first) * a.chunk_rows - run;   // rights before chunk
-      run += v;
-    }
-    a.segL[s] = run;
+  const int total = a.seg_first[nseg];
+  // A: exclusive scan of the per-chunk left counts.
+  int carry = 0;
+  for (int base = 0; base < total; base += blockDim.x) {
+    const int c = base + threadIdx.x;
+    int tot;
+    const int ex = block_exscan(c < total ? a.lbase[c] : 0, &tot);
+    if (c < total) a.cscan[c] = carry + ex;
+    carry += tot;
   }
+  if (threadIdx.x == 0) a.cscan[total] = carry;
+  __syncthreads();
+  for (int s = threadIdx.x; s < nseg; s += blockDim.x)
+    a.segL[s] = is_split(a, s / E) ? a.cscan[a.seg_first[s + 1]] - a.cscan[a.seg_first[s]] : 0;
   __syncthreads();
   // B: child tables per node.
   for (int j = threadIdx.x; j < a.n_nodes; j += blockDim.x) {
-    const bool split = a.node_active[j] && a.dec_feat[j] >= 0;
+    const bool split = is_split(a, j);
     const int l = 2 * j, r = 2 * j + 1;
     if (!split) {
       a.c_active[l] = 0; a.c_active[r] = 0;
@@ -105,36 +135,37 @@ k_part_scan(PartArgs a) {
     a.c_node_start[r] = start + Lt; a.c_node_count[r] = Rt;
   }
   __syncthreads();
-  // C: absolute destination bases per chunk.
-  for (int s = threadIdx.x; s < nseg; s += blockDim.x) {
-    if (!is_split(a, s / E)) continue;
+  // C: absolute destination bases per chunk (only split segments own chunks).
+  for (int c = threadIdx.x; c < total; c += blockDim.x) {
+    const int s = upper_index(a.seg_first, nseg, c);
     const int j = s / E, e = s - j * E;
-    const int ls = a.c_seg_start[(2 * j) * E + e];
-    const int rs = a.c_seg_start[(2 * j + 1) * E + e];
-    for (int c = a.seg_first[s]; c < a.seg_first[s + 1]; ++c) {
-      a.lbase[c] += ls;
-      a.rbase[c] += rs;
-    }
+    const int first = a.seg_first[s];
+    const int lb = a.cscan[c] - a.cscan[first];
+    a.lbase[c] = lb + a.c_seg_start[(2 * j) * E + e];
+    a.rbase[c] = (c - first) * a.chunk_rows - lb + a.c_seg_start[(2 * j + 1) * E + e];
   }
   // D: build list (smaller child, left on ties: warpgbm/core.py:1438) in node order.
-  if (threadIdx.x == 0) {
-    int rank = 0;
-    for (int j = 0; j < a.n_nodes; ++j) {
-      const bool split = a.node_active[j] && a.dec_feat[j] >= 0;
-      if (!split) continue;
+  int rank0 = 0;
+  for (int base = 0; base < a.n_nodes; base += blockDim.x) {
+    const int j = base + threadIdx.x;
+    const bool split = j < a.n_nodes && is_split(a, j);
+    int tot;
+    const int rank = rank0 + block_exscan(split ? 1 : 0, &tot);
+    if (split) {
       const int l = 2 * j, r = 2 * j + 1;
       const int built = (a.c_node_count[l] <= a.c_node_count[r]) ? l : r;
-      const int other = built ^ 1;
       a.sib[3 * rank + 0] = j;
       a.sib[3 * rank + 1] = built;
-      a.sib[3 * rank + 2] = other;
-      ++rank;
+      a.sib[3 * rank + 2] = built ^ 1;
     }
-    *a.n_split = rank;
-    *a.n_build = rank * E;
+    rank0 += tot;
+  }
+  if (threadIdx.x == 0) {
+    *a.n_split = rank0;
+    *a.n_build = rank0 * E;
   }
   __syncthreads();
-  const int ns = *a.n_split;
+  const int ns = rank0;
   for (int t = threadIdx.x; t < ns * E; t += blockDim.x) {
     const int q = t / E, e = t - q * E;
     const int c = a.sib[3 * q + 1];
@@ -440,7 +471,7 @@ using namespace wgbm;
 extern "C" size_t wgbm_partition_workspace_bytes(int64_t n_samples) {
   if (n_samples < 0) return 0;
   const int64_t mc = ceil_div(n_samples, kPartChunk) + 1;
-  return (size_t)(4 * (16 + 3 * mc) + 64 * 4) + (size_t)n_samples + 1024;
+  return (size_t)(4 * (16 + 4 * mc + 1) + 64 * 4) + (size_t)n_samples + 1024;
 }
 
 // Stable split of one node's row list (plain path): rows with
@@ -461,7 +492,8 @@ extern "C" int wgbm_partition_rows(const int8_t* bin_indices, int64_t n_total_ro
   int32_t* sc = w;            // 64 ints of scalars and tiny tables
   int32_t* lbase = w + 64;
   int32_t* rbase = lbase + mc;
-  int32_t* multi = rbase + mc;
+  int32_t* cscan = rbase + mc;   // mc + 1
+  int32_t* multi = cscan + mc + 1;
   PartArgs a{};
   a.codes = bin_indices; a.row_stride = n_master_feat; a.n_rows = n_total_rows;
   a.feat_idx = nullptr;
@@ -483,7 +515,7 @@ extern "C" int wgbm_partition_rows(const int8_t* bin_indices, int64_t n_total_ro
   a.n_build = sc + 25; a.n_split = sc + 26; a.sib = sc + 27;  // 3 ints
   a.multi = multi;
   a.flag = reinterpret_cast<uint8_t*>(multi + mc);
-  a.lbase = lbase; a.rbase = rbase;
+  a.lbase = lbase; a.rbase = rbase; a.cscan = cscan;
   a.perm_in = rows; a.gh_in = nullptr;
   a.perm_out = out_rows; a.gh_out = nullptr;
   a.chunk_rows = kPartChunk;

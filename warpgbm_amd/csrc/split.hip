@@ -139,87 +139,71 @@ k_select_split(const float* __restrict__ gain, const float* __restrict__ dir, in
 }
 
 // ---------------------------------------------------------------------------
-// Batched level scan.  Thread per (node, local feature); the per-era gains
-// are the same float32 values k_compute_split would produce, and the era
-// sums run in era order, so the candidate is bit-identical to the plain op
-// followed by k_select_split.  One candidate per (node, workgroup) goes to
-// `cand`; k_level_pick reduces them in workgroup order.
-constexpr int kScanBlk = 32;   // thresholds per register block (E > 1)
+// Batched level scan.  One wave per (node, local feature), lane b owning
+// threshold b (and b + 64 when B > 65).  Each lane sums the bins in bin
+// order -- the totals over all B bins, the left sums over bins 0..b -- so
+// every float64 sum is the one the sequential per-feature loop of
+// k_compute_split forms, and the per-era float32 gains are bit-identical;
+// era sums run in era order as in eval_cand.  One candidate per (node,
+// workgroup of kScanFeat features) goes to `cand`; k_level_pick reduces them
+// (cand_better is a strict total order, so the reduction order is free).
+constexpr int kScanFeat = 4;   // features (waves) per workgroup
 
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(64 * kScanFeat)
 k_level_scan(const LevelScanArgs a) {
-  const int wg_per_node = (a.n_feat + blockDim.x - 1) / blockDim.x;
+  const int wg_per_node = (a.n_feat + kScanFeat - 1) / kScanFeat;
   const int node = blockIdx.x / wg_per_node;
   const int wg = blockIdx.x - node * wg_per_node;
-  const int f = wg * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const int f = wg * kScanFeat + (threadIdx.x >> 6);
   Cand best = cand_none();
   const bool active = node < a.n_nodes && a.node_active[node] != 0;
   if (active && f < a.n_feat) {
     const int B = a.n_bins, E = a.n_eras;
     const int req = a.mono_req ? (int)a.mono_req[a.feat_idx ? a.feat_idx[f] : f] : 0;
     const int64_t node_off = (int64_t)node * E * a.n_feat * B;
-    if (E == 1) {
-      const float* g = a.G + node_off + (int64_t)f * B;
-      const float* h = a.H + node_off + (int64_t)f * B;
-      double Gt = 0.0, Ht = 0.0;
-      for (int b = 0; b < B; ++b) { Gt += (double)g[b]; Ht += (double)h[b]; }
-      double GL = 0.0, HL = 0.0;
-      for (int b = 0; b < B - 1; ++b) {
-        GL += (double)g[b];
-        HL += (double)h[b];
+    for (int b0 = 0; b0 < B - 1; b0 += 64) {
+      const int b = b0 + lane;            // this lane's threshold
+      float sg = 0.0f;
+      int sd = 0;
+      bool mono = true;
+      float g1 = 0.0f, d1 = 0.0f;
+      for (int e = 0; e < E; ++e) {
+        const float* g = a.G + node_off + ((int64_t)e * a.n_feat + f) * B;
+        const float* h = a.H + node_off + ((int64_t)e * a.n_feat + f) * B;
+        double Gt = 0.0, Ht = 0.0, GL = 0.0, HL = 0.0;
+        for (int i = 0; i < B; ++i) {
+          const double gi = (double)g[i], hi = (double)h[i];
+          Gt += gi;
+          Ht += hi;
+          if (i <= b) { GL += gi; HL += hi; }
+        }
         float ga, di;
         gain_dir(GL, HL, Gt, Ht, (double)a.mcw, (double)a.lam, ga, di);
-        const bool ok = (ga > a.msg) && (req == 0 || di == (float)req);
-        Cand k{0, ok ? ga : -INFINITY, f * (B - 1) + b};
-        if (cand_better(k, best)) best = k;
+        sg += ga;
+        sd += (int)di;
+        mono = mono && (req == 0 || di == (float)req);
+        g1 = ga; d1 = di;
       }
-    } else {
-      for (int b0 = 0; b0 < B - 1; b0 += kScanBlk) {
-        const int nb = min(kScanBlk, B - 1 - b0);
-        float sg[kScanBlk];
-        int sd[kScanBlk];
-        bool mono[kScanBlk];
-#pragma unroll
-        for (int q = 0; q < kScanBlk; ++q) { sg[q] = 0.0f; sd[q] = 0; mono[q] = true; }
-        for (int e = 0; e < E; ++e) {
-          const float* g = a.G + node_off + ((int64_t)e * a.n_feat + f) * B;
-          const float* h = a.H + node_off + ((int64_t)e * a.n_feat + f) * B;
-          double Gt = 0.0, Ht = 0.0;
-          for (int b = 0; b < B; ++b) { Gt += (double)g[b]; Ht += (double)h[b]; }
-          double GL = 0.0, HL = 0.0;
-          for (int b = 0; b < b0; ++b) { GL += (double)g[b]; HL += (double)h[b]; }
-#pragma unroll
-          for (int q = 0; q < kScanBlk; ++q) {
-            if (q < nb) {
-              GL += (double)g[b0 + q];
-              HL += (double)h[b0 + q];
-              float ga, di;
-              gain_dir(GL, HL, Gt, Ht, (double)a.mcw, (double)a.lam, ga, di);
-              sg[q] += ga;
-              sd[q] += (int)di;
-              mono[q] = mono[q] && (req == 0 || di == (float)req);
-            }
-          }
+      if (b < B - 1) {
+        Cand k;
+        if (E == 1) {
+          k = Cand{0, (g1 > a.msg && (req == 0 || d1 == (float)req)) ? g1 : -INFINITY, f * (B - 1) + b};
+        } else {
+          const float cr = sg / (float)E;
+          k = Cand{sd < 0 ? -sd : sd, (cr > a.msg && mono) ? cr : -INFINITY, f * (B - 1) + b};
         }
-#pragma unroll
-        for (int q = 0; q < kScanBlk; ++q) {
-          if (q < nb) {
-            const float cr = sg[q] / (float)E;
-            Cand k{sd[q] < 0 ? -sd[q] : sd[q], (cr > a.msg && mono[q]) ? cr : -INFINITY,
-                   f * (B - 1) + b0 + q};
-            if (cand_better(k, best)) best = k;
-          }
-        }
+        if (cand_better(k, best)) best = k;
       }
     }
   }
-  __shared__ Cand s_c[4];
+  __shared__ Cand s_c[kScanFeat];
   best = wave_best(best);
-  if ((threadIdx.x & 63) == 0) s_c[threadIdx.x >> 6] = best;
+  if (lane == 0) s_c[threadIdx.x >> 6] = best;
   __syncthreads();
   if (threadIdx.x == 0 && node < a.n_nodes) {
     Cand b = s_c[0];
-    for (int w = 1; w < (int)(blockDim.x >> 6); ++w)
+    for (int w = 1; w < kScanFeat; ++w)
       if (cand_better(s_c[w], b)) b = s_c[w];
     const int64_t o = (int64_t)node * wg_per_node + wg;
     a.cand_agree[o] = b.agree;
@@ -228,36 +212,30 @@ k_level_scan(const LevelScanArgs a) {
   }
 }
 
-// Per node: reduce the workgroup candidates in order, write the decision
-// (local feature, bin) or (-1, -1), and the per-era gain of the winner
-// (feature importance, warpgbm/core.py:1418-1423).
+// Winner per node, then (optionally) its per-era gains.
 __global__ void __launch_bounds__(64)
 k_level_pick(const LevelScanArgs a) {
   const int node = blockIdx.x;
   if (node >= a.n_nodes) return;
-  const int wg_per_node = (a.n_feat + 255) / 256;
-  __shared__ int s_f, s_b;
-  if (threadIdx.x == 0) {
-    int fl = -1, bin = -1;
-    if (a.node_active[node]) {
-      Cand b = cand_none();
-      for (int w = 0; w < wg_per_node; ++w) {
-        const int64_t o = (int64_t)node * wg_per_node + w;
-        Cand k{a.cand_agree[o], a.cand_crit[o], a.cand_idx[o]};
-        if (cand_better(k, b)) b = k;
-      }
-      if (b.idx != 0x7fffffff && b.crit != -INFINITY) {
-        fl = b.idx / (a.n_bins - 1);
-        bin = b.idx % (a.n_bins - 1);
-      }
+  const int wg_per_node = (a.n_feat + kScanFeat - 1) / kScanFeat;
+  Cand b = cand_none();
+  if (a.node_active[node]) {
+    for (int w = threadIdx.x; w < wg_per_node; w += 64) {
+      const int64_t o = (int64_t)node * wg_per_node + w;
+      const Cand k{a.cand_agree[o], a.cand_crit[o], a.cand_idx[o]};
+      if (cand_better(k, b)) b = k;
     }
+  }
+  b = wave_best(b);
+  int fl = -1, bin = -1;
+  if (b.idx != 0x7fffffff && b.crit != -INFINITY) {
+    fl = b.idx / (a.n_bins - 1);
+    bin = b.idx % (a.n_bins - 1);
+  }
+  if (threadIdx.x == 0) {
     a.dec_feat[node] = fl;
     a.dec_bin[node] = bin;
-    s_f = fl;
-    s_b = bin;
   }
-  __syncthreads();
-  const int fl = s_f, bin = s_b;
   if (a.era_gain == nullptr) return;
   const int B = a.n_bins, E = a.n_eras;
   for (int e = threadIdx.x; e < E; e += blockDim.x) {
@@ -267,9 +245,9 @@ k_level_pick(const LevelScanArgs a) {
       const float* g = a.G + off;
       const float* h = a.H + off;
       double Gt = 0.0, Ht = 0.0;
-      for (int b = 0; b < B; ++b) { Gt += (double)g[b]; Ht += (double)h[b]; }
+      for (int q = 0; q < B; ++q) { Gt += (double)g[q]; Ht += (double)h[q]; }
       double GL = 0.0, HL = 0.0;
-      for (int b = 0; b <= bin; ++b) { GL += (double)g[b]; HL += (double)h[b]; }
+      for (int q = 0; q <= bin; ++q) { GL += (double)g[q]; HL += (double)h[q]; }
       float di;
       gain_dir(GL, HL, Gt, Ht, (double)a.mcw, (double)a.lam, ga, di);
     }
@@ -277,10 +255,13 @@ k_level_pick(const LevelScanArgs a) {
   }
 }
 
+int64_t level_scan_groups(int n_feat) { return n_feat > 0 ? ceil_div(n_feat, kScanFeat) : 1; }
+
 int level_scan_launch(const LevelScanArgs& a, hipStream_t st) {
   if (a.n_nodes == 0) return kOk;
-  const int wg_per_node = (int)ceil_div(a.n_feat, 256);
-  hipLaunchKernelGGL(k_level_scan, dim3((unsigned)(a.n_nodes * (int64_t)wg_per_node)), dim3(256), 0, st, a);
+  const int wg_per_node = (int)ceil_div(a.n_feat, kScanFeat);
+  hipLaunchKernelGGL(k_level_scan, dim3((unsigned)(a.n_nodes * (int64_t)wg_per_node)), dim3(64 * kScanFeat), 0,
+                     st, a);
   WGBM_LAUNCHED("k_level_scan");
   hipLaunchKernelGGL(k_level_pick, dim3((unsigned)a.n_nodes), dim3(64), 0, st, a);
   WGBM_LAUNCHED("k_level_pick");

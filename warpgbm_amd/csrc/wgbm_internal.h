@@ -4,15 +4,20 @@
 
 namespace wgbm {
 
-// Order-defining histogram configuration: chunk_rows (C) and waves (W) fix
-// the float32 summation order; fpl (features per lane) only changes the
-// load width.  Plain and batched builds must use the same C and W.
+// Histogram tiling.  Sums are exact fixed-point integers, so none of these
+// changes a bit of a histogram, only the speed.  chunk_rows is the largest
+// chunk; each launch's plan picks its chunk size between kChunkMin and it so
+// that the launch has about kHistSlots work items.
 struct HistConfig {
   int chunk_rows = 0;
   int waves = 0;
-  int fpl = 0;
-  int depth = 0;  // rows whose code loads are in flight per wave (load block)
+  int fpl = 0;     // features per lane (load width)
+  int depth = 0;   // rows whose code loads are in flight per wave (load block)
 };
+
+// Workgroup slots of the histogram build: two per CU on MI355X's 256 CUs.
+constexpr int kHistSlots = 512;
+constexpr int kChunkMin = 1024;
 
 struct HistArgs {
   const int8_t* codes;     // [n_rows, row_stride] row-major bin codes
@@ -31,6 +36,7 @@ struct HistArgs {
   int32_t* seg_first;      // [nseg+1], written by the plan kernel
   int32_t* seg_pfirst;     // [nseg] first partial slot of multi-chunk segments
   const int32_t* nseg_dev; // device scalar: number of segments
+  int32_t* chunk_dev;      // device scalar: rows per chunk, written by the plan
   int32_t chunk_rows;
   int32_t nslab;
   int32_t wide_ok;
@@ -52,13 +58,19 @@ HistConfig pick_hist_config(int n_feat, int n_bins, int64_t row_stride, bool one
 HistConfig auto_hist_config(int n_feat, int n_bins, int64_t row_stride, bool one_plane);
 void set_hist_override(const HistConfig& c);
 size_t lds_bytes(const HistConfig& c, int n_bins, bool count_mode);
-// Partial words (one longlong2 each) needed for rows_max rows.
+// Bounds for rows_max rows in nseg_max segments: chunks of a launch,
+// multi-chunk segments, and partial words (one longlong2 each).
+int64_t hist_max_chunks(const HistConfig& c, int n_feat, int64_t rows_max, int64_t nseg_max);
+int64_t hist_max_multi(const HistConfig& c, int n_feat, int64_t rows_max);
 size_t hist_partial_words(const HistConfig& c, int n_feat, int n_bins, int64_t rows_max);
 // Launch plan + build + merge.  rows_max / nseg_max bound the device-side
 // row and segment counts (they size the grid).  multi/n_multi: scratch of
-// rows_max/(C+1)+1 ints and one int.
+// hist_max_multi ints and one int; a.chunk_dev: one int.
 int hist_launch(HistArgs a, const HistConfig& c, int64_t rows_max, int64_t nseg_max,
                 int32_t* multi, int32_t* n_multi, hipStream_t st);
+// True while wgbm_profile_begin's event timing is on (events are recorded
+// around every histogram launch, so such launches are not graph-captured).
+bool hist_timer_on();
 
 // Batched level scan: candidates per (node, workgroup of 256 features),
 // then a per-node pick.  Histograms are [n_nodes][E][n_feat][B].
@@ -70,7 +82,7 @@ struct LevelScanArgs {
   const int32_t* feat_idx;      // local -> global feature (nullptr: identity)
   const int8_t* mono_req;       // [global features] required direction, or nullptr
   float msg, mcw, lam;
-  int32_t* cand_agree;          // [n_nodes * ceil(n_feat/256)]
+  int32_t* cand_agree;          // [n_nodes * level_scan_groups(n_feat)]
   float* cand_crit;
   int32_t* cand_idx;
   int32_t* dec_feat;            // [n_nodes] local feature or -1
@@ -78,6 +90,7 @@ struct LevelScanArgs {
   float* era_gain;              // [n_nodes][E] gain of the winner, or nullptr
 };
 int level_scan_launch(const LevelScanArgs& a, hipStream_t st);
+int64_t level_scan_groups(int n_feat);  // candidate slots per node
 
 // Chunk plan shared by histogram, partition and leaf kernels: the row
 // count of segment s is seg_count[s] (segments), node_count[s] for leaf
@@ -95,6 +108,11 @@ struct PlanSrc {
   const int32_t* dec_feat;
   const int32_t* node_count;
   int32_t n_eras;
+  // Adaptive chunk size (histogram plans): when chunk_out is set, the plan
+  // picks rows per chunk in [chunk_min, chunk_rows] for about chunk_target
+  // chunks and stores it there.
+  int32_t* chunk_out;
+  int32_t chunk_min, chunk_target;
 };
 void plan_launch(const PlanSrc& src, int chunk_rows, int32_t* seg_first, int32_t* multi, int32_t* n_multi,
                  hipStream_t st);

@@ -4,12 +4,13 @@
 
 namespace wgbm {
 
-constexpr int kPartChunk = 4096;
+constexpr int kPartChunk = 1024;  // rows per partition workgroup (4 passes of 256)
 
 struct PartArgs {
   const int8_t* codes;
   int64_t row_stride;
   int64_t n_rows;
+  const int8_t* codes_t;        // column-major copy [columns][n_rows], or nullptr
   const int32_t* feat_idx;      // local -> global feature (nullptr: dec_feat is global)
   int32_t n_nodes;              // nodes of this level
   int32_t n_eras;
@@ -32,6 +33,7 @@ struct PartArgs {
   int32_t* n_multi;
   int32_t* lbase;               // [max chunks]
   int32_t* rbase;
+  int32_t* cscan;               // [max chunks + 1]
   int32_t* segL;                // [n_nodes*E]
   // next level tables ([2*n_nodes(*E)])
   int32_t* c_seg_start;

@@ -59,7 +59,10 @@ class PredictServer:
 
     def __init__(self, forest, memory_budget=1 << 30, max_batch_rows=200_000, slots=3):
         self.forest = forest
-        self.device = forest.device
+        dev = torch.device(forest.device)
+        if dev.index is None:
+            dev = torch.device(dev.type, torch.cuda.current_device())
+        self.device = dev
         self.max_batch_rows = int(max_batch_rows)
         self.budget = int(memory_budget)
         K = forest.n_classes
@@ -99,7 +102,19 @@ class PredictServer:
 
     # ---------------------------------------------------------------- worker
     def _loop(self):
-        torch.cuda.set_device(self.device)
+        try:
+            torch.cuda.set_device(self.device)
+            self._serve()
+        except BaseException as e:  # fail every pending request instead of hanging its caller
+            with self._cv:
+                pending = [r for s in self._busy for r in s.reqs] + list(self._queue)
+                self._queue.clear()
+            for r in pending:
+                if not r.future.done():
+                    r.future.set_exception(e)
+            raise
+
+    def _serve(self):
         while True:
             with self._cv:
                 if self._stop and not self._busy and not self._queue:
