@@ -40,6 +40,8 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-plain", action="store_true")
     p.add_argument("--profile-only", action="store_true", help="one fit, no extras (for rocprof)")
+    p.add_argument("--no-serve", action="store_true", help="skip the served-predict measurement")
+    p.add_argument("--serve-seconds", type=float, default=2.0, help="arrival window per serving run")
     p.add_argument("--normal-edges", action="store_true",
                    help="bin with the exact N(0,1) quantiles instead of sorting the columns (no sort kernels: "
                         "for counter-collection runs, where the profiler fails inside torch.sort)")
@@ -115,6 +117,153 @@ def measured_traffic(args):
     except OSError:
         return None
     return None
+
+
+def poisson_arrivals(rng, qps, seconds):
+    t, out = 0.0, []
+    while True:
+        t += rng.exponential(1.0 / qps)
+        if t >= seconds:
+            return np.array(out)
+        out.append(t)
+
+
+def run_unbatched(hf, reqs, arrivals, dev):
+    """One predict launch per request, served in arrival order by one loop
+    (pinned staging, H2D, traversal, D2H); latency = completion - arrival."""
+    import torch
+    F = hf.n_feat
+    rmax = max(r.shape[0] for r in reqs)
+    h_in = torch.empty((rmax, F), dtype=torch.int8, pin_memory=True)
+    d_in = torch.empty((rmax, F), dtype=torch.int8, device=dev)
+    d_out = torch.empty((rmax, hf.n_classes), dtype=torch.float32, device=dev)
+    h_out = torch.empty((rmax, hf.n_classes), dtype=torch.float32, pin_memory=True)
+    ws = hf.workspace(rmax)
+    lat, rows = [], 0
+    t0 = time.perf_counter()
+    for a, r in zip(arrivals, reqs):
+        now = time.perf_counter() - t0
+        if now < a:
+            time.sleep(a - now)
+        n = r.shape[0]
+        h_in.numpy()[:n] = r
+        d_in[:n].copy_(h_in[:n], non_blocking=True)
+        hf.predict_into(d_in[:n], d_out[:n], workspace=ws)
+        h_out[:n].copy_(d_out[:n], non_blocking=True)
+        torch.cuda.synchronize(dev)
+        lat.append((time.perf_counter() - t0 - a) * 1e3)
+        rows += n
+    span = time.perf_counter() - t0
+    return np.array(lat), rows / span
+
+
+def run_server(hf, reqs, arrivals, budget):
+    from warpgbm_amd.server import PredictServer
+    srv = PredictServer(hf, memory_budget=budget, max_batch_rows=max(200_000, max(r.shape[0] for r in reqs)))
+    for f in [srv.submit(r) for r in reqs[:4]]:  # warm the worker thread and its streams
+        f.result(timeout=60)
+    srv.latencies_ms.clear()
+    srv.batches = 0
+    futs = []
+    t0 = time.perf_counter()
+    for a, r in zip(arrivals, reqs):
+        now = time.perf_counter() - t0
+        if now < a:
+            time.sleep(a - now)
+        futs.append(srv.submit(r))
+    outs = [f.result(timeout=60) for f in futs]
+    span = time.perf_counter() - t0
+    srv.close()
+    lat = np.array(srv.latencies_ms)
+    return lat, sum(r.shape[0] for r in reqs) / span, srv.batches, outs
+
+
+def serving_bench(args, dev):
+    """Served predict: a 1000-tree depth-8 forest over 400 features (trained
+    like the 1M x 1000 workload on a 200k x 400 matrix) resident in HBM;
+    requests are int8 code rows drawn from that matrix, Poisson arrivals at
+    a fixed QPS, admitted by a 1 GiB memory budget.  Two request mixes:
+    10k rows each, and log-uniform row counts in [1, 100k]."""
+    import torch
+    from warpgbm_amd import WarpGBM
+    from warpgbm_amd.serving import HeapForest
+    a2 = argparse.Namespace(rows=200_000, features=400, bins=32)
+    X, y = make_data(a2, dev)
+    codes, edges = bin_on_device(X, 32, args.normal_edges)
+    del X
+    m = WarpGBM(objective="regression", num_bins=32, max_depth=8, n_estimators=1000, learning_rate=0.01,
+                engine="batched", device=dev)
+    import contextlib, io
+    t0 = time.perf_counter()
+    with contextlib.redirect_stdout(io.StringIO()):
+        m.fit_binned(codes, y, bin_edges=edges)
+    fit_s = time.perf_counter() - t0
+    tb = m._last_tables
+    hf = HeapForest.from_tables(tb["status"].reshape(-1, tb["status"].shape[-1]),
+                                tb["feat"].reshape(-1, tb["feat"].shape[-1]),
+                                tb["bin"].reshape(-1, tb["bin"].shape[-1]),
+                                tb["value"].reshape(-1, tb["value"].shape[-1]), 8, 0.01, 400,
+                                [m.base_prediction], device=dev)
+    host = codes.cpu().numpy()
+    rng = np.random.default_rng(0)
+    ref_model = m
+    # A serving process keeps its long-lived objects (the fitted model's
+    # nested tree dicts: ~500k of them here) out of the cyclic collector, or
+    # every full collection stalls all threads for tens of milliseconds.
+    import gc
+    gc.collect()
+    gc.freeze()
+    # kernel-only throughput on 10k rows
+    req = codes[:10000].contiguous()
+    out = torch.empty((10000, 1), dtype=torch.float32, device=dev)
+    ws = hf.workspace(10000)
+    for _ in range(3):
+        hf.predict_into(req, out, workspace=ws)
+    torch.cuda.synchronize(dev)
+    t0 = time.perf_counter()
+    for _ in range(20):
+        hf.predict_into(req, out, workspace=ws)
+    torch.cuda.synchronize(dev)
+    k_ms = (time.perf_counter() - t0) / 20 * 1e3
+    ref = ref_model.predict_binned(req)
+    err = float((out[:, 0] - ref).abs().max().item())
+    res = {"forest": "1000 trees, depth 8, 400 features (trained on 200k x 400)", "fit_s": fit_s,
+           "kernel_ms_10k_rows": k_ms, "kernel_rows_per_s": 10000 / (k_ms * 1e-3),
+           "node_visits_per_s": 10000 * 1000 * 8 / (k_ms * 1e-3),
+           "max_abs_diff_vs_reference_op": err, "memory_budget_bytes": 1 << 30}
+    def make_reqs(mix, k):
+        sizes = (np.full(k, 10000) if mix == "10k"
+                 else np.floor(np.exp(rng.uniform(0, np.log(1e5), k))).astype(int).clip(1, 100000))
+        starts = rng.integers(0, host.shape[0] - sizes + 1)
+        return [host[s:s + n] for s, n in zip(starts, sizes)], sizes
+
+    for mix, load in (("10k", 0.5), ("loguniform", 0.5), ("10k", 1.5)):
+        # QPS: a fraction of what the one-launch-per-request loop sustains back to back
+        cal, _ = make_reqs(mix, 40)
+        lc, _ = run_unbatched(hf, cal, np.zeros(len(cal)), dev)
+        service_ms = float(np.diff(np.concatenate([[0.0], np.sort(lc)])).mean())
+        qps = load / (service_ms * 1e-3)
+        arr = poisson_arrivals(rng, qps, args.serve_seconds)
+        reqs, sizes = make_reqs(mix, len(arr))
+        lu, ru = run_unbatched(hf, reqs, arr, dev)
+        lb, rb, nb, outs = run_server(hf, reqs, arr, 1 << 30)
+        # served results equal the one-launch-per-request results bit for bit
+        chk = min(len(reqs), 20)
+        same = all(np.array_equal(outs[i][:, 0], hf.predict(torch.from_numpy(reqs[i]).to(dev))[:, 0].cpu().numpy())
+                   for i in range(chk))
+        key = mix if load < 1 else f"{mix}_overload"
+        res[key] = {"qps": qps, "load_vs_unbatched_capacity": load, "unbatched_service_ms": service_ms,
+                    "requests": len(reqs), "rows": int(sizes.sum()),
+                    "batched": {"p50_ms": float(np.percentile(lb, 50)), "p99_ms": float(np.percentile(lb, 99)),
+                                "rows_per_s": rb, "launches": nb},
+                    "unbatched": {"p50_ms": float(np.percentile(lu, 50)), "p99_ms": float(np.percentile(lu, 99)),
+                                  "rows_per_s": ru, "launches": len(reqs)},
+                    "served_equals_single_launch": bool(same)}
+        r_ = res[key]
+        log(f"serve {key}: qps {qps:.0f} batched p50 {r_['batched']['p50_ms']:.3f} p99 "
+            f"{r_['batched']['p99_ms']:.3f} ms ({nb} launches); unbatched p50 "
+            f"{r_['unbatched']['p50_ms']:.3f} p99 {r_['unbatched']['p99_ms']:.3f} ms")
+    return res
 
 
 def trees_identical(fa, fb):
@@ -256,6 +405,10 @@ def main():
     pred_err = float((out[:, 0] - ref).abs().max().item())
     log(f"predict 10k rows: p50 {np.percentile(lat, 50):.3f} ms p99 {np.percentile(lat, 99):.3f} ms")
 
+    serve = None
+    if not args.no_serve:
+        serve = serving_bench(args, dev)
+
     cpu = None
     if not args.no_cpu:
         n = min(args.cpu_rows, args.rows)
@@ -292,6 +445,7 @@ def main():
         "predict": {"rows_per_request": 10000, "p50_ms": float(np.percentile(lat, 50)),
                     "p99_ms": float(np.percentile(lat, 99)), "rows_per_s": 10000 / (np.mean(lat) * 1e-3),
                     "max_abs_diff_vs_reference_op": pred_err, "mode": "one request per launch (unbatched)"},
+        "serving": serve,
         "binning_s": bin_s,
         "cpu_baseline": cpu,
     }

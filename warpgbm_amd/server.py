@@ -55,6 +55,11 @@ class PredictServer:
     memory_budget: device bytes the in-flight requests may hold (codes,
     outputs and reduction workspace per row); a request needing more than
     the whole budget is rejected.  max_batch_rows caps one launch.
+
+    Latency tails: the worker is a Python thread, so a full cyclic garbage
+    collection anywhere in the process (which walks every live object, e.g.
+    a fitted model's nested tree dicts) stalls it.  Serving processes should
+    call ``gc.freeze()`` once their long-lived objects exist.
     """
 
     def __init__(self, forest, memory_budget=1 << 30, max_batch_rows=200_000, slots=3):
@@ -68,6 +73,7 @@ class PredictServer:
         K = forest.n_classes
         self.bytes_per_row = forest.n_feat + 8 * K + 4 * K * forest.n_blocks
         self._slots = [_Slot(forest, self.max_batch_rows, self.device) for _ in range(slots)]
+        self._warm_slots()
         self._free = collections.deque(self._slots)
         self._busy = collections.deque()
         self._queue = collections.deque()
@@ -78,6 +84,18 @@ class PredictServer:
         self.latencies_ms = []
         self.batches = 0
         self._thread.start()
+
+    def _warm_slots(self):
+        """One tiny launch and copy on every slot's stream: the runtime creates
+        a stream's hardware queue at its first use (tens of ms), which must
+        not land on the first requests."""
+        for s in self._slots:
+            s.h_codes[:1].zero_()
+            with torch.cuda.stream(s.stream):
+                s.d_codes[:1].copy_(s.h_codes[:1], non_blocking=True)
+                self.forest.predict_into(s.d_codes[:1], s.d_out[:1], stream=s.stream.cuda_stream, workspace=s.ws)
+                s.h_out[:1].copy_(s.d_out[:1], non_blocking=True)
+            s.stream.synchronize()
 
     # ---------------------------------------------------------------- client
     def submit(self, codes: np.ndarray) -> Future:
