@@ -122,6 +122,17 @@ int wgbm_grad_softmax(const float* scores, const int32_t* labels, const float* w
 int wgbm_edges_from_sorted(const float* sorted, int64_t n, int32_t ncol, int32_t num_bins, float* edges,
                            int32_t edge_row_stride, hipStream_t stream);
 
+/* Quantile edges of every column of a row-major float32 matrix X [n_rows,
+ * ld] (first n_cols columns) without sorting: a radix select of the order
+ * statistics each edge needs, then torch.quantile's linear interpolation.
+ * edges [n_cols][edge_row_stride], num_bins-1 used per column.  Same values
+ * as wgbm_edges_from_sorted on the sorted columns (zeros order -0.0 first).
+ * Replaces the per-feature torch.quantile loop of preprocess_gpu_data
+ * (warpgbm/core.py:1271-1278). */
+size_t wgbm_quantile_edges_workspace_bytes(int64_t n_rows, int32_t n_cols);
+int wgbm_quantile_edges(const float* X, int64_t n_rows, int32_t n_cols, int64_t ld, int32_t num_bins, float* edges,
+                        int32_t edge_row_stride, void* workspace, size_t workspace_bytes, hipStream_t stream);
+
 /* Whole-matrix binning: X [n, n_feat] row-major float32 with edges
  * [n_feat, n_edges] -> codes [n, n_feat] int8, one launch
  * (replaces the per-column loops of warpgbm/core.py:1271-1282, :1893-1898). */
@@ -215,6 +226,12 @@ int wgbm_heap_predict(const int8_t* codes, int64_t n_rows, int32_t n_feat, const
                       const int32_t* class_first, const int32_t* blocks_of_class, int32_t n_classes,
                       int32_t n_blocks, const float* base, float* out, void* workspace, size_t workspace_bytes,
                       hipStream_t stream);
+/* Traversal variant of wgbm_heap_predict, for tuning (process-wide): 0
+ * automatic, 1 node reads through the cache with two trees per lane in
+ * flight, 2 the same with four, 3 nodes staged in LDS 16 trees at a time.
+ * Every variant adds a row's trees in the same order: results are equal bit
+ * for bit.  No counterpart in the reference. */
+int wgbm_set_predict_mode(int32_t mode);
 
 #ifdef __cplusplus
 }

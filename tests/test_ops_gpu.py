@@ -140,8 +140,9 @@ def test_select_conflicting_eras_is_leaf(dev):
     d[1] = -1.0
     req = np.array([-1], np.int8)  # monotone increasing -> required direction -1
     out = torch.empty(2, dtype=torch.int32, device=dev)
-    _lib.call("wgbm_select_split", _t(gain, dev).data_ptr(), _t(d, dev).data_ptr(), E, F, B, 0.0,
-              _t(req, dev).data_ptr(), None, out.data_ptr(), torch.cuda.current_stream().cuda_stream)
+    gd, dd, rd = _t(gain, dev), _t(d, dev), _t(req, dev)  # held until the kernel has run
+    _lib.call("wgbm_select_split", gd.data_ptr(), dd.data_ptr(), E, F, B, 0.0, rd.data_ptr(), None, out.data_ptr(),
+              torch.cuda.current_stream().cuda_stream)
     torch.cuda.synchronize()
     assert out.cpu().tolist() == [-1, -1]
     assert O.select_split(gain, d, 0.0, req) == (-1, -1)
@@ -217,6 +218,57 @@ def test_quantile_edges_and_binning_bit_exact(dev, N, B):
     nz = ok & (e != 0)
     assert np.array_equal(e[nz].view(np.int32), ref_edges[nz].view(np.int32))
     assert np.array_equal(codes.cpu().numpy(), ref_codes)
+
+
+@pytest.mark.parametrize("N,F,nb", [(3000, 100, 31), (777, 68, 127), (513, 7, 9), (64, 4, 1)])
+def test_bin_matrix_any_edges(dev, N, F, nb):
+    """Whole-matrix binning against the early-exit scan for edges that are
+    not sorted and hold NaN (tile path: F % 4 == 0; scalar path otherwise)."""
+    from warpgbm_amd import _lib
+    rng = np.random.default_rng(F + nb)
+    X = rng.standard_normal((N, F)).astype(np.float32)
+    X[rng.random((N, F)) < 0.01] = np.nan
+    edges = np.sort(rng.standard_normal((F, nb)), axis=1).astype(np.float32)
+    edges[::3] = rng.standard_normal((len(edges[::3]), nb)).astype(np.float32)  # unsorted rows
+    edges[1::5, nb // 2] = np.nan
+    codes = torch.empty((N, F), dtype=torch.int8, device=dev)
+    ws = torch.empty(_lib.query("wgbm_bin_matrix_workspace_bytes", F, nb), dtype=torch.uint8, device=dev)
+    st = torch.cuda.current_stream().cuda_stream
+    Xd, Ed = _t(X, dev), _t(edges, dev)  # held: a freed temporary's memory could be reused by the next upload
+    _lib.call("wgbm_bin_matrix", Xd.data_ptr(), N, F, Ed.data_ptr(), nb, codes.data_ptr(), ws.data_ptr(), ws.numel(),
+              st)
+    torch.cuda.synchronize()
+    assert np.array_equal(codes.cpu().numpy(), O.bin_with_edges(X, edges))
+
+
+@pytest.mark.parametrize("N,F,B", [(20000, 9, 32), (12345, 5, 10), (1000, 3, 127), (7, 4, 5), (1, 2, 32),
+                                   (2, 3, 3), (100003, 2, 64)])
+def test_quantile_select_edges_bit_exact(dev, N, F, B):
+    """Radix-select edges equal the oracle's sorted-column edges bit for bit
+    (both order -0.0 before +0.0), ties, signed zeros and NaN columns included."""
+    from warpgbm_amd import _lib
+    rng = np.random.default_rng(N + B)
+    X = rng.standard_normal((N, F)).astype(np.float32)
+    if F > 2:
+        X[:, 1] = np.round(X[:, 1] * 2)  # heavy ties
+        X[:, 2] = np.where(rng.random(N) < 0.5, np.float32(0.0), np.float32(-0.0))  # signed zeros only
+    if N > 5 and F > 3:
+        X[5, 3] = np.nan
+    ld = F + 3  # a row stride wider than the columns
+    Xp = np.zeros((N, ld), np.float32)
+    Xp[:, :F] = X
+    edges = torch.full((F, B + 1), 7.0, dtype=torch.float32, device=dev)
+    ws = torch.empty(_lib.query("wgbm_quantile_edges_workspace_bytes", N, F), dtype=torch.uint8, device=dev)
+    Xd = _t(Xp, dev)
+    _lib.call("wgbm_quantile_edges", Xd.data_ptr(), N, F, ld, B, edges.data_ptr(), B + 1, ws.data_ptr(), ws.numel(),
+              torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    e = edges.cpu().numpy()
+    ref = np.stack([O.quantile_edges_f32(X[:, f], B) for f in range(F)])
+    assert np.array_equal(e[:, :B - 1].view(np.int32), ref.view(np.int32)) or np.array_equal(
+        np.isnan(e[:, :B - 1]), np.isnan(ref)) and np.array_equal(e[:, :B - 1][~np.isnan(ref)].view(np.int32),
+                                                                  ref[~np.isnan(ref)].view(np.int32))
+    assert np.all(e[:, B - 1:] == 7.0)  # padding of the edge rows untouched
 
 
 def test_bin_column_reference_op(dev):

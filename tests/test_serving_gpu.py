@@ -114,3 +114,22 @@ def test_server_rejects_over_budget(model_and_codes):
         assert small.shape == (10, 1)
     finally:
         srv.close()
+
+
+@pytest.mark.parametrize("tree_block,depth_trees", [(64, 30), (7, 30), (16, 30)])
+def test_traversal_variants_equal_bit_for_bit(model_and_codes, tree_block, depth_trees):
+    """Node reads through the cache (2 or 4 trees in flight per lane) and
+    nodes staged in LDS add every row's trees in the same order."""
+    from warpgbm_amd import _lib
+    from warpgbm_amd.serving import HeapForest
+    m, codes = model_and_codes
+    hf = HeapForest.from_model(m, tree_block=tree_block)
+    outs = []
+    try:
+        for mode in (1, 2, 3):
+            _lib.call("wgbm_set_predict_mode", mode)
+            outs.append(hf.predict(codes[:1000 + 37])[:, 0].cpu().numpy())
+    finally:
+        _lib.call("wgbm_set_predict_mode", 0)
+    assert np.array_equal(outs[0], outs[1])
+    assert np.array_equal(outs[0], outs[2])

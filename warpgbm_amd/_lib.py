@@ -68,6 +68,9 @@ SIGNATURES = {
     "wgbm_fit_graph_rounds": (i32, []),
     "wgbm_heap_compile": (C.c_int, [vp, vp, vp, vp, i32, i32, i32, f32, vp, vp, vp]),
     "wgbm_heap_predict_workspace_bytes": (sz, [i64, i32, i32]),
+    "wgbm_set_predict_mode": (C.c_int, [i32]),
+    "wgbm_quantile_edges_workspace_bytes": (sz, [i64, i32]),
+    "wgbm_quantile_edges": (C.c_int, [vp, i64, i32, i64, i32, vp, i32, vp, sz, vp]),
     "wgbm_heap_predict": (C.c_int, [vp, i64, i32, vp, vp, i32, i32, i32, vp, vp, i32, i32, vp, vp, vp, sz, vp]),
 }
 

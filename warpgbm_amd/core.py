@@ -594,18 +594,20 @@ class WarpGBM(BaseEstimator, RegressorMixin):
             return bin_indices, era_indices, bin_edges, unique_eras, Y_gpu
 
     def _quantile_edges(self, X_float):
-        """Per-feature float32 quantile edges: columns sorted on the device in
-        blocks, edges interpolated by the HIP edge kernel."""
+        """Per-feature float32 quantile edges (torch.quantile semantics,
+        warpgbm/core.py:1271-1278) by radix selection on the device, no sort.
+        Columns go in blocks whose column-major copy fits 2 GiB."""
         N, F = X_float.shape
         B = self.num_bins
+        X_float = X_float.contiguous()
         edges = torch.empty((F, B - 1), dtype=torch.float32, device=self.device)
-        blk = max(1, min(F, (1 << 28) // max(N, 1)))
+        blk = max(1, min(F, (2 << 30) // max(4 * N, 1)))
+        ws = self._ws.get("quantile", _lib.query("wgbm_quantile_edges_workspace_bytes", N, blk), self.device)
         for f0 in range(0, F, blk):
             f1 = min(F, f0 + blk)
-            srt = torch.sort(X_float[:, f0:f1], dim=0).values.contiguous()
-            _lib.call("wgbm_edges_from_sorted", srt.data_ptr(), N, f1 - f0, B, edges[f0].data_ptr(), B - 1,
-                      _stream(self.device))
-            del srt
+            _lib.call("wgbm_quantile_edges", X_float.data_ptr() + 4 * f0, N, f1 - f0, F, B, edges[f0].data_ptr(),
+                      B - 1, ws.data_ptr(), ws.numel(), _stream(self.device))
+        self._ws._bufs.pop("quantile", None)  # the column-major copy is not needed after binning
         return edges
 
     def _bin_matrix(self, X_float, edges):

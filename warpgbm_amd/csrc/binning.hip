@@ -49,6 +49,49 @@ k_bin_matrix4(const float4* __restrict__ X, const float* __restrict__ et, uint32
   out[i] = c0 | (c1 << 8) | (c2 << 16) | (c3 << 24);
 }
 
+// Tiled whole-matrix binning (F % 4 == 0, 16-byte aligned rows): a
+// workgroup takes 64 features x kBinRows rows.  The tile's edges sit in LDS
+// as [edge][64 features], so a lane's four features' k-th edges are one
+// 16-byte LDS read.  The code is the count of leading edges with x >= e
+// (a branch-free running AND), which equals the reference's early-exit scan
+// for any edges, NaN included.
+constexpr int kBinRows = 256;
+
+__global__ void __launch_bounds__(256)
+k_bin_tile(const float4* __restrict__ X, const float* __restrict__ et, uint32_t* __restrict__ out, int64_t n,
+           int F, int nb) {
+  __shared__ float4 s_e[127][16];
+  const int f0 = blockIdx.y * 64;
+  const int nf = min(64, F - f0);  // a multiple of 4
+  for (int q = threadIdx.x; q < nb * 16; q += 256) {
+    const int k = q >> 4, c = q & 15;
+    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (4 * c < nf) v = *reinterpret_cast<const float4*>(et + (int64_t)k * F + f0 + 4 * c);
+    s_e[k][c] = v;
+  }
+  __syncthreads();
+  const int c = threadIdx.x & 15;   // float4 column within the tile
+  const int r = threadIdx.x >> 4;   // row within a pass of 16
+  if (4 * c >= nf) return;
+  const int64_t F4 = F >> 2;
+  const int64_t row0 = (int64_t)blockIdx.x * kBinRows;
+  for (int rr = r; rr < kBinRows; rr += 16) {
+    const int64_t row = row0 + rr;
+    if (row >= n) break;
+    const int64_t idx = row * F4 + (f0 >> 2) + c;
+    const float4 v = X[idx];
+    uint32_t a0 = 1, a1 = 1, a2 = 1, a3 = 1, c0 = 0, c1 = 0, c2 = 0, c3 = 0;
+    for (int k = 0; k < nb; ++k) {
+      const float4 e = s_e[k][c];
+      a0 &= (uint32_t)(v.x >= e.x); c0 += a0;
+      a1 &= (uint32_t)(v.y >= e.y); c1 += a1;
+      a2 &= (uint32_t)(v.z >= e.z); c2 += a2;
+      a3 &= (uint32_t)(v.w >= e.w); c3 += a3;
+    }
+    out[idx] = c0 | (c1 << 8) | (c2 << 16) | (c3 << 24);
+  }
+}
+
 __global__ void __launch_bounds__(256)
 k_bin_matrix1(const float* __restrict__ X, const float* __restrict__ et, int8_t* __restrict__ out, int64_t n,
               int F, int nb) {
@@ -84,7 +127,8 @@ k_edges_from_sorted(const float* __restrict__ sorted, int64_t n, int ncol, int n
   } else {
     const float rank = q * (float)(n - 1);
     const float lo_f = floorf(rank), hi_f = ceilf(rank);
-    const int64_t lo = (int64_t)lo_f, hi = (int64_t)hi_f;
+    // (float)(n-1) rounds up above 2^24 rows: keep both ranks in the column
+    const int64_t lo = min((int64_t)lo_f, n - 1), hi = min((int64_t)hi_f, n - 1);
     const float w = rank - lo_f;
     const float a = sorted[lo * ncol + c];
     const float b = sorted[hi * ncol + c];
@@ -138,7 +182,12 @@ extern "C" int wgbm_bin_matrix(const float* X, int64_t n, int32_t F, const float
     WGBM_LAUNCHED("k_transpose_edges");
   }
   const bool vec = (F % 4 == 0) && ((uintptr_t)X % 16 == 0) && ((uintptr_t)out % 4 == 0);
-  if (vec) {
+  if (vec && n_edges > 0 && ceil_div(n, kBinRows) < 0x7fffffff) {
+    const dim3 grid((unsigned)ceil_div(n, kBinRows), (unsigned)ceil_div(F, 64));
+    hipLaunchKernelGGL(k_bin_tile, grid, dim3(256), 0, stream, reinterpret_cast<const float4*>(X), et,
+                       reinterpret_cast<uint32_t*>(out), n, (int)F, (int)n_edges);
+    WGBM_LAUNCHED("k_bin_tile");
+  } else if (vec) {
     const int64_t n4 = total / 4;
     hipLaunchKernelGGL(k_bin_matrix4, dim3((unsigned)ceil_div(n4, 256)), dim3(256), 0, stream,
                        reinterpret_cast<const float4*>(X), et, reinterpret_cast<uint32_t*>(out), n4, (int)F,

@@ -57,17 +57,29 @@ struct HeapForest {
 constexpr int kRowsPerWG = 64;
 
 // Partial sums for (row tile, class, tree block): out_part[(cls*n_blocks +
-// blk) * n + row] = sum over the block's trees (wave order, then tree order).
+// blk) * n + row] = sum over the block's trees.  Wave q of the workgroup
+// adds trees q, q+4, q+8, ... of the block in that order, then the four wave
+// sums are combined in wave order, so the result does not depend on how the
+// traversal is scheduled.
+//
+// NODES_LDS: the block's nodes are staged in LDS kSubTrees trees at a time
+// (every wave walks kSubTrees/4 trees of a sub-block at once, node reads
+// from LDS); otherwise node reads go through the L1/L2 (the whole forest is
+// a few MB) with INFLIGHT trees per lane at once.
+constexpr int kSubTrees = 16;
+
+template <bool NODES_LDS, int INFLIGHT>
 __global__ void __launch_bounds__(256)
 k_predict_heap(const int8_t* __restrict__ codes, int64_t n, int32_t F, HeapForest hf,
                const int32_t* __restrict__ class_first, int32_t n_classes, int32_t n_blocks,
                float* __restrict__ part) {
-  extern __shared__ int8_t s_codes[];  // [64][F]
+  extern __shared__ int8_t s_dyn[];  // [64][F] codes, then (NODES_LDS) [kSubTrees][internal] nodes
   __shared__ float s_sum[4][kRowsPerWG];
   const int64_t row0 = (int64_t)blockIdx.x * kRowsPerWG;
   const int cls = blockIdx.y / n_blocks;
   const int blk = blockIdx.y - cls * n_blocks;
   const int nrows = (int)min((int64_t)kRowsPerWG, n - row0);
+  int8_t* s_codes = s_dyn;
   // stage codes (row-major, contiguous rows) with 4-byte loads where aligned
   const int64_t bytes = (int64_t)nrows * F;
   const int8_t* src = codes + row0 * F;
@@ -79,38 +91,67 @@ k_predict_heap(const int8_t* __restrict__ codes, int64_t n, int32_t F, HeapFores
   } else {
     for (int64_t q = threadIdx.x; q < bytes; q += 256) s_codes[q] = src[q];
   }
-  __syncthreads();
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int t_lo = class_first[cls] + blk * hf.tree_block;
   const int t_hi = min(t_lo + hf.tree_block, class_first[cls + 1]);
   const int internal = (1 << hf.depth) - 1;
   const int8_t* myrow = s_codes + (int64_t)(lane < nrows ? lane : 0) * F;
   float acc = 0.0f;
-  int t = t_lo + wv;
-  // two independent trees in flight per lane
-  for (; t + 4 < t_hi; t += 8) {
-    const uint32_t* n0 = hf.nodes + (int64_t)t * internal;
-    const uint32_t* n1 = hf.nodes + (int64_t)(t + 4) * internal;
-    int a = 0, b = 0;
-    for (int d = 0; d < hf.depth; ++d) {
-      const uint32_t x = n0[a], y = n1[b];
-      const int ca = myrow[x >> 8], cb = myrow[y >> 8];
-      a = 2 * a + 1 + (ca > (int)(x & 0xff));
-      b = 2 * b + 1 + (cb > (int)(y & 0xff));
+  if constexpr (NODES_LDS) {
+    uint32_t* s_nodes = reinterpret_cast<uint32_t*>(s_dyn + (((int64_t)kRowsPerWG * F + 15) & ~(int64_t)15));
+    for (int t0 = t_lo; t0 < t_hi; t0 += kSubTrees) {
+      const int nt = min(kSubTrees, t_hi - t0);
+      __syncthreads();  // previous sub-block's nodes are no longer read (and codes are staged)
+      const uint32_t* g = hf.nodes + (int64_t)t0 * internal;
+      for (int q = threadIdx.x; q < nt * internal; q += 256) s_nodes[q] = g[q];
+      __syncthreads();
+      constexpr int PER = kSubTrees / 4;  // trees of the sub-block per wave
+      int at[PER];
+#pragma unroll
+      for (int k = 0; k < PER; ++k) at[k] = 0;
+      for (int d = 0; d < hf.depth; ++d) {
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+          const int tt = wv + 4 * k;  // tree within the sub-block
+          if (tt < nt) {
+            const uint32_t x = s_nodes[tt * internal + at[k]];
+            at[k] = 2 * at[k] + 1 + ((int)myrow[x >> 8] > (int)(x & 0xff));
+          }
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < PER; ++k) {
+        const int tt = wv + 4 * k;
+        if (tt < nt) acc = acc + hf.leaves[(int64_t)(t0 + tt) * (internal + 1) + (at[k] - internal)];
+      }
     }
-    const float va = hf.leaves[(int64_t)t * (internal + 1) + (a - internal)];
-    const float vb = hf.leaves[(int64_t)(t + 4) * (internal + 1) + (b - internal)];
-    acc = acc + va;
-    acc = acc + vb;
-  }
-  for (; t < t_hi; t += 4) {
-    const uint32_t* n0 = hf.nodes + (int64_t)t * internal;
-    int a = 0;
-    for (int d = 0; d < hf.depth; ++d) {
-      const uint32_t x = n0[a];
-      a = 2 * a + 1 + ((int)myrow[x >> 8] > (int)(x & 0xff));
+  } else {
+    __syncthreads();
+    int t = t_lo + wv;
+    for (; t + 4 * (INFLIGHT - 1) < t_hi; t += 4 * INFLIGHT) {
+      int at[INFLIGHT];
+#pragma unroll
+      for (int k = 0; k < INFLIGHT; ++k) at[k] = 0;
+      for (int d = 0; d < hf.depth; ++d) {
+#pragma unroll
+        for (int k = 0; k < INFLIGHT; ++k) {
+          const uint32_t x = hf.nodes[(int64_t)(t + 4 * k) * internal + at[k]];
+          at[k] = 2 * at[k] + 1 + ((int)myrow[x >> 8] > (int)(x & 0xff));
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < INFLIGHT; ++k)
+        acc = acc + hf.leaves[(int64_t)(t + 4 * k) * (internal + 1) + (at[k] - internal)];
     }
-    acc = acc + hf.leaves[(int64_t)t * (internal + 1) + (a - internal)];
+    for (; t < t_hi; t += 4) {
+      const uint32_t* n0 = hf.nodes + (int64_t)t * internal;
+      int a = 0;
+      for (int d = 0; d < hf.depth; ++d) {
+        const uint32_t x = n0[a];
+        a = 2 * a + 1 + ((int)myrow[x >> 8] > (int)(x & 0xff));
+      }
+      acc = acc + hf.leaves[(int64_t)t * (internal + 1) + (a - internal)];
+    }
   }
   s_sum[wv][lane] = acc;
   __syncthreads();
@@ -119,6 +160,21 @@ k_predict_heap(const int8_t* __restrict__ codes, int64_t n, int32_t F, HeapFores
     const float v = ((s_sum[0][r] + s_sum[1][r]) + s_sum[2][r]) + s_sum[3][r];
     part[((int64_t)cls * n_blocks + blk) * n + row0 + r] = v;
   }
+}
+
+static int g_predict_mode = 0;  // 0 automatic, 1 global x2, 2 global x4, 3 nodes in LDS
+
+// Allow up to 160 KB of dynamic LDS for a kernel, once per device.
+template <bool NODES_LDS, int INFLIGHT>
+static int predict_lds_opt_in(size_t bytes) {
+  static bool done[64] = {};
+  int dev = 0;
+  WGBM_HIP(hipGetDevice(&dev));
+  if (bytes <= 64 * 1024 || (dev >= 0 && dev < 64 && done[dev])) return kOk;
+  WGBM_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_predict_heap<NODES_LDS, INFLIGHT>),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+  if (dev >= 0 && dev < 64) done[dev] = true;
+  return kOk;
 }
 
 // out[row*K + cls] = base[cls] + sum of block partials in block order.
@@ -191,6 +247,14 @@ extern "C" int wgbm_heap_compile(const int32_t* status, const int32_t* feat, con
   return kOk;
 }
 
+// Traversal variant for tuning: 0 automatic, 1 nodes through the cache with
+// two trees per lane in flight, 2 the same with four, 3 nodes staged in LDS.
+extern "C" int wgbm_set_predict_mode(int32_t mode) {
+  WGBM_REQUIRE(mode >= 0 && mode <= 3, "predict mode must be 0..3, got %d", mode);
+  g_predict_mode = mode;
+  return kOk;
+}
+
 extern "C" size_t wgbm_heap_predict_workspace_bytes(int64_t n_rows, int32_t n_classes, int32_t n_blocks) {
   return (size_t)n_rows * n_classes * n_blocks * sizeof(float);
 }
@@ -213,9 +277,29 @@ extern "C" int wgbm_heap_predict(const int8_t* codes, int64_t n_rows, int32_t n_
   HeapForest hf{nodes, leaves, n_trees, depth, tree_block};
   float* part = static_cast<float*>(workspace);
   const dim3 grid((unsigned)ceil_div(n_rows, kRowsPerWG), (unsigned)(n_classes * n_blocks));
-  const size_t lds = (size_t)n_feat * kRowsPerWG;
-  hipLaunchKernelGGL(k_predict_heap, grid, dim3(256), lds, stream, codes, n_rows, n_feat, hf, class_first,
-                     n_classes, n_blocks, part);
+  const size_t code_lds = ((size_t)n_feat * kRowsPerWG + 15) & ~(size_t)15;
+  const size_t node_lds = (size_t)kSubTrees * ((1u << depth) - 1) * 4;
+  int mode = g_predict_mode;
+  if (mode == 0) mode = 2;
+  if (mode == 3 && code_lds + node_lds > 64 * 1024) mode = 2;  // sub-block does not fit beside the codes
+  int rc = kOk;
+  switch (mode) {
+    case 1:
+      rc = predict_lds_opt_in<false, 2>(code_lds);
+      if (rc != kOk) return rc;
+      hipLaunchKernelGGL((k_predict_heap<false, 2>), grid, dim3(256), code_lds, stream, codes, n_rows, n_feat, hf,
+                         class_first, n_classes, n_blocks, part);
+      break;
+    case 3:
+      hipLaunchKernelGGL((k_predict_heap<true, 1>), grid, dim3(256), code_lds + node_lds, stream, codes, n_rows,
+                         n_feat, hf, class_first, n_classes, n_blocks, part);
+      break;
+    default:
+      rc = predict_lds_opt_in<false, 4>(code_lds);
+      if (rc != kOk) return rc;
+      hipLaunchKernelGGL((k_predict_heap<false, 4>), grid, dim3(256), code_lds, stream, codes, n_rows, n_feat, hf,
+                         class_first, n_classes, n_blocks, part);
+  }
   WGBM_LAUNCHED("k_predict_heap");
   hipLaunchKernelGGL(k_predict_heap_final, dim3((unsigned)ceil_div(n_rows * n_classes, 256)), dim3(256), 0, stream,
                      part, n_rows, n_classes, n_blocks, blocks_of_class, base, out);

@@ -24,12 +24,13 @@ import torch
 
 
 class _Request:
-    __slots__ = ("codes", "future", "t_submit", "rows")
+    __slots__ = ("codes", "future", "t_submit", "t_launch", "rows")
 
     def __init__(self, codes, future):
         self.codes = codes
         self.future = future
         self.t_submit = time.perf_counter()
+        self.t_launch = 0.0
         self.rows = codes.shape[0]
 
 
@@ -82,19 +83,27 @@ class PredictServer:
         self._stop = False
         self._thread = threading.Thread(target=self._loop, name="wgbm-predict-server", daemon=True)
         self.latencies_ms = []
+        self.queue_ms = []  # submit -> launch of each completed request
+        self.batch_log = None  # set to a list to record (slot, rows, t_launch, t_copied, t_done) per batch
         self.batches = 0
         self._thread.start()
 
     def _warm_slots(self):
-        """One tiny launch and copy on every slot's stream: the runtime creates
-        a stream's hardware queue at its first use (tens of ms), which must
-        not land on the first requests."""
+        """A launch and large copies on every slot's stream: the runtime
+        creates a stream's kernel queue and its copy-engine queue at their
+        first use (tens of ms each), which must not land on the first
+        requests.  Small and large copies take different engines, so the
+        warm-up copies a few MB each way."""
+        F = self.forest.n_feat
         for s in self._slots:
-            s.h_codes[:1].zero_()
+            n = min(s.h_codes.shape[0], max(1, (4 << 20) // max(F, 1)))
+            s.h_codes[:n].zero_()
             with torch.cuda.stream(s.stream):
-                s.d_codes[:1].copy_(s.h_codes[:1], non_blocking=True)
-                self.forest.predict_into(s.d_codes[:1], s.d_out[:1], stream=s.stream.cuda_stream, workspace=s.ws)
-                s.h_out[:1].copy_(s.d_out[:1], non_blocking=True)
+                for m in (1, n):
+                    s.d_codes[:m].copy_(s.h_codes[:m], non_blocking=True)
+                    self.forest.predict_into(s.d_codes[:m], s.d_out[:m], stream=s.stream.cuda_stream,
+                                             workspace=s.ws)
+                    s.h_out[:m].copy_(s.d_out[:m], non_blocking=True)
             s.stream.synchronize()
 
     # ---------------------------------------------------------------- client
@@ -167,6 +176,9 @@ class PredictServer:
 
     def _launch(self, slot, reqs, rows, nbytes):
         o = 0
+        t = time.perf_counter()
+        for r in reqs:
+            r.t_launch = t
         hc = slot.h_codes.numpy()
         for r in reqs:
             hc[o:o + r.rows] = r.codes
@@ -179,6 +191,7 @@ class PredictServer:
             ev = torch.cuda.Event()
             ev.record(slot.stream)
         slot.event, slot.reqs, slot.rows, slot.bytes = ev, reqs, rows, nbytes
+        slot.t_launch, slot.t_copied = t, time.perf_counter()
         self.batches += 1
         self._busy.append(slot)
 
@@ -187,12 +200,16 @@ class PredictServer:
         done = False
         while self._busy and self._busy[0].event.query():
             slot = self._busy.popleft()
+            if self.batch_log is not None:
+                self.batch_log.append((self._slots.index(slot), slot.rows, slot.t_launch, slot.t_copied,
+                                       time.perf_counter()))
             out = slot.h_out.numpy()
             o = 0
             now = time.perf_counter()
             for r in slot.reqs:
                 r.future.set_result(out[o:o + r.rows].copy())
                 self.latencies_ms.append((now - r.t_submit) * 1e3)
+                self.queue_ms.append((r.t_launch - r.t_submit) * 1e3)
                 o += r.rows
             with self._cv:
                 self._in_flight_bytes -= slot.bytes
