@@ -190,6 +190,38 @@ k_transpose_codes(const int8_t* __restrict__ codes, int64_t n_rows, int64_t n_co
   }
 }
 
+// Same transpose with 4-byte loads and stores (n_cols % 4 == 0, n_rows % 4
+// == 0, 4-byte aligned): tile rows are read as words, and each output word
+// packs one column's codes of 4 consecutive rows.
+__global__ void __launch_bounds__(256)
+k_transpose_codes4(const int8_t* __restrict__ codes, int64_t n_rows, int64_t n_cols, int8_t* __restrict__ codes_t) {
+  __shared__ uint32_t tile[64][17];  // [row][word]: 64 rows x 64 codes (+1 word pad)
+  const int64_t r0 = (int64_t)blockIdx.x * 64, c0 = (int64_t)blockIdx.y * 64;
+  const int t = threadIdx.x;
+  const int w = t & 15;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int r = (t >> 4) + 16 * k;
+    const int64_t rr = r0 + r, cc = c0 + 4 * w;
+    tile[r][w] = (rr < n_rows && cc < n_cols) ? *reinterpret_cast<const uint32_t*>(codes + rr * n_cols + cc) : 0u;
+  }
+  __syncthreads();
+  const int q = t & 15;  // output word: rows 4q .. 4q+3
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int c = (t >> 4) + 16 * k;  // column within the tile
+    const int64_t cc = c0 + c, rr = r0 + 4 * q;
+    if (cc < n_cols && rr < n_rows) {
+      const int sh = 8 * (c & 3);
+      const uint32_t b0 = (tile[4 * q + 0][c >> 2] >> sh) & 0xffu;
+      const uint32_t b1 = (tile[4 * q + 1][c >> 2] >> sh) & 0xffu;
+      const uint32_t b2 = (tile[4 * q + 2][c >> 2] >> sh) & 0xffu;
+      const uint32_t b3 = (tile[4 * q + 3][c >> 2] >> sh) & 0xffu;
+      *reinterpret_cast<uint32_t*>(codes_t + cc * n_rows + rr) = b0 | (b1 << 8) | (b2 << 16) | (b3 << 24);
+    }
+  }
+}
+
 __global__ void __launch_bounds__(256)
 k_root_perm(const int32_t* __restrict__ order, int64_t N, int K, int32_t* __restrict__ perm) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -541,9 +573,16 @@ extern "C" int wgbm_fit_rounds(const wgbm_fit_config* cfg, const wgbm_fit_io* io
   hipLaunchKernelGGL(k_root_perm, dim3((unsigned)ceil_div(L.KN, 256)), dim3(256), 0, st, io->root_order, N, K,
                      I(L.perm_root));
   WGBM_LAUNCHED("k_root_perm");
-  hipLaunchKernelGGL(k_transpose_codes, dim3((unsigned)ceil_div(N, 64), (unsigned)ceil_div(c.n_cols, 64)), dim3(256),
-                     0, st, io->codes, N, c.n_cols, reinterpret_cast<int8_t*>(ws + L.codes_t));
-  WGBM_LAUNCHED("k_transpose_codes");
+  const bool words = c.n_cols % 4 == 0 && N % 4 == 0 && (uintptr_t)io->codes % 4 == 0;
+  if (words) {
+    hipLaunchKernelGGL(k_transpose_codes4, dim3((unsigned)ceil_div(N, 64), (unsigned)ceil_div(c.n_cols, 64)),
+                       dim3(256), 0, st, io->codes, N, c.n_cols, reinterpret_cast<int8_t*>(ws + L.codes_t));
+    WGBM_LAUNCHED("k_transpose_codes4");
+  } else {
+    hipLaunchKernelGGL(k_transpose_codes, dim3((unsigned)ceil_div(N, 64), (unsigned)ceil_div(c.n_cols, 64)),
+                       dim3(256), 0, st, io->codes, N, c.n_cols, reinterpret_cast<int8_t*>(ws + L.codes_t));
+    WGBM_LAUNCHED("k_transpose_codes");
+  }
 
   g_graph_rounds = 0;
   hipLaunchKernelGGL(k_round_init, dim3(1), dim3(64), 0, st, I(L.round_ctr), 0);

@@ -61,6 +61,7 @@ __global__ void __launch_bounds__(256)
 k_bin_tile(const float4* __restrict__ X, const float* __restrict__ et, uint32_t* __restrict__ out, int64_t n,
            int F, int nb) {
   __shared__ float4 s_e[127][16];
+  __shared__ int s_sorted[16];
   const int f0 = blockIdx.y * 64;
   const int nf = min(64, F - f0);  // a multiple of 4
   for (int q = threadIdx.x; q < nb * 16; q += 256) {
@@ -70,25 +71,59 @@ k_bin_tile(const float4* __restrict__ X, const float* __restrict__ et, uint32_t*
     s_e[k][c] = v;
   }
   __syncthreads();
+  if (threadIdx.x < 16) {  // are the column's four edge lists non-decreasing and NaN-free?
+    const int c = threadIdx.x;
+    bool ok = true;
+    float4 p = s_e[0][c];
+    ok = !(p.x != p.x || p.y != p.y || p.z != p.z || p.w != p.w);
+    for (int k = 1; k < nb; ++k) {
+      const float4 e = s_e[k][c];
+      ok = ok && (e.x >= p.x) && (e.y >= p.y) && (e.z >= p.z) && (e.w >= p.w);  // false for NaN
+      p = e;
+    }
+    s_sorted[c] = ok;
+  }
+  __syncthreads();
   const int c = threadIdx.x & 15;   // float4 column within the tile
   const int r = threadIdx.x >> 4;   // row within a pass of 16
   if (4 * c >= nf) return;
+  // Sorted NaN-free edges: the leading count is the plain count of edges <= x.
+  const bool sorted = s_sorted[c] != 0;
   const int64_t F4 = F >> 2;
   const int64_t row0 = (int64_t)blockIdx.x * kBinRows;
-  for (int rr = r; rr < kBinRows; rr += 16) {
-    const int64_t row = row0 + rr;
-    if (row >= n) break;
-    const int64_t idx = row * F4 + (f0 >> 2) + c;
-    const float4 v = X[idx];
-    uint32_t a0 = 1, a1 = 1, a2 = 1, a3 = 1, c0 = 0, c1 = 0, c2 = 0, c3 = 0;
-    for (int k = 0; k < nb; ++k) {
-      const float4 e = s_e[k][c];
-      a0 &= (uint32_t)(v.x >= e.x); c0 += a0;
-      a1 &= (uint32_t)(v.y >= e.y); c1 += a1;
-      a2 &= (uint32_t)(v.z >= e.z); c2 += a2;
-      a3 &= (uint32_t)(v.w >= e.w); c3 += a3;
+  constexpr int RB = 4;  // rows per thread whose loads are in flight together
+  for (int rr = r; rr < kBinRows; rr += 16 * RB) {
+    float4 v[RB];
+    int64_t idx[RB];
+#pragma unroll
+    for (int j = 0; j < RB; ++j) {
+      const int64_t row = row0 + rr + 16 * j;
+      idx[j] = row < n ? row * F4 + (f0 >> 2) + c : -1;
+      v[j] = idx[j] >= 0 ? X[idx[j]] : make_float4(0.f, 0.f, 0.f, 0.f);
     }
-    out[idx] = c0 | (c1 << 8) | (c2 << 16) | (c3 << 24);
+#pragma unroll
+    for (int j = 0; j < RB; ++j) {
+      uint32_t c0 = 0, c1 = 0, c2 = 0, c3 = 0;
+      if (sorted) {
+        for (int k = 0; k < nb; ++k) {
+          const float4 e = s_e[k][c];
+          c0 += (uint32_t)(v[j].x >= e.x);
+          c1 += (uint32_t)(v[j].y >= e.y);
+          c2 += (uint32_t)(v[j].z >= e.z);
+          c3 += (uint32_t)(v[j].w >= e.w);
+        }
+      } else {
+        uint32_t a0 = 1, a1 = 1, a2 = 1, a3 = 1;
+        for (int k = 0; k < nb; ++k) {
+          const float4 e = s_e[k][c];
+          a0 &= (uint32_t)(v[j].x >= e.x); c0 += a0;
+          a1 &= (uint32_t)(v[j].y >= e.y); c1 += a1;
+          a2 &= (uint32_t)(v[j].z >= e.z); c2 += a2;
+          a3 &= (uint32_t)(v[j].w >= e.w); c3 += a3;
+        }
+      }
+      if (idx[j] >= 0) out[idx[j]] = c0 | (c1 << 8) | (c2 << 16) | (c3 << 24);
+    }
   }
 }
 
