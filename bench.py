@@ -41,6 +41,8 @@ def parse():
     p.add_argument("--no-plain", action="store_true")
     p.add_argument("--profile-only", action="store_true", help="one fit, no extras (for rocprof)")
     p.add_argument("--no-serve", action="store_true", help="skip the served-predict measurement")
+    p.add_argument("--no-graph", action="store_true",
+                   help="launch every round eagerly (no hipGraph replay), e.g. under counter collection")
     p.add_argument("--serve-seconds", type=float, default=2.0, help="arrival window per serving run")
     p.add_argument("--normal-edges", action="store_true",
                    help="bin with the exact N(0,1) quantiles instead of sorting the columns (no sort kernels: "
@@ -314,6 +316,8 @@ def main():
     from warpgbm_amd.serving import HeapForest
 
     dev = torch.device("cuda:0")
+    if args.no_graph:
+        _lib.call("wgbm_set_fit_graph", 0)
     log(f"device {torch.cuda.get_device_name(0)}; generating {args.rows}x{args.features}")
     X, y = make_data(args, dev)
     t0 = time.perf_counter()

@@ -207,6 +207,10 @@ int wgbm_fit_rounds(const wgbm_fit_config* config, const wgbm_fit_io* io, int32_
  * the call ran eagerly: fewer than 3 rounds, or histogram timing on).
  * Diagnostic; no counterpart in the reference. */
 int32_t wgbm_fit_graph_rounds(void);
+/* Process-wide switch for that capture (default on); off, every round is
+ * launched eagerly (e.g. under per-dispatch counter collection).  No
+ * counterpart in the reference. */
+int wgbm_set_fit_graph(int32_t enable);
 
 /* ---------------------------------------------------------------- serving predictor */
 

@@ -66,6 +66,7 @@ SIGNATURES = {
     "wgbm_fit_workspace_bytes": (sz, [C.POINTER(FitConfig)]),
     "wgbm_fit_rounds": (C.c_int, [C.POINTER(FitConfig), C.POINTER(FitIO), i32, vp, sz, vp]),
     "wgbm_fit_graph_rounds": (i32, []),
+    "wgbm_set_fit_graph": (C.c_int, [i32]),
     "wgbm_heap_compile": (C.c_int, [vp, vp, vp, vp, i32, i32, i32, f32, vp, vp, vp]),
     "wgbm_heap_predict_workspace_bytes": (sz, [i64, i32, i32]),
     "wgbm_set_predict_mode": (C.c_int, [i32]),

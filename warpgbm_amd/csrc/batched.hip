@@ -542,8 +542,14 @@ __global__ void k_round_init(int32_t* round_ctr, int r0) {
 // Rounds replayed from one captured round need at least this many rounds.
 constexpr int kGraphMinRounds = 3;
 static thread_local int g_graph_rounds = 0;  // rounds of the last call replayed from a graph
+static int g_graph_enabled = 1;
 
 extern "C" int32_t wgbm_fit_graph_rounds(void) { return g_graph_rounds; }
+
+extern "C" int wgbm_set_fit_graph(int32_t enable) {
+  g_graph_enabled = enable != 0;
+  return kOk;
+}
 
 // Non-blocking stream (default priority) per device for graph replays.
 static hipStream_t graph_stream() {
@@ -587,7 +593,7 @@ extern "C" int wgbm_fit_rounds(const wgbm_fit_config* cfg, const wgbm_fit_io* io
   g_graph_rounds = 0;
   hipLaunchKernelGGL(k_round_init, dim3(1), dim3(64), 0, st, I(L.round_ctr), 0);
   WGBM_LAUNCHED("k_round_init");
-  if (n_rounds < kGraphMinRounds || hist_timer_on()) {
+  if (n_rounds < kGraphMinRounds || hist_timer_on() || !g_graph_enabled) {
     for (int r = 0; r < n_rounds; ++r) {
       rc = enqueue_round(c, L, io, ws, st);
       if (rc != kOk) return rc;
