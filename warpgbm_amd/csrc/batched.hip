@@ -31,11 +31,12 @@ struct FitLayout {
   size_t perm_root, gh_root, perm_pp[2], gh_pp[2];
   size_t hg[2], hh[2];
   struct Tables { size_t seg_start, seg_count, node_active, node_start, node_count, stats; } tab[2];
-  size_t dec_feat, dec_bin, era_gain, cand_agree, cand_crit, cand_idx;
+  size_t dec_feat, dec_bin, era_gain, cand_agree, cand_crit, cand_idx, era_g, era_d;
+  int64_t gain_nodes;
   size_t b_seg_start, b_seg_count, b_seg_slot, n_build, sib, n_split;
   size_t h_seg_first, h_seg_pfirst, h_nseg, h_multi, h_n_multi, h_chunk, h_part;
   size_t stat_part, round_ctr, fidx_cur;
-  size_t p_flag, p_seg_first, p_nseg, p_multi, p_n_multi, p_lbase, p_rbase, p_cscan, p_segL, codes_t;
+  size_t p_flag, p_seg_first, p_nseg, p_multi, p_n_multi, p_lbase, p_rbase, p_cscan, p_segL, p_segP, codes_t;
   size_t l_seg_first, l_nseg, l_multi, l_n_multi, l_part, l_val;
   size_t root_seg_start, root_seg_count, root_seg_slot, root_nseg, era_start;
   // sizes
@@ -76,7 +77,12 @@ static FitLayout fit_layout(const wgbm_fit_config& c) {
     L.tab[i].node_count = take(4 * L.max_nodes);
     L.tab[i].stats = take(4 * kStatWords * L.max_nodes);
   }
-  const int64_t wgpn = level_scan_groups(c.n_feat);
+  const int64_t wgpn = level_scan_groups(c.n_feat, c.n_eras, c.n_bins);
+  // E > 1: per-era gains of as many nodes as fit 1 GiB, scanned in groups
+  L.gain_nodes = level_scan_gain_nodes(L.hist_nodes, c.n_eras, c.n_feat, c.n_bins, (size_t)1 << 30);
+  const int64_t gvals = L.gain_nodes * E * c.n_feat * (int64_t)(c.n_bins - 1);
+  L.era_g = take(4 * gvals);
+  L.era_d = take(gvals);
   L.dec_feat = take(4 * L.max_nodes);
   L.dec_bin = take(4 * L.max_nodes);
   L.era_gain = take(4 * MS);
@@ -108,6 +114,7 @@ static FitLayout fit_layout(const wgbm_fit_config& c) {
   L.p_cscan = take(4 * (L.part_chunks + 1));
   L.codes_t = take((size_t)N * c.n_cols);  // column-major copy of the bin matrix
   L.p_segL = take(4 * MS);
+  L.p_segP = take(4 * 2 * (MS + 1));
   // statistics partials: per block of the objective (K classes) or per
   // partition chunk and child side
   const int64_t stat_parts = (int64_t)K * kStatBlocks > 2 * L.part_chunks ? (int64_t)K * kStatBlocks : 2 * L.part_chunks;
@@ -278,8 +285,11 @@ k_record_level(int d, int D, int K, int E, int n_nodes, const int32_t* __restric
                const int32_t* __restrict__ node_count, const float* __restrict__ era_gain,
                const int32_t* __restrict__ round_ctr, int32_t* status, int32_t* feat, int32_t* bin, float* value,
                int32_t* count, float* out_era_gain) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= n_nodes) return;
+  // thread per (node, era): era 0's thread writes the node's fields
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)n_nodes * E) return;
+  const int j = (int)(t / E);
+  const int e0 = (int)(t - (int64_t)j * E);
   const int NH = (2 << D) - 1;
   const int c = j >> d, jj = j & ((1 << d) - 1);
   const int64_t o = ((int64_t)round_ctr[0] * K + c) * NH + ((1 << d) - 1) + jj;
@@ -295,14 +305,14 @@ k_record_level(int d, int D, int K, int E, int n_nodes, const int32_t* __restric
       v = leaf_val[j];
     }
   }
-  status[o] = st;
-  feat[o] = f;
-  bin[o] = b;
-  value[o] = v;
-  count[o] = active[j] ? node_count[j] : 0;
-  if (out_era_gain)
-    for (int e = 0; e < E; ++e)
-      out_era_gain[o * E + e] = (st == 1) ? era_gain[(int64_t)j * E + e] : 0.0f;
+  if (e0 == 0) {
+    status[o] = st;
+    feat[o] = f;
+    bin[o] = b;
+    value[o] = v;
+    count[o] = active[j] ? node_count[j] : 0;
+  }
+  if (out_era_gain) out_era_gain[o * E + e0] = (st == 1) ? era_gain[(int64_t)j * E + e0] : 0.0f;
 }
 
 }  // namespace wgbm
@@ -454,6 +464,9 @@ static int enqueue_round(const wgbm_fit_config& c, const FitLayout& L, const wgb
         s.dec_feat = I(L.dec_feat);
         s.dec_bin = I(L.dec_bin);
         s.era_gain = Fp(L.era_gain);
+        s.era_g = Fp(L.era_g);
+        s.era_d = reinterpret_cast<int8_t*>(ws + L.era_d);
+        s.gain_nodes = L.gain_nodes;
         rc = level_scan_launch(s, st);
         if (rc != kOk) return rc;
       }
@@ -480,7 +493,7 @@ static int enqueue_round(const wgbm_fit_config& c, const FitLayout& L, const wgb
       la.part = reinterpret_cast<double*>(ws + L.l_part);
       rc = leaf_launch(la, L.KN, st);
       if (rc != kOk) return rc;
-      hipLaunchKernelGGL(k_record_level, dim3((unsigned)ceil_div(n_nodes, 256)), dim3(256), 0, st, d, D, K, E,
+      hipLaunchKernelGGL(k_record_level, dim3((unsigned)ceil_div((int64_t)n_nodes * E, 256)), dim3(256), 0, st, d, D, K, E,
                          n_nodes, I(cur.node_active), I(L.dec_feat), I(L.dec_bin), fidx, Fp(L.l_val),
                          I(cur.node_count), Fp(L.era_gain), I(L.round_ctr), io->node_status, io->node_feat,
                          io->node_bin, io->node_value, io->node_count, io->node_era_gain);
@@ -514,6 +527,7 @@ static int enqueue_round(const wgbm_fit_config& c, const FitLayout& L, const wgb
         p.cscan = I(L.p_cscan);
         p.codes_t = reinterpret_cast<const int8_t*>(ws + L.codes_t);
         p.segL = I(L.p_segL);
+        p.segP = I(L.p_segP);
         p.c_seg_start = I(nxt.seg_start);
         p.c_seg_count = I(nxt.seg_count);
         p.c_active = I(nxt.node_active);

@@ -104,35 +104,54 @@ k_part_scan(PartArgs a) {
   for (int s = threadIdx.x; s < nseg; s += blockDim.x)
     a.segL[s] = is_split(a, s / E) ? a.cscan[a.seg_first[s + 1]] - a.cscan[a.seg_first[s]] : 0;
   __syncthreads();
-  // B: child tables per node.
+  // B: child tables.  Exclusive scans over all segments of the left and
+  // right counts give each era's offset inside its node's children
+  // (segP[0..nseg] left, segP[nseg+1..2nseg+1] right).
+  int* PL = a.segP;
+  int* PR = a.segP + nseg + 1;
+  int cl = 0, cr = 0;
+  for (int base = 0; base < nseg; base += blockDim.x) {
+    const int s = base + threadIdx.x;
+    const bool sp = s < nseg && is_split(a, s / E);
+    const int L = sp ? a.segL[s] : 0;
+    const int R = sp ? max(a.seg_count[s], 0) - L : 0;
+    int tl, tr;
+    const int el = block_exscan(L, &tl);
+    const int er = block_exscan(R, &tr);
+    if (s < nseg) { PL[s] = cl + el; PR[s] = cr + er; }
+    cl += tl;
+    cr += tr;
+  }
+  if (threadIdx.x == 0) { PL[nseg] = cl; PR[nseg] = cr; }
+  __syncthreads();
   for (int j = threadIdx.x; j < a.n_nodes; j += blockDim.x) {
-    const bool split = is_split(a, j);
     const int l = 2 * j, r = 2 * j + 1;
-    if (!split) {
+    if (!is_split(a, j)) {
       a.c_active[l] = 0; a.c_active[r] = 0;
       a.c_node_start[l] = 0; a.c_node_start[r] = 0;
       a.c_node_count[l] = 0; a.c_node_count[r] = 0;
-      for (int e = 0; e < E; ++e) {
-        a.c_seg_start[l * E + e] = 0; a.c_seg_count[l * E + e] = 0;
-        a.c_seg_start[r * E + e] = 0; a.c_seg_count[r * E + e] = 0;
-      }
       continue;
     }
     const int start = a.node_start[j];
-    int Lt = 0;
-    for (int e = 0; e < E; ++e) Lt += a.segL[j * E + e];
-    const int Rt = a.node_count[j] - Lt;
-    int runl = start, runr = start + Lt;
-    for (int e = 0; e < E; ++e) {
-      const int L = a.segL[j * E + e];
-      const int R = max(a.seg_count[j * E + e], 0) - L;
-      a.c_seg_start[l * E + e] = runl; a.c_seg_count[l * E + e] = L;
-      a.c_seg_start[r * E + e] = runr; a.c_seg_count[r * E + e] = R;
-      runl += L; runr += R;
-    }
+    const int Lt = PL[(j + 1) * E] - PL[j * E];
     a.c_active[l] = 1; a.c_active[r] = 1;
     a.c_node_start[l] = start; a.c_node_count[l] = Lt;
-    a.c_node_start[r] = start + Lt; a.c_node_count[r] = Rt;
+    a.c_node_start[r] = start + Lt; a.c_node_count[r] = a.node_count[j] - Lt;
+  }
+  for (int s = threadIdx.x; s < nseg; s += blockDim.x) {
+    const int j = s / E, e = s - j * E;
+    const int l = 2 * j, r = 2 * j + 1;
+    if (!is_split(a, j)) {
+      a.c_seg_start[l * E + e] = 0; a.c_seg_count[l * E + e] = 0;
+      a.c_seg_start[r * E + e] = 0; a.c_seg_count[r * E + e] = 0;
+      continue;
+    }
+    const int start = a.node_start[j];
+    const int Lt = PL[(j + 1) * E] - PL[j * E];
+    a.c_seg_start[l * E + e] = start + PL[s] - PL[j * E];
+    a.c_seg_count[l * E + e] = PL[s + 1] - PL[s];
+    a.c_seg_start[r * E + e] = start + Lt + PR[s] - PR[j * E];
+    a.c_seg_count[r * E + e] = PR[s + 1] - PR[s];
   }
   __syncthreads();
   // C: absolute destination bases per chunk (only split segments own chunks).
@@ -513,6 +532,7 @@ extern "C" int wgbm_partition_rows(const int8_t* bin_indices, int64_t n_total_ro
   a.c_node_start = sc + 18; a.c_node_count = sc + 20;
   a.b_seg_start = sc + 22; a.b_seg_count = sc + 23; a.b_seg_slot = sc + 24;
   a.n_build = sc + 25; a.n_split = sc + 26; a.sib = sc + 27;  // 3 ints
+  a.segP = sc + 32;                                             // 4 ints
   a.multi = multi;
   a.flag = reinterpret_cast<uint8_t*>(multi + mc);
   a.lbase = lbase; a.rbase = rbase; a.cscan = cscan;

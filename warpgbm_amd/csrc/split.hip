@@ -214,10 +214,9 @@ k_level_scan(const LevelScanArgs a) {
 
 // Winner per node, then (optionally) its per-era gains.
 __global__ void __launch_bounds__(64)
-k_level_pick(const LevelScanArgs a) {
+k_level_pick(const LevelScanArgs a, int wg_per_node) {
   const int node = blockIdx.x;
   if (node >= a.n_nodes) return;
-  const int wg_per_node = (a.n_feat + kScanFeat - 1) / kScanFeat;
   Cand b = cand_none();
   if (a.node_active[node]) {
     for (int w = threadIdx.x; w < wg_per_node; w += 64) {
@@ -255,15 +254,116 @@ k_level_pick(const LevelScanArgs a) {
   }
 }
 
-int64_t level_scan_groups(int n_feat) { return n_feat > 0 ? ceil_div(n_feat, kScanFeat) : 1; }
+constexpr int kSelBlock = 256;  // (feature, threshold) candidates per workgroup of the E > 1 selection
+
+int64_t level_scan_groups(int n_feat, int n_eras, int n_bins) {
+  if (n_feat <= 0) return 1;
+  if (n_eras == 1) return ceil_div(n_feat, kScanFeat);
+  return ceil_div((int64_t)n_feat * (n_bins - 1), kSelBlock);
+}
+
+int64_t level_scan_gain_nodes(int64_t n_nodes, int n_eras, int n_feat, int n_bins, size_t cap) {
+  if (n_eras == 1) return 0;
+  const int64_t per = (int64_t)n_eras * n_feat * (n_bins - 1) * 5;  // float gain + int8 direction
+  int64_t k = per > 0 ? (int64_t)(cap / per) : n_nodes;
+  if (k < 1) k = 1;
+  return k < n_nodes ? k : n_nodes;
+}
+
+// E > 1, pass 1: per-era gain and direction of every (node, era, feature,
+// threshold), exactly as k_compute_split forms them (float64 prefix in bin
+// order, one rounding).  Thread per (node of the group, era, feature).
+__global__ void __launch_bounds__(256)
+k_level_gains(const LevelScanArgs a, int node0, int n_group) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int E = a.n_eras, F = a.n_feat, B = a.n_bins;
+  if (t >= (int64_t)n_group * E * F) return;
+  const int jl = (int)(t / ((int64_t)E * F));
+  if (!a.node_active[node0 + jl]) return;
+  const int64_t off = ((int64_t)node0 * E * F + t) * B;  // histogram [node][era][feature][bin]
+  const float* g = a.G + off;
+  const float* h = a.H + off;
+  double Gt = 0.0, Ht = 0.0;
+  for (int b = 0; b < B; ++b) { Gt += (double)g[b]; Ht += (double)h[b]; }
+  double GL = 0.0, HL = 0.0;
+  float* go = a.era_g + t * (B - 1);
+  int8_t* dO = a.era_d + t * (B - 1);
+  for (int b = 0; b < B - 1; ++b) {
+    GL += (double)g[b];
+    HL += (double)h[b];
+    float ga, di;
+    gain_dir(GL, HL, Gt, Ht, (double)a.mcw, (double)a.lam, ga, di);
+    go[b] = ga;
+    dO[b] = (int8_t)di;
+  }
+}
+
+// E > 1, pass 2: thread per (node, feature, threshold) sums its per-era gains
+// and directions in era order (eval_cand's order) and applies the monotone
+// mask; one candidate per workgroup of kSelBlock thresholds.
+__global__ void __launch_bounds__(kSelBlock)
+k_level_sel(const LevelScanArgs a, int node0, int n_group) {
+  const int E = a.n_eras, F = a.n_feat, B = a.n_bins;
+  const int FB = F * (B - 1);
+  const int groups = (FB + kSelBlock - 1) / kSelBlock;
+  const int jl = blockIdx.x / groups;
+  const int wg = blockIdx.x - jl * groups;
+  const int node = node0 + jl;
+  const int c = wg * kSelBlock + threadIdx.x;
+  Cand best = cand_none();
+  if (jl < n_group && a.node_active[node] && c < FB) {
+    const int f = c / (B - 1);
+    const int req = a.mono_req ? (int)a.mono_req[a.feat_idx ? a.feat_idx[f] : f] : 0;
+    const float* gp = a.era_g + (int64_t)jl * E * FB + c;
+    const int8_t* dp = a.era_d + (int64_t)jl * E * FB + c;
+    float sg = 0.0f;
+    int sd = 0;
+    bool mono = true;
+    for (int e = 0; e < E; ++e) {
+      const int d = dp[(int64_t)e * FB];
+      sg += gp[(int64_t)e * FB];
+      sd += d;
+      mono = mono && (req == 0 || d == req);
+    }
+    const float cr = sg / (float)E;
+    best = Cand{sd < 0 ? -sd : sd, (cr > a.msg && mono) ? cr : -INFINITY, c};
+  }
+  __shared__ Cand s_c[kSelBlock / 64];
+  best = wave_best(best);
+  if ((threadIdx.x & 63) == 0) s_c[threadIdx.x >> 6] = best;
+  __syncthreads();
+  if (threadIdx.x == 0 && jl < n_group) {
+    Cand b = s_c[0];
+    for (int w = 1; w < kSelBlock / 64; ++w)
+      if (cand_better(s_c[w], b)) b = s_c[w];
+    const int64_t o = (int64_t)node * groups + wg;
+    a.cand_agree[o] = b.agree;
+    a.cand_crit[o] = b.crit;
+    a.cand_idx[o] = b.idx;
+  }
+}
 
 int level_scan_launch(const LevelScanArgs& a, hipStream_t st) {
   if (a.n_nodes == 0) return kOk;
-  const int wg_per_node = (int)ceil_div(a.n_feat, kScanFeat);
-  hipLaunchKernelGGL(k_level_scan, dim3((unsigned)(a.n_nodes * (int64_t)wg_per_node)), dim3(64 * kScanFeat), 0,
-                     st, a);
-  WGBM_LAUNCHED("k_level_scan");
-  hipLaunchKernelGGL(k_level_pick, dim3((unsigned)a.n_nodes), dim3(64), 0, st, a);
+  if (a.n_eras == 1) {
+    const int wg_per_node = (int)ceil_div(a.n_feat, kScanFeat);
+    hipLaunchKernelGGL(k_level_scan, dim3((unsigned)(a.n_nodes * (int64_t)wg_per_node)), dim3(64 * kScanFeat), 0,
+                       st, a);
+    WGBM_LAUNCHED("k_level_scan");
+  } else {
+    WGBM_REQUIRE(a.era_g != nullptr && a.era_d != nullptr && a.gain_nodes >= 1, "E > 1 scan needs gain buffers");
+    const int64_t groups = level_scan_groups(a.n_feat, a.n_eras, a.n_bins);
+    for (int64_t n0 = 0; n0 < a.n_nodes; n0 += a.gain_nodes) {
+      const int ng = (int)(a.n_nodes - n0 < a.gain_nodes ? a.n_nodes - n0 : a.gain_nodes);
+      const int64_t th = (int64_t)ng * a.n_eras * a.n_feat;
+      hipLaunchKernelGGL(k_level_gains, dim3((unsigned)ceil_div(th, 256)), dim3(256), 0, st, a, (int)n0, ng);
+      WGBM_LAUNCHED("k_level_gains");
+      hipLaunchKernelGGL(k_level_sel, dim3((unsigned)(ng * groups)), dim3(kSelBlock), 0, st, a, (int)n0, ng);
+      WGBM_LAUNCHED("k_level_sel");
+    }
+  }
+  hipLaunchKernelGGL(k_level_pick, dim3((unsigned)a.n_nodes), dim3(64), 0, st, a,
+                     (int)level_scan_groups(a.n_feat, a.n_eras, a.n_bins));
   WGBM_LAUNCHED("k_level_pick");
   return kOk;
 }

@@ -88,9 +88,17 @@ struct LevelScanArgs {
   int32_t* dec_feat;            // [n_nodes] local feature or -1
   int32_t* dec_bin;             // [n_nodes] bin or -1
   float* era_gain;              // [n_nodes][E] gain of the winner, or nullptr
+  // E > 1: per-era gains and directions of a group of nodes
+  // ([nodes of the group][E][n_feat][B-1]); level_scan_gain_nodes(...) nodes
+  // fit in the buffers.
+  float* era_g;
+  int8_t* era_d;
+  int64_t gain_nodes;           // nodes whose gains fit era_g / era_d
 };
 int level_scan_launch(const LevelScanArgs& a, hipStream_t st);
-int64_t level_scan_groups(int n_feat);  // candidate slots per node
+int64_t level_scan_groups(int n_feat, int n_eras, int n_bins);  // candidate slots per node
+// Nodes per pass of the E > 1 scan for a gain buffer of at most cap bytes.
+int64_t level_scan_gain_nodes(int64_t n_nodes, int n_eras, int n_feat, int n_bins, size_t cap);
 
 // Chunk plan shared by histogram, partition and leaf kernels: the row
 // count of segment s is seg_count[s] (segments), node_count[s] for leaf

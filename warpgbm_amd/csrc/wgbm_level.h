@@ -35,6 +35,7 @@ struct PartArgs {
   int32_t* rbase;
   int32_t* cscan;               // [max chunks + 1]
   int32_t* segL;                // [n_nodes*E]
+  int32_t* segP;                // [2*(n_nodes*E+1)] scratch
   // next level tables ([2*n_nodes(*E)])
   int32_t* c_seg_start;
   int32_t* c_seg_count;
