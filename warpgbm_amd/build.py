@@ -32,6 +32,7 @@ FLAGS = [
     "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-result",
+    "-Wshadow",
     f"-I{ROOT / 'include'}",
 ]
 
@@ -46,6 +47,11 @@ def _compile(src: Path) -> Path:
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src.name}:\n{r.stdout}\n{r.stderr}")
+    # Source warnings are surfaced (the host pass's notice about the
+    # device-only target feature is not one).
+    warn = [ln for ln in r.stderr.splitlines() if "warning:" in ln and "packed-fp32-ops" not in ln]
+    if warn:
+        print(f"{src.name}: " + "\n".join(warn), file=sys.stderr)
     return obj
 
 

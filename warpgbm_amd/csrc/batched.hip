@@ -149,26 +149,34 @@ __global__ void k_root_tables(const int32_t* __restrict__ era_count, int E, int 
   const int r = round_ctr[1];
   if (feat_all)
     for (int i = threadIdx.x; i < n_feat; i += blockDim.x) fidx_cur[i] = feat_all[(int64_t)r * n_feat + i];
-  __syncthreads();  // every thread has read round_ctr[1]
-  if (threadIdx.x != 0) return;
-  round_ctr[0] = r;
-  round_ctr[1] = r + 1;
-  int run = 0;
-  for (int e = 0; e < E; ++e) { era_start[e] = run; run += era_count[e]; }
-  for (int c = 0; c < K; ++c) {
-    for (int e = 0; e < E; ++e) {
-      const int s = c * E + e;
-      seg_start[s] = c * N + era_start[e];
-      seg_count[s] = era_count[e];
-      seg_slot[s] = s;
-      t_seg_start[s] = seg_start[s];
-      t_seg_count[s] = seg_count[s];
-    }
+  int run = 0;  // era starts: exclusive scan of the era counts
+  for (int base = 0; base < E; base += blockDim.x) {
+    const int e = base + threadIdx.x;
+    int tot;
+    const int ex = block_exscan(e < E ? era_count[e] : 0, &tot);  // syncs: every thread has read round_ctr[1]
+    if (e < E) era_start[e] = run + ex;
+    run += tot;
+  }
+  __syncthreads();
+  for (int s = threadIdx.x; s < K * E; s += blockDim.x) {
+    const int c = s / E, e = s - c * E;
+    const int st = c * N + era_start[e];
+    seg_start[s] = st;
+    seg_count[s] = era_count[e];
+    seg_slot[s] = s;
+    t_seg_start[s] = st;
+    t_seg_count[s] = era_count[e];
+  }
+  for (int c = threadIdx.x; c < K; c += blockDim.x) {
     t_active[c] = 1;
     t_node_start[c] = c * N;
     t_node_count[c] = run;
   }
-  *nseg = K * E;
+  if (threadIdx.x == 0) {
+    round_ctr[0] = r;
+    round_ctr[1] = r + 1;
+    *nseg = K * E;
+  }
 }
 
 // codes_t[c][r] = codes[r][c], through 64 x 64 LDS tiles (row-contiguous
@@ -439,6 +447,7 @@ static int enqueue_round(const wgbm_fit_config& c, const FitLayout& L, const wgb
           a.seg_count = I(L.b_seg_count);
           a.seg_slot = I(L.b_seg_slot);
           a.nseg_dev = I(L.n_build);
+          a.planned = 1;  // planned beside the previous level's child statistics
           rc = hist_launch(a, L.hc, L.KN / 2 + 1, ((int64_t)K << (d - 1)) * E, I(L.h_multi), I(L.h_n_multi), st);
           if (rc != kOk) return rc;
           rc = subtract_launch(Fp(L.hg[(d - 1) & 1]), Fp(L.hh[(d - 1) & 1]), hg, hh, I(L.sib), I(L.n_split),
@@ -491,15 +500,8 @@ static int enqueue_round(const wgbm_fit_config& c, const FitLayout& L, const wgb
       la.leaf_val = Fp(L.l_val);
       la.node_sum = nullptr;
       la.part = reinterpret_cast<double*>(ws + L.l_part);
-      rc = leaf_launch(la, L.KN, st);
-      if (rc != kOk) return rc;
-      hipLaunchKernelGGL(k_record_level, dim3((unsigned)ceil_div((int64_t)n_nodes * E, 256)), dim3(256), 0, st, d, D, K, E,
-                         n_nodes, I(cur.node_active), I(L.dec_feat), I(L.dec_bin), fidx, Fp(L.l_val),
-                         I(cur.node_count), Fp(L.era_gain), I(L.round_ctr), io->node_status, io->node_feat,
-                         io->node_bin, io->node_value, io->node_count, io->node_era_gain);
-      WGBM_LAUNCHED("k_record_level");
+      PartArgs p{};
       if (d < D) {
-        PartArgs p{};
         p.codes = io->codes;
         p.row_stride = c.n_cols;
         p.n_rows = N;
@@ -542,7 +544,31 @@ static int enqueue_round(const wgbm_fit_config& c, const FitLayout& L, const wgb
         p.sib = I(L.sib);
         p.n_split = I(L.n_split);
         p.chunk_rows = kPartChunk;
-        rc = partition_launch(p, L.KN, st);
+        // the leaf and partition plans depend only on this level's decisions
+        rc = plans2_launch(leaf_plan_job(la), part_plan_job(p), st);
+        if (rc != kOk) return rc;
+      }
+      rc = leaf_launch(la, L.KN, st, d < D);
+      if (rc != kOk) return rc;
+      hipLaunchKernelGGL(k_record_level, dim3((unsigned)ceil_div((int64_t)n_nodes * E, 256)), dim3(256), 0, st, d, D, K, E,
+                         n_nodes, I(cur.node_active), I(L.dec_feat), I(L.dec_bin), fidx, Fp(L.l_val),
+                         I(cur.node_count), Fp(L.era_gain), I(L.round_ctr), io->node_status, io->node_feat,
+                         io->node_bin, io->node_value, io->node_count, io->node_era_gain);
+      WGBM_LAUNCHED("k_record_level");
+      if (d < D) {
+        // the next level's histogram plan runs beside the child statistics
+        PlanJob next{};
+        if (d + 1 < D) {
+          HistArgs na{};
+          na.n_feat = c.n_feat;
+          na.seg_count = I(L.b_seg_count);
+          na.nseg_dev = I(L.n_build);
+          na.chunk_dev = I(L.h_chunk);
+          na.seg_first = I(L.h_seg_first);
+          na.seg_pfirst = I(L.h_seg_pfirst);
+          next = hist_plan_job(na, L.hc, I(L.h_multi), I(L.h_n_multi));
+        }
+        rc = partition_launch(p, L.KN, st, true, d + 1 < D ? &next : nullptr);
         if (rc != kOk) return rc;
       }
     }
@@ -633,19 +659,19 @@ extern "C" int wgbm_fit_rounds(const wgbm_fit_config* cfg, const wgbm_fit_io* io
     if (crc == kOk && ce == hipSuccess && graph &&
         hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess)
       exec = nullptr;
-    if (graph) hipGraphDestroy(graph);
+    if (graph) (void)hipGraphDestroy(graph);
   }
   if (exec == nullptr) {
     // Capture is an optimisation: on any failure run the rounds eagerly.
     (void)hipGetLastError();
     for (int r = 1; r < n_rounds; ++r) {
       rc = enqueue_round(c, L, io, ws, gs);
-      if (rc != kOk) { hipEventDestroy(ev_in); hipEventDestroy(ev_out); return rc; }
+      if (rc != kOk) { (void)hipEventDestroy(ev_in); (void)hipEventDestroy(ev_out); return rc; }
     }
   }
   for (int r = 1; exec && r < n_rounds; ++r) {
     const hipError_t le = hipGraphLaunch(exec, gs);
-    if (le != hipSuccess) { hipGraphExecDestroy(exec); WGBM_HIP(le); }
+    if (le != hipSuccess) { (void)hipGraphExecDestroy(exec); WGBM_HIP(le); }
     ++g_graph_rounds;
   }
   WGBM_HIP(hipEventRecord(ev_out, gs));
@@ -653,8 +679,8 @@ extern "C" int wgbm_fit_rounds(const wgbm_fit_config* cfg, const wgbm_fit_io* io
   // The executable graph must outlive its launches: wait for them here
   // (the caller reads the node tables right after this call anyway).
   WGBM_HIP(hipEventSynchronize(ev_out));
-  if (exec) hipGraphExecDestroy(exec);
-  hipEventDestroy(ev_in);
-  hipEventDestroy(ev_out);
+  if (exec) (void)hipGraphExecDestroy(exec);
+  (void)hipEventDestroy(ev_in);
+  (void)hipEventDestroy(ev_out);
   return kOk;
 }

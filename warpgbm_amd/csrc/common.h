@@ -62,4 +62,27 @@ __device__ __forceinline__ int lanes_below(uint64_t mask) {
                                    __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0));
 }
 
+// Exclusive prefix sum of v over the block (blockDim.x a multiple of 64, at
+// most 1024; every thread calls it); *total receives the block's sum.
+__device__ __forceinline__ int block_exscan(int v, int* total) {
+  __shared__ int ws[16];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int x = v;
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(x, o);
+    if (lane >= o) x += y;
+  }
+  __syncthreads();  // ws may still be read by a previous call
+  if (lane == 63) ws[w] = x;
+  __syncthreads();
+  int pre = 0, tot = 0;
+  for (int k = 0; k < (int)(blockDim.x >> 6); ++k) {
+    const int c = ws[k];
+    if (k < w) pre += c;
+    tot += c;
+  }
+  *total = tot;
+  return pre + x - v;
+}
+
 }  // namespace wgbm

@@ -32,6 +32,7 @@
 #include "common.h"
 #include "wgbm_internal.h"
 #include "wgbm_stats.h"
+#include "wgbm_plan.h"
 #include <utility>
 #include <vector>
 
@@ -71,83 +72,17 @@ __device__ __forceinline__ int code_byte(const typename CodeVec<FPL>::T& v, int 
 // written; count < 0 marks a padding segment that owns no chunk.
 // Segments with more than one chunk are listed in `multi` for the merge.
 // Row count of segment s for the three kinds of plan.
-__device__ __forceinline__ int plan_count(const PlanSrc& p, int s) {
-  if (p.mode == kPlanSegments) return p.seg_count[s];
-  if (p.mode == kPlanLeaves) {
-    const bool leaf = p.node_active[s] && (p.dec_feat == nullptr || p.dec_feat[s] < 0);
-    return leaf ? p.node_count[s] : -1;
-  }
-  const int j = s / p.n_eras;  // kPlanSplits: segments of split nodes
-  const bool split = p.node_active[j] && p.dec_feat[j] >= 0;
-  return split ? p.seg_count[s] : -1;
-}
-
 __global__ void __launch_bounds__(1024)
 k_hist_plan(const PlanSrc src, int32_t chunk_rows, int32_t* __restrict__ seg_first,
             int32_t* __restrict__ seg_pfirst, int32_t* __restrict__ multi, int32_t* __restrict__ n_multi) {
-  __shared__ int32_t s_sum[3][1024 / kWave];
-  __shared__ int32_t s_carry[3];
-  const int nseg = src.nseg_dev ? *src.nseg_dev : src.nseg;
-  if (src.nseg_out && threadIdx.x == 0) *src.nseg_out = nseg;
-  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
-  if (src.chunk_out) {
-    // Rows per chunk for this launch: enough chunks to give every slot a
-    // work item (chunk_target of them), within [chunk_min, chunk_rows].
-    __shared__ int s_rows;
-    if (t == 0) s_rows = 0;
-    __syncthreads();
-    int mine = 0;
-    for (int s = t; s < nseg; s += blockDim.x) mine += max(plan_count(src, s), 0);
-    for (int o = 32; o >= 1; o >>= 1) mine += __shfl_xor(mine, o);
-    if (lane == 0) atomicAdd(&s_rows, mine);
-    __syncthreads();
-    const int want = (s_rows + src.chunk_target - 1) / src.chunk_target;
-    chunk_rows = min(max((want + 255) & ~255, src.chunk_min), chunk_rows);
-    if (t == 0) *src.chunk_out = chunk_rows;
-  }
-  if (t < 3) s_carry[t] = 0;
-  __syncthreads();
-  for (int base = 0; base < nseg; base += 1024) {
-    const int s = base + t;
-    int v[3] = {0, 0, 0};  // chunks, is-multi, partial chunks
-    if (s < nseg) {
-      const int c = plan_count(src, s);
-      v[0] = c < 0 ? 0 : (c == 0 ? 1 : (c + chunk_rows - 1) / chunk_rows);
-      v[1] = v[0] > 1;
-      v[2] = v[1] ? v[0] : 0;
-    }
-    int x[3] = {v[0], v[1], v[2]};
-    for (int o = 1; o < 64; o <<= 1) {
-#pragma unroll
-      for (int q = 0; q < 3; ++q) {
-        const int y = __shfl_up(x[q], o);
-        if (lane >= o) x[q] += y;
-      }
-    }
-    if (lane == 63) { s_sum[0][wv] = x[0]; s_sum[1][wv] = x[1]; s_sum[2][wv] = x[2]; }
-    __syncthreads();
-    int pre[3] = {0, 0, 0};
-    for (int k = 0; k < wv; ++k) { pre[0] += s_sum[0][k]; pre[1] += s_sum[1][k]; pre[2] += s_sum[2][k]; }
-    const int c0 = s_carry[0], c1 = s_carry[1], c2 = s_carry[2];
-    if (s < nseg) {
-      seg_first[s] = c0 + pre[0] + x[0] - v[0];
-      if (seg_pfirst) seg_pfirst[s] = c2 + pre[2] + x[2] - v[2];
-      if (v[1] && multi) multi[c1 + pre[1] + x[1] - 1] = s;
-    }
-    __syncthreads();
-    if (t == 1023) {
-      int tot[3] = {0, 0, 0};
-      for (int k = 0; k < 16; ++k) { tot[0] += s_sum[0][k]; tot[1] += s_sum[1][k]; tot[2] += s_sum[2][k]; }
-      s_carry[0] = c0 + tot[0];
-      s_carry[1] = c1 + tot[1];
-      s_carry[2] = c2 + tot[2];
-    }
-    __syncthreads();
-  }
-  if (t == 0) {
-    seg_first[nseg] = s_carry[0];
-    if (n_multi) *n_multi = s_carry[1];
-  }
+  plan_body(src, chunk_rows, seg_first, seg_pfirst, multi, n_multi);
+}
+
+// Two independent plans in one launch (workgroup 0 and 1).
+__global__ void __launch_bounds__(1024)
+k_plans2(const PlanJob a, const PlanJob b) {
+  const PlanJob& j = blockIdx.x == 0 ? a : b;
+  plan_body(j.src, j.chunk_rows, j.seg_first, j.seg_pfirst, j.multi, j.n_multi);
 }
 
 // ---------------------------------------------------------------------------
@@ -593,6 +528,12 @@ void plan_launch(const PlanSrc& src, int chunk_rows, int32_t* seg_first, int32_t
                      n_multi);
 }
 
+int plans2_launch(const PlanJob& a, const PlanJob& b, hipStream_t st) {
+  hipLaunchKernelGGL(k_plans2, dim3(2), dim3(1024), 0, st, a, b);
+  WGBM_LAUNCHED("k_plans2");
+  return kOk;
+}
+
 // ---------------------------------------------------------------------------
 // Automatic configurations keep two workgroups' sub-histograms per CU
 // (64 KB each); explicit ones may use a workgroup's full 160 KB.
@@ -679,19 +620,21 @@ bool hist_timer_on() { return g_timer.on; }
 static void timer_mark(std::vector<std::pair<hipEvent_t, hipEvent_t>>& v, bool start, hipStream_t st) {
   if (!g_timer.on) return;
   if (start) {
-    hipEvent_t a, b;
-    hipEventCreate(&a);
-    hipEventCreate(&b);
+    hipEvent_t a = nullptr, b = nullptr;
+    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
+      g_timer.on = false;  // timing is best effort: stop rather than record garbage
+      return;
+    }
     v.emplace_back(a, b);
-    hipEventRecord(a, st);
-  } else {
-    hipEventRecord(v.back().second, st);
+    (void)hipEventRecord(a, st);
+  } else if (!v.empty()) {
+    (void)hipEventRecord(v.back().second, st);
   }
 }
 
 static void timer_clear() {
   for (auto* v : {&g_timer.build, &g_timer.merge})
-    for (auto& p : *v) { hipEventDestroy(p.first); hipEventDestroy(p.second); }
+    for (auto& p : *v) { (void)hipEventDestroy(p.first); (void)hipEventDestroy(p.second); }
   g_timer.build.clear();
   g_timer.merge.clear();
 }
@@ -741,8 +684,8 @@ static int launch_fpl(const HistArgs& a, const HistConfig& c, int64_t max_chunks
   if (rc != kOk) return rc;
   if (max_multi > 0) {
     const int SL = kWave * FPL * a.n_bins;
-    const int64_t items = max_multi * a.nslab * ceil_div(SL, kMergeElems);
-    const int64_t mg = items < 2048 ? items : 2048;
+    const int64_t merge_items = max_multi * a.nslab * ceil_div(SL, kMergeElems);
+    const int64_t mg = merge_items < 2048 ? merge_items : 2048;
     timer_mark(g_timer.merge, true, st);
     hipLaunchKernelGGL((k_hist_merge<FPL>), dim3((unsigned)mg), dim3(kMergeElems * kMergeParts), 0, st, a, multi,
                        n_multi);
@@ -765,8 +708,28 @@ int clamp_depth(int fpl, int depth) {
   return p;
 }
 
+PlanJob hist_plan_job(const HistArgs& a, const HistConfig& c, int32_t* multi, int32_t* n_multi) {
+  PlanJob j{};
+  j.src.mode = kPlanSegments;
+  j.src.seg_count = a.seg_count;
+  j.src.nseg_dev = a.nseg_dev;
+  j.src.chunk_out = a.chunk_dev;
+  j.src.chunk_min = kChunkMin < c.chunk_rows ? kChunkMin : c.chunk_rows;
+  j.src.chunk_target = (int32_t)hist_chunk_target(c, a.n_feat);
+  j.chunk_rows = c.chunk_rows;
+  j.seg_first = a.seg_first;
+  j.seg_pfirst = a.seg_pfirst;
+  j.multi = multi;
+  j.n_multi = n_multi;
+  return j;
+}
+
 int hist_launch(HistArgs a, const HistConfig& c, int64_t rows_max, int64_t nseg_max,
                 int32_t* multi, int32_t* n_multi, hipStream_t st) {
+  WGBM_REQUIRE(a.codes && a.perm && (a.gh || (a.g && a.h)) && a.seg_start && a.seg_count && a.seg_slot &&
+                   a.seg_first && a.seg_pfirst && a.nseg_dev && a.out_g && a.out_h && a.part && a.stats && multi &&
+                   n_multi,
+               "histogram: missing table or scratch pointer");
   a.chunk_rows = c.chunk_rows;
   a.nslab = (int)ceil_div(a.n_feat, kWave * c.fpl);
   a.wide_ok = (a.row_stride % c.fpl == 0) && ((uintptr_t)a.codes % (uintptr_t)c.fpl == 0);
@@ -775,16 +738,12 @@ int hist_launch(HistArgs a, const HistConfig& c, int64_t rows_max, int64_t nseg_
     set_error("histogram LDS budget exceeded (%d bins, %d waves, %d features per lane)", a.n_bins, c.waves, c.fpl);
     return kTooBig;
   }
-  PlanSrc src{};
-  src.mode = kPlanSegments;
-  src.seg_count = a.seg_count;
-  src.nseg_dev = a.nseg_dev;
-  src.chunk_out = a.chunk_dev;
-  src.chunk_min = kChunkMin < c.chunk_rows ? kChunkMin : c.chunk_rows;
-  src.chunk_target = (int32_t)hist_chunk_target(c, a.n_feat);
-  hipLaunchKernelGGL(k_hist_plan, dim3(1), dim3(1024), 0, st, src, c.chunk_rows, a.seg_first, a.seg_pfirst, multi,
-                     n_multi);
-  WGBM_LAUNCHED("k_hist_plan");
+  if (!a.planned) {
+    const PlanJob j = hist_plan_job(a, c, multi, n_multi);
+    hipLaunchKernelGGL(k_hist_plan, dim3(1), dim3(1024), 0, st, j.src, j.chunk_rows, j.seg_first, j.seg_pfirst,
+                       j.multi, j.n_multi);
+    WGBM_LAUNCHED("k_hist_plan");
+  }
   const int64_t max_chunks = hist_max_chunks(c, a.n_feat, rows_max, nseg_max);
   const int64_t max_multi = hist_max_multi(c, a.n_feat, rows_max);
   if (a.n_feat == 0) return kOk;

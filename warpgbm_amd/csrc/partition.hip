@@ -11,6 +11,7 @@
 #include "wgbm_internal.h"
 #include "wgbm_level.h"
 #include "wgbm_stats.h"
+#include "wgbm_plan.h"
 
 namespace wgbm {
 
@@ -59,28 +60,6 @@ k_part_count(PartArgs a) {
 
 // One workgroup: per-segment left totals and chunk offsets, child segment
 // tables, smaller-child build list and sibling map.
-// Exclusive prefix sum of v over the block (blockDim.x a multiple of 64, at
-// most 1024; every thread calls it); *total receives the block's sum.
-__device__ __forceinline__ int block_exscan(int v, int* total) {
-  __shared__ int ws[16];
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  int x = v;
-  for (int o = 1; o < 64; o <<= 1) {
-    const int y = __shfl_up(x, o);
-    if (lane >= o) x += y;
-  }
-  __syncthreads();  // ws may still be read by a previous call
-  if (lane == 63) ws[w] = x;
-  __syncthreads();
-  int pre = 0, tot = 0;
-  for (int k = 0; k < (int)(blockDim.x >> 6); ++k) {
-    const int c = ws[k];
-    if (k < w) pre += c;
-    tot += c;
-  }
-  *total = tot;
-  return pre + x - v;
-}
 
 // One workgroup: per-chunk destinations, child tables and the build list.
 // Chunks of a segment are contiguous, so one scan over all chunks gives
@@ -264,19 +243,55 @@ k_child_stats(PartArgs a) {
   if (threadIdx.x == 0) acc.store(a.c_stats + (int64_t)c * kStatWords);
 }
 
-int partition_launch(const PartArgs& a, int64_t rows_max, hipStream_t st) {
+PlanJob part_plan_job(const PartArgs& a) {
+  PlanJob j{};
+  j.src.mode = kPlanSplits;
+  j.src.nseg = a.n_nodes * a.n_eras;
+  j.src.nseg_out = a.nseg_dev;
+  j.src.seg_count = a.seg_count;
+  j.src.node_active = a.node_active;
+  j.src.dec_feat = a.dec_feat;
+  j.src.n_eras = a.n_eras;
+  j.chunk_rows = a.chunk_rows;
+  j.seg_first = a.seg_first;
+  j.multi = a.multi;
+  j.n_multi = a.n_multi;
+  return j;
+}
+
+// Child statistics (workgroups 0 .. 2n-1, as k_child_stats) and the next
+// level's histogram plan (last workgroup) in one launch.
+__global__ void __launch_bounds__(1024)
+k_child_stats_plan(PartArgs a, const PlanJob next) {
+  const int c = blockIdx.x;
+  if (c == 2 * a.n_nodes) {
+    plan_body(next.src, next.chunk_rows, next.seg_first, next.seg_pfirst, next.multi, next.n_multi);
+    return;
+  }
+  const int j = c >> 1, side = c & 1;
+  const int first = a.seg_first[j * a.n_eras], last = a.seg_first[(j + 1) * a.n_eras];
+  StatAcc acc;
+  for (int k = first + threadIdx.x; k < last; k += blockDim.x)
+    acc.load_merge(a.stat_part + ((int64_t)k * 2 + side) * kStatWords);
+  acc.block_reduce();
+  if (threadIdx.x == 0) acc.store(a.c_stats + (int64_t)c * kStatWords);
+}
+
+int partition_launch(const PartArgs& a, int64_t rows_max, hipStream_t st, bool planned, const PlanJob* next_hist) {
   const int nseg = a.n_nodes * a.n_eras;
   if (nseg == 0) return kOk;
-  PlanSrc src{};
-  src.mode = kPlanSplits;
-  src.nseg = nseg;
-  src.nseg_out = a.nseg_dev;
-  src.seg_count = a.seg_count;
-  src.node_active = a.node_active;
-  src.dec_feat = a.dec_feat;
-  src.n_eras = a.n_eras;
-  plan_launch(src, a.chunk_rows, a.seg_first, a.multi, a.n_multi, st);
-  WGBM_LAUNCHED("k_hist_plan(partition)");
+  WGBM_REQUIRE(a.codes && a.node_active && a.dec_feat && a.dec_bin && a.seg_start && a.seg_count && a.node_start &&
+                   a.node_count && a.perm_in && a.perm_out && a.flag && a.seg_first && a.nseg_dev && a.multi &&
+                   a.n_multi && a.lbase && a.rbase && a.cscan && a.segL && a.segP && a.c_seg_start &&
+                   a.c_seg_count && a.c_active && a.c_node_start && a.c_node_count && a.b_seg_start &&
+                   a.b_seg_count && a.b_seg_slot && a.n_build && a.sib && a.n_split && a.chunk_rows > 0,
+               "partition: missing table or scratch pointer");
+  WGBM_REQUIRE(!a.c_stats || a.stat_part, "partition: child statistics need their scratch");
+  if (!planned) {
+    const PlanJob j = part_plan_job(a);
+    plan_launch(j.src, j.chunk_rows, j.seg_first, j.multi, j.n_multi, st);
+    WGBM_LAUNCHED("k_hist_plan(partition)");
+  }
   const int64_t max_chunks = ceil_div(rows_max, a.chunk_rows) + nseg;
   hipLaunchKernelGGL(k_part_count, dim3((unsigned)max_chunks), dim3(256), 0, st, a);
   WGBM_LAUNCHED("k_part_count");
@@ -284,7 +299,10 @@ int partition_launch(const PartArgs& a, int64_t rows_max, hipStream_t st) {
   WGBM_LAUNCHED("k_part_scan");
   hipLaunchKernelGGL(k_part_scatter, dim3((unsigned)max_chunks), dim3(256), 0, st, a);
   WGBM_LAUNCHED("k_part_scatter");
-  if (a.c_stats) {
+  if (a.c_stats && next_hist) {
+    hipLaunchKernelGGL(k_child_stats_plan, dim3((unsigned)(2 * a.n_nodes + 1)), dim3(1024), 0, st, a, *next_hist);
+    WGBM_LAUNCHED("k_child_stats_plan");
+  } else if (a.c_stats) {
     hipLaunchKernelGGL(k_child_stats, dim3((unsigned)(2 * a.n_nodes)), dim3(256), 0, st, a);
     WGBM_LAUNCHED("k_child_stats");
   }
@@ -374,17 +392,31 @@ k_leaf_apply(LeafArgs a) {
   }
 }
 
-int leaf_launch(const LeafArgs& a, int64_t rows_max, hipStream_t st) {
+PlanJob leaf_plan_job(const LeafArgs& a) {
+  PlanJob j{};
+  j.src.mode = kPlanLeaves;
+  j.src.nseg = a.n_nodes;
+  j.src.nseg_out = a.nseg_dev;
+  j.src.node_active = a.node_active;
+  j.src.dec_feat = a.dec_feat;
+  j.src.node_count = a.node_count;
+  j.chunk_rows = kLeafChunk;
+  j.seg_first = a.seg_first;
+  j.multi = a.multi;
+  j.n_multi = a.n_multi;
+  return j;
+}
+
+int leaf_launch(const LeafArgs& a, int64_t rows_max, hipStream_t st, bool planned) {
   if (a.n_nodes == 0) return kOk;
-  PlanSrc src{};
-  src.mode = kPlanLeaves;
-  src.nseg = a.n_nodes;
-  src.nseg_out = a.nseg_dev;
-  src.node_active = a.node_active;
-  src.dec_feat = a.dec_feat;
-  src.node_count = a.node_count;
-  plan_launch(src, kLeafChunk, a.seg_first, a.multi, a.n_multi, st);
-  WGBM_LAUNCHED("k_hist_plan(leaf)");
+  WGBM_REQUIRE(a.node_active && a.node_start && a.node_count && a.seg_first && a.nseg_dev && a.multi && a.n_multi &&
+                   a.perm && (a.gh || (a.g && a.h)) && a.leaf_val && a.part,
+               "leaf: missing table or scratch pointer");
+  if (!planned) {
+    const PlanJob j = leaf_plan_job(a);
+    plan_launch(j.src, j.chunk_rows, j.seg_first, j.multi, j.n_multi, st);
+    WGBM_LAUNCHED("k_hist_plan(leaf)");
+  }
   const int64_t max_chunks = ceil_div(rows_max, kLeafChunk) + a.n_nodes;
   hipLaunchKernelGGL(k_leaf_partial, dim3((unsigned)max_chunks), dim3(256), 0, st, a);
   WGBM_LAUNCHED("k_leaf_partial");

@@ -49,6 +49,7 @@ struct HistArgs {
   int32_t dual;            // set by hist_launch: one- and two-plane variants both launched
   int32_t accumulate;      // 1: out += result (reference op), 0: out = result
   int32_t lds_pitch;       // floats between consecutive (j, bin) rows of a plane
+  int32_t planned;         // 1: the plan (hist_plan_job) already ran in an earlier fused launch
 };
 
 // Configuration used by every histogram build: the automatic choice with
@@ -124,5 +125,18 @@ struct PlanSrc {
 };
 void plan_launch(const PlanSrc& src, int chunk_rows, int32_t* seg_first, int32_t* multi, int32_t* n_multi,
                  hipStream_t st);
+
+// A plan to run inside a fused launch (plan_body runs on one 1024-thread
+// workgroup).
+struct PlanJob {
+  PlanSrc src;
+  int32_t chunk_rows;
+  int32_t* seg_first;
+  int32_t* seg_pfirst;
+  int32_t* multi;
+  int32_t* n_multi;
+};
+int plans2_launch(const PlanJob& a, const PlanJob& b, hipStream_t st);
+PlanJob hist_plan_job(const HistArgs& a, const HistConfig& c, int32_t* multi, int32_t* n_multi);
 
 }  // namespace wgbm

@@ -1,6 +1,7 @@
 // Level-wise tree growth: partition and leaf argument blocks.
 #pragma once
 #include "common.h"
+#include "wgbm_internal.h"
 
 namespace wgbm {
 
@@ -79,8 +80,14 @@ struct LeafArgs {
   double* part;                 // [max chunks * 2]
 };
 
-int partition_launch(const PartArgs& a, int64_t rows_max, hipStream_t st);
-int leaf_launch(const LeafArgs& a, int64_t rows_max, hipStream_t st);
+// planned: the plan ran in an earlier fused launch (plans2_launch of
+// leaf_plan_job / part_plan_job).  next_hist: run the next level's histogram
+// plan in the same launch as the child statistics.
+int partition_launch(const PartArgs& a, int64_t rows_max, hipStream_t st, bool planned = false,
+                     const PlanJob* next_hist = nullptr);
+int leaf_launch(const LeafArgs& a, int64_t rows_max, hipStream_t st, bool planned = false);
+PlanJob part_plan_job(const PartArgs& a);
+PlanJob leaf_plan_job(const LeafArgs& a);
 int64_t leaf_max_chunks(int64_t rows_max, int64_t n_nodes);
 int subtract_launch(const float* pg, const float* ph, float* cg, float* ch, const int32_t* sib,
                     const int32_t* n_split, int64_t max_split, int64_t node_elems, hipStream_t st);

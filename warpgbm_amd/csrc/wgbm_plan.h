@@ -1,0 +1,92 @@
+// Chunk plans (device side): one 1024-thread workgroup turns per-segment row
+// counts into chunk offsets, the list of multi-chunk segments and their
+// partial slots.  Shared by the histogram, partition and leaf launches and by
+// the fused launches that run a plan beside other work.
+#pragma once
+#include "common.h"
+#include "wgbm_internal.h"
+
+namespace wgbm {
+
+__device__ __forceinline__ int plan_count(const PlanSrc& p, int s) {
+  if (p.mode == kPlanSegments) return p.seg_count[s];
+  if (p.mode == kPlanLeaves) {
+    const bool leaf = p.node_active[s] && (p.dec_feat == nullptr || p.dec_feat[s] < 0);
+    return leaf ? p.node_count[s] : -1;
+  }
+  const int j = s / p.n_eras;  // kPlanSplits: segments of split nodes
+  const bool split = p.node_active[j] && p.dec_feat[j] >= 0;
+  return split ? p.seg_count[s] : -1;
+}
+
+// One workgroup of 1024 threads computes a plan.
+__device__ __forceinline__ void plan_body(const PlanSrc& src, int32_t chunk_rows, int32_t* __restrict__ seg_first,
+                          int32_t* __restrict__ seg_pfirst, int32_t* __restrict__ multi,
+                          int32_t* __restrict__ n_multi) {
+  __shared__ int32_t s_sum[3][1024 / kWave];
+  __shared__ int32_t s_carry[3];
+  const int nseg = src.nseg_dev ? *src.nseg_dev : src.nseg;
+  if (src.nseg_out && threadIdx.x == 0) *src.nseg_out = nseg;
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  if (src.chunk_out) {
+    // Rows per chunk for this launch: enough chunks to give every slot a
+    // work item (chunk_target of them), within [chunk_min, chunk_rows].
+    __shared__ int s_rows;
+    if (t == 0) s_rows = 0;
+    __syncthreads();
+    int mine = 0;
+    for (int s = t; s < nseg; s += blockDim.x) mine += max(plan_count(src, s), 0);
+    for (int o = 32; o >= 1; o >>= 1) mine += __shfl_xor(mine, o);
+    if (lane == 0) atomicAdd(&s_rows, mine);
+    __syncthreads();
+    const int want = (s_rows + src.chunk_target - 1) / src.chunk_target;
+    chunk_rows = min(max((want + 255) & ~255, src.chunk_min), chunk_rows);
+    if (t == 0) *src.chunk_out = chunk_rows;
+  }
+  if (t < 3) s_carry[t] = 0;
+  __syncthreads();
+  for (int base = 0; base < nseg; base += 1024) {
+    const int s = base + t;
+    int v[3] = {0, 0, 0};  // chunks, is-multi, partial chunks
+    if (s < nseg) {
+      const int c = plan_count(src, s);
+      v[0] = c < 0 ? 0 : (c == 0 ? 1 : (c + chunk_rows - 1) / chunk_rows);
+      v[1] = v[0] > 1;
+      v[2] = v[1] ? v[0] : 0;
+    }
+    int x[3] = {v[0], v[1], v[2]};
+    for (int o = 1; o < 64; o <<= 1) {
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        const int y = __shfl_up(x[q], o);
+        if (lane >= o) x[q] += y;
+      }
+    }
+    if (lane == 63) { s_sum[0][wv] = x[0]; s_sum[1][wv] = x[1]; s_sum[2][wv] = x[2]; }
+    __syncthreads();
+    int pre[3] = {0, 0, 0};
+    for (int k = 0; k < wv; ++k) { pre[0] += s_sum[0][k]; pre[1] += s_sum[1][k]; pre[2] += s_sum[2][k]; }
+    const int c0 = s_carry[0], c1 = s_carry[1], c2 = s_carry[2];
+    if (s < nseg) {
+      seg_first[s] = c0 + pre[0] + x[0] - v[0];
+      if (seg_pfirst) seg_pfirst[s] = c2 + pre[2] + x[2] - v[2];
+      if (v[1] && multi) multi[c1 + pre[1] + x[1] - 1] = s;
+    }
+    __syncthreads();
+    if (t == 1023) {
+      int tot[3] = {0, 0, 0};
+      for (int k = 0; k < 16; ++k) { tot[0] += s_sum[0][k]; tot[1] += s_sum[1][k]; tot[2] += s_sum[2][k]; }
+      s_carry[0] = c0 + tot[0];
+      s_carry[1] = c1 + tot[1];
+      s_carry[2] = c2 + tot[2];
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    seg_first[nseg] = s_carry[0];
+    if (n_multi) *n_multi = s_carry[1];
+  }
+}
+
+
+}  // namespace wgbm
