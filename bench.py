@@ -41,6 +41,7 @@ def parse():
     p.add_argument("--no-plain", action="store_true")
     p.add_argument("--profile-only", action="store_true", help="one fit, no extras (for rocprof)")
     p.add_argument("--no-serve", action="store_true", help="skip the served-predict measurement")
+    p.add_argument("--no-suite", action="store_true", help="skip the many-era and multiclass workloads")
     p.add_argument("--no-graph", action="store_true",
                    help="launch every round eagerly (no hipGraph replay), e.g. under counter collection")
     p.add_argument("--serve-seconds", type=float, default=2.0, help="arrival window per serving run")
@@ -270,6 +271,75 @@ def serving_bench(args, dev):
     return res
 
 
+def era_workload(seed=0):
+    """500k x 400 pre-binned int8 codes in [0, 5); 200 contiguous eras with
+    lognormal(7.8, 0.5) sizes; target quintile-quantised from a linear signal
+    on 30 features whose sign flips on 10 of them in a random 40 % of eras."""
+    rng = np.random.default_rng(seed)
+    N, F, E = 500_000, 400, 200
+    X = rng.integers(0, 5, (N, F)).astype(np.int8)
+    s = rng.lognormal(7.8, 0.5, E)
+    s = np.floor(s / s.sum() * N).astype(int)
+    s[np.argmax(s)] += N - s.sum()
+    era = np.repeat(np.arange(E), s).astype(np.int32)
+    c = rng.standard_normal(30)
+    flip = np.ones((E, 30))
+    flip[np.ix_(rng.choice(E, int(0.4 * E), replace=False), np.arange(10))] = -1
+    sig = ((X[:, :30].astype(np.float32) - 2.0) * (c * flip[era])).sum(1)
+    y = (np.searchsorted(np.quantile(sig, [0.2, 0.4, 0.6, 0.8]), sig) / 4.0).astype(np.float32)
+    return X, y, era
+
+
+def multiclass_workload(seed=0):
+    """200k x 100 float32 ~N(0,1); 5 classes = argmax of 5 random projections
+    of 30 features plus Gumbel noise."""
+    rng = np.random.default_rng(seed)
+    X = rng.standard_normal((200_000, 100)).astype(np.float32)
+    lab = np.argmax(X[:, :30] @ rng.standard_normal((30, 5)) + rng.gumbel(size=(200_000, 5)), axis=1)
+    return X, lab
+
+
+def suite(dev):
+    """The era-splitting and multiclass workloads: full batched fits (X in
+    host memory, so binning and the host-to-device copy are included) and the
+    plain per-node engine on the first trees, with a tree-equality check."""
+    import contextlib, io
+    import torch
+    from warpgbm_amd import WarpGBM
+
+    def fit(kw, X, y, **fk):
+        m = WarpGBM(**kw)
+        torch.cuda.synchronize(dev)
+        t0 = time.perf_counter()
+        with contextlib.redirect_stdout(io.StringIO()):
+            m.fit(X, y, **fk)
+        torch.cuda.synchronize(dev)
+        return m, time.perf_counter() - t0
+
+    out = {}
+    X, y, era = era_workload()
+    kw = dict(objective="regression", num_bins=5, max_depth=8, n_estimators=200, learning_rate=0.01, device=dev)
+    fit(dict(kw, engine="batched", n_estimators=5), X, y, era_id=era)  # warm-up
+    mb, tb = fit(dict(kw, engine="batched"), X, y, era_id=era)
+    k = 10
+    mp, tp = fit(dict(kw, engine="plain", n_estimators=k), X, y, era_id=era)
+    out["eras_500kx400_E200_d8_200trees"] = {
+        "batched_fit_s": tb, "plain_fit_s_extrapolated": tp * 200 / k, "plain_trees_timed": k,
+        "speedup": tp * 200 / k / tb, "trees_identical": bool(trees_identical(mb.forest[:k], mp.forest[:k]))}
+    log(f"eras workload: batched {tb:.2f} s, plain {tp * 200 / k:.1f} s (extrapolated from {k} trees)")
+    X, lab = multiclass_workload()
+    kw = dict(objective="multiclass", num_bins=32, max_depth=6, n_estimators=200, learning_rate=0.1, device=dev)
+    fit(dict(kw, engine="batched", n_estimators=5), X, lab)
+    mb, tb = fit(dict(kw, engine="batched"), X, lab)
+    mp, tp = fit(dict(kw, engine="plain", n_estimators=k), X, lab)
+    same = all(trees_identical(a, b) for a, b in zip(mb.forest[:k], mp.forest[:k]))
+    out["multiclass_200kx100_K5_d6_200rounds"] = {
+        "batched_fit_s": tb, "plain_fit_s_extrapolated": tp * 200 / k, "plain_rounds_timed": k,
+        "speedup": tp * 200 / k / tb, "trees_identical": bool(same)}
+    log(f"multiclass workload: batched {tb:.2f} s, plain {tp * 200 / k:.1f} s (extrapolated from {k} rounds)")
+    return out
+
+
 def trees_identical(fa, fb):
     import torch
 
@@ -415,6 +485,10 @@ def main():
     if not args.no_serve:
         serve = serving_bench(args, dev)
 
+    others = None
+    if not args.no_suite:
+        others = suite(dev)
+
     cpu = None
     if not args.no_cpu:
         n = min(args.cpu_rows, args.rows)
@@ -452,6 +526,7 @@ def main():
                     "p99_ms": float(np.percentile(lat, 99)), "rows_per_s": 10000 / (np.mean(lat) * 1e-3),
                     "max_abs_diff_vs_reference_op": pred_err, "mode": "one request per launch (unbatched)"},
         "serving": serve,
+        "other_workloads": others,
         "binning_s": bin_s,
         "cpu_baseline": cpu,
     }

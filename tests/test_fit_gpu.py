@@ -181,3 +181,63 @@ def test_graph_replay_equals_eager():
         msg = _trees_equal(mg.forest[i], me.forest[i], f"tree{i}:")
         assert msg is None, msg
     assert np.array_equal(mg.predict(X), me.predict(X))
+
+
+def test_many_eras_prebinned_batched_equals_plain_and_oracle():
+    """Pre-binned int8 codes in [0, 5) with 40 eras of uneven size (the
+    many-era path: per-era gains, ordered era sums): both engines agree and
+    the first tree matches the oracle grown from the same float32 targets."""
+    from warpgbm_amd import WarpGBM
+    rng = np.random.default_rng(21)
+    n, f, E = 16000, 24, 40
+    X = rng.integers(0, 5, (n, f)).astype(np.int8)
+    sizes = rng.lognormal(5.0, 0.5, E)
+    sizes = np.floor(sizes / sizes.sum() * n).astype(int)
+    sizes[np.argmax(sizes)] += n - sizes.sum()
+    era = np.repeat(np.arange(E), sizes).astype(np.int32)
+    flip = np.where(rng.random(E) < 0.4, -1.0, 1.0)
+    sig = (X[:, 0] - 2.0) * flip[era] + 0.7 * (X[:, 1] - 2.0) + 0.3 * rng.standard_normal(n)
+    y = (np.searchsorted(np.quantile(sig, [0.2, 0.4, 0.6, 0.8]), sig) / 4.0).astype(np.float32)
+    kw = dict(objective="regression", num_bins=5, max_depth=5, n_estimators=6, learning_rate=0.1)
+    mb = WarpGBM(engine="batched", **kw).fit(X, y, era_id=era)
+    mp = WarpGBM(engine="plain", **kw).fit(X, y, era_id=era)
+    for i in range(6):
+        msg = _trees_equal(mb.forest[i], mp.forest[i], f"tree{i}:")
+        assert msg is None, msg
+    np.testing.assert_array_equal(mb.per_era_feature_importance_, mp.per_era_feature_importance_)
+    g = (y - np.float32(mb.base_prediction)).astype(np.float32)
+    _oracle_check_tree(mb.forest[0], X, g, np.ones_like(g), era, E, 5, 5, 1e-6)
+
+
+def test_multiclass_round0_matches_oracle():
+    """Softmax targets of the first round (device float32) match the FP64
+    oracle within 1e-6, and every class tree of round 0 matches the oracle
+    tree grown from the same float32 targets."""
+    from warpgbm_amd import WarpGBM, _lib
+    rng = np.random.default_rng(8)
+    n, f, K = 5000, 12, 4
+    X = rng.standard_normal((n, f)).astype(np.float32)
+    lab = np.argmax(X[:, :6] @ rng.standard_normal((6, K)) + rng.gumbel(size=(n, K)), axis=1)
+    m = WarpGBM(objective="multiclass", num_bins=32, max_depth=3, n_estimators=1, learning_rate=0.1).fit(X, lab)
+    codes, _ = O.bin_matrix(X, 32)
+    counts = np.bincount(lab, minlength=K)
+    sc = np.empty((n, K), np.float32)
+    for k in range(K):
+        sc[:, k] = np.float32(np.log(np.float32(np.float32(counts[k]) / np.float32(n)) + np.float32(1e-10)))
+    dev = torch.device("cuda")
+    s_d = torch.from_numpy(sc).to(dev)
+    l_d = torch.from_numpy(lab.astype(np.int32)).to(dev)
+    w_d = torch.ones(n, dtype=torch.float32, device=dev)
+    g_d = torch.empty(K * n, dtype=torch.float32, device=dev)
+    h_d = torch.empty(K * n, dtype=torch.float32, device=dev)
+    _lib.call("wgbm_grad_softmax", s_d.data_ptr(), l_d.data_ptr(), w_d.data_ptr(), n, K, g_d.data_ptr(),
+              h_d.data_ptr(), torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    g = g_d.cpu().numpy().reshape(K, n)
+    h = h_d.cpu().numpy().reshape(K, n)
+    g64, h64 = O.softmax_grad_hess(sc.astype(np.float64), lab, np.ones(n))
+    np.testing.assert_allclose(g, g64.T, atol=1e-6)
+    np.testing.assert_allclose(h, h64.T, atol=1e-6)
+    eras = np.zeros(n, np.int32)
+    for k in range(K):
+        _oracle_check_tree(m.forest[0][k], codes, g[k], h[k], eras, 1, 32, 3, 1e-6)
