@@ -62,7 +62,7 @@ def test_workspace_queries():
     assert lib.wgbm_fit_workspace_bytes(ctypes.byref(c)) == 0
     out = (ctypes.c_int32 * 4)()
     lib.wgbm_get_hist_config(1000, 32, 1000, 0, out)
-    assert out[0] > 0 and out[1] in (1, 2, 4, 8) and out[2] in (1, 2, 4, 8, 16) and out[3] in (1, 2, 4, 8, 16, 32, 64)
+    assert out[0] > 0 and out[1] in (1, 2, 4, 8, 16) and out[2] in (1, 2, 4, 8, 16) and out[3] in (1, 2, 4, 8, 16, 32, 64)
 
 
 def test_hyperparameter_validation_messages():

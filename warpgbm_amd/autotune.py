@@ -63,7 +63,7 @@ def candidates(n_feat: int, n_bins: int, n_cols: int, const_hessian: bool = Fals
             continue
         if fpl > 1 and 64 * fpl > 2 * max(n_feat, 1):
             continue  # slab far wider than the features
-        for c, w, d in itertools.product((4096, 8192, 16384), (4, 8), depths):
+        for c, w, d in itertools.product((4096, 8192, 16384), (8, 16) if fpl <= 4 else (4, 8), depths):
             out.append((c, w, fpl, d))
     return out
 

@@ -329,7 +329,7 @@ using namespace wgbm;
 
 extern "C" int wgbm_set_hist_config(int32_t chunk_rows, int32_t waves, int32_t features_per_lane,
                                     int32_t rows_in_flight) {
-  WGBM_REQUIRE(chunk_rows >= 0 && (waves == 0 || waves == 1 || waves == 2 || waves == 4 || waves == 8),
+  WGBM_REQUIRE(chunk_rows >= 0 && (waves == 0 || waves == 1 || waves == 2 || waves == 4 || waves == 8 || waves == 16),
                "bad histogram configuration");
   WGBM_REQUIRE(features_per_lane == 0 || features_per_lane == 1 || features_per_lane == 2 ||
                    features_per_lane == 4 || features_per_lane == 8 || features_per_lane == 16,
