@@ -416,69 +416,6 @@ static int enqueue_round(const wgbm_fit_config& c, const FitLayout& L, const wgb
       const float2* gh_d = d == 0 ? G2(L.gh_root) : G2(L.gh_pp[d & 1]);
       float* hg = Fp(L.hg[d & 1]);
       float* hh = Fp(L.hh[d & 1]);
-      if (d < D) {
-        HistArgs a{};
-        a.codes = io->codes;
-        a.row_stride = c.n_cols;
-        a.n_rows = N;
-        a.feat_idx = fidx;
-        a.n_feat = c.n_feat;
-        a.n_bins = c.n_bins;
-        a.perm = perm_d;
-        a.gh = gh_d;
-        a.seg_first = I(L.h_seg_first);
-        a.seg_pfirst = I(L.h_seg_pfirst);
-        a.chunk_dev = I(L.h_chunk);
-        a.out_g = hg;
-        a.out_h = hh;
-        a.part = reinterpret_cast<longlong2*>(ws + L.h_part);
-        a.slots_per_node = E;
-        a.hess_const = fit_one_plane(c) ? 1 : 0;
-        a.stats = I(cur.stats);
-        a.accumulate = 0;
-        if (d == 0) {
-          a.seg_start = I(L.root_seg_start);
-          a.seg_count = I(L.root_seg_count);
-          a.seg_slot = I(L.root_seg_slot);
-          a.nseg_dev = I(L.root_nseg);
-          rc = hist_launch(a, L.hc, L.KN, (int64_t)K * E, I(L.h_multi), I(L.h_n_multi), st);
-        } else {
-          a.seg_start = I(L.b_seg_start);
-          a.seg_count = I(L.b_seg_count);
-          a.seg_slot = I(L.b_seg_slot);
-          a.nseg_dev = I(L.n_build);
-          a.planned = 1;  // planned beside the previous level's child statistics
-          rc = hist_launch(a, L.hc, L.KN / 2 + 1, ((int64_t)K << (d - 1)) * E, I(L.h_multi), I(L.h_n_multi), st);
-          if (rc != kOk) return rc;
-          rc = subtract_launch(Fp(L.hg[(d - 1) & 1]), Fp(L.hh[(d - 1) & 1]), hg, hh, I(L.sib), I(L.n_split),
-                               (int64_t)K << (d - 1), L.node_elems, st);
-        }
-        if (rc != kOk) return rc;
-        LevelScanArgs s{};
-        s.G = hg;
-        s.H = hh;
-        s.n_nodes = n_nodes;
-        s.n_eras = E;
-        s.n_feat = c.n_feat;
-        s.n_bins = c.n_bins;
-        s.node_active = I(cur.node_active);
-        s.feat_idx = fidx;
-        s.mono_req = io->mono_required;
-        s.msg = c.min_split_gain;
-        s.mcw = c.min_child_weight;
-        s.lam = c.lambda;
-        s.cand_agree = I(L.cand_agree);
-        s.cand_crit = Fp(L.cand_crit);
-        s.cand_idx = I(L.cand_idx);
-        s.dec_feat = I(L.dec_feat);
-        s.dec_bin = I(L.dec_bin);
-        s.era_gain = Fp(L.era_gain);
-        s.era_g = Fp(L.era_g);
-        s.era_d = reinterpret_cast<int8_t*>(ws + L.era_d);
-        s.gain_nodes = L.gain_nodes;
-        rc = level_scan_launch(s, st);
-        if (rc != kOk) return rc;
-      }
       LeafArgs la{};
       la.n_nodes = n_nodes;
       la.node_active = I(cur.node_active);
@@ -544,8 +481,70 @@ static int enqueue_round(const wgbm_fit_config& c, const FitLayout& L, const wgb
         p.sib = I(L.sib);
         p.n_split = I(L.n_split);
         p.chunk_rows = kPartChunk;
-        // the leaf and partition plans depend only on this level's decisions
-        rc = plans2_launch(leaf_plan_job(la), part_plan_job(p), st);
+      }
+      if (d < D) {
+        HistArgs a{};
+        a.codes = io->codes;
+        a.row_stride = c.n_cols;
+        a.n_rows = N;
+        a.feat_idx = fidx;
+        a.n_feat = c.n_feat;
+        a.n_bins = c.n_bins;
+        a.perm = perm_d;
+        a.gh = gh_d;
+        a.seg_first = I(L.h_seg_first);
+        a.seg_pfirst = I(L.h_seg_pfirst);
+        a.chunk_dev = I(L.h_chunk);
+        a.out_g = hg;
+        a.out_h = hh;
+        a.part = reinterpret_cast<longlong2*>(ws + L.h_part);
+        a.slots_per_node = E;
+        a.hess_const = fit_one_plane(c) ? 1 : 0;
+        a.stats = I(cur.stats);
+        a.accumulate = 0;
+        if (d == 0) {
+          a.seg_start = I(L.root_seg_start);
+          a.seg_count = I(L.root_seg_count);
+          a.seg_slot = I(L.root_seg_slot);
+          a.nseg_dev = I(L.root_nseg);
+          rc = hist_launch(a, L.hc, L.KN, (int64_t)K * E, I(L.h_multi), I(L.h_n_multi), st);
+        } else {
+          a.seg_start = I(L.b_seg_start);
+          a.seg_count = I(L.b_seg_count);
+          a.seg_slot = I(L.b_seg_slot);
+          a.nseg_dev = I(L.n_build);
+          a.planned = 1;  // planned beside the previous level's child statistics
+          rc = hist_launch(a, L.hc, L.KN / 2 + 1, ((int64_t)K << (d - 1)) * E, I(L.h_multi), I(L.h_n_multi), st);
+          if (rc != kOk) return rc;
+          rc = subtract_launch(Fp(L.hg[(d - 1) & 1]), Fp(L.hh[(d - 1) & 1]), hg, hh, I(L.sib), I(L.n_split),
+                               (int64_t)K << (d - 1), L.node_elems, st);
+        }
+        if (rc != kOk) return rc;
+        LevelScanArgs s{};
+        s.G = hg;
+        s.H = hh;
+        s.n_nodes = n_nodes;
+        s.n_eras = E;
+        s.n_feat = c.n_feat;
+        s.n_bins = c.n_bins;
+        s.node_active = I(cur.node_active);
+        s.feat_idx = fidx;
+        s.mono_req = io->mono_required;
+        s.msg = c.min_split_gain;
+        s.mcw = c.min_child_weight;
+        s.lam = c.lambda;
+        s.cand_agree = I(L.cand_agree);
+        s.cand_crit = Fp(L.cand_crit);
+        s.cand_idx = I(L.cand_idx);
+        s.dec_feat = I(L.dec_feat);
+        s.dec_bin = I(L.dec_bin);
+        s.era_gain = Fp(L.era_gain);
+        s.era_g = Fp(L.era_g);
+        s.era_d = reinterpret_cast<int8_t*>(ws + L.era_d);
+        s.gain_nodes = L.gain_nodes;
+        // decisions, then the leaf and partition plans, in one launch
+        const PlanJob leaf_job = leaf_plan_job(la), part_job = part_plan_job(p);
+        rc = level_scan_launch(s, st, &leaf_job, &part_job);
         if (rc != kOk) return rc;
       }
       rc = leaf_launch(la, L.KN, st, d < D);

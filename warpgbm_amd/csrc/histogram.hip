@@ -78,12 +78,6 @@ k_hist_plan(const PlanSrc src, int32_t chunk_rows, int32_t* __restrict__ seg_fir
   plan_body(src, chunk_rows, seg_first, seg_pfirst, multi, n_multi);
 }
 
-// Two independent plans in one launch (workgroup 0 and 1).
-__global__ void __launch_bounds__(1024)
-k_plans2(const PlanJob a, const PlanJob b) {
-  const PlanJob& j = blockIdx.x == 0 ? a : b;
-  plan_body(j.src, j.chunk_rows, j.seg_first, j.seg_pfirst, j.multi, j.n_multi);
-}
 
 // ---------------------------------------------------------------------------
 // Shifts of a segment's node and the partial word of one bin.
@@ -531,11 +525,6 @@ void plan_launch(const PlanSrc& src, int chunk_rows, int32_t* seg_first, int32_t
                      n_multi);
 }
 
-int plans2_launch(const PlanJob& a, const PlanJob& b, hipStream_t st) {
-  hipLaunchKernelGGL(k_plans2, dim3(2), dim3(1024), 0, st, a, b);
-  WGBM_LAUNCHED("k_plans2");
-  return kOk;
-}
 
 // ---------------------------------------------------------------------------
 // Automatic configurations keep two workgroups' sub-histograms per CU

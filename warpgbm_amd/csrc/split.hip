@@ -15,6 +15,7 @@
 // winner whose crit is -inf (no valid candidate) means "leaf".
 #include "common.h"
 #include "wgbm_internal.h"
+#include "wgbm_plan.h"
 #include <math.h>
 
 namespace wgbm {
@@ -212,14 +213,12 @@ k_level_scan(const LevelScanArgs a) {
   }
 }
 
-// Winner per node, then (optionally) its per-era gains.
-__global__ void __launch_bounds__(64)
-k_level_pick(const LevelScanArgs a, int wg_per_node) {
-  const int node = blockIdx.x;
-  if (node >= a.n_nodes) return;
+// Winner of one node, then (optionally) its per-era gains; run by one wave
+// (lane = 0..63).
+__device__ void pick_node(const LevelScanArgs& a, int node, int wg_per_node, int lane) {
   Cand b = cand_none();
   if (a.node_active[node]) {
-    for (int w = threadIdx.x; w < wg_per_node; w += 64) {
+    for (int w = lane; w < wg_per_node; w += 64) {
       const int64_t o = (int64_t)node * wg_per_node + w;
       const Cand k{a.cand_agree[o], a.cand_crit[o], a.cand_idx[o]};
       if (cand_better(k, b)) b = k;
@@ -231,13 +230,13 @@ k_level_pick(const LevelScanArgs a, int wg_per_node) {
     fl = b.idx / (a.n_bins - 1);
     bin = b.idx % (a.n_bins - 1);
   }
-  if (threadIdx.x == 0) {
+  if (lane == 0) {
     a.dec_feat[node] = fl;
     a.dec_bin[node] = bin;
   }
   if (a.era_gain == nullptr) return;
   const int B = a.n_bins, E = a.n_eras;
-  for (int e = threadIdx.x; e < E; e += blockDim.x) {
+  for (int e = lane; e < E; e += 64) {
     float ga = 0.0f;
     if (fl >= 0) {
       const int64_t off = (((int64_t)node * E + e) * a.n_feat + fl) * B;
@@ -252,6 +251,24 @@ k_level_pick(const LevelScanArgs a, int wg_per_node) {
     }
     a.era_gain[(int64_t)node * E + e] = ga;
   }
+}
+
+__global__ void __launch_bounds__(64)
+k_level_pick(const LevelScanArgs a, int wg_per_node) {
+  if (blockIdx.x < a.n_nodes) pick_node(a, blockIdx.x, wg_per_node, threadIdx.x);
+}
+
+// One workgroup: the winners of every node (wave w takes nodes w, w+16, ...),
+// then the leaf plan and the partition plan of the level, which depend only
+// on those decisions.
+__global__ void __launch_bounds__(1024)
+k_decide_plans(const LevelScanArgs a, int wg_per_node, const PlanJob leaf, const PlanJob part) {
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int node = wv; node < a.n_nodes; node += 16) pick_node(a, node, wg_per_node, lane);
+  __syncthreads();
+  plan_body(leaf.src, leaf.chunk_rows, leaf.seg_first, leaf.seg_pfirst, leaf.multi, leaf.n_multi);
+  __syncthreads();  // the plan's shared state is reused by the next one
+  plan_body(part.src, part.chunk_rows, part.seg_first, part.seg_pfirst, part.multi, part.n_multi);
 }
 
 constexpr int kSelBlock = 256;  // (feature, threshold) candidates per workgroup of the E > 1 selection
@@ -343,7 +360,7 @@ k_level_sel(const LevelScanArgs a, int node0, int n_group) {
   }
 }
 
-int level_scan_launch(const LevelScanArgs& a, hipStream_t st) {
+int level_scan_launch(const LevelScanArgs& a, hipStream_t st, const PlanJob* leaf, const PlanJob* part) {
   if (a.n_nodes == 0) return kOk;
   if (a.n_eras == 1) {
     const int wg_per_node = (int)ceil_div(a.n_feat, kScanFeat);
@@ -362,9 +379,14 @@ int level_scan_launch(const LevelScanArgs& a, hipStream_t st) {
       WGBM_LAUNCHED("k_level_sel");
     }
   }
-  hipLaunchKernelGGL(k_level_pick, dim3((unsigned)a.n_nodes), dim3(64), 0, st, a,
-                     (int)level_scan_groups(a.n_feat, a.n_eras, a.n_bins));
-  WGBM_LAUNCHED("k_level_pick");
+  const int groups = (int)level_scan_groups(a.n_feat, a.n_eras, a.n_bins);
+  if (leaf && part) {
+    hipLaunchKernelGGL(k_decide_plans, dim3(1), dim3(1024), 0, st, a, groups, *leaf, *part);
+    WGBM_LAUNCHED("k_decide_plans");
+  } else {
+    hipLaunchKernelGGL(k_level_pick, dim3((unsigned)a.n_nodes), dim3(64), 0, st, a, groups);
+    WGBM_LAUNCHED("k_level_pick");
+  }
   return kOk;
 }
 

@@ -96,7 +96,11 @@ struct LevelScanArgs {
   int8_t* era_d;
   int64_t gain_nodes;           // nodes whose gains fit era_g / era_d
 };
-int level_scan_launch(const LevelScanArgs& a, hipStream_t st);
+struct PlanJob;
+// leaf / part: when both are given, the per-node decisions and the level's
+// leaf and partition plans run in one single-workgroup launch.
+int level_scan_launch(const LevelScanArgs& a, hipStream_t st, const PlanJob* leaf = nullptr,
+                      const PlanJob* part = nullptr);
 int64_t level_scan_groups(int n_feat, int n_eras, int n_bins);  // candidate slots per node
 // Nodes per pass of the E > 1 scan for a gain buffer of at most cap bytes.
 int64_t level_scan_gain_nodes(int64_t n_nodes, int n_eras, int n_feat, int n_bins, size_t cap);
@@ -136,7 +140,6 @@ struct PlanJob {
   int32_t* multi;
   int32_t* n_multi;
 };
-int plans2_launch(const PlanJob& a, const PlanJob& b, hipStream_t st);
 PlanJob hist_plan_job(const HistArgs& a, const HistConfig& c, int32_t* multi, int32_t* n_multi);
 
 }  // namespace wgbm

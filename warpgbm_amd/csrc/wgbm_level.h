@@ -80,8 +80,8 @@ struct LeafArgs {
   double* part;                 // [max chunks * 2]
 };
 
-// planned: the plan ran in an earlier fused launch (plans2_launch of
-// leaf_plan_job / part_plan_job).  next_hist: run the next level's histogram
+// planned: the plan ran in an earlier fused launch (leaf_plan_job /
+// part_plan_job beside the level's decisions).  next_hist: run the next level's histogram
 // plan in the same launch as the child statistics.
 int partition_launch(const PartArgs& a, int64_t rows_max, hipStream_t st, bool planned = false,
                      const PlanJob* next_hist = nullptr);
