@@ -1,0 +1,207 @@
+"""Behavioural acceptance tests of the drop-in estimator on the GPU.
+
+These restate, in fresh code, the end-to-end properties the reference's own
+test-suite asserts (warpgbm tests/: predict chunking, warm start, save/load,
+monotone constraints, sample weights, random state, multiclass, importances,
+fit quality, evaluation / early stopping).  This engine is deterministic, so
+where the reference allows a tolerance for float-atomic noise these tests
+ask for equality.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+
+def _reg(n=6000, f=16, seed=0):
+    rng = np.random.default_rng(seed)
+    X = rng.standard_normal((n, f)).astype(np.float32)
+    y = (X[:, 0] * 2 + np.sin(X[:, 1]) + 0.5 * X[:, 2] * X[:, 3] + 0.1 * rng.standard_normal(n)).astype(np.float32)
+    return X, y
+
+
+def _cls(n=6000, f=12, K=3, seed=1):
+    rng = np.random.default_rng(seed)
+    X = rng.standard_normal((n, f)).astype(np.float32)
+    lab = np.argmax(X[:, :6] @ rng.standard_normal((6, K)) + 0.3 * rng.gumbel(size=(n, K)), axis=1)
+    return X, lab
+
+
+@pytest.mark.parametrize("engine", ["batched", "plain"])
+def test_predict_chunking(engine):
+    """tests/test_predict_chunking.py:19-40: predicting in pieces equals one call."""
+    from warpgbm_amd import WarpGBM
+    X, y = _reg()
+    m = WarpGBM(num_bins=32, max_depth=4, n_estimators=15, engine=engine).fit(X, y)
+    full = m.predict(X)
+    parts = np.concatenate([m.predict(X[:2500]), m.predict(X[2500:])])
+    assert np.array_equal(full, parts)
+    Xc, lab = _cls()
+    mc = WarpGBM(objective="multiclass", num_bins=16, max_depth=3, n_estimators=8, engine=engine).fit(Xc, lab)
+    pf = mc.predict_proba(Xc)
+    pp = np.concatenate([mc.predict_proba(Xc[:3000]), mc.predict_proba(Xc[3000:])])
+    assert np.array_equal(pf, pp)
+
+
+@pytest.mark.parametrize("objective", ["regression", "multiclass"])
+def test_warm_start_equals_one_fit(objective):
+    """tests/test_warm_start.py:15-196: 6 + 6 trees with warm start == 12 trees."""
+    from warpgbm_amd import WarpGBM
+    X, y = _reg() if objective == "regression" else _cls()
+    kw = dict(objective=objective, num_bins=16, max_depth=3, learning_rate=0.2)
+    full = WarpGBM(n_estimators=12, **kw).fit(X, y)
+    ws = WarpGBM(n_estimators=6, warm_start=True, **kw).fit(X, y)
+    ws.n_estimators = 12
+    ws.fit(X, y)
+    assert ws._trees_trained == 12
+    if objective == "regression":
+        assert np.array_equal(ws.predict(X), full.predict(X))
+    else:
+        assert np.array_equal(ws.predict_proba(X), full.predict_proba(X))
+    assert np.array_equal(ws.feature_importance_, full.feature_importance_)
+
+
+def test_save_load_predicts_identically(tmp_path):
+    """tests/test_warm_start.py:202-300: a saved and reloaded model predicts the same."""
+    from warpgbm_amd import WarpGBM
+    X, y = _reg()
+    m = WarpGBM(num_bins=32, max_depth=4, n_estimators=10).fit(X, y)
+    path = tmp_path / "m.pkl"
+    m.save_model(str(path))
+    m2 = WarpGBM().load_model(str(path))
+    assert np.array_equal(m2.predict(X), m.predict(X))
+    Xc, lab = _cls()
+    mc = WarpGBM(objective="multiclass", num_bins=16, max_depth=3, n_estimators=5).fit(Xc, lab)
+    mc.save_model(str(path))
+    mc2 = WarpGBM().load_model(str(path))
+    assert np.array_equal(mc2.predict_proba(Xc), mc.predict_proba(Xc))
+    assert np.array_equal(mc2.predict(Xc), mc.predict(Xc))
+
+
+def _mono_data(seed, n=600):
+    rng = np.random.default_rng(seed)
+    x0 = rng.uniform(-2.0, 2.0, n).astype(np.float32)
+    X = np.column_stack([x0, rng.normal(0.0, 0.3, n)]).astype(np.float32)
+    return X, x0, rng
+
+
+@pytest.mark.parametrize("case", ["increasing", "decreasing", "eras"])
+def test_monotone_constraint_holds_on_a_grid(case):
+    """tests/test_monotonic_constraints.py:27-120: along the constrained
+    feature (the signal's feature) the predictions are monotone on a grid,
+    without and with eras (two eras with different slopes); the list and the
+    dict forms of the constraint give the same model."""
+    from warpgbm_amd import WarpGBM
+    X, x0, rng = _mono_data({"increasing": 7, "decreasing": 9, "eras": 33}[case])
+    noise = 0.04 * rng.standard_normal(len(x0)).astype(np.float32)
+    era = None
+    if case == "increasing":
+        y, sign = 3.0 * x0 + noise, 1
+    elif case == "decreasing":
+        y, sign = -3.0 * x0 + noise, -1
+    else:
+        era = (x0 > 0).astype(np.int32)
+        y, sign = np.where(era == 0, 2.0 * x0, 0.8 * x0) + noise, 1
+    y = y.astype(np.float32)
+    kw = dict(num_bins=32, max_depth=3, n_estimators=40, learning_rate=0.1, random_state=1)
+    m = WarpGBM(monotonic_constraints=[sign, 0], **kw).fit(X, y, era_id=era)
+    md = WarpGBM(monotonic_constraints={0: sign}, **kw).fit(X, y, era_id=era)
+    grid = np.linspace(-2.0, 2.0, 240).astype(np.float32)
+    Q = np.column_stack([grid, np.zeros_like(grid)]).astype(np.float32)
+    p = m.predict(Q)
+    assert np.all(sign * np.diff(p) >= -1e-5)
+    assert np.array_equal(p, md.predict(Q))
+    assert np.array_equal(m.predict(X), md.predict(X))
+
+
+def test_uniform_weights_equal_unweighted():
+    """tests/test_sample_weight.py:7-76: a constant weight changes nothing."""
+    from warpgbm_amd import WarpGBM
+    X, y = _reg()
+    kw = dict(num_bins=32, max_depth=4, n_estimators=10)
+    a = WarpGBM(**kw).fit(X, y)
+    b = WarpGBM(**kw).fit(X, y, sample_weight=np.full(len(y), 3.5, np.float32))
+    assert np.array_equal(a.predict(X), b.predict(X))
+
+
+def test_weighted_mean_tracked_without_splits():
+    """tests/test_sample_weight.py:142-178: with min_child_weight above the
+    total weight nothing splits, and the model converges to the weighted mean."""
+    from warpgbm_amd import WarpGBM
+    rng = np.random.default_rng(4)
+    n = 2000
+    X = rng.standard_normal((n, 5)).astype(np.float32)
+    y = np.where(rng.random(n) < 0.5, 0.0, 10.0).astype(np.float32)
+    w = np.where(y > 5, 3.0, 1.0).astype(np.float32)
+    m = WarpGBM(num_bins=16, max_depth=3, n_estimators=60, learning_rate=0.3, min_child_weight=n * 10).fit(
+        X, y, sample_weight=w)
+    assert all("leaf_value" in t for t in m.forest)
+    target = float(np.sum(w * y) / np.sum(w))
+    assert abs(float(np.mean(m.predict(X))) - target) < 0.15
+
+
+def test_random_state_reproducible_with_column_sampling():
+    """tests/test_random_state.py:7-135: same seed, same model."""
+    from warpgbm_amd import WarpGBM
+    X, y = _reg()
+    kw = dict(num_bins=32, max_depth=4, n_estimators=10, colsample_bytree=0.5, random_state=7)
+    a = WarpGBM(**kw).fit(X, y)
+    b = WarpGBM(**kw).fit(X, y)
+    assert np.array_equal(a.predict(X), b.predict(X))
+    c = WarpGBM(**dict(kw, random_state=8)).fit(X, y)
+    assert not np.array_equal(a.predict(X), c.predict(X))
+
+
+def test_multiclass_string_labels_probabilities_accuracy():
+    """tests/test_multiclass.py: string labels round-trip, rows sum to 1, the
+    model learns a separable problem."""
+    from warpgbm_amd import WarpGBM
+    X, lab = _cls(K=4)
+    names = np.array(["alpha", "beta", "gamma", "delta"])[lab]
+    m = WarpGBM(objective="multiclass", num_bins=32, max_depth=4, n_estimators=30, learning_rate=0.2).fit(X, names)
+    p = m.predict_proba(X)
+    np.testing.assert_allclose(p.sum(axis=1), 1.0, atol=1e-5)
+    pred = m.predict(X)
+    assert set(np.unique(pred)) <= set(names)
+    assert np.mean(pred == names) > 0.8
+
+
+def test_single_era_importance_equals_total():
+    """tests/test_feature_importance.py:265-298: one era -> per-era == total."""
+    from warpgbm_amd import WarpGBM
+    X, y = _reg()
+    m = WarpGBM(num_bins=32, max_depth=4, n_estimators=10).fit(X, y)
+    per = m.get_per_era_feature_importance(normalize=False)
+    assert per.shape == (1, X.shape[1])
+    np.testing.assert_allclose(per[0], m.get_feature_importance(normalize=False))
+    assert np.argmax(m.get_feature_importance()) == 0
+
+
+def test_fit_quality_correlation():
+    """tests/test_fit_predict_corr.py:8-52 (smaller): held-out correlation > 0.9."""
+    from warpgbm_amd import WarpGBM
+    rng = np.random.default_rng(9)
+    n, f = 60000, 100
+    X = rng.standard_normal((n, f)).astype(np.float32)
+    y = (X[:, :10] @ rng.standard_normal(10) + 0.3 * rng.standard_normal(n)).astype(np.float32)
+    m = WarpGBM(num_bins=32, max_depth=5, n_estimators=200, learning_rate=0.1).fit(X[:50000], y[:50000])
+    assert np.corrcoef(m.predict(X[50000:]), y[50000:])[0, 1] > 0.9
+
+
+@pytest.mark.parametrize("engine", ["batched", "plain"])
+def test_eval_set_and_early_stopping(engine):
+    """core.py:1488-1516: eval loss recorded every n trees; early stopping
+    ends training once the eval loss rises; both engines stop at the same tree."""
+    from warpgbm_amd import WarpGBM
+    X, y = _reg(n=4000)
+    rng = np.random.default_rng(11)
+    Xe = rng.standard_normal((1000, X.shape[1])).astype(np.float32)
+    ye = rng.standard_normal(1000).astype(np.float32)  # unrelated: eval loss soon rises
+    m = WarpGBM(num_bins=32, max_depth=4, n_estimators=200, learning_rate=0.3, engine=engine)
+    m.fit(X, y, X_eval=Xe, y_eval=ye, eval_every_n_trees=1, early_stopping_rounds=3)
+    assert 0 < m._trees_trained < 200
+    assert len(m.eval_loss) == m._trees_trained
+    ref = WarpGBM(num_bins=32, max_depth=4, n_estimators=200, learning_rate=0.3, engine="plain")
+    ref.fit(X, y, X_eval=Xe, y_eval=ye, eval_every_n_trees=1, early_stopping_rounds=3)
+    assert ref._trees_trained == m._trees_trained
