@@ -9,30 +9,29 @@
 // Work decomposition (one wave = 64 lanes):
 //   * a FEATURE SLAB is 64*FPL consecutive local features; lane l owns
 //     features [l*FPL, l*FPL+FPL) of the slab and loads their FPL code bytes
-//     of a row with one load (1..16 bytes);
+//     of a row with one load (1..16 bytes; byte gathers when the features
+//     are not the identity columns);
 //   * a CHUNK is up to C consecutive positions of one segment; a workgroup
-//     of W waves takes (chunk, slab) and splits the chunk into W contiguous
-//     sub-ranges, one per wave;
-//   * every wave accumulates its rows IN ORDER into its own LDS
-//     sub-histogram (lane-private addresses, so no two lanes ever touch the
-//     same word: ds_add_f32 without contention, deterministic);
-//   * the workgroup adds its W sub-histograms in wave order, and a
-//     segment's chunks are added in chunk order.
-// The float32 result therefore depends only on (C, W) and on the order of
-// positions, never on scheduling: plain per-node builds and batched level
-// builds of the same rows are bit-identical.
+//     of W waves takes a work item (chunk, slab), splits the chunk into W
+//     contiguous sub-ranges, one per wave, and every wave adds its rows into
+//     ONE shared LDS histogram of the item;
+//   * a segment's chunk partials are summed by k_hist_merge.
+// Sums are exact 64-bit fixed point (each row adds 2^44 + round(g * 2^sg)
+// with one ds_add_u64, the row count riding in the top bits, and
+// round(h * 2^sh) into a second plane unless the node's hessian is
+// constant), so the result is independent of scheduling, chunking and the
+// order of atomics: plain per-node builds and batched level builds of the
+// same rows are bit-identical, and every bin is rounded to float32 once.
 //
-// LDS layout per wave: [FPL][B][64] float2 (g, h), word (j*B + bin)*64 +
-// lane.  Lane l only ever touches its own words, so updates are plain
-// read-modify-write (ds_read_b64 / ds_write_b64, no atomics: LDS float
-// atomics serialise lanes), and whatever bins the 64 lanes hit, their 8-byte
-// words fall in distinct bank pairs.  A lane reads U rows' words at once and
-// forwards in registers when two of those rows share a bin, so the sum is
-// still the rows' sequential sum.  The read-out walks words in lane order.
+// LDS layout of a plane: [FPL][B + 1][64] words, word (j*(B+1) + bin)*64 +
+// lane.  Lane l only touches its own column, so the 64 lanes of every add
+// hit 64 different banks whatever bins they carry; bin B is a spare row
+// that absorbs codes outside [0, B).
 #include "common.h"
 #include "wgbm_internal.h"
 #include "wgbm_stats.h"
 #include "wgbm_plan.h"
+#include <type_traits>
 #include <utility>
 #include <vector>
 
@@ -164,7 +163,7 @@ k_hist_build(const HistArgs a) {
   // Features past n_feat (and a lane with none) add into their own lane's
   // columns, which the read-out skips.
   bool ident = a.wide_ok;
-  int cols[FPL];
+  uint32_t cols[FPL];  // byte offsets of the lane's columns in a row
   bool jon[FPL];
 #pragma unroll
   for (int j = 0; j < FPL; ++j) {
@@ -173,7 +172,7 @@ k_hist_build(const HistArgs a) {
     // A column outside the matrix contributes nothing (and is never read).
     jon[j] = fl < a.n_feat && (uint64_t)c < (uint64_t)a.row_stride;
     if (!jon[j]) c = 0;
-    cols[j] = c;
+    cols[j] = (uint32_t)c;
     if (fl < a.n_feat && c != fl) ident = false;
   }
   // Byte offset of (j, bin 0) in this lane's column; a column outside the
@@ -189,22 +188,8 @@ k_hist_build(const HistArgs a) {
   using TV = typename CodeVec<FPL>::T;
   const int8_t* __restrict__ codes = a.codes;
   const int64_t stride = a.row_stride;
-  static_assert(L <= 64 && (64 % L) == 0, "load block");
+  static_assert(L <= 32 && (kWave % L) == 0, "load block: an even number of groups per block");
 
-  auto load_codes = [&](int row) -> TV {
-    TV v;
-    if (!lane_on) {
-      v = TV{};
-    } else if (ident) {
-      v = *reinterpret_cast<const TV*>(codes + (int64_t)row * stride + fl0);
-    } else {
-      uint8_t tmp[FPL];
-#pragma unroll
-      for (int j = 0; j < FPL; ++j) tmp[j] = (uint8_t)codes[(int64_t)row * stride + cols[j]];
-      __builtin_memcpy(&v, tmp, sizeof(TV));
-    }
-    return v;
-  };
   auto add_row = [&](const TV& v, unsigned long long qp, unsigned long long qh) {
 #pragma unroll
     for (int j = 0; j < FPL; ++j) {
@@ -222,51 +207,105 @@ k_hist_build(const HistArgs a) {
     }
   };
 
-  for (int base = w0; base < w1; base += kWave) {
-    const int pos = base + lane;
-    const bool okp = pos < w1;
-    int myrow = okp ? a.perm[pos] : 0;
-    // Rows outside the matrix are skipped (they add zero words).
-    const bool okr = okp && (uint64_t)myrow < (uint64_t)a.n_rows;
-    if (!okr) myrow = 0;
-    float g = 0.0f, h = 0.0f;
-    if (okr) {
-      if constexpr (GH_POS) {
-        const float2 v = a.gh[pos];
-        g = v.x; h = v.y;
+  // Row id at position pn of this wave's range: the load is always issued
+  // (at a clamped position) and checked only where the id is used, so that
+  // it stays in flight meanwhile.  valid_row gives the id, or -1 outside the
+  // range or the matrix.
+  auto perm_at = [&](int pn) -> int { return a.perm[min(pn, w1 - 1)]; };
+  auto valid_row = [&](int pn, int t) -> int {
+    return (pn < w1 && (uint64_t)t < (uint64_t)a.n_rows) ? t : -1;
+  };
+  auto gh_at = [&](int pn) -> float2 { return a.gh[min(pn, w1 - 1)]; };
+
+  // Software pipeline over the wave's positions in blocks of 64: while the
+  // adds of one group of L rows run, the next group's code loads and the next
+  // block's row indices (and position-ordered g, h) are in flight.  Every
+  // load is issued unconditionally -- a lane without features reads the
+  // row's first bytes, a position without a row reads row 0 and adds a zero
+  // word -- so the compiler can count loads in flight and each group waits
+  // only for its own rows.
+  auto run = [&](auto ident_tag) {
+    constexpr bool ID = decltype(ident_tag)::value;
+    // A byte gather holds FPL registers per row in flight: groups of L/FPL rows.
+    constexpr int LG = ID ? L : (L / FPL > 0 ? L / FPL : 1);
+    constexpr int NG = kWave / LG;  // groups per 64-position block (even)
+    const uint32_t colo = lane_on ? (uint32_t)fl0 : 0u;
+    auto load_codes = [&](int row) -> TV {
+      const int8_t* rp = codes + (int64_t)max(row, 0) * stride;
+      TV v;
+      if constexpr (ID) {
+        v = *reinterpret_cast<const TV*>(rp + colo);
       } else {
-        g = a.g[myrow]; h = a.h[myrow];
+        uint8_t tmp[FPL];
+#pragma unroll
+        for (int j = 0; j < FPL; ++j) tmp[j] = (uint8_t)rp[cols[j]];
+        __builtin_memcpy(&v, tmp, sizeof(TV));
       }
+      return v;
+    };
+    // Fixed-point words of a block's rows (lane = position in the block).
+    unsigned long long myqp, myqh;
+    auto block_words = [&](int rowv_, float2 ghv_) {
+      const bool okr = rowv_ >= 0;
+      float g = ghv_.x, h = ghv_.y;
+      if constexpr (!GH_POS) {
+        const int r0 = max(rowv_, 0);
+        g = a.g[r0]; h = a.h[r0];
+      }
+      myqp = okr ? (1ull << kCountShift) + (unsigned long long)to_fixed(g, ns.sg) : 0ull;
+      myqh = okr ? (unsigned long long)to_fixed(h, ns.sh) : 0ull;
+    };
+    int rowv = valid_row(w0 + lane, perm_at(w0 + lane));
+    block_words(rowv, GH_POS ? gh_at(w0 + lane) : make_float2(0.0f, 0.0f));
+    TV A[LG], Bv[LG];
+#pragma unroll
+    for (int q = 0; q < LG; ++q) A[q] = load_codes(__builtin_amdgcn_readlane(rowv, q));
+    // Loop invariant: rowv, myqp, myqh describe this block and its first
+    // group's loads are in flight; nothing loaded is used at the loop head.
+    for (int base = w0; base < w1; base += kWave) {
+      // The next block's row ids (and g, h) go in flight now.
+      const int pn = base + kWave + lane;
+      const int rown_raw = perm_at(pn);
+      const float2 ghn = GH_POS ? gh_at(pn) : make_float2(0.0f, 0.0f);
+      const int cnt = min(kWave, w1 - base);
+      auto add_group = [&](const TV* V, int g0) {
+        if (g0 >= cnt) return;  // uniform: a partial last block
+#pragma unroll
+        for (int q = 0; q < LG; ++q) {
+          const int r = g0 + q;
+          const unsigned long long qp =
+              ((unsigned long long)(uint32_t)__builtin_amdgcn_readlane((int)(myqp >> 32), r) << 32) |
+              (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)myqp, r);
+          unsigned long long qh = 0ull;
+          if constexpr (!ONE_PLANE)
+            qh = ((unsigned long long)(uint32_t)__builtin_amdgcn_readlane((int)(myqh >> 32), r) << 32) |
+                 (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)myqh, r);
+          add_row(V[q], qp, qh);
+        }
+      };
+      int rown = -1;
+#pragma unroll
+      for (int gq = 0; gq < NG; gq += 2) {
+#pragma unroll
+        for (int q = 0; q < LG; ++q) Bv[q] = load_codes(__builtin_amdgcn_readlane(rowv, (gq + 1) * LG + q));
+        add_group(A, gq * LG);
+        if (gq + 2 < NG) {
+#pragma unroll
+          for (int q = 0; q < LG; ++q) A[q] = load_codes(__builtin_amdgcn_readlane(rowv, (gq + 2) * LG + q));
+        } else {
+          rown = valid_row(pn, rown_raw);
+#pragma unroll
+          for (int q = 0; q < LG; ++q) A[q] = load_codes(__builtin_amdgcn_readlane(rown, q));
+        }
+        add_group(Bv, (gq + 1) * LG);
+      }
+      rowv = rown;
+      block_words(rowv, ghn);
     }
-    const unsigned long long myqp =
-        okr ? (1ull << kCountShift) + (unsigned long long)to_fixed(g, ns.sg) : 0ull;
-    const unsigned long long myqh = okr ? (unsigned long long)to_fixed(h, ns.sh) : 0ull;
-    const int cnt = min(kWave, w1 - base);
-    // Software pipeline: the next L rows' code loads are issued before the
-    // current L rows are added, so every wave keeps loads in flight.
-    TV cur[L], nxt[L];
-#pragma unroll
-    for (int q = 0; q < L; ++q) cur[q] = load_codes(__builtin_amdgcn_readlane(myrow, q));
-    for (int u = 0; u < cnt; u += L) {
-      if (u + L < cnt) {
-#pragma unroll
-        for (int q = 0; q < L; ++q) nxt[q] = load_codes(__builtin_amdgcn_readlane(myrow, (u + L + q) & 63));
-      }
-#pragma unroll
-      for (int q = 0; q < L; ++q) {
-        const int r = (u + q) & 63;
-        const unsigned long long qp =
-            ((unsigned long long)(uint32_t)__builtin_amdgcn_readlane((int)(myqp >> 32), r) << 32) |
-            (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)myqp, r);
-        unsigned long long qh = 0ull;
-        if constexpr (!ONE_PLANE)
-          qh = ((unsigned long long)(uint32_t)__builtin_amdgcn_readlane((int)(myqh >> 32), r) << 32) |
-               (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)myqh, r);
-        add_row(cur[q], qp, qh);
-      }
-#pragma unroll
-      for (int q = 0; q < L; ++q) cur[q] = nxt[q];
-    }
+  };
+  if (a.n_rows > 0 && w0 < w1) {
+    if (ident) run(std::true_type{});
+    else run(std::false_type{});
   }
   __syncthreads();
 
@@ -553,6 +592,7 @@ HistConfig pick_hist_config(int n_feat, int n_bins, int64_t row_stride, bool one
   // 16 waves with 8 rows in flight build the root 10 % faster than 8 waves
   // with 16.
   c.waves = g_override.waves > 0 ? g_override.waves : (c.fpl <= 4 ? 16 : 8);
+  if (c.fpl > 4 && c.waves > 8) c.waves = 8;
   c.depth = clamp_depth(c.fpl, c.waves, g_override.depth);
   return c;
 }
@@ -664,8 +704,12 @@ static int launch_waves(const HistArgs& a, const HistConfig& c, int64_t groups, 
 
 template <int FPL, int L, bool CNT>
 static int launch_mode(const HistArgs& a, const HistConfig& c, int64_t groups, hipStream_t st) {
-  return c.waves > 8 ? launch_waves<FPL, L, CNT, true>(a, c, groups, st)
-                     : launch_waves<FPL, L, CNT, false>(a, c, groups, st);
+  // 16-wave workgroups exist for FPL <= 4 at the shallower depth only (the
+  // configuration picks them so): they must fit 64 VGPRs.
+  if constexpr (FPL <= 4 && L == (FPL == 1 ? 16 : 8 * (FPL == 2 ? 2 : 1))) {
+    if (c.waves > 8) return launch_waves<FPL, L, CNT, true>(a, c, groups, st);
+  }
+  return launch_waves<FPL, L, CNT, false>(a, c, groups, st);
 }
 
 template <int FPL, int L>
@@ -701,11 +745,11 @@ static int launch_fpl(const HistArgs& a, const HistConfig& c, int64_t max_chunks
 
 // Load depth (rows in flight per wave) allowed per features-per-lane.
 int clamp_depth(int fpl, int waves, int depth) {
-  const int hi = fpl == 1 ? 64 : fpl == 2 ? 32 : fpl == 4 ? 16 : fpl == 8 ? 8 : 4;
+  int hi = fpl <= 2 ? 32 : fpl == 4 ? 16 : fpl == 8 ? 8 : 4;
   const int lo = hi / 2;
-  // measured: FPL 1 at 64 rows in flight runs 3x slower than at 32; 16-wave
-  // workgroups keep the shallower depth within their 64 VGPRs
-  const int def = (fpl == 1 || waves > 8) ? lo : hi;
+  // 16-wave workgroups keep the shallower depth within their 64 VGPRs
+  if (waves > 8) hi = lo;
+  const int def = hi;
   int d = depth > 0 ? depth : def;
   if (d > hi) d = hi;
   if (d < lo) d = lo;
@@ -757,7 +801,7 @@ int hist_launch(HistArgs a, const HistConfig& c, int64_t rows_max, int64_t nseg_
   return c.depth <= D0 ? launch_fpl<FPL, D0>(a, c, max_chunks, max_multi, multi, n_multi, st) \
                        : launch_fpl<FPL, D1>(a, c, max_chunks, max_multi, multi, n_multi, st);
   switch (c.fpl) {
-    case 1: WGBM_D2(1, 32, 64)
+    case 1: WGBM_D2(1, 16, 32)
     case 2: WGBM_D2(2, 16, 32)
     case 4: WGBM_D2(4, 8, 16)
     case 8: WGBM_D2(8, 4, 8)
