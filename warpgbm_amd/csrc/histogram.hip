@@ -311,6 +311,8 @@ k_hist_build(const HistArgs a) {
 
   // Decode and write the chunk result.  Thread t reads word t (lane-fastest:
   // conflict free) = feature lane*FPL + j, bin, with (j, bin) = divmod(t/64, B).
+  // Chunk partials are stored in this word order (consecutive threads write
+  // consecutive words); k_hist_merge maps words to (feature, bin).
   const int SL = kWave * FPL * B;
   const int f_base = slab * kWave * FPL;
   const bool direct = (nch == 1);
@@ -332,7 +334,7 @@ k_hist_build(const HistArgs a) {
       if (a.accumulate) { a.out_g[d] += fg; a.out_h[d] += fh; }
       else { a.out_g[d] = fg; a.out_h[d] = fh; }
     } else {
-      const int64_t d = ((int64_t)(a.seg_pfirst[s] + k) * a.nslab + slab) * SL + fl * B + bin;
+      const int64_t d = ((int64_t)(a.seg_pfirst[s] + k) * a.nslab + slab) * SL + t;
       if constexpr (ONE_PLANE) reinterpret_cast<unsigned long long*>(a.part)[d] = w;  // packed count | qg sum
       else a.part[d] = make_longlong2(gsum, hsum);
     }
@@ -343,11 +345,23 @@ k_hist_build(const HistArgs a) {
 
 // Sum the chunk partials of every multi-chunk segment (exact integers) and
 // write the float32 histogram.  Work items (segment, slab, block of 32
-// elements) are taken round-robin; the 8 groups of 32 threads of a
-// workgroup sum every 8th chunk of the item and reduce through LDS, so a
-// segment of many chunks keeps many loads in flight.
+// words) are taken round-robin; the 8 groups of 32 threads of a workgroup
+// sum every 8th chunk of the item, four chunks' loads in flight per thread,
+// and reduce through LDS.  Partials are in the build's word order: word e =
+// (j, bin, lane) with (j, bin) = divmod(e / 64, B), feature lane*FPL + j.
 constexpr int kMergeElems = 32;
 constexpr int kMergeParts = 8;
+constexpr int kMergeUnroll = 4;
+
+struct MergeSum {
+  long long g = 0, x = 0;  // x: count (one plane) or hessian sum
+  __device__ __forceinline__ void add_packed(unsigned long long w) {
+    const long long gs = ((long long)(w << (64 - kCountShift))) >> (64 - kCountShift);
+    g += gs;
+    x += (long long)((w - (unsigned long long)gs) >> kCountShift);
+  }
+  __device__ __forceinline__ void add_pair(longlong2 v) { g += v.x; x += v.y; }
+};
 
 template <int FPL>
 __global__ void __launch_bounds__(kMergeElems * kMergeParts)
@@ -371,35 +385,44 @@ k_hist_merge(const HistArgs a, const int32_t* __restrict__ multi, const int32_t*
     const int slot = a.seg_slot[s];
     const NodeShift ns = node_shift(a.stats + (int64_t)(slot / a.slots_per_node) * kStatWords);
     const bool one_plane = a.hess_const || ns.count_mode;
-    long long sg = 0, sx = 0;  // sx: count (one plane) or hessian sum
+    MergeSum acc;
     if (e < SL) {
       const int64_t off = ((int64_t)first * a.nslab + slab) * SL + e;
       const int64_t step = (int64_t)a.nslab * SL;
+      // Chunks k, k + 8, k + 16, k + 24 at once; a chunk past the last one
+      // re-reads the last chunk and is dropped.
       if (one_plane) {
-        // one-plane partials: packed words (count << 44 | chunk qg sum)
         const unsigned long long* p = reinterpret_cast<const unsigned long long*>(a.part) + off;
-        for (int k = part; k < nch; k += kMergeParts) {
-          const unsigned long long w = p[(int64_t)k * step];
-          const long long gs = ((long long)(w << (64 - kCountShift))) >> (64 - kCountShift);
-          sg += gs;
-          sx += (long long)((w - (unsigned long long)gs) >> kCountShift);
+        for (int k = part; k < nch; k += kMergeUnroll * kMergeParts) {
+          unsigned long long w[kMergeUnroll];
+#pragma unroll
+          for (int u = 0; u < kMergeUnroll; ++u) w[u] = p[(int64_t)min(k + u * kMergeParts, nch - 1) * step];
+#pragma unroll
+          for (int u = 0; u < kMergeUnroll; ++u)
+            if (k + u * kMergeParts < nch) acc.add_packed(w[u]);
         }
       } else {
         const longlong2* p = a.part + off;
-        for (int k = part; k < nch; k += kMergeParts) {
-          const longlong2 v = p[(int64_t)k * step];
-          sg += v.x;
-          sx += v.y;
+        for (int k = part; k < nch; k += kMergeUnroll * kMergeParts) {
+          longlong2 v[kMergeUnroll];
+#pragma unroll
+          for (int u = 0; u < kMergeUnroll; ++u) v[u] = p[(int64_t)min(k + u * kMergeParts, nch - 1) * step];
+#pragma unroll
+          for (int u = 0; u < kMergeUnroll; ++u)
+            if (k + u * kMergeParts < nch) acc.add_pair(v[u]);
         }
       }
     }
-    s_g[part][el] = sg;
-    s_h[part][el] = sx;
+    s_g[part][el] = acc.g;
+    s_h[part][el] = acc.x;
     __syncthreads();
     if (part == 0 && e < SL) {
+      long long sg = acc.g, sx = acc.x;
       for (int q = 1; q < kMergeParts; ++q) { sg += s_g[q][el]; sx += s_h[q][el]; }
-      const int fl = e / B;
-      const int bin = e - fl * B;
+      const int rr = e >> 6;
+      const int j = rr / B;
+      const int bin = rr - j * B;
+      const int fl = (e & 63) * FPL + j;
       const int f_base = slab * kWave * FPL;
       if (f_base + fl < a.n_feat) {
         const long long sh = one_plane ? sx * ns.qh_const : sx;
