@@ -285,43 +285,6 @@ k_grad_softmax_pos(const float* __restrict__ F, const int32_t* __restrict__ y, c
   }
 }
 
-// Record level d of every class tree into the round's node tables.
-__global__ void __launch_bounds__(256)
-k_record_level(int d, int D, int K, int E, int n_nodes, const int32_t* __restrict__ active,
-               const int32_t* __restrict__ dec_feat, const int32_t* __restrict__ dec_bin,
-               const int32_t* __restrict__ feat_idx, const float* __restrict__ leaf_val,
-               const int32_t* __restrict__ node_count, const float* __restrict__ era_gain,
-               const int32_t* __restrict__ round_ctr, int32_t* status, int32_t* feat, int32_t* bin, float* value,
-               int32_t* count, float* out_era_gain) {
-  // thread per (node, era): era 0's thread writes the node's fields
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (int64_t)n_nodes * E) return;
-  const int j = (int)(t / E);
-  const int e0 = (int)(t - (int64_t)j * E);
-  const int NH = (2 << D) - 1;
-  const int c = j >> d, jj = j & ((1 << d) - 1);
-  const int64_t o = ((int64_t)round_ctr[0] * K + c) * NH + ((1 << d) - 1) + jj;
-  int st = 0, f = -1, b = -1;
-  float v = 0.0f;
-  if (active[j]) {
-    if (d < D && dec_feat[j] >= 0) {
-      st = 1;
-      f = feat_idx ? feat_idx[dec_feat[j]] : dec_feat[j];
-      b = dec_bin[j];
-    } else {
-      st = 2;
-      v = leaf_val[j];
-    }
-  }
-  if (e0 == 0) {
-    status[o] = st;
-    feat[o] = f;
-    bin[o] = b;
-    value[o] = v;
-    count[o] = active[j] ? node_count[j] : 0;
-  }
-  if (out_era_gain) out_era_gain[o * E + e0] = (st == 1) ? era_gain[(int64_t)j * E + e0] : 0.0f;
-}
 
 }  // namespace wgbm
 
@@ -547,29 +510,53 @@ static int enqueue_round(const wgbm_fit_config& c, const FitLayout& L, const wgb
         rc = level_scan_launch(s, st, &leaf_job, &part_job);
         if (rc != kOk) return rc;
       }
-      rc = leaf_launch(la, L.KN, st, d < D);
-      if (rc != kOk) return rc;
-      hipLaunchKernelGGL(k_record_level, dim3((unsigned)ceil_div((int64_t)n_nodes * E, 256)), dim3(256), 0, st, d, D, K, E,
-                         n_nodes, I(cur.node_active), I(L.dec_feat), I(L.dec_bin), fidx, Fp(L.l_val),
-                         I(cur.node_count), Fp(L.era_gain), I(L.round_ctr), io->node_status, io->node_feat,
-                         io->node_bin, io->node_value, io->node_count, io->node_era_gain);
-      WGBM_LAUNCHED("k_record_level");
-      if (d < D) {
-        // the next level's histogram plan runs beside the child statistics
-        PlanJob next{};
-        if (d + 1 < D) {
-          HistArgs na{};
-          na.n_feat = c.n_feat;
-          na.seg_count = I(L.b_seg_count);
-          na.nseg_dev = I(L.n_build);
-          na.chunk_dev = I(L.h_chunk);
-          na.seg_first = I(L.h_seg_first);
-          na.seg_pfirst = I(L.h_seg_pfirst);
-          next = hist_plan_job(na, L.hc, I(L.h_multi), I(L.h_n_multi));
-        }
-        rc = partition_launch(p, L.KN, st, true, d + 1 < D ? &next : nullptr);
-        if (rc != kOk) return rc;
+      RecordArgs r{};
+      r.d = d;
+      r.D = D;
+      r.K = K;
+      r.E = E;
+      r.n_nodes = n_nodes;
+      r.active = I(cur.node_active);
+      r.dec_feat = I(L.dec_feat);
+      r.dec_bin = I(L.dec_bin);
+      r.feat_idx = fidx;
+      r.node_count = I(cur.node_count);
+      r.era_gain = Fp(L.era_gain);
+      r.round_ctr = I(L.round_ctr);
+      r.status = io->node_status;
+      r.feat = io->node_feat;
+      r.bin = io->node_bin;
+      r.value = io->node_value;
+      r.count = io->node_count;
+      r.out_era_gain = io->node_era_gain;
+      if (d == 0 && D == 0) {  // no decisions ran: the leaf plan of the only level
+        const PlanJob j = leaf_plan_job(la);
+        plan_launch(j.src, j.chunk_rows, j.seg_first, j.multi, j.n_multi, st);
+        WGBM_LAUNCHED("k_hist_plan(leaf)");
       }
+      // The plan that runs beside the child statistics: the next level's
+      // histogram plan, or before the last level its leaf plan.
+      PlanJob next{};
+      if (d + 1 < D) {
+        HistArgs na{};
+        na.n_feat = c.n_feat;
+        na.seg_count = I(L.b_seg_count);
+        na.nseg_dev = I(L.n_build);
+        na.chunk_dev = I(L.h_chunk);
+        na.seg_first = I(L.h_seg_first);
+        na.seg_pfirst = I(L.h_seg_pfirst);
+        next = hist_plan_job(na, L.hc, I(L.h_multi), I(L.h_n_multi));
+      } else if (d + 1 == D) {
+        LeafArgs ln = la;
+        ln.n_nodes = n_nodes * 2;
+        ln.node_active = I(nxt.node_active);
+        ln.dec_feat = nullptr;
+        ln.node_start = I(nxt.node_start);
+        ln.node_count = I(nxt.node_count);
+        next = leaf_plan_job(ln);
+      }
+      rc = level_tail_launch(d < D ? &p : nullptr, la, r, L.KN, st, d < D ? &next : nullptr);
+      if (rc != kOk) return rc;
     }
   return kOk;
 }

@@ -33,9 +33,10 @@ __device__ __forceinline__ bool goes_left(const PartArgs& a, int row, int col, i
   return (int)code <= bin;
 }
 
-__global__ void __launch_bounds__(256)
-k_part_count(PartArgs a) {
-  const int chunk = blockIdx.x;
+// Block bodies take the chunk as an argument: the kernels below run them one
+// chunk per workgroup, and the batched engine's fused level launches run
+// several of them side by side over disjoint workgroup ranges.
+__device__ __forceinline__ void part_count_block(const PartArgs& a, int chunk) {
   const int nseg = *a.nseg_dev;
   if (chunk >= a.seg_first[nseg]) return;
   const int s = upper_index(a.seg_first, nseg, chunk);
@@ -57,6 +58,9 @@ k_part_count(PartArgs a) {
   __syncthreads();
   if (threadIdx.x == 0) a.lbase[chunk] = s_c[0] + s_c[1] + s_c[2] + s_c[3];
 }
+
+__global__ void __launch_bounds__(256)
+k_part_count(PartArgs a) { part_count_block(a, blockIdx.x); }
 
 // One workgroup: per-segment left totals and chunk offsets, child segment
 // tables, smaller-child build list and sibling map.
@@ -173,9 +177,7 @@ k_part_scan(PartArgs a) {
   }
 }
 
-__global__ void __launch_bounds__(256)
-k_part_scatter(PartArgs a) {
-  const int chunk = blockIdx.x;
+__device__ __forceinline__ void part_scatter_block(const PartArgs& a, int chunk) {
   const int nseg = *a.nseg_dev;
   if (chunk >= a.seg_first[nseg]) return;
   const int s = upper_index(a.seg_first, nseg, chunk);
@@ -228,6 +230,9 @@ k_part_scatter(PartArgs a) {
     }
   }
 }
+
+__global__ void __launch_bounds__(256)
+k_part_scatter(PartArgs a) { part_scatter_block(a, blockIdx.x); }
 
 // Child c of node j = c / 2 (side c % 2) folds the partials of j's chunks,
 // which are contiguous: [seg_first[j*E], seg_first[(j+1)*E]).
@@ -326,9 +331,7 @@ __device__ __forceinline__ void leaf_gh(const LeafArgs& a, int p, double& g, dou
   }
 }
 
-__global__ void __launch_bounds__(256)
-k_leaf_partial(LeafArgs a) {
-  const int chunk = blockIdx.x;
+__device__ __forceinline__ void leaf_partial_block(const LeafArgs& a, int chunk) {
   const int nseg = *a.nseg_dev;
   if (chunk >= a.seg_first[nseg]) return;
   const int j = upper_index(a.seg_first, nseg, chunk);
@@ -354,24 +357,32 @@ k_leaf_partial(LeafArgs a) {
   if (threadIdx.x == 0) { a.part[2 * (int64_t)chunk] = s_g[0]; a.part[2 * (int64_t)chunk + 1] = s_h[0]; }
 }
 
-// Per chunk: the node's total from its chunk sums (chunk order, float64),
-// leaf = f32(sum g / (sum h + lambda)) written once per node (chunk 0), and
-// pred[row*K + class] += f32(lr) * leaf for the chunk's rows.
 __global__ void __launch_bounds__(256)
-k_leaf_apply(LeafArgs a) {
-  const int chunk = blockIdx.x;
+k_leaf_partial(LeafArgs a) { leaf_partial_block(a, blockIdx.x); }
+
+// Leaf j's total from its chunk sums (chunk order, float64) and its value
+// f32(sum g / (sum h + lambda)).
+__device__ __forceinline__ float leaf_value(const LeafArgs& a, int j, double& sg, double& sh) {
+  sg = 0.0;
+  sh = 0.0;
+  for (int c = a.seg_first[j]; c < a.seg_first[j + 1]; ++c) {
+    sg += a.part[2 * (int64_t)c];
+    sh += a.part[2 * (int64_t)c + 1];
+  }
+  return (float)(sg / (sh + a.lam));
+}
+
+// Per chunk: the leaf value, written once per node (chunk 0), and
+// pred[row*K + class] += f32(lr) * leaf for the chunk's rows.
+__device__ __forceinline__ void leaf_apply_block(const LeafArgs& a, int chunk) {
   const int nseg = *a.nseg_dev;
   if (chunk >= a.seg_first[nseg]) return;
   const int j = upper_index(a.seg_first, nseg, chunk);
   const int k = chunk - a.seg_first[j];
   __shared__ float s_v;
   if (threadIdx.x == 0) {
-    double sg = 0.0, sh = 0.0;
-    for (int c = a.seg_first[j]; c < a.seg_first[j + 1]; ++c) {
-      sg += a.part[2 * (int64_t)c];
-      sh += a.part[2 * (int64_t)c + 1];
-    }
-    const float v = (float)(sg / (sh + a.lam));
+    double sg, sh;
+    const float v = leaf_value(a, j, sg, sh);
     s_v = v;
     if (k == 0) {
       a.leaf_val[j] = v;
@@ -391,6 +402,9 @@ k_leaf_apply(LeafArgs a) {
     *d = *d + step;
   }
 }
+
+__global__ void __launch_bounds__(256)
+k_leaf_apply(LeafArgs a) { leaf_apply_block(a, blockIdx.x); }
 
 PlanJob leaf_plan_job(const LeafArgs& a) {
   PlanJob j{};
@@ -426,6 +440,103 @@ int leaf_launch(const LeafArgs& a, int64_t rows_max, hipStream_t st, bool planne
 }
 
 int64_t leaf_max_chunks(int64_t rows_max, int64_t n_nodes) { return ceil_div(rows_max, kLeafChunk) + n_nodes; }
+
+// ----------------------------------------------------------------- fused level tail
+// Tree record of one level, thread per (node, era): era 0's thread writes
+// the node's fields at heap index (2^d - 1) + node of class c's tree of round
+// round_ctr[0]; a leaf's value comes from its chunk sums exactly as
+// leaf_apply_block forms it.
+__device__ __forceinline__ void record_thread(const RecordArgs& r, const LeafArgs& la, int64_t t) {
+  if (t >= (int64_t)r.n_nodes * r.E) return;
+  const int j = (int)(t / r.E);
+  const int e0 = (int)(t - (int64_t)j * r.E);
+  const int NH = (2 << r.D) - 1;
+  const int c = j >> r.d, jj = j & ((1 << r.d) - 1);
+  const int64_t o = ((int64_t)r.round_ctr[0] * r.K + c) * NH + ((1 << r.d) - 1) + jj;
+  int st = 0, f = -1, b = -1;
+  float v = 0.0f;
+  if (r.active[j]) {
+    if (r.d < r.D && r.dec_feat[j] >= 0) {
+      st = 1;
+      f = r.feat_idx ? r.feat_idx[r.dec_feat[j]] : r.dec_feat[j];
+      b = r.dec_bin[j];
+    } else {
+      st = 2;
+      double sg, sh;
+      v = leaf_value(la, j, sg, sh);
+    }
+  }
+  if (e0 == 0) {
+    r.status[o] = st;
+    r.feat[o] = f;
+    r.bin[o] = b;
+    r.value[o] = v;
+    r.count[o] = r.active[j] ? r.node_count[j] : 0;
+  }
+  if (r.out_era_gain) r.out_era_gain[o * r.E + e0] = (st == 1) ? r.era_gain[(int64_t)j * r.E + e0] : 0.0f;
+}
+
+// Partition counts of the split segments (workgroups [0, n_count)) and leaf
+// chunk sums (the rest).
+__global__ void __launch_bounds__(256)
+k_count_leafsum(PartArgs p, LeafArgs la, int n_count) {
+  const int b = blockIdx.x;
+  if (b < n_count) part_count_block(p, b);
+  else leaf_partial_block(la, b - n_count);
+}
+
+// Stable scatter of the split segments (workgroups [0, n_move)), leaf score
+// updates ([n_move, n_move + n_apply)) and the level's tree record (the
+// rest, 256 (node, era) pairs per workgroup).
+__global__ void __launch_bounds__(256)
+k_move_apply_record(PartArgs p, LeafArgs la, RecordArgs r, int n_move, int n_apply) {
+  const int b = blockIdx.x;
+  if (b < n_move) part_scatter_block(p, b);
+  else if (b < n_move + n_apply) leaf_apply_block(la, b - n_move);
+  else record_thread(r, la, (int64_t)(b - n_move - n_apply) * 256 + threadIdx.x);
+}
+
+int level_tail_launch(const PartArgs* p, const LeafArgs& la, const RecordArgs& r, int64_t rows_max, hipStream_t st,
+                      const PlanJob* next) {
+  const int nseg = p ? p->n_nodes * p->n_eras : 0;
+  if (p) {
+    WGBM_REQUIRE(p->codes && p->node_active && p->dec_feat && p->dec_bin && p->seg_start && p->seg_count &&
+                     p->node_start && p->node_count && p->perm_in && p->perm_out && p->flag && p->seg_first &&
+                     p->nseg_dev && p->lbase && p->rbase && p->cscan && p->segL && p->segP && p->c_seg_start &&
+                     p->c_seg_count && p->c_active && p->c_node_start && p->c_node_count && p->b_seg_start &&
+                     p->b_seg_count && p->b_seg_slot && p->n_build && p->sib && p->n_split && p->chunk_rows > 0 &&
+                     p->c_stats && p->stat_part,
+                 "level tail: missing partition table or scratch pointer");
+  }
+  WGBM_REQUIRE(la.node_active && la.node_start && la.node_count && la.seg_first && la.nseg_dev && la.perm &&
+                   (la.gh || (la.g && la.h)) && la.leaf_val && la.part && r.active && r.node_count && r.round_ctr &&
+                   r.status && r.feat && r.bin && r.value && r.count && (r.d == r.D || (r.dec_feat && r.dec_bin)),
+               "level tail: missing leaf or record pointer");
+  const int64_t n_count = p ? ceil_div(rows_max, p->chunk_rows) + nseg : 0;
+  const int64_t n_leaf = leaf_max_chunks(rows_max, la.n_nodes);
+  const int64_t n_rec = ceil_div((int64_t)r.n_nodes * r.E, 256);
+  if (n_count + n_leaf + n_rec > 0x7fffffff) { set_error("level tail grid too large"); return kTooBig; }
+  hipLaunchKernelGGL(k_count_leafsum, dim3((unsigned)(n_count + n_leaf)), dim3(256), 0, st,
+                     p ? *p : PartArgs{}, la, (int)n_count);
+  WGBM_LAUNCHED("k_count_leafsum");
+  if (p) {
+    hipLaunchKernelGGL(k_part_scan, dim3(1), dim3(1024), 0, st, *p);
+    WGBM_LAUNCHED("k_part_scan");
+  }
+  hipLaunchKernelGGL(k_move_apply_record, dim3((unsigned)(n_count + n_leaf + n_rec)), dim3(256), 0, st,
+                     p ? *p : PartArgs{}, la, r, (int)n_count, (int)n_leaf);
+  WGBM_LAUNCHED("k_move_apply_record");
+  if (p) {
+    if (next) {
+      hipLaunchKernelGGL(k_child_stats_plan, dim3((unsigned)(2 * p->n_nodes + 1)), dim3(1024), 0, st, *p, *next);
+      WGBM_LAUNCHED("k_child_stats_plan");
+    } else {
+      hipLaunchKernelGGL(k_child_stats, dim3((unsigned)(2 * p->n_nodes)), dim3(256), 0, st, *p);
+      WGBM_LAUNCHED("k_child_stats");
+    }
+  }
+  return kOk;
+}
 
 // ----------------------------------------------------------------- subtract
 // other = parent - built for every split of the previous level

@@ -173,11 +173,19 @@ k_level_scan(const LevelScanArgs a) {
         const float* g = a.G + node_off + ((int64_t)e * a.n_feat + f) * B;
         const float* h = a.H + node_off + ((int64_t)e * a.n_feat + f) * B;
         double Gt = 0.0, Ht = 0.0, GL = 0.0, HL = 0.0;
-        for (int i = 0; i < B; ++i) {
-          const double gi = (double)g[i], hi = (double)h[i];
-          Gt += gi;
-          Ht += hi;
-          if (i <= b) { GL += gi; HL += hi; }
+        // Lane i loads bin i0 + i (one coalesced load); every lane then sums
+        // the bins in bin order, taking bin q from lane q.
+        for (int i0 = 0; i0 < B; i0 += 64) {
+          const int iq = min(i0 + lane, B - 1);
+          const float gv = g[iq], hv = h[iq];
+          const int n = min(64, B - i0);
+          for (int q = 0; q < n; ++q) {
+            const double gi = (double)__builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, gv), q));
+            const double hi = (double)__builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, hv), q));
+            Gt += gi;
+            Ht += hi;
+            if (i0 + q <= b) { GL += gi; HL += hi; }
+          }
         }
         float ga, di;
         gain_dir(GL, HL, Gt, Ht, (double)a.mcw, (double)a.lam, ga, di);
@@ -218,10 +226,18 @@ k_level_scan(const LevelScanArgs a) {
 __device__ void pick_node(const LevelScanArgs& a, int node, int wg_per_node, int lane) {
   Cand b = cand_none();
   if (a.node_active[node]) {
-    for (int w = lane; w < wg_per_node; w += 64) {
-      const int64_t o = (int64_t)node * wg_per_node + w;
-      const Cand k{a.cand_agree[o], a.cand_crit[o], a.cand_idx[o]};
-      if (cand_better(k, b)) b = k;
+    // Four candidates' loads per lane in flight, then the comparisons.
+    constexpr int U = 4;
+    for (int w0 = lane; w0 < wg_per_node; w0 += 64 * U) {
+      Cand k[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int64_t o = (int64_t)node * wg_per_node + min(w0 + 64 * u, wg_per_node - 1);
+        k[u] = Cand{a.cand_agree[o], a.cand_crit[o], a.cand_idx[o]};
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (w0 + 64 * u < wg_per_node && cand_better(k[u], b)) b = k[u];
     }
   }
   b = wave_best(b);
@@ -300,18 +316,37 @@ k_level_gains(const LevelScanArgs a, int node0, int n_group) {
   const int64_t off = ((int64_t)node0 * E * F + t) * B;  // histogram [node][era][feature][bin]
   const float* g = a.G + off;
   const float* h = a.H + off;
+  // Bins are read U at a time (independent loads in flight), summed in bin
+  // order.
+  constexpr int U = 8;
   double Gt = 0.0, Ht = 0.0;
-  for (int b = 0; b < B; ++b) { Gt += (double)g[b]; Ht += (double)h[b]; }
+  for (int b0 = 0; b0 < B; b0 += U) {
+    float gv[U], hv[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) { const int q = min(b0 + u, B - 1); gv[u] = g[q]; hv[u] = h[q]; }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (b0 + u < B) { Gt += (double)gv[u]; Ht += (double)hv[u]; }
+  }
   double GL = 0.0, HL = 0.0;
   float* go = a.era_g + t * (B - 1);
   int8_t* dO = a.era_d + t * (B - 1);
-  for (int b = 0; b < B - 1; ++b) {
-    GL += (double)g[b];
-    HL += (double)h[b];
-    float ga, di;
-    gain_dir(GL, HL, Gt, Ht, (double)a.mcw, (double)a.lam, ga, di);
-    go[b] = ga;
-    dO[b] = (int8_t)di;
+  for (int b0 = 0; b0 < B - 1; b0 += U) {
+    float gv[U], hv[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) { const int q = min(b0 + u, B - 1); gv[u] = g[q]; hv[u] = h[q]; }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int b = b0 + u;
+      if (b < B - 1) {
+        GL += (double)gv[u];
+        HL += (double)hv[u];
+        float ga, di;
+        gain_dir(GL, HL, Gt, Ht, (double)a.mcw, (double)a.lam, ga, di);
+        go[b] = ga;
+        dO[b] = (int8_t)di;
+      }
+    }
   }
 }
 

@@ -80,6 +80,34 @@ struct LeafArgs {
   double* part;                 // [max chunks * 2]
 };
 
+// Where the tree record of one level goes (flat node arrays of the caller,
+// heap order per (round, class) tree of 2^(D+1) - 1 nodes).
+struct RecordArgs {
+  int32_t d, D, K, E, n_nodes;
+  const int32_t* active;        // [n_nodes]
+  const int32_t* dec_feat;      // [n_nodes] local feature or -1 (unused at d == D)
+  const int32_t* dec_bin;
+  const int32_t* feat_idx;      // local -> global feature, or nullptr
+  const int32_t* node_count;
+  const float* era_gain;        // [n_nodes*E] winner's per-era gains (with out_era_gain)
+  const int32_t* round_ctr;     // device round index
+  int32_t* status;
+  int32_t* feat;
+  int32_t* bin;
+  float* value;
+  int32_t* count;
+  float* out_era_gain;          // nullable
+};
+
+// The batched engine's level tail in three or four launches: partition
+// counts with leaf chunk sums; the partition scan; stable scatter with leaf
+// score updates and the level's tree record; child statistics with `next`
+// (the next level's histogram or leaf plan) when given.  p == nullptr at
+// the last level (leaves and record only).  The leaf plan must already be
+// in la's plan tables.
+int level_tail_launch(const PartArgs* p, const LeafArgs& la, const RecordArgs& r, int64_t rows_max, hipStream_t st,
+                      const PlanJob* next);
+
 // planned: the plan ran in an earlier fused launch (leaf_plan_job /
 // part_plan_job beside the level's decisions).  next_hist: run the next level's histogram
 // plan in the same launch as the child statistics.
