@@ -477,10 +477,10 @@ static int enqueue_round(const wgbm_fit_config& c, const FitLayout& L, const wgb
           a.seg_slot = I(L.b_seg_slot);
           a.nseg_dev = I(L.n_build);
           a.planned = 1;  // planned beside the previous level's child statistics
+          // each built child also writes its sibling = parent - built
+          a.sub_g = Fp(L.hg[(d - 1) & 1]);
+          a.sub_h = Fp(L.hh[(d - 1) & 1]);
           rc = hist_launch(a, L.hc, L.KN / 2 + 1, ((int64_t)K << (d - 1)) * E, I(L.h_multi), I(L.h_n_multi), st);
-          if (rc != kOk) return rc;
-          rc = subtract_launch(Fp(L.hg[(d - 1) & 1]), Fp(L.hh[(d - 1) & 1]), hg, hh, I(L.sib), I(L.n_split),
-                               (int64_t)K << (d - 1), L.node_elems, st);
         }
         if (rc != kOk) return rc;
         LevelScanArgs s{};

@@ -100,6 +100,39 @@ __device__ __forceinline__ NodeShift node_shift(const int32_t* st) {
   return n;
 }
 
+// Sibling derivation: the parent's bin (loaded ahead of the result, so its
+// latency overlaps the build's own loads) and where the sibling's bin goes.
+struct SibBin {
+  float pg = 0.0f, ph = 0.0f;
+  int64_t ds = -1;  // -1: no sibling to write
+};
+
+__device__ __forceinline__ SibBin load_sib(const HistArgs& a, int slot, int f, int bin) {
+  SibBin b;
+  if (a.sub_g) {
+    const int E = a.slots_per_node;
+    const int c = slot / E, e = slot - c * E;
+    const int64_t dp = ((int64_t)((c >> 1) * E + e) * a.n_feat + f) * a.n_bins + bin;
+    b.ds = ((int64_t)((c ^ 1) * E + e) * a.n_feat + f) * a.n_bins + bin;
+    b.pg = a.sub_g[dp];
+    b.ph = a.sub_h[dp];
+  }
+  return b;
+}
+
+// Store one finished bin of slot `slot` and, when the launch derives
+// siblings, the sibling's bin = parent - this (float32).
+__device__ __forceinline__ void store_bin(const HistArgs& a, int slot, int f, int bin, float fg, float fh,
+                                          const SibBin& sb) {
+  const int64_t d = ((int64_t)slot * a.n_feat + f) * a.n_bins + bin;
+  if (a.accumulate) { a.out_g[d] += fg; a.out_h[d] += fh; }
+  else { a.out_g[d] = fg; a.out_h[d] = fh; }
+  if (sb.ds >= 0) {
+    a.out_g[sb.ds] = sb.pg - fg;
+    a.out_h[sb.ds] = sb.ph - fh;
+  }
+}
+
 // ONE_PLANE: only the T plane is allocated and H = count * qh, valid for a
 // node whose rows all have one hessian value (min h == max h).  Unless the
 // caller guarantees that for every node (a.hess_const), both variants are
@@ -316,27 +349,41 @@ k_hist_build(const HistArgs a) {
   const int SL = kWave * FPL * B;
   const int f_base = slab * kWave * FPL;
   const bool direct = (nch == 1);
-  for (int t = threadIdx.x; t < SL; t += blockDim.x) {
-    const int r = t >> 6;
-    const int ln = t & 63;
-    const int j = r / B;
-    const int bin = r - j * B;
-    const int fl = ln * FPL + j;
-    if (f_base + fl >= a.n_feat) continue;
-    const int word = t + j * kWave;  // skip the spare rows of features 0..j-1
-    const unsigned long long w = T[word];
-    const long long gsum = ((long long)(w << (64 - kCountShift))) >> (64 - kCountShift);
-    const long long count = (long long)((w - (unsigned long long)gsum) >> kCountShift);
-    const long long hsum = ONE_PLANE ? count * ns.qh_const : (long long)Hq[word];
-    if (direct) {
-      const int64_t d = ((int64_t)slot * a.n_feat + f_base + fl) * B + bin;
-      const float fg = from_fixed(gsum, ns.sg), fh = from_fixed(hsum, ns.sh);
-      if (a.accumulate) { a.out_g[d] += fg; a.out_h[d] += fh; }
-      else { a.out_g[d] = fg; a.out_h[d] = fh; }
-    } else {
-      const int64_t d = ((int64_t)(a.seg_pfirst[s] + k) * a.nslab + slab) * SL + t;
-      if constexpr (ONE_PLANE) reinterpret_cast<unsigned long long*>(a.part)[d] = w;  // packed count | qg sum
-      else a.part[d] = make_longlong2(gsum, hsum);
+  constexpr int kFlushBatch = 4;  // words per thread whose sibling loads are in flight together
+  for (int t0 = threadIdx.x; t0 < SL; t0 += kFlushBatch * blockDim.x) {
+    SibBin sb[kFlushBatch];
+    if (direct && a.sub_g) {
+#pragma unroll
+      for (int u = 0; u < kFlushBatch; ++u) {
+        const int t = t0 + u * blockDim.x;
+        const int r = t >> 6;
+        const int j = r / B;
+        const int fl = (t & 63) * FPL + j;
+        if (t < SL && f_base + fl < a.n_feat) sb[u] = load_sib(a, slot, f_base + fl, r - j * B);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kFlushBatch; ++u) {
+      const int t = t0 + u * blockDim.x;
+      if (t >= SL) break;
+      const int r = t >> 6;
+      const int ln = t & 63;
+      const int j = r / B;
+      const int bin = r - j * B;
+      const int fl = ln * FPL + j;
+      if (f_base + fl >= a.n_feat) continue;
+      const int word = t + j * kWave;  // skip the spare rows of features 0..j-1
+      const unsigned long long w = T[word];
+      const long long gsum = ((long long)(w << (64 - kCountShift))) >> (64 - kCountShift);
+      const long long count = (long long)((w - (unsigned long long)gsum) >> kCountShift);
+      const long long hsum = ONE_PLANE ? count * ns.qh_const : (long long)Hq[word];
+      if (direct) {
+        store_bin(a, slot, f_base + fl, bin, from_fixed(gsum, ns.sg), from_fixed(hsum, ns.sh), sb[u]);
+      } else {
+        const int64_t d = ((int64_t)(a.seg_pfirst[s] + k) * a.nslab + slab) * SL + t;
+        if constexpr (ONE_PLANE) reinterpret_cast<unsigned long long*>(a.part)[d] = w;  // packed count | qg sum
+        else a.part[d] = make_longlong2(gsum, hsum);
+      }
     }
   }
   __syncthreads();  // the read-out is done before the next item clears the LDS
@@ -385,6 +432,13 @@ k_hist_merge(const HistArgs a, const int32_t* __restrict__ multi, const int32_t*
     const int slot = a.seg_slot[s];
     const NodeShift ns = node_shift(a.stats + (int64_t)(slot / a.slots_per_node) * kStatWords);
     const bool one_plane = a.hess_const || ns.count_mode;
+    const int rr = e >> 6;
+    const int j = rr / B;
+    const int bin = rr - j * B;
+    const int fl = (e & 63) * FPL + j;
+    const int f_base = slab * kWave * FPL;
+    SibBin sb;
+    if (part == 0 && e < SL && f_base + fl < a.n_feat) sb = load_sib(a, slot, f_base + fl, bin);
     MergeSum acc;
     if (e < SL) {
       const int64_t off = ((int64_t)first * a.nslab + slab) * SL + e;
@@ -419,17 +473,9 @@ k_hist_merge(const HistArgs a, const int32_t* __restrict__ multi, const int32_t*
     if (part == 0 && e < SL) {
       long long sg = acc.g, sx = acc.x;
       for (int q = 1; q < kMergeParts; ++q) { sg += s_g[q][el]; sx += s_h[q][el]; }
-      const int rr = e >> 6;
-      const int j = rr / B;
-      const int bin = rr - j * B;
-      const int fl = (e & 63) * FPL + j;
-      const int f_base = slab * kWave * FPL;
       if (f_base + fl < a.n_feat) {
         const long long sh = one_plane ? sx * ns.qh_const : sx;
-        const int64_t d = ((int64_t)slot * a.n_feat + f_base + fl) * B + bin;
-        const float fg = from_fixed(sg, ns.sg), fh = from_fixed(sh, ns.sh);
-        if (a.accumulate) { a.out_g[d] += fg; a.out_h[d] += fh; }
-        else { a.out_g[d] = fg; a.out_h[d] = fh; }
+        store_bin(a, slot, f_base + fl, bin, from_fixed(sg, ns.sg), from_fixed(sh, ns.sh), sb);
       }
     }
     __syncthreads();
@@ -799,6 +845,8 @@ PlanJob hist_plan_job(const HistArgs& a, const HistConfig& c, int32_t* multi, in
 
 int hist_launch(HistArgs a, const HistConfig& c, int64_t rows_max, int64_t nseg_max,
                 int32_t* multi, int32_t* n_multi, hipStream_t st) {
+  WGBM_REQUIRE(!a.sub_g || (a.sub_h && !a.accumulate && a.slots_per_node > 0),
+               "histogram: sibling subtraction needs both parent planes and overwrite mode");
   WGBM_REQUIRE(a.codes && a.perm && (a.gh || (a.g && a.h)) && a.seg_start && a.seg_count && a.seg_slot &&
                    a.seg_first && a.seg_pfirst && a.nseg_dev && a.out_g && a.out_h && a.part && a.stats && multi &&
                    n_multi,

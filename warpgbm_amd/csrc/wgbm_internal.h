@@ -50,6 +50,12 @@ struct HistArgs {
   int32_t accumulate;      // 1: out += result (reference op), 0: out = result
   int32_t lds_pitch;       // floats between consecutive (j, bin) rows of a plane
   int32_t planned;         // 1: the plan (hist_plan_job) already ran in an earlier fused launch
+  // Sibling by subtraction (level builds): with sub_g set, the result of
+  // built slot s (node c = s / slots_per_node, era e) is also subtracted from
+  // slot (c >> 1, e) of sub_g / sub_h (the previous level's histograms) into
+  // slot (c ^ 1, e) of out: the sibling = parent - built, float32.
+  const float* sub_g;
+  const float* sub_h;
 };
 
 // Configuration used by every histogram build: the automatic choice with
