@@ -443,7 +443,7 @@ k_hist_merge(const HistArgs a, const int32_t* __restrict__ multi, const int32_t*
     if (e < SL) {
       const int64_t off = ((int64_t)first * a.nslab + slab) * SL + e;
       const int64_t step = (int64_t)a.nslab * SL;
-      // Chunks k, k + 8, k + 16, k + 24 at once; a chunk past the last one
+      // Four chunks kMergeParts apart at once; a chunk past the last one
       // re-reads the last chunk and is dropped.
       if (one_plane) {
         const unsigned long long* p = reinterpret_cast<const unsigned long long*>(a.part) + off;

@@ -46,11 +46,25 @@ __device__ __forceinline__ void part_count_block(const PartArgs& a, int chunk) {
   const int bin = a.dec_bin[j];
   const int c0 = a.seg_start[s] + k * a.chunk_rows;
   const int c1 = min(c0 + a.chunk_rows, a.seg_start[s] + max(a.seg_count[s], 0));
+  // U positions per thread at once: their row ids, then their codes, are
+  // loaded together (two dependent round trips per U positions).
+  constexpr int U = 4;
   int cnt = 0;
-  for (int p = c0 + threadIdx.x; p < c1; p += blockDim.x) {
-    const bool l = goes_left(a, a.perm_in[p], col, bin);
-    a.flag[p] = (uint8_t)l;  // the scatter reads this instead of the code
-    cnt += l;
+  for (int p0 = c0 + threadIdx.x; p0 < c1; p0 += U * blockDim.x) {
+    int rows[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) rows[u] = a.perm_in[min(p0 + u * (int)blockDim.x, c1 - 1)];
+    bool left[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) left[u] = goes_left(a, rows[u], col, bin);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int p = p0 + u * (int)blockDim.x;
+      if (p < c1) {
+        a.flag[p] = (uint8_t)left[u];  // the scatter reads this instead of the code
+        cnt += left[u];
+      }
+    }
   }
   for (int o = 32; o >= 1; o >>= 1) cnt += __shfl_xor(cnt, o);
   __shared__ int s_c[4];
@@ -188,38 +202,49 @@ __device__ __forceinline__ void part_scatter_block(const PartArgs& a, int chunk)
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   int lrun = a.lbase[chunk], rrun = a.rbase[chunk];
   StatAcc sl, sr;  // statistics of the chunk's left and right rows
-  for (int base = c0; base < c1; base += 256) {
-    const int p = base + threadIdx.x;
-    const bool ok = p < c1;
-    int row = 0;
-    float2 v = make_float2(0.0f, 0.0f);
-    bool left = false;
-    if (ok) {
-      row = a.perm_in[p];
-      if (a.gh_in) v = a.gh_in[p];
-      left = a.flag[p] != 0;
+  // Passes of 256 positions, U at a time: the U passes' loads are issued
+  // together, then each pass ranks its rows and stores them.
+  constexpr int U = 4;
+  for (int base0 = c0; base0 < c1; base0 += U * 256) {
+    int rows[U];
+    float2 vs[U];
+    bool lefts[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int p = min(base0 + u * 256 + (int)threadIdx.x, c1 - 1);
+      rows[u] = a.perm_in[p];
+      vs[u] = a.gh_in ? a.gh_in[p] : make_float2(0.0f, 0.0f);
+      lefts[u] = a.flag[p] != 0;
     }
-    const uint64_t lm = __ballot(ok && left);
-    const uint64_t rm = __ballot(ok && !left);
-    if (lane == 0) s_l[wv] = __popcll(lm);
-    __syncthreads();
-    int lpre = 0, ltot = 0;
-    for (int w = 0; w < 4; ++w) { const int c = s_l[w]; if (w < wv) lpre += c; ltot += c; }
-    const int n_here = min(256, c1 - base);
-    const int wave_lo = min(64 * wv, n_here);           // positions before this wave
-    const int rpre = wave_lo - lpre;
-    if (ok) {
-      const int dst = left ? (lrun + lpre + lanes_below(lm)) : (rrun + rpre + lanes_below(rm));
-      a.perm_out[dst] = row;
-      if (a.gh_out) a.gh_out[dst] = v;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int base = base0 + u * 256;
+      if (base >= c1) break;  // uniform
+      const int p = base + threadIdx.x;
+      const bool ok = p < c1;
+      const int row = rows[u];
+      const float2 v = ok ? vs[u] : make_float2(0.0f, 0.0f);
+      const bool left = ok && lefts[u];
+      const uint64_t lm = __ballot(ok && left);
+      const uint64_t rm = __ballot(ok && !left);
+      if (lane == 0) s_l[wv] = __popcll(lm);
+      __syncthreads();
+      int lpre = 0, ltot = 0;
+      for (int w = 0; w < 4; ++w) { const int c = s_l[w]; if (w < wv) lpre += c; ltot += c; }
+      const int n_here = min(256, c1 - base);
+      const int wave_lo = min(64 * wv, n_here);           // positions before this wave
+      const int rpre = wave_lo - lpre;
+      if (ok) {
+        const int dst = left ? (lrun + lpre + lanes_below(lm)) : (rrun + rpre + lanes_below(rm));
+        a.perm_out[dst] = row;
+        if (a.gh_out) a.gh_out[dst] = v;
+        if (left) sl.add(v.x, v.y);
+        else sr.add(v.x, v.y);
+      }
+      lrun += ltot;
+      rrun += n_here - ltot;
+      __syncthreads();
     }
-    if (ok) {
-      if (left) sl.add(v.x, v.y);
-      else sr.add(v.x, v.y);
-    }
-    lrun += ltot;
-    rrun += n_here - ltot;
-    __syncthreads();
   }
   if (a.c_stats) {  // one partial per chunk and side, folded per child by k_child_stats
     sl.block_reduce();
