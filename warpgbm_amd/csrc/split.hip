@@ -140,70 +140,72 @@ k_select_split(const float* __restrict__ gain, const float* __restrict__ dir, in
 }
 
 // ---------------------------------------------------------------------------
-// Batched level scan.  One wave per (node, local feature), lane b owning
-// threshold b (and b + 64 when B > 65).  Each lane sums the bins in bin
-// order -- the totals over all B bins, the left sums over bins 0..b -- so
-// every float64 sum is the one the sequential per-feature loop of
-// k_compute_split forms, and the per-era float32 gains are bit-identical;
-// era sums run in era order as in eval_cand.  One candidate per (node,
-// workgroup of kScanFeat features) goes to `cand`; k_level_pick reduces them
-// (cand_better is a strict total order, so the reduction order is free).
-constexpr int kScanFeat = 4;   // features (waves) per workgroup
+// Batched level scan, one era.  A wave scans FPW features of a node (two
+// when a feature's B bins fit half a wave, else one): sub-lane b of a
+// feature owns threshold b (and b + LPF, ... when B - 1 > LPF).  Each lane
+// sums the bins in bin order -- the left sums over bins 0..b and the totals
+// over all B bins -- so every float64 sum is the one the sequential
+// per-feature loop of k_compute_split forms and the float32 gains are
+// bit-identical.  When all B bins fit the feature's lanes, the total is the
+// left sum of sub-lane B - 1 (the same bins added in the same order).  One
+// candidate per (node, workgroup of kScanFeat waves) goes to `cand`;
+// pick_node reduces them (cand_better is a strict total order, so the
+// reduction order is free).
+constexpr int kScanFeat = 4;   // waves per workgroup
+
+__host__ __device__ __forceinline__ int scan_fpw(int n_bins) { return n_bins <= 32 ? 2 : 1; }
+
+__device__ __forceinline__ double shfl_f64(double v, int src) {
+  const long long u = __builtin_bit_cast(long long, v);
+  const int lo = __shfl((int)(uint32_t)u, src), hi = __shfl((int)(uint32_t)(u >> 32), src);
+  return __builtin_bit_cast(double, ((long long)(uint32_t)hi << 32) | (uint32_t)lo);
+}
 
 __global__ void __launch_bounds__(64 * kScanFeat)
 k_level_scan(const LevelScanArgs a) {
-  const int wg_per_node = (a.n_feat + kScanFeat - 1) / kScanFeat;
+  const int B = a.n_bins;
+  const int FPW = scan_fpw(B);
+  const int LPF = kWave / FPW;  // lanes per feature
+  const int per_wg = kScanFeat * FPW;
+  const int wg_per_node = (a.n_feat + per_wg - 1) / per_wg;
   const int node = blockIdx.x / wg_per_node;
   const int wg = blockIdx.x - node * wg_per_node;
   const int lane = threadIdx.x & 63;
-  const int f = wg * kScanFeat + (threadIdx.x >> 6);
+  const int half = lane / LPF, sl = lane - half * LPF;
+  const int src0 = half * LPF;  // first lane of this lane's feature
+  const int f = wg * per_wg + (threadIdx.x >> 6) * FPW + half;
   Cand best = cand_none();
   const bool active = node < a.n_nodes && a.node_active[node] != 0;
-  if (active && f < a.n_feat) {
-    const int B = a.n_bins, E = a.n_eras;
-    const int req = a.mono_req ? (int)a.mono_req[a.feat_idx ? a.feat_idx[f] : f] : 0;
-    const int64_t node_off = (int64_t)node * E * a.n_feat * B;
-    for (int b0 = 0; b0 < B - 1; b0 += 64) {
-      const int b = b0 + lane;            // this lane's threshold
-      float sg = 0.0f;
-      int sd = 0;
-      bool mono = true;
-      float g1 = 0.0f, d1 = 0.0f;
-      for (int e = 0; e < E; ++e) {
-        const float* g = a.G + node_off + ((int64_t)e * a.n_feat + f) * B;
-        const float* h = a.H + node_off + ((int64_t)e * a.n_feat + f) * B;
-        double Gt = 0.0, Ht = 0.0, GL = 0.0, HL = 0.0;
-        // Lane i loads bin i0 + i (one coalesced load); every lane then sums
-        // the bins in bin order, taking bin q from lane q.
-        for (int i0 = 0; i0 < B; i0 += 64) {
-          const int iq = min(i0 + lane, B - 1);
-          const float gv = g[iq], hv = h[iq];
-          const int n = min(64, B - i0);
-          for (int q = 0; q < n; ++q) {
-            const double gi = (double)__builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, gv), q));
-            const double hi = (double)__builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, hv), q));
-            Gt += gi;
-            Ht += hi;
-            if (i0 + q <= b) { GL += gi; HL += hi; }
-          }
-        }
-        float ga, di;
-        gain_dir(GL, HL, Gt, Ht, (double)a.mcw, (double)a.lam, ga, di);
-        sg += ga;
-        sd += (int)di;
-        mono = mono && (req == 0 || di == (float)req);
-        g1 = ga; d1 = di;
+  // Every lane runs the loops (the shuffles read all lanes); lanes without a
+  // feature read feature 0 and offer no candidate.
+  const int fc = f < a.n_feat ? f : 0;
+  const int req = a.mono_req ? (int)a.mono_req[a.feat_idx ? a.feat_idx[fc] : fc] : 0;
+  const int64_t off = ((int64_t)(node < a.n_nodes ? node : 0) * a.n_feat + fc) * B;
+  const float* g = a.G + off;
+  const float* h = a.H + off;
+  const bool total_in_lane = B <= LPF;
+  for (int b0 = 0; b0 < B - 1; b0 += LPF) {
+    const int b = b0 + sl;  // this lane's threshold
+    double Gt = 0.0, Ht = 0.0, GL = 0.0, HL = 0.0;
+    for (int i0 = 0; i0 < B; i0 += LPF) {
+      const int iq = min(i0 + sl, B - 1);
+      const float gv = g[iq], hv = h[iq];
+      const int n = min(LPF, B - i0);
+      for (int q = 0; q < n; ++q) {
+        const double gi = (double)__shfl(gv, src0 + q), hi = (double)__shfl(hv, src0 + q);
+        if (!total_in_lane) { Gt += gi; Ht += hi; }
+        if (i0 + q <= b) { GL += gi; HL += hi; }
       }
-      if (b < B - 1) {
-        Cand k;
-        if (E == 1) {
-          k = Cand{0, (g1 > a.msg && (req == 0 || d1 == (float)req)) ? g1 : -INFINITY, f * (B - 1) + b};
-        } else {
-          const float cr = sg / (float)E;
-          k = Cand{sd < 0 ? -sd : sd, (cr > a.msg && mono) ? cr : -INFINITY, f * (B - 1) + b};
-        }
-        if (cand_better(k, best)) best = k;
-      }
+    }
+    if (total_in_lane) {
+      Gt = shfl_f64(GL, src0 + B - 1);
+      Ht = shfl_f64(HL, src0 + B - 1);
+    }
+    float ga, di;
+    gain_dir(GL, HL, Gt, Ht, (double)a.mcw, (double)a.lam, ga, di);
+    if (active && f < a.n_feat && b < B - 1) {
+      const Cand k{0, (ga > a.msg && (req == 0 || di == (float)req)) ? ga : -INFINITY, f * (B - 1) + b};
+      if (cand_better(k, best)) best = k;
     }
   }
   __shared__ Cand s_c[kScanFeat];
@@ -211,13 +213,13 @@ k_level_scan(const LevelScanArgs a) {
   if (lane == 0) s_c[threadIdx.x >> 6] = best;
   __syncthreads();
   if (threadIdx.x == 0 && node < a.n_nodes) {
-    Cand b = s_c[0];
+    Cand bb = s_c[0];
     for (int w = 1; w < kScanFeat; ++w)
-      if (cand_better(s_c[w], b)) b = s_c[w];
+      if (cand_better(s_c[w], bb)) bb = s_c[w];
     const int64_t o = (int64_t)node * wg_per_node + wg;
-    a.cand_agree[o] = b.agree;
-    a.cand_crit[o] = b.crit;
-    a.cand_idx[o] = b.idx;
+    a.cand_agree[o] = bb.agree;
+    a.cand_crit[o] = bb.crit;
+    a.cand_idx[o] = bb.idx;
   }
 }
 
@@ -252,16 +254,31 @@ __device__ void pick_node(const LevelScanArgs& a, int node, int wg_per_node, int
   }
   if (a.era_gain == nullptr) return;
   const int B = a.n_bins, E = a.n_eras;
+  if (E == 1) {
+    // One era: its gain is the winner's criterion (the same float64 sums in
+    // bin order, the same formula and rounding).
+    if (lane == 0) a.era_gain[node] = fl >= 0 ? b.crit : 0.0f;
+    return;
+  }
+  constexpr int U = 8;  // bins loaded at once
   for (int e = lane; e < E; e += 64) {
     float ga = 0.0f;
     if (fl >= 0) {
       const int64_t off = (((int64_t)node * E + e) * a.n_feat + fl) * B;
       const float* g = a.G + off;
       const float* h = a.H + off;
-      double Gt = 0.0, Ht = 0.0;
-      for (int q = 0; q < B; ++q) { Gt += (double)g[q]; Ht += (double)h[q]; }
-      double GL = 0.0, HL = 0.0;
-      for (int q = 0; q <= bin; ++q) { GL += (double)g[q]; HL += (double)h[q]; }
+      double Gt = 0.0, Ht = 0.0, GL = 0.0, HL = 0.0;
+      for (int q0 = 0; q0 < B; q0 += U) {
+        float gv[U], hv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) { const int q = min(q0 + u, B - 1); gv[u] = g[q]; hv[u] = h[q]; }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int q = q0 + u;
+          if (q < B) { Gt += (double)gv[u]; Ht += (double)hv[u]; }
+          if (q <= bin) { GL += (double)gv[u]; HL += (double)hv[u]; }
+        }
+      }
       float di;
       gain_dir(GL, HL, Gt, Ht, (double)a.mcw, (double)a.lam, ga, di);
     }
@@ -291,7 +308,7 @@ constexpr int kSelBlock = 256;  // (feature, threshold) candidates per workgroup
 
 int64_t level_scan_groups(int n_feat, int n_eras, int n_bins) {
   if (n_feat <= 0) return 1;
-  if (n_eras == 1) return ceil_div(n_feat, kScanFeat);
+  if (n_eras == 1) return ceil_div(n_feat, kScanFeat * scan_fpw(n_bins));
   return ceil_div((int64_t)n_feat * (n_bins - 1), kSelBlock);
 }
 
@@ -398,7 +415,7 @@ k_level_sel(const LevelScanArgs a, int node0, int n_group) {
 int level_scan_launch(const LevelScanArgs& a, hipStream_t st, const PlanJob* leaf, const PlanJob* part) {
   if (a.n_nodes == 0) return kOk;
   if (a.n_eras == 1) {
-    const int wg_per_node = (int)ceil_div(a.n_feat, kScanFeat);
+    const int wg_per_node = (int)level_scan_groups(a.n_feat, 1, a.n_bins);
     hipLaunchKernelGGL(k_level_scan, dim3((unsigned)(a.n_nodes * (int64_t)wg_per_node)), dim3(64 * kScanFeat), 0,
                        st, a);
     WGBM_LAUNCHED("k_level_scan");
