@@ -43,7 +43,11 @@ int wgbm_check_device(char* name_out, int32_t name_len);
  * caller zero-fills).  Deterministic: see the histogram kernel notes.
  * The reference's threads_per_block / rows_per_thread launch knobs are not
  * arguments: this engine's tiling comes from wgbm_set_hist_config. */
-size_t wgbm_histogram3_workspace_bytes(int64_t n_samples, int32_t n_feat, int32_t num_bins, int32_t n_eras);
+/* Workspace of wgbm_compute_histogram3 for n_samples listed rows, n_feat
+ * local features of a bin matrix with n_master_feat columns (the column
+ * count decides which tilings are possible). */
+size_t wgbm_histogram3_workspace_bytes(int64_t n_samples, int32_t n_feat, int32_t num_bins, int32_t n_eras,
+                                       int64_t n_master_feat);
 int wgbm_compute_histogram3(const int8_t* bin_indices, int64_t n_total_rows, int64_t n_master_feat,
                             const float* grads, const float* hess, const int32_t* sample_indices,
                             int64_t n_samples, const int32_t* feature_indices, int32_t n_feat,

@@ -52,8 +52,12 @@ def test_library_rejects_bad_arguments_without_a_device():
 def test_workspace_queries():
     from warpgbm_amd import _lib
     lib = _lib.load()
-    assert lib.wgbm_histogram3_workspace_bytes(1000, 50, 32, 1) > 0
-    assert lib.wgbm_histogram3_workspace_bytes(1000, 50, 32, 7) > lib.wgbm_histogram3_workspace_bytes(1000, 50, 32, 1)
+    assert lib.wgbm_histogram3_workspace_bytes(1000, 50, 32, 1, 50) > 0
+    assert (lib.wgbm_histogram3_workspace_bytes(1000, 50, 32, 7, 50)
+            > lib.wgbm_histogram3_workspace_bytes(1000, 50, 32, 1, 50))
+    # the bin matrix's column count decides the tiling, hence the workspace:
+    # a feature subset of a wide matrix may need more than n_feat columns would
+    assert lib.wgbm_histogram3_workspace_bytes(1000, 150, 8, 2, 300) > 0
     c = _lib.FitConfig(n_rows=1_000_000, n_cols=1000, n_feat=1000, n_bins=32, n_eras=1, n_classes=1, max_depth=5,
                        objective=0, learning_rate=0.01, min_split_gain=0.0, min_child_weight=20.0, lambda_=1e-6)
     ws = lib.wgbm_fit_workspace_bytes(ctypes.byref(c))

@@ -58,6 +58,32 @@ def test_histogram_matches_oracle(dev, N, F, B, E, subset):
     np.testing.assert_allclose(Gg.sum(axis=(0, 2)), g[rows].astype(np.float64).sum(), rtol=1e-4, atol=1e-2)
 
 
+@pytest.mark.parametrize("cfg", [(0, 8, 1, 0), (0, 16, 1, 0), (0, 8, 2, 0), (0, 16, 2, 0), (0, 8, 4, 0),
+                                 (0, 16, 4, 0), (0, 4, 4, 16), (0, 8, 8, 0), (0, 8, 16, 0), (1024, 16, 4, 0)])
+@pytest.mark.parametrize("subset", [False, True])
+def test_histogram_every_tiling_matches_oracle(dev, cfg, subset):
+    """Every (chunk, waves, features per lane, depth) the autotuner can
+    install builds the same histogram, for identity columns and for a
+    feature subset (byte gathers)."""
+    from warpgbm_amd import _lib
+    rng = np.random.default_rng(17)
+    N, F, B, E = 20000, 300, 8, 2
+    codes = rng.integers(0, B, size=(N, F), dtype=np.int8)
+    g = rng.standard_normal(N).astype(np.float32)
+    h = rng.uniform(0.5, 1.5, N).astype(np.float32)
+    eras = rng.integers(0, E, N).astype(np.int32)
+    rows = np.sort(rng.choice(N, size=N * 3 // 4, replace=False)).astype(np.int32)
+    feats = rng.permutation(F)[: F // 2] if subset else np.arange(F)
+    _lib.call("wgbm_set_hist_config", *cfg)
+    try:
+        Gg, Hg = _hist_gpu(dev, codes, g, h, rows, feats, eras, E, B)
+    finally:
+        _lib.call("wgbm_set_hist_config", 0, 0, 0, 0)
+    Go, Ho = O.histogram(codes, g, h, rows, feats, eras, E, B)
+    assert _rel_err(Gg, Go, np.abs(g[rows]).sum()) < 1e-5
+    assert _rel_err(Hg, Ho, np.abs(h[rows]).sum()) < 1e-5
+
+
 def test_histogram_counts_exact_and_drops_out_of_range(dev):
     rng = np.random.default_rng(3)
     N, F, B = 10000, 20, 16

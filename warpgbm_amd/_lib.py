@@ -43,7 +43,7 @@ SIGNATURES = {
     "wgbm_abi_version": (C.c_int, []),
     "wgbm_build_arch": (C.c_char_p, []),
     "wgbm_check_device": (C.c_int, [C.c_char_p, i32]),
-    "wgbm_histogram3_workspace_bytes": (sz, [i64, i32, i32, i32]),
+    "wgbm_histogram3_workspace_bytes": (sz, [i64, i32, i32, i32, i64]),
     "wgbm_compute_histogram3": (C.c_int, [vp, i64, i64, vp, vp, vp, i64, vp, i32, vp, vp, vp, i32, i32, vp, sz, vp]),
     "wgbm_compute_split": (C.c_int, [vp, vp, i32, i32, i32, f32, f32, f32, vp, vp, vp]),
     "wgbm_bin_column": (C.c_int, [vp, i64, vp, i32, vp, vp]),

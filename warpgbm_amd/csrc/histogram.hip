@@ -148,9 +148,7 @@ k_hist_build(const HistArgs a) {
   const int lane = lane_id();
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int W = blockDim.x >> 6;
-  const int nseg = *a.nseg_dev;
-  const int total = a.seg_first[nseg];
-  const int C = a.chunk_dev ? *a.chunk_dev : a.chunk_rows;  // rows per chunk of this launch
+  const int total = *a.chunk_total;  // chunks of this launch (the plan's count)
   // Work items b = (chunk, slab), taken round-robin by a grid of at most
   // one workgroup per slot (gridDim.x a multiple of 8).  XCD-aware mapping:
   // dispatch sends block b to XCD b % 8, and b % 8 is fixed per workgroup,
@@ -162,18 +160,14 @@ k_hist_build(const HistArgs a) {
   const int slab = i % a.nslab;
   const int chunk = ((i / a.nslab) << 3) + (b & 7);
   if (chunk >= total) break;
-  const int s = upper_index(a.seg_first, nseg, chunk);
-  const int k = chunk - a.seg_first[s];
-  const int nch = a.seg_first[s + 1] - a.seg_first[s];
-  const int seg_start = a.seg_start[s];
-  const int seg_end = seg_start + max(a.seg_count[s], 0);
-  const int c0 = seg_start + k * C;
-  const int c1 = min(c0 + C, seg_end);
+  // The plan's descriptor of the chunk: positions [c0, c1), output slot and
+  // partial index (-1: the segment has this one chunk; write the result).
+  const int4 dsc = a.chunk_desc[chunk];
+  const int c0 = dsc.x, c1 = dsc.y, slot = dsc.z, pidx = dsc.w;
   const int len = max(c1 - c0, 0);
   const int per = (len + W - 1) / W;
   const int w0 = min(c0 + wave * per, c1);
   const int w1 = min(w0 + per, c1);
-  const int slot = a.seg_slot[s];
   const NodeShift ns = node_shift(a.stats + (int64_t)(slot / a.slots_per_node) * kStatWords);
   if (a.dual && ONE_PLANE != ns.count_mode) continue;  // the other variant builds this node
 
@@ -316,22 +310,25 @@ k_hist_build(const HistArgs a) {
           add_row(V[q], qp, qh);
         }
       };
-      int rown = -1;
-#pragma unroll
-      for (int gq = 0; gq < NG; gq += 2) {
+      // Pairs of groups; the loop stays rolled (a kernel that fits the
+      // instruction cache), the last pair is peeled: its second load fetches
+      // the next block's first group.
+#pragma unroll 1
+      for (int gq = 0; gq < NG - 2; gq += 2) {
 #pragma unroll
         for (int q = 0; q < LG; ++q) Bv[q] = load_codes(__builtin_amdgcn_readlane(rowv, (gq + 1) * LG + q));
         add_group(A, gq * LG);
-        if (gq + 2 < NG) {
 #pragma unroll
-          for (int q = 0; q < LG; ++q) A[q] = load_codes(__builtin_amdgcn_readlane(rowv, (gq + 2) * LG + q));
-        } else {
-          rown = valid_row(pn, rown_raw);
-#pragma unroll
-          for (int q = 0; q < LG; ++q) A[q] = load_codes(__builtin_amdgcn_readlane(rown, q));
-        }
+        for (int q = 0; q < LG; ++q) A[q] = load_codes(__builtin_amdgcn_readlane(rowv, (gq + 2) * LG + q));
         add_group(Bv, (gq + 1) * LG);
       }
+#pragma unroll
+      for (int q = 0; q < LG; ++q) Bv[q] = load_codes(__builtin_amdgcn_readlane(rowv, (NG - 1) * LG + q));
+      add_group(A, (NG - 2) * LG);
+      const int rown = valid_row(pn, rown_raw);
+#pragma unroll
+      for (int q = 0; q < LG; ++q) A[q] = load_codes(__builtin_amdgcn_readlane(rown, q));
+      add_group(Bv, (NG - 1) * LG);
       rowv = rown;
       block_words(rowv, ghn);
     }
@@ -348,7 +345,7 @@ k_hist_build(const HistArgs a) {
   // consecutive words); k_hist_merge maps words to (feature, bin).
   const int SL = kWave * FPL * B;
   const int f_base = slab * kWave * FPL;
-  const bool direct = (nch == 1);
+  const bool direct = pidx < 0;
   constexpr int kFlushBatch = 4;  // words per thread whose sibling loads are in flight together
   for (int t0 = threadIdx.x; t0 < SL; t0 += kFlushBatch * blockDim.x) {
     SibBin sb[kFlushBatch];
@@ -380,7 +377,7 @@ k_hist_build(const HistArgs a) {
       if (direct) {
         store_bin(a, slot, f_base + fl, bin, from_fixed(gsum, ns.sg), from_fixed(hsum, ns.sh), sb[u]);
       } else {
-        const int64_t d = ((int64_t)(a.seg_pfirst[s] + k) * a.nslab + slab) * SL + t;
+        const int64_t d = ((int64_t)pidx * a.nslab + slab) * SL + t;
         if constexpr (ONE_PLANE) reinterpret_cast<unsigned long long*>(a.part)[d] = w;  // packed count | qg sum
         else a.part[d] = make_longlong2(gsum, hsum);
       }
@@ -835,6 +832,10 @@ PlanJob hist_plan_job(const HistArgs& a, const HistConfig& c, int32_t* multi, in
   j.src.chunk_out = a.chunk_dev;
   j.src.chunk_min = kChunkMin < c.chunk_rows ? kChunkMin : c.chunk_rows;
   j.src.chunk_target = (int32_t)hist_chunk_target(c, a.n_feat);
+  j.src.seg_start = a.seg_start;
+  j.src.seg_slot = a.seg_slot;
+  j.src.chunk_desc = a.chunk_desc;
+  j.src.chunk_total = a.chunk_total;
   j.chunk_rows = c.chunk_rows;
   j.seg_first = a.seg_first;
   j.seg_pfirst = a.seg_pfirst;
@@ -847,6 +848,7 @@ int hist_launch(HistArgs a, const HistConfig& c, int64_t rows_max, int64_t nseg_
                 int32_t* multi, int32_t* n_multi, hipStream_t st) {
   WGBM_REQUIRE(!a.sub_g || (a.sub_h && !a.accumulate && a.slots_per_node > 0),
                "histogram: sibling subtraction needs both parent planes and overwrite mode");
+  WGBM_REQUIRE(a.chunk_desc && a.chunk_total, "histogram: missing chunk descriptor scratch");
   WGBM_REQUIRE(a.codes && a.perm && (a.gh || (a.g && a.h)) && a.seg_start && a.seg_count && a.seg_slot &&
                    a.seg_first && a.seg_pfirst && a.nseg_dev && a.out_g && a.out_h && a.part && a.stats && multi &&
                    n_multi,
@@ -919,8 +921,8 @@ extern "C" int wgbm_profile_end(double* out4) {
 
 // Workspace carve-up for the reference-style histogram op.
 struct Hist3Layout {
-  size_t seg_start, seg_count, seg_slot, seg_first, seg_pfirst, nseg, multi, n_multi, chunk, sorted, cnt, stats,
-      stat_part, part, total;
+  size_t seg_start, seg_count, seg_slot, seg_first, seg_pfirst, nseg, multi, n_multi, chunk, desc, total_chunks,
+      sorted, cnt, stats, stat_part, part, total;
 };
 
 static Hist3Layout hist3_layout(int64_t n, int n_feat, int n_bins, int n_eras, const HistConfig& c) {
@@ -938,6 +940,8 @@ static Hist3Layout hist3_layout(int64_t n, int n_feat, int n_bins, int n_eras, c
   L.multi = o;     o = al(o + 4 * max_multi);
   L.chunk = o;     o = al(o + 4);
   L.n_multi = o;   o = al(o + 4);
+  L.desc = o;      o = al(o + sizeof(int4) * hist_max_chunks(c, n_feat, n, E));
+  L.total_chunks = o; o = al(o + 4);
   const int64_t nb = ceil_div(n, kSortBlock);
   L.sorted = o;    o = al(o + (E > 1 ? 4 * n : 0));
   L.cnt = o;       o = al(o + (E > 1 ? 4 * nb * E : 0));
@@ -949,9 +953,10 @@ static Hist3Layout hist3_layout(int64_t n, int n_feat, int n_bins, int n_eras, c
 }
 
 extern "C" size_t wgbm_histogram3_workspace_bytes(int64_t n_rows, int32_t n_feat, int32_t n_bins,
-                                                  int32_t n_eras) {
-  if (n_rows < 0 || n_feat < 0 || n_bins < 1 || n_eras < 1) return 0;
-  const HistConfig c = pick_hist_config(n_feat, n_bins, n_feat, false);
+                                                  int32_t n_eras, int64_t n_master_feat) {
+  if (n_rows < 0 || n_feat < 0 || n_bins < 1 || n_eras < 1 || n_master_feat < 0) return 0;
+  // the same configuration wgbm_compute_histogram3 picks for this matrix
+  const HistConfig c = pick_hist_config(n_feat, n_bins, n_master_feat, false);
   return hist3_layout(n_rows, n_feat, n_bins, n_eras, c).total;
 }
 
@@ -992,6 +997,8 @@ extern "C" int wgbm_compute_histogram3(const int8_t* bin_indices, int64_t n_tota
   a.seg_pfirst = P(L.seg_pfirst);
   a.nseg_dev = P(L.nseg);
   a.chunk_dev = P(L.chunk);
+  a.chunk_desc = reinterpret_cast<int4*>(ws + L.desc);
+  a.chunk_total = P(L.total_chunks);
   a.out_g = grad_hist;
   a.out_h = hess_hist;
   a.part = reinterpret_cast<longlong2*>(ws + L.part);

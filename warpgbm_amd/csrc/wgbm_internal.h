@@ -37,6 +37,9 @@ struct HistArgs {
   int32_t* seg_pfirst;     // [nseg] first partial slot of multi-chunk segments
   const int32_t* nseg_dev; // device scalar: number of segments
   int32_t* chunk_dev;      // device scalar: rows per chunk, written by the plan
+  int4* chunk_desc;        // [max chunks] per chunk {first position, end position, slot, partial
+                           // index or -1 (single-chunk segment)}, written by the plan
+  int32_t* chunk_total;    // device scalar: chunks of the launch, written by the plan
   int32_t chunk_rows;
   int32_t nslab;
   int32_t wide_ok;
@@ -132,6 +135,13 @@ struct PlanSrc {
   // chunks and stores it there.
   int32_t* chunk_out;
   int32_t chunk_min, chunk_target;
+  // Chunk descriptors (histogram plans): with chunk_desc set, the plan also
+  // writes every chunk's {first position, end position, slot, partial index
+  // or -1} from seg_start / seg_slot, and the chunk count to chunk_total.
+  const int32_t* seg_start;
+  const int32_t* seg_slot;
+  int4* chunk_desc;
+  int32_t* chunk_total;
 };
 void plan_launch(const PlanSrc& src, int chunk_rows, int32_t* seg_first, int32_t* multi, int32_t* n_multi,
                  hipStream_t st);

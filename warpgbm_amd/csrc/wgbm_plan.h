@@ -68,9 +68,20 @@ __device__ __forceinline__ void plan_body(const PlanSrc& src, int32_t chunk_rows
     for (int k = 0; k < wv; ++k) { pre[0] += s_sum[0][k]; pre[1] += s_sum[1][k]; pre[2] += s_sum[2][k]; }
     const int c0 = s_carry[0], c1 = s_carry[1], c2 = s_carry[2];
     if (s < nseg) {
-      seg_first[s] = c0 + pre[0] + x[0] - v[0];
-      if (seg_pfirst) seg_pfirst[s] = c2 + pre[2] + x[2] - v[2];
+      const int first = c0 + pre[0] + x[0] - v[0];
+      const int pfirst = c2 + pre[2] + x[2] - v[2];
+      seg_first[s] = first;
+      if (seg_pfirst) seg_pfirst[s] = pfirst;
       if (v[1] && multi) multi[c1 + pre[1] + x[1] - 1] = s;
+      if (src.chunk_desc && v[0] > 0) {
+        const int start = src.seg_start[s];
+        const int end = start + max(plan_count(src, s), 0);
+        const int slot = src.seg_slot[s];
+        for (int k = 0; k < v[0]; ++k) {
+          const int a0 = start + k * chunk_rows;
+          src.chunk_desc[first + k] = make_int4(a0, min(a0 + chunk_rows, end), slot, v[1] ? pfirst + k : -1);
+        }
+      }
     }
     __syncthreads();
     if (t == 1023) {
@@ -85,6 +96,7 @@ __device__ __forceinline__ void plan_body(const PlanSrc& src, int32_t chunk_rows
   if (t == 0) {
     seg_first[nseg] = s_carry[0];
     if (n_multi) *n_multi = s_carry[1];
+    if (src.chunk_total) *src.chunk_total = s_carry[0];
   }
 }
 

@@ -64,7 +64,7 @@ def compute_histogram3(bin_indices, grads, hess, sample_indices, feature_indices
         raise RuntimeError("grads, hess and era_indices need one value per row of bin_indices")
     n = sample_indices.numel()
     ws = _workspace(bin_indices.device,
-                    _lib.query("wgbm_histogram3_workspace_bytes", n, n_feat, int(num_bins), n_eras))
+                    _lib.query("wgbm_histogram3_workspace_bytes", n, n_feat, int(num_bins), n_eras, n_cols))
     _lib.call("wgbm_compute_histogram3", bin_indices.data_ptr(), n_rows, n_cols, grads.data_ptr(),
               hess.data_ptr(), sample_indices.data_ptr(), n, feature_indices.data_ptr(), n_feat,
               era_indices.data_ptr(), grad_hist.data_ptr(), hess_hist.data_ptr(), n_eras, int(num_bins),
