@@ -21,6 +21,7 @@
 #include "wgbm_internal.h"
 #include "wgbm_level.h"
 #include "wgbm_stats.h"
+#include "wgbm_plan.h"
 #include "../../include/warpgbm_amd.h"
 
 namespace wgbm {
@@ -144,41 +145,51 @@ static FitLayout fit_layout(const wgbm_fit_config& c) {
 // subset, and write the root tables (class c occupies positions
 // [c*N, (c+1)*N) in root order).  The round index lives on the device so
 // that one captured round can be replayed for every round.
-__global__ void k_root_tables(const int32_t* __restrict__ era_count, int E, int K, int N,
-                              int32_t* era_start, int32_t* seg_start, int32_t* seg_count, int32_t* seg_slot,
-                              int32_t* nseg, int32_t* t_seg_start, int32_t* t_seg_count, int32_t* t_active,
-                              int32_t* t_node_start, int32_t* t_node_count, int32_t* round_ctr,
-                              const int32_t* __restrict__ feat_all, int n_feat, int32_t* fidx_cur) {
-  const int r = round_ctr[1];
-  if (feat_all)
-    for (int i = threadIdx.x; i < n_feat; i += blockDim.x) fidx_cur[i] = feat_all[(int64_t)r * n_feat + i];
+struct RootArgs {
+  const int32_t* era_count;
+  int E, K, N;
+  int32_t *era_start, *seg_start, *seg_count, *seg_slot, *nseg;
+  int32_t *t_seg_start, *t_seg_count, *t_active, *t_node_start, *t_node_count;
+  int32_t* round_ctr;
+  const int32_t* feat_all;
+  int n_feat;
+  int32_t* fidx_cur;
+};
+
+// One workgroup: the round's root segment tables, level-0 node tables and
+// feature subset; advances the device round counter.
+__device__ __forceinline__ void root_tables_block(const RootArgs& a) {
+  const int E = a.E, K = a.K, N = a.N;
+  const int r = a.round_ctr[1];
+  if (a.feat_all)
+    for (int i = threadIdx.x; i < a.n_feat; i += blockDim.x) a.fidx_cur[i] = a.feat_all[(int64_t)r * a.n_feat + i];
   int run = 0;  // era starts: exclusive scan of the era counts
   for (int base = 0; base < E; base += blockDim.x) {
     const int e = base + threadIdx.x;
     int tot;
-    const int ex = block_exscan(e < E ? era_count[e] : 0, &tot);  // syncs: every thread has read round_ctr[1]
-    if (e < E) era_start[e] = run + ex;
+    const int ex = block_exscan(e < E ? a.era_count[e] : 0, &tot);  // syncs: every thread has read round_ctr[1]
+    if (e < E) a.era_start[e] = run + ex;
     run += tot;
   }
   __syncthreads();
   for (int s = threadIdx.x; s < K * E; s += blockDim.x) {
     const int c = s / E, e = s - c * E;
-    const int st = c * N + era_start[e];
-    seg_start[s] = st;
-    seg_count[s] = era_count[e];
-    seg_slot[s] = s;
-    t_seg_start[s] = st;
-    t_seg_count[s] = era_count[e];
+    const int st = c * N + a.era_start[e];
+    a.seg_start[s] = st;
+    a.seg_count[s] = a.era_count[e];
+    a.seg_slot[s] = s;
+    a.t_seg_start[s] = st;
+    a.t_seg_count[s] = a.era_count[e];
   }
   for (int c = threadIdx.x; c < K; c += blockDim.x) {
-    t_active[c] = 1;
-    t_node_start[c] = c * N;
-    t_node_count[c] = run;
+    a.t_active[c] = 1;
+    a.t_node_start[c] = c * N;
+    a.t_node_count[c] = run;
   }
   if (threadIdx.x == 0) {
-    round_ctr[0] = r;
-    round_ctr[1] = r + 1;
-    *nseg = K * E;
+    a.round_ctr[0] = r;
+    a.round_ctr[1] = r + 1;
+    *a.nseg = K * E;
   }
 }
 
@@ -250,12 +261,12 @@ k_root_perm(const int32_t* __restrict__ order, int64_t N, int K, int32_t* __rest
 
 // Regression gradients in root position order, with one statistics partial
 // per block (grid-stride over positions).
-__global__ void __launch_bounds__(256)
-k_grad_regression_pos(const float* __restrict__ y, const float* __restrict__ pred, const float* __restrict__ w,
-                      int64_t n, const int32_t* __restrict__ perm0, float2* __restrict__ gh,
-                      int32_t* __restrict__ part) {
+__device__ __forceinline__ void grad_regression_block(const float* __restrict__ y, const float* __restrict__ pred,
+                                                      const float* __restrict__ w, int64_t n,
+                                                      const int32_t* __restrict__ perm0, float2* __restrict__ gh,
+                                                      int32_t* __restrict__ part, int blk, int nblk) {
   StatAcc acc;
-  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
+  for (int64_t p = (int64_t)blk * blockDim.x + threadIdx.x; p < n; p += (int64_t)nblk * blockDim.x) {
     const int64_t r = perm0[p];
     const float h = w[r];
     const float g = (y[r] - pred[r]) * h;
@@ -263,12 +274,29 @@ k_grad_regression_pos(const float* __restrict__ y, const float* __restrict__ pre
     acc.add(g, h);
   }
   acc.block_reduce();
-  if (threadIdx.x == 0) acc.store(part + (int64_t)blockIdx.x * kStatWords);
+  if (threadIdx.x == 0) acc.store(part + (int64_t)blk * kStatWords);
 }
 
+// Round start, regression: gradient blocks [0, nblk) and the root tables
+// (last workgroup) in one launch.
 __global__ void __launch_bounds__(256)
-k_grad_softmax_pos(const float* __restrict__ F, const int32_t* __restrict__ y, const float* __restrict__ w,
-                   int64_t n, int K, const int32_t* __restrict__ perm0, float2* __restrict__ gh) {
+k_round_start_regression(const float* __restrict__ y, const float* __restrict__ pred, const float* __restrict__ w,
+                         int64_t n, const int32_t* __restrict__ perm0, float2* __restrict__ gh,
+                         int32_t* __restrict__ part, int nblk, const RootArgs ra) {
+  if ((int)blockIdx.x == nblk) root_tables_block(ra);
+  else grad_regression_block(y, pred, w, n, perm0, gh, part, blockIdx.x, nblk);
+}
+
+// Round start, multiclass: softmax gradients (blocks [0, nblk)) and the root
+// tables (last workgroup).
+__global__ void __launch_bounds__(256)
+k_round_start_softmax(const float* __restrict__ F, const int32_t* __restrict__ y, const float* __restrict__ w,
+                      int64_t n, int K, const int32_t* __restrict__ perm0, float2* __restrict__ gh, int nblk,
+                      const RootArgs ra) {
+  if ((int)blockIdx.x == nblk) {
+    root_tables_block(ra);
+    return;
+  }
   const int64_t p0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const bool ok = p0 < n;
   const int64_t p = ok ? p0 : n - 1;
@@ -288,6 +316,21 @@ k_grad_softmax_pos(const float* __restrict__ F, const int32_t* __restrict__ y, c
   }
 }
 
+
+// The root statistics of every class (workgroups [0, K)) and the root
+// histogram plan (workgroup K) in one launch of 1024-thread workgroups.
+__global__ void __launch_bounds__(1024)
+k_root_stats_plan(const int32_t* __restrict__ part, int n_part, int32_t* __restrict__ out, int K, const PlanJob j) {
+  if ((int)blockIdx.x == K) {
+    plan_body(j.src, j.chunk_rows, j.seg_first, j.seg_pfirst, j.multi, j.n_multi);
+    return;
+  }
+  const int32_t* p = part + (int64_t)blockIdx.x * n_part * kStatWords;
+  StatAcc acc;
+  for (int i = threadIdx.x; i < n_part; i += blockDim.x) acc.load_merge(p + (int64_t)i * kStatWords);
+  acc.block_reduce();
+  if (threadIdx.x == 0) acc.store(out + (int64_t)blockIdx.x * kStatWords);
+}
 
 }  // namespace wgbm
 
@@ -352,28 +395,60 @@ static int enqueue_round(const wgbm_fit_config& c, const FitLayout& L, const wgb
   const int K = c.n_classes, E = c.n_eras, D = c.max_depth;
   const int64_t N = c.n_rows;
   const int32_t* fidx = io->feat_idx ? I(L.fidx_cur) : nullptr;
-    // level-0 tables (rewritten every round: partition overwrites table 0 at level 1)
-    hipLaunchKernelGGL(k_root_tables, dim3(1), dim3(256), 0, st, io->era_count, E, K, (int)N, I(L.era_start),
-                       I(L.root_seg_start), I(L.root_seg_count), I(L.root_seg_slot), I(L.root_nseg),
-                       I(L.tab[0].seg_start), I(L.tab[0].seg_count), I(L.tab[0].node_active),
-                       I(L.tab[0].node_start), I(L.tab[0].node_count), I(L.round_ctr), io->feat_idx, c.n_feat,
-                       I(L.fidx_cur));
-    WGBM_LAUNCHED("k_root_tables");
+    // Round start in two launches: gradients with the level-0 tables
+    // (rewritten every round: partition overwrites table 0 at level 1),
+    // then the root statistics with the root histogram plan.
+    RootArgs ra{};
+    ra.era_count = io->era_count;
+    ra.E = E;
+    ra.K = K;
+    ra.N = (int)N;
+    ra.era_start = I(L.era_start);
+    ra.seg_start = I(L.root_seg_start);
+    ra.seg_count = I(L.root_seg_count);
+    ra.seg_slot = I(L.root_seg_slot);
+    ra.nseg = I(L.root_nseg);
+    ra.t_seg_start = I(L.tab[0].seg_start);
+    ra.t_seg_count = I(L.tab[0].seg_count);
+    ra.t_active = I(L.tab[0].node_active);
+    ra.t_node_start = I(L.tab[0].node_start);
+    ra.t_node_count = I(L.tab[0].node_count);
+    ra.round_ctr = I(L.round_ctr);
+    ra.feat_all = io->feat_idx;
+    ra.n_feat = c.n_feat;
+    ra.fidx_cur = I(L.fidx_cur);
     const int nsb = stats_blocks(N);
     if (c.objective == 0) {
-      hipLaunchKernelGGL(k_grad_regression_pos, dim3((unsigned)nsb), dim3(256), 0, st, io->y, io->scores, io->weight,
-                         N, I(L.perm_root), G2(L.gh_root), I(L.stat_part));
-      WGBM_LAUNCHED("k_grad_regression_pos");
+      hipLaunchKernelGGL(k_round_start_regression, dim3((unsigned)(nsb + 1)), dim3(256), 0, st, io->y, io->scores,
+                         io->weight, N, I(L.perm_root), G2(L.gh_root), I(L.stat_part), nsb, ra);
+      WGBM_LAUNCHED("k_round_start_regression");
     } else {
-      hipLaunchKernelGGL(k_grad_softmax_pos, dim3((unsigned)ceil_div(N, 256)), dim3(256), 0, st, io->scores,
-                         io->labels, io->weight, N, K, I(L.perm_root), G2(L.gh_root));
-      WGBM_LAUNCHED("k_grad_softmax_pos");
+      const int gb = (int)ceil_div(N, 256);
+      hipLaunchKernelGGL(k_round_start_softmax, dim3((unsigned)(gb + 1)), dim3(256), 0, st, io->scores, io->labels,
+                         io->weight, N, K, I(L.perm_root), G2(L.gh_root), gb, ra);
+      WGBM_LAUNCHED("k_round_start_softmax");
       hipLaunchKernelGGL(k_gh_stats, dim3((unsigned)nsb, (unsigned)K), dim3(256), 0, st, G2(L.gh_root), N,
                          I(L.stat_part));
       WGBM_LAUNCHED("k_gh_stats");
     }
-    hipLaunchKernelGGL(k_stats_reduce, dim3((unsigned)K), dim3(256), 0, st, I(L.stat_part), nsb, I(L.tab[0].stats));
-    WGBM_LAUNCHED("k_stats_reduce");
+    PlanJob root_plan{};
+    {
+      HistArgs ra_h{};
+      ra_h.n_feat = c.n_feat;
+      ra_h.seg_count = I(L.root_seg_count);
+      ra_h.seg_start = I(L.root_seg_start);
+      ra_h.seg_slot = I(L.root_seg_slot);
+      ra_h.nseg_dev = I(L.root_nseg);
+      ra_h.chunk_dev = I(L.h_chunk);
+      ra_h.chunk_desc = reinterpret_cast<int4*>(ws + L.h_desc);
+      ra_h.chunk_total = I(L.h_total);
+      ra_h.seg_first = I(L.h_seg_first);
+      ra_h.seg_pfirst = I(L.h_seg_pfirst);
+      root_plan = hist_plan_job(ra_h, L.hc, I(L.h_multi), I(L.h_n_multi));
+    }
+    hipLaunchKernelGGL(k_root_stats_plan, dim3((unsigned)(K + (D > 0 ? 1 : 0))), dim3(1024), 0, st, I(L.stat_part), nsb,
+                       I(L.tab[0].stats), K, root_plan);
+    WGBM_LAUNCHED("k_root_stats_plan");
     for (int d = 0; d <= D; ++d) {
       const int n_nodes = K << d;
       const FitLayout::Tables& cur = L.tab[d & 1];
@@ -475,6 +550,7 @@ static int enqueue_round(const wgbm_fit_config& c, const FitLayout& L, const wgb
           a.seg_count = I(L.root_seg_count);
           a.seg_slot = I(L.root_seg_slot);
           a.nseg_dev = I(L.root_nseg);
+          a.planned = 1;  // planned beside the root statistics
           rc = hist_launch(a, L.hc, L.KN, (int64_t)K * E, I(L.h_multi), I(L.h_n_multi), st);
         } else {
           a.seg_start = I(L.b_seg_start);

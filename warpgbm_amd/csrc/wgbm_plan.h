@@ -73,14 +73,39 @@ __device__ __forceinline__ void plan_body(const PlanSrc& src, int32_t chunk_rows
       seg_first[s] = first;
       if (seg_pfirst) seg_pfirst[s] = pfirst;
       if (v[1] && multi) multi[c1 + pre[1] + x[1] - 1] = s;
-      if (src.chunk_desc && v[0] > 0) {
+    }
+    if (src.chunk_desc) {
+      // Chunk descriptors of this block of segments, all threads at once:
+      // the segments' tables go to LDS, then thread t takes chunks t,
+      // t + 1024, ... and finds its segment by binary search (the last
+      // segment whose first chunk is <= c; a segment without chunks has the
+      // next one's first chunk).
+      __shared__ int s_first[1024], s_start[1024], s_end[1024], s_slot[1024], s_pf[1024];
+      if (s < nseg) {
         const int start = src.seg_start[s];
-        const int end = start + max(plan_count(src, s), 0);
-        const int slot = src.seg_slot[s];
-        for (int k = 0; k < v[0]; ++k) {
-          const int a0 = start + k * chunk_rows;
-          src.chunk_desc[first + k] = make_int4(a0, min(a0 + chunk_rows, end), slot, v[1] ? pfirst + k : -1);
+        s_first[t] = c0 + pre[0] + x[0] - v[0];
+        s_start[t] = start;
+        s_end[t] = start + max(plan_count(src, s), 0);
+        s_slot[t] = src.seg_slot[s];
+        s_pf[t] = v[1] ? c2 + pre[2] + x[2] - v[2] : -1;
+      } else {
+        s_first[t] = 0x7fffffff;
+      }
+      __syncthreads();
+      int blk_chunks = 0;
+      for (int k = 0; k < 16; ++k) blk_chunks += s_sum[0][k];
+      const int nb = min(1024, nseg - base);
+      for (int c = c0 + t; c < c0 + blk_chunks; c += 1024) {
+        int lo = 0, hi = nb;  // count of segments with first <= c
+        while (lo < hi) {
+          const int mid = (lo + hi) >> 1;
+          if (s_first[mid] <= c) lo = mid + 1;
+          else hi = mid;
         }
+        const int j = lo - 1;
+        const int k = c - s_first[j];
+        const int a0 = s_start[j] + k * chunk_rows;
+        src.chunk_desc[c] = make_int4(a0, min(a0 + chunk_rows, s_end[j]), s_slot[j], s_pf[j] >= 0 ? s_pf[j] + k : -1);
       }
     }
     __syncthreads();
