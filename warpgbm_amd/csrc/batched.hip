@@ -35,7 +35,8 @@ struct FitLayout {
   size_t dec_feat, dec_bin, era_gain, cand_agree, cand_crit, cand_idx, era_g, era_d;
   int64_t gain_nodes;
   size_t b_seg_start, b_seg_count, b_seg_slot, n_build, sib, n_split;
-  size_t h_seg_first, h_seg_pfirst, h_nseg, h_multi, h_n_multi, h_chunk, h_part, h_desc, h_total;
+  size_t h_seg_first, h_seg_pfirst, h_nseg, h_multi, h_n_multi, h_chunk, h_part, h_desc, h_total, h_mdesc,
+      h_mparts;
   size_t stat_part, round_ctr, fidx_cur;
   size_t p_flag, p_seg_first, p_nseg, p_multi, p_n_multi, p_lbase, p_rbase, p_cscan, p_segL, p_segP, codes_t;
   size_t l_seg_first, l_nseg, l_multi, l_n_multi, l_part, l_val;
@@ -105,6 +106,8 @@ static FitLayout fit_layout(const wgbm_fit_config& c) {
   // chunk descriptors: enough for any level (every segment ragged)
   L.h_desc = take(sizeof(int4) * hist_max_chunks(L.hc, c.n_feat, L.KN, MS));
   L.h_total = take(4);
+  L.h_mdesc = take(sizeof(int4) * hist_max_multi(L.hc, c.n_feat, L.KN));
+  L.h_mparts = take(4);
   L.hist_part_floats = (int64_t)hist_partial_words(L.hc, c.n_feat, c.n_bins, L.KN);
   L.h_part = take(sizeof(longlong2) * L.hist_part_floats);
   L.part_chunks = ceil_div(L.KN, kPartChunk) + MS;
@@ -442,6 +445,8 @@ static int enqueue_round(const wgbm_fit_config& c, const FitLayout& L, const wgb
       ra_h.chunk_dev = I(L.h_chunk);
       ra_h.chunk_desc = reinterpret_cast<int4*>(ws + L.h_desc);
       ra_h.chunk_total = I(L.h_total);
+      ra_h.multi_desc = reinterpret_cast<int4*>(ws + L.h_mdesc);
+      ra_h.merge_parts = I(L.h_mparts);
       ra_h.seg_first = I(L.h_seg_first);
       ra_h.seg_pfirst = I(L.h_seg_pfirst);
       root_plan = hist_plan_job(ra_h, L.hc, I(L.h_multi), I(L.h_n_multi));
@@ -538,6 +543,8 @@ static int enqueue_round(const wgbm_fit_config& c, const FitLayout& L, const wgb
         a.chunk_dev = I(L.h_chunk);
         a.chunk_desc = reinterpret_cast<int4*>(ws + L.h_desc);
         a.chunk_total = I(L.h_total);
+        a.multi_desc = reinterpret_cast<int4*>(ws + L.h_mdesc);
+        a.merge_parts = I(L.h_mparts);
         a.out_g = hg;
         a.out_h = hh;
         a.part = reinterpret_cast<longlong2*>(ws + L.h_part);
@@ -626,6 +633,8 @@ static int enqueue_round(const wgbm_fit_config& c, const FitLayout& L, const wgb
         na.chunk_dev = I(L.h_chunk);
         na.chunk_desc = reinterpret_cast<int4*>(ws + L.h_desc);
         na.chunk_total = I(L.h_total);
+        na.multi_desc = reinterpret_cast<int4*>(ws + L.h_mdesc);
+        na.merge_parts = I(L.h_mparts);
         na.seg_start = I(L.b_seg_start);
         na.seg_slot = I(L.b_seg_slot);
         na.seg_first = I(L.h_seg_first);

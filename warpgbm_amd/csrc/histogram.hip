@@ -388,14 +388,16 @@ k_hist_build(const HistArgs a) {
 }
 
 // Sum the chunk partials of every multi-chunk segment (exact integers) and
-// write the float32 histogram.  Work items (segment, slab, block of 32
-// words) are taken round-robin; the 8 groups of 32 threads of a workgroup
-// sum every 8th chunk of the item, four chunks' loads in flight per thread,
-// and reduce through LDS.  Partials are in the build's word order: word e =
-// (j, bin, lane) with (j, bin) = divmod(e / 64, B), feature lane*FPL + j.
-constexpr int kMergeElems = 32;
-constexpr int kMergeParts = 8;
-constexpr int kMergeUnroll = 4;
+// write the float32 histogram.  A workgroup of 256 threads takes a work item
+// (segment, slab, block of 256 / P words): P thread groups share a word, group
+// p summing chunks p, p + P, ... with eight loads in flight per thread, and
+// reduce through LDS.  P (1, 2, 4 or 8) comes from the plan: the fewest
+// groups that keep each group at or below 16 chunks (at most 8).  Partials
+// are in the build's word order: word e = (j, bin, lane) with
+// (j, bin) = divmod(e / 64, B), feature lane*FPL + j.
+constexpr int kMergeThreads = 256;
+constexpr int kMergeMaxParts = 8;
+constexpr int kMergeUnroll = 8;
 
 struct MergeSum {
   long long g = 0, x = 0;  // x: count (one plane) or hessian sum
@@ -408,25 +410,25 @@ struct MergeSum {
 };
 
 template <int FPL>
-__global__ void __launch_bounds__(kMergeElems * kMergeParts)
-k_hist_merge(const HistArgs a, const int32_t* __restrict__ multi, const int32_t* __restrict__ n_multi) {
-  __shared__ long long s_g[kMergeParts][kMergeElems];
-  __shared__ long long s_h[kMergeParts][kMergeElems];
+__global__ void __launch_bounds__(kMergeThreads)
+k_hist_merge(const HistArgs a, const int32_t* __restrict__ n_multi) {
+  __shared__ long long s_g[kMergeThreads];
+  __shared__ long long s_h[kMergeThreads];
+  const int P = *a.merge_parts;
+  const int WPI = kMergeThreads / P;  // words per item
   const int B = a.n_bins;
   const int SL = kWave * FPL * B;
-  const int neb = (SL + kMergeElems - 1) / kMergeElems;
+  const int neb = (SL + WPI - 1) / WPI;
   const int64_t n_items = (int64_t)(*n_multi) * a.nslab * neb;
-  const int el = threadIdx.x % kMergeElems;
-  const int part = threadIdx.x / kMergeElems;
+  const int el = threadIdx.x % WPI;
+  const int part = threadIdx.x / WPI;
   for (int64_t it = blockIdx.x; it < n_items; it += gridDim.x) {
     const int m = (int)(it / (a.nslab * neb));
     const int r = (int)(it - (int64_t)m * a.nslab * neb);
     const int slab = r / neb;
-    const int e = (r - slab * neb) * kMergeElems + el;
-    const int s = multi[m];
-    const int first = a.seg_pfirst[s];
-    const int nch = a.seg_first[s + 1] - a.seg_first[s];
-    const int slot = a.seg_slot[s];
+    const int e = (r - slab * neb) * WPI + el;
+    const int4 md = a.multi_desc[m];
+    const int first = md.x, nch = md.y, slot = md.z;
     const NodeShift ns = node_shift(a.stats + (int64_t)(slot / a.slots_per_node) * kStatWords);
     const bool one_plane = a.hess_const || ns.count_mode;
     const int rr = e >> 6;
@@ -440,42 +442,44 @@ k_hist_merge(const HistArgs a, const int32_t* __restrict__ multi, const int32_t*
     if (e < SL) {
       const int64_t off = ((int64_t)first * a.nslab + slab) * SL + e;
       const int64_t step = (int64_t)a.nslab * SL;
-      // Four chunks kMergeParts apart at once; a chunk past the last one
+      // kMergeUnroll chunks P apart at once; a chunk past the last one
       // re-reads the last chunk and is dropped.
       if (one_plane) {
         const unsigned long long* p = reinterpret_cast<const unsigned long long*>(a.part) + off;
-        for (int k = part; k < nch; k += kMergeUnroll * kMergeParts) {
+        for (int k = part; k < nch; k += kMergeUnroll * P) {
           unsigned long long w[kMergeUnroll];
 #pragma unroll
-          for (int u = 0; u < kMergeUnroll; ++u) w[u] = p[(int64_t)min(k + u * kMergeParts, nch - 1) * step];
+          for (int u = 0; u < kMergeUnroll; ++u) w[u] = p[(int64_t)min(k + u * P, nch - 1) * step];
 #pragma unroll
           for (int u = 0; u < kMergeUnroll; ++u)
-            if (k + u * kMergeParts < nch) acc.add_packed(w[u]);
+            if (k + u * P < nch) acc.add_packed(w[u]);
         }
       } else {
         const longlong2* p = a.part + off;
-        for (int k = part; k < nch; k += kMergeUnroll * kMergeParts) {
+        for (int k = part; k < nch; k += kMergeUnroll * P) {
           longlong2 v[kMergeUnroll];
 #pragma unroll
-          for (int u = 0; u < kMergeUnroll; ++u) v[u] = p[(int64_t)min(k + u * kMergeParts, nch - 1) * step];
+          for (int u = 0; u < kMergeUnroll; ++u) v[u] = p[(int64_t)min(k + u * P, nch - 1) * step];
 #pragma unroll
           for (int u = 0; u < kMergeUnroll; ++u)
-            if (k + u * kMergeParts < nch) acc.add_pair(v[u]);
+            if (k + u * P < nch) acc.add_pair(v[u]);
         }
       }
     }
-    s_g[part][el] = acc.g;
-    s_h[part][el] = acc.x;
-    __syncthreads();
+    if (P > 1) {
+      s_g[threadIdx.x] = acc.g;
+      s_h[threadIdx.x] = acc.x;
+      __syncthreads();
+    }
     if (part == 0 && e < SL) {
       long long sg = acc.g, sx = acc.x;
-      for (int q = 1; q < kMergeParts; ++q) { sg += s_g[q][el]; sx += s_h[q][el]; }
+      for (int q = 1; q < P; ++q) { sg += s_g[q * WPI + el]; sx += s_h[q * WPI + el]; }
       if (f_base + fl < a.n_feat) {
         const long long sh = one_plane ? sx * ns.qh_const : sx;
         store_bin(a, slot, f_base + fl, bin, from_fixed(sg, ns.sg), from_fixed(sh, ns.sh), sb);
       }
     }
-    __syncthreads();
+    if (P > 1) __syncthreads();
   }
 }
 
@@ -798,11 +802,10 @@ static int launch_fpl(const HistArgs& a, const HistConfig& c, int64_t max_chunks
   if (rc != kOk) return rc;
   if (max_multi > 0) {
     const int SL = kWave * FPL * a.n_bins;
-    const int64_t merge_items = max_multi * a.nslab * ceil_div(SL, kMergeElems);
+    const int64_t merge_items = max_multi * a.nslab * ceil_div(SL, kMergeThreads / kMergeMaxParts);
     const int64_t mg = merge_items < 2048 ? merge_items : 2048;
     timer_mark(g_timer.merge, true, st);
-    hipLaunchKernelGGL((k_hist_merge<FPL>), dim3((unsigned)mg), dim3(kMergeElems * kMergeParts), 0, st, a, multi,
-                       n_multi);
+    hipLaunchKernelGGL((k_hist_merge<FPL>), dim3((unsigned)mg), dim3(kMergeThreads), 0, st, a, n_multi);
     WGBM_LAUNCHED("k_hist_merge");
     timer_mark(g_timer.merge, false, st);
   }
@@ -836,6 +839,8 @@ PlanJob hist_plan_job(const HistArgs& a, const HistConfig& c, int32_t* multi, in
   j.src.seg_slot = a.seg_slot;
   j.src.chunk_desc = a.chunk_desc;
   j.src.chunk_total = a.chunk_total;
+  j.src.multi_desc = a.multi_desc;
+  j.src.merge_parts = a.merge_parts;
   j.chunk_rows = c.chunk_rows;
   j.seg_first = a.seg_first;
   j.seg_pfirst = a.seg_pfirst;
@@ -848,7 +853,8 @@ int hist_launch(HistArgs a, const HistConfig& c, int64_t rows_max, int64_t nseg_
                 int32_t* multi, int32_t* n_multi, hipStream_t st) {
   WGBM_REQUIRE(!a.sub_g || (a.sub_h && !a.accumulate && a.slots_per_node > 0),
                "histogram: sibling subtraction needs both parent planes and overwrite mode");
-  WGBM_REQUIRE(a.chunk_desc && a.chunk_total, "histogram: missing chunk descriptor scratch");
+  WGBM_REQUIRE(a.chunk_desc && a.chunk_total && a.multi_desc && a.merge_parts,
+               "histogram: missing chunk descriptor scratch");
   WGBM_REQUIRE(a.codes && a.perm && (a.gh || (a.g && a.h)) && a.seg_start && a.seg_count && a.seg_slot &&
                    a.seg_first && a.seg_pfirst && a.nseg_dev && a.out_g && a.out_h && a.part && a.stats && multi &&
                    n_multi,
@@ -922,7 +928,7 @@ extern "C" int wgbm_profile_end(double* out4) {
 // Workspace carve-up for the reference-style histogram op.
 struct Hist3Layout {
   size_t seg_start, seg_count, seg_slot, seg_first, seg_pfirst, nseg, multi, n_multi, chunk, desc, total_chunks,
-      sorted, cnt, stats, stat_part, part, total;
+      mdesc, mparts, sorted, cnt, stats, stat_part, part, total;
 };
 
 static Hist3Layout hist3_layout(int64_t n, int n_feat, int n_bins, int n_eras, const HistConfig& c) {
@@ -942,6 +948,8 @@ static Hist3Layout hist3_layout(int64_t n, int n_feat, int n_bins, int n_eras, c
   L.n_multi = o;   o = al(o + 4);
   L.desc = o;      o = al(o + sizeof(int4) * hist_max_chunks(c, n_feat, n, E));
   L.total_chunks = o; o = al(o + 4);
+  L.mdesc = o;     o = al(o + sizeof(int4) * max_multi);
+  L.mparts = o;    o = al(o + 4);
   const int64_t nb = ceil_div(n, kSortBlock);
   L.sorted = o;    o = al(o + (E > 1 ? 4 * n : 0));
   L.cnt = o;       o = al(o + (E > 1 ? 4 * nb * E : 0));
@@ -999,6 +1007,8 @@ extern "C" int wgbm_compute_histogram3(const int8_t* bin_indices, int64_t n_tota
   a.chunk_dev = P(L.chunk);
   a.chunk_desc = reinterpret_cast<int4*>(ws + L.desc);
   a.chunk_total = P(L.total_chunks);
+  a.multi_desc = reinterpret_cast<int4*>(ws + L.mdesc);
+  a.merge_parts = P(L.mparts);
   a.out_g = grad_hist;
   a.out_h = hess_hist;
   a.part = reinterpret_cast<longlong2*>(ws + L.part);

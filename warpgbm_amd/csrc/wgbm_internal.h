@@ -40,6 +40,8 @@ struct HistArgs {
   int4* chunk_desc;        // [max chunks] per chunk {first position, end position, slot, partial
                            // index or -1 (single-chunk segment)}, written by the plan
   int32_t* chunk_total;    // device scalar: chunks of the launch, written by the plan
+  int4* multi_desc;        // [max multi] per multi-chunk segment {first partial, chunks, slot, 0}
+  int32_t* merge_parts;    // device scalar: thread groups per word in the merge (1, 2, 4 or 8)
   int32_t chunk_rows;
   int32_t nslab;
   int32_t wide_ok;
@@ -142,6 +144,11 @@ struct PlanSrc {
   const int32_t* seg_slot;
   int4* chunk_desc;
   int32_t* chunk_total;
+  // With multi_desc set: {first partial, chunks, slot, 0} of every
+  // multi-chunk segment, and merge_parts = 1, 2, 4 or 8 thread groups per
+  // word so that each group sums at most 16 chunks where possible.
+  int4* multi_desc;
+  int32_t* merge_parts;
 };
 void plan_launch(const PlanSrc& src, int chunk_rows, int32_t* seg_first, int32_t* multi, int32_t* n_multi,
                  hipStream_t st);

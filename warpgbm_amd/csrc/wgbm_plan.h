@@ -43,7 +43,9 @@ __device__ __forceinline__ void plan_body(const PlanSrc& src, int32_t chunk_rows
     chunk_rows = min(max((want + 255) & ~255, src.chunk_min), chunk_rows);
     if (t == 0) *src.chunk_out = chunk_rows;
   }
+  __shared__ int s_maxnch;  // most chunks of a multi-chunk segment
   if (t < 3) s_carry[t] = 0;
+  if (t == 0) s_maxnch = 0;
   __syncthreads();
   for (int base = 0; base < nseg; base += 1024) {
     const int s = base + t;
@@ -73,6 +75,10 @@ __device__ __forceinline__ void plan_body(const PlanSrc& src, int32_t chunk_rows
       seg_first[s] = first;
       if (seg_pfirst) seg_pfirst[s] = pfirst;
       if (v[1] && multi) multi[c1 + pre[1] + x[1] - 1] = s;
+      if (v[1] && src.multi_desc) {
+        src.multi_desc[c1 + pre[1] + x[1] - 1] = make_int4(pfirst, v[0], src.seg_slot[s], 0);
+        atomicMax(&s_maxnch, v[0]);
+      }
     }
     if (src.chunk_desc) {
       // Chunk descriptors of this block of segments, all threads at once:
@@ -122,6 +128,10 @@ __device__ __forceinline__ void plan_body(const PlanSrc& src, int32_t chunk_rows
     seg_first[nseg] = s_carry[0];
     if (n_multi) *n_multi = s_carry[1];
     if (src.chunk_total) *src.chunk_total = s_carry[0];
+    if (src.merge_parts) {
+      const int mx = s_maxnch;
+      *src.merge_parts = mx <= 16 ? 1 : mx <= 32 ? 2 : mx <= 64 ? 4 : 8;
+    }
   }
 }
 
