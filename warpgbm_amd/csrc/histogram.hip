@@ -668,19 +668,30 @@ HistConfig pick_hist_config(int n_feat, int n_bins, int64_t row_stride, bool one
 }
 
 HistConfig auto_hist_config(int n_feat, int n_bins, int64_t row_stride, bool one_plane) {
-  (void)n_feat;
   HistConfig c;
   (void)one_plane;
   c.chunk_rows = 8192;
   c.waves = 8;
-  // Widest per-lane load whose one-plane histogram lets two workgroups of 8
-  // waves share a CU's 160 KB of LDS (the two-plane variant then fits one).
-  // Measured at 1M x 1000 x 32 bins: FPL 4 (67 KB) beats FPL 2 (34 KB,
-  // four workgroups) by 1.4x and FPL 8 (135 KB, one workgroup) by 1.1x.
-  int fpl = 16;
-  c.fpl = fpl;
-  while (fpl > 1 && (lds_bytes(c, n_bins, true) > 80 * 1024 || row_stride % fpl != 0)) c.fpl = fpl >>= 1;
-  c.fpl = fpl;
+  // Per-lane load width: the widest whose one-plane histogram lets two
+  // workgroups share a CU's 160 KB of LDS (the two-plane variant then fits
+  // one) and whose slabs keep at least 3/4 of the lanes on a feature (an idle
+  // lane's adds still take LDS cycles); with none, the best lane use.
+  // Measured at 1M x 1000 x 32 bins: FPL 4 (67 KB) beats FPL 2 by 1.4x and
+  // FPL 8 (135 KB, one workgroup) by 1.1x; at 500k x 400 x 5 bins FPL 8
+  // (78 % of lanes) builds 1.28x faster than FPL 16 (39 %) and byte loads
+  // (FPL 1, 89 %) are slower than both.
+  const int F = n_feat > 0 ? n_feat : 1;
+  int best = 1;
+  double best_use = -1.0;
+  for (int fpl = 16; fpl >= 1; fpl >>= 1) {
+    c.fpl = fpl;
+    if (fpl > 1 && (lds_bytes(c, n_bins, true) > 80 * 1024 || row_stride % fpl != 0)) continue;
+    const int64_t lanes = ceil_div(F, kWave * fpl) * kWave * fpl;
+    const double use = (double)F / (double)lanes;
+    if (use >= 0.75) { best = fpl; break; }
+    if (use > best_use) { best_use = use; best = fpl; }
+  }
+  c.fpl = best;
   return c;
 }
 
@@ -776,7 +787,7 @@ template <int FPL, int L, bool CNT>
 static int launch_mode(const HistArgs& a, const HistConfig& c, int64_t groups, hipStream_t st) {
   // 16-wave workgroups exist for FPL <= 4 at the shallower depth only (the
   // configuration picks them so): they must fit 64 VGPRs.
-  if constexpr (FPL <= 4 && L == (FPL == 1 ? 16 : 8 * (FPL == 2 ? 2 : 1))) {
+  if constexpr (FPL <= 4 && L == (FPL == 1 ? 16 : 8)) {
     if (c.waves > 8) return launch_waves<FPL, L, CNT, true>(a, c, groups, st);
   }
   return launch_waves<FPL, L, CNT, false>(a, c, groups, st);
@@ -814,9 +825,10 @@ static int launch_fpl(const HistArgs& a, const HistConfig& c, int64_t max_chunks
 
 // Load depth (rows in flight per wave) allowed per features-per-lane.
 int clamp_depth(int fpl, int waves, int depth) {
-  int hi = fpl <= 2 ? 32 : fpl == 4 ? 16 : fpl == 8 ? 8 : 4;
+  int hi = fpl == 1 ? 32 : fpl <= 4 ? 16 : fpl == 8 ? 8 : 4;
   const int lo = hi / 2;
-  // 16-wave workgroups keep the shallower depth within their 64 VGPRs
+  // 16-wave workgroups keep the shallower depth within their 64 VGPRs (at
+  // FPL 2 the two-plane variant spilled with 16 rows per group)
   if (waves > 8) hi = lo;
   const int def = hi;
   int d = depth > 0 ? depth : def;
@@ -881,7 +893,7 @@ int hist_launch(HistArgs a, const HistConfig& c, int64_t rows_max, int64_t nseg_
                        : launch_fpl<FPL, D1>(a, c, max_chunks, max_multi, multi, n_multi, st);
   switch (c.fpl) {
     case 1: WGBM_D2(1, 16, 32)
-    case 2: WGBM_D2(2, 16, 32)
+    case 2: WGBM_D2(2, 8, 16)
     case 4: WGBM_D2(4, 8, 16)
     case 8: WGBM_D2(8, 4, 8)
     case 16: WGBM_D2(16, 2, 4)
