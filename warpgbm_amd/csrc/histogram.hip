@@ -168,6 +168,15 @@ k_hist_build(const HistArgs a) {
   const int per = (len + W - 1) / W;
   const int w0 = min(c0 + wave * per, c1);
   const int w1 = min(w0 + per, c1);
+  // The wave's first row ids (and position-ordered g, h) are loaded before
+  // the node statistics and the LDS clear, so the three overlap.
+  const bool has_rows = a.n_rows > 0 && w0 < w1;
+  int first_raw = 0;
+  float2 first_gh = make_float2(0.0f, 0.0f);
+  if (has_rows) {
+    first_raw = a.perm[min(w0 + lane, w1 - 1)];
+    if constexpr (GH_POS) first_gh = a.gh[min(w0 + lane, w1 - 1)];
+  }
   const NodeShift ns = node_shift(a.stats + (int64_t)(slot / a.slots_per_node) * kStatWords);
   if (a.dual && ONE_PLANE != ns.count_mode) continue;  // the other variant builds this node
 
@@ -282,8 +291,8 @@ k_hist_build(const HistArgs a) {
       myqp = okr ? (1ull << kCountShift) + (unsigned long long)to_fixed(g, ns.sg) : 0ull;
       myqh = okr ? (unsigned long long)to_fixed(h, ns.sh) : 0ull;
     };
-    int rowv = valid_row(w0 + lane, perm_at(w0 + lane));
-    block_words(rowv, GH_POS ? gh_at(w0 + lane) : make_float2(0.0f, 0.0f));
+    int rowv = valid_row(w0 + lane, first_raw);
+    block_words(rowv, first_gh);
     TV A[LG], Bv[LG];
 #pragma unroll
     for (int q = 0; q < LG; ++q) A[q] = load_codes(__builtin_amdgcn_readlane(rowv, q));
@@ -333,7 +342,7 @@ k_hist_build(const HistArgs a) {
       block_words(rowv, ghn);
     }
   };
-  if (a.n_rows > 0 && w0 < w1) {
+  if (has_rows) {
     if (ident) run(std::true_type{});
     else run(std::false_type{});
   }
@@ -378,8 +387,14 @@ k_hist_build(const HistArgs a) {
         store_bin(a, slot, f_base + fl, bin, from_fixed(gsum, ns.sg), from_fixed(hsum, ns.sh), sb[u]);
       } else {
         const int64_t d = ((int64_t)pidx * a.nslab + slab) * SL + t;
-        if constexpr (ONE_PLANE) reinterpret_cast<unsigned long long*>(a.part)[d] = w;  // packed count | qg sum
-        else a.part[d] = make_longlong2(gsum, hsum);
+        // Partials are read once, by the merge: streaming stores keep them
+        // out of this XCD's L2 (less to write back at the end of the launch).
+        if constexpr (ONE_PLANE) {
+          __builtin_nontemporal_store(w, reinterpret_cast<unsigned long long*>(a.part) + d);  // packed count | qg sum
+        } else {
+          __builtin_nontemporal_store(gsum, &a.part[d].x);
+          __builtin_nontemporal_store(hsum, &a.part[d].y);
+        }
       }
     }
   }
