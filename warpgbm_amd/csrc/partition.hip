@@ -191,6 +191,147 @@ k_part_scan(PartArgs a) {
   }
 }
 
+// k_part_scan with every table of the level in LDS: one round of loads, the
+// scans in LDS, the outputs stored once.  Same outputs as k_part_scan (lbase
+// and rbase, child tables, build list, sibling map); the launch picks it
+// when the level has at most kScanLdsChunks chunks and kScanLdsSeg segments.
+constexpr int kScanLdsChunks = 4096;
+constexpr int kScanLdsSeg = 512;
+
+__global__ void __launch_bounds__(1024)
+k_part_scan_lds(PartArgs a) {
+  __shared__ int s_cs[kScanLdsChunks + 1];   // exclusive scan of the chunks' left counts
+  __shared__ int s_sf[kScanLdsSeg + 1];      // first chunk of each segment
+  __shared__ int s_PL[kScanLdsSeg + 1], s_PR[kScanLdsSeg + 1];
+  __shared__ int s_split[kScanLdsSeg];       // per node
+  __shared__ int s_cst[2 * kScanLdsSeg];     // children's segment starts [child * E + e]
+  __shared__ int s_ccount[2 * kScanLdsSeg];  // children's segment counts
+  __shared__ int s_nc[2 * kScanLdsSeg];      // children's node counts
+  __shared__ int s_built[kScanLdsSeg];       // built child of each split, in node order
+  const int E = a.n_eras;
+  const int nseg = a.n_nodes * E;
+  const int t = threadIdx.x;
+  const int total = a.seg_first[nseg];
+  // Loads: segment and node tables, four chunk counts per thread.
+  for (int s = t; s <= nseg; s += blockDim.x) s_sf[s] = a.seg_first[s];
+  for (int j = t; j < a.n_nodes; j += blockDim.x) s_split[j] = is_split(a, j) ? 1 : 0;
+  constexpr int CPT = kScanLdsChunks / 1024;  // chunks per thread
+  int v[CPT];
+  int mine = 0;
+#pragma unroll
+  for (int q = 0; q < CPT; ++q) {
+    const int c = t * CPT + q;
+    v[q] = c < total ? a.lbase[c] : 0;
+    mine += v[q];
+  }
+  int tot;
+  int run = block_exscan(mine, &tot);  // syncs: s_sf and s_split are complete
+#pragma unroll
+  for (int q = 0; q < CPT; ++q) {
+    const int c = t * CPT + q;
+    if (c <= total) s_cs[c] = run;
+    run += v[q];
+  }
+  if (t == 0) s_cs[total] = tot;
+  __syncthreads();
+  // Left / right row counts per segment, scanned over all segments.
+  int L = 0, R = 0;
+  if (t < nseg && s_split[t / E]) {
+    L = s_cs[s_sf[t + 1]] - s_cs[s_sf[t]];
+    R = max(a.seg_count[t], 0) - L;
+  }
+  int tl, tr;
+  const int el = block_exscan(L, &tl);
+  const int er = block_exscan(R, &tr);
+  if (t < nseg) { s_PL[t] = el; s_PR[t] = er; }
+  if (t == 0) { s_PL[nseg] = tl; s_PR[nseg] = tr; }
+  __syncthreads();
+  // Child tables.
+  for (int j = t; j < a.n_nodes; j += blockDim.x) {
+    const int l = 2 * j, r = 2 * j + 1;
+    if (!s_split[j]) {
+      a.c_active[l] = 0; a.c_active[r] = 0;
+      a.c_node_start[l] = 0; a.c_node_start[r] = 0;
+      a.c_node_count[l] = 0; a.c_node_count[r] = 0;
+      s_nc[l] = 0; s_nc[r] = 0;
+      continue;
+    }
+    const int start = a.node_start[j];
+    const int Lt = s_PL[(j + 1) * E] - s_PL[j * E];
+    const int Rt = a.node_count[j] - Lt;
+    a.c_active[l] = 1; a.c_active[r] = 1;
+    a.c_node_start[l] = start; a.c_node_count[l] = Lt;
+    a.c_node_start[r] = start + Lt; a.c_node_count[r] = Rt;
+    s_nc[l] = Lt; s_nc[r] = Rt;
+  }
+  for (int s = t; s < nseg; s += blockDim.x) {
+    const int j = s / E, e = s - j * E;
+    const int l = 2 * j, r = 2 * j + 1;
+    int ls = 0, lc = 0, rs = 0, rc = 0;
+    if (s_split[j]) {
+      const int start = a.node_start[j];
+      const int Lt = s_PL[(j + 1) * E] - s_PL[j * E];
+      ls = start + s_PL[s] - s_PL[j * E];
+      lc = s_PL[s + 1] - s_PL[s];
+      rs = start + Lt + s_PR[s] - s_PR[j * E];
+      rc = s_PR[s + 1] - s_PR[s];
+    }
+    a.c_seg_start[l * E + e] = ls; a.c_seg_count[l * E + e] = lc;
+    a.c_seg_start[r * E + e] = rs; a.c_seg_count[r * E + e] = rc;
+    s_cst[l * E + e] = ls; s_ccount[l * E + e] = lc;
+    s_cst[r * E + e] = rs; s_ccount[r * E + e] = rc;
+  }
+  __syncthreads();
+  // Absolute destination bases per chunk (only split segments own chunks).
+#pragma unroll
+  for (int q = 0; q < CPT; ++q) {
+    const int c = t * CPT + q;
+    if (c >= total) break;
+    const int s = upper_index(s_sf, nseg, c);
+    const int j = s / E, e = s - j * E;
+    const int first = s_sf[s];
+    const int lb = s_cs[c] - s_cs[first];
+    a.lbase[c] = lb + s_cst[(2 * j) * E + e];
+    a.rbase[c] = (c - first) * a.chunk_rows - lb + s_cst[(2 * j + 1) * E + e];
+  }
+  // Build list (smaller child, left on ties: warpgbm/core.py:1438) in node order.
+  const int sp = t < a.n_nodes ? s_split[t] : 0;
+  int ns;
+  const int rank = block_exscan(sp, &ns);
+  if (sp) {
+    const int l = 2 * t, r = 2 * t + 1;
+    const int built = (s_nc[l] <= s_nc[r]) ? l : r;
+    a.sib[3 * rank + 0] = t;
+    a.sib[3 * rank + 1] = built;
+    a.sib[3 * rank + 2] = built ^ 1;
+    s_built[rank] = built;
+  }
+  if (t == 0) {
+    *a.n_split = ns;
+    *a.n_build = ns * E;
+  }
+  __syncthreads();
+  for (int u = t; u < ns * E; u += blockDim.x) {
+    const int q = u / E, e = u - q * E;
+    const int c = s_built[q];
+    a.b_seg_start[u] = s_cst[c * E + e];
+    a.b_seg_count[u] = s_ccount[c * E + e];
+    a.b_seg_slot[u] = c * E + e;
+  }
+}
+
+static int part_scan_launch(const PartArgs& a, int64_t max_chunks, hipStream_t st) {
+  const int64_t nseg = (int64_t)a.n_nodes * a.n_eras;
+  if (max_chunks + 1 <= kScanLdsChunks && nseg <= kScanLdsSeg && a.n_nodes <= 1024) {
+    hipLaunchKernelGGL(k_part_scan_lds, dim3(1), dim3(1024), 0, st, a);
+    WGBM_LAUNCHED("k_part_scan_lds");
+  } else {
+    hipLaunchKernelGGL(k_part_scan, dim3(1), dim3(1024), 0, st, a);
+    WGBM_LAUNCHED("k_part_scan");
+  }
+  return kOk;
+}
+
 __device__ __forceinline__ void part_scatter_block(const PartArgs& a, int chunk) {
   const int nseg = *a.nseg_dev;
   if (chunk >= a.seg_first[nseg]) return;
@@ -325,8 +466,8 @@ int partition_launch(const PartArgs& a, int64_t rows_max, hipStream_t st, bool p
   const int64_t max_chunks = ceil_div(rows_max, a.chunk_rows) + nseg;
   hipLaunchKernelGGL(k_part_count, dim3((unsigned)max_chunks), dim3(256), 0, st, a);
   WGBM_LAUNCHED("k_part_count");
-  hipLaunchKernelGGL(k_part_scan, dim3(1), dim3(1024), 0, st, a);
-  WGBM_LAUNCHED("k_part_scan");
+  const int rc = part_scan_launch(a, max_chunks, st);
+  if (rc != kOk) return rc;
   hipLaunchKernelGGL(k_part_scatter, dim3((unsigned)max_chunks), dim3(256), 0, st, a);
   WGBM_LAUNCHED("k_part_scatter");
   if (a.c_stats && next_hist) {
@@ -545,8 +686,8 @@ int level_tail_launch(const PartArgs* p, const LeafArgs& la, const RecordArgs& r
                      p ? *p : PartArgs{}, la, (int)n_count);
   WGBM_LAUNCHED("k_count_leafsum");
   if (p) {
-    hipLaunchKernelGGL(k_part_scan, dim3(1), dim3(1024), 0, st, *p);
-    WGBM_LAUNCHED("k_part_scan");
+    const int rc = part_scan_launch(*p, n_count, st);
+    if (rc != kOk) return rc;
   }
   hipLaunchKernelGGL(k_move_apply_record, dim3((unsigned)(n_count + n_leaf + n_rec)), dim3(256), 0, st,
                      p ? *p : PartArgs{}, la, r, (int)n_count, (int)n_leaf);
