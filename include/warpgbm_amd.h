@@ -197,6 +197,13 @@ typedef struct wgbm_fit_io {
   float* node_value;            /* leaf value */
   int32_t* node_count;          /* rows in the node */
   float* node_era_gain;         /* [.., n_eras] per-era gain of a split, or NULL */
+  /* [n_rounds] training loss after each round, or NULL: weighted MSE of
+   * the scores (regression) or weighted log loss of their softmax, clamped
+   * to [1e-15, 1 - 1e-15] (multiclass) -- the per-tree training loss that
+   * compute_eval / compute_eval_multiclass log (warpgbm/core.py:1488-1516,
+   * :1793-1845; metrics.py weighted_mse_torch / weighted_log_loss_torch),
+   * summed in float64 here. */
+  float* train_loss;
 } wgbm_fit_io;
 
 size_t wgbm_fit_workspace_bytes(const wgbm_fit_config* config);

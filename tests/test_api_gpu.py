@@ -189,6 +189,40 @@ def test_fit_quality_correlation():
     assert np.corrcoef(m.predict(X[50000:]), y[50000:])[0, 1] > 0.9
 
 
+@pytest.mark.parametrize("objective", ["regression", "multiclass"])
+def test_eval_losses_match_plain(objective):
+    """The batched engine keeps eval scores on the device (each call adds its
+    new trees) and computes every tree's training loss on the device, running
+    the rounds between evaluations in one call; the losses it logs match the
+    plain engine's torch-computed ones (core.py:1488-1516, :1793-1845)."""
+    from warpgbm_amd import WarpGBM
+    rng = np.random.default_rng(21)
+    n, f = 6000, 20
+    X = rng.standard_normal((n, f)).astype(np.float32)
+    Xe = rng.standard_normal((1500, f)).astype(np.float32)
+    if objective == "regression":
+        w = rng.standard_normal(5)
+        y = (X[:, :5] @ w + 0.2 * rng.standard_normal(n)).astype(np.float32)
+        ye = (Xe[:, :5] @ w).astype(np.float32)
+        kw = dict(objective="regression", learning_rate=0.2)
+    else:
+        W = rng.standard_normal((5, 3))
+        y = np.argmax(X[:, :5] @ W + rng.gumbel(size=(n, 3)), axis=1)
+        ye = np.argmax(Xe[:, :5] @ W, axis=1)
+        kw = dict(objective="multiclass", learning_rate=0.3)
+    fits = {}
+    for engine in ("batched", "plain"):
+        m = WarpGBM(num_bins=16, max_depth=3, n_estimators=23, engine=engine, **kw)
+        m.fit(X, y, X_eval=Xe, y_eval=ye, eval_every_n_trees=4)
+        fits[engine] = m
+    b, p = fits["batched"], fits["plain"]
+    assert b._trees_trained == p._trees_trained == 23
+    assert len(b.training_loss) == len(p.training_loss) == 23
+    assert len(b.eval_loss) == len(p.eval_loss) == 6  # trees 0, 4, 8, 12, 16, 20
+    np.testing.assert_allclose(b.training_loss, p.training_loss, rtol=1e-5, atol=1e-7)
+    np.testing.assert_allclose(b.eval_loss, p.eval_loss, rtol=1e-5, atol=1e-7)
+
+
 @pytest.mark.parametrize("engine", ["batched", "plain"])
 def test_eval_set_and_early_stopping(engine):
     """core.py:1488-1516: eval loss recorded every n trees; early stopping

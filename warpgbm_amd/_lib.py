@@ -33,9 +33,11 @@ class FitIO(C.Structure):
     _fields_ = [
         ("codes", vp), ("y", vp), ("labels", vp), ("weight", vp), ("root_order", vp), ("era_count", vp),
         ("mono_required", vp), ("feat_idx", vp), ("scores", vp), ("node_status", vp), ("node_feat", vp),
-        ("node_bin", vp), ("node_value", vp), ("node_count", vp), ("node_era_gain", vp),
+        ("node_bin", vp), ("node_value", vp), ("node_count", vp), ("node_era_gain", vp), ("train_loss", vp),
     ]
 
+
+ABI_VERSION = 2  # wgbm_abi_version() of the library this binding matches
 
 # name -> (restype, argtypes)
 SIGNATURES = {
@@ -103,6 +105,10 @@ def load():
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args
+    if lib.wgbm_abi_version() != ABI_VERSION:
+        _load_error = (f"{_LIB_PATH} has ABI {lib.wgbm_abi_version()}, this package needs {ABI_VERSION}; "
+                       "rebuild with `python -m warpgbm_amd.build`")
+        raise ImportError(_load_error)
     _lib = lib
     return lib
 

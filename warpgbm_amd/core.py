@@ -512,7 +512,9 @@ class WarpGBM(BaseEstimator, RegressorMixin):
                 np.ascontiguousarray(sample_weight_eval, np.float32)).to(self.device)
             self.eval_every_n_trees = eval_every_n_trees
             self.early_stopping_rounds = early_stopping_rounds
+            self._eval_scores = None  # batched engine: raw eval scores kept on the device
         else:
+            self._eval_scores = None
             self.bin_indices_eval = None
             self.Y_gpu_eval = None
             self.sample_weight_eval_gpu = None
@@ -825,27 +827,42 @@ class WarpGBM(BaseEstimator, RegressorMixin):
             return
         cfg = self._fit_config(k, n_classes)
         ws = self._ws.get("fit", _lib.query("wgbm_fit_workspace_bytes", ctypes.byref(cfg)), self.device)
-        step = total if self.eval_every_n_trees is None else 1
+        every = self.eval_every_n_trees
+        if every is not None:
+            # raw scores of the eval rows under the trees so far (warm start
+            # included); each call adds its new trees to them
+            if n_classes == 1 and self.objective == "regression":
+                self._eval_scores = self.predict_binned(self.bin_indices_eval).reshape(-1, 1)
+            else:
+                self._eval_scores = self._raw_multiclass_scores(self.bin_indices_eval)
         i = start
         while i < self.n_estimators and not self.stop:
-            n = min(step, self.n_estimators - i)
+            if every is None:
+                n = self.n_estimators - i
+            else:
+                # end the call at the next tree the eval runs after (tree index
+                # a multiple of eval_every_n_trees): rounds in between only log
+                # their training loss, computed on the device
+                n = min(-(-i // every) * every - i + 1, self.n_estimators - i)
             feats = None
             if self.colsample_bytree < 1.0:
                 feats = torch.stack([torch.randperm(self.num_features, device=self.device, dtype=torch.int32)[:k]
                                      for _ in range(n)]).contiguous()
-            tables = self._run_rounds(cfg, ws, n, feats)
+            tables = self._run_rounds(cfg, ws, n, feats, with_loss=every is not None)
             self._append_rounds(tables, n, n_classes, feats)
-            for r in range(n):
-                self._trees_trained = i + r + 1
-                self._invalidate_prediction_cache()
-            if self.eval_every_n_trees is not None:
-                if n_classes == 1 and self.objective == "regression":
-                    self.compute_eval(i)
-                else:
-                    self.compute_eval_multiclass(i)
+            self._trees_trained = i + n
+            self._invalidate_prediction_cache()
+            if every is not None:
+                self._eval_add_rounds(tables, n, n_classes)
+                for r in range(n):
+                    loss = float(tables["train_loss"][r])
+                    if n_classes == 1 and self.objective == "regression":
+                        self.compute_eval(i + r, train_loss=loss)
+                    else:
+                        self.compute_eval_multiclass(i + r, train_loss=loss)
             i += n
 
-    def _run_rounds(self, cfg, ws, n, feats):
+    def _run_rounds(self, cfg, ws, n, feats, with_loss=False):
         K = cfg.n_classes
         NH = 2 ** (self.max_depth + 1) - 1
         dev = self.device
@@ -858,6 +875,8 @@ class WarpGBM(BaseEstimator, RegressorMixin):
             "count": torch.empty(shape, dtype=torch.int32, device=dev),
             "era_gain": torch.empty(shape + (self.num_eras,), dtype=torch.float32, device=dev),
         }
+        if with_loss:
+            t["train_loss"] = torch.empty((n,), dtype=torch.float32, device=dev)
         io = _lib.FitIO()
         io.codes = self.bin_indices.data_ptr()
         io.y = self.Y_gpu.data_ptr() if cfg.objective == 0 else None
@@ -874,11 +893,24 @@ class WarpGBM(BaseEstimator, RegressorMixin):
         io.node_value = t["value"].data_ptr()
         io.node_count = t["count"].data_ptr()
         io.node_era_gain = t["era_gain"].data_ptr()
+        io.train_loss = t["train_loss"].data_ptr() if with_loss else None
         _lib.call("wgbm_fit_rounds", ctypes.byref(cfg), ctypes.byref(io), n, ws.data_ptr(), ws.numel(),
                   _stream(dev))
         tables = {key: v.cpu().numpy() for key, v in t.items()}
         self._last_tables = tables
         return tables
+
+    def _eval_add_rounds(self, tables, n, K):
+        """Add the n new rounds' trees to the device eval scores: the rounds'
+        node tables compile to a heap forest (no per-tree host flattening) whose
+        traversal gives each eval row's sum over those trees."""
+        from .serving import HeapForest
+        NH = tables["status"].shape[-1]
+        hf = HeapForest.from_tables(tables["status"][:n].reshape(n * K, NH), tables["feat"][:n].reshape(n * K, NH),
+                                    tables["bin"][:n].reshape(n * K, NH), tables["value"][:n].reshape(n * K, NH),
+                                    self.max_depth, self.learning_rate, self.num_features, [0.0] * K,
+                                    n_classes=K, device=self.device)
+        self._eval_scores += hf.predict(self.bin_indices_eval)
 
     def _append_rounds(self, tables, n, K, feats):
         """Node tables of n rounds -> the reference's nested tree dicts, and
@@ -928,14 +960,18 @@ class WarpGBM(BaseEstimator, RegressorMixin):
             return weighted_rmsle_torch(y_true, y_pred, sample_weight=sample_weight).item()
         raise ValueError(f"Invalid eval_metric: {self.eval_metric}.")
 
-    def compute_eval(self, i):
-        """Training loss every tree, eval loss every n trees, early stop (core.py:1488-1516)."""
+    def compute_eval(self, i, train_loss=None):
+        """Training loss every tree, eval loss every n trees, early stop (core.py:1488-1516).
+        train_loss: the loss after tree i when the caller already has it (the
+        batched engine computes it on the device)."""
         if self.eval_every_n_trees is None:
             return
-        train_loss = weighted_mse_torch(self.Y_gpu, self.gradients, sample_weight=self.sample_weight_gpu).item()
+        if train_loss is None:
+            train_loss = weighted_mse_torch(self.Y_gpu, self.gradients, sample_weight=self.sample_weight_gpu).item()
         self.training_loss.append(train_loss)
         if i % self.eval_every_n_trees == 0:
-            preds = self.predict_binned(self.bin_indices_eval)
+            preds = (self._eval_scores[:, 0] if getattr(self, "_eval_scores", None) is not None
+                     else self.predict_binned(self.bin_indices_eval))
             loss = self.get_eval_metric(self.Y_gpu_eval, preds, sample_weight=self.sample_weight_eval_gpu)
             self.eval_loss.append(loss)
             if len(self.eval_loss) > self.early_stopping_rounds:
@@ -944,15 +980,17 @@ class WarpGBM(BaseEstimator, RegressorMixin):
             print(f"🌲 Tree {i+1}/{self.n_estimators} | Train MSE: {train_loss:.6f} | "
                   f"Eval {self.eval_metric}: {loss:.6f}")
 
-    def compute_eval_multiclass(self, i):
-        """warpgbm/core.py:1793-1845."""
+    def compute_eval_multiclass(self, i, train_loss=None):
+        """warpgbm/core.py:1793-1845 (train_loss: as in compute_eval)."""
         if self.eval_every_n_trees is None:
             return
-        probs = softmax(self.gradients, dim=1)
-        train_loss = weighted_log_loss_torch(self.Y_gpu, probs, sample_weight=self.sample_weight_gpu).item()
+        if train_loss is None:
+            probs = softmax(self.gradients, dim=1)
+            train_loss = weighted_log_loss_torch(self.Y_gpu, probs, sample_weight=self.sample_weight_gpu).item()
         self.training_loss.append(train_loss)
         if i % self.eval_every_n_trees == 0:
-            ep = self.predict_proba_binned(self.bin_indices_eval)
+            ep = (softmax(self._eval_scores, dim=1) if getattr(self, "_eval_scores", None) is not None
+                  else self.predict_proba_binned(self.bin_indices_eval))
             if self.eval_metric == "accuracy":
                 loss = 1.0 - weighted_accuracy_torch(self.Y_gpu_eval, torch.argmax(ep, dim=1),
                                                      sample_weight=self.sample_weight_eval_gpu).item()
@@ -993,6 +1031,10 @@ class WarpGBM(BaseEstimator, RegressorMixin):
 
     def predict_proba_binned(self, bin_indices):
         """Uniform log(1/K) prior + per-class forests, softmax (core.py:2021-2058)."""
+        return softmax(self._raw_multiclass_scores(bin_indices), dim=1)
+
+    def _raw_multiclass_scores(self, bin_indices):
+        """Per-class raw scores: the log(1/K) prior plus each class's forest."""
         n = bin_indices.size(0)
         K = self.num_classes
         F = torch.zeros((n, K), device=self.device, dtype=torch.float32)
@@ -1007,7 +1049,7 @@ class WarpGBM(BaseEstimator, RegressorMixin):
             col = torch.zeros(n, device=self.device, dtype=torch.float32)
             _nk.predict_forest(bic, tt.contiguous(), self.learning_rate, col)
             F[:, k] += col
-        return softmax(F, dim=1)
+        return F
 
     # ------------------------------------------------------------ importance
     def get_feature_importance(self, importance_type="gain", normalize=True):

@@ -36,7 +36,7 @@ struct FitLayout {
   int64_t gain_nodes;
   size_t b_seg_start, b_seg_count, b_seg_slot, n_build, sib, n_split;
   size_t h_seg_first, h_seg_pfirst, h_nseg, h_multi, h_n_multi, h_chunk, h_part, h_desc, h_total, h_mdesc,
-      h_mparts;
+      h_mparts, loss_part;
   size_t stat_part, round_ctr, fidx_cur;
   size_t p_flag, p_seg_first, p_nseg, p_multi, p_n_multi, p_lbase, p_rbase, p_cscan, p_segL, p_segP, codes_t;
   size_t l_seg_first, l_nseg, l_multi, l_n_multi, l_part, l_val;
@@ -108,6 +108,7 @@ static FitLayout fit_layout(const wgbm_fit_config& c) {
   L.h_total = take(4);
   L.h_mdesc = take(sizeof(int4) * hist_max_multi(L.hc, c.n_feat, L.KN));
   L.h_mparts = take(4);
+  L.loss_part = take(8 * 2 * kStatBlocks);
   L.hist_part_floats = (int64_t)hist_partial_words(L.hc, c.n_feat, c.n_bins, L.KN);
   L.h_part = take(sizeof(longlong2) * L.hist_part_floats);
   L.part_chunks = ceil_div(L.KN, kPartChunk) + MS;
@@ -319,6 +320,58 @@ k_round_start_softmax(const float* __restrict__ F, const int32_t* __restrict__ y
   }
 }
 
+
+// Training loss after a round: per-block float64 partials of (sum w * loss,
+// sum w), then one workgroup folds them and writes train_loss[round].
+// Regression: loss = (y - s)^2.  Multiclass: loss = -log(clamp(softmax(s)_y,
+// 1e-15, 1 - 1e-15)), the softmax formed as the gradient kernel forms it.
+__global__ void __launch_bounds__(256)
+k_loss_part(const float* __restrict__ y, const int32_t* __restrict__ labels, const float* __restrict__ s,
+            const float* __restrict__ w, int64_t n, int K, double* __restrict__ part) {
+  double num = 0.0, den = 0.0;
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+    const double wr = (double)w[r];
+    double loss;
+    if (labels == nullptr) {
+      const double e = (double)(y[r] - s[r]);
+      loss = e * e;
+    } else {
+      const float* f = s + r * K;
+      float mx = f[0];
+      for (int k = 1; k < K; ++k) mx = fmaxf(mx, f[k]);
+      float den_s = 0.0f;
+      for (int k = 0; k < K; ++k) den_s += expf(f[k] - mx);
+      const float p = fminf(fmaxf(expf(f[labels[r]] - mx) / den_s, 1e-15f), 1.0f - 1e-15f);
+      loss = -(double)logf(p);
+    }
+    num += wr * loss;
+    den += wr;
+  }
+  __shared__ double sn[256], sd[256];
+  sn[threadIdx.x] = num;
+  sd[threadIdx.x] = den;
+  __syncthreads();
+  for (int o = 128; o >= 1; o >>= 1) {
+    if ((int)threadIdx.x < o) { sn[threadIdx.x] += sn[threadIdx.x + o]; sd[threadIdx.x] += sd[threadIdx.x + o]; }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) { part[2 * blockIdx.x] = sn[0]; part[2 * blockIdx.x + 1] = sd[0]; }
+}
+
+__global__ void __launch_bounds__(256)
+k_loss_final(const double* __restrict__ part, int nb, const int32_t* __restrict__ round_ctr, float* __restrict__ out) {
+  __shared__ double sn[256], sd[256];
+  double num = 0.0, den = 0.0;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) { num += part[2 * b]; den += part[2 * b + 1]; }
+  sn[threadIdx.x] = num;
+  sd[threadIdx.x] = den;
+  __syncthreads();
+  for (int o = 128; o >= 1; o >>= 1) {
+    if ((int)threadIdx.x < o) { sn[threadIdx.x] += sn[threadIdx.x + o]; sd[threadIdx.x] += sd[threadIdx.x + o]; }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[round_ctr[0]] = (float)(sn[0] / sd[0]);
+}
 
 // The root statistics of every class (workgroups [0, K)) and the root
 // histogram plan (workgroup K) in one launch of 1024-thread workgroups.
@@ -651,6 +704,14 @@ static int enqueue_round(const wgbm_fit_config& c, const FitLayout& L, const wgb
       }
       rc = level_tail_launch(d < D ? &p : nullptr, la, r, L.KN, st, d < D ? &next : nullptr);
       if (rc != kOk) return rc;
+    }
+    if (io->train_loss) {  // the round's training loss, after its last score update
+      hipLaunchKernelGGL(k_loss_part, dim3((unsigned)nsb), dim3(256), 0, st, io->y, c.objective == 1 ? io->labels : nullptr,
+                         io->scores, io->weight, N, K, reinterpret_cast<double*>(ws + L.loss_part));
+      WGBM_LAUNCHED("k_loss_part");
+      hipLaunchKernelGGL(k_loss_final, dim3(1), dim3(256), 0, st, reinterpret_cast<const double*>(ws + L.loss_part), nsb,
+                         I(L.round_ctr), io->train_loss);
+      WGBM_LAUNCHED("k_loss_final");
     }
   return kOk;
 }
