@@ -38,7 +38,7 @@ struct FitLayout {
   size_t h_seg_first, h_seg_pfirst, h_nseg, h_multi, h_n_multi, h_chunk, h_part, h_desc, h_total, h_mdesc,
       h_mparts, loss_part;
   size_t stat_part, round_ctr, fidx_cur;
-  size_t p_flag, p_seg_first, p_nseg, p_multi, p_n_multi, p_lbase, p_rbase, p_cscan, p_segL, p_segP, codes_t;
+  size_t p_flag, p_seg_first, p_nseg, p_multi, p_n_multi, p_lbase, p_rbase, p_cscan, p_segL, p_segP, codes_t, p_cseg;
   size_t l_seg_first, l_nseg, l_multi, l_n_multi, l_part, l_val;
   size_t root_seg_start, root_seg_count, root_seg_slot, root_nseg, era_start;
   // sizes
@@ -118,6 +118,7 @@ static FitLayout fit_layout(const wgbm_fit_config& c) {
   L.p_multi = take(4 * (L.KN / (kPartChunk + 1) + 2));
   L.p_n_multi = take(4);
   L.p_lbase = take(4 * L.part_chunks);
+  L.p_cseg = take(4 * L.part_chunks);
   L.p_rbase = take(4 * L.part_chunks);
   L.p_cscan = take(4 * (L.part_chunks + 1));
   L.codes_t = take((size_t)N * c.n_cols);  // column-major copy of the bin matrix
@@ -561,6 +562,7 @@ static int enqueue_round(const wgbm_fit_config& c, const FitLayout& L, const wgb
         p.multi = I(L.p_multi);
         p.n_multi = I(L.p_n_multi);
         p.lbase = I(L.p_lbase);
+        p.chunk_seg = I(L.p_cseg);
         p.rbase = I(L.p_rbase);
         p.cscan = I(L.p_cscan);
         p.codes_t = reinterpret_cast<const int8_t*>(ws + L.codes_t);

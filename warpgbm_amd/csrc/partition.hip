@@ -36,10 +36,15 @@ __device__ __forceinline__ bool goes_left(const PartArgs& a, int row, int col, i
 // Block bodies take the chunk as an argument: the kernels below run them one
 // chunk per workgroup, and the batched engine's fused level launches run
 // several of them side by side over disjoint workgroup ranges.
+// Segment of a chunk: the plan's map when present, else a search.
+__device__ __forceinline__ int chunk_segment(const PartArgs& a, int nseg, int chunk) {
+  return a.chunk_seg ? a.chunk_seg[chunk] : upper_index(a.seg_first, nseg, chunk);
+}
+
 __device__ __forceinline__ void part_count_block(const PartArgs& a, int chunk) {
   const int nseg = *a.nseg_dev;
   if (chunk >= a.seg_first[nseg]) return;
-  const int s = upper_index(a.seg_first, nseg, chunk);
+  const int s = chunk_segment(a, nseg, chunk);
   const int k = chunk - a.seg_first[s];
   const int j = s / a.n_eras;
   const int col = split_col(a, j);
@@ -153,7 +158,7 @@ k_part_scan(PartArgs a) {
   __syncthreads();
   // C: absolute destination bases per chunk (only split segments own chunks).
   for (int c = threadIdx.x; c < total; c += blockDim.x) {
-    const int s = upper_index(a.seg_first, nseg, c);
+    const int s = chunk_segment(a, nseg, c);
     const int j = s / E, e = s - j * E;
     const int first = a.seg_first[s];
     const int lb = a.cscan[c] - a.cscan[first];
@@ -335,7 +340,7 @@ static int part_scan_launch(const PartArgs& a, int64_t max_chunks, hipStream_t s
 __device__ __forceinline__ void part_scatter_block(const PartArgs& a, int chunk) {
   const int nseg = *a.nseg_dev;
   if (chunk >= a.seg_first[nseg]) return;
-  const int s = upper_index(a.seg_first, nseg, chunk);
+  const int s = chunk_segment(a, nseg, chunk);
   const int k = chunk - a.seg_first[s];
   const int c0 = a.seg_start[s] + k * a.chunk_rows;
   const int c1 = min(c0 + a.chunk_rows, a.seg_start[s] + max(a.seg_count[s], 0));
@@ -423,6 +428,7 @@ PlanJob part_plan_job(const PartArgs& a) {
   j.src.node_active = a.node_active;
   j.src.dec_feat = a.dec_feat;
   j.src.n_eras = a.n_eras;
+  j.src.chunk_seg = a.chunk_seg;
   j.chunk_rows = a.chunk_rows;
   j.seg_first = a.seg_first;
   j.multi = a.multi;

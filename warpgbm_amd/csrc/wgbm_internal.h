@@ -149,6 +149,8 @@ struct PlanSrc {
   // word so that each group sums at most 16 chunks where possible.
   int4* multi_desc;
   int32_t* merge_parts;
+  // With chunk_seg set: the segment of every chunk (partition plans).
+  int32_t* chunk_seg;
 };
 void plan_launch(const PlanSrc& src, int chunk_rows, int32_t* seg_first, int32_t* multi, int32_t* n_multi,
                  hipStream_t st);

@@ -32,6 +32,7 @@ struct PartArgs {
   int32_t* nseg_dev;            // scalar
   int32_t* multi;               // plan scratch
   int32_t* n_multi;
+  int32_t* chunk_seg;           // [max chunks] segment of each chunk, written by the plan, or nullptr
   int32_t* lbase;               // [max chunks]
   int32_t* rbase;
   int32_t* cscan;               // [max chunks + 1]

@@ -80,7 +80,7 @@ __device__ __forceinline__ void plan_body(const PlanSrc& src, int32_t chunk_rows
         atomicMax(&s_maxnch, v[0]);
       }
     }
-    if (src.chunk_desc) {
+    if (src.chunk_desc || src.chunk_seg) {
       // Chunk descriptors of this block of segments, all threads at once:
       // the segments' tables go to LDS, then thread t takes chunks t,
       // t + 1024, ... and finds its segment by binary search (the last
@@ -88,12 +88,14 @@ __device__ __forceinline__ void plan_body(const PlanSrc& src, int32_t chunk_rows
       // next one's first chunk).
       __shared__ int s_first[1024], s_start[1024], s_end[1024], s_slot[1024], s_pf[1024];
       if (s < nseg) {
-        const int start = src.seg_start[s];
         s_first[t] = c0 + pre[0] + x[0] - v[0];
-        s_start[t] = start;
-        s_end[t] = start + max(plan_count(src, s), 0);
-        s_slot[t] = src.seg_slot[s];
-        s_pf[t] = v[1] ? c2 + pre[2] + x[2] - v[2] : -1;
+        if (src.chunk_desc) {
+          const int start = src.seg_start[s];
+          s_start[t] = start;
+          s_end[t] = start + max(plan_count(src, s), 0);
+          s_slot[t] = src.seg_slot[s];
+          s_pf[t] = v[1] ? c2 + pre[2] + x[2] - v[2] : -1;
+        }
       } else {
         s_first[t] = 0x7fffffff;
       }
@@ -109,9 +111,13 @@ __device__ __forceinline__ void plan_body(const PlanSrc& src, int32_t chunk_rows
           else hi = mid;
         }
         const int j = lo - 1;
-        const int k = c - s_first[j];
-        const int a0 = s_start[j] + k * chunk_rows;
-        src.chunk_desc[c] = make_int4(a0, min(a0 + chunk_rows, s_end[j]), s_slot[j], s_pf[j] >= 0 ? s_pf[j] + k : -1);
+        if (src.chunk_seg) src.chunk_seg[c] = base + j;
+        if (src.chunk_desc) {
+          const int k = c - s_first[j];
+          const int a0 = s_start[j] + k * chunk_rows;
+          src.chunk_desc[c] = make_int4(a0, min(a0 + chunk_rows, s_end[j]), s_slot[j],
+                                        s_pf[j] >= 0 ? s_pf[j] + k : -1);
+        }
       }
     }
     __syncthreads();
