@@ -1,4 +1,4 @@
-// Chunk plans (device side): one 1024-thread workgroup turns per-segment row
+// Chunk plans (device side): one workgroup turns per-segment row
 // counts into chunk offsets, the list of multi-chunk segments and their
 // partial slots.  Shared by the histogram, partition and leaf launches and by
 // the fused launches that run a plan beside other work.
@@ -19,7 +19,7 @@ __device__ __forceinline__ int plan_count(const PlanSrc& p, int s) {
   return split ? p.seg_count[s] : -1;
 }
 
-// One workgroup of 1024 threads computes a plan.
+// One workgroup (any multiple of 64 threads up to 1024) computes a plan.
 __device__ __forceinline__ void plan_body(const PlanSrc& src, int32_t chunk_rows, int32_t* __restrict__ seg_first,
                           int32_t* __restrict__ seg_pfirst, int32_t* __restrict__ multi,
                           int32_t* __restrict__ n_multi) {
@@ -28,6 +28,7 @@ __device__ __forceinline__ void plan_body(const PlanSrc& src, int32_t chunk_rows
   const int nseg = src.nseg_dev ? *src.nseg_dev : src.nseg;
   if (src.nseg_out && threadIdx.x == 0) *src.nseg_out = nseg;
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  const int nt = blockDim.x, nw = nt >> 6;
   if (src.chunk_out) {
     // Rows per chunk for this launch: enough chunks to give every slot a
     // work item (chunk_target of them), within [chunk_min, chunk_rows].
@@ -47,7 +48,7 @@ __device__ __forceinline__ void plan_body(const PlanSrc& src, int32_t chunk_rows
   if (t < 3) s_carry[t] = 0;
   if (t == 0) s_maxnch = 0;
   __syncthreads();
-  for (int base = 0; base < nseg; base += 1024) {
+  for (int base = 0; base < nseg; base += nt) {
     const int s = base + t;
     int v[3] = {0, 0, 0};  // chunks, is-multi, partial chunks
     if (s < nseg) {
@@ -83,7 +84,7 @@ __device__ __forceinline__ void plan_body(const PlanSrc& src, int32_t chunk_rows
     if (src.chunk_desc || src.chunk_seg) {
       // Chunk descriptors of this block of segments, all threads at once:
       // the segments' tables go to LDS, then thread t takes chunks t,
-      // t + 1024, ... and finds its segment by binary search (the last
+      // t + blockDim, ... and finds its segment by binary search (the last
       // segment whose first chunk is <= c; a segment without chunks has the
       // next one's first chunk).
       __shared__ int s_first[1024], s_start[1024], s_end[1024], s_slot[1024], s_pf[1024];
@@ -101,9 +102,9 @@ __device__ __forceinline__ void plan_body(const PlanSrc& src, int32_t chunk_rows
       }
       __syncthreads();
       int blk_chunks = 0;
-      for (int k = 0; k < 16; ++k) blk_chunks += s_sum[0][k];
-      const int nb = min(1024, nseg - base);
-      for (int c = c0 + t; c < c0 + blk_chunks; c += 1024) {
+      for (int k = 0; k < nw; ++k) blk_chunks += s_sum[0][k];
+      const int nb = min(nt, nseg - base);
+      for (int c = c0 + t; c < c0 + blk_chunks; c += nt) {
         int lo = 0, hi = nb;  // count of segments with first <= c
         while (lo < hi) {
           const int mid = (lo + hi) >> 1;
@@ -121,9 +122,9 @@ __device__ __forceinline__ void plan_body(const PlanSrc& src, int32_t chunk_rows
       }
     }
     __syncthreads();
-    if (t == 1023) {
+    if (t == nt - 1) {
       int tot[3] = {0, 0, 0};
-      for (int k = 0; k < 16; ++k) { tot[0] += s_sum[0][k]; tot[1] += s_sum[1][k]; tot[2] += s_sum[2][k]; }
+      for (int k = 0; k < nw; ++k) { tot[0] += s_sum[0][k]; tot[1] += s_sum[1][k]; tot[2] += s_sum[2][k]; }
       s_carry[0] = c0 + tot[0];
       s_carry[1] = c1 + tot[1];
       s_carry[2] = c2 + tot[2];
