@@ -309,6 +309,7 @@ k_hist_build(const HistArgs a) {
 #pragma unroll
         for (int q = 0; q < LG; ++q) {
           const int r = g0 + q;
+          if (r >= cnt) break;  // uniform: no adds for positions past the wave's range
           const unsigned long long qp =
               ((unsigned long long)(uint32_t)__builtin_amdgcn_readlane((int)(myqp >> 32), r) << 32) |
               (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)myqp, r);

@@ -31,17 +31,26 @@ __device__ __forceinline__ void plan_body(const PlanSrc& src, int32_t chunk_rows
   const int nt = blockDim.x, nw = nt >> 6;
   if (src.chunk_out) {
     // Rows per chunk for this launch: enough chunks to give every slot a
-    // work item (chunk_target of them), within [chunk_min, chunk_rows].
-    __shared__ int s_rows;
-    if (t == 0) s_rows = 0;
+    // work item (chunk_target of them), within [chunk_min, chunk_rows].  Each
+    // non-empty segment may end in a ragged chunk, so the rows are spread over
+    // chunk_target minus that many chunks (at least half the target): the
+    // launch then has at most chunk_target chunks and no slot takes a second
+    // work item after the others have finished.
+    __shared__ int s_rows, s_live;
+    if (t == 0) { s_rows = 0; s_live = 0; }
     __syncthreads();
-    int mine = 0;
-    for (int s = t; s < nseg; s += blockDim.x) mine += max(plan_count(src, s), 0);
-    for (int o = 32; o >= 1; o >>= 1) mine += __shfl_xor(mine, o);
-    if (lane == 0) atomicAdd(&s_rows, mine);
+    int mine = 0, live = 0;
+    for (int s = t; s < nseg; s += blockDim.x) {
+      const int c = max(plan_count(src, s), 0);
+      mine += c;
+      live += c > 0;
+    }
+    for (int o = 32; o >= 1; o >>= 1) { mine += __shfl_xor(mine, o); live += __shfl_xor(live, o); }
+    if (lane == 0) { atomicAdd(&s_rows, mine); atomicAdd(&s_live, live); }
     __syncthreads();
-    const int want = (s_rows + src.chunk_target - 1) / src.chunk_target;
-    chunk_rows = min(max((want + 255) & ~255, src.chunk_min), chunk_rows);
+    const int tgt = max(src.chunk_target - s_live, (src.chunk_target + 1) / 2);
+    const int want = (s_rows + tgt - 1) / tgt;
+    chunk_rows = min(max((want + 63) & ~63, src.chunk_min), chunk_rows);  // whole 64-row blocks
     if (t == 0) *src.chunk_out = chunk_rows;
   }
   __shared__ int s_maxnch;  // most chunks of a multi-chunk segment
