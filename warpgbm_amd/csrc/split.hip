@@ -260,29 +260,48 @@ __device__ void pick_node(const LevelScanArgs& a, int node, int wg_per_node, int
     if (lane == 0) a.era_gain[node] = fl >= 0 ? b.crit : 0.0f;
     return;
   }
-  constexpr int U = 8;  // bins loaded at once
-  for (int e = lane; e < E; e += 64) {
-    float ga = 0.0f;
+  // A lane takes eras lane, lane + 64, ...: EU eras' bins (U at a time) are
+  // loaded together, then each era's sums run in bin order.
+  constexpr int U = 8, EU = 4;
+  for (int e0 = lane; e0 < E; e0 += 64 * EU) {
+    double Gt[EU], Ht[EU], GL[EU], HL[EU];
+#pragma unroll
+    for (int k = 0; k < EU; ++k) { Gt[k] = 0.0; Ht[k] = 0.0; GL[k] = 0.0; HL[k] = 0.0; }
     if (fl >= 0) {
-      const int64_t off = (((int64_t)node * E + e) * a.n_feat + fl) * B;
-      const float* g = a.G + off;
-      const float* h = a.H + off;
-      double Gt = 0.0, Ht = 0.0, GL = 0.0, HL = 0.0;
       for (int q0 = 0; q0 < B; q0 += U) {
-        float gv[U], hv[U];
+        float gv[EU][U], hv[EU][U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) { const int q = min(q0 + u, B - 1); gv[u] = g[q]; hv[u] = h[q]; }
+        for (int k = 0; k < EU; ++k) {
+          const int e = min(e0 + 64 * k, E - 1);
+          const int64_t off = (((int64_t)node * E + e) * a.n_feat + fl) * B;
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-          const int q = q0 + u;
-          if (q < B) { Gt += (double)gv[u]; Ht += (double)hv[u]; }
-          if (q <= bin) { GL += (double)gv[u]; HL += (double)hv[u]; }
+          for (int u = 0; u < U; ++u) {
+            const int q = min(q0 + u, B - 1);
+            gv[k][u] = a.G[off + q];
+            hv[k][u] = a.H[off + q];
+          }
         }
+#pragma unroll
+        for (int k = 0; k < EU; ++k)
+#pragma unroll
+          for (int u = 0; u < U; ++u) {
+            const int q = q0 + u;
+            if (q < B) { Gt[k] += (double)gv[k][u]; Ht[k] += (double)hv[k][u]; }
+            if (q <= bin) { GL[k] += (double)gv[k][u]; HL[k] += (double)hv[k][u]; }
+          }
       }
-      float di;
-      gain_dir(GL, HL, Gt, Ht, (double)a.mcw, (double)a.lam, ga, di);
     }
-    a.era_gain[(int64_t)node * E + e] = ga;
+#pragma unroll
+    for (int k = 0; k < EU; ++k) {
+      const int e = e0 + 64 * k;
+      if (e >= E) break;
+      float ga = 0.0f;
+      if (fl >= 0) {
+        float di;
+        gain_dir(GL[k], HL[k], Gt[k], Ht[k], (double)a.mcw, (double)a.lam, ga, di);
+      }
+      a.era_gain[(int64_t)node * E + e] = ga;
+    }
   }
 }
 

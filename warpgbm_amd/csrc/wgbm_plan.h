@@ -19,6 +19,37 @@ __device__ __forceinline__ int plan_count(const PlanSrc& p, int s) {
   return split ? p.seg_count[s] : -1;
 }
 
+// Chunks of segment s: a segment with rows owns ceil(rows / chunk_rows), an
+// empty one owns one (empty) chunk so that its slot is written, a padding or
+// non-selected segment (count < 0) owns none.
+__device__ __forceinline__ int plan_chunks(const PlanSrc& p, int s, int chunk_rows) {
+  const int c = plan_count(p, s);
+  return c < 0 ? 0 : (c == 0 ? 1 : (c + chunk_rows - 1) / chunk_rows);
+}
+
+constexpr int kPlanCpt = 32;        // segments per thread per pass
+constexpr int kPlanOwnChunks = 32;  // a segment with more chunks is written by the whole workgroup
+constexpr int kPlanBig = 256;       // such segments listed per pass
+
+// Chunk tables of segment s (chunks first .. first + n - 1, partials from pf
+// or -1): chunks k = k0, k0 + step, ... get their segment and descriptor.
+__device__ __forceinline__ void plan_write_chunks(const PlanSrc& src, int s, int first, int n, int pf, int k0,
+                                                  int step, int chunk_rows) {
+  int start = 0, end = 0, slot = 0;
+  if (src.chunk_desc) {
+    start = src.seg_start[s];
+    end = start + max(plan_count(src, s), 0);
+    slot = src.seg_slot[s];
+  }
+  for (int k = k0; k < n; k += step) {
+    if (src.chunk_seg) src.chunk_seg[first + k] = s;
+    if (src.chunk_desc) {
+      const int a0 = start + k * chunk_rows;
+      src.chunk_desc[first + k] = make_int4(a0, min(a0 + chunk_rows, end), slot, pf >= 0 ? pf + k : -1);
+    }
+  }
+}
+
 // One workgroup (any multiple of 64 threads up to 1024) computes a plan.
 __device__ __forceinline__ void plan_body(const PlanSrc& src, int32_t chunk_rows, int32_t* __restrict__ seg_first,
                           int32_t* __restrict__ seg_pfirst, int32_t* __restrict__ multi,
@@ -54,17 +85,24 @@ __device__ __forceinline__ void plan_body(const PlanSrc& src, int32_t chunk_rows
     if (t == 0) *src.chunk_out = chunk_rows;
   }
   __shared__ int s_maxnch;  // most chunks of a multi-chunk segment
+  __shared__ int s_nbig;
+  __shared__ int s_big[kPlanBig][3];  // {segment, first chunk, first partial or -1}
   if (t < 3) s_carry[t] = 0;
   if (t == 0) s_maxnch = 0;
   __syncthreads();
-  for (int base = 0; base < nseg; base += nt) {
-    const int s = base + t;
-    int v[3] = {0, 0, 0};  // chunks, is-multi, partial chunks
-    if (s < nseg) {
-      const int c = plan_count(src, s);
-      v[0] = c < 0 ? 0 : (c == 0 ? 1 : (c + chunk_rows - 1) / chunk_rows);
-      v[1] = v[0] > 1;
-      v[2] = v[1] ? v[0] : 0;
+  const bool chunk_tables = src.chunk_desc || src.chunk_seg;
+  // Blocked scan: in each pass thread t takes cpt consecutive segments, so a
+  // pass covers up to blockDim * kPlanCpt segments with one workgroup scan.
+  for (int base = 0; base < nseg; base += nt * kPlanCpt) {
+    const int cpt = min(kPlanCpt, (nseg - base + nt - 1) / nt);
+    const int s0 = base + t * cpt;
+    const int s1 = min(s0 + cpt, nseg);
+    int v[3] = {0, 0, 0};  // chunks, multi-chunk segments, their chunks
+    for (int s = s0; s < s1; ++s) {
+      const int n = plan_chunks(src, s, chunk_rows);
+      v[0] += n;
+      v[1] += n > 1;
+      v[2] += n > 1 ? n : 0;
     }
     int x[3] = {v[0], v[1], v[2]};
     for (int o = 1; o < 64; o <<= 1) {
@@ -75,59 +113,46 @@ __device__ __forceinline__ void plan_body(const PlanSrc& src, int32_t chunk_rows
       }
     }
     if (lane == 63) { s_sum[0][wv] = x[0]; s_sum[1][wv] = x[1]; s_sum[2][wv] = x[2]; }
+    if (t == 0) s_nbig = 0;
     __syncthreads();
     int pre[3] = {0, 0, 0};
     for (int k = 0; k < wv; ++k) { pre[0] += s_sum[0][k]; pre[1] += s_sum[1][k]; pre[2] += s_sum[2][k]; }
     const int c0 = s_carry[0], c1 = s_carry[1], c2 = s_carry[2];
-    if (s < nseg) {
-      const int first = c0 + pre[0] + x[0] - v[0];
-      const int pfirst = c2 + pre[2] + x[2] - v[2];
-      seg_first[s] = first;
-      if (seg_pfirst) seg_pfirst[s] = pfirst;
-      if (v[1] && multi) multi[c1 + pre[1] + x[1] - 1] = s;
-      if (v[1] && src.multi_desc) {
-        src.multi_desc[c1 + pre[1] + x[1] - 1] = make_int4(pfirst, v[0], src.seg_slot[s], 0);
-        atomicMax(&s_maxnch, v[0]);
+    int r0 = c0 + pre[0] + x[0] - v[0];
+    int r1 = c1 + pre[1] + x[1] - v[1];
+    int r2 = c2 + pre[2] + x[2] - v[2];
+    for (int s = s0; s < s1; ++s) {
+      const int n = plan_chunks(src, s, chunk_rows);
+      const int pf = n > 1 ? r2 : -1;
+      seg_first[s] = r0;
+      if (seg_pfirst) seg_pfirst[s] = r2;
+      if (n > 1) {
+        if (multi) multi[r1] = s;
+        if (src.multi_desc) {
+          src.multi_desc[r1] = make_int4(r2, n, src.seg_slot[s], 0);
+          atomicMax(&s_maxnch, n);
+        }
       }
+      if (chunk_tables && n > 0) {
+        // a segment of few chunks writes them itself; larger ones are
+        // written by the whole workgroup below (or here if the list is full)
+        int q = kPlanBig;
+        if (n > kPlanOwnChunks) q = atomicAdd(&s_nbig, 1);
+        if (q < kPlanBig) {
+          s_big[q][0] = s; s_big[q][1] = r0; s_big[q][2] = pf;
+        } else {
+          plan_write_chunks(src, s, r0, n, pf, 0, 1, chunk_rows);
+        }
+      }
+      r0 += n;
+      if (n > 1) { r1 += 1; r2 += n; }
     }
-    if (src.chunk_desc || src.chunk_seg) {
-      // Chunk descriptors of this block of segments, all threads at once:
-      // the segments' tables go to LDS, then thread t takes chunks t,
-      // t + blockDim, ... and finds its segment by binary search (the last
-      // segment whose first chunk is <= c; a segment without chunks has the
-      // next one's first chunk).
-      __shared__ int s_first[1024], s_start[1024], s_end[1024], s_slot[1024], s_pf[1024];
-      if (s < nseg) {
-        s_first[t] = c0 + pre[0] + x[0] - v[0];
-        if (src.chunk_desc) {
-          const int start = src.seg_start[s];
-          s_start[t] = start;
-          s_end[t] = start + max(plan_count(src, s), 0);
-          s_slot[t] = src.seg_slot[s];
-          s_pf[t] = v[1] ? c2 + pre[2] + x[2] - v[2] : -1;
-        }
-      } else {
-        s_first[t] = 0x7fffffff;
-      }
-      __syncthreads();
-      int blk_chunks = 0;
-      for (int k = 0; k < nw; ++k) blk_chunks += s_sum[0][k];
-      const int nb = min(nt, nseg - base);
-      for (int c = c0 + t; c < c0 + blk_chunks; c += nt) {
-        int lo = 0, hi = nb;  // count of segments with first <= c
-        while (lo < hi) {
-          const int mid = (lo + hi) >> 1;
-          if (s_first[mid] <= c) lo = mid + 1;
-          else hi = mid;
-        }
-        const int j = lo - 1;
-        if (src.chunk_seg) src.chunk_seg[c] = base + j;
-        if (src.chunk_desc) {
-          const int k = c - s_first[j];
-          const int a0 = s_start[j] + k * chunk_rows;
-          src.chunk_desc[c] = make_int4(a0, min(a0 + chunk_rows, s_end[j]), s_slot[j],
-                                        s_pf[j] >= 0 ? s_pf[j] + k : -1);
-        }
+    __syncthreads();
+    if (chunk_tables) {
+      const int nbig = min(s_nbig, kPlanBig);
+      for (int q = 0; q < nbig; ++q) {
+        const int s = s_big[q][0];
+        plan_write_chunks(src, s, s_big[q][1], plan_chunks(src, s, chunk_rows), s_big[q][2], t, nt, chunk_rows);
       }
     }
     __syncthreads();
