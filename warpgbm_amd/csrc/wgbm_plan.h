@@ -27,9 +27,130 @@ __device__ __forceinline__ int plan_chunks(const PlanSrc& p, int s, int chunk_ro
   return c < 0 ? 0 : (c == 0 ? 1 : (c + chunk_rows - 1) / chunk_rows);
 }
 
-constexpr int kPlanCpt = 32;        // segments per thread per pass
-constexpr int kPlanOwnChunks = 32;  // a segment with more chunks is written by the whole workgroup
+constexpr int kPlanStage = 8192;    // shared words per pass (staged chunk counts, or segment tables)
+constexpr int kPlanLoads = 8;       // staging loads a thread issues together
+constexpr int kPlanOwnChunks = 32;  // blocked pass: a segment with more chunks is written by the whole workgroup
 constexpr int kPlanBig = 256;       // such segments listed per pass
+constexpr int kPlanStriped = 2;     // plans of at most this many segments per thread use the striped pass
+
+// Scan state shared by the passes of one plan.
+struct PlanShared {
+  int32_t* sum;    // [3][waves] per-wave totals of the current pass
+  int32_t* carry;  // [3] chunks, multi-chunk segments and their chunks before the pass
+  int* maxnch;     // most chunks of a multi-chunk segment
+  int32_t* buf;    // [kPlanStage]
+};
+
+// Exclusive workgroup scan of three per-thread values: x = inclusive wave
+// scan on return, pre = totals of the lower waves.  Ends with the per-wave
+// totals in sh.sum (one barrier).
+__device__ __forceinline__ void plan_scan3(const PlanShared& sh, const int v[3], int x[3], int pre[3]) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  x[0] = v[0]; x[1] = v[1]; x[2] = v[2];
+  for (int o = 1; o < 64; o <<= 1) {
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const int y = __shfl_up(x[q], o);
+      if (lane >= o) x[q] += y;
+    }
+  }
+  if (lane == 63) {
+    const int nw = blockDim.x >> 6;
+    sh.sum[wv] = x[0]; sh.sum[nw + wv] = x[1]; sh.sum[2 * nw + wv] = x[2];
+  }
+  __syncthreads();
+  const int nw = blockDim.x >> 6;
+  pre[0] = pre[1] = pre[2] = 0;
+  for (int k = 0; k < wv; ++k) { pre[0] += sh.sum[k]; pre[1] += sh.sum[nw + k]; pre[2] += sh.sum[2 * nw + k]; }
+}
+
+// Adds the pass's totals to the carry (after the pass's last use of it).
+__device__ __forceinline__ void plan_carry(const PlanShared& sh, int c0, int c1, int c2) {
+  __syncthreads();
+  const int nw = blockDim.x >> 6;
+  if ((int)threadIdx.x == (int)blockDim.x - 1) {
+    int tot[3] = {0, 0, 0};
+    for (int k = 0; k < nw; ++k) { tot[0] += sh.sum[k]; tot[1] += sh.sum[nw + k]; tot[2] += sh.sum[2 * nw + k]; }
+    sh.carry[0] = c0 + tot[0];
+    sh.carry[1] = c1 + tot[1];
+    sh.carry[2] = c2 + tot[2];
+  }
+  __syncthreads();
+}
+
+// Segment s's own tables: first chunk, first partial, multi-chunk list.
+__device__ __forceinline__ void plan_write_segment(const PlanShared& sh, const PlanSrc& src, int s, int n, int first,
+                                                   int pfirst, int mi, int32_t* __restrict__ seg_first,
+                                                   int32_t* __restrict__ seg_pfirst, int32_t* __restrict__ multi) {
+  seg_first[s] = first;
+  if (seg_pfirst) seg_pfirst[s] = pfirst;
+  if (n > 1) {
+    if (multi) multi[mi] = s;
+    if (src.multi_desc) {
+      src.multi_desc[mi] = make_int4(pfirst, n, src.seg_slot[s], 0);
+      atomicMax(sh.maxnch, n);
+    }
+  }
+}
+
+// Striped pass over segments [base, base + blockDim): thread t takes segment
+// base + t.  The chunk tables are written by all threads at once: the pass's
+// segment tables go to LDS, thread t takes chunks t, t + blockDim, ... and
+// finds its segment by binary search (the last segment whose first chunk is
+// <= c; a segment without chunks has the next one's first chunk).
+__device__ __forceinline__ void plan_pass_striped(const PlanShared& sh, const PlanSrc& src, int nseg, int base,
+                                                  int chunk_rows, int32_t* __restrict__ seg_first,
+                                                  int32_t* __restrict__ seg_pfirst, int32_t* __restrict__ multi) {
+  const int t = threadIdx.x, nt = blockDim.x, nw = nt >> 6;
+  const int s = base + t;
+  const int n = s < nseg ? plan_chunks(src, s, chunk_rows) : 0;
+  const int v[3] = {n, n > 1, n > 1 ? n : 0};
+  int x[3], pre[3];
+  plan_scan3(sh, v, x, pre);
+  const int c0 = sh.carry[0], c1 = sh.carry[1], c2 = sh.carry[2];
+  const int first = c0 + pre[0] + x[0] - v[0];
+  const int pfirst = c2 + pre[2] + x[2] - v[2];
+  if (s < nseg) plan_write_segment(sh, src, s, n, first, pfirst, c1 + pre[1] + x[1] - 1, seg_first, seg_pfirst, multi);
+  if (src.chunk_desc || src.chunk_seg) {
+    int32_t* s_first = sh.buf;
+    int32_t* s_start = sh.buf + 1024;
+    int32_t* s_end = sh.buf + 2048;
+    int32_t* s_slot = sh.buf + 3072;
+    int32_t* s_pf = sh.buf + 4096;
+    if (s < nseg) {
+      s_first[t] = first;
+      if (src.chunk_desc) {
+        const int start = src.seg_start[s];
+        s_start[t] = start;
+        s_end[t] = start + max(plan_count(src, s), 0);
+        s_slot[t] = src.seg_slot[s];
+        s_pf[t] = n > 1 ? pfirst : -1;
+      }
+    } else {
+      s_first[t] = 0x7fffffff;
+    }
+    __syncthreads();
+    int blk_chunks = 0;
+    for (int k = 0; k < nw; ++k) blk_chunks += sh.sum[k];
+    const int nb = min(nt, nseg - base);
+    for (int c = c0 + t; c < c0 + blk_chunks; c += nt) {
+      int lo = 0, hi = nb;  // count of segments with first <= c
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (s_first[mid] <= c) lo = mid + 1;
+        else hi = mid;
+      }
+      const int j = lo - 1;
+      if (src.chunk_seg) src.chunk_seg[c] = base + j;
+      if (src.chunk_desc) {
+        const int k = c - s_first[j];
+        const int a0 = s_start[j] + k * chunk_rows;
+        src.chunk_desc[c] = make_int4(a0, min(a0 + chunk_rows, s_end[j]), s_slot[j], s_pf[j] >= 0 ? s_pf[j] + k : -1);
+      }
+    }
+  }
+  plan_carry(sh, c0, c1, c2);
+}
 
 // Chunk tables of segment s (chunks first .. first + n - 1, partials from pf
 // or -1): chunks k = k0, k0 + step, ... get their segment and descriptor.
@@ -50,16 +171,92 @@ __device__ __forceinline__ void plan_write_chunks(const PlanSrc& src, int s, int
   }
 }
 
+// Blocked pass over up to kPlanStage segments from base: their chunk counts
+// are staged in LDS (coalesced loads, kPlanLoads in flight per thread), then
+// thread t takes cpt consecutive ones, with one workgroup scan.  A segment of
+// at most kPlanOwnChunks chunks writes its chunk tables itself; larger ones
+// are listed and written by the whole workgroup.  Returns the segments done.
+__device__ __forceinline__ int plan_pass_blocked(const PlanShared& sh, const PlanSrc& src, int nseg, int base,
+                                                 int chunk_rows, int32_t* __restrict__ seg_first,
+                                                 int32_t* __restrict__ seg_pfirst, int32_t* __restrict__ multi) {
+  __shared__ int s_nbig;
+  __shared__ int s_big[kPlanBig][3];  // {segment, first chunk, first partial or -1}
+  const int t = threadIdx.x, nt = blockDim.x;
+  int32_t* s_n = sh.buf;
+  const int span = min((kPlanStage / nt) * nt, nseg - base);
+  for (int k0 = 0; k0 < span; k0 += kPlanLoads * nt) {
+    int nv[kPlanLoads];
+#pragma unroll
+    for (int u = 0; u < kPlanLoads; ++u) {
+      const int k = k0 + u * nt + t;
+      nv[u] = k < span ? plan_chunks(src, base + k, chunk_rows) : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < kPlanLoads; ++u) {
+      const int k = k0 + u * nt + t;
+      if (k < span) s_n[k] = nv[u];
+    }
+  }
+  if (t == 0) s_nbig = 0;
+  __syncthreads();
+  const int cpt = (span + nt - 1) / nt;
+  const int s0 = base + t * cpt;
+  const int s1 = min(s0 + cpt, base + span);
+  int v[3] = {0, 0, 0};  // chunks, multi-chunk segments, their chunks
+  for (int s = s0; s < s1; ++s) {
+    const int n = s_n[s - base];
+    v[0] += n;
+    v[1] += n > 1;
+    v[2] += n > 1 ? n : 0;
+  }
+  int x[3], pre[3];
+  plan_scan3(sh, v, x, pre);
+  const int c0 = sh.carry[0], c1 = sh.carry[1], c2 = sh.carry[2];
+  int r0 = c0 + pre[0] + x[0] - v[0];
+  int r1 = c1 + pre[1] + x[1] - v[1];
+  int r2 = c2 + pre[2] + x[2] - v[2];
+  const bool chunk_tables = src.chunk_desc || src.chunk_seg;
+  for (int s = s0; s < s1; ++s) {
+    const int n = s_n[s - base];
+    const int pf = n > 1 ? r2 : -1;
+    plan_write_segment(sh, src, s, n, r0, r2, r1, seg_first, seg_pfirst, multi);
+    if (chunk_tables && n > 0) {
+      int q = kPlanBig;
+      if (n > kPlanOwnChunks) q = atomicAdd(&s_nbig, 1);
+      if (q < kPlanBig) {
+        s_big[q][0] = s; s_big[q][1] = r0; s_big[q][2] = pf;
+      } else {
+        plan_write_chunks(src, s, r0, n, pf, 0, 1, chunk_rows);
+      }
+    }
+    r0 += n;
+    if (n > 1) { r1 += 1; r2 += n; }
+  }
+  __syncthreads();
+  if (chunk_tables) {
+    const int nbig = min(s_nbig, kPlanBig);
+    for (int q = 0; q < nbig; ++q) {
+      const int s = s_big[q][0];
+      plan_write_chunks(src, s, s_big[q][1], s_n[s - base], s_big[q][2], t, nt, chunk_rows);
+    }
+  }
+  plan_carry(sh, c0, c1, c2);
+  return span;
+}
+
 // One workgroup (any multiple of 64 threads up to 1024) computes a plan.
 __device__ __forceinline__ void plan_body(const PlanSrc& src, int32_t chunk_rows, int32_t* __restrict__ seg_first,
                           int32_t* __restrict__ seg_pfirst, int32_t* __restrict__ multi,
                           int32_t* __restrict__ n_multi) {
-  __shared__ int32_t s_sum[3][1024 / kWave];
+  __shared__ int32_t s_sum[3 * (1024 / kWave)];
   __shared__ int32_t s_carry[3];
+  __shared__ int s_maxnch;
+  __shared__ int32_t s_buf[kPlanStage];
+  const PlanShared sh{s_sum, s_carry, &s_maxnch, s_buf};
   const int nseg = src.nseg_dev ? *src.nseg_dev : src.nseg;
   if (src.nseg_out && threadIdx.x == 0) *src.nseg_out = nseg;
-  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
-  const int nt = blockDim.x, nw = nt >> 6;
+  const int t = threadIdx.x, lane = t & 63;
+  const int nt = blockDim.x;
   if (src.chunk_out) {
     // Rows per chunk for this launch: enough chunks to give every slot a
     // work item (chunk_target of them), within [chunk_min, chunk_rows].  Each
@@ -71,7 +268,7 @@ __device__ __forceinline__ void plan_body(const PlanSrc& src, int32_t chunk_rows
     if (t == 0) { s_rows = 0; s_live = 0; }
     __syncthreads();
     int mine = 0, live = 0;
-    for (int s = t; s < nseg; s += blockDim.x) {
+    for (int s = t; s < nseg; s += nt) {
       const int c = max(plan_count(src, s), 0);
       mine += c;
       live += c > 0;
@@ -84,86 +281,17 @@ __device__ __forceinline__ void plan_body(const PlanSrc& src, int32_t chunk_rows
     chunk_rows = min(max((want + 63) & ~63, src.chunk_min), chunk_rows);  // whole 64-row blocks
     if (t == 0) *src.chunk_out = chunk_rows;
   }
-  __shared__ int s_maxnch;  // most chunks of a multi-chunk segment
-  __shared__ int s_nbig;
-  __shared__ int s_big[kPlanBig][3];  // {segment, first chunk, first partial or -1}
   if (t < 3) s_carry[t] = 0;
   if (t == 0) s_maxnch = 0;
   __syncthreads();
-  const bool chunk_tables = src.chunk_desc || src.chunk_seg;
-  // Blocked scan: in each pass thread t takes cpt consecutive segments, so a
-  // pass covers up to blockDim * kPlanCpt segments with one workgroup scan.
-  for (int base = 0; base < nseg; base += nt * kPlanCpt) {
-    const int cpt = min(kPlanCpt, (nseg - base + nt - 1) / nt);
-    const int s0 = base + t * cpt;
-    const int s1 = min(s0 + cpt, nseg);
-    int v[3] = {0, 0, 0};  // chunks, multi-chunk segments, their chunks
-    for (int s = s0; s < s1; ++s) {
-      const int n = plan_chunks(src, s, chunk_rows);
-      v[0] += n;
-      v[1] += n > 1;
-      v[2] += n > 1 ? n : 0;
-    }
-    int x[3] = {v[0], v[1], v[2]};
-    for (int o = 1; o < 64; o <<= 1) {
-#pragma unroll
-      for (int q = 0; q < 3; ++q) {
-        const int y = __shfl_up(x[q], o);
-        if (lane >= o) x[q] += y;
-      }
-    }
-    if (lane == 63) { s_sum[0][wv] = x[0]; s_sum[1][wv] = x[1]; s_sum[2][wv] = x[2]; }
-    if (t == 0) s_nbig = 0;
-    __syncthreads();
-    int pre[3] = {0, 0, 0};
-    for (int k = 0; k < wv; ++k) { pre[0] += s_sum[0][k]; pre[1] += s_sum[1][k]; pre[2] += s_sum[2][k]; }
-    const int c0 = s_carry[0], c1 = s_carry[1], c2 = s_carry[2];
-    int r0 = c0 + pre[0] + x[0] - v[0];
-    int r1 = c1 + pre[1] + x[1] - v[1];
-    int r2 = c2 + pre[2] + x[2] - v[2];
-    for (int s = s0; s < s1; ++s) {
-      const int n = plan_chunks(src, s, chunk_rows);
-      const int pf = n > 1 ? r2 : -1;
-      seg_first[s] = r0;
-      if (seg_pfirst) seg_pfirst[s] = r2;
-      if (n > 1) {
-        if (multi) multi[r1] = s;
-        if (src.multi_desc) {
-          src.multi_desc[r1] = make_int4(r2, n, src.seg_slot[s], 0);
-          atomicMax(&s_maxnch, n);
-        }
-      }
-      if (chunk_tables && n > 0) {
-        // a segment of few chunks writes them itself; larger ones are
-        // written by the whole workgroup below (or here if the list is full)
-        int q = kPlanBig;
-        if (n > kPlanOwnChunks) q = atomicAdd(&s_nbig, 1);
-        if (q < kPlanBig) {
-          s_big[q][0] = s; s_big[q][1] = r0; s_big[q][2] = pf;
-        } else {
-          plan_write_chunks(src, s, r0, n, pf, 0, 1, chunk_rows);
-        }
-      }
-      r0 += n;
-      if (n > 1) { r1 += 1; r2 += n; }
-    }
-    __syncthreads();
-    if (chunk_tables) {
-      const int nbig = min(s_nbig, kPlanBig);
-      for (int q = 0; q < nbig; ++q) {
-        const int s = s_big[q][0];
-        plan_write_chunks(src, s, s_big[q][1], plan_chunks(src, s, chunk_rows), s_big[q][2], t, nt, chunk_rows);
-      }
-    }
-    __syncthreads();
-    if (t == nt - 1) {
-      int tot[3] = {0, 0, 0};
-      for (int k = 0; k < nw; ++k) { tot[0] += s_sum[0][k]; tot[1] += s_sum[1][k]; tot[2] += s_sum[2][k]; }
-      s_carry[0] = c0 + tot[0];
-      s_carry[1] = c1 + tot[1];
-      s_carry[2] = c2 + tot[2];
-    }
-    __syncthreads();
+  // Few segments per thread: striped passes (one dependent load round, the
+  // chunk tables from LDS); many: blocked passes over LDS-staged counts.
+  if (nseg <= kPlanStriped * nt) {
+    for (int base = 0; base < nseg; base += nt)
+      plan_pass_striped(sh, src, nseg, base, chunk_rows, seg_first, seg_pfirst, multi);
+  } else {
+    for (int base = 0; base < nseg;)
+      base += plan_pass_blocked(sh, src, nseg, base, chunk_rows, seg_first, seg_pfirst, multi);
   }
   if (t == 0) {
     seg_first[nseg] = s_carry[0];
