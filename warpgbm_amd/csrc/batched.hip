@@ -505,7 +505,7 @@ static int enqueue_round(const wgbm_fit_config& c, const FitLayout& L, const wgb
       ra_h.seg_pfirst = I(L.h_seg_pfirst);
       root_plan = hist_plan_job(ra_h, L.hc, I(L.h_multi), I(L.h_n_multi));
     }
-    hipLaunchKernelGGL(k_root_stats_plan, dim3((unsigned)(K + (D > 0 ? 1 : 0))), dim3(1024), 0, st, I(L.stat_part), nsb,
+    hipLaunchKernelGGL(k_root_stats_plan, dim3((unsigned)(K + (D > 0 ? 1 : 0))), dim3(kPlanThreads), 0, st, I(L.stat_part), nsb,
                        I(L.tab[0].stats), K, root_plan);
     WGBM_LAUNCHED("k_root_stats_plan");
     for (int d = 0; d <= D; ++d) {

@@ -646,8 +646,13 @@ __global__ void k_set_scalar(int32_t* p, int v) {
 
 void plan_launch(const PlanSrc& src, int chunk_rows, int32_t* seg_first, int32_t* multi, int32_t* n_multi,
                  hipStream_t st) {
-  hipLaunchKernelGGL(k_hist_plan, dim3(1), dim3(1024), 0, st, src, chunk_rows, seg_first, (int32_t*)nullptr, multi,
+  hipLaunchKernelGGL(k_hist_plan, dim3(1), dim3(kPlanThreads), 0, st, src, chunk_rows, seg_first, (int32_t*)nullptr, multi,
                      n_multi);
+}
+
+void plan_job_launch(const PlanJob& j, hipStream_t st) {
+  hipLaunchKernelGGL(k_hist_plan, dim3(1), dim3(kPlanThreads), 0, st, j.src, j.chunk_rows, j.seg_first, j.seg_pfirst,
+                     j.multi, j.n_multi);
 }
 
 
@@ -896,9 +901,7 @@ int hist_launch(HistArgs a, const HistConfig& c, int64_t rows_max, int64_t nseg_
     return kTooBig;
   }
   if (!a.planned) {
-    const PlanJob j = hist_plan_job(a, c, multi, n_multi);
-    hipLaunchKernelGGL(k_hist_plan, dim3(1), dim3(1024), 0, st, j.src, j.chunk_rows, j.seg_first, j.seg_pfirst,
-                       j.multi, j.n_multi);
+    plan_job_launch(hist_plan_job(a, c, multi, n_multi), st);
     WGBM_LAUNCHED("k_hist_plan");
   }
   const int64_t max_chunks = hist_max_chunks(c, a.n_feat, rows_max, nseg_max);

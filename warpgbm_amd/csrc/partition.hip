@@ -477,7 +477,7 @@ int partition_launch(const PartArgs& a, int64_t rows_max, hipStream_t st, bool p
   hipLaunchKernelGGL(k_part_scatter, dim3((unsigned)max_chunks), dim3(256), 0, st, a);
   WGBM_LAUNCHED("k_part_scatter");
   if (a.c_stats && next_hist) {
-    hipLaunchKernelGGL(k_child_stats_plan, dim3((unsigned)(2 * a.n_nodes + 1)), dim3(1024), 0, st, a, *next_hist);
+    hipLaunchKernelGGL(k_child_stats_plan, dim3((unsigned)(2 * a.n_nodes + 1)), dim3(kPlanThreads), 0, st, a, *next_hist);
     WGBM_LAUNCHED("k_child_stats_plan");
   } else if (a.c_stats) {
     hipLaunchKernelGGL(k_child_stats, dim3((unsigned)(2 * a.n_nodes)), dim3(256), 0, st, a);
@@ -700,7 +700,7 @@ int level_tail_launch(const PartArgs* p, const LeafArgs& la, const RecordArgs& r
   WGBM_LAUNCHED("k_move_apply_record");
   if (p) {
     if (next) {
-      hipLaunchKernelGGL(k_child_stats_plan, dim3((unsigned)(2 * p->n_nodes + 1)), dim3(1024), 0, st, *p, *next);
+      hipLaunchKernelGGL(k_child_stats_plan, dim3((unsigned)(2 * p->n_nodes + 1)), dim3(kPlanThreads), 0, st, *p, *next);
       WGBM_LAUNCHED("k_child_stats_plan");
     } else {
       hipLaunchKernelGGL(k_child_stats, dim3((unsigned)(2 * p->n_nodes)), dim3(256), 0, st, *p);

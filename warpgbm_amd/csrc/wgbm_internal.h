@@ -108,8 +108,9 @@ struct LevelScanArgs {
   int64_t gain_nodes;           // nodes whose gains fit era_g / era_d
 };
 struct PlanJob;
-// leaf / part: when both are given, the per-node decisions and the level's
-// leaf and partition plans run in one single-workgroup launch.
+// leaf / part: when both are given, the level's leaf and partition plans run
+// after the per-node decisions (in the same single-workgroup launch for one
+// era and few nodes, else in launches of their own).
 int level_scan_launch(const LevelScanArgs& a, hipStream_t st, const PlanJob* leaf = nullptr,
                       const PlanJob* part = nullptr);
 int64_t level_scan_groups(int n_feat, int n_eras, int n_bins);  // candidate slots per node
@@ -166,5 +167,6 @@ struct PlanJob {
   int32_t* n_multi;
 };
 PlanJob hist_plan_job(const HistArgs& a, const HistConfig& c, int32_t* multi, int32_t* n_multi);
+void plan_job_launch(const PlanJob& j, hipStream_t st);  // the plan in a launch of its own
 
 }  // namespace wgbm

@@ -27,11 +27,14 @@ __device__ __forceinline__ int plan_chunks(const PlanSrc& p, int s, int chunk_ro
   return c < 0 ? 0 : (c == 0 ? 1 : (c + chunk_rows - 1) / chunk_rows);
 }
 
+constexpr int kPlanThreads = 1024;  // workgroup size of every plan (the kernels are launched with it)
+constexpr int kPlanWaves = kPlanThreads / kWave;
 constexpr int kPlanStage = 8192;    // shared words per pass (staged chunk counts, or segment tables)
 constexpr int kPlanLoads = 8;       // staging loads a thread issues together
 constexpr int kPlanOwnChunks = 32;  // blocked pass: a segment with more chunks is written by the whole workgroup
 constexpr int kPlanBig = 256;       // such segments listed per pass
 constexpr int kPlanStriped = 2;     // plans of at most this many segments per thread use the striped pass
+static_assert(kPlanStage % kPlanThreads == 0 && 5 * kPlanThreads <= kPlanStage, "plan LDS layout");
 
 // Scan state shared by the passes of one plan.
 struct PlanShared {
@@ -54,12 +57,9 @@ __device__ __forceinline__ void plan_scan3(const PlanShared& sh, const int v[3],
       if (lane >= o) x[q] += y;
     }
   }
-  if (lane == 63) {
-    const int nw = blockDim.x >> 6;
-    sh.sum[wv] = x[0]; sh.sum[nw + wv] = x[1]; sh.sum[2 * nw + wv] = x[2];
-  }
+  constexpr int nw = kPlanWaves;
+  if (lane == 63) { sh.sum[wv] = x[0]; sh.sum[nw + wv] = x[1]; sh.sum[2 * nw + wv] = x[2]; }
   __syncthreads();
-  const int nw = blockDim.x >> 6;
   pre[0] = pre[1] = pre[2] = 0;
   for (int k = 0; k < wv; ++k) { pre[0] += sh.sum[k]; pre[1] += sh.sum[nw + k]; pre[2] += sh.sum[2 * nw + k]; }
 }
@@ -67,9 +67,10 @@ __device__ __forceinline__ void plan_scan3(const PlanShared& sh, const int v[3],
 // Adds the pass's totals to the carry (after the pass's last use of it).
 __device__ __forceinline__ void plan_carry(const PlanShared& sh, int c0, int c1, int c2) {
   __syncthreads();
-  const int nw = blockDim.x >> 6;
-  if ((int)threadIdx.x == (int)blockDim.x - 1) {
+  constexpr int nw = kPlanWaves;
+  if ((int)threadIdx.x == kPlanThreads - 1) {
     int tot[3] = {0, 0, 0};
+#pragma unroll
     for (int k = 0; k < nw; ++k) { tot[0] += sh.sum[k]; tot[1] += sh.sum[nw + k]; tot[2] += sh.sum[2 * nw + k]; }
     sh.carry[0] = c0 + tot[0];
     sh.carry[1] = c1 + tot[1];
@@ -93,15 +94,16 @@ __device__ __forceinline__ void plan_write_segment(const PlanShared& sh, const P
   }
 }
 
-// Striped pass over segments [base, base + blockDim): thread t takes segment
+// Striped pass over segments [base, base + kPlanThreads): thread t takes segment
 // base + t.  The chunk tables are written by all threads at once: the pass's
-// segment tables go to LDS, thread t takes chunks t, t + blockDim, ... and
+// segment tables go to LDS, thread t takes chunks t, t + kPlanThreads, ... and
 // finds its segment by binary search (the last segment whose first chunk is
 // <= c; a segment without chunks has the next one's first chunk).
 __device__ __forceinline__ void plan_pass_striped(const PlanShared& sh, const PlanSrc& src, int nseg, int base,
                                                   int chunk_rows, int32_t* __restrict__ seg_first,
                                                   int32_t* __restrict__ seg_pfirst, int32_t* __restrict__ multi) {
-  const int t = threadIdx.x, nt = blockDim.x, nw = nt >> 6;
+  const int t = threadIdx.x;
+  constexpr int nt = kPlanThreads, nw = kPlanWaves;
   const int s = base + t;
   const int n = s < nseg ? plan_chunks(src, s, chunk_rows) : 0;
   const int v[3] = {n, n > 1, n > 1 ? n : 0};
@@ -131,6 +133,7 @@ __device__ __forceinline__ void plan_pass_striped(const PlanShared& sh, const Pl
     }
     __syncthreads();
     int blk_chunks = 0;
+#pragma unroll
     for (int k = 0; k < nw; ++k) blk_chunks += sh.sum[k];
     const int nb = min(nt, nseg - base);
     for (int c = c0 + t; c < c0 + blk_chunks; c += nt) {
@@ -181,9 +184,10 @@ __device__ __forceinline__ int plan_pass_blocked(const PlanShared& sh, const Pla
                                                  int32_t* __restrict__ seg_pfirst, int32_t* __restrict__ multi) {
   __shared__ int s_nbig;
   __shared__ int s_big[kPlanBig][3];  // {segment, first chunk, first partial or -1}
-  const int t = threadIdx.x, nt = blockDim.x;
+  const int t = threadIdx.x;
+  constexpr int nt = kPlanThreads;
   int32_t* s_n = sh.buf;
-  const int span = min((kPlanStage / nt) * nt, nseg - base);
+  const int span = min(kPlanStage, nseg - base);
   for (int k0 = 0; k0 < span; k0 += kPlanLoads * nt) {
     int nv[kPlanLoads];
 #pragma unroll
@@ -244,11 +248,11 @@ __device__ __forceinline__ int plan_pass_blocked(const PlanShared& sh, const Pla
   return span;
 }
 
-// One workgroup (any multiple of 64 threads up to 1024) computes a plan.
+// One workgroup of kPlanThreads threads computes a plan.
 __device__ __forceinline__ void plan_body(const PlanSrc& src, int32_t chunk_rows, int32_t* __restrict__ seg_first,
                           int32_t* __restrict__ seg_pfirst, int32_t* __restrict__ multi,
                           int32_t* __restrict__ n_multi) {
-  __shared__ int32_t s_sum[3 * (1024 / kWave)];
+  __shared__ int32_t s_sum[3 * kPlanWaves];
   __shared__ int32_t s_carry[3];
   __shared__ int s_maxnch;
   __shared__ int32_t s_buf[kPlanStage];
@@ -256,7 +260,7 @@ __device__ __forceinline__ void plan_body(const PlanSrc& src, int32_t chunk_rows
   const int nseg = src.nseg_dev ? *src.nseg_dev : src.nseg;
   if (src.nseg_out && threadIdx.x == 0) *src.nseg_out = nseg;
   const int t = threadIdx.x, lane = t & 63;
-  const int nt = blockDim.x;
+  constexpr int nt = kPlanThreads;
   if (src.chunk_out) {
     // Rows per chunk for this launch: enough chunks to give every slot a
     // work item (chunk_target of them), within [chunk_min, chunk_rows].  Each
