@@ -209,6 +209,33 @@ def test_many_eras_prebinned_batched_equals_plain_and_oracle():
     _oracle_check_tree(mb.forest[0], X, g, np.ones_like(g), era, E, 5, 5, 1e-6)
 
 
+def test_deep_levels_of_many_eras_match_oracle():
+    """400 eras at depth 6: the deepest level has 12800 (node, era) segments
+    and ~10k partition chunks, so the level's plans and partition scan run
+    their multi-pass forms (staged scans of 4096 values, gather loops of
+    several rounds).  The first tree matches the oracle and both engines
+    agree on every tree."""
+    from warpgbm_amd import WarpGBM
+    rng = np.random.default_rng(33)
+    n, f, E = 240000, 8, 400
+    X = rng.integers(0, 5, (n, f)).astype(np.int8)
+    sizes = rng.lognormal(5.0, 0.4, E)
+    sizes = np.floor(sizes / sizes.sum() * n).astype(int)
+    sizes[np.argmax(sizes)] += n - sizes.sum()
+    era = np.repeat(np.arange(E), sizes).astype(np.int32)
+    w = np.array([1.0, 0.9, 0.8, 0.7, 0.6, 0.5, 0.45, 0.4], np.float32)
+    sig = X.astype(np.float32) @ w + 0.2 * rng.standard_normal(n).astype(np.float32)
+    y = (sig / 8.0).astype(np.float32)
+    kw = dict(objective="regression", num_bins=5, max_depth=6, n_estimators=3, learning_rate=0.1)
+    mb = WarpGBM(engine="batched", **kw).fit(X, y, era_id=era)
+    mp = WarpGBM(engine="plain", **kw).fit(X, y, era_id=era)
+    for i in range(3):
+        msg = _trees_equal(mb.forest[i], mp.forest[i], f"tree{i}:")
+        assert msg is None, msg
+    g = (y - np.float32(mb.base_prediction)).astype(np.float32)
+    _oracle_check_tree(mb.forest[0], X, g, np.ones_like(g), era, E, 5, 6, 1e-6)
+
+
 def test_multiclass_round0_matches_oracle():
     """Softmax targets of the first round (device float32) match the FP64
     oracle within 1e-6, and every class tree of round 0 matches the oracle

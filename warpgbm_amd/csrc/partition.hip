@@ -84,49 +84,91 @@ k_part_count(PartArgs a) { part_count_block(a, blockIdx.x); }
 // One workgroup: per-segment left totals and chunk offsets, child segment
 // tables, smaller-child build list and sibling map.
 
+constexpr int kPsStage = 4096;  // values per staged scan pass (two arrays of them in LDS)
+constexpr int kPsLoads = 8;     // items a thread loads together in the gather loops
+
+// Exclusive scan of two values per item over n items (every thread calls
+// it): passes of kPsStage items are loaded into LDS (kPsLoads loads in flight
+// per thread, coalesced), each thread scans kPsStage / blockDim consecutive
+// ones, then put(i, prefix0, prefix1) stores coalesced.  Returns the totals.
+template <class Get, class Put>
+__device__ __forceinline__ int2 scan2_staged(int n, Get get, Put put, int* s0, int* s1) {
+  const int t = threadIdx.x, nt = blockDim.x;
+  const int cpt = kPsStage / nt;
+  int2 carry = make_int2(0, 0);
+  for (int base = 0; base < n; base += kPsStage) {
+    const int span = min(kPsStage, n - base);
+    for (int k0 = 0; k0 < span; k0 += kPsLoads * nt) {
+      int2 v[kPsLoads];
+#pragma unroll
+      for (int u = 0; u < kPsLoads; ++u) {
+        const int k = k0 + u * nt + t;
+        v[u] = k < span ? get(base + k) : make_int2(0, 0);
+      }
+#pragma unroll
+      for (int u = 0; u < kPsLoads; ++u) {
+        const int k = k0 + u * nt + t;
+        if (k < span) { s0[k] = v[u].x; s1[k] = v[u].y; }
+      }
+    }
+    __syncthreads();
+    int a0 = 0, a1 = 0;
+    for (int i = 0; i < cpt; ++i) {
+      const int j = t * cpt + i;
+      if (j < span) { a0 += s0[j]; a1 += s1[j]; }
+    }
+    int tot0, tot1;
+    int r0 = carry.x + block_exscan(a0, &tot0);
+    int r1 = carry.y + block_exscan(a1, &tot1);
+    for (int i = 0; i < cpt; ++i) {
+      const int j = t * cpt + i;
+      if (j < span) {
+        const int x0 = s0[j], x1 = s1[j];
+        s0[j] = r0; s1[j] = r1;
+        r0 += x0; r1 += x1;
+      }
+    }
+    __syncthreads();
+    for (int k = t; k < span; k += nt) put(base + k, s0[k], s1[k]);
+    carry.x += tot0;
+    carry.y += tot1;
+    __syncthreads();  // the next pass reuses the stage
+  }
+  return carry;
+}
+
 // One workgroup: per-chunk destinations, child tables and the build list.
 // Chunks of a segment are contiguous, so one scan over all chunks gives
-// every chunk its left rows before it within its segment.
+// every chunk its left rows before it within its segment.  Scans go through
+// LDS in passes; the gather loops issue kPsLoads items' loads together.
 __global__ void __launch_bounds__(1024)
 k_part_scan(PartArgs a) {
+  __shared__ int s_v0[kPsStage], s_v1[kPsStage];
   const int E = a.n_eras;
   const int nseg = a.n_nodes * E;
+  const int t = threadIdx.x, nt = blockDim.x;
   const int total = a.seg_first[nseg];
   // A: exclusive scan of the per-chunk left counts.
-  int carry = 0;
-  for (int base = 0; base < total; base += blockDim.x) {
-    const int c = base + threadIdx.x;
-    int tot;
-    const int ex = block_exscan(c < total ? a.lbase[c] : 0, &tot);
-    if (c < total) a.cscan[c] = carry + ex;
-    carry += tot;
-  }
-  if (threadIdx.x == 0) a.cscan[total] = carry;
-  __syncthreads();
-  for (int s = threadIdx.x; s < nseg; s += blockDim.x)
-    a.segL[s] = is_split(a, s / E) ? a.cscan[a.seg_first[s + 1]] - a.cscan[a.seg_first[s]] : 0;
+  const int2 A = scan2_staged(
+      total, [&](int c) { return make_int2(a.lbase[c], 0); }, [&](int c, int x, int) { a.cscan[c] = x; }, s_v0, s_v1);
+  if (t == 0) a.cscan[total] = A.x;
   __syncthreads();
   // B: child tables.  Exclusive scans over all segments of the left and
   // right counts give each era's offset inside its node's children
   // (segP[0..nseg] left, segP[nseg+1..2nseg+1] right).
   int* PL = a.segP;
   int* PR = a.segP + nseg + 1;
-  int cl = 0, cr = 0;
-  for (int base = 0; base < nseg; base += blockDim.x) {
-    const int s = base + threadIdx.x;
-    const bool sp = s < nseg && is_split(a, s / E);
-    const int L = sp ? a.segL[s] : 0;
-    const int R = sp ? max(a.seg_count[s], 0) - L : 0;
-    int tl, tr;
-    const int el = block_exscan(L, &tl);
-    const int er = block_exscan(R, &tr);
-    if (s < nseg) { PL[s] = cl + el; PR[s] = cr + er; }
-    cl += tl;
-    cr += tr;
-  }
-  if (threadIdx.x == 0) { PL[nseg] = cl; PR[nseg] = cr; }
+  const int2 B = scan2_staged(
+      nseg,
+      [&](int s) {
+        if (!is_split(a, s / E)) return make_int2(0, 0);
+        const int L = a.cscan[a.seg_first[s + 1]] - a.cscan[a.seg_first[s]];
+        return make_int2(L, max(a.seg_count[s], 0) - L);
+      },
+      [&](int s, int pl, int pr) { PL[s] = pl; PR[s] = pr; }, s_v0, s_v1);
+  if (t == 0) { PL[nseg] = B.x; PR[nseg] = B.y; }
   __syncthreads();
-  for (int j = threadIdx.x; j < a.n_nodes; j += blockDim.x) {
+  for (int j = t; j < a.n_nodes; j += nt) {
     const int l = 2 * j, r = 2 * j + 1;
     if (!is_split(a, j)) {
       a.c_active[l] = 0; a.c_active[r] = 0;
@@ -140,35 +182,69 @@ k_part_scan(PartArgs a) {
     a.c_node_start[l] = start; a.c_node_count[l] = Lt;
     a.c_node_start[r] = start + Lt; a.c_node_count[r] = a.node_count[j] - Lt;
   }
-  for (int s = threadIdx.x; s < nseg; s += blockDim.x) {
-    const int j = s / E, e = s - j * E;
-    const int l = 2 * j, r = 2 * j + 1;
-    if (!is_split(a, j)) {
-      a.c_seg_start[l * E + e] = 0; a.c_seg_count[l * E + e] = 0;
-      a.c_seg_start[r * E + e] = 0; a.c_seg_count[r * E + e] = 0;
-      continue;
+  for (int s0 = t; s0 < nseg; s0 += kPsLoads * nt) {
+    int pl0[kPsLoads], pl1[kPsLoads], pr0[kPsLoads], pr1[kPsLoads], nl0[kPsLoads], nl1[kPsLoads], nr0[kPsLoads],
+        st[kPsLoads];
+    bool sp[kPsLoads];
+#pragma unroll
+    for (int u = 0; u < kPsLoads; ++u) {
+      const int s = min(s0 + u * nt, nseg - 1);
+      const int j = s / E;
+      sp[u] = is_split(a, j);
+      pl0[u] = PL[s]; pl1[u] = PL[s + 1]; pr0[u] = PR[s]; pr1[u] = PR[s + 1];
+      nl0[u] = PL[j * E]; nl1[u] = PL[(j + 1) * E]; nr0[u] = PR[j * E];
+      st[u] = a.node_start[j];
     }
-    const int start = a.node_start[j];
-    const int Lt = PL[(j + 1) * E] - PL[j * E];
-    a.c_seg_start[l * E + e] = start + PL[s] - PL[j * E];
-    a.c_seg_count[l * E + e] = PL[s + 1] - PL[s];
-    a.c_seg_start[r * E + e] = start + Lt + PR[s] - PR[j * E];
-    a.c_seg_count[r * E + e] = PR[s + 1] - PR[s];
+#pragma unroll
+    for (int u = 0; u < kPsLoads; ++u) {
+      const int s = s0 + u * nt;
+      if (s >= nseg) break;
+      const int j = s / E, e = s - j * E;
+      const int l = 2 * j, r = 2 * j + 1;
+      if (!sp[u]) {
+        a.c_seg_start[l * E + e] = 0; a.c_seg_count[l * E + e] = 0;
+        a.c_seg_start[r * E + e] = 0; a.c_seg_count[r * E + e] = 0;
+        continue;
+      }
+      const int Lt = nl1[u] - nl0[u];
+      a.c_seg_start[l * E + e] = st[u] + pl0[u] - nl0[u];
+      a.c_seg_count[l * E + e] = pl1[u] - pl0[u];
+      a.c_seg_start[r * E + e] = st[u] + Lt + pr0[u] - nr0[u];
+      a.c_seg_count[r * E + e] = pr1[u] - pr0[u];
+    }
   }
   __syncthreads();
   // C: absolute destination bases per chunk (only split segments own chunks).
-  for (int c = threadIdx.x; c < total; c += blockDim.x) {
-    const int s = chunk_segment(a, nseg, c);
-    const int j = s / E, e = s - j * E;
-    const int first = a.seg_first[s];
-    const int lb = a.cscan[c] - a.cscan[first];
-    a.lbase[c] = lb + a.c_seg_start[(2 * j) * E + e];
-    a.rbase[c] = (c - first) * a.chunk_rows - lb + a.c_seg_start[(2 * j + 1) * E + e];
+  for (int c0 = t; c0 < total; c0 += kPsLoads * nt) {
+    int sg[kPsLoads], fi[kPsLoads], cs[kPsLoads], cf[kPsLoads], ls[kPsLoads], rs[kPsLoads];
+#pragma unroll
+    for (int u = 0; u < kPsLoads; ++u) {
+      const int c = min(c0 + u * nt, total - 1);
+      sg[u] = chunk_segment(a, nseg, c);
+      cs[u] = a.cscan[c];
+    }
+#pragma unroll
+    for (int u = 0; u < kPsLoads; ++u) {
+      const int j = sg[u] / E, e = sg[u] - j * E;
+      fi[u] = a.seg_first[sg[u]];
+      ls[u] = a.c_seg_start[(2 * j) * E + e];
+      rs[u] = a.c_seg_start[(2 * j + 1) * E + e];
+    }
+#pragma unroll
+    for (int u = 0; u < kPsLoads; ++u) cf[u] = a.cscan[fi[u]];
+#pragma unroll
+    for (int u = 0; u < kPsLoads; ++u) {
+      const int c = c0 + u * nt;
+      if (c >= total) break;
+      const int lb = cs[u] - cf[u];
+      a.lbase[c] = lb + ls[u];
+      a.rbase[c] = (c - fi[u]) * a.chunk_rows - lb + rs[u];
+    }
   }
   // D: build list (smaller child, left on ties: warpgbm/core.py:1438) in node order.
   int rank0 = 0;
-  for (int base = 0; base < a.n_nodes; base += blockDim.x) {
-    const int j = base + threadIdx.x;
+  for (int base = 0; base < a.n_nodes; base += nt) {
+    const int j = base + t;
     const bool split = j < a.n_nodes && is_split(a, j);
     int tot;
     const int rank = rank0 + block_exscan(split ? 1 : 0, &tot);
@@ -181,18 +257,32 @@ k_part_scan(PartArgs a) {
     }
     rank0 += tot;
   }
-  if (threadIdx.x == 0) {
+  if (t == 0) {
     *a.n_split = rank0;
     *a.n_build = rank0 * E;
   }
   __syncthreads();
   const int ns = rank0;
-  for (int t = threadIdx.x; t < ns * E; t += blockDim.x) {
-    const int q = t / E, e = t - q * E;
-    const int c = a.sib[3 * q + 1];
-    a.b_seg_start[t] = a.c_seg_start[c * E + e];
-    a.b_seg_count[t] = a.c_seg_count[c * E + e];
-    a.b_seg_slot[t] = c * E + e;
+  for (int q0 = t; q0 < ns * E; q0 += kPsLoads * nt) {
+    int ch[kPsLoads], ss[kPsLoads], sc[kPsLoads];
+#pragma unroll
+    for (int u = 0; u < kPsLoads; ++u) ch[u] = a.sib[3 * (min(q0 + u * nt, ns * E - 1) / E) + 1];
+#pragma unroll
+    for (int u = 0; u < kPsLoads; ++u) {
+      const int q = min(q0 + u * nt, ns * E - 1);
+      const int e = q - (q / E) * E;
+      ss[u] = a.c_seg_start[ch[u] * E + e];
+      sc[u] = a.c_seg_count[ch[u] * E + e];
+    }
+#pragma unroll
+    for (int u = 0; u < kPsLoads; ++u) {
+      const int q = q0 + u * nt;
+      if (q >= ns * E) break;
+      const int e = q - (q / E) * E;
+      a.b_seg_start[q] = ss[u];
+      a.b_seg_count[q] = sc[u];
+      a.b_seg_slot[q] = ch[u] * E + e;
+    }
   }
 }
 
