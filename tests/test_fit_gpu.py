@@ -210,11 +210,12 @@ def test_many_eras_prebinned_batched_equals_plain_and_oracle():
 
 
 def test_deep_levels_of_many_eras_match_oracle():
-    """400 eras at depth 6: the deepest level has 12800 (node, era) segments
-    and ~10k partition chunks, so the level's plans and partition scan run
-    their multi-pass forms (staged scans of 4096 values, gather loops of
-    several rounds).  The first tree matches the oracle and both engines
-    agree on every tree."""
+    """400 eras at depth 8: levels 5 and 6 have 12800 and 25600 (node, era)
+    segments and ~10k partition chunks, so the level's plans and partition
+    scan run their multi-pass forms (staged scans of 4096 values, gather
+    loops of several rounds), and the 128-node level selects its splits in
+    the fused per-era gain kernel.  The first tree matches the oracle and
+    both engines agree on every tree."""
     from warpgbm_amd import WarpGBM
     rng = np.random.default_rng(33)
     n, f, E = 240000, 8, 400
@@ -224,16 +225,16 @@ def test_deep_levels_of_many_eras_match_oracle():
     sizes[np.argmax(sizes)] += n - sizes.sum()
     era = np.repeat(np.arange(E), sizes).astype(np.int32)
     w = np.array([1.0, 0.9, 0.8, 0.7, 0.6, 0.5, 0.45, 0.4], np.float32)
-    sig = X.astype(np.float32) @ w + 0.2 * rng.standard_normal(n).astype(np.float32)
+    sig = X[:, :8].astype(np.float32) @ w + 0.2 * rng.standard_normal(n).astype(np.float32)
     y = (sig / 8.0).astype(np.float32)
-    kw = dict(objective="regression", num_bins=5, max_depth=6, n_estimators=3, learning_rate=0.1)
+    kw = dict(objective="regression", num_bins=5, max_depth=8, n_estimators=3, learning_rate=0.1)
     mb = WarpGBM(engine="batched", **kw).fit(X, y, era_id=era)
     mp = WarpGBM(engine="plain", **kw).fit(X, y, era_id=era)
     for i in range(3):
         msg = _trees_equal(mb.forest[i], mp.forest[i], f"tree{i}:")
         assert msg is None, msg
     g = (y - np.float32(mb.base_prediction)).astype(np.float32)
-    _oracle_check_tree(mb.forest[0], X, g, np.ones_like(g), era, E, 5, 6, 1e-6)
+    _oracle_check_tree(mb.forest[0], X, g, np.ones_like(g), era, E, 5, 8, 1e-6)
 
 
 def test_multiclass_round0_matches_oracle():
