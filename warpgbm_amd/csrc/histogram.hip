@@ -133,6 +133,8 @@ __device__ __forceinline__ void store_bin(const HistArgs& a, int slot, int f, in
   }
 }
 
+constexpr int kDirectStage = 8;  // words per thread of a staged single-chunk read-out
+
 // ONE_PLANE: only the T plane is allocated and H = count * qh, valid for a
 // node whose rows all have one hessian value (min h == max h).  Unless the
 // caller guarantees that for every node (a.hess_const), both variants are
@@ -357,6 +359,80 @@ k_hist_build(const HistArgs a) {
   const int f_base = slab * kWave * FPL;
   const bool direct = pidx < 0;
   constexpr int kFlushBatch = 4;  // words per thread whose sibling loads are in flight together
+  if (!WIDE && direct && SL <= kDirectStage * (int)blockDim.x) {
+    // A single-chunk segment's float32 bins go to [feature][bin] of its slot,
+    // where word order would put consecutive lanes FPL * B floats apart.  The
+    // decoded bins are staged in the freed LDS plane in output order, then
+    // the result, the parent's bins and the sibling's are read and written
+    // contiguously.  (Not in the 16-wave variant: its 64 VGPRs are kept for
+    // the row loop, and its launches have few single-chunk segments.)
+    float rg[kDirectStage], rh[kDirectStage];
+#pragma unroll
+    for (int k = 0; k < kDirectStage; ++k) {
+      const int t = threadIdx.x + k * blockDim.x;
+      rg[k] = 0.0f;
+      rh[k] = 0.0f;
+      if (t < SL) {
+        const int j = (t >> 6) / B;
+        const int word = t + j * kWave;  // skip the spare rows of features 0..j-1
+        const unsigned long long w = T[word];
+        const long long gsum = ((long long)(w << (64 - kCountShift))) >> (64 - kCountShift);
+        const long long count = (long long)((w - (unsigned long long)gsum) >> kCountShift);
+        const long long hsum = ONE_PLANE ? count * ns.qh_const : (long long)Hq[word];
+        rg[k] = from_fixed(gsum, ns.sg);
+        rh[k] = from_fixed(hsum, ns.sh);
+      }
+    }
+    __syncthreads();  // every word is read before the plane is reused
+    float* og = reinterpret_cast<float*>(s64);  // [64 * FPL][B] of the slab, then the hessians
+    float* oh = og + SL;
+#pragma unroll
+    for (int k = 0; k < kDirectStage; ++k) {
+      const int t = threadIdx.x + k * blockDim.x;
+      if (t < SL) {
+        const int r = t >> 6;
+        const int j = r / B;
+        const int o = ((t & 63) * FPL + j) * B + (r - j * B);
+        og[o] = rg[k];
+        oh[o] = rh[k];
+      }
+    }
+    __syncthreads();
+    const int n_out = min(kWave * FPL, a.n_feat - f_base) * B;
+    const int64_t d0 = ((int64_t)slot * a.n_feat + f_base) * B;
+    int64_t p0 = -1, s0 = -1;
+    if (a.sub_g) {
+      const int E = a.slots_per_node;
+      const int c = slot / E, e = slot - c * E;
+      p0 = ((int64_t)((c >> 1) * E + e) * a.n_feat + f_base) * B;
+      s0 = ((int64_t)((c ^ 1) * E + e) * a.n_feat + f_base) * B;
+    }
+    for (int o0 = threadIdx.x; o0 < n_out; o0 += kFlushBatch * blockDim.x) {
+      float pg[kFlushBatch], ph[kFlushBatch];
+      if (p0 >= 0) {
+#pragma unroll
+        for (int u = 0; u < kFlushBatch; ++u) {
+          const int o = min(o0 + u * (int)blockDim.x, n_out - 1);
+          pg[u] = a.sub_g[p0 + o];
+          ph[u] = a.sub_h[p0 + o];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < kFlushBatch; ++u) {
+        const int o = o0 + u * blockDim.x;
+        if (o >= n_out) break;
+        const float fg = og[o], fh = oh[o];
+        if (a.accumulate) { a.out_g[d0 + o] += fg; a.out_h[d0 + o] += fh; }
+        else { a.out_g[d0 + o] = fg; a.out_h[d0 + o] = fh; }
+        if (p0 >= 0) {
+          a.out_g[s0 + o] = pg[u] - fg;
+          a.out_h[s0 + o] = ph[u] - fh;
+        }
+      }
+    }
+    __syncthreads();  // the read-out is done before the next item clears the LDS
+    continue;
+  }
   for (int t0 = threadIdx.x; t0 < SL; t0 += kFlushBatch * blockDim.x) {
     SibBin sb[kFlushBatch];
     if (direct && a.sub_g) {
