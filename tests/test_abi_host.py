@@ -132,3 +132,58 @@ def test_tables_to_dicts_and_flatten_roundtrip():
     np.testing.assert_allclose(got, ref, rtol=1e-6)
     assert m._feature_of(t["feature"]) == rec.tree["feature"]
     assert m.per_era_feature_importance_[0, rec.tree["feature"]] > 0
+
+
+def test_tables_to_dicts_many_trees_and_importance_order():
+    """n rounds x K classes of random heap tables: every tree equals a
+    recursive reading of its heap (split: feature/bin/children, leaf:
+    value/rows, absent slots unused), and per-era importances are the split
+    gains added in float32 round by round, class by class, in depth-first
+    order (the order the per-node path adds them)."""
+    import torch
+    from warpgbm_amd import WarpGBM
+    rng = np.random.default_rng(4)
+    n, K, D, F, E = 5, 3, 6, 9, 2
+    NH = 2 ** (D + 1) - 1
+    st = np.zeros((n, K, NH), np.int32)
+    ft = rng.integers(0, F, (n, K, NH)).astype(np.int32)
+    bt = rng.integers(0, 31, (n, K, NH)).astype(np.int32)
+    vt = rng.standard_normal((n, K, NH)).astype(np.float32)
+    ct = rng.integers(1, 999, (n, K, NH)).astype(np.int32)
+    eg = (rng.standard_normal((n, K, NH, E)) * 1e3).astype(np.float32)
+
+    def grow(r, c, h, d):
+        if d < D and rng.random() < 0.75:
+            st[r, c, h] = 1
+            grow(r, c, 2 * h + 1, d + 1)
+            grow(r, c, 2 * h + 2, d + 1)
+        else:
+            st[r, c, h] = 2
+
+    for r in range(n):
+        for c in range(K):
+            grow(r, c, 0, 0)
+    imp = np.zeros((E, F), np.float32)
+
+    def read(r, c, h):
+        if st[r, c, h] == 1:
+            imp[:, ft[r, c, h]] += eg[r, c, h]
+            return (int(ft[r, c, h]), int(bt[r, c, h]), read(r, c, 2 * h + 1), read(r, c, 2 * h + 2))
+        return (float(vt[r, c, h]), int(ct[r, c, h]))
+
+    want = [[read(r, c, 0) for c in range(K)] for r in range(n)]
+    m = WarpGBM(max_depth=D, n_estimators=n)
+    m.objective = "multiclass"
+    m._feature_ids = torch.arange(F, dtype=torch.int32)
+    m.per_era_feature_importance_ = np.zeros((E, F), np.float32)
+    m.forest = []
+    m._trees_trained = 0
+    m._append_rounds({"status": st, "feat": ft, "bin": bt, "value": vt, "count": ct, "era_gain": eg}, n, K, None)
+
+    def as_tuple(t):
+        if "leaf_value" in t:
+            return (t["leaf_value"], t["samples"])
+        return (m._feature_of(t["feature"]), t["bin"], as_tuple(t["left"]), as_tuple(t["right"]))
+
+    assert [[as_tuple(t) for t in trees] for trees in m.forest] == want
+    np.testing.assert_array_equal(m.per_era_feature_importance_, imp)

@@ -63,6 +63,24 @@ class _Workspace:
         return b
 
 
+_PREORDER = {}
+
+
+def _heap_preorder(n_slots):
+    """Heap slots 0..n_slots-1 in depth-first preorder (left child first)."""
+    p = _PREORDER.get(n_slots)
+    if p is None:
+        out, stack = [], [0]
+        while stack:
+            h = stack.pop()
+            out.append(h)
+            if 2 * h + 2 < n_slots:
+                stack.append(2 * h + 2)
+                stack.append(2 * h + 1)
+        p = _PREORDER[n_slots] = np.asarray(out, dtype=np.int64)
+    return p
+
+
 class WarpGBM(BaseEstimator, RegressorMixin):
     def __init__(
         self,
@@ -914,41 +932,43 @@ class WarpGBM(BaseEstimator, RegressorMixin):
 
     def _append_rounds(self, tables, n, K, feats):
         """Node tables of n rounds -> the reference's nested tree dicts, and
-        per-era importances (gain of every split added to its feature)."""
+        per-era importances (gain of every split added to its feature).
+        Heap slot status: 1 split, 2 leaf, 0 not in the tree."""
         st, ft, bt, vt, ct, eg = (tables[k] for k in ("status", "feat", "bin", "value", "count", "era_gain"))
-        order = []  # (round, class, node) of every split in growth (depth-first) order
+        NH = st.shape[-1]
+        pre = _heap_preorder(NH)
+        # gains added split by split in growth (depth-first) order, as the
+        # per-node path does: rounds, then classes, then splits in preorder
+        rr, cc, pp = np.nonzero(st[:n][:, :, pre] == 1)
+        if len(rr):
+            hh = pre[pp]
+            np.add.at(self.per_era_feature_importance_.T, ft[rr, cc, hh], eg[rr, cc, hh])
         ids = self._feature_ids
         views = {}  # 0-d views into the feature-id vector, one per feature
-
-        def view(f):
-            v = views.get(f)
-            if v is None:
-                v = views[f] = ids[f]
-            return v
-
+        # Present slots of all n*K trees, flat index t = (r*K + c)*NH + h;
+        # walked backwards, so children (t + h + 1, t + h + 2) come first.
+        flat = st[:n].reshape(-1)
+        idx = np.flatnonzero(flat)
+        T, Hs, S = idx.tolist(), (idx % NH).tolist(), flat[idx].tolist()
+        Fs, Bs = ft[:n].reshape(-1)[idx].tolist(), bt[:n].reshape(-1)[idx].tolist()
+        Vs, Ns = vt[:n].reshape(-1)[idx].tolist(), ct[:n].reshape(-1)[idx].tolist()
+        nodes = {}
+        for i in range(len(T) - 1, -1, -1):
+            t, h = T[i], Hs[i]
+            if S[i] == 1:
+                f = Fs[i]
+                fv = views.get(f)
+                if fv is None:
+                    fv = views[f] = ids[f]
+                nodes[t] = {"feature": fv, "bin": Bs[i], "left": nodes[t + h + 1], "right": nodes[t + h + 2]}
+            else:
+                nodes[t] = {"leaf_value": Vs[i], "samples": Ns[i]}
         for r in range(n):
-            trees = []
-            for c in range(K):
-                s_, f_, b_ = st[r, c].tolist(), ft[r, c].tolist(), bt[r, c].tolist()
-                v_, n_ = vt[r, c].tolist(), ct[r, c].tolist()
-
-                def build(h):
-                    if s_[h] == 1:
-                        order.append((r, c, h))
-                        return {"feature": view(f_[h]), "bin": b_[h], "left": build(2 * h + 1),
-                                "right": build(2 * h + 2)}
-                    return {"leaf_value": v_[h], "samples": n_[h]}
-
-                trees.append(build(0))
+            trees = [nodes[(r * K + c) * NH] for c in range(K)]
             if self.objective == "regression":
                 self.forest[self._trees_trained + r] = trees[0]
             else:
                 self.forest.append(trees)
-        if order:
-            # gains added split by split in growth order, as the per-node path does
-            o = np.asarray(order)
-            np.add.at(self.per_era_feature_importance_.T, ft[o[:, 0], o[:, 1], o[:, 2]],
-                      eg[o[:, 0], o[:, 1], o[:, 2]])
 
     # ------------------------------------------------------------ evaluation
     def get_eval_metric(self, y_true, y_pred, sample_weight=None):
