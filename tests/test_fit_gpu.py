@@ -235,6 +235,14 @@ def test_deep_levels_of_many_eras_match_oracle():
         assert msg is None, msg
     g = (y - np.float32(mb.base_prediction)).astype(np.float32)
     _oracle_check_tree(mb.forest[0], X, g, np.ones_like(g), era, E, 5, 8, 1e-6)
+    # monotone constraints on the same levels: every era's direction must
+    # match (the fused kernel's mask), against the per-node selection op
+    kw_m = dict(kw, n_estimators=2, monotonic_constraints={0: 1, 2: -1, 5: 1})
+    mb = WarpGBM(engine="batched", **kw_m).fit(X, y, era_id=era)
+    mp = WarpGBM(engine="plain", **kw_m).fit(X, y, era_id=era)
+    for i in range(2):
+        msg = _trees_equal(mb.forest[i], mp.forest[i], f"monotone tree{i}:")
+        assert msg is None, msg
 
 
 def test_multiclass_round0_matches_oracle():
