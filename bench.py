@@ -380,7 +380,9 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (this engine runs on a "
-              "single MI355X; see DESIGN.md for how the path would shard).", file=sys.stderr)
+              "single MI355X; rows would shard across GPUs, each level's histograms reduce-scattered by "
+              "feature and the split candidates all-gathered, which this build does not implement).",
+              file=sys.stderr)
         sys.exit(2)
     import torch
     from warpgbm_amd import WarpGBM, _lib

@@ -23,7 +23,8 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 # -ffp-contract=off: every float expression is rounded as written (the
 # quantile edges and the gain scan are defined without fused multiply-add).
-# Packed-FP32 instructions are disabled (see DESIGN notes on gfx950 codegen).
+# Packed-FP32 instructions are disabled: on gfx950 some of their forms have
+# given wrong results without an error.
 FLAGS = [
     f"--offload-arch={ARCH}",
     "-O3",
