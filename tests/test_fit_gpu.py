@@ -337,3 +337,29 @@ def test_benchmark_size_trees_match_plain_and_oracle_argmax():
     left, right = check(t, np.arange(N), "root")
     check(t["left"], left, "rootL")
     check(t["right"], right, "rootR")
+
+
+def test_skewed_eras_multi_chunk_segments_match_oracle():
+    """1M rows in 200 eras, one era holding 95 % of them, depth 6: levels of
+    3200 and 6400 (node, era) segments whose giant-era segments span 29-58
+    partition chunks and several histogram chunks, so the blocked plan
+    passes write multi-chunk segments both per thread and by the whole
+    workgroup.  Both engines agree and the first tree matches the oracle."""
+    from warpgbm_amd import WarpGBM
+    rng = np.random.default_rng(8)
+    n, f, E = 1_000_000, 8, 200
+    X = rng.integers(0, 5, (n, f)).astype(np.int8)
+    big = int(0.95 * n)
+    sizes = np.full(E - 1, (n - big) // (E - 1))
+    sizes[-1] += n - big - sizes.sum()
+    era = np.concatenate([np.zeros(big, np.int32), np.repeat(np.arange(1, E), sizes).astype(np.int32)])
+    w = np.array([1.0, 0.9, 0.8, 0.7, 0.6, 0.5, 0.45, 0.4], np.float32)
+    y = ((X.astype(np.float32) @ w + 0.2 * rng.standard_normal(n).astype(np.float32)) / 8.0).astype(np.float32)
+    kw = dict(objective="regression", num_bins=5, max_depth=6, n_estimators=2, learning_rate=0.1)
+    mb = WarpGBM(engine="batched", **kw).fit(X, y, era_id=era)
+    mp = WarpGBM(engine="plain", **kw).fit(X, y, era_id=era)
+    for i in range(2):
+        msg = _trees_equal(mb.forest[i], mp.forest[i], f"tree{i}:")
+        assert msg is None, msg
+    g = (y - np.float32(mb.base_prediction)).astype(np.float32)
+    _oracle_check_tree(mb.forest[0], X, g, np.ones_like(g), era, E, 5, 6, 1e-6)

@@ -84,6 +84,27 @@ def test_histogram_every_tiling_matches_oracle(dev, cfg, subset):
     assert _rel_err(Hg, Ho, np.abs(h[rows]).sum()) < 1e-5
 
 
+@pytest.mark.parametrize("const_h", [True, False])
+def test_histogram_bins_of_many_rows(dev, const_h):
+    """Bins of 150k rows (2 bins, 300k rows, one era): with one hessian value
+    H is the exact count * h (a fixed-point count * h product wrapped past
+    2^63 from 2^16 rows); with varying hessians the per-segment fixed-point
+    sum stays within 64 bits (per-row bits follow the matrix's row count)."""
+    rng = np.random.default_rng(77)
+    N, F, B = 300_000, 3, 2
+    codes = rng.integers(0, B, size=(N, F), dtype=np.int8)
+    g = rng.standard_normal(N).astype(np.float32)
+    h = np.full(N, 0.75, np.float32) if const_h else rng.uniform(0.5, 1.0, N).astype(np.float32)
+    eras = np.zeros(N, np.int32)
+    rows = np.arange(N, dtype=np.int32)
+    Gg, Hg = _hist_gpu(dev, codes, g, h, rows, np.arange(F), eras, 1, B)
+    Go, Ho = O.histogram(codes, g, h, rows, np.arange(F), eras, 1, B)
+    if const_h:
+        np.testing.assert_array_equal(Hg, Ho.astype(np.float32))  # count * 0.75, exact
+    assert _rel_err(Hg, Ho, np.abs(h).sum()) < 1e-6
+    assert _rel_err(Gg, Go, np.abs(g).sum()) < 1e-6
+
+
 def test_histogram_counts_exact_and_drops_out_of_range(dev):
     rng = np.random.default_rng(3)
     N, F, B = 10000, 20, 16

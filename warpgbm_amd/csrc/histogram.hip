@@ -82,22 +82,27 @@ k_hist_plan(const PlanSrc src, int32_t chunk_rows, int32_t* __restrict__ seg_fir
 // Shifts of a segment's node and the partial word of one bin.
 struct NodeShift {
   int sg, sh;
-  bool count_mode;    // every row of the node has the same hessian
-  long long qh_const; // its fixed-point value in count mode
+  bool count_mode;  // every row of the node has the same hessian
+  float h_const;    // that hessian in count mode
 };
 
-__device__ __forceinline__ NodeShift node_shift(const int32_t* st) {
+__device__ __forceinline__ NodeShift node_shift(const int32_t* st, int h_bits) {
   NodeShift n;
   const float gmax = __builtin_bit_cast(float, st[0]);
   const float hmin = ord_float(st[1]);
-  const float hmax = ord_float(st[2]);
   const float habs = __builtin_bit_cast(float, st[3]);
   n.sg = fixed_shift(gmax, kGBits);
-  n.sh = fixed_shift(habs, kHBits);
+  n.sh = fixed_shift(habs, h_bits);
   n.count_mode = st[1] == st[2];
-  n.qh_const = n.count_mode ? to_fixed(hmin, n.sh) : 0;
-  (void)hmax;
+  n.h_const = hmin;
   return n;
+}
+
+// A bin's float32 hessian: count * h in count mode (exact in float64, one
+// rounding; no fixed-point product that could wrap for a bin of many rows),
+// else the fixed-point sum.
+__device__ __forceinline__ float bin_hess(const NodeShift& ns, bool count_mode, long long x) {
+  return count_mode ? (float)((double)x * (double)ns.h_const) : from_fixed(x, ns.sh);
 }
 
 // Sibling derivation: the parent's bin (loaded ahead of the result, so its
@@ -135,7 +140,7 @@ __device__ __forceinline__ void store_bin(const HistArgs& a, int slot, int f, in
 
 constexpr int kDirectStage = 8;  // words per thread of a staged single-chunk read-out
 
-// ONE_PLANE: only the T plane is allocated and H = count * qh, valid for a
+// ONE_PLANE: only the T plane is allocated and H = count * h, valid for a
 // node whose rows all have one hessian value (min h == max h).  Unless the
 // caller guarantees that for every node (a.hess_const), both variants are
 // launched over the same grid (a.dual) and each workgroup of each variant
@@ -179,7 +184,7 @@ k_hist_build(const HistArgs a) {
     first_raw = a.perm[min(w0 + lane, w1 - 1)];
     if constexpr (GH_POS) first_gh = a.gh[min(w0 + lane, w1 - 1)];
   }
-  const NodeShift ns = node_shift(a.stats + (int64_t)(slot / a.slots_per_node) * kStatWords);
+  const NodeShift ns = node_shift(a.stats + (int64_t)(slot / a.slots_per_node) * kStatWords, a.h_bits);
   if (a.dual && ONE_PLANE != ns.count_mode) continue;  // the other variant builds this node
 
   // One shared histogram per workgroup: T plane (count << 44 | sum qg) and,
@@ -378,9 +383,8 @@ k_hist_build(const HistArgs a) {
         const unsigned long long w = T[word];
         const long long gsum = ((long long)(w << (64 - kCountShift))) >> (64 - kCountShift);
         const long long count = (long long)((w - (unsigned long long)gsum) >> kCountShift);
-        const long long hsum = ONE_PLANE ? count * ns.qh_const : (long long)Hq[word];
         rg[k] = from_fixed(gsum, ns.sg);
-        rh[k] = from_fixed(hsum, ns.sh);
+        rh[k] = bin_hess(ns, ONE_PLANE, ONE_PLANE ? count : (long long)Hq[word]);
       }
     }
     __syncthreads();  // every word is read before the plane is reused
@@ -459,9 +463,9 @@ k_hist_build(const HistArgs a) {
       const unsigned long long w = T[word];
       const long long gsum = ((long long)(w << (64 - kCountShift))) >> (64 - kCountShift);
       const long long count = (long long)((w - (unsigned long long)gsum) >> kCountShift);
-      const long long hsum = ONE_PLANE ? count * ns.qh_const : (long long)Hq[word];
+      const long long hsum = ONE_PLANE ? count : (long long)Hq[word];  // count, or the fixed-point sum
       if (direct) {
-        store_bin(a, slot, f_base + fl, bin, from_fixed(gsum, ns.sg), from_fixed(hsum, ns.sh), sb[u]);
+        store_bin(a, slot, f_base + fl, bin, from_fixed(gsum, ns.sg), bin_hess(ns, ONE_PLANE, hsum), sb[u]);
       } else {
         const int64_t d = ((int64_t)pidx * a.nslab + slab) * SL + t;
         // Partials are read once, by the merge: streaming stores keep them
@@ -521,7 +525,7 @@ k_hist_merge(const HistArgs a, const int32_t* __restrict__ n_multi) {
     const int e = (r - slab * neb) * WPI + el;
     const int4 md = a.multi_desc[m];
     const int first = md.x, nch = md.y, slot = md.z;
-    const NodeShift ns = node_shift(a.stats + (int64_t)(slot / a.slots_per_node) * kStatWords);
+    const NodeShift ns = node_shift(a.stats + (int64_t)(slot / a.slots_per_node) * kStatWords, a.h_bits);
     const bool one_plane = a.hess_const || ns.count_mode;
     const int rr = e >> 6;
     const int j = rr / B;
@@ -567,8 +571,7 @@ k_hist_merge(const HistArgs a, const int32_t* __restrict__ n_multi) {
       long long sg = acc.g, sx = acc.x;
       for (int q = 1; q < P; ++q) { sg += s_g[q * WPI + el]; sx += s_h[q * WPI + el]; }
       if (f_base + fl < a.n_feat) {
-        const long long sh = one_plane ? sx * ns.qh_const : sx;
-        store_bin(a, slot, f_base + fl, bin, from_fixed(sg, ns.sg), from_fixed(sh, ns.sh), sb);
+        store_bin(a, slot, f_base + fl, bin, from_fixed(sg, ns.sg), bin_hess(ns, one_plane, sx), sb);
       }
     }
     if (P > 1) __syncthreads();
@@ -970,6 +973,7 @@ int hist_launch(HistArgs a, const HistConfig& c, int64_t rows_max, int64_t nseg_
                "histogram: missing table or scratch pointer");
   a.chunk_rows = c.chunk_rows;
   a.nslab = (int)ceil_div(a.n_feat, kWave * c.fpl);
+  a.h_bits = hess_bits(a.n_rows);
   a.wide_ok = (a.row_stride % c.fpl == 0) && ((uintptr_t)a.codes % (uintptr_t)c.fpl == 0);
   a.lds_pitch = lds_pitch(c, a.n_bins);
   if (lds_bytes(c, a.n_bins, a.hess_const != 0) > kLdsMax) {

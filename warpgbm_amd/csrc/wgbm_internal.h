@@ -45,6 +45,7 @@ struct HistArgs {
   int32_t chunk_rows;
   int32_t nslab;
   int32_t wide_ok;
+  int32_t h_bits;          // per-row hessian fixed-point bits, from n_rows (hess_bits)
   float* out_g;            // [slot][n_feat][n_bins]
   float* out_h;
   longlong2* part;         // chunk partials (exact fixed-point sums of g, h)

@@ -87,9 +87,19 @@ __device__ __forceinline__ float from_fixed(long long s, int shift) {
   return (float)ldexp((double)s, -shift);
 }
 
-// Bits of the per-row fixed-point gradient and hessian.
+// Bits of the per-row fixed-point gradient: a chunk's sum stays below 2^43
+// (packed beside the count) and any segment's below 2^63.
 constexpr int kGBits = 28;
-constexpr int kHBits = 48;
+// Bits of the per-row fixed-point hessian for a bin matrix of n_rows rows:
+// a bin's sum over any segment (at most n_rows rows) stays below 2^62, and a
+// chunk's LDS word below 2^62 at the cap of 48.  Matrices of up to 2^14 rows
+// keep all 48 bits.
+__host__ __device__ __forceinline__ int hess_bits(int64_t n_rows) {
+  int len = 0;
+  while (len < 62 && (int64_t(1) << len) <= n_rows) ++len;  // bits of n_rows
+  const int b = 62 - len;
+  return b < 48 ? (b > 16 ? b : 16) : 48;
+}
 // Packed LDS word of the gradient plane: count << 44 | gradient sum (two's
 // complement in the low 44 bits; a chunk's |sum| < 2^(28+14) < 2^43).
 constexpr int kCountShift = 44;
