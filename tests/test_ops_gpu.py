@@ -40,6 +40,7 @@ def _rel_err(got, ref, scale):
     (30000, 100, 5, 4, False),
     (3000, 1000, 32, 1, False),
     (40000, 64, 127, 1, False),
+    (60000, 6, 5, 20000, False),  # 80 KB of era counters in LDS
 ])
 def test_histogram_matches_oracle(dev, N, F, B, E, subset):
     rng = np.random.default_rng(N + F)

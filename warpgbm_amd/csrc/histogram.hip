@@ -1140,7 +1140,13 @@ extern "C" int wgbm_compute_histogram3(const int8_t* bin_indices, int64_t n_tota
     WGBM_LAUNCHED("k_single_segment");
   } else {
     const int nb = (int)ceil_div(n, kSortBlock);
-    const size_t lds = (size_t)n_eras * 4;
+    const size_t lds = (size_t)n_eras * 4;  // one counter per era (at most 128 KB)
+    if (lds > 64 * 1024) {  // beyond the default dynamic LDS limit: opt in
+      WGBM_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_era_count),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+      WGBM_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_era_scatter),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    }
     hipLaunchKernelGGL(k_era_count, dim3(nb), dim3(256), lds, stream, sample_indices, n, n_total_rows, era_indices,
                        (int)n_eras, P(L.cnt));
     WGBM_LAUNCHED("k_era_count");
