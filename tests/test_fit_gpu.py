@@ -409,3 +409,53 @@ def test_thousands_of_eras_match_plain_and_oracle():
     rec = O.grow_tree(X, g, np.ones_like(g), era, E, 5, 3, 1, 0.0, 1e-6)
     msg = _compare_dict_to_oracle(mb.forest[0], rec.tree, "root")
     assert msg is None, msg
+
+
+@pytest.mark.parametrize("B", [2, 127])
+def test_bin_count_extremes_match_plain_and_oracle(B):
+    """The smallest and largest bin counts (2 and 127): quantile binning,
+    one-feature-per-lane histograms of 128 spare-padded bins, the one-feature
+    wave scan; both engines agree and every tree matches the oracle under
+    teacher forcing."""
+    from warpgbm_amd import WarpGBM
+    X, y = _data_cfg1(n=12000, f=20, seed=B)
+    kw = dict(objective="regression", num_bins=B, max_depth=4, n_estimators=4, learning_rate=0.1)
+    mb = WarpGBM(engine="batched", **kw).fit(X, y)
+    mp = WarpGBM(engine="plain", **kw).fit(X, y)
+    for i in range(4):
+        msg = _trees_equal(mb.forest[i], mp.forest[i], f"tree{i}:")
+        assert msg is None, msg
+    codes, _ = O.bin_matrix(X, B)
+    eras = np.zeros(len(y), np.int32)
+    scores = np.full(len(y), np.float32(mb.base_prediction), np.float32)
+    for i in range(4):
+        g = (y - scores).astype(np.float32)
+        _oracle_check_tree(mb.forest[i], codes, g, np.ones_like(g), eras, 1, B, 4, 1e-6)
+        vals = O.predict_tree(_to_plain(mb.forest[i]), codes)
+        scores = (scores + (np.float32(0.1) * vals.astype(np.float32)).astype(np.float32)).astype(np.float32)
+
+
+def test_multiclass_with_eras_and_shallow_trees_match_plain():
+    """Softmax classes and eras together (slots = class x node x era), depth 1
+    stumps and a node too small to split: both engines grow the same trees
+    and give the same probabilities."""
+    from warpgbm_amd import WarpGBM
+    rng = np.random.default_rng(17)
+    n, f, K, E = 8000, 10, 3, 5
+    X = rng.standard_normal((n, f)).astype(np.float32)
+    era = rng.integers(0, E, n).astype(np.int32)
+    lab = np.argmax(X[:, :5] @ rng.standard_normal((5, K)) + rng.gumbel(size=(n, K)), axis=1)
+    for depth in (1, 3):
+        kw = dict(objective="multiclass", num_bins=16, max_depth=depth, n_estimators=3, learning_rate=0.2)
+        mb = WarpGBM(engine="batched", **kw).fit(X, lab, era_id=era)
+        mp = WarpGBM(engine="plain", **kw).fit(X, lab, era_id=era)
+        for i in range(3):
+            for k in range(K):
+                msg = _trees_equal(mb.forest[i][k], mp.forest[i][k], f"depth{depth} round{i} class{k}:")
+                assert msg is None, msg
+        np.testing.assert_array_equal(mb.per_era_feature_importance_, mp.per_era_feature_importance_)
+        assert np.array_equal(mb.predict_proba(X), mp.predict_proba(X))
+    # fewer rows than two minimum children: every tree is a single leaf
+    m = WarpGBM(objective="regression", num_bins=8, max_depth=3, n_estimators=2, engine="batched")
+    m.fit(X[:30], X[:30, 0].copy())
+    assert all("leaf_value" in t and t["samples"] == 30 for t in m.forest)
