@@ -459,3 +459,16 @@ def test_multiclass_with_eras_and_shallow_trees_match_plain():
     m = WarpGBM(objective="regression", num_bins=8, max_depth=3, n_estimators=2, engine="batched")
     m.fit(X[:30], X[:30, 0].copy())
     assert all("leaf_value" in t and t["samples"] == 30 for t in m.forest)
+
+
+def test_depth_beyond_the_level_tables_runs_per_node():
+    """max_depth 22 exceeds the batched engine's level tables (depth <= 20):
+    the fit runs on the per-node engine and grows the oracle's tree (growth
+    stops where children would fall below min_child_weight)."""
+    from warpgbm_amd import WarpGBM
+    X, y = _data_cfg1(n=3000, f=12, seed=22)
+    m = WarpGBM(objective="regression", num_bins=16, max_depth=22, n_estimators=1, learning_rate=0.1)
+    m.fit(X, y)
+    codes, _ = O.bin_matrix(X, 16)
+    g = (y - np.float32(m.base_prediction)).astype(np.float32)
+    _oracle_check_tree(m.forest[0], codes, g, np.ones_like(g), np.zeros(len(y), np.int32), 1, 16, 22, 1e-6)

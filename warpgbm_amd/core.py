@@ -656,6 +656,15 @@ class WarpGBM(BaseEstimator, RegressorMixin):
             return torch.as_tensor(X_np, device=self.device, dtype=torch.int8).contiguous()
         return self.bin_data_with_existing_edges(X_np)
 
+    def _use_plain(self):
+        """The per-node engine runs when asked for, and for trees the batched
+        engine's level tables do not hold (depth above 20, or classes x 2^depth
+        of 2^24 or more); both engines grow identical trees."""
+        if self.engine == "plain":
+            return True
+        K = self.num_classes if self.objective == "multiclass" else 1
+        return self.max_depth > 20 or (K << self.max_depth) >= (1 << 24)
+
     # ------------------------------------------------------------ plain per-node engine
     def compute_histograms(self, sample_indices, feature_indices, grad, hess):
         """G, H [E, F, B] of one node (warpgbm/core.py:1287-1308)."""
@@ -752,7 +761,7 @@ class WarpGBM(BaseEstimator, RegressorMixin):
         start = self._trees_trained if self.warm_start and self._is_fitted else 0
         if len(self.forest) < self.n_estimators:
             self.forest.extend([{} for _ in range(self.n_estimators - len(self.forest))])
-        if self.engine == "plain":
+        if self._use_plain():
             self._plain_setup(k)
             self.grad = torch.empty(self.num_samples, dtype=torch.float32, device=self.device)
             self.hess = torch.empty_like(self.grad)
@@ -785,7 +794,7 @@ class WarpGBM(BaseEstimator, RegressorMixin):
         k = self._n_sampled()
         start = self._trees_trained if self.warm_start and self._is_fitted else 0
         K = self.num_classes
-        if self.engine == "plain":
+        if self._use_plain():
             self._plain_setup(k)
             g_all = torch.empty((K, self.num_samples), dtype=torch.float32, device=self.device)
             h_all = torch.empty_like(g_all)
