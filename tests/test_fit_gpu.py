@@ -472,3 +472,23 @@ def test_depth_beyond_the_level_tables_runs_per_node():
     codes, _ = O.bin_matrix(X, 16)
     g = (y - np.float32(m.base_prediction)).astype(np.float32)
     _oracle_check_tree(m.forest[0], codes, g, np.ones_like(g), np.zeros(len(y), np.int32), 1, 16, 22, 1e-6)
+
+
+def test_column_sampling_with_eras_batched_equals_plain():
+    """colsample_bytree < 1 with eras: each round's feature subset (the same
+    seeded draws in both engines) indexes the per-era histograms, gains and
+    importances; the engines grow identical trees."""
+    from warpgbm_amd import WarpGBM
+    rng = np.random.default_rng(41)
+    n, f, E = 9000, 16, 3
+    X = rng.standard_normal((n, f)).astype(np.float32)
+    era = rng.integers(0, E, n).astype(np.int32)
+    y = (X[:, 0] + 0.7 * X[:, 3] - 0.5 * X[:, 7] + 0.2 * rng.standard_normal(n)).astype(np.float32)
+    kw = dict(objective="regression", num_bins=16, max_depth=3, n_estimators=6, learning_rate=0.2,
+              colsample_bytree=0.5, random_state=5)
+    mb = WarpGBM(engine="batched", **kw).fit(X, y, era_id=era)
+    mp = WarpGBM(engine="plain", **kw).fit(X, y, era_id=era)
+    for i in range(6):
+        msg = _trees_equal(mb.forest[i], mp.forest[i], f"tree{i}:")
+        assert msg is None, msg
+    np.testing.assert_array_equal(mb.per_era_feature_importance_, mp.per_era_feature_importance_)
