@@ -394,12 +394,12 @@ k_level_gains(const LevelScanArgs a, int node0, int n_group) {
 // and directions in era order (eval_cand's order) and applies the monotone
 // mask; one candidate per workgroup of kSelBlock thresholds.
 __global__ void __launch_bounds__(kSelBlock)
-k_level_sel(const LevelScanArgs a, int node0, int n_group, int groups) {
+k_level_sel(const LevelScanArgs a, int node0, int n_group) {
   const int E = a.n_eras, F = a.n_feat, B = a.n_bins;
   const int FB = F * (B - 1);
-  const int sel_groups = (FB + kSelBlock - 1) / kSelBlock;  // workgroups per node; slots per node: groups
-  const int jl = blockIdx.x / sel_groups;
-  const int wg = blockIdx.x - jl * sel_groups;
+  const int groups = (FB + kSelBlock - 1) / kSelBlock;  // workgroups (candidate slots) per node
+  const int jl = blockIdx.x / groups;
+  const int wg = blockIdx.x - jl * groups;
   const int node = node0 + jl;
   const int c = wg * kSelBlock + threadIdx.x;
   Cand best = cand_none();
@@ -446,15 +446,6 @@ k_level_sel(const LevelScanArgs a, int node0, int n_group, int groups) {
     a.cand_agree[o] = b.agree;
     a.cand_crit[o] = b.crit;
     a.cand_idx[o] = b.idx;
-  }
-  if (wg == 0 && jl < n_group) {
-    const Cand none = cand_none();
-    for (int w = sel_groups + (int)threadIdx.x; w < groups; w += kSelBlock) {
-      const int64_t o = (int64_t)node * groups + w;
-      a.cand_agree[o] = none.agree;
-      a.cand_crit[o] = none.crit;
-      a.cand_idx[o] = none.idx;
-    }
   }
 }
 
@@ -554,9 +545,7 @@ int level_scan_launch(const LevelScanArgs& a, hipStream_t st, const PlanJob* lea
       const int64_t th = (int64_t)ng * a.n_eras * a.n_feat;
       hipLaunchKernelGGL(k_level_gains, dim3((unsigned)ceil_div(th, 256)), dim3(256), 0, st, a, (int)n0, ng);
       WGBM_LAUNCHED("k_level_gains");
-      const int64_t sel_groups = ceil_div((int64_t)a.n_feat * (a.n_bins - 1), kSelBlock);
-      hipLaunchKernelGGL(k_level_sel, dim3((unsigned)(ng * sel_groups)), dim3(kSelBlock), 0, st, a, (int)n0, ng,
-                         (int)groups);
+      hipLaunchKernelGGL(k_level_sel, dim3((unsigned)(ng * groups)), dim3(kSelBlock), 0, st, a, (int)n0, ng);
       WGBM_LAUNCHED("k_level_sel");
     }
   }
