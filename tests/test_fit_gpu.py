@@ -492,3 +492,37 @@ def test_column_sampling_with_eras_batched_equals_plain():
         msg = _trees_equal(mb.forest[i], mp.forest[i], f"tree{i}:")
         assert msg is None, msg
     np.testing.assert_array_equal(mb.per_era_feature_importance_, mp.per_era_feature_importance_)
+
+
+def test_weights_many_classes_and_wide_sampling_batched_equals_plain():
+    """Sample weights with softmax (two hessian planes) and with eras;
+    20 classes (20 trees per round in one level launch); 1000 features
+    sampled at 30 % per round: both engines grow identical trees."""
+    from warpgbm_amd import WarpGBM
+    rng = np.random.default_rng(53)
+    n, f = 6000, 12
+    X = rng.standard_normal((n, f)).astype(np.float32)
+    w = rng.uniform(0.2, 3.0, n).astype(np.float32)
+    era = rng.integers(0, 4, n).astype(np.int32)
+    cases = []
+    lab3 = np.argmax(X[:, :4] @ rng.standard_normal((4, 3)) + rng.gumbel(size=(n, 3)), axis=1)
+    cases.append((dict(objective="multiclass", num_bins=16, max_depth=3, n_estimators=3), X, lab3,
+                  dict(sample_weight=w)))
+    y = (X[:, 0] - 0.5 * X[:, 2] + 0.2 * rng.standard_normal(n)).astype(np.float32)
+    cases.append((dict(objective="regression", num_bins=16, max_depth=3, n_estimators=3), X, y,
+                  dict(sample_weight=w, era_id=era)))
+    lab20 = np.argmax(X[:, :6] @ rng.standard_normal((6, 20)) + rng.gumbel(size=(n, 20)), axis=1)
+    cases.append((dict(objective="multiclass", num_bins=8, max_depth=2, n_estimators=2), X, lab20, {}))
+    Xw = rng.standard_normal((3000, 1000)).astype(np.float32)
+    yw = (Xw[:, 5] + Xw[:, 500] - Xw[:, 999] + 0.1 * rng.standard_normal(3000)).astype(np.float32)
+    cases.append((dict(objective="regression", num_bins=8, max_depth=3, n_estimators=4, colsample_bytree=0.3,
+                       random_state=3), Xw, yw, {}))
+    for ci, (kw, Xc, yc, fk) in enumerate(cases):
+        mb = WarpGBM(engine="batched", learning_rate=0.2, **kw).fit(Xc, yc, **fk)
+        mp = WarpGBM(engine="plain", learning_rate=0.2, **kw).fit(Xc, yc, **fk)
+        for i in range(kw["n_estimators"]):
+            tb, tp = mb.forest[i], mp.forest[i]
+            pairs = zip(tb, tp) if kw["objective"] == "multiclass" else [(tb, tp)]
+            for k, (a, b) in enumerate(pairs):
+                msg = _trees_equal(a, b, f"case{ci} tree{i} class{k}:")
+                assert msg is None, msg
