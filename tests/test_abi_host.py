@@ -187,3 +187,21 @@ def test_tables_to_dicts_many_trees_and_importance_order():
 
     assert [[as_tuple(t) for t in trees] for trees in m.forest] == want
     np.testing.assert_array_equal(m.per_era_feature_importance_, imp)
+
+
+def test_engine_routing_by_depth_and_classes():
+    """The batched engine takes every tree its level tables hold (depth <= 20
+    and classes x 2^depth < 2^24); deeper trees and engine="plain" run per
+    node."""
+    from warpgbm_amd import WarpGBM
+    m = WarpGBM(max_depth=20)
+    m.objective, m.num_classes = "regression", None
+    assert not m._use_plain()
+    m.max_depth = 21
+    assert m._use_plain()
+    m = WarpGBM(max_depth=20, objective="multiclass")
+    m.num_classes = 16  # 16 << 20 == 2^24
+    assert m._use_plain()
+    m.num_classes = 15
+    assert not m._use_plain()
+    assert WarpGBM(max_depth=3, engine="plain")._use_plain()
