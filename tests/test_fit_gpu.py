@@ -526,3 +526,31 @@ def test_weights_many_classes_and_wide_sampling_batched_equals_plain():
             for k, (a, b) in enumerate(pairs):
                 msg = _trees_equal(a, b, f"case{ci} tree{i} class{k}:")
                 assert msg is None, msg
+
+
+def test_missing_values_bin_and_predict_like_the_oracle():
+    """Float inputs with NaN: a column holding any NaN, even one, gets NaN
+    quantile edges (as torch.quantile gives) and every value of it code 0.
+    Device codes equal the oracle's bit for bit, both engines grow the same
+    trees and predictions match the oracle's forest on the same codes."""
+    from warpgbm_amd import WarpGBM
+    X, y = _data_cfg1(n=6000, f=12, seed=61)
+    rng = np.random.default_rng(61)
+    X[rng.random(len(X)) < 0.02, 3] = np.nan
+    X[17, 5] = np.nan
+    kw = dict(objective="regression", num_bins=16, max_depth=3, n_estimators=3, learning_rate=0.1)
+    mb = WarpGBM(engine="batched", **kw).fit(X, y)
+    mp = WarpGBM(engine="plain", **kw).fit(X, y)
+    for i in range(3):
+        msg = _trees_equal(mb.forest[i], mp.forest[i], f"tree{i}:")
+        assert msg is None, msg
+    codes, edges = O.bin_matrix(X, 16)
+    for col in (3, 5):
+        assert np.isnan(edges[col]).all() and (codes[:, col] == 0).all()
+    be = mb.bin_edges.cpu().numpy() if torch.is_tensor(mb.bin_edges) else np.asarray(mb.bin_edges)
+    np.testing.assert_array_equal(be, edges)  # NaN where the oracle has NaN
+    Xd = torch.from_numpy(X).to("cuda")
+    np.testing.assert_array_equal(mb._bin_matrix(Xd, torch.from_numpy(edges).to("cuda")).cpu().numpy(), codes)
+    flats = [O.flatten_tree(_to_plain(t), 2 ** 4) for t in mb.forest]
+    ref = O.predict_forest_flat(codes, flats, 0.1, base=mb.base_prediction)
+    np.testing.assert_allclose(mb.predict(X), ref, rtol=1e-5, atol=1e-5)
