@@ -239,3 +239,20 @@ def test_eval_set_and_early_stopping(engine):
     ref = WarpGBM(num_bins=32, max_depth=4, n_estimators=200, learning_rate=0.3, engine="plain")
     ref.fit(X, y, X_eval=Xe, y_eval=ye, eval_every_n_trees=1, early_stopping_rounds=3)
     assert ref._trees_trained == m._trees_trained
+
+
+def test_integer_codes_of_any_width_and_empty_integer_input():
+    """Pre-binned codes given as int32 train and predict exactly like the
+    same int8 codes (the pre-binned bypass keys on integer dtype and
+    max < num_bins); zero rows of integer input raise ValueError at the
+    pre-binned check, as the reference's does (warpgbm/core.py:1967-1969)."""
+    from warpgbm_amd import WarpGBM
+    rng = np.random.default_rng(71)
+    X8 = rng.integers(0, 6, (4000, 9)).astype(np.int8)
+    y = (X8[:, 0] - 0.5 * X8[:, 4] + 0.3 * rng.standard_normal(4000)).astype(np.float32)
+    kw = dict(objective="regression", num_bins=6, max_depth=3, n_estimators=3, learning_rate=0.2)
+    m8 = WarpGBM(**kw).fit(X8, y)
+    m32 = WarpGBM(**kw).fit(X8.astype(np.int32), y)
+    np.testing.assert_array_equal(m8.predict(X8), m32.predict(X8.astype(np.int32)))
+    with pytest.raises(ValueError):
+        m8.predict(X8[:0])
