@@ -256,3 +256,22 @@ def test_integer_codes_of_any_width_and_empty_integer_input():
     np.testing.assert_array_equal(m8.predict(X8), m32.predict(X8.astype(np.int32)))
     with pytest.raises(ValueError):
         m8.predict(X8[:0])
+
+
+def test_staged_host_copies_equal_direct_copies(monkeypatch):
+    """Host arrays copied through the pinned staging buffers (here with 4 KB
+    chunks, so ragged last chunks and many buffer reuses occur) equal direct
+    copies for float32, float64 -> float32, int32 and int8 data."""
+    import warpgbm_amd.core as core
+    monkeypatch.setattr(core, "_STAGE_BYTES", 4096)
+    dev = torch.device("cuda")
+    rng = np.random.default_rng(5)
+    cases = [(rng.standard_normal((3001, 7)).astype(np.float32), torch.float32),
+             (rng.standard_normal((2999, 5)), torch.float32),
+             (rng.integers(-9, 9, (4097, 3)).astype(np.int32), None),
+             (rng.integers(0, 5, (12345, 2)).astype(np.int8), None)]
+    for a, dt in cases:
+        got = core._to_device(a, dev, dt).cpu()
+        ref = torch.from_numpy(a).to(dt if dt is not None else torch.from_numpy(a).dtype)
+        assert got.dtype == ref.dtype and got.shape == ref.shape
+        assert torch.equal(got, ref)

@@ -63,6 +63,43 @@ class _Workspace:
         return b
 
 
+_STAGE_BYTES = 64 << 20  # pinned staging chunk for large host-to-device copies
+
+
+def _to_device(X_np, device, dtype=torch.float32):
+    """Copy of a host array on the device (converted to dtype on the host
+    first when given).  Arrays of four or more staging chunks go through two
+    pinned buffers in chunks of _STAGE_BYTES, so the host-side copy of one
+    chunk overlaps the DMA of the previous one (a pageable copy stages
+    through the driver's buffers one at a time)."""
+    device = torch.device(device)
+    src = torch.from_numpy(np.ascontiguousarray(X_np))
+    if dtype is not None and src.dtype != dtype:
+        src = src.to(dtype)
+    es = src.element_size()
+    if device.type != "cuda" or src.numel() * es < 4 * _STAGE_BYTES:
+        return src.to(device)
+    out = torch.empty(src.shape, dtype=src.dtype, device=device)
+    flat_src, flat_out = src.reshape(-1), out.reshape(-1)
+    step = max(_STAGE_BYTES // es, 1)
+    bufs = [torch.empty(step, dtype=src.dtype, pin_memory=True) for _ in range(2)]
+    done = [None, None]
+    stream = torch.cuda.current_stream(device)
+    for i, off in enumerate(range(0, flat_src.numel(), step)):
+        k = i & 1
+        if done[k] is not None:
+            done[k].synchronize()  # the buffer's previous DMA has finished
+        m = min(step, flat_src.numel() - off)
+        bufs[k][:m].copy_(flat_src[off:off + m])
+        flat_out[off:off + m].copy_(bufs[k][:m], non_blocking=True)
+        done[k] = torch.cuda.Event()
+        done[k].record(stream)
+    for e in done:
+        if e is not None:
+            e.synchronize()  # the staging buffers are released below
+    return out
+
+
 _PREORDER = {}
 
 
@@ -604,10 +641,10 @@ class WarpGBM(BaseEstimator, RegressorMixin):
             unique_eras = torch.from_numpy(uniq).to(self.device)
             if np.issubdtype(X_np.dtype, np.integer) and np.all(X_np.max(axis=0) < self.num_bins):
                 print("Detected pre-binned integer input — skipping quantile binning.")
-                bin_indices = torch.as_tensor(X_np, device=self.device).to(torch.int8).contiguous()
+                bin_indices = _to_device(X_np, self.device, None).to(torch.int8).contiguous()
                 bin_edges = torch.arange(1, self.num_bins, dtype=torch.float32).repeat(F, 1).to(self.device)
                 return bin_indices, era_indices, bin_edges, unique_eras, Y_gpu
-            X_float = torch.as_tensor(X_np, device=self.device, dtype=torch.float32).contiguous()
+            X_float = _to_device(X_np, self.device)
             bin_edges = self._quantile_edges(X_float)
             bin_indices = self._bin_matrix(X_float, bin_edges)
             del X_float
