@@ -678,7 +678,7 @@ class WarpGBM(BaseEstimator, RegressorMixin):
 
     def bin_data_with_existing_edges(self, X_np):
         """Inference binning with stored edges, one launch (core.py:1882-1900)."""
-        X_float = torch.as_tensor(np.ascontiguousarray(X_np), device=self.device, dtype=torch.float32)
+        X_float = _to_device(X_np, self.device)
         edges = self.bin_edges.to(self.device).contiguous()
         return self._bin_matrix(X_float.contiguous(), edges)
 
@@ -690,7 +690,7 @@ class WarpGBM(BaseEstimator, RegressorMixin):
                 print("Detected pre-binned input at predict-time — skipping binning.")
                 pre = True
         if pre:
-            return torch.as_tensor(X_np, device=self.device, dtype=torch.int8).contiguous()
+            return _to_device(X_np, self.device, None).to(torch.int8).contiguous()
         return self.bin_data_with_existing_edges(X_np)
 
     def _use_plain(self):
