@@ -554,3 +554,54 @@ def test_missing_values_bin_and_predict_like_the_oracle():
     flats = [O.flatten_tree(_to_plain(t), 2 ** 4) for t in mb.forest]
     ref = O.predict_forest_flat(codes, flats, 0.1, base=mb.base_prediction)
     np.testing.assert_allclose(mb.predict(X), ref, rtol=1e-5, atol=1e-5)
+
+
+def test_random_configurations_batched_equals_plain_and_oracle():
+    """24 seeded random configurations (objective, rows, features, bins 2..127,
+    depth 1..6, eras, weights, column sampling, monotone constraints,
+    min_child_weight, float or pre-binned inputs): both engines grow the
+    same trees, and unweighted regression without sampling or constraints
+    grows the oracle's first tree."""
+    import contextlib
+    import io
+    from warpgbm_amd import WarpGBM
+    rng = np.random.default_rng(2024)
+    for ci in range(24):
+        obj = "multiclass" if rng.random() < 0.25 else "regression"
+        N, F = int(rng.integers(200, 8000)), int(rng.integers(1, 30))
+        B = int(rng.choice([2, 3, 5, 8, 16, 31, 32, 64, 127]))
+        D, E = int(rng.integers(1, 7)), int(rng.choice([1, 1, 2, 3, 7, 40]))
+        K = int(rng.integers(2, 6)) if obj == "multiclass" else 1
+        pre = rng.random() < 0.3
+        X = rng.integers(0, B, (N, F)).astype(np.int8) if pre else rng.standard_normal((N, F)).astype(np.float32)
+        era = rng.integers(0, E, N).astype(np.int32)
+        w = rng.uniform(0.1, 3.0, N).astype(np.float32) if rng.random() < 0.3 else None
+        cs = float(rng.choice([1.0, 1.0, 0.5]))
+        mono = {0: int(rng.choice([-1, 1]))} if obj == "regression" and rng.random() < 0.2 else None
+        mcw = int(rng.choice([1, 5, 20]))
+        Xf = X.astype(np.float32)
+        if obj == "regression":
+            y = (Xf[:, 0] - 0.5 * Xf[:, min(1, F - 1)] + 0.3 * rng.standard_normal(N)).astype(np.float32)
+        else:
+            y = np.arange(N) % K  # every class present
+            rng.shuffle(y)
+        kw = dict(objective=obj, num_bins=B, max_depth=D, n_estimators=2, learning_rate=0.2, colsample_bytree=cs,
+                  random_state=ci, min_child_weight=mcw)
+        if mono:
+            kw["monotonic_constraints"] = mono
+        fk = dict(era_id=era) if w is None else dict(era_id=era, sample_weight=w)
+        with contextlib.redirect_stdout(io.StringIO()):
+            mb = WarpGBM(engine="batched", **kw).fit(X, y, **fk)
+            mp = WarpGBM(engine="plain", **kw).fit(X, y, **fk)
+        for i in range(2):
+            pairs = zip(mb.forest[i], mp.forest[i]) if obj == "multiclass" else [(mb.forest[i], mp.forest[i])]
+            for k, (a, b) in enumerate(pairs):
+                msg = _trees_equal(a, b, f"case {ci} ({obj} N={N} F={F} B={B} D={D} E={E}) tree{i} class{k}:")
+                assert msg is None, msg
+        if obj == "regression" and w is None and cs == 1.0 and mono is None:
+            codes, _ = O.bin_matrix(X, B)
+            g = (y - np.float32(mb.base_prediction)).astype(np.float32)
+            eu = np.unique(era, return_inverse=True)[1].astype(np.int32)
+            rec = O.grow_tree(codes, g, np.ones_like(g), eu, int(eu.max()) + 1, B, D, mcw, 0.0, 1e-6)
+            msg = _compare_dict_to_oracle(mb.forest[0], rec.tree, f"case {ci} root")
+            assert msg is None, msg
