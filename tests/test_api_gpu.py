@@ -275,3 +275,25 @@ def test_staged_host_copies_equal_direct_copies(monkeypatch):
         ref = torch.from_numpy(a).to(dt if dt is not None else torch.from_numpy(a).dtype)
         assert got.dtype == ref.dtype and got.shape == ref.shape
         assert torch.equal(got, ref)
+
+
+def test_autotune_installs_a_config_and_trees_do_not_change(tmp_path, monkeypatch):
+    """autotune.tune times every candidate histogram configuration for a
+    shape, installs the fastest and caches it; the trees grown under it are
+    the trees of the automatic configuration (exact histograms), and a
+    second call is served from the cache."""
+    from warpgbm_amd import WarpGBM, autotune
+    monkeypatch.setenv("WARPGBM_AMD_TUNE_CACHE", str(tmp_path / "tune.json"))
+    X, y = _reg(n=6000, f=16, seed=9)
+    kw = dict(objective="regression", num_bins=32, max_depth=4, n_estimators=3, learning_rate=0.1)
+    ref = WarpGBM(**kw).fit(X, y)
+    try:
+        best, times = autotune.tune(16, 32, rows=20000, use_cache=True)
+        assert best in times and len(times) >= 2
+        tuned = WarpGBM(**kw).fit(X, y)
+        assert np.array_equal(ref.predict(X), tuned.predict(X))
+        again, times2 = autotune.tune(16, 32, rows=20000, use_cache=True)
+        assert again == best and times2 == {}
+        assert (tmp_path / "tune.json").exists()
+    finally:
+        autotune.reset()
