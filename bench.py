@@ -42,6 +42,8 @@ def parse():
     p.add_argument("--profile-only", action="store_true", help="one fit, no extras (for rocprof)")
     p.add_argument("--no-serve", action="store_true", help="skip the served-predict measurement")
     p.add_argument("--no-suite", action="store_true", help="skip the many-era and multiclass workloads")
+    p.add_argument("--suite-only", action="store_true",
+                   help="run only the many-era and multiclass workloads (for rocprof); prints their JSON")
     p.add_argument("--no-host-fit", action="store_true", help="skip the fit from host memory")
     p.add_argument("--no-graph", action="store_true",
                    help="launch every round eagerly (no hipGraph replay), e.g. under counter collection")
@@ -391,6 +393,9 @@ def main():
     dev = torch.device("cuda:0")
     if args.no_graph:
         _lib.call("wgbm_set_fit_graph", 0)
+    if args.suite_only:
+        print(json.dumps({"suite": suite(dev)}), flush=True)
+        return
     log(f"device {torch.cuda.get_device_name(0)}; generating {args.rows}x{args.features}")
     X, y = make_data(args, dev)
     t0 = time.perf_counter()

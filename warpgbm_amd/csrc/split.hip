@@ -531,8 +531,8 @@ int level_scan_launch(const LevelScanArgs& a, hipStream_t st, const PlanJob* lea
     WGBM_LAUNCHED("k_level_scan");
   } else if (a.n_bins >= 2 && a.n_bins <= 8 && a.n_nodes >= kGainSelMinNodes) {
     // many nodes: the fused selection; fewer: per-era gains, then selection
-    // (200 eras x 400 features: 179 us against 292 at 128 nodes, about even
-    // at 64)
+    // (200 eras x 400 features: 179-205 us against ~290 at 128 nodes, about
+    // even at 64)
     const int groups = (int)level_scan_groups(a.n_feat, a.n_eras, a.n_bins);
     hipLaunchKernelGGL(k_level_gainsel<8>, dim3((unsigned)((int64_t)a.n_nodes * groups)), dim3(kSelBlock), 0, st, a,
                        groups);
