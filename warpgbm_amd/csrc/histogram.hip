@@ -82,7 +82,8 @@ k_hist_plan(const PlanSrc src, int32_t chunk_rows, int32_t* __restrict__ seg_fir
 // Shifts of a segment's node and the partial word of one bin.
 struct NodeShift {
   int sg, sh;
-  float h_const;    // the node's hessian, used when every row has it (one plane)
+  bool count_mode;  // every row of the node has the same hessian
+  float h_const;    // that hessian (used by the one-plane variant)
 };
 
 __device__ __forceinline__ NodeShift node_shift(const int32_t* st, int h_bits) {
@@ -92,6 +93,7 @@ __device__ __forceinline__ NodeShift node_shift(const int32_t* st, int h_bits) {
   const float habs = __builtin_bit_cast(float, st[3]);
   n.sg = fixed_shift(gmax, kGBits);
   n.sh = fixed_shift(habs, h_bits);
+  n.count_mode = st[1] == st[2];
   n.h_const = hmin;
   return n;
 }
@@ -139,8 +141,10 @@ __device__ __forceinline__ void store_bin(const HistArgs& a, int slot, int f, in
 constexpr int kDirectStage = 8;  // words per thread of a staged single-chunk read-out
 
 // ONE_PLANE: only the T plane is allocated and H = count * h, launched when
-// the caller guarantees one hessian value for every row (a.hess_const); the
-// two-plane variant builds every node otherwise.
+// the caller guarantees one hessian value for every row (a.hess_const);
+// otherwise the two-plane variant builds every node, or with a.dual both
+// variants run over the same grid and each workgroup builds only the nodes
+// whose statistics select its variant.
 // WIDE: 16-wave workgroups, two per CU: every SIMD then runs 8 waves (the
 // second bound is waves per execution unit), so the kernel must fit in 64
 // VGPRs.
@@ -181,6 +185,7 @@ k_hist_build(const HistArgs a) {
     if constexpr (GH_POS) first_gh = a.gh[min(w0 + lane, w1 - 1)];
   }
   const NodeShift ns = node_shift(a.stats + (int64_t)(slot / a.slots_per_node) * kStatWords, a.h_bits);
+  if (a.dual && ONE_PLANE != ns.count_mode) continue;  // the other variant builds this node
 
   // One shared histogram per workgroup: T plane (count << 44 | sum qg) and,
   // unless the hessian is constant, an H plane (sum qh).  A plane holds
@@ -521,7 +526,7 @@ k_hist_merge(const HistArgs a, const int32_t* __restrict__ n_multi) {
     const int4 md = a.multi_desc[m];
     const int first = md.x, nch = md.y, slot = md.z;
     const NodeShift ns = node_shift(a.stats + (int64_t)(slot / a.slots_per_node) * kStatWords, a.h_bits);
-    const bool one_plane = a.hess_const != 0;  // the variant that built the partials
+    const bool one_plane = a.hess_const || (a.dual && ns.count_mode);  // the variant that built the partials
     const int rr = e >> 6;
     const int j = rr / B;
     const int bin = rr - j * B;
@@ -902,8 +907,13 @@ static int launch_fpl(const HistArgs& a, const HistConfig& c, int64_t max_chunks
   // h * 2^sh = mantissa(h) * 2^(h_bits - 24), an integer for h_bits >= 24, so
   // the bin is count * h exactly, rounded once: the one-plane variant's value
   // bit for bit, without a second launch for such nodes.
-  const int rc = a.hess_const ? launch_mode<FPL, L, true>(a, c, groups, st)
-                              : launch_mode<FPL, L, false>(a, c, groups, st);
+  int rc;
+  if (a.hess_const) {
+    rc = launch_mode<FPL, L, true>(a, c, groups, st);
+  } else {
+    rc = a.dual ? launch_mode<FPL, L, true>(a, c, groups, st) : kOk;
+    if (rc == kOk) rc = launch_mode<FPL, L, false>(a, c, groups, st);
+  }
   if (rc != kOk) return rc;
   if (max_multi > 0) {
     const int SL = kWave * FPL * a.n_bins;
@@ -1120,6 +1130,7 @@ extern "C" int wgbm_compute_histogram3(const int8_t* bin_indices, int64_t n_tota
   a.stats = P(L.stats);
   a.slots_per_node = n_eras;
   a.accumulate = 1;
+  a.dual = 1;  // per-node op: constant-hessian nodes (regression) take one plane
   const int n = (int)n_samples;
   const int nsb = stats_blocks(n);
   hipLaunchKernelGGL(k_list_stats, dim3((unsigned)nsb), dim3(256), 0, stream, sample_indices, n, n_total_rows, grads,

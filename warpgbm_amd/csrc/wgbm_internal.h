@@ -52,6 +52,9 @@ struct HistArgs {
   const int32_t* stats;    // per node: max|g|, min h, max h, max|h| (see wgbm_stats.h)
   int32_t slots_per_node;  // histogram slots of one node (eras); node = slot / this
   int32_t hess_const;      // 1: the caller guarantees one hessian value for every row
+  int32_t dual;            // without that guarantee: 1 launches both variants, each node built by the one
+                           // its statistics select (fewer LDS adds for constant-hessian nodes, one launch
+                           // more); 0 builds every node with two planes
   int32_t accumulate;      // 1: out += result (reference op), 0: out = result
   int32_t lds_pitch;       // floats between consecutive (j, bin) rows of a plane
   int32_t planned;         // 1: the plan (hist_plan_job) already ran in an earlier fused launch
