@@ -26,3 +26,23 @@ def dev():
     if not torch.cuda.is_available():
         pytest.fail("gpu test collected on a host without a GPU")
     return torch.device("cuda:0")
+
+
+def pytest_terminal_summary(terminalreporter):
+    """Largest error against the FP64 oracle seen by each parity check."""
+    try:
+        import _parity  # the same module object the test files import (tests/ is on sys.path)
+    except Exception:
+        return
+    if not _parity.RECORD:
+        return
+    tr = terminalreporter
+    tr.write_sep("-", "max error vs FP64 oracle (per check)")
+    for name in sorted(_parity.RECORD):
+        v, b = _parity.RECORD[name]
+        tr.write_line(f"{name:60s} {v:10.3e}  (bound {b:.1e})")
+    path = os.environ.get("WGBM_PARITY_REPORT")
+    if path:
+        import json
+        with open(path, "w") as f:
+            json.dump({k: {"max_error": v, "bound": b} for k, (v, b) in sorted(_parity.RECORD.items())}, f, indent=1)

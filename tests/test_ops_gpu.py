@@ -9,6 +9,7 @@ import numpy as np
 import pytest
 
 from oracle import gbm_oracle as O
+import _parity as PAR
 
 pytestmark = pytest.mark.gpu
 
@@ -34,6 +35,16 @@ def _rel_err(got, ref, scale):
     return float(np.max(np.abs(got.astype(np.float64) - ref)) / max(scale, 1e-30))
 
 
+def _check_hist_bins(name, Gg, Hg, codes, g, h, rows, feats, eras, E, B):
+    """Every bin within 1e-5 of the FP64 oracle relative to its own sum of
+    |g| (|h|); empty bins exactly zero."""
+    Go, Ho = O.histogram(codes, g, h, rows, feats, eras, E, B)
+    AG, AH = O.histogram(codes, np.abs(g.astype(np.float64)), np.abs(h.astype(np.float64)), rows, feats, eras, E, B)
+    PAR.check_hist(name + " G", Gg, Go, AG)
+    PAR.check_hist(name + " H", Hg, Ho, AH)
+    return Go, Ho, AG, AH
+
+
 @pytest.mark.parametrize("N,F,B,E,subset", [
     (5000, 50, 32, 1, False),
     (20000, 37, 7, 1, True),
@@ -51,10 +62,7 @@ def test_histogram_matches_oracle(dev, N, F, B, E, subset):
     rows = np.sort(rng.choice(N, size=N * 2 // 3, replace=False)).astype(np.int32)
     feats = rng.permutation(F)[: F // 2] if subset else np.arange(F)
     Gg, Hg = _hist_gpu(dev, codes, g, h, rows, feats, eras, E, B)
-    Go, Ho = O.histogram(codes, g, h, rows, feats, eras, E, B)
-    # FP32 accumulation vs FP64: error relative to the node's sum of |g| (|h|).
-    assert _rel_err(Gg, Go, np.abs(g[rows]).sum()) < 1e-5
-    assert _rel_err(Hg, Ho, np.abs(h[rows]).sum()) < 1e-5
+    _check_hist_bins("histogram op", Gg, Hg, codes, g, h, rows, feats, eras, E, B)
     # invariants: every row lands in exactly one bin per feature
     np.testing.assert_allclose(Gg.sum(axis=(0, 2)), g[rows].astype(np.float64).sum(), rtol=1e-4, atol=1e-2)
 
@@ -80,9 +88,7 @@ def test_histogram_every_tiling_matches_oracle(dev, cfg, subset):
         Gg, Hg = _hist_gpu(dev, codes, g, h, rows, feats, eras, E, B)
     finally:
         _lib.call("wgbm_set_hist_config", 0, 0, 0, 0)
-    Go, Ho = O.histogram(codes, g, h, rows, feats, eras, E, B)
-    assert _rel_err(Gg, Go, np.abs(g[rows]).sum()) < 1e-5
-    assert _rel_err(Hg, Ho, np.abs(h[rows]).sum()) < 1e-5
+    _check_hist_bins("histogram op, every tiling", Gg, Hg, codes, g, h, rows, feats, eras, E, B)
 
 
 @pytest.mark.parametrize("const_h", [True, False])
@@ -99,11 +105,69 @@ def test_histogram_bins_of_many_rows(dev, const_h):
     eras = np.zeros(N, np.int32)
     rows = np.arange(N, dtype=np.int32)
     Gg, Hg = _hist_gpu(dev, codes, g, h, rows, np.arange(F), eras, 1, B)
-    Go, Ho = O.histogram(codes, g, h, rows, np.arange(F), eras, 1, B)
+    Go, Ho, _, _ = _check_hist_bins("histogram op, 150k-row bins", Gg, Hg, codes, g, h, rows, np.arange(F), eras, 1, B)
     if const_h:
         np.testing.assert_array_equal(Hg, Ho.astype(np.float32))  # count * 0.75, exact
-    assert _rel_err(Hg, Ho, np.abs(h).sum()) < 1e-6
-    assert _rel_err(Gg, Go, np.abs(g).sum()) < 1e-6
+
+
+def _hdr_gradients(rng, n, codes_col, B, lo=-5.0, hi=3.0):
+    """Gradients whose magnitude follows one feature's bin: bin b holds
+    |g| = 10^(lo + (hi - lo) b / (B - 1)) (times a factor in [1, 2)), random
+    signs, so the smallest bins hold only rows 10^(hi - lo) below the
+    node's largest |g| -- the case a fixed-point scale taken from the node's
+    max |g| rounds worst."""
+    mag = 10.0 ** (lo + (hi - lo) * codes_col.astype(np.float64) / (B - 1))
+    return (mag * rng.uniform(1.0, 2.0, n) * rng.choice([-1.0, 1.0], n)).astype(np.float32)
+
+
+@pytest.mark.parametrize("hess", ["constant", "varying"])
+@pytest.mark.parametrize("E", [1, 3])
+def test_histogram_high_dynamic_range(dev, hess, E):
+    """|g| from 1e-5 to 2e3 in one node, small magnitudes concentrated in
+    whole bins.  Constant hessians take the count-packed 34-bit words with
+    the exact fine path for small rows; varying hessians the 49-bit words.
+    Every bin of every feature stays within 1e-5 of its own sum of |g|."""
+    rng = np.random.default_rng(101 + E)
+    N, F, B = 40000, 24, 16
+    codes = rng.integers(0, B, size=(N, F), dtype=np.int8)
+    g = _hdr_gradients(rng, N, codes[:, 0], B)
+    h = np.ones(N, np.float32) if hess == "constant" else rng.uniform(0.1, 3.0, N).astype(np.float32)
+    eras = rng.integers(0, E, N).astype(np.int32)
+    rows = np.sort(rng.choice(N, size=N * 3 // 4, replace=False)).astype(np.int32)
+    feats = np.arange(F)
+    Gg, Hg = _hist_gpu(dev, codes, g, h, rows, feats, eras, E, B)
+    _check_hist_bins(f"histogram op, |g| 1e-5..2e3 ({hess} h)", Gg, Hg, codes, g, h, rows, feats, eras, E, B)
+    # the small bins really are 1e-8 of the node's max
+    assert np.abs(g[rows][codes[rows, 0] == 0]).max() < 2e-8 * np.abs(g[rows]).max()
+    # a second call on the same op gives the same bits (fine words left at zero)
+    Gg2, Hg2 = _hist_gpu(dev, codes, g, h, rows, feats, eras, E, B)
+    assert np.array_equal(Gg, Gg2) and np.array_equal(Hg, Hg2)
+
+
+def test_histogram_late_round_multiclass_node(dev):
+    """A late-round softmax node: most rows classified with probability
+    within 1e-6..1e-3 of 0 or 1 (|g| tiny, hessians clamped near 1e-6), a
+    few misclassified rows with |g| near 1; confident rows cluster by
+    feature value.  Gradients and hessians of one class, as the objective
+    computes them in float32."""
+    rng = np.random.default_rng(55)
+    N, F, B, K = 30000, 16, 32, 5
+    codes = rng.integers(0, B, size=(N, F), dtype=np.int8)
+    y = rng.integers(0, K, N)
+    margin = 2.0 + 14.0 * (codes[:, 0].astype(np.float64) / (B - 1))  # confident where feature 0 is high
+    scores = rng.standard_normal((N, K)) * 0.5
+    scores[np.arange(N), y] += margin
+    wrong = rng.random(N) < 0.02
+    scores[wrong, (y[wrong] + 1) % K] += 2 * margin[wrong]
+    gk, hk = O.softmax_grad_hess(scores.astype(np.float32), y, np.ones(N))
+    g = gk[:, 2].astype(np.float32)
+    h = hk[:, 2].astype(np.float32)
+    rows = np.arange(N, dtype=np.int32)
+    eras = np.zeros(N, np.int32)
+    Gg, Hg = _hist_gpu(dev, codes, g, h, rows, np.arange(F), eras, 1, B)
+    nz = np.abs(g[g != 0])
+    assert nz.min() < 1e-6 * nz.max()  # a high dynamic range really is present
+    _check_hist_bins("histogram op, late-round softmax node", Gg, Hg, codes, g, h, rows, np.arange(F), eras, 1, B)
 
 
 def test_histogram_counts_exact_and_drops_out_of_range(dev):
@@ -166,12 +230,20 @@ def test_compute_split_and_select_match_oracle(dev, E):
     _lib.call("wgbm_select_split", gain.data_ptr(), d.data_ptr(), E, F, B, 0.0, None, None, out.data_ptr(),
               torch.cuda.current_stream().cuda_stream)
     torch.cuda.synchronize()
-    gain_o, dir_o = O.split_gains(*O.histogram(codes, g, h, rows, np.arange(F), eras, E, B), 20.0, 1e-6)
+    Go, Ho = O.histogram(codes, g, h, rows, np.arange(F), eras, E, B)
+    AG, AH = O.histogram(codes, np.abs(g.astype(np.float64)), h, rows, np.arange(F), eras, E, B)
+    gain_o, dir_o = O.split_gains(Go, Ho, 20.0, 1e-6)
     gg = gain.cpu().numpy()
-    scale = np.abs(gain_o).max()
-    assert np.max(np.abs(gg - gain_o)) <= 1e-5 * scale
+    # per threshold: the error the per-bin histogram bound can propagate
+    gbound, dbound, margin = PAR.gain_error_bound(Go, Ho, AG, AH, 1e-6)
+    ok = (O.split_gains(Go, Ho, 20.0, 1e-6)[1] != 0)  # thresholds past min_child_weight
+    err = np.abs(gg.astype(np.float64) - gain_o)
+    assert np.all(err[ok] <= gbound[ok]), float(np.max((err / gbound)[ok]))
+    assert np.all(gg[~ok] == 0.0)
+    PAR.record("compute_split gain (error / per-threshold bound)", float(np.max((err / gbound)[ok])), 1.0)
     dd = d.cpu().numpy()
-    clear = np.abs(gain_o) > 1e-3 * scale
+    clear = ok & (margin > dbound)  # direction fixed by the FP64 values beyond any histogram error
+    assert clear.sum() > 0.9 * ok.sum()
     assert np.array_equal(dd[clear], dir_o[clear])
     f, b = O.select_split(gain_o, dir_o, 0.0)
     assert tuple(out.cpu().numpy().tolist()) == (f, b)

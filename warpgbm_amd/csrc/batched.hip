@@ -35,7 +35,7 @@ struct FitLayout {
   size_t dec_feat, dec_bin, era_gain, cand_agree, cand_crit, cand_idx, era_g, era_d;
   int64_t gain_nodes;
   size_t b_seg_start, b_seg_count, b_seg_slot, n_build, sib, n_split;
-  size_t h_seg_first, h_seg_pfirst, h_nseg, h_multi, h_n_multi, h_chunk, h_part, h_desc, h_total, h_mdesc,
+  size_t h_seg_first, h_seg_pfirst, h_nseg, h_multi, h_n_multi, h_chunk, h_part, h_pflag, h_fine, h_desc, h_total, h_mdesc,
       h_mparts, loss_part;
   size_t stat_part, round_ctr, fidx_cur;
   size_t p_flag, p_seg_first, p_nseg, p_multi, p_n_multi, p_lbase, p_rbase, p_cscan, p_segL, p_segP, codes_t, p_cseg;
@@ -111,6 +111,8 @@ static FitLayout fit_layout(const wgbm_fit_config& c) {
   L.loss_part = take(8 * 2 * kStatBlocks);
   L.hist_part_floats = (int64_t)hist_partial_words(L.hc, c.n_feat, c.n_bins, L.KN);
   L.h_part = take(sizeof(longlong2) * L.hist_part_floats);
+  L.h_pflag = take(4 * hist_partial_flags(L.hc, c.n_feat, L.KN));
+  L.h_fine = take(8 * L.hist_nodes * L.node_elems);  // fine sums: one slot per parent (and per root) node-era
   L.part_chunks = ceil_div(L.KN, kPartChunk) + MS;
   L.p_flag = take(L.KN);
   L.p_seg_first = take(4 * (MS + 1));
@@ -603,6 +605,8 @@ static int enqueue_round(const wgbm_fit_config& c, const FitLayout& L, const wgb
         a.out_g = hg;
         a.out_h = hh;
         a.part = reinterpret_cast<longlong2*>(ws + L.h_part);
+        a.pflag = I(L.h_pflag);
+        a.fine = reinterpret_cast<long long*>(ws + L.h_fine);
         a.slots_per_node = E;
         a.hess_const = fit_one_plane(c) ? 1 : 0;
         a.stats = I(cur.stats);
@@ -773,6 +777,9 @@ extern "C" int wgbm_fit_rounds(const wgbm_fit_config* cfg, const wgbm_fit_io* io
     WGBM_LAUNCHED("k_transpose_codes");
   }
 
+  // Fine-row sums start at zero; every histogram read-out leaves them so
+  // (set here, before any capture: a graph holds no memset).
+  WGBM_HIP(hipMemsetAsync(ws + L.h_fine, 0, 8 * (size_t)L.hist_nodes * L.node_elems, st));
   g_graph_rounds = 0;
   hipLaunchKernelGGL(k_round_init, dim3(1), dim3(64), 0, st, I(L.round_ctr), 0);
   WGBM_LAUNCHED("k_round_init");

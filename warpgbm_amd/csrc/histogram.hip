@@ -81,21 +81,63 @@ k_hist_plan(const PlanSrc src, int32_t chunk_rows, int32_t* __restrict__ seg_fir
 // ---------------------------------------------------------------------------
 // Shifts of a segment's node and the partial word of one bin.
 struct NodeShift {
-  int sg, sh;
+  int sg, sh, sf;   // gradient, hessian and fine-gradient shifts
   bool count_mode;  // every row of the node has the same hessian
+  bool packed;      // gradient words carry the row count (34-bit gradient, fine rows)
   float h_const;    // that hessian (used by the one-plane variant)
 };
 
-__device__ __forceinline__ NodeShift node_shift(const int32_t* st, int h_bits) {
+// one_plane: the node is built by the one-plane variant, whose words always
+// carry the count.  Otherwise the words carry it exactly when the node is
+// in count mode, so both variants encode a count-mode node alike.
+__device__ __forceinline__ NodeShift node_shift(const int32_t* st, int h_bits, int fine_x, bool one_plane) {
   NodeShift n;
   const float gmax = __builtin_bit_cast(float, st[0]);
   const float hmin = ord_float(st[1]);
   const float habs = __builtin_bit_cast(float, st[3]);
-  n.sg = fixed_shift(gmax, kGBits);
-  n.sh = fixed_shift(habs, h_bits);
   n.count_mode = st[1] == st[2];
+  n.packed = one_plane || n.count_mode;
+  n.sg = fixed_shift(gmax, n.packed ? kGBitsCount : kGBitsWide);
+  n.sf = n.sg + fine_x;
+  n.sh = fixed_shift(habs, h_bits);
   n.h_const = hmin;
   return n;
+}
+
+// Gradient sum and row count of a gradient word (count 0 when the word
+// carries none).
+__device__ __forceinline__ void decode_word(unsigned long long w, bool packed, long long& gsum, long long& count) {
+  if (packed) {
+    gsum = ((long long)(w << (64 - kCountShift))) >> (64 - kCountShift);
+    count = (long long)((w - (unsigned long long)gsum) >> kCountShift);
+  } else {
+    gsum = (long long)w;
+    count = 0;
+  }
+}
+
+// A bin's float32 gradient: the coarse sum (scale 2^-sg) plus, when a fine
+// row landed in the bin, the fine sum (scale 2^-sf); one rounding to float32
+// after at most two in float64.
+__device__ __forceinline__ float bin_grad(const NodeShift& ns, double coarse, long long fine) {
+  double v = ldexp(coarse, -ns.sg);
+  if (fine != 0) v += ldexp((double)fine, -ns.sf);
+  return (float)v;
+}
+
+// Fine slot of a histogram slot (see HistArgs::fine).
+__device__ __forceinline__ int fine_slot(const HistArgs& a, int slot) {
+  if (!a.sub_g) return slot;
+  const int E = a.slots_per_node;
+  const int c = slot / E;
+  return (c >> 1) * E + (slot - c * E);
+}
+
+// Takes a fine word back to zero and returns it.
+__device__ __forceinline__ long long take_fine(const HistArgs& a, int fslot, int f, int bin) {
+  unsigned long long* p = reinterpret_cast<unsigned long long*>(a.fine) +
+                          ((int64_t)fslot * a.n_feat + f) * a.n_bins + bin;
+  return (long long)__hip_atomic_exchange(p, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // A bin's float32 hessian: count * h in count mode (exact in float64, one
@@ -184,8 +226,13 @@ k_hist_build(const HistArgs a) {
     first_raw = a.perm[min(w0 + lane, w1 - 1)];
     if constexpr (GH_POS) first_gh = a.gh[min(w0 + lane, w1 - 1)];
   }
-  const NodeShift ns = node_shift(a.stats + (int64_t)(slot / a.slots_per_node) * kStatWords, a.h_bits);
+  const NodeShift ns = node_shift(a.stats + (int64_t)(slot / a.slots_per_node) * kStatWords, a.h_bits, a.fine_x,
+                                  ONE_PLANE);
   if (a.dual && ONE_PLANE != ns.count_mode) continue;  // the other variant builds this node
+  const bool packed = ONE_PLANE || ns.packed;
+  const int fslot = fine_slot(a, slot);
+  __shared__ int s_fine;  // some wave added a fine row to this item's sums
+  if (threadIdx.x == 0) s_fine = 0;
 
   // One shared histogram per workgroup: T plane (count << 44 | sum qg) and,
   // unless the hessian is constant, an H plane (sum qh).  A plane holds
@@ -286,8 +333,9 @@ k_hist_build(const HistArgs a) {
       }
       return v;
     };
-    // Fixed-point words of a block's rows (lane = position in the block).
-    unsigned long long myqp, myqh;
+    // Fixed-point words of a block's rows (lane = position in the block);
+    // fmask: the block's fine rows (their words carry only the count).
+    unsigned long long myqp, myqh, fmask;
     auto block_words = [&](int rowv_, float2 ghv_) {
       const bool okr = rowv_ >= 0;
       float g = ghv_.x, h = ghv_.y;
@@ -295,9 +343,53 @@ k_hist_build(const HistArgs a) {
         const int r0 = max(rowv_, 0);
         g = a.g[r0]; h = a.h[r0];
       }
-      myqp = okr ? (1ull << kCountShift) + (unsigned long long)to_fixed(g, ns.sg) : 0ull;
+      const long long q = to_fixed(g, ns.sg);
+      if (packed) {
+        const bool fine = okr && g != 0.0f && q < kFineLimit && q > -kFineLimit;
+        myqp = okr ? (1ull << kCountShift) + (fine ? 0ull : (unsigned long long)q) : 0ull;
+        fmask = __ballot(fine);
+      } else {
+        myqp = okr ? (unsigned long long)q : 0ull;
+        fmask = 0ull;
+      }
       myqh = okr ? (unsigned long long)to_fixed(h, ns.sh) : 0ull;
     };
+    // Fine rows of the wave's positions, after its main loop (they are
+    // rare: count-mode rows with |g| below 2^-16 of the node's max): each
+    // lane adds a fine row's gradient at the fine scale to the global words
+    // of its features' bins (codes outside [0, B) and columns outside the
+    // matrix add nothing).
+    auto fine_pass = [&]() {
+      if (lane == 0) s_fine = 1;
+      for (int bpos = w0; bpos < w1; bpos += kWave) {
+        const int pos = bpos + lane;
+        const int row = valid_row(pos, perm_at(pos));
+        const float g = GH_POS ? gh_at(pos).x : a.g[max(row, 0)];
+        const long long q = to_fixed(g, ns.sg);
+        unsigned long long m = __ballot(row >= 0 && g != 0.0f && q < kFineLimit && q > -kFineLimit);
+        while (m) {
+          const int r = __builtin_ctzll(m);
+          m &= m - 1ull;
+          const int rr = __builtin_amdgcn_readlane(row, r);
+          const float gr = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, g), r));
+          const unsigned long long qf = (unsigned long long)to_fixed(gr, ns.sf);
+          const TV v = load_codes(rr);
+#pragma unroll
+          for (int j = 0; j < FPL; ++j) {
+            const uint32_t bin = code_u8<FPL>(v, j);
+            if (bin < jcap[j]) {  // a column outside the matrix has a cap of 0
+              unsigned long long* p = reinterpret_cast<unsigned long long*>(a.fine) +
+                                      ((int64_t)fslot * a.n_feat + fl0 + j) * B + bin;
+              __hip_atomic_fetch_add(p, qf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+          }
+        }
+      }
+      // The adds are complete at L2 before this wave reaches the barrier
+      // that precedes the read-out (which takes the words back to zero).
+      __builtin_amdgcn_s_waitcnt(0);
+    };
+    unsigned long long any_fine = 0ull;
     int rowv = valid_row(w0 + lane, first_raw);
     block_words(rowv, first_gh);
     TV A[LG], Bv[LG];
@@ -346,9 +438,11 @@ k_hist_build(const HistArgs a) {
 #pragma unroll
       for (int q = 0; q < LG; ++q) A[q] = load_codes(__builtin_amdgcn_readlane(rown, q));
       add_group(Bv, (NG - 1) * LG);
+      any_fine |= fmask;
       rowv = rown;
       block_words(rowv, ghn);
     }
+    if (any_fine) fine_pass();
   };
   if (has_rows) {
     if (ident) run(std::true_type{});
@@ -363,8 +457,9 @@ k_hist_build(const HistArgs a) {
   const int SL = kWave * FPL * B;
   const int f_base = slab * kWave * FPL;
   const bool direct = pidx < 0;
+  const bool item_fine = s_fine != 0;  // fine words to take in (the row loop ended at a barrier)
   constexpr int kFlushBatch = 4;  // words per thread whose sibling loads are in flight together
-  if (!WIDE && direct && SL <= kDirectStage * (int)blockDim.x) {
+  if (!WIDE && direct && !item_fine && SL <= kDirectStage * (int)blockDim.x) {
     // A single-chunk segment's float32 bins go to [feature][bin] of its slot,
     // where word order would put consecutive lanes FPL * B floats apart.  The
     // decoded bins are staged in the freed LDS plane in output order, then
@@ -381,9 +476,9 @@ k_hist_build(const HistArgs a) {
         const int j = (t >> 6) / B;
         const int word = t + j * kWave;  // skip the spare rows of features 0..j-1
         const unsigned long long w = T[word];
-        const long long gsum = ((long long)(w << (64 - kCountShift))) >> (64 - kCountShift);
-        const long long count = (long long)((w - (unsigned long long)gsum) >> kCountShift);
-        rg[k] = from_fixed(gsum, ns.sg);
+        long long gsum, count;
+        decode_word(w, packed, gsum, count);
+        rg[k] = bin_grad(ns, (double)gsum, 0);
         rh[k] = bin_hess(ns, ONE_PLANE, ONE_PLANE ? count : (long long)Hq[word]);
       }
     }
@@ -437,6 +532,25 @@ k_hist_build(const HistArgs a) {
     __syncthreads();  // the read-out is done before the next item clears the LDS
     continue;
   }
+  if (direct && item_fine) {
+    // A single-chunk segment that took fine rows: every bin adds its fine word.
+    for (int t = threadIdx.x; t < SL; t += blockDim.x) {
+      const int r = t >> 6;
+      const int j = r / B;
+      const int bin = r - j * B;
+      const int fl = (t & 63) * FPL + j;
+      if (f_base + fl >= a.n_feat) continue;
+      const SibBin sb = a.sub_g ? load_sib(a, slot, f_base + fl, bin) : SibBin{};
+      const int word = t + j * kWave;
+      long long gsum, count;
+      decode_word(T[word], packed, gsum, count);
+      const long long hsum = ONE_PLANE ? count : (long long)Hq[word];
+      const long long fw = take_fine(a, fslot, f_base + fl, bin);
+      store_bin(a, slot, f_base + fl, bin, bin_grad(ns, (double)gsum, fw), bin_hess(ns, ONE_PLANE, hsum), sb);
+    }
+    __syncthreads();  // the read-out is done before the next item clears the LDS
+    continue;
+  }
   for (int t0 = threadIdx.x; t0 < SL; t0 += kFlushBatch * blockDim.x) {
     SibBin sb[kFlushBatch];
     if (direct && a.sub_g) {
@@ -461,11 +575,11 @@ k_hist_build(const HistArgs a) {
       if (f_base + fl >= a.n_feat) continue;
       const int word = t + j * kWave;  // skip the spare rows of features 0..j-1
       const unsigned long long w = T[word];
-      const long long gsum = ((long long)(w << (64 - kCountShift))) >> (64 - kCountShift);
-      const long long count = (long long)((w - (unsigned long long)gsum) >> kCountShift);
+      long long gsum, count;
+      decode_word(w, packed, gsum, count);
       const long long hsum = ONE_PLANE ? count : (long long)Hq[word];  // count, or the fixed-point sum
       if (direct) {
-        store_bin(a, slot, f_base + fl, bin, from_fixed(gsum, ns.sg), bin_hess(ns, ONE_PLANE, hsum), sb[u]);
+        store_bin(a, slot, f_base + fl, bin, bin_grad(ns, (double)gsum, 0), bin_hess(ns, ONE_PLANE, hsum), sb[u]);
       } else {
         const int64_t d = ((int64_t)pidx * a.nslab + slab) * SL + t;
         // Partials are read once, by the merge: streaming stores keep them
@@ -479,6 +593,7 @@ k_hist_build(const HistArgs a) {
       }
     }
   }
+  if (!direct && threadIdx.x == 0) a.pflag[(int64_t)pidx * a.nslab + slab] = item_fine ? 1 : 0;
   __syncthreads();  // the read-out is done before the next item clears the LDS
   }
 }
@@ -496,19 +611,22 @@ constexpr int kMergeMaxParts = 8;
 constexpr int kMergeUnroll = 8;
 
 struct MergeSum {
-  long long g = 0, x = 0;  // x: count (one plane) or hessian sum
+  I128 g;          // gradient sum
+  long long x = 0; // count (one plane) or hessian sum
   __device__ __forceinline__ void add_packed(unsigned long long w) {
-    const long long gs = ((long long)(w << (64 - kCountShift))) >> (64 - kCountShift);
-    g += gs;
-    x += (long long)((w - (unsigned long long)gs) >> kCountShift);
+    long long gs, c;
+    decode_word(w, true, gs, c);
+    g.add(gs);
+    x += c;
   }
-  __device__ __forceinline__ void add_pair(longlong2 v) { g += v.x; x += v.y; }
+  __device__ __forceinline__ void add_pair(longlong2 v) { g.add(v.x); x += v.y; }
 };
 
 template <int FPL>
 __global__ void __launch_bounds__(kMergeThreads)
 k_hist_merge(const HistArgs a, const int32_t* __restrict__ n_multi) {
-  __shared__ long long s_g[kMergeThreads];
+  __shared__ unsigned long long s_glo[kMergeThreads];
+  __shared__ long long s_ghi[kMergeThreads];
   __shared__ long long s_h[kMergeThreads];
   const int P = *a.merge_parts;
   const int WPI = kMergeThreads / P;  // words per item
@@ -525,8 +643,10 @@ k_hist_merge(const HistArgs a, const int32_t* __restrict__ n_multi) {
     const int e = (r - slab * neb) * WPI + el;
     const int4 md = a.multi_desc[m];
     const int first = md.x, nch = md.y, slot = md.z;
-    const NodeShift ns = node_shift(a.stats + (int64_t)(slot / a.slots_per_node) * kStatWords, a.h_bits);
+    NodeShift ns = node_shift(a.stats + (int64_t)(slot / a.slots_per_node) * kStatWords, a.h_bits, a.fine_x, false);
     const bool one_plane = a.hess_const || (a.dual && ns.count_mode);  // the variant that built the partials
+    if (one_plane && !ns.packed) ns = node_shift(a.stats + (int64_t)(slot / a.slots_per_node) * kStatWords,
+                                                 a.h_bits, a.fine_x, true);
     const int rr = e >> 6;
     const int j = rr / B;
     const int bin = rr - j * B;
@@ -563,15 +683,29 @@ k_hist_merge(const HistArgs a, const int32_t* __restrict__ n_multi) {
       }
     }
     if (P > 1) {
-      s_g[threadIdx.x] = acc.g;
+      s_glo[threadIdx.x] = acc.g.lo;
+      s_ghi[threadIdx.x] = acc.g.hi;
       s_h[threadIdx.x] = acc.x;
-      __syncthreads();
     }
+    // Fine rows in any of the segment's chunks (count-mode nodes only); the
+    // barrier also publishes the groups' sums.
+    int flagged = 0;
+    if (ns.packed)
+      for (int k = threadIdx.x; k < nch; k += kMergeThreads) flagged |= a.pflag[(int64_t)(first + k) * a.nslab + slab];
+    const bool any_fine = __syncthreads_or(flagged) != 0;
     if (part == 0 && e < SL) {
-      long long sg = acc.g, sx = acc.x;
-      for (int q = 1; q < P; ++q) { sg += s_g[q * WPI + el]; sx += s_h[q * WPI + el]; }
+      I128 sg = acc.g;
+      long long sx = acc.x;
+      for (int q = 1; q < P; ++q) {
+        I128 o;
+        o.lo = s_glo[q * WPI + el];
+        o.hi = s_ghi[q * WPI + el];
+        sg.add(o);
+        sx += s_h[q * WPI + el];
+      }
       if (f_base + fl < a.n_feat) {
-        store_bin(a, slot, f_base + fl, bin, from_fixed(sg, ns.sg), bin_hess(ns, one_plane, sx), sb);
+        const long long fw = any_fine ? take_fine(a, fine_slot(a, slot), f_base + fl, bin) : 0;
+        store_bin(a, slot, f_base + fl, bin, bin_grad(ns, sg.to_double(), fw), bin_hess(ns, one_plane, sx), sb);
       }
     }
     if (P > 1) __syncthreads();
@@ -739,7 +873,9 @@ void plan_job_launch(const PlanJob& j, hipStream_t st) {
 // Automatic configurations keep two workgroups' sub-histograms per CU
 // (64 KB each); explicit ones may use a workgroup's full 160 KB.
 constexpr size_t kLdsBudget = 64 * 1024;
-constexpr size_t kLdsMax = 160 * 1024;
+// A workgroup's LDS is 160 KB; the build kernel's own static variables take
+// a few bytes of it, so its dynamic histogram may use 160 KB less 1 KB.
+constexpr size_t kLdsMax = 160 * 1024 - 1024;
 
 int lds_pitch(const HistConfig&, int) { return 64; }
 
@@ -816,11 +952,15 @@ int64_t hist_max_multi(const HistConfig& c, int n_feat, int64_t rows_max) {
 }
 
 size_t hist_partial_words(const HistConfig& c, int n_feat, int n_bins, int64_t rows_max) {
+  return hist_partial_flags(c, n_feat, rows_max) * kWave * c.fpl * n_bins;
+}
+
+size_t hist_partial_flags(const HistConfig& c, int n_feat, int64_t rows_max) {
   const int nslab = (int)ceil_div(n_feat, kWave * c.fpl);
   // Multi-chunk segments hold more than C rows each, so their chunks number
   // at most 2 * rows / C <= 2 * cap.
   const int64_t chunks = 2 * hist_chunk_cap(c, n_feat, rows_max) + 2;
-  return (size_t)chunks * nslab * kWave * c.fpl * n_bins;
+  return (size_t)chunks * nslab;
 }
 
 // ---------------------------------------------------------------------------
@@ -972,12 +1112,13 @@ int hist_launch(HistArgs a, const HistConfig& c, int64_t rows_max, int64_t nseg_
   WGBM_REQUIRE(a.chunk_desc && a.chunk_total && a.multi_desc && a.merge_parts,
                "histogram: missing chunk descriptor scratch");
   WGBM_REQUIRE(a.codes && a.perm && (a.gh || (a.g && a.h)) && a.seg_start && a.seg_count && a.seg_slot &&
-                   a.seg_first && a.seg_pfirst && a.nseg_dev && a.out_g && a.out_h && a.part && a.stats && multi &&
-                   n_multi,
+                   a.seg_first && a.seg_pfirst && a.nseg_dev && a.out_g && a.out_h && a.part && a.pflag &&
+                   a.fine && a.stats && multi && n_multi,
                "histogram: missing table or scratch pointer");
   a.chunk_rows = c.chunk_rows;
   a.nslab = (int)ceil_div(a.n_feat, kWave * c.fpl);
   a.h_bits = hess_bits(a.n_rows);
+  a.fine_x = fine_extra(a.n_rows);
   a.wide_ok = (a.row_stride % c.fpl == 0) && ((uintptr_t)a.codes % (uintptr_t)c.fpl == 0);
   a.lds_pitch = lds_pitch(c, a.n_bins);
   if (lds_bytes(c, a.n_bins, a.hess_const != 0) > kLdsMax) {
@@ -1043,7 +1184,7 @@ extern "C" int wgbm_profile_end(double* out4) {
 // Workspace carve-up for the reference-style histogram op.
 struct Hist3Layout {
   size_t seg_start, seg_count, seg_slot, seg_first, seg_pfirst, nseg, multi, n_multi, chunk, desc, total_chunks,
-      mdesc, mparts, sorted, cnt, stats, stat_part, part, total;
+      mdesc, mparts, sorted, cnt, stats, stat_part, part, pflag, fine, total;
 };
 
 static Hist3Layout hist3_layout(int64_t n, int n_feat, int n_bins, int n_eras, const HistConfig& c) {
@@ -1071,6 +1212,8 @@ static Hist3Layout hist3_layout(int64_t n, int n_feat, int n_bins, int n_eras, c
   L.stats = o;     o = al(o + 4 * kStatWords);
   L.stat_part = o; o = al(o + 4 * kStatWords * kStatBlocks);
   L.part = o;      o = al(o + sizeof(longlong2) * hist_partial_words(c, n_feat, n_bins, n));
+  L.pflag = o;     o = al(o + 4 * hist_partial_flags(c, n_feat, n));
+  L.fine = o;      o = al(o + 8 * (size_t)E * n_feat * n_bins);
   L.total = o;
   return L;
 }
@@ -1127,10 +1270,15 @@ extern "C" int wgbm_compute_histogram3(const int8_t* bin_indices, int64_t n_tota
   a.out_g = grad_hist;
   a.out_h = hess_hist;
   a.part = reinterpret_cast<longlong2*>(ws + L.part);
+  a.pflag = P(L.pflag);
+  a.fine = reinterpret_cast<long long*>(ws + L.fine);
   a.stats = P(L.stats);
   a.slots_per_node = n_eras;
   a.accumulate = 1;
   a.dual = 1;  // per-node op: constant-hessian nodes (regression) take one plane
+  // The fine sums start at zero (the workspace may be fresh memory); the
+  // read-outs leave them at zero.
+  WGBM_HIP(hipMemsetAsync(ws + L.fine, 0, L.total - L.fine, stream));
   const int n = (int)n_samples;
   const int nsb = stats_blocks(n);
   hipLaunchKernelGGL(k_list_stats, dim3((unsigned)nsb), dim3(256), 0, stream, sample_indices, n, n_total_rows, grads,

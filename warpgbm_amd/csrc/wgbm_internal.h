@@ -49,6 +49,13 @@ struct HistArgs {
   float* out_g;            // [slot][n_feat][n_bins]
   float* out_h;
   longlong2* part;         // chunk partials (exact fixed-point sums of g, h)
+  int32_t* pflag;          // [partial chunk][slab]: 1 when the chunk added fine rows
+  // Exact fine-row gradient sums, [fine slot][n_feat][n_bins] int64, zero
+  // between launches (each read-out takes its words back to zero).  The fine
+  // slot of slot s is s, or with sibling derivation its parent's slot
+  // (one child per parent is built).  fine_x: the fine words' extra shift.
+  long long* fine;
+  int32_t fine_x;
   const int32_t* stats;    // per node: max|g|, min h, max h, max|h| (see wgbm_stats.h)
   int32_t slots_per_node;  // histogram slots of one node (eras); node = slot / this
   int32_t hess_const;      // 1: the caller guarantees one hessian value for every row
@@ -78,6 +85,7 @@ size_t lds_bytes(const HistConfig& c, int n_bins, bool count_mode);
 int64_t hist_max_chunks(const HistConfig& c, int n_feat, int64_t rows_max, int64_t nseg_max);
 int64_t hist_max_multi(const HistConfig& c, int n_feat, int64_t rows_max);
 size_t hist_partial_words(const HistConfig& c, int n_feat, int n_bins, int64_t rows_max);
+size_t hist_partial_flags(const HistConfig& c, int n_feat, int64_t rows_max);  // [partial chunk][slab] ints
 // Launch plan + build + merge.  rows_max / nseg_max bound the device-side
 // row and segment counts (they size the grid).  multi/n_multi: scratch of
 // hist_max_multi ints and one int; a.chunk_dev: one int.

@@ -2,8 +2,8 @@
 //
 // Histograms accumulate integers: a row's gradient becomes
 // qg = round(g * 2^sg) and its hessian qh = round(h * 2^sh), with the
-// shifts chosen from the node's max |g| and max |h| so that |qg| < 2^28 and
-// |qh| < 2^48.  Integer sums are exact and independent of the order of the
+// shifts chosen from the node's max |g| and max |h| so that |qg| < 2^34
+// (2^49 without a packed count) and |qh| < 2^48.  Integer sums are exact and independent of the order of the
 // additions, so any number of waves may add into one LDS histogram and any
 // chunking gives the same bits; each bin's sum is rounded to float32 once.
 // A node's statistics (max |g|, min h, max h, max |h|) are kept as
@@ -87,9 +87,30 @@ __device__ __forceinline__ float from_fixed(long long s, int shift) {
   return (float)ldexp((double)s, -shift);
 }
 
-// Bits of the per-row fixed-point gradient: a chunk's sum stays below 2^43
-// (packed beside the count) and any segment's below 2^63.
-constexpr int kGBits = 28;
+// Gradient words.  A node whose rows share one hessian value ("count
+// mode": its H is count * h) packs the row count into the top 15 bits of
+// the gradient word (a chunk holds at most 2^14 rows) above a 34-bit
+// fixed-point gradient; any other node keeps a count-free 49-bit gradient
+// (its hessian has a plane of its own).  Either way a chunk's LDS sum stays
+// within 64 bits; chunk partials are summed in 128-bit integers.
+constexpr int kCountShift = 49;
+constexpr int kGBitsCount = 34;
+constexpr int kGBitsWide = 49;
+constexpr int kMaxChunkRows = 1 << 14;
+// Fine rows (count mode only).  A row whose 34-bit word would carry a
+// relative rounding error above 2^-18 -- g != 0 with |round(g * 2^sg)| <
+// 2^17 -- adds only its count to the LDS word; its gradient goes, 2^x times
+// finer (x = fine_extra), into an exact 64-bit global accumulator per
+// (slot, feature, bin) that the read-out adds in.
+constexpr long long kFineLimit = 1ll << 17;
+// x for a bin matrix of n_rows rows: a fine word is below 2^(18 + x), so a
+// bin's fine sum over any segment (at most n_rows rows) stays below 2^63.
+__host__ __device__ __forceinline__ int fine_extra(int64_t n_rows) {
+  int len = 0;
+  while (len < 62 && (int64_t(1) << len) <= n_rows) ++len;  // bits of n_rows
+  const int x = 45 - len;
+  return x < 30 ? (x > 1 ? x : 1) : 30;
+}
 // Bits of the per-row fixed-point hessian for a bin matrix of n_rows rows:
 // a bin's sum over any segment (at most n_rows rows) stays below 2^62, and a
 // chunk's LDS word below 2^62 at the cap of 48.  Matrices of up to 2^14 rows
@@ -100,10 +121,47 @@ __host__ __device__ __forceinline__ int hess_bits(int64_t n_rows) {
   const int b = 62 - len;
   return b < 48 ? (b > 16 ? b : 16) : 48;
 }
-// Packed LDS word of the gradient plane: count << 44 | gradient sum (two's
-// complement in the low 44 bits; a chunk's |sum| < 2^(28+14) < 2^43).
-constexpr int kCountShift = 44;
-constexpr int kMaxChunkRows = 1 << 14;
+
+// A signed 128-bit integer as two 64-bit halves (chunk partials of a
+// segment are summed exactly in it).
+struct I128 {
+  unsigned long long lo = 0;
+  long long hi = 0;
+  __device__ __forceinline__ void add(long long v) {
+    const unsigned long long s = lo + (unsigned long long)v;
+    hi += (v < 0 ? -1 : 0) + (s < lo ? 1 : 0);
+    lo = s;
+  }
+  __device__ __forceinline__ void add(const I128& o) {
+    const unsigned long long s = lo + o.lo;
+    hi += o.hi + (s < lo ? 1 : 0);
+    lo = s;
+  }
+  // Round to nearest double.  A value that fits 64 bits converts as that
+  // int64 does, so a sum read here and the same sum read from one chunk's
+  // 64-bit word give the same double.
+  __device__ __forceinline__ double to_double() const {
+    if (hi == ((long long)lo >> 63)) return (double)(long long)lo;
+    const bool neg = hi < 0;
+    unsigned long long h = (unsigned long long)hi, l = lo;
+    if (neg) { l = ~l + 1ull; h = ~h + (l == 0ull ? 1ull : 0ull); }
+    // |x| >= 2^63: keep its top 63 bits with the rest folded into a sticky
+    // low bit (10 bits beyond a double's 53, so one rounding is exact).
+    const int p = h ? 127 - __builtin_clzll(h) : 63 - __builtin_clzll(l);  // leading bit, >= 63
+    const int sh = p - 62;  // 1 .. 65
+    unsigned long long top, rest;
+    if (sh < 64) {
+      top = (h << (64 - sh)) | (l >> sh);
+      rest = l & ((1ull << sh) - 1ull);
+    } else {
+      top = h >> (sh - 64);
+      rest = l | (h & ((1ull << (sh - 64)) - 1ull));
+    }
+    const double m = (double)(top | (rest ? 1ull : 0ull));
+    const double r = ldexp(m, sh);
+    return neg ? -r : r;
+  }
+};
 
 // Two-stage statistics: at most kStatBlocks per-block partials per node
 // (stats_blocks(rows) of them), folded by k_stats_reduce (one block per node;
