@@ -241,6 +241,20 @@ int wgbm_heap_predict(const int8_t* codes, int64_t n_rows, int32_t n_feat, const
                       const int32_t* class_first, const int32_t* blocks_of_class, int32_t n_classes,
                       int32_t n_blocks, const float* base, float* out, void* workspace, size_t workspace_bytes,
                       hipStream_t stream);
+/* wgbm_heap_predict with no workspace and no allocation of any kind.  No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The per-block partial sums live in a
+ * file-scope device array of 4 Mi floats (16 MiB) that every GPU the library
+ * runs on holds until the process ends; calls with n_rows * n_classes *
+ * n_blocks above 4 Mi return an error.  Every call on one GPU shares that
+ * GPU's array, so calls that overlap in time on one GPU race: serialise
+ * them (one stream, or events).  Results equal wgbm_heap_predict's bit for
+ * bit. */
+int wgbm_heap_predict_noalloc(const int8_t* codes, int64_t n_rows, int32_t n_feat, const uint32_t* nodes,
+                              const float* leaves, int32_t n_trees, int32_t depth, int32_t tree_block,
+                              const int32_t* class_first, const int32_t* blocks_of_class, int32_t n_classes,
+                              int32_t n_blocks, const float* base, float* out, hipStream_t stream);
 /* Traversal variant of wgbm_heap_predict, for tuning (process-wide): 0
  * automatic, 1 node reads through the cache with two trees per lane in
  * flight, 2 the same with four, 3 nodes staged in LDS 16 trees at a time.

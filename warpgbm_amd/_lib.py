@@ -75,6 +75,7 @@ SIGNATURES = {
     "wgbm_quantile_edges_workspace_bytes": (sz, [i64, i32]),
     "wgbm_quantile_edges": (C.c_int, [vp, i64, i32, i64, i32, vp, i32, vp, sz, vp]),
     "wgbm_heap_predict": (C.c_int, [vp, i64, i32, vp, vp, i32, i32, i32, vp, vp, i32, i32, vp, vp, vp, sz, vp]),
+    "wgbm_heap_predict_noalloc": (C.c_int, [vp, i64, i32, vp, vp, i32, i32, i32, vp, vp, i32, i32, vp, vp, vp]),
 }
 
 _lib = None

@@ -16,6 +16,7 @@
 //    are combined in a fixed order (tree block, then wave), so a row's
 //    result does not depend on which other rows share the launch.
 #include "common.h"
+#include <mutex>
 
 namespace wgbm {
 
@@ -220,10 +221,6 @@ k_heap_compile(const int32_t* __restrict__ status, const int32_t* __restrict__ f
   }
 }
 
-}  // namespace wgbm
-
-using namespace wgbm;
-
 extern "C" int wgbm_predict_forest(const int8_t* bin_indices, int64_t n_rows, int64_t n_feat, const float* trees,
                                    int64_t n_trees, int64_t max_nodes, float learning_rate, float* out,
                                    hipStream_t stream) {
@@ -259,23 +256,13 @@ extern "C" size_t wgbm_heap_predict_workspace_bytes(int64_t n_rows, int32_t n_cl
   return (size_t)n_rows * n_classes * n_blocks * sizeof(float);
 }
 
-// Serve one batch of rows: out[row*K + k] = base[k] + sum over class-k trees.
-// class_first [K+1]: tree ranges per class (trees of a class contiguous);
-// blocks_of_class [K]: ceil(trees of class / tree_block).
-extern "C" int wgbm_heap_predict(const int8_t* codes, int64_t n_rows, int32_t n_feat, const uint32_t* nodes,
-                                 const float* leaves, int32_t n_trees, int32_t depth, int32_t tree_block,
-                                 const int32_t* class_first, const int32_t* blocks_of_class, int32_t n_classes,
-                                 int32_t n_blocks, const float* base, float* out, void* workspace,
-                                 size_t workspace_bytes, hipStream_t stream) {
-  WGBM_REQUIRE(n_rows >= 0 && n_feat >= 1 && depth >= 1 && depth <= 16 && tree_block >= 1 && n_classes >= 1 &&
-                   n_blocks >= 1,
-               "bad heap predict arguments");
-  WGBM_REQUIRE((int64_t)n_feat * kRowsPerWG <= 96 * 1024, "too many features for the LDS row tile (%d)", n_feat);
-  WGBM_REQUIRE(workspace_bytes >= wgbm_heap_predict_workspace_bytes(n_rows, n_classes, n_blocks),
-               "predict workspace too small");
-  if (n_rows == 0) return kOk;
+// Traversal + fixed-order reduction of one batch into out, with the per-block
+// partial sums in part ([n_classes * n_blocks][n_rows] floats).
+static int heap_predict_launch(const int8_t* codes, int64_t n_rows, int32_t n_feat, const uint32_t* nodes,
+                               const float* leaves, int32_t n_trees, int32_t depth, int32_t tree_block,
+                               const int32_t* class_first, const int32_t* blocks_of_class, int32_t n_classes,
+                               int32_t n_blocks, const float* base, float* out, float* part, hipStream_t stream) {
   HeapForest hf{nodes, leaves, n_trees, depth, tree_block};
-  float* part = static_cast<float*>(workspace);
   const dim3 grid((unsigned)ceil_div(n_rows, kRowsPerWG), (unsigned)(n_classes * n_blocks));
   const size_t code_lds = ((size_t)n_feat * kRowsPerWG + 15) & ~(size_t)15;
   const size_t node_lds = (size_t)kSubTrees * ((1u << depth) - 1) * 4;
@@ -305,4 +292,82 @@ extern "C" int wgbm_heap_predict(const int8_t* codes, int64_t n_rows, int32_t n_
                      part, n_rows, n_classes, n_blocks, blocks_of_class, base, out);
   WGBM_LAUNCHED("k_predict_heap_final");
   return kOk;
+}
+
+static int heap_predict_check(int64_t n_rows, int32_t n_feat, int32_t depth, int32_t tree_block, int32_t n_classes,
+                              int32_t n_blocks) {
+  WGBM_REQUIRE(n_rows >= 0 && n_feat >= 1 && depth >= 1 && depth <= 16 && tree_block >= 1 && n_classes >= 1 &&
+                   n_blocks >= 1,
+               "bad heap predict arguments");
+  WGBM_REQUIRE((int64_t)n_feat * kRowsPerWG <= 96 * 1024, "too many features for the LDS row tile (%d)", n_feat);
+  return kOk;
+}
+
+// Partial sums of wgbm_heap_predict_noalloc: one array per GPU, held for the
+// life of the process.  Its host-side name is not a device address; the
+// launcher uses the address hipGetSymbolAddress returns on each device.
+constexpr int64_t kNoallocFloats = int64_t(4) << 20;  // 16 MiB
+static __device__ __align__(256) float scratch[kNoallocFloats];
+static float* g_scratch_ptr[64] = {};
+static std::mutex g_scratch_mu;
+
+// This device's address of `scratch`, looked up once per device (with that
+// device current) on its first call.
+static int noalloc_scratch(float** out) {
+  int dev = 0;
+  WGBM_HIP(hipGetDevice(&dev));
+  WGBM_REQUIRE(dev >= 0 && dev < 64, "device index %d out of range", dev);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[dev] == nullptr) {
+    void* p = nullptr;
+    WGBM_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    WGBM_REQUIRE(p != nullptr, "no device address for the predict scratch array");
+    g_scratch_ptr[dev] = static_cast<float*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return kOk;
+}
+
+}  // namespace wgbm
+
+using namespace wgbm;
+
+// Serve one batch of rows: out[row*K + k] = base[k] + sum over class-k trees.
+// class_first [K+1]: tree ranges per class (trees of a class contiguous);
+// blocks_of_class [K]: ceil(trees of class / tree_block).
+extern "C" int wgbm_heap_predict(const int8_t* codes, int64_t n_rows, int32_t n_feat, const uint32_t* nodes,
+                                 const float* leaves, int32_t n_trees, int32_t depth, int32_t tree_block,
+                                 const int32_t* class_first, const int32_t* blocks_of_class, int32_t n_classes,
+                                 int32_t n_blocks, const float* base, float* out, void* workspace,
+                                 size_t workspace_bytes, hipStream_t stream) {
+  int rc = heap_predict_check(n_rows, n_feat, depth, tree_block, n_classes, n_blocks);
+  if (rc != kOk) return rc;
+  WGBM_REQUIRE(workspace_bytes >= wgbm_heap_predict_workspace_bytes(n_rows, n_classes, n_blocks),
+               "predict workspace too small");
+  if (n_rows == 0) return kOk;
+  return heap_predict_launch(codes, n_rows, n_feat, nodes, leaves, n_trees, depth, tree_block, class_first,
+                             blocks_of_class, n_classes, n_blocks, base, out, static_cast<float*>(workspace), stream);
+}
+
+// wgbm_heap_predict without a workspace: the partial sums live in this GPU's
+// `scratch` array (no allocation of any kind).  Calls on one GPU share it,
+// so calls that overlap in time on one GPU race.
+extern "C" int wgbm_heap_predict_noalloc(const int8_t* codes, int64_t n_rows, int32_t n_feat, const uint32_t* nodes,
+                                         const float* leaves, int32_t n_trees, int32_t depth, int32_t tree_block,
+                                         const int32_t* class_first, const int32_t* blocks_of_class,
+                                         int32_t n_classes, int32_t n_blocks, const float* base, float* out,
+                                         hipStream_t stream) {
+  int rc = heap_predict_check(n_rows, n_feat, depth, tree_block, n_classes, n_blocks);
+  if (rc != kOk) return rc;
+  if ((int64_t)n_classes * n_blocks > kNoallocFloats || n_rows > kNoallocFloats / ((int64_t)n_classes * n_blocks)) {
+    set_error("predict without a workspace holds at most %lld partial sums (rows x classes x tree blocks); "
+              "%lld x %d x %d exceeds it", (long long)kNoallocFloats, (long long)n_rows, n_classes, n_blocks);
+    return kTooBig;
+  }
+  float* part = nullptr;
+  rc = noalloc_scratch(&part);  // every call, including one with no rows, sets the device's address up
+  if (rc != kOk) return rc;
+  if (n_rows == 0) return kOk;
+  return heap_predict_launch(codes, n_rows, n_feat, nodes, leaves, n_trees, depth, tree_block, class_first,
+                             blocks_of_class, n_classes, n_blocks, base, out, part, stream);
 }
