@@ -28,4 +28,14 @@ Pinning status of each function (what checks it besides itself):
       flattened array; chunk invariance.
   softmax_grad_hess                 -- closed form on 2-class inputs
       (sigmoid) and rows of the softmax summing to 1.
+  era_inverse                       -- torch.unique(..., return_inverse=True)
+      on the CPU (the routine the reference calls).
+  fit_regression                    -- closed form of single-leaf trees
+      (min_child_weight above the row count): F_t = ybar_w + (ybar -
+      ybar_w) (1 - lr S_w / (S_w + lambda))^t with the unweighted base ybar.
+  per_era_importance                -- brute force: each split's era-e gain
+      recomputed from the node's era-e rows directly; one era equals the
+      total gain.
+  predict_proba_binned              -- K = 2 sigmoid closed form of the two
+      class forests, rows summing to 1, class-permutation equivariance.
 """

@@ -358,6 +358,22 @@ def fit_regression(codes, y, w, eras, num_eras, num_bins, max_depth, n_trees, lr
     return base, recs, scores
 
 
+def per_era_importance(records, num_eras, num_features):
+    """Per-era gain importance [E, F] of grown trees (float64).
+
+    Follows warpgbm/core.py:1418-1423: every split adds the per-era gains
+    of its chosen (feature, bin) to that feature's column, for every era.
+    ``records``: TreeRecords from grow_tree (their split nodes carry the
+    per-era gain vector of the winning threshold).
+    """
+    imp = np.zeros((int(num_eras), int(num_features)), dtype=np.float64)
+    for rec in records:
+        for node in rec.nodes.values():
+            if node[0] == "split":
+                imp[:, node[1]] += np.asarray(node[3], dtype=np.float64)
+    return imp
+
+
 # ---------------------------------------------------------------- prediction
 def flatten_tree(tree, max_nodes):
     """Preorder flattening to [max_nodes, 6], warpgbm/core.py:2283-2328.
@@ -426,3 +442,23 @@ def predict_forest_flat(codes, flat_trees, lr, base=0.0):
             node[nl] = np.where(c <= b, flat[node[nl], 2], flat[node[nl], 3]).astype(np.int64)
         out += lr * flat[node, 5].astype(np.float64)
     return out
+
+
+def predict_proba_binned(codes, forest_rounds, lr, num_classes):
+    """Class probabilities of a multiclass forest on bin codes [N, K].
+
+    Follows warpgbm/core.py:2021-2058: raw score F[:, k] starts at the
+    uniform prior log(1/K + 1e-10) (computed in float64, stored in a float32
+    tensor), every class-k tree adds lr * leaf, and the probabilities are the
+    max-subtracted softmax of F (metrics.py:12-15).  ``forest_rounds``: a
+    list of boosting rounds, each a list of K nested-dict trees (the
+    estimator's ``forest`` layout for classification).
+    """
+    K = int(num_classes)
+    N = codes.shape[0]
+    prior = float(np.float32(np.log(1.0 / K + 1e-10)))
+    F = np.full((N, K), prior, dtype=np.float64)
+    for trees in forest_rounds:
+        for k in range(K):
+            F[:, k] += float(lr) * predict_tree(trees[k], codes)
+    return softmax(F)

@@ -259,3 +259,107 @@ def test_planted_mistakes_are_caught(monkeypatch):
     bg, bd = _brute_gain(codes, g, h, np.arange(N), 1, 2, 20.0, 1e-6)
     assert abs(bad_gain[0, 1, 2] - bg) > 1e-3
     assert bad_dir[0, 1, 2] != bd
+
+
+def test_era_inverse_matches_torch_unique():
+    """era_inverse restates torch.unique(era, return_inverse=True)
+    (warpgbm/core.py:1284); checked against torch's CPU routine."""
+    rng = np.random.default_rng(12)
+    for era in (rng.integers(-5, 40, 3000), rng.choice([7, 3, 1000, -2], 500), np.zeros(10, np.int64)):
+        uniq, inv = O.era_inverse(era)
+        tu, ti = torch.unique(torch.from_numpy(np.asarray(era).astype(np.int32)), return_inverse=True)
+        assert np.array_equal(uniq, tu.numpy()) and np.array_equal(inv, ti.numpy())
+        assert inv.dtype == np.int32
+
+
+def test_fit_regression_single_leaf_closed_form():
+    """With min_child_weight above the row count every tree is one leaf and
+    F_t = ybar_w + (ybar - ybar_w) * (1 - lr * S_w / (S_w + lambda))^t, with
+    the UNWEIGHTED base ybar (warpgbm/core.py:926), g = (y - F) w and h = w
+    (core.py:1545-1547).  A weighted base, a flipped gradient sign or a
+    dropped lambda all change this sequence."""
+    rng = np.random.default_rng(13)
+    N, F, B = 500, 4, 8
+    codes = rng.integers(0, B, (N, F)).astype(np.int8)
+    y = rng.standard_normal(N) * 3 + 1.5
+    w = rng.uniform(0.2, 3.0, N)
+    lr, lam, T = 0.3, 40.0, 7
+    base, recs, scores = O.fit_regression(codes, y, w, np.zeros(N, int), 1, B, 3, T, lr,
+                                          min_child_weight=w.sum() + 1.0, lam=lam)
+    ybar, yw, Sw = y.mean(), (w * y).sum() / w.sum(), w.sum()
+    assert base == ybar
+    for t, rec in enumerate(recs):
+        assert list(rec.nodes) == [0] and rec.nodes[0][0] == "leaf"
+        Ft = yw + (ybar - yw) * (1 - lr * Sw / (Sw + lam)) ** t
+        assert abs(rec.nodes[0][1] - Sw * (yw - Ft) / (Sw + lam)) < 1e-12 * max(1.0, abs(Ft))
+    FT = yw + (ybar - yw) * (1 - lr * Sw / (Sw + lam)) ** T
+    np.testing.assert_allclose(scores, FT, rtol=1e-12)
+
+
+def _node_rows(tree, codes, rows=None, heap=0, out=None):
+    """Rows reaching each heap node of a nested-dict tree."""
+    if out is None:
+        out, rows = {}, np.arange(codes.shape[0])
+    out[heap] = rows
+    if "leaf_value" not in tree:
+        m = codes[rows, tree["feature"]] <= tree["bin"]
+        _node_rows(tree["left"], codes, rows[m], 2 * heap + 1, out)
+        _node_rows(tree["right"], codes, rows[~m], 2 * heap + 2, out)
+    return out
+
+
+def test_per_era_importance_brute_force():
+    """Each split's era-e gain recomputed from the node's era-e rows alone
+    (direct sums, no histogram) and added per feature equals
+    per_era_importance (warpgbm/core.py:1418-1423); summed over eras of a
+    one-era fit it is the total split gain."""
+    rng = np.random.default_rng(14)
+    N, F, B, E = 3000, 5, 6, 3
+    codes = rng.integers(0, B, (N, F)).astype(np.int8)
+    eras = rng.integers(0, E, N)
+    g = rng.standard_normal(N) + 1.5 * (codes[:, 2] > 2) - (codes[:, 4] > 3)
+    h = rng.uniform(0.5, 1.5, N)
+    recs = [O.grow_tree(codes, g * s, h, eras, E, B, 3, 20, -np.inf, 1e-6) for s in (1.0, -0.7)]
+    imp = O.per_era_importance(recs, E, F)
+    ref = np.zeros((E, F))
+    n_splits = 0
+    for rec in recs:
+        nodes = _node_rows(rec.tree, codes)
+        for heap, node in rec.nodes.items():
+            if node[0] != "split":
+                continue
+            n_splits += 1
+            for e in range(E):
+                r = nodes[heap][eras[nodes[heap]] == e]
+                ref[e, node[1]] += _brute_gain(codes, g * (1.0 if rec is recs[0] else -0.7), h, r, node[1],
+                                               node[2], 20, 1e-6)[0]
+    assert n_splits >= 6
+    np.testing.assert_allclose(imp, ref, rtol=1e-9, atol=1e-9)
+    one = O.grow_tree(codes, g, h, np.zeros(N, int), 1, B, 3, 20, 0.0, 1e-6)
+    tot = sum(node[3][0] for node in one.nodes.values() if node[0] == "split")
+    assert abs(O.per_era_importance([one], 1, F).sum() - tot) < 1e-9 * abs(tot)
+
+
+def test_predict_proba_binned_closed_forms():
+    """K = 2: p1 = sigmoid(lr * (s1 - s0)) with s_k the class-k forest sums
+    (the uniform prior cancels); rows sum to 1; permuting the classes of
+    every round permutes the columns."""
+    rng = np.random.default_rng(15)
+    N, F, B = 2000, 6, 8
+    codes = rng.integers(0, B, (N, F)).astype(np.int8)
+
+    def tree(seed):
+        r = np.random.default_rng(seed)
+        return O.grow_tree(codes, r.standard_normal(N) + 2 * (codes[:, seed % F] > 3), np.ones(N),
+                           np.zeros(N, int), 1, B, 3, 20, 0.0, 1e-6).tree
+
+    rounds = [[tree(1), tree(2)], [tree(3), tree(4)]]
+    p = O.predict_proba_binned(codes, rounds, 0.1, 2)
+    s = [sum(O.predict_tree(rd[k], codes) for rd in rounds) for k in range(2)]
+    np.testing.assert_allclose(p[:, 1], 1.0 / (1.0 + np.exp(-0.1 * (s[1] - s[0]))), rtol=1e-12)
+    rounds4 = [[tree(10 + 4 * i + k) for k in range(4)] for i in range(3)]
+    p4 = O.predict_proba_binned(codes, rounds4, 0.2, 4)
+    np.testing.assert_allclose(p4.sum(axis=1), 1.0, rtol=1e-12)
+    perm = [2, 0, 3, 1]
+    p4p = O.predict_proba_binned(codes, [[rd[j] for j in perm] for rd in rounds4], 0.2, 4)
+    np.testing.assert_allclose(p4p, p4[:, perm], rtol=1e-12)

@@ -1,7 +1,8 @@
 """Evaluation metrics and softmax over PyTorch tensors.
 
-Same functions and formulas as warpgbm/metrics.py (softmax :12-15, weighted
-metrics :44-127).  These run once per evaluation round on the host's
+The softmax and the weighted metrics the estimator's evaluation uses, with
+the formulas of warpgbm/metrics.py (softmax :12-15, weighted metrics
+:46-50, :86-127).  These run once per evaluation round on the host's
 orchestration side; they are not part of the training hot path.
 """
 from __future__ import annotations
@@ -20,43 +21,9 @@ def _weights(w, like):
     return torch.ones_like(like, dtype=torch.float32) if w is None else w.to(dtype=torch.float32)
 
 
-def rmsle_torch(y_true, y_pred, eps=1e-7):
-    lt = torch.log1p(torch.clamp(y_true, min=0) + eps)
-    lp = torch.log1p(torch.clamp(y_pred, min=0) + eps)
-    return torch.sqrt(torch.mean((lt - lp) ** 2))
-
-
-def log_loss_torch(y_true_labels, y_pred_probs, eps=1e-15):
-    p = torch.clamp(y_pred_probs, eps, 1 - eps)
-    idx = torch.arange(y_true_labels.shape[0], device=p.device)
-    return -torch.mean(torch.log(p[idx, y_true_labels.long()]))
-
-
-def accuracy_torch(y_true_labels, y_pred_labels):
-    return (y_true_labels == y_pred_labels).float().mean()
-
-
 def weighted_mse_torch(y_true, y_pred, sample_weight=None):
     w = _weights(sample_weight, y_true)
     return torch.sum(w * (y_true - y_pred) ** 2) / torch.sum(w)
-
-
-def weighted_multi_mse_torch(y_true, y_pred, sample_weight=None, valid_mask=None):
-    if y_true.ndim != 2 or y_pred.ndim != 2:
-        raise ValueError("weighted_multi_mse_torch expects 2D tensors [n_samples, n_outputs].")
-    if y_true.shape != y_pred.shape:
-        raise ValueError(f"y_true and y_pred must have matching shape, got {y_true.shape} and {y_pred.shape}.")
-    if valid_mask is None:
-        valid_mask = torch.isfinite(y_true)
-    w = _weights(sample_weight, y_true)
-    if w.ndim == 1:
-        w = w.view(-1, 1)
-    w = w * valid_mask.to(torch.float32)
-    err = torch.where(valid_mask, (y_true - y_pred) ** 2, torch.zeros_like(y_pred))
-    den = torch.sum(w)
-    if den <= 0:
-        raise ValueError("No valid weighted targets available for multi-output MSE.")
-    return torch.sum(w * err) / den
 
 
 def weighted_rmsle_torch(y_true, y_pred, sample_weight=None, eps=1e-7):
