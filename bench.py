@@ -274,32 +274,7 @@ def serving_bench(args, dev):
     return res
 
 
-def era_workload(seed=0):
-    """500k x 400 pre-binned int8 codes in [0, 5); 200 contiguous eras with
-    lognormal(7.8, 0.5) sizes; target quintile-quantised from a linear signal
-    on 30 features whose sign flips on 10 of them in a random 40 % of eras."""
-    rng = np.random.default_rng(seed)
-    N, F, E = 500_000, 400, 200
-    X = rng.integers(0, 5, (N, F)).astype(np.int8)
-    s = rng.lognormal(7.8, 0.5, E)
-    s = np.floor(s / s.sum() * N).astype(int)
-    s[np.argmax(s)] += N - s.sum()
-    era = np.repeat(np.arange(E), s).astype(np.int32)
-    c = rng.standard_normal(30)
-    flip = np.ones((E, 30))
-    flip[np.ix_(rng.choice(E, int(0.4 * E), replace=False), np.arange(10))] = -1
-    sig = ((X[:, :30].astype(np.float32) - 2.0) * (c * flip[era])).sum(1)
-    y = (np.searchsorted(np.quantile(sig, [0.2, 0.4, 0.6, 0.8]), sig) / 4.0).astype(np.float32)
-    return X, y, era
-
-
-def multiclass_workload(seed=0):
-    """200k x 100 float32 ~N(0,1); 5 classes = argmax of 5 random projections
-    of 30 features plus Gumbel noise."""
-    rng = np.random.default_rng(seed)
-    X = rng.standard_normal((200_000, 100)).astype(np.float32)
-    lab = np.argmax(X[:, :30] @ rng.standard_normal((30, 5)) + rng.gumbel(size=(200_000, 5)), axis=1)
-    return X, lab
+from warpgbm_amd.synthetic import era_workload, multiclass_workload  # noqa: E402  (shared with the tests)
 
 
 def suite(dev):

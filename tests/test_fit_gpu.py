@@ -11,6 +11,7 @@ import numpy as np
 import pytest
 
 from oracle import gbm_oracle as O
+import _parity as PAR
 
 pytestmark = pytest.mark.gpu
 torch = pytest.importorskip("torch")
@@ -283,11 +284,12 @@ def test_benchmark_size_trees_match_plain_and_oracle_argmax():
     """The benchmark's configuration at its full size (1M x 1000 ~N(0,1),
     y = tanh(X[:, :20] w) + noise, 32 quantile bins, depth 5), 3 trees from
     device-resident codes: the batched engine's trees equal the plain
-    engine's, and the first tree's splits at depths 0 and 1 are argmaxes of
-    the FP64 oracle's gains over all 1000 x 31 thresholds, with histograms
-    counted from the same codes and gradients (the engine's gain at its
-    split within 1e-9 relative of the oracle's best, so a float near-tie
-    cannot fail it)."""
+    engine's, and every split of the first tree is an argmax of the FP64
+    oracle's gains over all 1000 x 31 thresholds, with histograms counted
+    from the same codes and gradients over the node's rows (the engine's
+    gain at its split within 1e-9 relative of the oracle's best, so a float
+    near-tie cannot fail it); its leaves hold exactly the rows its splits
+    send there, with FP64 leaf values within 1e-5."""
     from warpgbm_amd import WarpGBM
     dev = torch.device("cuda")
     gen = torch.Generator(device=dev)
@@ -316,10 +318,24 @@ def test_benchmark_size_trees_match_plain_and_oracle_argmax():
     yh = y.cpu().numpy()
     g = (yh - np.float32(mb.base_prediction)).astype(np.float64)
 
-    def check(node, rows, path):
+    worst_leaf = [0.0]
+
+    def check(node, rows, path, depth):
+        """Every node of tree 0, following the engine's own partition: each
+        split is an argmax of the FP64 gains over the node's rows, each leaf
+        holds exactly the rows the splits send there (row -> leaf
+        assignment) with the FP64 leaf value."""
+        gr = g[rows]
+        if "leaf_value" in node:
+            assert node["samples"] == rows.size, f"{path}: {node['samples']} rows vs {rows.size}"
+            ref = gr.sum() / (rows.size + 1e-6)
+            err = abs(node["leaf_value"] - ref) / max(abs(ref), 1e-3)
+            worst_leaf[0] = max(worst_leaf[0], err)
+            assert err <= 1e-5, f"{path}: leaf {node['leaf_value']} vs {ref}"
+            return
+        assert depth < 5
         G = np.zeros((1, F, B))
         H = np.zeros((1, F, B))
-        gr = g[rows]
         for f in range(F):
             c = cols[f][rows]
             G[0, f] = np.bincount(c, weights=gr, minlength=B)
@@ -331,12 +347,11 @@ def test_benchmark_size_trees_match_plain_and_oracle_argmax():
         best = gain[0, of, ob]
         assert gain[0, f, b] >= best - 1e-9 * abs(best), f"{path}: ({f},{b}) gain {gain[0, f, b]} < oracle ({of},{ob}) {best}"
         go_left = cols[f][rows] <= b
-        return rows[go_left], rows[~go_left]
+        check(node["left"], rows[go_left], path + "L", depth + 1)
+        check(node["right"], rows[~go_left], path + "R", depth + 1)
 
-    t = mb.forest[0]
-    left, right = check(t, np.arange(N), "root")
-    check(t["left"], left, "rootL")
-    check(t["right"], right, "rootR")
+    check(mb.forest[0], np.arange(N), "root", 0)
+    PAR.record("1M x 1000 tree 0 leaf values (relative)", worst_leaf[0], 1e-5)
 
 
 def test_skewed_eras_multi_chunk_segments_match_oracle():
@@ -605,3 +620,35 @@ def test_random_configurations_batched_equals_plain_and_oracle():
             rec = O.grow_tree(codes, g, np.ones_like(g), eu, int(eu.max()) + 1, B, D, mcw, 0.0, 1e-6)
             msg = _compare_dict_to_oracle(mb.forest[0], rec.tree, f"case {ci} root")
             assert msg is None, msg
+
+
+@pytest.mark.parametrize("weighted", [False, True])
+def test_high_dynamic_range_residuals_batched_equals_plain_and_oracle(weighted):
+    """95 % of the rows carry residuals near 1e-6, 5 % near +-1e3 (balanced,
+    so the base is ~0): nodes holding both take the count-packed words with
+    the exact fine path (unit weights) or 49-bit words (weights), in the
+    per-node op and in the batched level builds alike.  Both engines grow
+    the same trees, and tree 0 equals the oracle's grown from the same
+    float32 targets."""
+    from warpgbm_amd import WarpGBM
+    rng = np.random.default_rng(61)
+    n, f = 30000, 10
+    X = rng.standard_normal((n, f)).astype(np.float32)
+    big = rng.random(n) < 0.05
+    y = (1e-6 * (X[:, 1] + 0.3 * rng.standard_normal(n))).astype(np.float32)
+    sgn = np.where(np.arange(big.sum()) % 2 == 0, 1.0, -1.0)
+    y[big] = (1e3 * sgn * (1.0 + 0.1 * (X[big, 0] > 0))).astype(np.float32)
+    w = rng.uniform(0.5, 2.0, n).astype(np.float32) if weighted else None
+    kw = dict(objective="regression", num_bins=16, max_depth=5, n_estimators=4, learning_rate=0.5,
+              min_child_weight=5)
+    mb = WarpGBM(engine="batched", **kw).fit(X, y, sample_weight=w)
+    mp = WarpGBM(engine="plain", **kw).fit(X, y, sample_weight=w)
+    for i in range(4):
+        msg = _trees_equal(mb.forest[i], mp.forest[i], f"tree{i}:")
+        assert msg is None, msg
+    codes, _ = O.bin_matrix(X, 16)
+    ww = np.ones(n, np.float32) if w is None else mb.sample_weight_gpu.cpu().numpy()
+    g = ((y - np.float32(mb.base_prediction)) * ww).astype(np.float32)
+    rec = O.grow_tree(codes, g, ww.astype(np.float64), np.zeros(n, np.int32), 1, 16, 5, 5, 0.0, 1e-6)
+    msg = _compare_dict_to_oracle(mb.forest[0], rec.tree, "root")
+    assert msg is None, msg
