@@ -407,8 +407,14 @@ def main():
         log("profile-only fit done")
         return
 
+    tune = None
     for i in range(args.warmup):
-        fit("batched")
+        mw = fit("batched")
+        if tune is None and mw.hist_config_ is not None:
+            tune = {k: (list(v) if isinstance(v, tuple) else v) for k, v in mw.hist_config_.items()}
+            log(f"autotune: config {tune['config']} ({tune['ms_per_round']:.3f} ms/round) vs automatic "
+                f"{tune['auto_config']} ({tune['auto_ms_per_round']:.3f} ms/round), {tune['candidates']} "
+                f"candidates in {tune['tune_s']:.1f} s")
         log(f"warmup {i} done")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -526,6 +532,7 @@ def main():
                      "algorithmic_bytes_per_launch": alg_bytes // max(n_build, 1),
                      "kernel_ms_per_fit": build_ms, "built_rows_per_fit": built_rows,
                      "merge_ms_per_fit": merge_ms},
+        "autotune": tune,
         "plain": plain,
         "predict": {"rows_per_request": 10000, "p50_ms": float(np.percentile(lat, 50)),
                     "p99_ms": float(np.percentile(lat, 99)), "rows_per_s": 10000 / (np.mean(lat) * 1e-3),

@@ -155,6 +155,16 @@ int wgbm_bin_matrix(const float* X, int64_t n, int32_t n_feat, const float* edge
  * fit with constant weights): out4 = {chunk_rows, waves, fpl, depth}. */
 int wgbm_set_hist_config(int32_t chunk_rows, int32_t waves, int32_t features_per_lane, int32_t rows_in_flight);
 int wgbm_get_hist_config(int32_t n_feat, int32_t num_bins, int64_t n_cols, int32_t const_hessian, int32_t* out4);
+/* The same knobs for one histogram shape only (n_feat features of a bin
+ * matrix with n_cols columns, num_bins bins, const_hessian as above); they
+ * take precedence over wgbm_set_hist_config for that shape.  All-zero knobs
+ * remove the entry.  The autotuner installs its choice here, so a tiling
+ * tuned for one shape never reaches another.  A configuration that does not
+ * fit a workgroup's LDS at some shape is replaced by the automatic one.
+ * Takes over from the reference's threads_per_block / rows_per_thread
+ * launch knobs (warpgbm/core.py:68-69, :200-207). */
+int wgbm_set_hist_config_shape(int32_t n_feat, int32_t num_bins, int64_t n_cols, int32_t const_hessian,
+                               int32_t chunk_rows, int32_t waves, int32_t features_per_lane, int32_t rows_in_flight);
 
 /* Live timing of histogram launches with HIP events on their streams:
  * begin clears and enables; end waits and returns
@@ -241,6 +251,16 @@ int wgbm_heap_predict(const int8_t* codes, int64_t n_rows, int32_t n_feat, const
                       const int32_t* class_first, const int32_t* blocks_of_class, int32_t n_classes,
                       int32_t n_blocks, const float* base, float* out, void* workspace, size_t workspace_bytes,
                       hipStream_t stream);
+/* Heap-forest predict in tree order: out[row*K + k] = base[k] + lr * leaf of
+ * every class-k tree, added one tree at a time in tree order (no
+ * workspace).  The scores of a row equal, bit for bit, the scores training
+ * accumulated for it and what wgbm_predict_forest adds on the reference's
+ * flattened forest; this is the estimator's predict path
+ * (warpgbm/core.py:1902-1962, :2021-2058) with a compiled forest instead of
+ * the per-tree host flattening of core.py:2283-2328. */
+int wgbm_heap_predict_tree_order(const int8_t* codes, int64_t n_rows, int32_t n_feat, const uint32_t* nodes,
+                                 const float* leaves, int32_t n_trees, int32_t depth, const int32_t* class_first,
+                                 int32_t n_classes, const float* base, float* out, hipStream_t stream);
 /* wgbm_heap_predict with no workspace and no allocation of any kind.  No
  * counterpart in the reference: it is for callers that cannot allocate
  * device memory (code under HIP graph capture, tight serving loops,

@@ -205,3 +205,62 @@ def test_engine_routing_by_depth_and_classes():
     m.num_classes = 15
     assert not m._use_plain()
     assert WarpGBM(max_depth=3, engine="plain")._use_plain()
+
+
+def test_engine_routing_by_class_count_rows_and_memory(monkeypatch):
+    """More than 256 classes, classes x rows of 2^31 or more, or a batched
+    workspace above 80 % of the device's free memory route the fit to the
+    per-node engine (the batched engine's limits; the per-node engine has
+    none of them); node tables are written in calls of bounded size."""
+    import torch
+    from warpgbm_amd import WarpGBM, _lib
+    m = WarpGBM(objective="multiclass", max_depth=4, num_bins=32)
+    m.num_classes = 300
+    assert m._use_plain()
+    m.num_classes = 5
+    assert not m._use_plain()
+    m.num_samples, m.num_features, m.num_eras = 500_000, 2000, 500
+    m.max_depth = 10
+    m.num_bins = 5
+    assert not m._use_plain()  # no device to ask: the batched engine stays
+    need = _lib.query("wgbm_fit_workspace_bytes", ctypes.byref(m._fit_config(m._n_sampled(), 5)))
+    assert need > 10 << 30
+    monkeypatch.setattr(torch.cuda, "is_available", lambda: True)
+    monkeypatch.setattr(torch.cuda, "mem_get_info", lambda dev=None: (int(need / 0.8) + 1, 288 << 30))
+    m.device = "cuda:0"
+    assert not m._use_plain()
+    monkeypatch.setattr(torch.cuda, "mem_get_info", lambda dev=None: (int(need / 0.8) - 1, 288 << 30))
+    assert m._use_plain()
+    m.num_samples = (1 << 31) // 5 + 1
+    assert m._use_plain()
+    # node tables: 1000 rounds of depth-12, 500-era trees go in several calls
+    m.max_depth, m.num_eras = 12, 500
+    per_round = 5 * (2 ** 13 - 1) * (20 + 4 * 500)
+    assert m._rounds_per_call(5) == (1 << 30) // per_round
+    m1 = WarpGBM(max_depth=3)
+    m1.num_eras = 1
+    assert m1._rounds_per_call(1) > 1000
+
+
+def test_server_refuses_requests_after_close():
+    """submit() after close() (or after the worker thread died) raises
+    instead of queueing a request nothing would serve."""
+    import collections
+    import threading
+    from types import SimpleNamespace
+    from warpgbm_amd.server import PredictServer
+    srv = PredictServer.__new__(PredictServer)  # no device: the worker and streams are stand-ins
+    srv.forest = SimpleNamespace(n_feat=4, n_classes=1)
+    srv.max_batch_rows, srv.budget, srv.bytes_per_row = 100, 1 << 20, 16
+    srv._cv = threading.Condition()
+    srv._queue = collections.deque()
+    srv._stop = False
+    srv._thread = threading.Thread(target=lambda: None)
+    srv._thread.start()
+    srv._thread.join()  # a worker that has ended
+    with pytest.raises(RuntimeError, match="closed"):
+        srv.submit(np.zeros((3, 4), np.int8))
+    srv._stop = True
+    with pytest.raises(RuntimeError, match="closed"):
+        srv.submit(np.zeros((3, 4), np.int8))
+    assert not srv._queue

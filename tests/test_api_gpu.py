@@ -297,3 +297,38 @@ def test_autotune_installs_a_config_and_trees_do_not_change(tmp_path, monkeypatc
         assert (tmp_path / "tune.json").exists()
     finally:
         autotune.reset()
+
+
+def test_fit_tunes_the_histogram_tiling_per_shape(tmp_path, monkeypatch):
+    """A batched fit times the candidate tilings on a few rounds of its own
+    data (one LDS plane with equal weights, two with weights), installs the
+    fastest for that histogram shape only, and grows the trees an untuned
+    fit grows; the next fit of the shape takes the choice from the cache.
+    A tiling chosen for one shape never reaches another: after tuning at 5
+    bins, a 127-bin fit still runs."""
+    from warpgbm_amd import WarpGBM, autotune, _lib
+    monkeypatch.setenv("WARPGBM_AMD_TUNE_CACHE", str(tmp_path / "tune.json"))
+    monkeypatch.setattr(autotune, "_MEM", {})
+    X, y = _reg(n=20000, f=64, seed=4)
+    kw = dict(objective="regression", num_bins=32, max_depth=5, n_estimators=4, learning_rate=0.1)
+    try:
+        ref = WarpGBM(autotune=False, **kw).fit(X, y)
+        tuned = WarpGBM(**kw).fit(X, y)
+        info = tuned.hist_config_
+        assert info is not None and not info["cached"] and info["candidates"] >= 4
+        assert tuple(autotune.current(64, 32, 64, True)) == tuple(info["config"])
+        assert np.array_equal(ref.predict(X), tuned.predict(X))
+        for a, b in zip(ref.forest, tuned.forest):
+            assert a["leaf_value"] == b["leaf_value"] if "leaf_value" in a else a["bin"] == b["bin"]
+        again = WarpGBM(**kw).fit(X, y)
+        assert again.hist_config_["cached"] and tuple(again.hist_config_["config"]) == tuple(info["config"])
+        w = np.random.default_rng(1).uniform(0.5, 2, len(y)).astype(np.float32)
+        wt = WarpGBM(**kw).fit(X, y, sample_weight=w)  # two planes: a shape of its own
+        assert not wt.hist_config_["cached"]
+        Xi = np.random.default_rng(2).integers(0, 5, (20000, 64)).astype(np.int8)
+        m5 = WarpGBM(**dict(kw, num_bins=5)).fit(Xi, y)
+        m127 = WarpGBM(**dict(kw, num_bins=127)).fit(X, y)
+        assert m5.hist_config_ is not None and m127.hist_config_ is not None
+    finally:
+        for nb, one in ((32, 1), (32, 0), (5, 1), (127, 1)):
+            _lib.call("wgbm_set_hist_config_shape", 64, nb, 64, one, 0, 0, 0, 0)

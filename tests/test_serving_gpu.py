@@ -133,3 +133,29 @@ def test_traversal_variants_equal_bit_for_bit(model_and_codes, tree_block, depth
         _lib.call("wgbm_set_predict_mode", 0)
     assert np.array_equal(outs[0], outs[1])
     assert np.array_equal(outs[0], outs[2])
+
+
+def test_tree_order_heap_predict_equals_reference_op_bitwise(model_and_codes):
+    """The estimator's predict path (compiled heap forest, trees added in
+    tree order) equals the reference op on the flattened forest bit for bit,
+    for the trained model and for the same model saved and loaded (compiled
+    from its tree dicts)."""
+    import os
+    import tempfile
+    from warpgbm_amd import WarpGBM
+    from warpgbm_amd.cuda import node_kernel as nk
+    from warpgbm_amd.serving import HeapForest
+    m, codes = model_and_codes
+    n = codes.shape[0]
+    tt = m._get_cached_regression_tree_tensor()
+    ref = torch.full((n,), float(np.float32(m.base_prediction)), dtype=torch.float32, device=codes.device)
+    nk.predict_forest(codes, tt.contiguous(), m.learning_rate, ref)
+    hf = HeapForest.from_model(m)
+    assert torch.equal(hf.predict_tree_order(codes)[:, 0], ref)
+    assert torch.equal(m.predict_binned(codes), ref)
+    with tempfile.TemporaryDirectory() as d:
+        p = os.path.join(d, "m.pkl")
+        m.save_model(p)
+        m2 = WarpGBM().load_model(p)
+        m2.device = m.device
+        assert torch.equal(m2.predict_binned(codes), ref)

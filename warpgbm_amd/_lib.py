@@ -62,6 +62,7 @@ SIGNATURES = {
     "wgbm_bin_matrix_workspace_bytes": (sz, [i32, i32]),
     "wgbm_bin_matrix": (C.c_int, [vp, i64, i32, vp, i32, vp, vp, sz, vp]),
     "wgbm_set_hist_config": (C.c_int, [i32, i32, i32, i32]),
+    "wgbm_set_hist_config_shape": (C.c_int, [i32, i32, i64, i32, i32, i32, i32, i32]),
     "wgbm_get_hist_config": (C.c_int, [i32, i32, i64, i32, C.POINTER(i32)]),
     "wgbm_profile_begin": (C.c_int, []),
     "wgbm_profile_end": (C.c_int, [C.POINTER(f64)]),
@@ -75,6 +76,7 @@ SIGNATURES = {
     "wgbm_quantile_edges_workspace_bytes": (sz, [i64, i32]),
     "wgbm_quantile_edges": (C.c_int, [vp, i64, i32, i64, i32, vp, i32, vp, sz, vp]),
     "wgbm_heap_predict": (C.c_int, [vp, i64, i32, vp, vp, i32, i32, i32, vp, vp, i32, i32, vp, vp, vp, sz, vp]),
+    "wgbm_heap_predict_tree_order": (C.c_int, [vp, i64, i32, vp, vp, i32, i32, vp, i32, vp, vp, vp]),
     "wgbm_heap_predict_noalloc": (C.c_int, [vp, i64, i32, vp, vp, i32, i32, i32, vp, vp, i32, i32, vp, vp, vp]),
 }
 

@@ -1,14 +1,22 @@
 """Autotuner for the histogram kernel's tiling.
 
-For a histogram shape (features, bins, matrix columns) it times candidate
-configurations -- rows per chunk C, waves per workgroup W (the waves of a
+A configuration is rows per chunk C, waves per workgroup W (the waves of a
 workgroup share one LDS histogram), features per lane FPL (load width and
-LDS footprint) and load depth L -- on synthetic codes of that shape with
-the engine's own histogram op, and installs the fastest with
-``wgbm_set_hist_config``.  Results are cached per shape in memory and in a
-JSON file (``$WARPGBM_AMD_TUNE_CACHE``, default
-``~/.cache/warpgbm_amd/hist_tune.json``).
+LDS footprint, i.e. how many sub-histograms a workgroup holds) and load
+depth L (rows whose code loads are in flight per wave).  These take over
+from the reference's threads_per_block / rows_per_thread launch knobs
+(warpgbm/core.py:68-69, :200-207).
 
+* ``tune_fit`` (called by ``WarpGBM.fit`` on the batched engine): times
+  every candidate on a few boosting rounds of the fit's own data -- the
+  level builds of every depth, with the variant that fit runs (one LDS plane
+  when weights are equal) -- and installs the fastest for that histogram
+  shape only (``wgbm_set_hist_config_shape``).
+* ``tune``: times candidates on one synthetic root build with the per-node
+  histogram op and installs the winner process-wide (``wgbm_set_hist_config``).
+
+Results are cached per shape and device in memory and in a JSON file
+(``$WARPGBM_AMD_TUNE_CACHE``, default ``~/.cache/warpgbm_amd/hist_tune.json``).
 Histograms are exact fixed-point sums, so every configuration yields the
 same bits: tuning changes speed only, never a tree.
 """
@@ -54,12 +62,13 @@ def lds_bytes(fpl: int, n_bins: int, planes: int) -> int:
 
 
 def candidates(n_feat: int, n_bins: int, n_cols: int, const_hessian: bool = False):
-    """Configurations whose shared histogram fits a workgroup's 160 KB LDS
-    (two planes unless every row has the same hessian)."""
+    """Configurations whose shared histogram fits a workgroup's LDS (160 KB
+    less the kernel's own 1 KB; two planes unless every row has the same
+    hessian)."""
     out = []
     planes = 1 if const_hessian else 2
     for fpl, depths in ((1, (32, 64)), (2, (16, 32)), (4, (8, 16)), (8, (4, 8)), (16, (2, 4))):
-        if n_cols % fpl or lds_bytes(fpl, n_bins, planes) > 160 * 1024:
+        if n_cols % fpl or lds_bytes(fpl, n_bins, planes) > 159 * 1024:
             continue
         if fpl > 1 and 64 * fpl > 2 * max(n_feat, 1):
             continue  # slab far wider than the features
@@ -133,3 +142,49 @@ def tune(n_feat: int, n_bins: int, n_cols: int | None = None, rows: int = 262_14
         disk[key] = {"config": list(best), "ms": times[best]}
         _save_cache(disk)
     return best, times
+
+
+def tune_fit(model, cfg, n_feat, rounds: int = 2, repeats: int = 2, use_cache: bool = True, verbose=False):
+    """Pick the histogram configuration of a batched fit and install it for
+    the fit's histogram shape.  ``model``: a WarpGBM whose training data are
+    set up on the device; ``cfg``: its wgbm_fit_config.  Each candidate runs
+    ``rounds`` rounds on a copy of the scores (``WarpGBM._trial_rounds_ms``),
+    timed with HIP events around every histogram build and merge; the best
+    of ``repeats`` runs counts.  Returns a dict describing the choice."""
+    import time
+    one_plane = cfg.objective == 0 and cfg.const_weight != 0
+    B, n_cols = int(cfg.n_bins), int(cfg.n_cols)
+    dev = torch.device(model.device)
+    key = f"fit:{n_feat}x{B}x{n_cols}:{'one' if one_plane else 'two'}-plane@{torch.cuda.get_device_name(dev)}"
+    disk = _load_cache() if use_cache else {}
+    hit = _MEM.get(key) if use_cache else None
+    if hit is None and use_cache and key in disk:
+        hit = dict(disk[key], config=tuple(disk[key]["config"]))
+    if hit is not None:
+        _lib.call("wgbm_set_hist_config_shape", n_feat, B, n_cols, int(one_plane), *hit["config"])
+        _MEM[key] = hit
+        return dict(hit, cached=True)
+    t0 = time.perf_counter()
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(0)
+    _lib.call("wgbm_set_hist_config_shape", n_feat, B, n_cols, int(one_plane), 0, 0, 0, 0)
+    auto = current(n_feat, B, n_cols, one_plane)
+    model._trial_rounds_ms(cfg, 1, gen)  # warm-up: first launches of the kernels
+    times = {}
+    for c in [auto] + candidates(n_feat, B, n_cols, one_plane):
+        _lib.call("wgbm_set_hist_config_shape", n_feat, B, n_cols, int(one_plane), *c)
+        c = current(n_feat, B, n_cols, one_plane)  # the configuration in effect (depths are clamped)
+        if c in times:
+            continue
+        times[c] = min(model._trial_rounds_ms(cfg, rounds, gen) for _ in range(repeats)) / rounds
+        if verbose:
+            print(c, f"{times[c]:.3f} ms per round")
+    best = min(times, key=times.get)
+    _lib.call("wgbm_set_hist_config_shape", n_feat, B, n_cols, int(one_plane), *best)
+    res = {"config": tuple(best), "ms_per_round": times[best], "auto_config": tuple(auto),
+           "auto_ms_per_round": times[auto], "candidates": len(times), "tune_s": time.perf_counter() - t0}
+    _MEM[key] = res
+    if use_cache:
+        disk[key] = dict(res, config=list(best), auto_config=list(auto))
+        _save_cache(disk)
+    return dict(res, cached=False)

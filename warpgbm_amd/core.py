@@ -137,6 +137,7 @@ class WarpGBM(BaseEstimator, RegressorMixin):
         warm_start=False,
         monotonic_constraints=None,
         engine="batched",
+        autotune=True,
     ):
         self._validate_hyperparams(
             objective=objective, num_bins=num_bins, max_depth=max_depth, learning_rate=learning_rate,
@@ -162,6 +163,12 @@ class WarpGBM(BaseEstimator, RegressorMixin):
         self.warm_start = warm_start
         self.monotonic_constraints = monotonic_constraints
         self.engine = engine
+        # The batched engine times the histogram tilings on the first fit of
+        # each shape (a few rounds on the fit's own data, cached per shape) and
+        # trains with the fastest; it takes over from threads_per_block /
+        # rows_per_thread, which this engine accepts and ignores.
+        self.autotune = autotune
+        self.hist_config_ = None
         self.forest = [{} for _ in range(self.n_estimators)]
         self.bin_edges = None
         self.base_prediction = None
@@ -364,6 +371,7 @@ class WarpGBM(BaseEstimator, RegressorMixin):
 
     # ------------------------------------------------------------ caches
     def _invalidate_prediction_cache(self):
+        self._heap_cache = None
         self._cached_regression_tree_tensor = None
         self._cached_multi_regression_tree_tensors = None
         self._cached_multiclass_tree_tensors = None
@@ -449,6 +457,7 @@ class WarpGBM(BaseEstimator, RegressorMixin):
             self.forest = [{} for _ in range(self.n_estimators)] if self.objective == "regression" else []
             self.training_loss = []
             self.eval_loss = []
+            self._round_tables = []
             self._invalidate_prediction_cache()
         else:
             if X.shape[1] != self.num_features:
@@ -516,6 +525,7 @@ class WarpGBM(BaseEstimator, RegressorMixin):
         self._trees_trained = 0
         self.forest = [{} for _ in range(self.n_estimators)]
         self.training_loss, self.eval_loss = [], []
+        self._round_tables = []
         self._invalidate_prediction_cache()
         self.num_samples, self.num_features = N, F
         self.num_outputs = 1
@@ -693,14 +703,49 @@ class WarpGBM(BaseEstimator, RegressorMixin):
             return _to_device(X_np, self.device, None).to(torch.int8).contiguous()
         return self.bin_data_with_existing_edges(X_np)
 
+    # Share of the device's free memory the batched engine's workspace may
+    # take; above it the per-node engine (which needs about one node's
+    # histograms at a time) trains instead.
+    _BATCHED_MEM_SHARE = 0.8
+    # Bytes of node tables (device and host) one wgbm_fit_rounds call may
+    # write; more rounds go in several calls.
+    _TABLE_BYTES_PER_CALL = 1 << 30
+
+    def _n_channels(self):
+        return self.num_classes if self.objective in ("multiclass", "binary") else 1
+
     def _use_plain(self):
-        """The per-node engine runs when asked for, and for trees the batched
-        engine's level tables do not hold (depth above 20, or classes x 2^depth
-        of 2^24 or more); both engines grow identical trees."""
+        """The per-node engine runs when asked for, and for fits the batched
+        engine cannot hold: trees deeper than its level tables (depth above
+        20, or classes x 2^depth of 2^24 or more), more than 256 classes,
+        classes x rows of 2^31 or more, or a workspace above a share of the
+        device's free memory.  Both engines grow identical trees."""
         if self.engine == "plain":
             return True
-        K = self.num_classes if self.objective == "multiclass" else 1
-        return self.max_depth > 20 or (K << self.max_depth) >= (1 << 24)
+        K = self._n_channels()
+        if self.max_depth > 20 or (K << self.max_depth) >= (1 << 24) or K > 256:
+            return True
+        n = getattr(self, "num_samples", None)
+        if n is None:
+            return False
+        if n * K >= 0x7fffffff:
+            return True
+        need = _lib.query("wgbm_fit_workspace_bytes", ctypes.byref(self._fit_config(self._n_sampled(), K)))
+        if need == 0:
+            return True
+        dev = torch.device(self.device)
+        if dev.type == "cuda" and torch.cuda.is_available():
+            free, _ = torch.cuda.mem_get_info(dev)
+            if need > self._BATCHED_MEM_SHARE * free:
+                print(f"Batched engine needs {need / 2**30:.1f} GiB of workspace "
+                      f"({free / 2**30:.1f} GiB free): training per node instead.")
+                return True
+        return False
+
+    def _rounds_per_call(self, K):
+        """Rounds whose node tables fit _TABLE_BYTES_PER_CALL."""
+        per_round = K * (2 ** (self.max_depth + 1) - 1) * (20 + 4 * self.num_eras)
+        return max(1, int(self._TABLE_BYTES_PER_CALL // per_round))
 
     # ------------------------------------------------------------ plain per-node engine
     def compute_histograms(self, sample_indices, feature_indices, grad, hess):
@@ -799,6 +844,7 @@ class WarpGBM(BaseEstimator, RegressorMixin):
         if len(self.forest) < self.n_estimators:
             self.forest.extend([{} for _ in range(self.n_estimators - len(self.forest))])
         if self._use_plain():
+            self._round_tables = None  # trees without level tables: the heap forest is built from the dicts
             self._plain_setup(k)
             self.grad = torch.empty(self.num_samples, dtype=torch.float32, device=self.device)
             self.hess = torch.empty_like(self.grad)
@@ -832,6 +878,7 @@ class WarpGBM(BaseEstimator, RegressorMixin):
         start = self._trees_trained if self.warm_start and self._is_fitted else 0
         K = self.num_classes
         if self._use_plain():
+            self._round_tables = None
             self._plain_setup(k)
             g_all = torch.empty((K, self.num_samples), dtype=torch.float32, device=self.device)
             h_all = torch.empty_like(g_all)
@@ -877,8 +924,12 @@ class WarpGBM(BaseEstimator, RegressorMixin):
         c.lambda_ = float(self.L2_reg)
         # Equal weights make regression hessians constant: the histogram
         # kernel then keeps one LDS plane (checked once per fit on device).
-        w = self.sample_weight_gpu
+        w = getattr(self, "sample_weight_gpu", None)
         if getattr(self, "_const_weight", None) is None:
+            if w is None:
+                # a sizing query before the weights exist: assume equal weights
+                c.const_weight = 1
+                return c
             self._const_weight = bool(w.numel() == 0 or bool((w == w[0]).all()))
         c.const_weight = int(self._const_weight)
         return c
@@ -890,6 +941,9 @@ class WarpGBM(BaseEstimator, RegressorMixin):
         if total <= 0:
             return
         cfg = self._fit_config(k, n_classes)
+        if self.autotune:
+            from . import autotune
+            self.hist_config_ = autotune.tune_fit(self, cfg, k)
         ws = self._ws.get("fit", _lib.query("wgbm_fit_workspace_bytes", ctypes.byref(cfg)), self.device)
         every = self.eval_every_n_trees
         if every is not None:
@@ -900,6 +954,7 @@ class WarpGBM(BaseEstimator, RegressorMixin):
             else:
                 self._eval_scores = self._raw_multiclass_scores(self.bin_indices_eval)
         i = start
+        cap = self._rounds_per_call(n_classes)
         while i < self.n_estimators and not self.stop:
             if every is None:
                 n = self.n_estimators - i
@@ -908,6 +963,7 @@ class WarpGBM(BaseEstimator, RegressorMixin):
                 # a multiple of eval_every_n_trees): rounds in between only log
                 # their training loss, computed on the device
                 n = min(-(-i // every) * every - i + 1, self.n_estimators - i)
+            n = min(n, cap)
             feats = None
             if self.colsample_bytree < 1.0:
                 feats = torch.stack([torch.randperm(self.num_features, device=self.device, dtype=torch.int32)[:k]
@@ -964,6 +1020,26 @@ class WarpGBM(BaseEstimator, RegressorMixin):
         self._last_tables = tables
         return tables
 
+    def _trial_rounds_ms(self, cfg, n_rounds, feats_gen=None):
+        """Histogram time (build + merge launches, ms) of n_rounds boosting
+        rounds run on a copy of the scores: the model is left as it was."""
+        ws = self._ws.get("fit", _lib.query("wgbm_fit_workspace_bytes", ctypes.byref(cfg)), self.device)
+        feats = None
+        if cfg.n_feat < self.num_features:
+            feats = torch.stack([torch.randperm(self.num_features, generator=feats_gen, device=self.device,
+                                                dtype=torch.int32)[:cfg.n_feat] for _ in range(n_rounds)])
+            feats = feats.contiguous()
+        saved, last = self.gradients, getattr(self, "_last_tables", None)
+        self.gradients = saved.clone()
+        out = (ctypes.c_double * 4)()
+        try:
+            _lib.call("wgbm_profile_begin")
+            self._run_rounds(cfg, ws, n_rounds, feats)
+        finally:
+            _lib.call("wgbm_profile_end", out)
+            self.gradients, self._last_tables = saved, last
+        return out[0] + out[2]
+
     def _eval_add_rounds(self, tables, n, K):
         """Add the n new rounds' trees to the device eval scores: the rounds'
         node tables compile to a heap forest (no per-tree host flattening) whose
@@ -982,6 +1058,8 @@ class WarpGBM(BaseEstimator, RegressorMixin):
         Heap slot status: 1 split, 2 leaf, 0 not in the tree."""
         st, ft, bt, vt, ct, eg = (tables[k] for k in ("status", "feat", "bin", "value", "count", "era_gain"))
         NH = st.shape[-1]
+        if getattr(self, "_round_tables", None) is not None:
+            self._round_tables.append({"status": st[:n], "feat": ft[:n], "bin": bt[:n], "value": vt[:n]})
         pre = _heap_preorder(NH)
         # gains added split by split in growth (depth-first) order, as the
         # per-node path does: rounds, then classes, then splits in preorder
@@ -1070,8 +1148,41 @@ class WarpGBM(BaseEstimator, RegressorMixin):
                   f"Train LogLoss: {train_loss:.6f} | Eval {self.eval_metric}: {loss:.6f}")
 
     # ------------------------------------------------------------ prediction
+    def _heap_forest(self):
+        """The trained forest compiled for the device (cached per tree count):
+        from the batched engine's node tables when they cover every trained
+        round, else from the nested tree dicts (per-node engine, loaded or
+        warm-started models).  None for trees deeper than 16 levels."""
+        from .serving import HeapForest
+        if self.max_depth > 16 or not self._trees_trained:
+            return None
+        key = (self._trees_trained, str(self.device))
+        hc = getattr(self, "_heap_cache", None)
+        if hc is not None and hc[0] == key:
+            return hc[1]
+        K = self._n_channels()
+        if self.objective == "regression":
+            base = [float(self.base_prediction)]
+        else:
+            base = [float(np.log(np.float32(1.0 / K) + 1e-10))] * K  # the predict-time prior (core.py:2034-2037)
+        tabs = getattr(self, "_round_tables", None)
+        if tabs and sum(t["status"].shape[0] for t in tabs) == self._trees_trained:
+            NH = tabs[0]["status"].shape[-1]
+            cat = {k: np.concatenate([t[k] for t in tabs]).reshape(-1, NH) for k in ("status", "feat", "bin", "value")}
+            hf = HeapForest.from_tables(cat["status"], cat["feat"], cat["bin"], cat["value"], self.max_depth,
+                                        self.learning_rate, self.num_features, base, K, device=self.device)
+        else:
+            hf = HeapForest.from_model(self)
+        self._heap_cache = (key, hf)
+        return hf
+
     def predict_binned(self, bin_indices):
-        """base + sum_t lr*leaf over the flattened forest (core.py:1902-1962)."""
+        """base + sum_t lr*leaf in tree order (core.py:1902-1962): the
+        compiled heap forest, so no per-tree host flattening; trees deeper
+        than the heap forest's 16 levels use the flattened-forest op."""
+        hf = self._heap_forest()
+        if hf is not None:
+            return hf.predict_tree_order(bin_indices)[:, 0]
         tt = self._get_cached_regression_tree_tensor()
         out = torch.full((bin_indices.size(0),), np.float32(self.base_prediction), device=self.device,
                          dtype=torch.float32)
@@ -1101,6 +1212,9 @@ class WarpGBM(BaseEstimator, RegressorMixin):
 
     def _raw_multiclass_scores(self, bin_indices):
         """Per-class raw scores: the log(1/K) prior plus each class's forest."""
+        hf = self._heap_forest()
+        if hf is not None:
+            return hf.predict_tree_order(bin_indices)
         n = bin_indices.size(0)
         K = self.num_classes
         F = torch.zeros((n, K), device=self.device, dtype=torch.float32)
@@ -1196,6 +1310,7 @@ class WarpGBM(BaseEstimator, RegressorMixin):
                                                         self.monotonic_constraints_vector_ is not None)
         self.monotonic_required_split_direction_gpu_ = None
         self._feature_ids = None
+        self._round_tables = None  # a loaded forest is compiled from its tree dicts
         self._invalidate_prediction_cache()
         print(f"Model loaded from {path} ({self._trees_trained} trees)")
         return self

@@ -31,6 +31,7 @@
 #include "wgbm_internal.h"
 #include "wgbm_stats.h"
 #include "wgbm_plan.h"
+#include <mutex>
 #include <type_traits>
 #include <utility>
 #include <vector>
@@ -884,22 +885,63 @@ size_t lds_bytes(const HistConfig& c, int n_bins, bool count_mode) {
 }
 
 static HistConfig g_override;   // all-zero: automatic
+// Configurations installed for one histogram shape (the autotuner's
+// choices); they take precedence over the process-wide override.
+struct ShapeConfig {
+  int n_feat, n_bins;
+  int64_t n_cols;
+  bool one_plane;
+  HistConfig c;
+};
+static std::vector<ShapeConfig> g_shape_cfg;
+static std::mutex g_cfg_mu;
 int clamp_depth(int fpl, int waves, int depth);
 
-void set_hist_override(const HistConfig& c) { g_override = c; }
+void set_hist_override(const HistConfig& c) {
+  std::lock_guard<std::mutex> lock(g_cfg_mu);
+  g_override = c;
+}
 
-HistConfig pick_hist_config(int n_feat, int n_bins, int64_t row_stride, bool one_plane) {
-  HistConfig c = auto_hist_config(n_feat, n_bins, row_stride, one_plane);
-  if (g_override.chunk_rows > 0) c.chunk_rows = g_override.chunk_rows < kMaxChunkRows ? g_override.chunk_rows
-                                                                                     : kMaxChunkRows;
-  if (g_override.fpl > 0 && row_stride % g_override.fpl == 0) c.fpl = g_override.fpl;
+void set_hist_shape_config(int n_feat, int n_bins, int64_t n_cols, bool one_plane, const HistConfig& c) {
+  std::lock_guard<std::mutex> lock(g_cfg_mu);
+  for (size_t i = 0; i < g_shape_cfg.size(); ++i) {
+    ShapeConfig& e = g_shape_cfg[i];
+    if (e.n_feat == n_feat && e.n_bins == n_bins && e.n_cols == n_cols && e.one_plane == one_plane) {
+      if (c.chunk_rows == 0 && c.waves == 0 && c.fpl == 0 && c.depth == 0) g_shape_cfg.erase(g_shape_cfg.begin() + i);
+      else e.c = c;
+      return;
+    }
+  }
+  if (c.chunk_rows || c.waves || c.fpl || c.depth) g_shape_cfg.push_back({n_feat, n_bins, n_cols, one_plane, c});
+}
+
+static HistConfig apply_override(HistConfig c, const HistConfig& o, int64_t row_stride) {
+  if (o.chunk_rows > 0) c.chunk_rows = o.chunk_rows < kMaxChunkRows ? o.chunk_rows : kMaxChunkRows;
+  if (o.fpl > 0 && row_stride % o.fpl == 0) c.fpl = o.fpl;
   // 16-wave workgroups (two per CU, 64 VGPRs) for narrow lanes; wider lanes
   // need more registers than that.  Measured at 1M x 1000 x 32 bins, FPL 4:
   // 16 waves with 8 rows in flight build the root 10 % faster than 8 waves
   // with 16.
-  c.waves = g_override.waves > 0 ? g_override.waves : (c.fpl <= 4 ? 16 : 8);
+  c.waves = o.waves > 0 ? o.waves : (c.fpl <= 4 ? 16 : 8);
   if (c.fpl > 4 && c.waves > 8) c.waves = 8;
-  c.depth = clamp_depth(c.fpl, c.waves, g_override.depth);
+  c.depth = clamp_depth(c.fpl, c.waves, o.depth);
+  return c;
+}
+
+HistConfig pick_hist_config(int n_feat, int n_bins, int64_t row_stride, bool one_plane) {
+  const HistConfig a = auto_hist_config(n_feat, n_bins, row_stride, one_plane);
+  HistConfig o;
+  {
+    std::lock_guard<std::mutex> lock(g_cfg_mu);
+    o = g_override;
+    for (const ShapeConfig& e : g_shape_cfg)
+      if (e.n_feat == n_feat && e.n_bins == n_bins && e.n_cols == row_stride && e.one_plane == one_plane) o = e.c;
+  }
+  const HistConfig c = apply_override(a, o, row_stride);
+  // An override chosen for another shape may not fit this one's LDS (e.g.
+  // 16 features per lane tuned for 5 bins, then 127 bins): the automatic
+  // configuration builds it instead.
+  if (lds_bytes(c, n_bins, one_plane) > kLdsMax) return apply_override(a, HistConfig{}, row_stride);
   return c;
 }
 

@@ -163,6 +163,64 @@ k_predict_heap(const int8_t* __restrict__ codes, int64_t n, int32_t F, HeapFores
   }
 }
 
+// Tree-order traversal: one lane per row adds base[cls] and then the
+// class's trees one by one in tree order, exactly as training accumulates a
+// row's score and as the reference op adds its trees, so predicting the
+// training rows reproduces the training scores bit for bit.  INFLIGHT trees
+// are walked at once (their leaves are still added in order).  A one-wave
+// workgroup stages its 64 rows' codes in LDS.
+template <int INFLIGHT>
+__global__ void __launch_bounds__(64)
+k_predict_heap_ordered(const int8_t* __restrict__ codes, int64_t n, int32_t F, HeapForest hf,
+                       const int32_t* __restrict__ class_first, int32_t n_classes, const float* __restrict__ base,
+                       float* __restrict__ out) {
+  extern __shared__ int8_t s_codes[];  // [64][F]
+  const int64_t row0 = (int64_t)blockIdx.x * kRowsPerWG;
+  const int cls = blockIdx.y;
+  const int nrows = (int)min((int64_t)kRowsPerWG, n - row0);
+  const int64_t bytes = (int64_t)nrows * F;
+  const int8_t* src = codes + row0 * F;
+  if (((uintptr_t)src & 3) == 0) {
+    const int64_t w4 = bytes >> 2;
+    for (int64_t q = threadIdx.x; q < w4; q += 64)
+      reinterpret_cast<uint32_t*>(s_codes)[q] = reinterpret_cast<const uint32_t*>(src)[q];
+    for (int64_t q = (w4 << 2) + threadIdx.x; q < bytes; q += 64) s_codes[q] = src[q];
+  } else {
+    for (int64_t q = threadIdx.x; q < bytes; q += 64) s_codes[q] = src[q];
+  }
+  __syncthreads();
+  const int lane = threadIdx.x;
+  const int8_t* myrow = s_codes + (int64_t)(lane < nrows ? lane : 0) * F;
+  const int internal = (1 << hf.depth) - 1;
+  const int t1 = class_first[cls + 1];
+  float acc = base[cls];
+  int t = class_first[cls];
+  for (; t + INFLIGHT <= t1; t += INFLIGHT) {
+    int at[INFLIGHT];
+#pragma unroll
+    for (int k = 0; k < INFLIGHT; ++k) at[k] = 0;
+    for (int d = 0; d < hf.depth; ++d) {
+#pragma unroll
+      for (int k = 0; k < INFLIGHT; ++k) {
+        const uint32_t x = hf.nodes[(int64_t)(t + k) * internal + at[k]];
+        at[k] = 2 * at[k] + 1 + ((int)myrow[x >> 8] > (int)(x & 0xff));
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < INFLIGHT; ++k) acc = acc + hf.leaves[(int64_t)(t + k) * (internal + 1) + (at[k] - internal)];
+  }
+  for (; t < t1; ++t) {
+    const uint32_t* n0 = hf.nodes + (int64_t)t * internal;
+    int a = 0;
+    for (int d = 0; d < hf.depth; ++d) {
+      const uint32_t x = n0[a];
+      a = 2 * a + 1 + ((int)myrow[x >> 8] > (int)(x & 0xff));
+    }
+    acc = acc + hf.leaves[(int64_t)t * (internal + 1) + (a - internal)];
+  }
+  if (lane < nrows) out[(row0 + lane) * n_classes + cls] = acc;
+}
+
 static int g_predict_mode = 0;  // 0 automatic, 1 global x2, 2 global x4, 3 nodes in LDS
 
 // Allow up to 160 KB of dynamic LDS for a kernel, once per device.
@@ -370,4 +428,32 @@ extern "C" int wgbm_heap_predict_noalloc(const int8_t* codes, int64_t n_rows, in
   if (n_rows == 0) return kOk;
   return heap_predict_launch(codes, n_rows, n_feat, nodes, leaves, n_trees, depth, tree_block, class_first,
                              blocks_of_class, n_classes, n_blocks, base, out, part, stream);
+}
+
+// Tree-order predict of a heap forest (no workspace): out[row*K + k] =
+// base[k] + the class-k trees' leaves added one by one in tree order.
+extern "C" int wgbm_heap_predict_tree_order(const int8_t* codes, int64_t n_rows, int32_t n_feat,
+                                            const uint32_t* nodes, const float* leaves, int32_t n_trees,
+                                            int32_t depth, const int32_t* class_first, int32_t n_classes,
+                                            const float* base, float* out, hipStream_t stream) {
+  WGBM_REQUIRE(n_rows >= 0 && n_feat >= 1 && depth >= 1 && depth <= 16 && n_classes >= 1 && n_trees >= 0,
+               "bad heap predict arguments");
+  WGBM_REQUIRE((int64_t)n_feat * kRowsPerWG <= 96 * 1024, "too many features for the LDS row tile (%d)", n_feat);
+  if (n_rows == 0) return kOk;
+  HeapForest hf{nodes, leaves, n_trees, depth, 1};
+  const size_t lds = (size_t)n_feat * kRowsPerWG;
+  if (lds > 64 * 1024) {
+    static bool done[64] = {};
+    int dev = 0;
+    WGBM_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64 || !done[dev]) {
+      WGBM_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_predict_heap_ordered<4>),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, 96 * 1024));
+      if (dev >= 0 && dev < 64) done[dev] = true;
+    }
+  }
+  hipLaunchKernelGGL((k_predict_heap_ordered<4>), dim3((unsigned)ceil_div(n_rows, kRowsPerWG), (unsigned)n_classes),
+                     dim3(64), lds, stream, codes, n_rows, n_feat, hf, class_first, n_classes, base, out);
+  WGBM_LAUNCHED("k_predict_heap_ordered");
+  return kOk;
 }

@@ -79,6 +79,8 @@ struct HistArgs {
 HistConfig pick_hist_config(int n_feat, int n_bins, int64_t row_stride, bool one_plane);
 HistConfig auto_hist_config(int n_feat, int n_bins, int64_t row_stride, bool one_plane);
 void set_hist_override(const HistConfig& c);
+// Configuration for one shape (all-zero c removes it); precedes the override.
+void set_hist_shape_config(int n_feat, int n_bins, int64_t n_cols, bool one_plane, const HistConfig& c);
 size_t lds_bytes(const HistConfig& c, int n_bins, bool count_mode);
 // Bounds for rows_max rows in nseg_max segments: chunks of a launch,
 // multi-chunk segments, and partial words (one longlong2 each).

@@ -121,6 +121,9 @@ class PredictServer:
             raise ValueError("request exceeds the server's batch or memory budget")
         fut = Future()
         with self._cv:
+            # after close() (or a worker that died) nothing would ever serve it
+            if self._stop or not self._thread.is_alive():
+                raise RuntimeError("the predict server is closed")
             self._queue.append(_Request(codes, fut))
             self._cv.notify()
         return fut

@@ -117,6 +117,20 @@ class HeapForest:
                   out.data_ptr(), ws.data_ptr(), ws.numel(), st)
         return out
 
+    def predict_tree_order(self, codes, stream=None):
+        """[n, K] scores with every class's trees added one by one in tree
+        order after the base (the order training accumulates scores in)."""
+        n = codes.shape[0]
+        if codes.dtype != torch.int8 or codes.dim() != 2 or codes.shape[1] != self.n_feat:
+            raise RuntimeError(f"codes must be int8 [n, {self.n_feat}]")
+        codes = codes.contiguous()
+        out = torch.empty((n, self.n_classes), dtype=torch.float32, device=self.device)
+        st = stream if stream is not None else _stream(self.device)
+        _lib.call("wgbm_heap_predict_tree_order", codes.data_ptr(), n, self.n_feat, self.nodes.data_ptr(),
+                  self.leaves.data_ptr(), self.n_trees, self.depth, self.class_first.data_ptr(), self.n_classes,
+                  self.base.data_ptr(), out.data_ptr(), st)
+        return out
+
     def predict(self, codes):
         out = torch.empty((codes.shape[0], self.n_classes), dtype=torch.float32, device=self.device)
         return self.predict_into(codes.contiguous(), out)
