@@ -45,6 +45,8 @@ def parse():
     p.add_argument("--suite-only", action="store_true",
                    help="run only the many-era and multiclass workloads (for rocprof); prints their JSON")
     p.add_argument("--no-host-fit", action="store_true", help="skip the fit from host memory")
+    p.add_argument("--serve-only", action="store_true",
+                   help="run only the served-predict measurement; prints its JSON")
     p.add_argument("--no-graph", action="store_true",
                    help="launch every round eagerly (no hipGraph replay), e.g. under counter collection")
     p.add_argument("--serve-seconds", type=float, default=2.0, help="arrival window per serving run")
@@ -167,7 +169,7 @@ def run_unbatched(hf, reqs, arrivals, dev):
 
 def run_server(hf, reqs, arrivals, budget):
     from warpgbm_amd.server import PredictServer
-    srv = PredictServer(hf, memory_budget=budget, max_batch_rows=max(200_000, max(r.shape[0] for r in reqs)))
+    srv = PredictServer(hf, memory_budget=budget)
     for f in [srv.submit(r) for r in reqs[:4]]:  # warm the worker thread and its streams
         f.result(timeout=60)
     srv.latencies_ms.clear()
@@ -247,7 +249,7 @@ def serving_bench(args, dev):
 
     for mix, load in (("10k", 0.5), ("loguniform", 0.5), ("10k", 1.5)):
         # QPS: a fraction of what the one-launch-per-request loop sustains back to back
-        cal, _ = make_reqs(mix, 40)
+        cal, _ = make_reqs(mix, 400)  # enough draws that the log-uniform mean service time is stable
         lc, _ = run_unbatched(hf, cal, np.zeros(len(cal)), dev)
         service_ms = float(np.diff(np.concatenate([[0.0], np.sort(lc)])).mean())
         qps = load / (service_ms * 1e-3)
@@ -370,6 +372,9 @@ def main():
         _lib.call("wgbm_set_fit_graph", 0)
     if args.suite_only:
         print(json.dumps({"suite": suite(dev)}), flush=True)
+        return
+    if args.serve_only:
+        print(json.dumps({"serving": serving_bench(args, dev)}), flush=True)
         return
     log(f"device {torch.cuda.get_device_name(0)}; generating {args.rows}x{args.features}")
     X, y = make_data(args, dev)
