@@ -335,6 +335,69 @@ def trees_identical(fa, fb):
     return all(same(a, b) for a, b in zip(fa, fb))
 
 
+def cpu_baselines_other():
+    """The FP64 NumPy oracle on the other workloads, on the host cores of the
+    box (one core: NumPy without threads), each on a stated sample scaled
+    linearly: the 20k x 50 regression in full (20 trees); the era workload's
+    first tree on its first 100k rows (x 5 rows x 200 trees); the multiclass
+    workload's first round (5 class trees) on 50k rows (x 4 rows x 200
+    rounds); served predict of the 1000-tree depth-8 forest per request-size
+    class (a forest of the same shape grown by the oracle on random targets,
+    timed per row)."""
+    from oracle import gbm_oracle as O
+    from warpgbm_amd.synthetic import era_workload, multiclass_workload
+    out = {"cores_available": len(os.sched_getaffinity(0)), "cores_used": 1, "kind": "port"}
+    rng = np.random.default_rng(0)
+    # 20k x 50 regression, 32 bins, depth 4, 20 trees, lr 0.1: in full
+    X = rng.standard_normal((20000, 50)).astype(np.float32)
+    y = (X[:, :10] @ rng.standard_normal(10) + rng.normal(0, 0.1, 20000)).astype(np.float32)
+    t0 = time.perf_counter()
+    codes, _ = O.bin_matrix(X, 32)
+    O.fit_regression(codes, y, np.ones(20000), np.zeros(20000, np.int32), 1, 32, 4, 20, 0.1)
+    out["regression_20kx50_d4_20trees"] = {"value_s": time.perf_counter() - t0, "sample": "full fit incl. binning"}
+    # era workload: first tree, first 100k rows
+    Xe, ye, era = era_workload()
+    n = 100_000
+    g = (ye[:n] - np.float32(ye.mean())).astype(np.float64)
+    t0 = time.perf_counter()
+    O.grow_tree(Xe[:n], g, np.ones(n), era[:n], int(era.max()) + 1, 5, 8, 20, 0.0, 1e-6)
+    dt = time.perf_counter() - t0
+    out["eras_500kx400_E200_d8_200trees"] = {"value_s": dt * 5 * 200, "sample_s": dt,
+                                             "sample": "1 tree on the first 100k rows, x 5 (rows) x 200 (trees)"}
+    del Xe
+    # multiclass: first round on 50k rows
+    Xm, lab = multiclass_workload()
+    n = 50_000
+    t0 = time.perf_counter()
+    cm, _ = O.bin_matrix(Xm[:n], 32)
+    counts = np.bincount(lab[:n], minlength=5)
+    sc = np.tile(np.log(counts / n + 1e-10), (n, 1))
+    gk, hk = O.softmax_grad_hess(sc, lab[:n], np.ones(n))
+    for k in range(5):
+        O.grow_tree(cm, gk[:, k], hk[:, k], np.zeros(n, np.int32), 1, 32, 6, 20, 0.0, 1e-6)
+    dt = time.perf_counter() - t0
+    out["multiclass_200kx100_K5_d6_200rounds"] = {"value_s": dt * 4 * 200, "sample_s": dt,
+                                                  "sample": "round 1 (5 trees) on 50k rows, x 4 (rows) x 200 (rounds)"}
+    del Xm
+    # served predict: oracle traversal of 1000 depth-8 trees
+    F, nr = 400, 20000
+    cs = rng.integers(0, 32, (nr, F)).astype(np.int8)
+    tree = O.grow_tree(cs, rng.standard_normal(nr), np.ones(nr), np.zeros(nr, np.int32), 1, 32, 8, 1, -np.inf,
+                       1e-6).tree
+    flat = O.flatten_tree(tree, 2 ** 9)
+    per_class = {}
+    for rows in (100, 10_000):
+        reqc = cs[:rows]
+        t0 = time.perf_counter()
+        O.predict_forest_flat(reqc, [flat] * 100, 0.01)
+        dt = (time.perf_counter() - t0) * 10  # 100 of the 1000 trees timed
+        per_class[f"{rows}_rows_ms"] = dt * 1e3
+    per_class["100000_rows_ms"] = per_class["10000_rows_ms"] * 10
+    out["serving_1000x8_per_request"] = dict(per_class, sample=("100 of 1000 trees timed per request size, "
+                                                                "x 10; 100k rows = 10 x the 10k time"))
+    return out
+
+
 def cpu_baseline(codes_host, y_host, args):
     """FP64 oracle, one tree on the first cpu_rows rows, extrapolated."""
     from oracle import gbm_oracle as O
@@ -348,6 +411,7 @@ def cpu_baseline(codes_host, y_host, args):
         "value": dt * scale,
         "unit": "s",
         "cores": 1,
+        "cores_available": len(os.sched_getaffinity(0)),
         "kind": "port",
         "sample": (f"FP64 NumPy oracle grow_tree, 1 tree (depth {args.depth}, {args.bins} bins) on the first "
                    f"{n} rows x {args.features} features took {dt:.2f} s on 1 host core; value = that x "
@@ -505,13 +569,16 @@ def main():
     if not args.no_suite:
         others = suite(dev)
 
-    cpu = None
+    cpu = cpu_other = None
     if not args.no_cpu:
         n = min(args.cpu_rows, args.rows)
         ch = codes[:n].cpu().numpy()
         yh = y[:n].cpu().numpy()
         cpu = cpu_baseline(ch, yh, args)
         log(f"cpu oracle: {cpu['value']:.0f} s (extrapolated)")
+        cpu_other = cpu_baselines_other()
+        log("cpu oracle, other workloads: " + ", ".join(
+            f"{k} {v['value_s']:.0f} s" for k, v in cpu_other.items() if isinstance(v, dict) and "value_s" in v))
 
     res = {
         "metric": "fit sec (1M×1K, 100 trees) batched vs plain; hist GB/s; predict p99 ms",
@@ -547,6 +614,7 @@ def main():
         "other_workloads": others,
         "binning_s": bin_s,
         "cpu_baseline": cpu,
+        "cpu_baselines_other": cpu_other,
     }
     print(json.dumps(res))
 
