@@ -39,7 +39,11 @@ def parse():
     p.add_argument("--cpu-rows", type=int, default=200_000, help="rows of the CPU oracle sample")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-plain", action="store_true")
-    p.add_argument("--profile-only", action="store_true", help="one fit, no extras (for rocprof)")
+    p.add_argument("--profile-only", action="store_true",
+                   help="binning and one fit with the automatic tiling, no extras (for rocprof)")
+    p.add_argument("--profile-serve", action="store_true",
+                   help="the 1000-tree depth-8 serving forest: 40 traversal launches of 10k rows and 10 of "
+                        "100k rows, block and tree order (for rocprof)")
     p.add_argument("--no-serve", action="store_true", help="skip the served-predict measurement")
     p.add_argument("--no-suite", action="store_true", help="skip the many-era and multiclass workloads")
     p.add_argument("--suite-only", action="store_true",
@@ -276,6 +280,38 @@ def serving_bench(args, dev):
     return res
 
 
+def profile_serve(dev):
+    """Traversal launches on the serving forest for rocprof: 40 of 10k rows
+    and 10 of 100k rows with the block-order serving kernel, 10 of 100k rows
+    in tree order."""
+    import contextlib, io
+    import torch
+    from warpgbm_amd import WarpGBM
+    from warpgbm_amd.serving import HeapForest
+    a2 = argparse.Namespace(rows=200_000, features=400, bins=32)
+    X, y = make_data(a2, dev)
+    codes, edges = bin_on_device(X, 32)
+    del X
+    m = WarpGBM(objective="regression", num_bins=32, max_depth=8, n_estimators=1000, learning_rate=0.01,
+                engine="batched", autotune=False, device=dev)
+    with contextlib.redirect_stdout(io.StringIO()):
+        m.fit_binned(codes, y, bin_edges=edges)
+    tb = m._last_tables
+    hf = HeapForest.from_tables(*(tb[k].reshape(-1, tb[k].shape[-1]) for k in ("status", "feat", "bin", "value")),
+                                8, 0.01, 400, [m.base_prediction], device=dev)
+    for rows, reps in ((10_000, 40), (100_000, 10)):
+        req = codes[:rows].contiguous()
+        out = torch.empty((rows, 1), dtype=torch.float32, device=dev)
+        ws = hf.workspace(rows)
+        for _ in range(reps):
+            hf.predict_into(req, out, workspace=ws)
+    req = codes[:100_000].contiguous()
+    for _ in range(10):
+        hf.predict_tree_order(req)
+    torch.cuda.synchronize()
+    log("profile-serve done")
+
+
 from warpgbm_amd.synthetic import era_workload, multiclass_workload  # noqa: E402  (shared with the tests)
 
 
@@ -437,6 +473,9 @@ def main():
     if args.suite_only:
         print(json.dumps({"suite": suite(dev)}), flush=True)
         return
+    if args.profile_serve:
+        profile_serve(dev)
+        return
     if args.serve_only:
         print(json.dumps({"serving": serving_bench(args, dev)}), flush=True)
         return
@@ -453,11 +492,11 @@ def main():
     kw = dict(objective="regression", num_bins=args.bins, max_depth=args.depth, n_estimators=args.trees,
               learning_rate=0.01, colsample_bytree=1.0, device=dev)
 
-    def fit(engine, trees=None):
+    def fit(engine, trees=None, autotune=True):
         k = dict(kw)
         if trees:
             k["n_estimators"] = trees
-        m = WarpGBM(engine=engine, **k)
+        m = WarpGBM(engine=engine, autotune=autotune, **k)
         import contextlib, io
         with contextlib.redirect_stdout(io.StringIO()):
             m.fit_binned(codes, y, bin_edges=edges)
@@ -471,7 +510,7 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
-        fit("batched")
+        fit("batched", autotune=False)
         torch.cuda.synchronize()
         log("profile-only fit done")
         return

@@ -231,7 +231,10 @@ k_transpose_codes(const int8_t* __restrict__ codes, int64_t n_rows, int64_t n_co
 // packs one column's codes of 4 consecutive rows.
 __global__ void __launch_bounds__(256)
 k_transpose_codes4(const int8_t* __restrict__ codes, int64_t n_rows, int64_t n_cols, int8_t* __restrict__ codes_t) {
-  __shared__ uint32_t tile[64][17];  // [row][word]: 64 rows x 64 codes (+1 word pad)
+  // [row][word]: 64 rows x 64 codes; word w of row r sits in column
+  // w ^ ((r >> 2) & 15), so the 16 row groups a wave reads in the second
+  // phase (rows 4q + i, one word column) fall in 16 different banks.
+  __shared__ uint32_t tile[64][16];
   const int64_t r0 = (int64_t)blockIdx.x * 64, c0 = (int64_t)blockIdx.y * 64;
   const int t = threadIdx.x;
   const int w = t & 15;
@@ -239,7 +242,8 @@ k_transpose_codes4(const int8_t* __restrict__ codes, int64_t n_rows, int64_t n_c
   for (int k = 0; k < 4; ++k) {
     const int r = (t >> 4) + 16 * k;
     const int64_t rr = r0 + r, cc = c0 + 4 * w;
-    tile[r][w] = (rr < n_rows && cc < n_cols) ? *reinterpret_cast<const uint32_t*>(codes + rr * n_cols + cc) : 0u;
+    tile[r][w ^ ((r >> 2) & 15)] =
+        (rr < n_rows && cc < n_cols) ? *reinterpret_cast<const uint32_t*>(codes + rr * n_cols + cc) : 0u;
   }
   __syncthreads();
   const int q = t & 15;  // output word: rows 4q .. 4q+3
@@ -249,10 +253,11 @@ k_transpose_codes4(const int8_t* __restrict__ codes, int64_t n_rows, int64_t n_c
     const int64_t cc = c0 + c, rr = r0 + 4 * q;
     if (cc < n_cols && rr < n_rows) {
       const int sh = 8 * (c & 3);
-      const uint32_t b0 = (tile[4 * q + 0][c >> 2] >> sh) & 0xffu;
-      const uint32_t b1 = (tile[4 * q + 1][c >> 2] >> sh) & 0xffu;
-      const uint32_t b2 = (tile[4 * q + 2][c >> 2] >> sh) & 0xffu;
-      const uint32_t b3 = (tile[4 * q + 3][c >> 2] >> sh) & 0xffu;
+      const int wc = (c >> 2) ^ q;  // rows 4q .. 4q+3 share the swizzle q
+      const uint32_t b0 = (tile[4 * q + 0][wc] >> sh) & 0xffu;
+      const uint32_t b1 = (tile[4 * q + 1][wc] >> sh) & 0xffu;
+      const uint32_t b2 = (tile[4 * q + 2][wc] >> sh) & 0xffu;
+      const uint32_t b3 = (tile[4 * q + 3][wc] >> sh) & 0xffu;
       *reinterpret_cast<uint32_t*>(codes_t + cc * n_rows + rr) = b0 | (b1 << 8) | (b2 << 16) | (b3 << 24);
     }
   }

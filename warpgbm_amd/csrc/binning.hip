@@ -50,45 +50,66 @@ k_bin_matrix4(const float4* __restrict__ X, const float* __restrict__ et, uint32
 }
 
 // Tiled whole-matrix binning (F % 4 == 0, 16-byte aligned rows): a
-// workgroup takes 64 features x kBinRows rows.  The tile's edges sit in LDS
-// as [edge][64 features], so a lane's four features' k-th edges are one
-// 16-byte LDS read.  The code is the count of leading edges with x >= e
-// (a branch-free running AND), which equals the reference's early-exit scan
-// for any edges, NaN included.
-constexpr int kBinRows = 256;
+// workgroup takes 64 features x kBinRows rows, with the tile's edges in LDS,
+// one row of kEdgeStride floats per feature (component-major: the four
+// features a lane reads, 4c .. 4c+3, sit in rows c, 16 + c, 32 + c, 48 + c,
+// and kEdgeStride = 129 puts row r's word k in bank (r + k) mod 32, so the
+// 16 columns of a wave read 16 different banks when their probes agree).
+// A feature whose edges are non-decreasing and NaN-free takes a branch-free
+// binary search for the number of edges <= x, its edge row padded with NaN
+// to 2^m - 1 entries (x >= NaN is false, so no probe passes the last edge;
+// NaN x gives 0); the probes of all 16 searches of a thread (4 rows x 4
+// features) are interleaved, so their LDS latencies overlap.  This equals
+// the reference's early-exit scan for such edges; any other feature takes
+// that scan as a running AND.
+constexpr int kBinRows = 1024;
+constexpr int kEdgeStride = 129;  // >= 127 edges, odd
+
+__device__ __forceinline__ int edge_row(int f) { return ((f & 3) << 4) + (f >> 2); }
+
+__device__ __forceinline__ uint32_t code_scan(const float* __restrict__ ef, int nb, float x) {
+  uint32_t a = 1, c = 0;
+  for (int k = 0; k < nb; ++k) {
+    a &= (uint32_t)(x >= ef[k]);
+    c += a;
+  }
+  return c;
+}
+
+__device__ __forceinline__ uint32_t code_search(const float* __restrict__ ef, int top, float x) {
+  int lo = 0;
+  for (int step = top; step > 0; step >>= 1) lo = (x >= ef[lo + step - 1]) ? lo + step : lo;
+  return (uint32_t)lo;
+}
 
 __global__ void __launch_bounds__(256)
 k_bin_tile(const float4* __restrict__ X, const float* __restrict__ et, uint32_t* __restrict__ out, int64_t n,
            int F, int nb) {
-  __shared__ float4 s_e[127][16];
-  __shared__ int s_sorted[16];
+  __shared__ float s_e[64 * kEdgeStride];
+  __shared__ int s_sorted[64];
   const int f0 = blockIdx.y * 64;
   const int nf = min(64, F - f0);  // a multiple of 4
-  for (int q = threadIdx.x; q < nb * 16; q += 256) {
-    const int k = q >> 4, c = q & 15;
-    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (4 * c < nf) v = *reinterpret_cast<const float4*>(et + (int64_t)k * F + f0 + 4 * c);
-    s_e[k][c] = v;
+  int top = 1;
+  while (2 * top <= nb) top *= 2;
+  const int padded = 2 * top - 1;  // >= nb; probes stay below it
+  for (int q = threadIdx.x; q < padded * 64; q += 256) {
+    const int k = q >> 6, f = q & 63;
+    s_e[edge_row(f) * kEdgeStride + k] = (f < nf && k < nb) ? et[(int64_t)k * F + f0 + f] : __builtin_nanf("");
   }
   __syncthreads();
-  if (threadIdx.x < 16) {  // are the column's four edge lists non-decreasing and NaN-free?
-    const int c = threadIdx.x;
-    bool ok = true;
-    float4 p = s_e[0][c];
-    ok = !(p.x != p.x || p.y != p.y || p.z != p.z || p.w != p.w);
-    for (int k = 1; k < nb; ++k) {
-      const float4 e = s_e[k][c];
-      ok = ok && (e.x >= p.x) && (e.y >= p.y) && (e.z >= p.z) && (e.w >= p.w);  // false for NaN
-      p = e;
-    }
-    s_sorted[c] = ok;
+  if (threadIdx.x < 64) {  // is the feature's edge list non-decreasing and NaN-free?
+    const float* ef = s_e + edge_row(threadIdx.x) * kEdgeStride;
+    bool ok = ef[0] == ef[0];
+    for (int k = 1; k < nb; ++k) ok = ok && (ef[k] >= ef[k - 1]);  // false for NaN
+    s_sorted[threadIdx.x] = ok;
   }
   __syncthreads();
   const int c = threadIdx.x & 15;   // float4 column within the tile
   const int r = threadIdx.x >> 4;   // row within a pass of 16
   if (4 * c >= nf) return;
-  // Sorted NaN-free edges: the leading count is the plain count of edges <= x.
-  const bool sorted = s_sorted[c] != 0;
+  const float* e0 = s_e + c * kEdgeStride;  // feature 4c; 4c + j is 16 j rows further
+  constexpr int kPlane = 16 * kEdgeStride;
+  const bool all_sorted = s_sorted[4 * c] && s_sorted[4 * c + 1] && s_sorted[4 * c + 2] && s_sorted[4 * c + 3];
   const int64_t F4 = F >> 2;
   const int64_t row0 = (int64_t)blockIdx.x * kBinRows;
   constexpr int RB = 4;  // rows per thread whose loads are in flight together
@@ -101,29 +122,44 @@ k_bin_tile(const float4* __restrict__ X, const float* __restrict__ et, uint32_t*
       idx[j] = row < n ? row * F4 + (f0 >> 2) + c : -1;
       v[j] = idx[j] >= 0 ? X[idx[j]] : make_float4(0.f, 0.f, 0.f, 0.f);
     }
+    uint32_t code[RB][4];
+    if (all_sorted) {
+      int lo[RB][4];
 #pragma unroll
-    for (int j = 0; j < RB; ++j) {
-      uint32_t c0 = 0, c1 = 0, c2 = 0, c3 = 0;
-      if (sorted) {
-        for (int k = 0; k < nb; ++k) {
-          const float4 e = s_e[k][c];
-          c0 += (uint32_t)(v[j].x >= e.x);
-          c1 += (uint32_t)(v[j].y >= e.y);
-          c2 += (uint32_t)(v[j].z >= e.z);
-          c3 += (uint32_t)(v[j].w >= e.w);
-        }
-      } else {
-        uint32_t a0 = 1, a1 = 1, a2 = 1, a3 = 1;
-        for (int k = 0; k < nb; ++k) {
-          const float4 e = s_e[k][c];
-          a0 &= (uint32_t)(v[j].x >= e.x); c0 += a0;
-          a1 &= (uint32_t)(v[j].y >= e.y); c1 += a1;
-          a2 &= (uint32_t)(v[j].z >= e.z); c2 += a2;
-          a3 &= (uint32_t)(v[j].w >= e.w); c3 += a3;
+      for (int j = 0; j < RB; ++j)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) lo[j][q] = 0;
+#pragma unroll
+      for (int s = 6; s >= 0; --s) {  // steps 64 .. 1, those up to top
+        const int step = 1 << s;
+        if (step > top) continue;  // uniform
+#pragma unroll
+        for (int j = 0; j < RB; ++j) {
+          const float xs[4] = {v[j].x, v[j].y, v[j].z, v[j].w};
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const float e = e0[q * kPlane + lo[j][q] + step - 1];
+            lo[j][q] = (xs[q] >= e) ? lo[j][q] + step : lo[j][q];
+          }
         }
       }
-      if (idx[j] >= 0) out[idx[j]] = c0 | (c1 << 8) | (c2 << 16) | (c3 << 24);
+#pragma unroll
+      for (int j = 0; j < RB; ++j)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) code[j][q] = (uint32_t)lo[j][q];
+    } else {
+#pragma unroll
+      for (int j = 0; j < RB; ++j) {
+        const float xs[4] = {v[j].x, v[j].y, v[j].z, v[j].w};
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          code[j][q] = s_sorted[4 * c + q] ? code_search(e0 + q * kPlane, top, xs[q])
+                                           : code_scan(e0 + q * kPlane, nb, xs[q]);
+      }
     }
+#pragma unroll
+    for (int j = 0; j < RB; ++j)
+      if (idx[j] >= 0) out[idx[j]] = code[j][0] | (code[j][1] << 8) | (code[j][2] << 16) | (code[j][3] << 24);
   }
 }
 
