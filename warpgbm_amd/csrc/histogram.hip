@@ -335,8 +335,10 @@ k_hist_build(const HistArgs a) {
       return v;
     };
     // Fixed-point words of a block's rows (lane = position in the block);
-    // fmask: the block's fine rows (their words carry only the count).
-    unsigned long long myqp, myqh, fmask;
+    // a fine row's word carries only its count, and lane_fine records that
+    // the lane met one (the fine pass after the loop adds their gradients).
+    unsigned long long myqp, myqh;
+    bool lane_fine = false;
     auto block_words = [&](int rowv_, float2 ghv_) {
       const bool okr = rowv_ >= 0;
       float g = ghv_.x, h = ghv_.y;
@@ -348,10 +350,9 @@ k_hist_build(const HistArgs a) {
       if (packed) {
         const bool fine = okr && g != 0.0f && q < kFineLimit && q > -kFineLimit;
         myqp = okr ? (1ull << kCountShift) + (fine ? 0ull : (unsigned long long)q) : 0ull;
-        fmask = __ballot(fine);
+        lane_fine = lane_fine || fine;
       } else {
         myqp = okr ? (unsigned long long)q : 0ull;
-        fmask = 0ull;
       }
       myqh = okr ? (unsigned long long)to_fixed(h, ns.sh) : 0ull;
     };
@@ -390,7 +391,6 @@ k_hist_build(const HistArgs a) {
       // that precedes the read-out (which takes the words back to zero).
       __builtin_amdgcn_s_waitcnt(0);
     };
-    unsigned long long any_fine = 0ull;
     int rowv = valid_row(w0 + lane, first_raw);
     block_words(rowv, first_gh);
     TV A[LG], Bv[LG];
@@ -439,11 +439,10 @@ k_hist_build(const HistArgs a) {
 #pragma unroll
       for (int q = 0; q < LG; ++q) A[q] = load_codes(__builtin_amdgcn_readlane(rown, q));
       add_group(Bv, (NG - 1) * LG);
-      any_fine |= fmask;
       rowv = rown;
       block_words(rowv, ghn);
     }
-    if (any_fine) fine_pass();
+    if (__ballot(lane_fine)) fine_pass();
   };
   if (has_rows) {
     if (ident) run(std::true_type{});
