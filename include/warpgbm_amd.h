@@ -112,6 +112,13 @@ int wgbm_hist_subtract(const float* parent_g, const float* parent_h, const float
 int wgbm_grad_regression(const float* y, const float* pred, const float* w, int64_t n, float* g_out, float* h_out,
                          hipStream_t stream);
 
+/* Multi-output targets with shared splits (warpgbm/core.py:1666-1679), y,
+ * pred, w [n, n_outputs] row-major; per output m, valid = isfinite(y):
+ * g[m*n + i] = valid ? (y - pred) * w : 0, h[m*n + i] = valid ? w : 0, and
+ * split_g[i] / split_h[i] = float32 sums of g / h over m in output order. */
+int wgbm_grad_multi_regression(const float* y, const float* pred, const float* w, int64_t n, int32_t n_outputs,
+                               float* g_out, float* h_out, float* split_g, float* split_h, hipStream_t stream);
+
 /* Softmax targets for all classes (warpgbm/core.py:339-367, :1755-1756):
  * g[k*n + i] = -(p_ik - [y_i == k]) * w_i, h[k*n + i] = max(p_ik(1-p_ik), 1e-6) * w_i. */
 int wgbm_grad_softmax(const float* scores, const int32_t* labels, const float* w, int64_t n, int32_t n_classes,

@@ -38,4 +38,11 @@ Pinning status of each function (what checks it besides itself):
       total gain.
   predict_proba_binned              -- K = 2 sigmoid closed form of the two
       class forests, rows summing to 1, class-permutation equivariance.
+  fit_multi_regression / grow_tree_multi / multi_regression_targets /
+  predict_multi                     -- one output with unit weights equals
+      fit_regression; an all-zero-weight output leaves the trees of the
+      other output's regression fit and gets zero leaves and base; NaN
+      targets equal zero weights; base by a row-by-row weighted sum; output
+      permutation equivariance; each leaf vector by direct sums over the
+      rows reaching it; predict_multi reproduces the fit's scores.
 """

@@ -358,6 +358,82 @@ def fit_regression(codes, y, w, eras, num_eras, num_bins, max_depth, n_trees, lr
     return base, recs, scores
 
 
+def multi_regression_targets(Y, scores, W):
+    """Per-output tree targets of the multi-output objective (float64).
+
+    Follows warpgbm/core.py:1666-1673: a target is valid where it is finite;
+    g = (y - F) * w and h = w on valid entries, 0 elsewhere.  Returns
+    (g, h), both [N, M].
+    """
+    Y = np.asarray(Y, np.float64)
+    valid = np.isfinite(Y)
+    w = np.where(valid, np.asarray(W, np.float64), 0.0)
+    g = np.where(valid, Y - np.asarray(scores, np.float64), 0.0) * w
+    return g, w
+
+
+def grow_tree_multi(codes, g, h, eras, num_eras, num_bins, max_depth, min_child_weight, min_split_gain, lam,
+                    feats=None, mono_required=None, split_g=None, split_h=None):
+    """Shared-split tree with a vector of leaf values (float64).
+
+    Follows warpgbm/core.py:1674-1679 (splits searched on the per-row sums
+    over outputs of g and h) and :1366-1375 (leaf m = sum g_m / (sum h_m +
+    lambda) over the leaf's rows).  ``split_g``/``split_h`` replace the
+    row sums when given (teacher forcing with a device build's float32
+    sums).  Returns a TreeRecord whose leaves hold lists of M values.
+    """
+    g = np.asarray(g, np.float64)
+    h = np.asarray(h, np.float64)
+    sg = g.sum(axis=1) if split_g is None else split_g
+    sh = h.sum(axis=1) if split_h is None else split_h
+    rec = grow_tree(codes, sg, sh, eras, num_eras, num_bins, max_depth, min_child_weight, min_split_gain, lam,
+                    feats=feats, mono_required=mono_required)
+    M = g.shape[1]
+
+    def vec(heap):
+        rows = np.nonzero(rec.leaf_of_row == heap)[0]
+        return [leaf_value(g[:, m], h[:, m], rows, lam) for m in range(M)]
+
+    def walk(node, heap):
+        if "leaf_value" in node:
+            v = vec(heap)
+            node["leaf_value"] = v
+            rec.nodes[heap] = ("leaf", v, rec.nodes[heap][2])
+            return
+        walk(node["left"], 2 * heap + 1)
+        walk(node["right"], 2 * heap + 2)
+
+    walk(rec.tree, 0)
+    return rec
+
+
+def fit_multi_regression(codes, Y, W, eras, num_eras, num_bins, max_depth, n_trees, lr, min_child_weight=20,
+                         min_split_gain=0.0, lam=1e-6):
+    """Boosting loop of the multi-output objective, warpgbm/core.py:1619-1717.
+
+    base_m = weighted mean of output m's valid targets, 0 when it has no
+    weight (core.py:1055-1063); each round grows one shared-split tree and
+    adds lr * leaf vector to every row's scores.  Returns (base [M], list
+    of TreeRecord, final scores [N, M]).
+    """
+    Y = np.asarray(Y, np.float64)
+    valid = np.isfinite(Y)
+    w = np.where(valid, np.asarray(W, np.float64), 0.0)
+    num = (np.where(valid, Y, 0.0) * w).sum(axis=0)
+    den = w.sum(axis=0)
+    base = np.where(den > 0, num / np.maximum(den, 1e-12), 0.0)
+    scores = np.tile(base, (Y.shape[0], 1))
+    recs = []
+    for _ in range(n_trees):
+        g, h = multi_regression_targets(Y, scores, W)
+        rec = grow_tree_multi(codes, g, h, eras, num_eras, num_bins, max_depth, min_child_weight, min_split_gain,
+                              lam)
+        vals = np.array([rec.nodes[int(k)][1] for k in rec.leaf_of_row], dtype=np.float64)
+        scores = scores + lr * vals
+        recs.append(rec)
+    return base, recs, scores
+
+
 def per_era_importance(records, num_eras, num_features):
     """Per-era gain importance [E, F] of grown trees (float64).
 
@@ -417,6 +493,27 @@ def predict_tree(tree, codes):
         walk(node["right"], rows[~m])
 
     walk(tree, np.arange(codes.shape[0]))
+    return out
+
+
+def predict_multi(codes, trees, lr, base):
+    """[N, M] = base + sum over trees of lr * leaf vector (float64), the
+    multi-output prediction of warpgbm/core.py:1904-1935 and the score
+    update of :1366-1375."""
+    base = np.asarray(base, np.float64).reshape(-1)
+    out = np.tile(base, (codes.shape[0], 1))
+    for tree in trees:
+        def walk(node, rows):
+            if rows.size == 0:
+                return
+            if "leaf_value" in node:
+                out[rows] += float(lr) * np.asarray(node["leaf_value"], np.float64)
+                return
+            m = codes[rows, int(node["feature"])] <= int(node["bin"])
+            walk(node["left"], rows[m])
+            walk(node["right"], rows[~m])
+
+        walk(tree, np.arange(codes.shape[0]))
     return out
 
 

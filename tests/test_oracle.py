@@ -363,3 +363,116 @@ def test_predict_proba_binned_closed_forms():
     perm = [2, 0, 3, 1]
     p4p = O.predict_proba_binned(codes, [[rd[j] for j in perm] for rd in rounds4], 0.2, 4)
     np.testing.assert_allclose(p4p, p4[:, perm], rtol=1e-12)
+
+
+def _multi_data(seed, N=1500, F=6, B=16, M=3):
+    rng = np.random.default_rng(seed)
+    codes = rng.integers(0, B, (N, F)).astype(np.int8)
+    Y = np.stack([2.0 * (codes[:, m % F] > B // 2) + codes[:, (m + 1) % F] * 0.3 * (m + 1) + rng.standard_normal(N)
+                  for m in range(M)], axis=1)
+    return rng, codes, Y
+
+
+def _same_structure(a, b):
+    if ("leaf_value" in a) != ("leaf_value" in b):
+        return False
+    if "leaf_value" in a:
+        return a["samples"] == b["samples"]
+    return ((a["feature"], a["bin"]) == (b["feature"], b["bin"]) and _same_structure(a["left"], b["left"])
+            and _same_structure(a["right"], b["right"]))
+
+
+def _leaves(t):
+    return [t["leaf_value"]] if "leaf_value" in t else _leaves(t["left"]) + _leaves(t["right"])
+
+
+def test_fit_multi_regression_one_output_is_the_regression_fit():
+    """M = 1 with unit weights reduces to the regression objective (pinned by
+    its closed form above): same trees, leaves and scores."""
+    _, codes, Y = _multi_data(21, M=1)
+    eras = np.zeros(len(Y), int)
+    base_m, recs_m, sc_m = O.fit_multi_regression(codes, Y, np.ones_like(Y), eras, 1, 16, 3, 6, 0.2)
+    base_r, recs_r, sc_r = O.fit_regression(codes, Y[:, 0], np.ones(len(Y)), eras, 1, 16, 3, 6, 0.2)
+    assert abs(base_m[0] - base_r) < 1e-12
+    for a, b in zip(recs_m, recs_r):
+        assert _same_structure(a.tree, b.tree)
+        np.testing.assert_allclose([v[0] for v in _leaves(a.tree)], _leaves(b.tree), rtol=1e-9, atol=1e-12)
+    np.testing.assert_allclose(sc_m[:, 0], sc_r, rtol=1e-9, atol=1e-9)
+
+
+def test_fit_multi_regression_unweighted_output_adds_nothing_to_the_splits():
+    """An output with weight 0 everywhere contributes nothing to the summed
+    split targets: the trees are output 0's regression trees, that output's
+    leaves are 0 and its base is 0 (core.py:1061-1063)."""
+    _, codes, Y = _multi_data(22, M=2)
+    W = np.ones_like(Y)
+    W[:, 1] = 0.0
+    eras = np.zeros(len(Y), int)
+    base, recs, sc = O.fit_multi_regression(codes, Y, W, eras, 1, 16, 3, 5, 0.3)
+    _, recs_r, sc_r = O.fit_regression(codes, Y[:, 0], np.ones(len(Y)), eras, 1, 16, 3, 5, 0.3)
+    assert base[1] == 0.0
+    for a, b in zip(recs, recs_r):
+        assert _same_structure(a.tree, b.tree)
+        lv = np.array(_leaves(a.tree))
+        np.testing.assert_allclose(lv[:, 0], _leaves(b.tree), rtol=1e-9, atol=1e-12)
+        assert np.all(lv[:, 1] == 0.0)
+    np.testing.assert_allclose(sc[:, 0], sc_r, rtol=1e-9, atol=1e-9)
+    assert np.all(sc[:, 1] == 0.0)
+
+
+def test_fit_multi_regression_missing_targets_are_zero_weights_and_base_brute_force():
+    """A NaN target is the same as that entry's weight set to 0 (same trees,
+    leaves, scores); the base is output by output the weighted mean of the
+    finite targets, summed row by row."""
+    rng, codes, Y = _multi_data(23, M=3)
+    W = rng.uniform(0.5, 2.0, Y.shape)
+    miss = rng.random(Y.shape) < 0.2
+    Yn = Y.copy()
+    Yn[miss] = np.nan
+    Yn[0, 0] = np.inf
+    miss[0, 0] = True
+    eras = np.zeros(len(Y), int)
+    b1, r1, s1 = O.fit_multi_regression(codes, Yn, W, eras, 1, 16, 3, 4, 0.3)
+    b2, r2, s2 = O.fit_multi_regression(codes, np.where(miss, 0.0, Y), np.where(miss, 0.0, W), eras, 1, 16, 3, 4, 0.3)
+    np.testing.assert_array_equal(b1, b2)
+    for a, b in zip(r1, r2):
+        assert _same_structure(a.tree, b.tree)
+        np.testing.assert_array_equal(np.array(_leaves(a.tree)), np.array(_leaves(b.tree)))
+    np.testing.assert_array_equal(s1, s2)
+    for m in range(3):
+        num = den = 0.0
+        for i in range(len(Y)):
+            if not miss[i, m]:
+                num += W[i, m] * Y[i, m]
+                den += W[i, m]
+        assert abs(b1[m] - num / den) < 1e-12 * max(1.0, abs(num / den))
+
+
+def test_fit_multi_regression_output_permutation_and_leaf_brute_force():
+    """Permuting the outputs permutes every leaf vector and the scores; each
+    leaf's value for output m equals sum g_m / (sum h_m + lambda) over the
+    rows that reach it, summed directly."""
+    rng, codes, Y = _multi_data(24, M=3)
+    W = rng.uniform(0.5, 2.0, Y.shape)
+    eras = np.zeros(len(Y), int)
+    perm = [2, 0, 1]
+    lam = 0.5
+    b1, r1, s1 = O.fit_multi_regression(codes, Y, W, eras, 1, 16, 3, 3, 0.3, lam=lam)
+    b2, r2, s2 = O.fit_multi_regression(codes, Y[:, perm], W[:, perm], eras, 1, 16, 3, 3, 0.3, lam=lam)
+    np.testing.assert_allclose(b2, b1[perm], rtol=1e-12)
+    np.testing.assert_allclose(s2, s1[:, perm], rtol=1e-9, atol=1e-9)
+    for a, b in zip(r1, r2):
+        assert _same_structure(a.tree, b.tree)
+        np.testing.assert_allclose(np.array(_leaves(b.tree)), np.array(_leaves(a.tree))[:, perm], rtol=1e-9,
+                                   atol=1e-12)
+    # leaf brute force on the first tree, whose targets start from the base
+    rows_of = _node_rows(r1[0].tree, codes)
+    g = (Y - b1) * W
+    for heap, node in r1[0].nodes.items():
+        if node[0] != "leaf":
+            continue
+        rows = rows_of[heap]
+        for m in range(3):
+            ref = sum(g[i, m] for i in rows) / (sum(W[i, m] for i in rows) + lam)
+            assert abs(node[1][m] - ref) < 1e-9 * max(1.0, abs(ref))
+    np.testing.assert_allclose(O.predict_multi(codes, [r.tree for r in r1], 0.3, b1), s1, rtol=1e-9, atol=1e-9)

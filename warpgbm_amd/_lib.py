@@ -58,6 +58,7 @@ SIGNATURES = {
     "wgbm_hist_subtract": (C.c_int, [vp, vp, vp, vp, vp, vp, i64, vp]),
     "wgbm_grad_regression": (C.c_int, [vp, vp, vp, i64, vp, vp, vp]),
     "wgbm_grad_softmax": (C.c_int, [vp, vp, vp, i64, i32, vp, vp, vp]),
+    "wgbm_grad_multi_regression": (C.c_int, [vp, vp, vp, i64, i32, vp, vp, vp, vp, vp]),
     "wgbm_edges_from_sorted": (C.c_int, [vp, i64, i32, i32, vp, i32, vp]),
     "wgbm_bin_matrix_workspace_bytes": (sz, [i32, i32]),
     "wgbm_bin_matrix": (C.c_int, [vp, i64, i32, vp, i32, vp, vp, sz, vp]),

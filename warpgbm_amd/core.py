@@ -37,6 +37,7 @@ from .metrics import (
     weighted_corr_loss_torch,
     weighted_log_loss_torch,
     weighted_mse_torch,
+    weighted_multi_mse_torch,
     weighted_rmsle_torch,
 )
 
@@ -439,13 +440,14 @@ class WarpGBM(BaseEstimator, RegressorMixin):
         early_stopping_rounds = self.validate_fit_params(
             X, y, era_id, X_eval, y_eval, eval_every_n_trees, early_stopping_rounds, eval_metric,
             sample_weight, sample_weight_eval)
-        if self.objective == "multi_regression":
-            raise NotImplementedError("objective='multi_regression' is not served by this engine yet.")
-        sample_weight = (np.ones(X.shape[0], dtype=np.float32) if sample_weight is None
-                         else self._normalize_sample_weight(sample_weight))
+        multi = self.objective == "multi_regression"
+        # multi-output weights default to ones of y's shape (core.py:759-772)
+        sample_weight = ((np.ones_like(y, dtype=np.float32) if multi else np.ones(X.shape[0], dtype=np.float32))
+                         if sample_weight is None else self._normalize_sample_weight(sample_weight))
         if X_eval is not None and y_eval is not None:
-            sample_weight_eval = (np.ones(X_eval.shape[0], dtype=np.float32) if sample_weight_eval is None
-                                  else self._normalize_sample_weight(sample_weight_eval))
+            sample_weight_eval = ((np.ones_like(y_eval, dtype=np.float32) if multi
+                                   else np.ones(X_eval.shape[0], dtype=np.float32))
+                                  if sample_weight_eval is None else self._normalize_sample_weight(sample_weight_eval))
         if self.random_state is not None:
             torch.manual_seed(self.random_state)
             np.random.seed(self.random_state)
@@ -454,7 +456,8 @@ class WarpGBM(BaseEstimator, RegressorMixin):
         if not self.warm_start or not self._is_fitted:
             self._is_fitted = False
             self._trees_trained = 0
-            self.forest = [{} for _ in range(self.n_estimators)] if self.objective == "regression" else []
+            self.forest = ([{} for _ in range(self.n_estimators)]
+                           if self.objective in ("regression", "multi_regression") else [])
             self.training_loss = []
             self.eval_loss = []
             self._round_tables = []
@@ -484,6 +487,9 @@ class WarpGBM(BaseEstimator, RegressorMixin):
                 raise ValueError(f"multiclass objective requires at least 2 classes, got {self.num_classes}")
             return self._fit_classification(X, y_enc, era_id, X_eval, y_eval, sample_weight, sample_weight_eval,
                                             eval_every_n_trees, early_stopping_rounds, eval_metric)
+        if multi:
+            return self._fit_multi_regression(X, y, era_id, X_eval, y_eval, sample_weight, sample_weight_eval,
+                                              eval_every_n_trees, early_stopping_rounds, eval_metric)
         return self._fit_regression(X, y, era_id, X_eval, y_eval, sample_weight, sample_weight_eval,
                                     eval_every_n_trees, early_stopping_rounds, eval_metric)
 
@@ -638,6 +644,64 @@ class WarpGBM(BaseEstimator, RegressorMixin):
         del self.Y_gpu
         return self
 
+    def _fit_multi_regression(self, X, y, era_id, X_eval, y_eval, sample_weight, sample_weight_eval,
+                              eval_every_n_trees, early_stopping_rounds, eval_metric):
+        """Vector-leaf multi-output regression (warpgbm/core.py:960-1129).
+
+        Targets stay as given on the device: a non-finite target is a missing
+        one (weight 0 in the gradients, the base and the losses), which is what
+        the reference's NaN fill plus finite mask amounts to."""
+        M = y.shape[1]
+        if self.warm_start and self._is_fitted and self.num_outputs is not None and self.num_outputs != M:
+            raise ValueError("Number of target outputs must match the fitted model when warm_start=True. "
+                             f"Got {M} and {self.num_outputs}.")
+        self.num_outputs = M
+        y32 = np.ascontiguousarray(y, dtype=np.float32)
+        w2 = self._expand_multi_output_weight(np.asarray(sample_weight, np.float32), M)
+        self._setup_common(X, y32, era_id, w2)
+        if self.warm_start and self._is_fitted and self._trees_trained > 0:
+            base = torch.as_tensor(self.base_prediction, dtype=torch.float32).reshape(-1)
+            if base.numel() != M:
+                raise ValueError("Stored base_prediction is incompatible with the requested number "
+                                 f"of outputs ({M}).")
+            self.base_prediction = base.to(self.device)
+            self.gradients = self._multi_scores(self.bin_indices)
+        else:
+            # weighted mean of the valid targets per output, 0 where an output has no weight
+            # (core.py:1055-1063), accumulated in float64
+            valid = np.isfinite(y32)
+            ew = np.where(valid, w2, 0).astype(np.float64)
+            num = (np.where(valid, y32, 0).astype(np.float64) * ew).sum(axis=0)
+            den = ew.sum(axis=0)
+            base = np.where(den > 0, num / np.maximum(den, 1e-12), 0.0).astype(np.float32)
+            self.base_prediction = torch.from_numpy(base).to(self.device)
+            self.gradients = self.base_prediction.reshape(1, M).repeat(self.num_samples, 1).contiguous()
+        if X_eval is not None and y_eval is not None:
+            self.bin_indices_eval = self.bin_inference_data(X_eval)
+            self.Y_gpu_eval = torch.from_numpy(np.ascontiguousarray(y_eval, np.float32)).to(self.device)
+            self.sample_weight_eval_gpu = torch.from_numpy(np.ascontiguousarray(
+                self._expand_multi_output_weight(np.asarray(sample_weight_eval, np.float32), M))).to(self.device)
+            self.eval_every_n_trees = eval_every_n_trees
+            self.early_stopping_rounds = early_stopping_rounds
+            self.eval_metric = eval_metric
+            self._eval_scores = None
+        else:
+            self._setup_eval(None, None, None, None, None, eval_metric, classification=False)
+        with torch.no_grad():
+            self.grow_forest_multi_regression()
+        del self.bin_indices
+        del self.Y_gpu
+        return self
+
+    @staticmethod
+    def _expand_multi_output_weight(w, n_outputs):
+        """1-D weights repeated over the outputs (warpgbm/core.py:580-586)."""
+        if w.ndim == 1:
+            return np.repeat(w[:, None], n_outputs, axis=1)
+        if w.ndim == 2:
+            return w
+        raise ValueError(f"sample weights must be 1D or 2D, got shape {w.shape}")
+
     # ------------------------------------------------------------ binning
     def preprocess_gpu_data(self, X_np, Y_np, era_id_np):
         """Codes, eras and edges on the device (warpgbm/core.py:1226-1285)."""
@@ -712,15 +776,18 @@ class WarpGBM(BaseEstimator, RegressorMixin):
     _TABLE_BYTES_PER_CALL = 1 << 30
 
     def _n_channels(self):
-        return self.num_classes if self.objective in ("multiclass", "binary") else 1
+        if self.objective in ("multiclass", "binary"):
+            return self.num_classes
+        return self.num_outputs if self.objective == "multi_regression" else 1
 
     def _use_plain(self):
-        """The per-node engine runs when asked for, and for fits the batched
-        engine cannot hold: trees deeper than its level tables (depth above
-        20, or classes x 2^depth of 2^24 or more), more than 256 classes,
-        classes x rows of 2^31 or more, or a workspace above a share of the
-        device's free memory.  Both engines grow identical trees."""
-        if self.engine == "plain":
+        """The per-node engine runs when asked for, for vector-leaf
+        multi-output regression, and for fits the batched engine cannot
+        hold: trees deeper than its level tables (depth above 20, or
+        classes x 2^depth of 2^24 or more), more than 256 classes, classes x
+        rows of 2^31 or more, or a workspace above a share of the device's
+        free memory.  Both engines grow identical trees."""
+        if self.engine == "plain" or self.objective == "multi_regression":
             return True
         K = self._n_channels()
         if self.max_depth > 20 or (K << self.max_depth) >= (1 << 24) or K > 256:
@@ -774,6 +841,16 @@ class WarpGBM(BaseEstimator, RegressorMixin):
     def _plain_leaf(self, node_indices, class_k):
         n = node_indices.numel()
         ws = self._ws.get("leaf", _lib.query("wgbm_leaf_workspace_bytes", n), self.device)
+        if self.objective == "multi_regression" and class_k is None:
+            # vector leaf: per output m, sum(g_m)/(sum(h_m)+lambda) over the node's
+            # rows, and that output's score column updated (core.py:1366-1375)
+            M = self.num_outputs
+            for m in range(M):
+                _lib.call("wgbm_leaf_value", self.grad_out[m].data_ptr(), self.hess_out[m].data_ptr(),
+                          node_indices.data_ptr(), n, self.num_samples, float(self.L2_reg),
+                          float(self.learning_rate), self.gradients.data_ptr(), M, m,
+                          self._leaf_out[m].data_ptr(), ws.data_ptr(), ws.numel(), _stream(self.device))
+            return {"leaf_value": self._leaf_out[:M].cpu().tolist(), "samples": int(n)}
         K = self.num_classes if class_k is not None else 1
         _lib.call("wgbm_leaf_value", self.grad.data_ptr(), self.hess.data_ptr(), node_indices.data_ptr(), n,
                   self.num_samples, float(self.L2_reg), float(self.learning_rate), self.gradients.data_ptr(), K,
@@ -906,6 +983,45 @@ class WarpGBM(BaseEstimator, RegressorMixin):
         self._is_fitted = True
         print(f"Finished training multiclass forest. Total rounds: {self._trees_trained} "
               f"({self._trees_trained * self.num_classes} trees)")
+
+    def grow_forest_multi_regression(self):
+        """One tree per round with splits shared by all outputs and a vector
+        of leaf values (warpgbm/core.py:1619-1717): the split search runs on
+        the per-row sums of the outputs' gradients and hessians, the leaves
+        on each output's own."""
+        if not self.warm_start or not self._is_fitted or self.per_era_feature_importance_ is None:
+            self.training_loss = []
+            self.eval_loss = []
+            self.per_era_feature_importance_ = np.zeros((self.num_eras, self.num_features), dtype=np.float32)
+        self.stop = False
+        k = self._n_sampled()
+        start = self._trees_trained if self.warm_start and self._is_fitted else 0
+        if len(self.forest) < self.n_estimators:
+            self.forest.extend([{} for _ in range(self.n_estimators - len(self.forest))])
+        self._round_tables = None
+        self._plain_setup(k)
+        N, M = self.num_samples, self.num_outputs
+        self._leaf_out = torch.empty(M, dtype=torch.float32, device=self.device)
+        self.grad_out = torch.empty((M, N), dtype=torch.float32, device=self.device)
+        self.hess_out = torch.empty_like(self.grad_out)
+        self.grad = torch.empty(N, dtype=torch.float32, device=self.device)
+        self.hess = torch.empty_like(self.grad)
+        for i in range(start, self.n_estimators):
+            _lib.call("wgbm_grad_multi_regression", self.Y_gpu.data_ptr(), self.gradients.data_ptr(),
+                      self.sample_weight_gpu.data_ptr(), N, M, self.grad_out.data_ptr(), self.hess_out.data_ptr(),
+                      self.grad.data_ptr(), self.hess.data_ptr(), _stream(self.device))
+            self._tree_features(k)
+            G, H = self.compute_histograms(self.root_node_indices, self.feat_indices_tree, self.grad, self.hess)
+            self.forest[i] = self.grow_tree(G, H, self.root_node_indices, 0)
+            self._trees_trained = i + 1
+            self._invalidate_prediction_cache()
+            self.compute_eval_multi_regression(i)
+            if self.stop:
+                break
+        del self.grad_out, self.hess_out
+        self.feature_importance_ = self.per_era_feature_importance_.sum(axis=0)
+        self._is_fitted = True
+        print(f"Finished training multi-output forest. Total trees: {self._trees_trained}")
 
     # ------------------------------------------------------------ batched engine
     def _fit_config(self, k, n_classes):
@@ -1147,7 +1263,69 @@ class WarpGBM(BaseEstimator, RegressorMixin):
             print(f"🌲 Round {i+1}/{self.n_estimators} ({self.num_classes} trees) | "
                   f"Train LogLoss: {train_loss:.6f} | Eval {self.eval_metric}: {loss:.6f}")
 
+    def compute_eval_multi_regression(self, i):
+        """warpgbm/core.py:1582-1617."""
+        if self.eval_every_n_trees is None:
+            return
+        train_loss = weighted_multi_mse_torch(self.Y_gpu, self.gradients, sample_weight=self.sample_weight_gpu).item()
+        self.training_loss.append(train_loss)
+        if i % self.eval_every_n_trees == 0:
+            preds = self.predict_binned(self.bin_indices_eval)
+            loss = weighted_multi_mse_torch(self.Y_gpu_eval, preds, sample_weight=self.sample_weight_eval_gpu).item()
+            self.eval_loss.append(loss)
+            if len(self.eval_loss) > self.early_stopping_rounds:
+                if self.eval_loss[-(self.early_stopping_rounds + 1)] < self.eval_loss[-1]:
+                    self.stop = True
+            print(f"🌲 Tree {i+1}/{self.n_estimators} | Train multi_regression MSE: {train_loss:.6f} | "
+                  f"Eval MSE: {loss:.6f}")
+
     # ------------------------------------------------------------ prediction
+    def _base_vector(self):
+        """Per-channel starting scores of a prediction."""
+        if self.objective == "regression":
+            return [float(self.base_prediction)]
+        if self.objective == "multi_regression":
+            return torch.as_tensor(self.base_prediction, dtype=torch.float32).reshape(-1).cpu().tolist()
+        K = self.num_classes
+        return [float(np.log(np.float32(1.0 / K) + 1e-10))] * K  # the predict-time prior (core.py:2034-2037)
+
+    def _multi_scores(self, bin_indices):
+        """[n, outputs] = base + every tree's lr * leaf vector, tree by tree
+        (core.py:1904-1935)."""
+        hf = self._heap_forest()
+        if hf is not None:
+            return hf.predict_tree_order(bin_indices)
+        n, M = bin_indices.size(0), self.num_outputs
+        out = torch.tensor(self._base_vector(), dtype=torch.float32, device=self.device).reshape(1, M).repeat(n, 1)
+        bic = bin_indices.contiguous()
+        for m, tt in enumerate(self._get_cached_multi_regression_tree_tensors()):
+            if tt.size(0) == 0:
+                continue
+            col = out[:, m].contiguous()
+            _nk.predict_forest(bic, tt.contiguous(), self.learning_rate, col)
+            out[:, m] = col
+        return out
+
+    def _get_cached_multi_regression_tree_tensors(self):
+        """One flattened forest per output (core.py:688-731)."""
+        max_nodes = 2 ** (self.max_depth + 1)
+        if (self._cached_multi_regression_tree_tensors is not None
+                and self._prediction_cache_matches("multi_regression", max_nodes, num_outputs=self.num_outputs)):
+            return self._cached_multi_regression_tree_tensors
+        trees = [t for t in self.forest[: self._trees_trained] if t]
+        out = []
+        for m in range(self.num_outputs):
+            if trees:
+                out.append(torch.stack([self.flatten_tree(t, max_nodes, output_index=m) for t in trees]).to(
+                    self.device))
+            else:
+                out.append(torch.empty((0, max_nodes, 6), dtype=torch.float32, device=self.device))
+        self._cached_multi_regression_tree_tensors = out
+        self._cached_regression_tree_tensor = None
+        self._cached_multiclass_tree_tensors = None
+        self._set_prediction_cache_meta("multi_regression", max_nodes, num_outputs=self.num_outputs)
+        return out
+
     def _heap_forest(self):
         """The trained forest compiled for the device (cached per tree count):
         from the batched engine's node tables when they cover every trained
@@ -1161,10 +1339,7 @@ class WarpGBM(BaseEstimator, RegressorMixin):
         if hc is not None and hc[0] == key:
             return hc[1]
         K = self._n_channels()
-        if self.objective == "regression":
-            base = [float(self.base_prediction)]
-        else:
-            base = [float(np.log(np.float32(1.0 / K) + 1e-10))] * K  # the predict-time prior (core.py:2034-2037)
+        base = self._base_vector()
         tabs = getattr(self, "_round_tables", None)
         if tabs and sum(t["status"].shape[0] for t in tabs) == self._trees_trained:
             NH = tabs[0]["status"].shape[-1]
@@ -1179,7 +1354,10 @@ class WarpGBM(BaseEstimator, RegressorMixin):
     def predict_binned(self, bin_indices):
         """base + sum_t lr*leaf in tree order (core.py:1902-1962): the
         compiled heap forest, so no per-tree host flattening; trees deeper
-        than the heap forest's 16 levels use the flattened-forest op."""
+        than the heap forest's 16 levels use the flattened-forest op.
+        Multi-output models return [n, outputs]."""
+        if self.objective == "multi_regression":
+            return self._multi_scores(bin_indices)
         hf = self._heap_forest()
         if hf is not None:
             return hf.predict_tree_order(bin_indices)[:, 0]
@@ -1197,7 +1375,7 @@ class WarpGBM(BaseEstimator, RegressorMixin):
             return self.label_encoder.inverse_transform(np.argmax(probs, axis=1))
         bi = self.bin_inference_data(X_np)
         preds = self.predict_binned(bi).cpu().numpy()
-        return preds.reshape(-1)
+        return preds if self.objective == "multi_regression" else preds.reshape(-1)
 
     def predict_proba(self, X_np):
         """Class probabilities (core.py:2006-2019)."""
@@ -1266,7 +1444,9 @@ class WarpGBM(BaseEstimator, RegressorMixin):
             raise ValueError("Cannot save unfitted model. Call fit() first.")
         state = {k: getattr(self, k) for k in self._STATE_KEYS}
         state.update({
-            "forest": self.forest, "bin_edges": self.bin_edges, "base_prediction": self.base_prediction,
+            "forest": self.forest, "bin_edges": self.bin_edges,
+            "base_prediction": (self.base_prediction.detach().cpu() if torch.is_tensor(self.base_prediction)
+                                else self.base_prediction),
             "num_features": self.num_features, "num_classes": self.num_classes, "num_outputs": self.num_outputs,
             "classes_": self.classes_, "label_encoder": self.label_encoder,
             "feature_importance_": self.feature_importance_,
@@ -1291,7 +1471,10 @@ class WarpGBM(BaseEstimator, RegressorMixin):
         self.forest = state["forest"]
         self.bin_edges = state["bin_edges"]
         self.base_prediction = state["base_prediction"]
-        if torch.is_tensor(self.base_prediction):
+        if self.objective == "multi_regression":  # a per-output vector on the device (core.py:2259-2270)
+            self.base_prediction = torch.as_tensor(self.base_prediction, dtype=torch.float32).reshape(-1).to(
+                self.device)
+        elif torch.is_tensor(self.base_prediction):
             self.base_prediction = float(self.base_prediction.item())
         self.num_features = state["num_features"]
         self.num_classes = state["num_classes"]

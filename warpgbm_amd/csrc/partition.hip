@@ -860,6 +860,33 @@ k_grad_regression(const float* __restrict__ y, const float* __restrict__ pred, c
   if (gh) gh[p] = make_float2(gr, wr);
 }
 
+// Multi-output squared error with shared splits, one boosting round
+// (warpgbm/core.py:1666-1679): per output m, valid = isfinite(y[r, m]),
+// g_m = valid ? (y - pred) * w : 0, h_m = valid ? w : 0, written
+// output-major (m*n + r); the split targets are the float32 sums over m in
+// output order, split_g[r] = sum_m g_m, split_h[r] = sum_m h_m.
+__global__ void __launch_bounds__(256)
+k_grad_multi_regression(const float* __restrict__ y, const float* __restrict__ pred, const float* __restrict__ w,
+                        int64_t n, int M, float* __restrict__ g, float* __restrict__ h,
+                        float* __restrict__ split_g, float* __restrict__ split_h) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  float sg = 0.0f, sh = 0.0f;
+  for (int m = 0; m < M; ++m) {
+    const int64_t i = r * M + m;
+    const float yv = y[i];
+    const bool valid = isfinite(yv);
+    const float wm = valid ? w[i] : 0.0f;
+    const float gm = valid ? (yv - pred[i]) * wm : 0.0f;
+    g[(int64_t)m * n + r] = gm;
+    h[(int64_t)m * n + r] = wm;
+    sg += gm;
+    sh += wm;
+  }
+  split_g[r] = sg;
+  split_h[r] = sh;
+}
+
 // Softmax cross-entropy, one boosting round (warpgbm/core.py:339-367,
 // :1755-1756; softmax of warpgbm/metrics.py:12-15): p = softmax(F_row),
 // g_k = -(p_k - [y == k]) * w, h_k = max(p_k (1 - p_k), 1e-6) * w.
@@ -1004,6 +1031,18 @@ extern "C" int wgbm_grad_regression(const float* y, const float* pred, const flo
   hipLaunchKernelGGL(k_grad_regression, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, stream, y, pred, w, n,
                      (const int32_t*)nullptr, g_out, h_out, (float2*)nullptr);
   WGBM_LAUNCHED("k_grad_regression");
+  return kOk;
+}
+
+extern "C" int wgbm_grad_multi_regression(const float* y, const float* pred, const float* w, int64_t n,
+                                          int32_t n_outputs, float* g_out, float* h_out, float* split_g,
+                                          float* split_h, hipStream_t stream) {
+  WGBM_REQUIRE(n >= 0 && n_outputs >= 1, "bad multi-output shape");
+  WGBM_REQUIRE(n * (int64_t)n_outputs < ((int64_t)1 << 62), "multi-output size out of range");
+  if (n == 0) return kOk;
+  hipLaunchKernelGGL(k_grad_multi_regression, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, stream, y, pred, w,
+                     n, (int)n_outputs, g_out, h_out, split_g, split_h);
+  WGBM_LAUNCHED("k_grad_multi_regression");
   return kOk;
 }
 

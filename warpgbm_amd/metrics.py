@@ -54,3 +54,22 @@ def weighted_corr_loss_torch(y_true, y_pred, sample_weight=None, eps=1e-12):
     vt = torch.sum(w * ct * ct) / ws
     vp = torch.sum(w * cp * cp) / ws
     return 1.0 - cov / torch.sqrt(torch.clamp(vt * vp, min=eps))
+
+
+def weighted_multi_mse_torch(y_true, y_pred, sample_weight=None, valid_mask=None):
+    """Weighted MSE over [n, outputs] targets (warpgbm/metrics.py:52-83):
+    entries outside valid_mask (default: the finite targets) count with
+    weight 0; 1-D weights apply to every output of a row."""
+    if y_true.ndim != 2 or y_true.shape != y_pred.shape:
+        raise ValueError(f"expected matching [n, outputs] tensors, got {tuple(y_true.shape)} and "
+                         f"{tuple(y_pred.shape)}")
+    valid = torch.isfinite(y_true) if valid_mask is None else valid_mask.to(torch.bool)
+    w = _weights(sample_weight, y_true)
+    if w.ndim == 1:
+        w = w.view(-1, 1)
+    err2 = torch.where(valid, (y_true - y_pred) ** 2, torch.zeros_like(y_pred))
+    ew = w * valid.to(torch.float32)
+    den = torch.sum(ew)
+    if den <= 0:
+        raise ValueError("No valid weighted targets available for multi-output MSE.")
+    return torch.sum(ew * err2) / den

@@ -68,30 +68,36 @@ class HeapForest:
 
     @classmethod
     def from_model(cls, model, tree_block=64):
-        """Compile a fitted WarpGBM's forest (nested dicts) for serving."""
+        """Compile a fitted WarpGBM's forest (nested dicts) for serving.
+        A multi-output tree (one vector of leaf values per leaf) becomes one
+        tree per output, sharing its splits: output m of tree i is tree
+        i * outputs + m, so output m is that forest's channel m."""
         D = model.max_depth
         slots = 2 ** (D + 1) - 1
-        if model.objective == "regression":
+        base = model._base_vector()
+        K = len(base)
+        if model.objective in ("regression", "multi_regression"):
             trees = [t for t in model.forest[: model._trees_trained] if t]
-            K = 1
-            base = [float(model.base_prediction)]
         else:
-            K = model.num_classes
             trees = [t for rnd in model.forest[: model._trees_trained] for t in rnd]
-            base = [float(np.log(np.float32(1.0 / K) + 1e-10))] * K
-        T = len(trees)
+        T = len(trees) * (K if model.objective == "multi_regression" else 1)
+        per = T // max(len(trees), 1)  # compiled trees per stored tree
         st = np.zeros((T, slots), np.int32)
         ft = np.zeros((T, slots), np.int32)
         bt = np.zeros((T, slots), np.int32)
         vt = np.zeros((T, slots), np.float32)
         for i, tree in enumerate(trees):
+            r = i * per
             stack = [(tree, 0)]
             while stack:
                 node, h = stack.pop()
                 if "leaf_value" in node:
-                    st[i, h], vt[i, h] = 2, float(node["leaf_value"])
+                    st[r:r + per, h] = 2
+                    vt[r:r + per, h] = np.asarray(node["leaf_value"], np.float32).reshape(-1)[:per]
                 else:
-                    st[i, h], ft[i, h], bt[i, h] = 1, model._feature_of(node["feature"]), int(node["bin"])
+                    st[r:r + per, h] = 1
+                    ft[r:r + per, h] = model._feature_of(node["feature"])
+                    bt[r:r + per, h] = int(node["bin"])
                     stack.append((node["left"], 2 * h + 1))
                     stack.append((node["right"], 2 * h + 2))
         return cls.from_tables(st, ft, bt, vt, D, model.learning_rate, model.num_features, base, K, tree_block,
