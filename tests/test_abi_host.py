@@ -264,3 +264,26 @@ def test_server_refuses_requests_after_close():
     with pytest.raises(RuntimeError, match="closed"):
         srv.submit(np.zeros((3, 4), np.int8))
     assert not srv._queue
+
+
+def test_reference_import_names_through_the_package_alias():
+    """A caller written against the reference (``from warpgbm import
+    WarpGBM``, ``from warpgbm.cuda import node_kernel``) runs unchanged once
+    the package is aliased as ``warpgbm``; run in a child interpreter so the
+    alias does not leak into other tests."""
+    import subprocess
+    import sys
+    code = (
+        "import sys, warpgbm_amd; sys.modules['warpgbm'] = warpgbm_amd\n"
+        "from warpgbm import WarpGBM\n"
+        "from warpgbm.cuda import node_kernel\n"
+        "from warpgbm.metrics import weighted_mse_torch, weighted_multi_mse_torch\n"
+        "assert WarpGBM is warpgbm_amd.WarpGBM\n"
+        "for name in ('compute_histogram3', 'compute_split', 'custom_cuda_binner', 'predict_forest'):\n"
+        "    assert callable(getattr(node_kernel, name)), name\n"
+        "m = WarpGBM(objective='multi_regression', n_estimators=3, device='cpu')\n"
+        "assert m.objective == 'multi_regression'\n"
+        "print('ok')\n")
+    root = Path(__file__).resolve().parents[1]
+    r = subprocess.run([sys.executable, "-c", code], cwd=root, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0 and r.stdout.strip().endswith("ok"), r.stderr[-2000:]

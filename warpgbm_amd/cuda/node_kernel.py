@@ -31,8 +31,20 @@ def _check(t: torch.Tensor, dtype: torch.dtype, name: str, ndim: int | None = No
         raise RuntimeError(f"{name} must be {ndim}-dimensional, got shape {tuple(t.shape)}")
 
 
-def _workspace(device, nbytes: int) -> torch.Tensor:
-    return torch.empty(max(int(nbytes), 1), dtype=torch.uint8, device=device)
+# Workspaces kept between calls, one per (device, stream): a call reuses the
+# buffer of the previous call on its stream (which the stream has finished
+# with by the time this call's kernels run) and only grows it.  The largest
+# workspace of each stream is held until the process ends.
+_WORKSPACES: dict = {}
+
+
+def _workspace(device, nbytes: int, stream: int) -> torch.Tensor:
+    key = (str(device), stream)
+    ws = _WORKSPACES.get(key)
+    if ws is None or ws.numel() < nbytes:
+        ws = torch.empty(max(int(nbytes), 256), dtype=torch.uint8, device=device)
+        _WORKSPACES[key] = ws
+    return ws
 
 
 def compute_histogram3(bin_indices, grads, hess, sample_indices, feature_indices, era_indices,
@@ -63,12 +75,13 @@ def compute_histogram3(bin_indices, grads, hess, sample_indices, feature_indices
     if grads.numel() < n_rows or hess.numel() < n_rows or era_indices.numel() < n_rows:
         raise RuntimeError("grads, hess and era_indices need one value per row of bin_indices")
     n = sample_indices.numel()
+    st = _stream(bin_indices)
     ws = _workspace(bin_indices.device,
-                    _lib.query("wgbm_histogram3_workspace_bytes", n, n_feat, int(num_bins), n_eras, n_cols))
+                    _lib.query("wgbm_histogram3_workspace_bytes", n, n_feat, int(num_bins), n_eras, n_cols), st)
     _lib.call("wgbm_compute_histogram3", bin_indices.data_ptr(), n_rows, n_cols, grads.data_ptr(),
               hess.data_ptr(), sample_indices.data_ptr(), n, feature_indices.data_ptr(), n_feat,
               era_indices.data_ptr(), grad_hist.data_ptr(), hess_hist.data_ptr(), n_eras, int(num_bins),
-              ws.data_ptr(), ws.numel(), _stream(bin_indices))
+              ws.data_ptr(), ws.numel(), st)
 
 
 def compute_split(G, H, min_split_gain, min_child_samples, eps, per_era_gain, per_era_direction, threads=128):
