@@ -7,6 +7,7 @@
 // matrix against edges [F, B-1] in one launch, reading 16 bytes of X and
 // writing 4 codes per thread (no per-feature launches, no transposes).
 #include "common.h"
+#include <algorithm>
 
 namespace wgbm {
 
@@ -50,27 +51,39 @@ k_bin_matrix4(const float4* __restrict__ X, const float* __restrict__ et, uint32
 }
 
 // Tiled whole-matrix binning (F % 4 == 0, 16-byte aligned rows): a
-// workgroup takes 64 features x kBinRows rows, with the tile's edges in LDS,
-// one row of kEdgeStride floats per feature (component-major: the four
-// features a lane reads, 4c .. 4c+3, sit in rows c, 16 + c, 32 + c, 48 + c,
-// and kEdgeStride = 129 puts row r's word k in bank (r + k) mod 32, so the
-// 16 columns of a wave read 16 different banks when their probes agree).
-// A feature whose edges are non-decreasing and NaN-free takes a branch-free
-// binary search for the number of edges <= x, its edge row padded with NaN
-// to 2^m - 1 entries (x >= NaN is false, so no probe passes the last edge;
-// NaN x gives 0); the probes of all 16 searches of a thread (4 rows x 4
-// features) are interleaved, so their LDS latencies overlap.  This equals
-// the reference's early-exit scan for such edges; any other feature takes
-// that scan as a running AND.
-constexpr int kBinRows = 1024;
-constexpr int kEdgeStride = 129;  // >= 127 edges, odd
+// workgroup takes kBinFeat = 256 features x kBinRows rows with the tile's
+// edges in LDS.  Lane l of every wave owns features 4l .. 4l+3 of the tile
+// and reads them as one float4, so a wave reads 1 KB of one row per load and
+// writes 256 contiguous code bytes; wave w takes rows w, w + 4, ...  Edge k
+// of feature 4l + q is LDS word (q * kp + k) * 64 + l: its bank is fixed by
+// the lane whatever edge it probes, so the 64 lanes hit each bank exactly
+// twice.  A feature whose edges are non-decreasing and NaN-free takes a
+// branch-free binary search for the number of edges <= x, its edges padded
+// with NaN to kp = 2^m - 1 entries (x >= NaN is false, so no probe passes the
+// last edge; NaN x gives 0); the probes of a thread's 16 searches (4 rows x 4
+// features) are issued together, then compared.  This equals the
+// reference's early-exit scan for such edges; any other feature takes that
+// scan as a running AND.
+constexpr int kBinRows = 256;
+constexpr int kBinFeat = 256;
+constexpr int kBinWGs = 8192;  // most workgroups of a launch (each loops over row blocks)
 
-__device__ __forceinline__ int edge_row(int f) { return ((f & 3) << 4) + (f >> 2); }
+__host__ __device__ __forceinline__ int bin_pad(int nb) {
+  int top = 1;
+  while (2 * top <= nb) top *= 2;
+  return 2 * top - 1;
+}
 
+// The edges, then one flag per feature.
+__host__ __forceinline__ size_t bin_tile_lds(int nb) {
+  return ((size_t)4 * bin_pad(nb) * 64 + kBinFeat) * sizeof(float);
+}
+
+// Edges of one feature: edge k at ef[k * 64].
 __device__ __forceinline__ uint32_t code_scan(const float* __restrict__ ef, int nb, float x) {
   uint32_t a = 1, c = 0;
   for (int k = 0; k < nb; ++k) {
-    a &= (uint32_t)(x >= ef[k]);
+    a &= (uint32_t)(x >= ef[k * 64]);
     c += a;
   }
   return c;
@@ -78,88 +91,122 @@ __device__ __forceinline__ uint32_t code_scan(const float* __restrict__ ef, int 
 
 __device__ __forceinline__ uint32_t code_search(const float* __restrict__ ef, int top, float x) {
   int lo = 0;
-  for (int step = top; step > 0; step >>= 1) lo = (x >= ef[lo + step - 1]) ? lo + step : lo;
+  for (int step = top; step > 0; step >>= 1) lo = (x >= ef[(lo + step - 1) * 64]) ? lo + step : lo;
   return (uint32_t)lo;
 }
 
 __global__ void __launch_bounds__(256)
 k_bin_tile(const float4* __restrict__ X, const float* __restrict__ et, uint32_t* __restrict__ out, int64_t n,
            int F, int nb) {
-  __shared__ float s_e[64 * kEdgeStride];
-  __shared__ int s_sorted[64];
-  const int f0 = blockIdx.y * 64;
-  const int nf = min(64, F - f0);  // a multiple of 4
-  int top = 1;
-  while (2 * top <= nb) top *= 2;
-  const int padded = 2 * top - 1;  // >= nb; probes stay below it
-  for (int q = threadIdx.x; q < padded * 64; q += 256) {
-    const int k = q >> 6, f = q & 63;
-    s_e[edge_row(f) * kEdgeStride + k] = (f < nf && k < nb) ? et[(int64_t)k * F + f0 + f] : __builtin_nanf("");
+  extern __shared__ float s_e[];  // no static LDS: s_e is at address 0
+  // Workgroup b takes feature group b % n_groups of row blocks b / n_groups,
+  // b / n_groups + gridDim.x / n_groups, ... (gridDim.x is a multiple of
+  // n_groups): the groups of a row block run side by side, so each row's
+  // bytes are read (and its codes written) by neighbouring workgroups at
+  // about the same time, and each workgroup loads its edges once.
+  const int n_groups = (F + kBinFeat - 1) / kBinFeat;
+  const int f0 = (int)(blockIdx.x % n_groups) * kBinFeat;
+  const int64_t n_rblk = (n + kBinRows - 1) / kBinRows;
+  const int64_t rstride = gridDim.x / n_groups;
+  const int nf = min(kBinFeat, F - f0);  // a multiple of 4
+  const int kp = bin_pad(nb);            // >= nb; probes stay below it
+  const int top = (kp + 1) >> 1;
+  int* s_sorted = reinterpret_cast<int*>(s_e + 4 * kp * 64);
+  for (int w = threadIdx.x; w < 4 * kp * 64; w += 256) {
+    const int l = w & 63, qk = w >> 6;  // word w: lane l, entry qk = q * kp + k
+    const int q = qk / kp, k = qk - q * kp;
+    const int f = 4 * l + q;
+    s_e[w] = (f < nf && k < nb) ? et[(int64_t)k * F + f0 + f] : __builtin_nanf("");
   }
   __syncthreads();
-  if (threadIdx.x < 64) {  // is the feature's edge list non-decreasing and NaN-free?
-    const float* ef = s_e + edge_row(threadIdx.x) * kEdgeStride;
+  {  // is feature t's edge list non-decreasing and NaN-free?
+    const int l = threadIdx.x >> 2, q = threadIdx.x & 3;
+    const float* ef = s_e + q * kp * 64 + l;
     bool ok = ef[0] == ef[0];
-    for (int k = 1; k < nb; ++k) ok = ok && (ef[k] >= ef[k - 1]);  // false for NaN
+    for (int k = 1; k < nb; ++k) ok = ok && (ef[k * 64] >= ef[(k - 1) * 64]);  // false for NaN
     s_sorted[threadIdx.x] = ok;
   }
   __syncthreads();
-  const int c = threadIdx.x & 15;   // float4 column within the tile
-  const int r = threadIdx.x >> 4;   // row within a pass of 16
-  if (4 * c >= nf) return;
-  const float* e0 = s_e + c * kEdgeStride;  // feature 4c; 4c + j is 16 j rows further
-  constexpr int kPlane = 16 * kEdgeStride;
-  const bool all_sorted = s_sorted[4 * c] && s_sorted[4 * c + 1] && s_sorted[4 * c + 2] && s_sorted[4 * c + 3];
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  if (4 * lane >= nf) return;
+  // LDS bytes: edge k of feature 4 lane + q at qb[q] + 256 k, so a probe is
+  // one ds_read with the step in its offset.
+  const char* sb = reinterpret_cast<const char*>(s_e);
+  const int qb[4] = {lane * 4, lane * 4 + kp * 256, lane * 4 + 2 * kp * 256, lane * 4 + 3 * kp * 256};
+  const bool all_sorted = s_sorted[4 * lane] && s_sorted[4 * lane + 1] && s_sorted[4 * lane + 2] &&
+                          s_sorted[4 * lane + 3];
   const int64_t F4 = F >> 2;
-  const int64_t row0 = (int64_t)blockIdx.x * kBinRows;
+  for (int64_t rblk = blockIdx.x / n_groups; rblk < n_rblk; rblk += rstride) {
+  // Rows: wave wv takes rows row0 + 4 i, i = 0 .. kBinRows/4 - 1; a row's
+  // address is a wave-uniform base plus the lane's float4 (or code word).
+  const int64_t row0 = rblk * kBinRows + wv;
+  const bool full = rblk * kBinRows + kBinRows <= n;  // uniform: every row of the block exists
   constexpr int RB = 4;  // rows per thread whose loads are in flight together
-  for (int rr = r; rr < kBinRows; rr += 16 * RB) {
-    float4 v[RB];
-    int64_t idx[RB];
+  constexpr int NI = kBinRows / 4 / RB;
+  static_assert(kBinRows % (4 * RB) == 0, "row block");
+  auto row_ok = [&](int i) { return full || row0 + 4 * i < n; };
+  auto load_rows = [&](int i0, float4* v) {
 #pragma unroll
     for (int j = 0; j < RB; ++j) {
-      const int64_t row = row0 + rr + 16 * j;
-      idx[j] = row < n ? row * F4 + (f0 >> 2) + c : -1;
-      v[j] = idx[j] >= 0 ? X[idx[j]] : make_float4(0.f, 0.f, 0.f, 0.f);
+      const float4* xb = X + (row0 + 4 * (i0 + j)) * F4 + (f0 >> 2);
+      v[j] = row_ok(i0 + j) ? xb[lane] : make_float4(0.f, 0.f, 0.f, 0.f);
     }
+  };
+  // Software pipeline: the next RB rows' loads are issued before this RB's
+  // searches, so they are in flight meanwhile.
+  float4 vn[RB];
+  load_rows(0, vn);
+  for (int it = 0; it < NI; ++it) {
+    float4 v[RB];
+#pragma unroll
+    for (int j = 0; j < RB; ++j) v[j] = vn[j];
+    if (it + 1 < NI) load_rows((it + 1) * RB, vn);  // uniform
     uint32_t code[RB][4];
     if (all_sorted) {
-      int lo[RB][4];
+      const char* lo[RB][4];  // the current lower bound's edge (256 bytes per edge passed)
 #pragma unroll
       for (int j = 0; j < RB; ++j)
 #pragma unroll
-        for (int q = 0; q < 4; ++q) lo[j][q] = 0;
+        for (int q = 0; q < 4; ++q) lo[j][q] = sb + qb[q];
 #pragma unroll
-      for (int s = 6; s >= 0; --s) {  // steps 64 .. 1, those up to top
-        const int step = 1 << s;
+      for (int sbit = 6; sbit >= 0; --sbit) {  // steps 64 .. 1, those up to top
+        const int step = 1 << sbit;
         if (step > top) continue;  // uniform
+        float e[RB][4];  // all probes of the step issued before any compare
+#pragma unroll
+        for (int j = 0; j < RB; ++j)
+#pragma unroll
+          for (int q = 0; q < 4; ++q) e[j][q] = *reinterpret_cast<const float*>(lo[j][q] + (step - 1) * 256);
 #pragma unroll
         for (int j = 0; j < RB; ++j) {
           const float xs[4] = {v[j].x, v[j].y, v[j].z, v[j].w};
 #pragma unroll
-          for (int q = 0; q < 4; ++q) {
-            const float e = e0[q * kPlane + lo[j][q] + step - 1];
-            lo[j][q] = (xs[q] >= e) ? lo[j][q] + step : lo[j][q];
-          }
+          for (int q = 0; q < 4; ++q) lo[j][q] += (xs[q] >= e[j][q]) ? step * 256 : 0;
         }
       }
 #pragma unroll
       for (int j = 0; j < RB; ++j)
 #pragma unroll
-        for (int q = 0; q < 4; ++q) code[j][q] = (uint32_t)lo[j][q];
+        for (int q = 0; q < 4; ++q) code[j][q] = (uint32_t)(lo[j][q] - (sb + qb[q])) >> 8;
     } else {
 #pragma unroll
       for (int j = 0; j < RB; ++j) {
         const float xs[4] = {v[j].x, v[j].y, v[j].z, v[j].w};
 #pragma unroll
-        for (int q = 0; q < 4; ++q)
-          code[j][q] = s_sorted[4 * c + q] ? code_search(e0 + q * kPlane, top, xs[q])
-                                           : code_scan(e0 + q * kPlane, nb, xs[q]);
+        for (int q = 0; q < 4; ++q) {
+          const float* ef = s_e + q * kp * 64 + lane;
+          code[j][q] = s_sorted[4 * lane + q] ? code_search(ef, top, xs[q]) : code_scan(ef, nb, xs[q]);
+        }
       }
     }
 #pragma unroll
-    for (int j = 0; j < RB; ++j)
-      if (idx[j] >= 0) out[idx[j]] = code[j][0] | (code[j][1] << 8) | (code[j][2] << 16) | (code[j][3] << 24);
+    for (int j = 0; j < RB; ++j) {
+      const int i = it * RB + j;
+      uint32_t* ob = out + (row0 + 4 * i) * F4 + (f0 >> 2);
+      if (row_ok(i)) ob[lane] = code[j][0] | (code[j][1] << 8) | (code[j][2] << 16) | (code[j][3] << 24);
+    }
+  }
   }
 }
 
@@ -253,9 +300,22 @@ extern "C" int wgbm_bin_matrix(const float* X, int64_t n, int32_t F, const float
     WGBM_LAUNCHED("k_transpose_edges");
   }
   const bool vec = (F % 4 == 0) && ((uintptr_t)X % 16 == 0) && ((uintptr_t)out % 4 == 0);
-  if (vec && n_edges > 0 && ceil_div(n, kBinRows) < 0x7fffffff) {
-    const dim3 grid((unsigned)ceil_div(n, kBinRows), (unsigned)ceil_div(F, 64));
-    hipLaunchKernelGGL(k_bin_tile, grid, dim3(256), 0, stream, reinterpret_cast<const float4*>(X), et,
+  if (vec && n_edges > 0) {
+    const size_t lds = bin_tile_lds(n_edges);
+    if (lds > 64 * 1024) {  // opt in once per device (up to 127 KB at 127 edges)
+      static bool done[64] = {};
+      int dev = 0;
+      WGBM_HIP(hipGetDevice(&dev));
+      if (dev < 0 || dev >= 64 || !done[dev]) {
+        WGBM_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_bin_tile),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)bin_tile_lds(127)));
+        if (dev >= 0 && dev < 64) done[dev] = true;
+      }
+    }
+    const int64_t ng = ceil_div(F, kBinFeat);
+    const int64_t per = std::max<int64_t>(1, std::min<int64_t>(ceil_div(n, kBinRows), kBinWGs / ng));
+    const dim3 grid((unsigned)(per * ng));
+    hipLaunchKernelGGL(k_bin_tile, grid, dim3(256), lds, stream, reinterpret_cast<const float4*>(X), et,
                        reinterpret_cast<uint32_t*>(out), n, (int)F, (int)n_edges);
     WGBM_LAUNCHED("k_bin_tile");
   } else if (vec) {
