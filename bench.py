@@ -36,7 +36,6 @@ def parse():
     p.add_argument("--bins", type=int, default=32)
     p.add_argument("--plain-trees", type=int, default=100,
                    help="trees of the plain per-node fit (its time is scaled to --trees if fewer)")
-    p.add_argument("--cpu-rows", type=int, default=200_000, help="rows of the CPU oracle sample")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-plain", action="store_true")
     p.add_argument("--profile-only", action="store_true",
@@ -371,51 +370,64 @@ def trees_identical(fa, fb):
     return all(same(a, b) for a, b in zip(fa, fb))
 
 
+def _cpu_threads():
+    """Host threads for the CPU baselines: the cores this process may use,
+    at most 16 (one GPU's share of the box)."""
+    return max(1, min(16, len(os.sched_getaffinity(0))))
+
+
+def _fast_codes(X, bins):
+    """Quantile codes per column (np.quantile edges, searchsorted) for the
+    CPU baselines: the training cost does not depend on how the codes were
+    binned, only on their shape and spread."""
+    q = np.linspace(0, 1, bins + 1)[1:-1]
+    out = np.empty(X.shape, np.int8)
+    for f in range(X.shape[1]):
+        e = np.quantile(X[:, f], q).astype(np.float32)
+        out[:, f] = np.searchsorted(e, X[:, f], side="right")
+    return out
+
+
 def cpu_baselines_other():
-    """The FP64 NumPy oracle on the other workloads, on the host cores of the
-    box (one core: NumPy without threads), each on a stated sample scaled
-    linearly: the 20k x 50 regression in full (20 trees); the era workload's
-    first tree on its first 100k rows (x 5 rows x 200 trees); the multiclass
-    workload's first round (5 class trees) on 50k rows (x 4 rows x 200
-    rounds); served predict of the 1000-tree depth-8 forest per request-size
-    class (a forest of the same shape grown by the oracle on random targets,
-    timed per row)."""
-    from oracle import gbm_oracle as O
+    """The multithreaded FP64 C restatement of the oracle on the other
+    workloads, on the box's host cores (training only; codes binned on the
+    host): the 20k x 50 regression in full (20 trees); the era workload's
+    first tree on all 500k rows (x 200 trees); the multiclass workload's
+    first two rounds on all 200k rows (x 100); served predict of the 1000-tree
+    depth-8 forest per request-size class by the NumPy oracle on one core (a
+    forest of the same shape grown on random targets, timed per row)."""
+    from oracle import cpu_gbm, gbm_oracle as O
     from warpgbm_amd.synthetic import era_workload, multiclass_workload
-    out = {"cores_available": len(os.sched_getaffinity(0)), "cores_used": 1, "kind": "port"}
+    T = _cpu_threads()
+    out = {"cores_available": len(os.sched_getaffinity(0)), "cores_used": T, "kind": "port",
+           "implementation": "oracle/cpu_gbm.c (OpenMP, float64), pinned against the NumPy oracle"}
     rng = np.random.default_rng(0)
     # 20k x 50 regression, 32 bins, depth 4, 20 trees, lr 0.1: in full
     X = rng.standard_normal((20000, 50)).astype(np.float32)
     y = (X[:, :10] @ rng.standard_normal(10) + rng.normal(0, 0.1, 20000)).astype(np.float32)
+    codes = _fast_codes(X, 32)
     t0 = time.perf_counter()
-    codes, _ = O.bin_matrix(X, 32)
-    O.fit_regression(codes, y, np.ones(20000), np.zeros(20000, np.int32), 1, 32, 4, 20, 0.1)
-    out["regression_20kx50_d4_20trees"] = {"value_s": time.perf_counter() - t0, "sample": "full fit incl. binning"}
-    # era workload: first tree, first 100k rows
+    cpu_gbm.fit_regression(codes, y, np.ones(20000), None, 1, 32, 4, 20, 0.1, threads=T)
+    out["regression_20kx50_d4_20trees"] = {"value_s": time.perf_counter() - t0, "sample": "full fit (20 trees)"}
+    # era workload: first tree on all rows
     Xe, ye, era = era_workload()
-    n = 100_000
-    g = (ye[:n] - np.float32(ye.mean())).astype(np.float64)
+    g = (ye - np.float32(ye.mean())).astype(np.float64)
     t0 = time.perf_counter()
-    O.grow_tree(Xe[:n], g, np.ones(n), era[:n], int(era.max()) + 1, 5, 8, 20, 0.0, 1e-6)
+    cpu_gbm.grow_tree(Xe, g, np.ones(len(g)), era, int(era.max()) + 1, 5, 8, 20, 0.0, 1e-6, threads=T)
     dt = time.perf_counter() - t0
-    out["eras_500kx400_E200_d8_200trees"] = {"value_s": dt * 5 * 200, "sample_s": dt,
-                                             "sample": "1 tree on the first 100k rows, x 5 (rows) x 200 (trees)"}
+    out["eras_500kx400_E200_d8_200trees"] = {"value_s": dt * 200, "sample_s": dt,
+                                             "sample": "tree 1 on all 500k rows, x 200 (trees)"}
     del Xe
-    # multiclass: first round on 50k rows
+    # multiclass: first two rounds on all rows
     Xm, lab = multiclass_workload()
-    n = 50_000
+    cm = _fast_codes(Xm, 32)
     t0 = time.perf_counter()
-    cm, _ = O.bin_matrix(Xm[:n], 32)
-    counts = np.bincount(lab[:n], minlength=5)
-    sc = np.tile(np.log(counts / n + 1e-10), (n, 1))
-    gk, hk = O.softmax_grad_hess(sc, lab[:n], np.ones(n))
-    for k in range(5):
-        O.grow_tree(cm, gk[:, k], hk[:, k], np.zeros(n, np.int32), 1, 32, 6, 20, 0.0, 1e-6)
+    cpu_gbm.fit_softmax(cm, lab, np.ones(len(lab)), 5, 32, 6, 2, 0.1, threads=T)
     dt = time.perf_counter() - t0
-    out["multiclass_200kx100_K5_d6_200rounds"] = {"value_s": dt * 4 * 200, "sample_s": dt,
-                                                  "sample": "round 1 (5 trees) on 50k rows, x 4 (rows) x 200 (rounds)"}
-    del Xm
-    # served predict: oracle traversal of 1000 depth-8 trees
+    out["multiclass_200kx100_K5_d6_200rounds"] = {"value_s": dt * 100, "sample_s": dt,
+                                                  "sample": "rounds 1-2 (10 trees) on all 200k rows, x 100"}
+    del Xm, cm
+    # served predict: oracle traversal of 1000 depth-8 trees (NumPy, one core)
     F, nr = 400, 20000
     cs = rng.integers(0, 32, (nr, F)).astype(np.int8)
     tree = O.grow_tree(cs, rng.standard_normal(nr), np.ones(nr), np.zeros(nr, np.int32), 1, 32, 8, 1, -np.inf,
@@ -429,29 +441,30 @@ def cpu_baselines_other():
         dt = (time.perf_counter() - t0) * 10  # 100 of the 1000 trees timed
         per_class[f"{rows}_rows_ms"] = dt * 1e3
     per_class["100000_rows_ms"] = per_class["10000_rows_ms"] * 10
-    out["serving_1000x8_per_request"] = dict(per_class, sample=("100 of 1000 trees timed per request size, "
-                                                                "x 10; 100k rows = 10 x the 10k time"))
+    out["serving_1000x8_per_request"] = dict(per_class, cores=1, sample=(
+        "NumPy oracle, 100 of 1000 trees timed per request size, x 10; 100k rows = 10 x the 10k time"))
     return out
 
 
 def cpu_baseline(codes_host, y_host, args):
-    """FP64 oracle, one tree on the first cpu_rows rows, extrapolated."""
-    from oracle import gbm_oracle as O
+    """The multithreaded FP64 C restatement of the oracle: the regression
+    fit on the full code matrix for a few trees, scaled to the fit's trees."""
+    from oracle import cpu_gbm
+    T = _cpu_threads()
     n = codes_host.shape[0]
-    g = (y_host - np.float32(y_host.mean())).astype(np.float64)
+    k = 2
     t0 = time.perf_counter()
-    O.grow_tree(codes_host, g, np.ones(n), np.zeros(n, np.int32), 1, args.bins, args.depth, 20, 0.0, 1e-6)
+    cpu_gbm.fit_regression(codes_host, y_host, np.ones(n), None, 1, args.bins, args.depth, k, 0.01, threads=T)
     dt = time.perf_counter() - t0
-    scale = (args.rows / n) * args.trees
     return {
-        "value": dt * scale,
+        "value": dt / k * args.trees,
         "unit": "s",
-        "cores": 1,
+        "cores": T,
         "cores_available": len(os.sched_getaffinity(0)),
         "kind": "port",
-        "sample": (f"FP64 NumPy oracle grow_tree, 1 tree (depth {args.depth}, {args.bins} bins) on the first "
-                   f"{n} rows x {args.features} features took {dt:.2f} s on 1 host core; value = that x "
-                   f"{args.rows // n} (rows) x {args.trees} (trees), linear extrapolation"),
+        "sample": (f"oracle/cpu_gbm.c (OpenMP, float64) regression fit of {k} trees (depth {args.depth}, "
+                   f"{args.bins} bins) on all {n} rows x {args.features} features took {dt:.2f} s on {T} host "
+                   f"threads; value = that / {k} x {args.trees} (trees)"),
     }
 
 
@@ -610,13 +623,13 @@ def main():
 
     cpu = cpu_other = None
     if not args.no_cpu:
-        n = min(args.cpu_rows, args.rows)
-        ch = codes[:n].cpu().numpy()
-        yh = y[:n].cpu().numpy()
+        ch = codes.cpu().numpy()
+        yh = y.cpu().numpy()
         cpu = cpu_baseline(ch, yh, args)
-        log(f"cpu oracle: {cpu['value']:.0f} s (extrapolated)")
+        del ch
+        log(f"cpu baseline: {cpu['value']:.0f} s (extrapolated)")
         cpu_other = cpu_baselines_other()
-        log("cpu oracle, other workloads: " + ", ".join(
+        log("cpu baselines, other workloads: " + ", ".join(
             f"{k} {v['value_s']:.0f} s" for k, v in cpu_other.items() if isinstance(v, dict) and "value_s" in v))
 
     res = {

@@ -5,6 +5,12 @@ TEST INFRASTRUCTURE ONLY.  Only ``tests/``, ``__graft_entry__.smoke()`` and the
 checker (or as the timed CPU baseline).  The product path (``warpgbm_amd``)
 never imports it and fails loudly when its HIP library is missing.
 
+``cpu_gbm`` (``cpu_gbm.c`` through ctypes, built by ``make -C oracle``) is a
+multithreaded FP64 C restatement of the tree growth and the regression and
+softmax boosting loops, the timed CPU baseline; tests/test_oracle.py pins it
+against ``gbm_oracle`` (same trees, leaves and row -> leaf maps, scores within
+1e-9).
+
 ``gbm_oracle`` is an FP64 NumPy restatement of the reference's kernel and
 host semantics (each function cites the reference file:line it follows).
 Bin edges and bin codes are defined in float32, bit for bit; everything

@@ -476,3 +476,71 @@ def test_fit_multi_regression_output_permutation_and_leaf_brute_force():
             ref = sum(g[i, m] for i in rows) / (sum(W[i, m] for i in rows) + lam)
             assert abs(node[1][m] - ref) < 1e-9 * max(1.0, abs(ref))
     np.testing.assert_allclose(O.predict_multi(codes, [r.tree for r in r1], 0.3, b1), s1, rtol=1e-9, atol=1e-9)
+
+
+# ---------------------------------------------------------------- C restatement
+def _c_tree_matches(ctab, rec_tree, rtol=1e-9):
+    from oracle import cpu_gbm
+    st, ft, bt, vt, ct = ctab
+    d = cpu_gbm.to_dict(st, ft, bt, vt, ct)
+
+    def cmp(a, b, path):
+        assert ("leaf_value" in a) == ("leaf_value" in b), f"{path}: leaf/split differ"
+        if "leaf_value" in a:
+            assert a["samples"] == b["samples"], path
+            assert abs(a["leaf_value"] - b["leaf_value"]) <= rtol * max(1.0, abs(b["leaf_value"])), path
+            return
+        assert (a["feature"], a["bin"]) == (int(b["feature"]), int(b["bin"])), path
+        cmp(a["left"], b["left"], path + "L")
+        cmp(a["right"], b["right"], path + "R")
+
+    cmp(d, rec_tree, "root")
+
+
+@pytest.mark.parametrize("E,mono", [(1, False), (1, True), (5, False), (5, True)])
+def test_c_restatement_grows_the_oracle_tree(E, mono):
+    """The multithreaded C tree growth (sibling histograms by subtraction,
+    thread-private sums) gives the NumPy oracle's tree: same splits, row
+    counts and leaf values, and the same row -> leaf map."""
+    from oracle import cpu_gbm
+    rng = np.random.default_rng(40 + E)
+    N, F, B, D = 6000, 10, 16, 5
+    codes = rng.integers(0, B, (N, F)).astype(np.int8)
+    g = np.sin(codes[:, 0] * 0.7) + 0.5 * (codes[:, 3] > 7) + 0.3 * rng.standard_normal(N)
+    h = rng.uniform(0.5, 1.5, N)
+    eras = rng.integers(0, E, N).astype(np.int32)
+    req = np.zeros(F, np.int8)
+    if mono:
+        req[0], req[3] = 1, -1
+    rec = O.grow_tree(codes, g, h, eras, E, B, D, 20, 0.0, 1e-6, mono_required=req if mono else None)
+    ctab, leaf = cpu_gbm.grow_tree(codes, g, h, eras, E, B, D, 20, 0.0, 1e-6, mono_required=req if mono else None,
+                                   threads=4)
+    _c_tree_matches(ctab, rec.tree)
+    np.testing.assert_array_equal(leaf, rec.leaf_of_row)
+
+
+def test_c_restatement_boosting_loops_match_the_oracle():
+    """Regression boosting equals fit_regression (scores within 1e-9); the
+    first softmax round's K trees equal the oracle trees grown on
+    softmax_grad_hess of the log-prior scores."""
+    from oracle import cpu_gbm
+    rng = np.random.default_rng(44)
+    N, F, B = 4000, 8, 32
+    codes = rng.integers(0, B, (N, F)).astype(np.int8)
+    y = codes[:, 0] * 0.1 - (codes[:, 2] > 20) + 0.2 * rng.standard_normal(N)
+    w = rng.uniform(0.5, 2.0, N)
+    eras = np.zeros(N, np.int32)
+    base_o, recs, sc_o = O.fit_regression(codes, y, w, eras, 1, B, 4, 5, 0.3)
+    base_c, tabs, sc_c = cpu_gbm.fit_regression(codes, y, w, eras, 1, B, 4, 5, 0.3, threads=3)
+    assert abs(base_c - base_o) < 1e-12
+    for t, rec in enumerate(recs):
+        _c_tree_matches(tuple(a[t] for a in tabs), rec.tree)
+    np.testing.assert_allclose(sc_c, sc_o, rtol=1e-9, atol=1e-9)
+    K = 3
+    lab = (codes[:, 1] % K).astype(np.int32)
+    tabs_k, _ = cpu_gbm.fit_softmax(codes, lab, np.ones(N), K, B, 3, 1, 0.1, threads=2)
+    counts = np.bincount(lab, minlength=K)
+    gk, hk = O.softmax_grad_hess(np.tile(np.log(counts / N + 1e-10), (N, 1)), lab, np.ones(N))
+    for k in range(K):
+        rec = O.grow_tree(codes, gk[:, k], hk[:, k], eras, 1, B, 3, 20, 0.0, 1e-6)
+        _c_tree_matches(tuple(a[0, k] for a in tabs_k), rec.tree)
