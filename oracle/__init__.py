@@ -1,8 +1,9 @@
 """CPU oracle for the WarpGBM training / inference hot path.
 
 TEST INFRASTRUCTURE ONLY.  Only ``tests/``, ``__graft_entry__.smoke()`` and the
-``cpu_baseline`` leg of ``bench.py`` may import this package, and only as the
-checker (or as the timed CPU baseline).  The product path (``warpgbm_amd``)
+``cpu_baseline`` legs of ``bench.py`` may import this package, and only as the
+checker (or as the timed CPU baseline); ``__graft_entry__.build()`` compiles
+its C restatement.  The product path (``warpgbm_amd``)
 never imports it and fails loudly when its HIP library is missing.
 
 ``cpu_gbm`` (``cpu_gbm.c`` through ctypes, built by ``make -C oracle``) is a
