@@ -555,20 +555,6 @@ def main():
     achieved = alg_bytes / (build_ms * 1e-3) / 1e9
     log(f"hist build {build_ms:.1f} ms over {n_build} launches ({achieved:.0f} GB/s), merge {merge_ms:.1f} ms")
 
-    # plain per-node engine on the same data
-    plain = None
-    if not args.no_plain:
-        pt = min(args.plain_trees, args.trees)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        mpl = fit("plain", trees=pt)
-        torch.cuda.synchronize()
-        pdt = (time.perf_counter() - t0) * (args.trees / pt)
-        same = trees_identical(mb.forest[:pt], mpl.forest[:pt])
-        plain = {"fit_s": pdt, "trees_timed": pt, "trees_identical_to_batched": bool(same),
-                 "speedup_batched_vs_plain": pdt / dt}
-        log(f"plain fit: {pdt:.2f} s (x{pdt / dt:.1f}); identical trees: {same}")
-
     # predict latency: 10k-row requests on the resident compiled forest
     tb = mb._last_tables
     hf = HeapForest.from_tables(tb["status"].reshape(-1, tb["status"].shape[-1]),
@@ -616,6 +602,23 @@ def main():
     serve = None
     if not args.no_serve:
         serve = serving_bench(args, dev)
+
+    # plain per-node engine on the same data
+    plain = None
+    if not args.no_plain:
+        pt = min(args.plain_trees, args.trees)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        mpl = fit("plain", trees=pt)
+        torch.cuda.synchronize()
+        pdt = (time.perf_counter() - t0) * (args.trees / pt)
+        same = trees_identical(mb.forest[:pt], mpl.forest[:pt])
+        plain = {"fit_s": pdt, "trees_timed": pt, "trees_identical_to_batched": bool(same),
+                 "speedup_batched_vs_plain": pdt / dt}
+        log(f"plain fit: {pdt:.2f} s (x{pdt / dt:.1f}); identical trees: {same}")
+        del mpl
+
+
 
     others = None
     if not args.no_suite:
