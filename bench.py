@@ -39,7 +39,8 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-plain", action="store_true")
     p.add_argument("--profile-only", action="store_true",
-                   help="binning and one fit with the automatic tiling, no extras (for rocprof)")
+                   help="binning, a 256 MiB copy (counter calibration), a tuning fit and one measured fit, "
+                        "no extras (for rocprof; the measured fit starts at the last k_round_init)")
     p.add_argument("--profile-serve", action="store_true",
                    help="the 1000-tree depth-8 serving forest: 40 traversal launches of 10k rows and 10 of "
                         "100k rows, block and tree order (for rocprof)")
@@ -523,7 +524,10 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
-        fit("batched", autotune=False)
+        fit("batched")  # tunes the histogram tiling for this shape (trial rounds)
+        torch.cuda.synchronize()
+        log("profile-only: tuning fit done")
+        fit("batched")  # the measured fit: tuned tiling, as the timed fits use
         torch.cuda.synchronize()
         log("profile-only fit done")
         return
