@@ -154,14 +154,17 @@ int wgbm_bin_matrix(const float* X, int64_t n, int32_t n_feat, const float* edge
 /* ---------------------------------------------------------------- batched level-wise training */
 
 /* Histogram tiling shared by every histogram build (plain and batched):
- * rows per chunk, waves per workgroup, features per lane (load width) and
- * rows whose code loads are in flight per wave.  0 = automatic.  Sums are
- * exact fixed-point integers, so the tiling changes speed, never a bit of a
- * histogram.  wgbm_get_hist_config returns the configuration used for a
- * shape (const_hessian: every row has the same hessian, as in a regression
- * fit with constant weights): out4 = {chunk_rows, waves, fpl, depth}. */
-int wgbm_set_hist_config(int32_t chunk_rows, int32_t waves, int32_t features_per_lane, int32_t rows_in_flight);
-int wgbm_get_hist_config(int32_t n_feat, int32_t num_bins, int64_t n_cols, int32_t const_hessian, int32_t* out4);
+ * rows per chunk, waves per workgroup, features per lane (load width), rows
+ * whose code loads are in flight per wave, and workgroup slots per launch
+ * (work items aimed at: 128, 256, 512 or 1024; fewer slots, fewer chunk
+ * partials to merge).  0 = automatic.  Sums are exact fixed-point integers,
+ * so the tiling changes speed, never a bit of a histogram.
+ * wgbm_get_hist_config returns the configuration used for a shape
+ * (const_hessian: every row has the same hessian, as in a regression fit
+ * with constant weights): out5 = {chunk_rows, waves, fpl, depth, slots}. */
+int wgbm_set_hist_config(int32_t chunk_rows, int32_t waves, int32_t features_per_lane, int32_t rows_in_flight,
+                         int32_t slots);
+int wgbm_get_hist_config(int32_t n_feat, int32_t num_bins, int64_t n_cols, int32_t const_hessian, int32_t* out5);
 /* The same knobs for one histogram shape only (n_feat features of a bin
  * matrix with n_cols columns, num_bins bins, const_hessian as above); they
  * take precedence over wgbm_set_hist_config for that shape.  All-zero knobs
@@ -171,7 +174,8 @@ int wgbm_get_hist_config(int32_t n_feat, int32_t num_bins, int64_t n_cols, int32
  * Takes over from the reference's threads_per_block / rows_per_thread
  * launch knobs (warpgbm/core.py:68-69, :200-207). */
 int wgbm_set_hist_config_shape(int32_t n_feat, int32_t num_bins, int64_t n_cols, int32_t const_hessian,
-                               int32_t chunk_rows, int32_t waves, int32_t features_per_lane, int32_t rows_in_flight);
+                               int32_t chunk_rows, int32_t waves, int32_t features_per_lane, int32_t rows_in_flight,
+                               int32_t slots);
 
 /* Live timing of histogram launches with HIP events on their streams:
  * begin clears and enables; end waits and returns

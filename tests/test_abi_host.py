@@ -40,10 +40,12 @@ def test_library_rejects_bad_arguments_without_a_device():
     lib = _lib.load()
     rc = lib.wgbm_compute_split(None, None, 1, 4, 1, 0.0, 1.0, 0.0, None, None, None)
     assert rc != 0 and b"shape" in lib.wgbm_last_error()
-    rc = lib.wgbm_set_hist_config(0, 3, 0, 0)
+    rc = lib.wgbm_set_hist_config(0, 3, 0, 0, 0)
     assert rc != 0
-    rc = lib.wgbm_set_hist_config(0, 0, 5, 0)
+    rc = lib.wgbm_set_hist_config(0, 0, 5, 0, 0)
     assert rc != 0 and b"features_per_lane" in lib.wgbm_last_error()
+    rc = lib.wgbm_set_hist_config(0, 0, 0, 0, 300)
+    assert rc != 0 and b"slots" in lib.wgbm_last_error()
     with pytest.raises(RuntimeError, match="num_bins"):
         _lib.call("wgbm_compute_histogram3", None, 10, 4, None, None, None, 5, None, 4, None, None, None, 1, 500,
                   None, 0, None)
@@ -64,9 +66,10 @@ def test_workspace_queries():
     assert 40e6 < ws < 2e9  # perm/gh buffers, histograms, partials
     c.n_bins = 1
     assert lib.wgbm_fit_workspace_bytes(ctypes.byref(c)) == 0
-    out = (ctypes.c_int32 * 4)()
+    out = (ctypes.c_int32 * 5)()
     lib.wgbm_get_hist_config(1000, 32, 1000, 0, out)
     assert out[0] > 0 and out[1] in (1, 2, 4, 8, 16) and out[2] in (1, 2, 4, 8, 16) and out[3] in (1, 2, 4, 8, 16, 32, 64)
+    assert out[4] == 512
 
 
 def test_hyperparameter_validation_messages():

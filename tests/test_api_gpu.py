@@ -331,4 +331,4 @@ def test_fit_tunes_the_histogram_tiling_per_shape(tmp_path, monkeypatch):
         assert m5.hist_config_ is not None and m127.hist_config_ is not None
     finally:
         for nb, one in ((32, 1), (32, 0), (5, 1), (127, 1)):
-            _lib.call("wgbm_set_hist_config_shape", 64, nb, 64, one, 0, 0, 0, 0)
+            _lib.call("wgbm_set_hist_config_shape", 64, nb, 64, one, 0, 0, 0, 0, 0)

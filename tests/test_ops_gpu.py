@@ -67,8 +67,9 @@ def test_histogram_matches_oracle(dev, N, F, B, E, subset):
     np.testing.assert_allclose(Gg.sum(axis=(0, 2)), g[rows].astype(np.float64).sum(), rtol=1e-4, atol=1e-2)
 
 
-@pytest.mark.parametrize("cfg", [(0, 8, 1, 0), (0, 16, 1, 0), (0, 8, 2, 0), (0, 16, 2, 0), (0, 8, 4, 0),
-                                 (0, 16, 4, 0), (0, 4, 4, 16), (0, 8, 8, 0), (0, 8, 16, 0), (1024, 16, 4, 0)])
+@pytest.mark.parametrize("cfg", [(0, 8, 1, 0, 0), (0, 16, 1, 0, 0), (0, 8, 2, 0, 0), (0, 16, 2, 0, 0),
+                                 (0, 8, 4, 0, 0), (0, 16, 4, 0, 256), (0, 4, 4, 16, 128), (0, 8, 8, 0, 1024),
+                                 (0, 8, 16, 0, 0), (1024, 16, 4, 0, 256)])
 @pytest.mark.parametrize("subset", [False, True])
 def test_histogram_every_tiling_matches_oracle(dev, cfg, subset):
     """Every (chunk, waves, features per lane, depth) the autotuner can
@@ -87,7 +88,7 @@ def test_histogram_every_tiling_matches_oracle(dev, cfg, subset):
     try:
         Gg, Hg = _hist_gpu(dev, codes, g, h, rows, feats, eras, E, B)
     finally:
-        _lib.call("wgbm_set_hist_config", 0, 0, 0, 0)
+        _lib.call("wgbm_set_hist_config", 0, 0, 0, 0, 0)
     _check_hist_bins("histogram op, every tiling", Gg, Hg, codes, g, h, rows, feats, eras, E, B)
 
 

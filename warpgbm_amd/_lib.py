@@ -37,7 +37,7 @@ class FitIO(C.Structure):
     ]
 
 
-ABI_VERSION = 2  # wgbm_abi_version() of the library this binding matches
+ABI_VERSION = 3  # wgbm_abi_version() of the library this binding matches
 
 # name -> (restype, argtypes)
 SIGNATURES = {
@@ -62,8 +62,8 @@ SIGNATURES = {
     "wgbm_edges_from_sorted": (C.c_int, [vp, i64, i32, i32, vp, i32, vp]),
     "wgbm_bin_matrix_workspace_bytes": (sz, [i32, i32]),
     "wgbm_bin_matrix": (C.c_int, [vp, i64, i32, vp, i32, vp, vp, sz, vp]),
-    "wgbm_set_hist_config": (C.c_int, [i32, i32, i32, i32]),
-    "wgbm_set_hist_config_shape": (C.c_int, [i32, i32, i64, i32, i32, i32, i32, i32]),
+    "wgbm_set_hist_config": (C.c_int, [i32, i32, i32, i32, i32]),
+    "wgbm_set_hist_config_shape": (C.c_int, [i32, i32, i64, i32, i32, i32, i32, i32, i32]),
     "wgbm_get_hist_config": (C.c_int, [i32, i32, i64, i32, C.POINTER(i32)]),
     "wgbm_profile_begin": (C.c_int, []),
     "wgbm_profile_end": (C.c_int, [C.POINTER(f64)]),

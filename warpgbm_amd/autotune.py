@@ -2,8 +2,9 @@
 
 A configuration is rows per chunk C, waves per workgroup W (the waves of a
 workgroup share one LDS histogram), features per lane FPL (load width and
-LDS footprint, i.e. how many sub-histograms a workgroup holds) and load
-depth L (rows whose code loads are in flight per wave).  These take over
+LDS footprint, i.e. how many sub-histograms a workgroup holds), load depth
+L (rows whose code loads are in flight per wave) and slots S (workgroups a
+launch aims at: fewer slots mean fewer chunk partials to write and merge).  These take over
 from the reference's threads_per_block / rows_per_thread launch knobs
 (warpgbm/core.py:68-69, :200-207).
 
@@ -72,24 +73,25 @@ def candidates(n_feat: int, n_bins: int, n_cols: int, const_hessian: bool = Fals
             continue
         if fpl > 1 and 64 * fpl > 2 * max(n_feat, 1):
             continue  # slab far wider than the features
-        for c, w, d in itertools.product((4096, 8192, 16384), (8, 16) if fpl <= 4 else (4, 8), depths):
-            out.append((c, w, fpl, d))
+        for c, w, d, s in itertools.product((4096, 8192, 16384), (8, 16) if fpl <= 4 else (4, 8), depths,
+                                            (256, 512)):
+            out.append((c, w, fpl, d, s))
     return out
 
 
 def current(n_feat: int, n_bins: int, n_cols: int, const_hessian: bool = False):
-    out = (ctypes.c_int32 * 4)()
+    out = (ctypes.c_int32 * 5)()
     _lib.call("wgbm_get_hist_config", n_feat, n_bins, n_cols, int(const_hessian), out)
     return tuple(out)
 
 
 def apply(cfg) -> None:
-    c, w, f, d = cfg
-    _lib.call("wgbm_set_hist_config", int(c), int(w), int(f), int(d))
+    c, w, f, d, s = cfg
+    _lib.call("wgbm_set_hist_config", int(c), int(w), int(f), int(d), int(s))
 
 
 def reset() -> None:
-    _lib.call("wgbm_set_hist_config", 0, 0, 0, 0)
+    _lib.call("wgbm_set_hist_config", 0, 0, 0, 0, 0)
 
 
 def tune(n_feat: int, n_bins: int, n_cols: int | None = None, rows: int = 262_144, device="cuda",
@@ -101,7 +103,7 @@ def tune(n_feat: int, n_bins: int, n_cols: int | None = None, rows: int = 262_14
     """
     from .cuda import node_kernel as nk
     n_cols = n_feat if n_cols is None else n_cols
-    key = f"{n_feat}x{n_bins}x{n_cols}@{torch.cuda.get_device_name(device)}"
+    key = f"op2:{n_feat}x{n_bins}x{n_cols}@{torch.cuda.get_device_name(device)}"
     disk = _load_cache() if use_cache else {}
     if use_cache and key in _MEM:
         apply(_MEM[key])
@@ -155,7 +157,7 @@ def tune_fit(model, cfg, n_feat, rounds: int = 2, repeats: int = 2, use_cache: b
     one_plane = cfg.objective == 0 and cfg.const_weight != 0
     B, n_cols = int(cfg.n_bins), int(cfg.n_cols)
     dev = torch.device(model.device)
-    key = f"fit:{n_feat}x{B}x{n_cols}:{'one' if one_plane else 'two'}-plane@{torch.cuda.get_device_name(dev)}"
+    key = f"fit2:{n_feat}x{B}x{n_cols}:{'one' if one_plane else 'two'}-plane@{torch.cuda.get_device_name(dev)}"
     disk = _load_cache() if use_cache else {}
     hit = _MEM.get(key) if use_cache else None
     if hit is None and use_cache and key in disk:
@@ -167,7 +169,7 @@ def tune_fit(model, cfg, n_feat, rounds: int = 2, repeats: int = 2, use_cache: b
     t0 = time.perf_counter()
     gen = torch.Generator(device=dev)
     gen.manual_seed(0)
-    _lib.call("wgbm_set_hist_config_shape", n_feat, B, n_cols, int(one_plane), 0, 0, 0, 0)
+    _lib.call("wgbm_set_hist_config_shape", n_feat, B, n_cols, int(one_plane), 0, 0, 0, 0, 0)
     auto = current(n_feat, B, n_cols, one_plane)
     model._trial_rounds_ms(cfg, 1, gen)  # warm-up: first launches of the kernels
     times = {}

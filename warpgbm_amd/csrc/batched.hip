@@ -400,26 +400,32 @@ k_root_stats_plan(const int32_t* __restrict__ part, int n_part, int32_t* __restr
 
 using namespace wgbm;
 
+static bool valid_slots(int32_t slots) {
+  return slots == 0 || slots == 128 || slots == 256 || slots == 512 || slots == 1024;
+}
+
 extern "C" int wgbm_set_hist_config(int32_t chunk_rows, int32_t waves, int32_t features_per_lane,
-                                    int32_t rows_in_flight) {
+                                    int32_t rows_in_flight, int32_t slots) {
   WGBM_REQUIRE(chunk_rows >= 0 && (waves == 0 || waves == 1 || waves == 2 || waves == 4 || waves == 8 || waves == 16),
                "bad histogram configuration");
   WGBM_REQUIRE(features_per_lane == 0 || features_per_lane == 1 || features_per_lane == 2 ||
                    features_per_lane == 4 || features_per_lane == 8 || features_per_lane == 16,
                "features_per_lane must be 0 (auto), 1, 2, 4, 8 or 16");
   WGBM_REQUIRE(rows_in_flight >= 0 && rows_in_flight <= 64, "rows_in_flight must be in [0, 64]");
+  WGBM_REQUIRE(valid_slots(slots), "slots must be 0 (auto), 128, 256, 512 or 1024");
   HistConfig c;
   c.chunk_rows = chunk_rows;
   c.waves = waves;
   c.fpl = features_per_lane;
   c.depth = rows_in_flight;
+  c.slots = slots;
   set_hist_override(c);
   return kOk;
 }
 
 extern "C" int wgbm_set_hist_config_shape(int32_t n_feat, int32_t n_bins, int64_t n_cols, int32_t const_hessian,
                                           int32_t chunk_rows, int32_t waves, int32_t features_per_lane,
-                                          int32_t rows_in_flight) {
+                                          int32_t rows_in_flight, int32_t slots) {
   WGBM_REQUIRE(n_feat >= 1 && n_bins >= 1 && n_bins <= 127 && n_cols >= 1, "bad histogram shape");
   WGBM_REQUIRE(chunk_rows >= 0 && (waves == 0 || waves == 1 || waves == 2 || waves == 4 || waves == 8 || waves == 16),
                "bad histogram configuration");
@@ -427,11 +433,13 @@ extern "C" int wgbm_set_hist_config_shape(int32_t n_feat, int32_t n_bins, int64_
                    features_per_lane == 4 || features_per_lane == 8 || features_per_lane == 16,
                "features_per_lane must be 0 (auto), 1, 2, 4, 8 or 16");
   WGBM_REQUIRE(rows_in_flight >= 0 && rows_in_flight <= 64, "rows_in_flight must be in [0, 64]");
+  WGBM_REQUIRE(valid_slots(slots), "slots must be 0 (auto), 128, 256, 512 or 1024");
   HistConfig c;
   c.chunk_rows = chunk_rows;
   c.waves = waves;
   c.fpl = features_per_lane;
   c.depth = rows_in_flight;
+  c.slots = slots;
   if (c.fpl > 0) {
     WGBM_REQUIRE(lds_bytes(c, n_bins, const_hessian != 0) <= 159 * 1024,
                  "%d features per lane need more LDS than a workgroup has at %d bins", c.fpl, n_bins);
@@ -441,12 +449,13 @@ extern "C" int wgbm_set_hist_config_shape(int32_t n_feat, int32_t n_bins, int64_
 }
 
 extern "C" int wgbm_get_hist_config(int32_t n_feat, int32_t n_bins, int64_t n_cols, int32_t const_hessian,
-                                    int32_t* out4) {
+                                    int32_t* out5) {
   const HistConfig c = fit_hist_config(n_feat, n_bins, n_cols, const_hessian != 0);
-  out4[0] = c.chunk_rows;
-  out4[1] = c.waves;
-  out4[2] = c.fpl;
-  out4[3] = c.depth;
+  out5[0] = c.chunk_rows;
+  out5[1] = c.waves;
+  out5[2] = c.fpl;
+  out5[3] = c.depth;
+  out5[4] = hist_slots(c);
   return kOk;
 }
 

@@ -31,6 +31,7 @@
 #include "wgbm_internal.h"
 #include "wgbm_stats.h"
 #include "wgbm_plan.h"
+#include <algorithm>
 #include <mutex>
 #include <type_traits>
 #include <utility>
@@ -636,7 +637,17 @@ k_hist_merge(const HistArgs a, const int32_t* __restrict__ n_multi) {
   const int64_t n_items = (int64_t)(*n_multi) * a.nslab * neb;
   const int el = threadIdx.x % WPI;
   const int part = threadIdx.x / WPI;
-  for (int64_t it = blockIdx.x; it < n_items; it += gridDim.x) {
+  // Consecutive items cover consecutive (j, bin) rows of one slab, so eight
+  // of them together write (and, with siblings, read the parent's) whole
+  // lines of the [feature][bin] output.  Virtual item vi runs on workgroup
+  // vi % gridDim.x, whose XCD is vi % 8 (gridDim.x is a multiple of 8); vi maps
+  // to items so that each group of eight consecutive items shares one XCD
+  // and runs side by side: those lines then meet in one L2.
+  const int64_t n_virtual = (n_items + 63) & ~(int64_t)63;
+  for (int64_t vi = blockIdx.x; vi < n_virtual; vi += gridDim.x) {
+    const int64_t q = vi >> 3;
+    const int64_t it = (((q >> 3) << 6) | ((vi & 7) << 3)) | (q & 7);
+    if (it >= n_items) continue;  // uniform
     const int m = (int)(it / (a.nslab * neb));
     const int r = (int)(it - (int64_t)m * a.nslab * neb);
     const int slab = r / neb;
@@ -696,12 +707,12 @@ k_hist_merge(const HistArgs a, const int32_t* __restrict__ n_multi) {
     if (part == 0 && e < SL) {
       I128 sg = acc.g;
       long long sx = acc.x;
-      for (int q = 1; q < P; ++q) {
+      for (int gp = 1; gp < P; ++gp) {
         I128 o;
-        o.lo = s_glo[q * WPI + el];
-        o.hi = s_ghi[q * WPI + el];
+        o.lo = s_glo[gp * WPI + el];
+        o.hi = s_ghi[gp * WPI + el];
         sg.add(o);
-        sx += s_h[q * WPI + el];
+        sx += s_h[gp * WPI + el];
       }
       if (f_base + fl < a.n_feat) {
         const long long fw = any_fine ? take_fine(a, fine_slot(a, slot), f_base + fl, bin) : 0;
@@ -906,12 +917,13 @@ void set_hist_shape_config(int n_feat, int n_bins, int64_t n_cols, bool one_plan
   for (size_t i = 0; i < g_shape_cfg.size(); ++i) {
     ShapeConfig& e = g_shape_cfg[i];
     if (e.n_feat == n_feat && e.n_bins == n_bins && e.n_cols == n_cols && e.one_plane == one_plane) {
-      if (c.chunk_rows == 0 && c.waves == 0 && c.fpl == 0 && c.depth == 0) g_shape_cfg.erase(g_shape_cfg.begin() + i);
+      if (c.chunk_rows == 0 && c.waves == 0 && c.fpl == 0 && c.depth == 0 && c.slots == 0)
+        g_shape_cfg.erase(g_shape_cfg.begin() + i);
       else e.c = c;
       return;
     }
   }
-  if (c.chunk_rows || c.waves || c.fpl || c.depth) g_shape_cfg.push_back({n_feat, n_bins, n_cols, one_plane, c});
+  if (c.chunk_rows || c.waves || c.fpl || c.depth || c.slots) g_shape_cfg.push_back({n_feat, n_bins, n_cols, one_plane, c});
 }
 
 static HistConfig apply_override(HistConfig c, const HistConfig& o, int64_t row_stride) {
@@ -924,6 +936,7 @@ static HistConfig apply_override(HistConfig c, const HistConfig& o, int64_t row_
   c.waves = o.waves > 0 ? o.waves : (c.fpl <= 4 ? 16 : 8);
   if (c.fpl > 4 && c.waves > 8) c.waves = 8;
   c.depth = clamp_depth(c.fpl, c.waves, o.depth);
+  c.slots = o.slots > 0 ? o.slots : kHistSlots;
   return c;
 }
 
@@ -974,7 +987,7 @@ HistConfig auto_hist_config(int n_feat, int n_bins, int64_t row_stride, bool one
 
 static int64_t hist_chunk_target(const HistConfig& c, int n_feat) {
   const int64_t nslab = ceil_div(n_feat > 0 ? n_feat : 1, kWave * c.fpl);
-  const int64_t t = kHistSlots / nslab;
+  const int64_t t = hist_slots(c) / nslab;
   return t > 0 ? t : 1;
 }
 
@@ -1077,11 +1090,11 @@ static int launch_mode(const HistArgs& a, const HistConfig& c, int64_t groups, h
 template <int FPL, int L>
 static int launch_fpl(const HistArgs& a, const HistConfig& c, int64_t max_chunks, int64_t max_multi,
                       int32_t* multi, int32_t* n_multi, hipStream_t st) {
-  // Work items (chunk, slab); a grid of at most kHistSlots workgroups loops
-  // over them.
+  // Work items (chunk, slab); a grid of at most hist_slots(c) workgroups
+  // (a multiple of 8) loops over them.
   const int64_t items = ceil_div(max_chunks, 8) * 8 * a.nslab;
   if (items > 0x7fffffff) { set_error("histogram grid too large"); return kTooBig; }
-  const int64_t groups = items < kHistSlots ? items : kHistSlots;
+  const int64_t groups = items < hist_slots(c) ? items : hist_slots(c);
   // One plane when the caller guarantees one hessian value for every row;
   // otherwise the two-plane variant builds every node.  For a node whose rows
   // happen to share one hessian h, the hessian plane sums count copies of
@@ -1099,7 +1112,7 @@ static int launch_fpl(const HistArgs& a, const HistConfig& c, int64_t max_chunks
   if (max_multi > 0) {
     const int SL = kWave * FPL * a.n_bins;
     const int64_t merge_items = max_multi * a.nslab * ceil_div(SL, kMergeThreads / kMergeMaxParts);
-    const int64_t mg = merge_items < 2048 ? merge_items : 2048;
+    const int64_t mg = std::min<int64_t>((merge_items + 7) & ~(int64_t)7, 2048);  // a multiple of 8
     timer_mark(g_timer.merge, true, st);
     hipLaunchKernelGGL((k_hist_merge<FPL>), dim3((unsigned)mg), dim3(kMergeThreads), 0, st, a, n_multi);
     WGBM_LAUNCHED("k_hist_merge");

@@ -13,10 +13,14 @@ struct HistConfig {
   int waves = 0;
   int fpl = 0;     // features per lane (load width)
   int depth = 0;   // rows whose code loads are in flight per wave (load block)
+  int slots = 0;   // workgroups of a launch (work items aimed at), 0: kHistSlots
 };
 
-// Workgroup slots of the histogram build: two per CU on MI355X's 256 CUs.
+// Default workgroup slots of the histogram build: two per CU on MI355X's 256
+// CUs.  Fewer slots mean fewer chunk partials to write and merge, more mean
+// more workgroups to share the LDS-bound adds; the autotuner picks per shape.
 constexpr int kHistSlots = 512;
+inline int hist_slots(const HistConfig& c) { return c.slots > 0 ? c.slots : kHistSlots; }
 constexpr int kChunkMin = 1024;
 
 struct HistArgs {
