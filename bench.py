@@ -317,8 +317,9 @@ from warpgbm_amd.synthetic import era_workload, multiclass_workload  # noqa: E40
 
 def suite(dev):
     """The era-splitting and multiclass workloads: full batched fits (X in
-    host memory, so binning and the host-to-device copy are included) and the
-    plain per-node engine on the first trees, with a tree-equality check."""
+    host memory, so binning and the host-to-device copy are included; after a
+    5-tree warm-up, the faster of two timed fits) and the plain per-node
+    engine on the first trees, with a tree-equality check."""
     import contextlib, io
     import torch
     from warpgbm_amd import WarpGBM
@@ -336,7 +337,9 @@ def suite(dev):
     X, y, era = era_workload()
     kw = dict(objective="regression", num_bins=5, max_depth=8, n_estimators=200, learning_rate=0.01, device=dev)
     fit(dict(kw, engine="batched", n_estimators=5), X, y, era_id=era)  # warm-up
-    mb, tb = fit(dict(kw, engine="batched"), X, y, era_id=era)
+    (mb, tb), (_, tb2) = fit(dict(kw, engine="batched"), X, y, era_id=era), fit(dict(kw, engine="batched"), X, y,
+                                                                                 era_id=era)
+    tb = min(tb, tb2)  # the faster of two timed fits
     k = 10
     mp, tp = fit(dict(kw, engine="plain", n_estimators=k), X, y, era_id=era)
     out["eras_500kx400_E200_d8_200trees"] = {
@@ -346,7 +349,8 @@ def suite(dev):
     X, lab = multiclass_workload()
     kw = dict(objective="multiclass", num_bins=32, max_depth=6, n_estimators=200, learning_rate=0.1, device=dev)
     fit(dict(kw, engine="batched", n_estimators=5), X, lab)
-    mb, tb = fit(dict(kw, engine="batched"), X, lab)
+    (mb, tb), (_, tb2) = fit(dict(kw, engine="batched"), X, lab), fit(dict(kw, engine="batched"), X, lab)
+    tb = min(tb, tb2)  # the faster of two timed fits
     mp, tp = fit(dict(kw, engine="plain", n_estimators=k), X, lab)
     same = all(trees_identical(a, b) for a, b in zip(mb.forest[:k], mp.forest[:k]))
     out["multiclass_200kx100_K5_d6_200rounds"] = {
