@@ -135,10 +135,15 @@ __device__ __forceinline__ int fine_slot(const HistArgs& a, int slot) {
   return (c >> 1) * E + (slot - c * E);
 }
 
-// Takes a fine word back to zero and returns it.
+// Takes a fine word back to zero and returns it.  No fine row adds to the
+// word while it is taken (the adds of its slot came before a barrier or an
+// earlier launch), so most words -- those no fine row reached -- are only
+// read; a non-zero one is exchanged for zero.
 __device__ __forceinline__ long long take_fine(const HistArgs& a, int fslot, int f, int bin) {
   unsigned long long* p = reinterpret_cast<unsigned long long*>(a.fine) +
                           ((int64_t)fslot * a.n_feat + f) * a.n_bins + bin;
+  const unsigned long long v = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (v == 0ull) return 0;
   return (long long)__hip_atomic_exchange(p, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
