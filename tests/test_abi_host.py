@@ -98,6 +98,33 @@ def test_monotone_normalisation():
         m._normalize_monotonic_constraints(3)
 
 
+def test_fit_argument_errors_before_any_device_work():
+    """tests/test_sample_weight.py:121-139 and
+    tests/test_monotonic_constraints.py:194-240: bad weights and bad
+    constraints are refused by fit with the reference's messages, before
+    the data reaches a device (so this runs without a GPU)."""
+    from warpgbm_amd import WarpGBM
+    rng = np.random.default_rng(3)
+    X = rng.standard_normal((120, 3)).astype(np.float32)
+    y = rng.standard_normal(120).astype(np.float32)
+    m = WarpGBM(n_estimators=5, max_depth=2)
+    w = np.ones(120, np.float32)
+    w[0] = 0.0
+    with pytest.raises(ValueError, match="strictly positive"):
+        m.fit(X, y, sample_weight=w)
+    with pytest.raises(ValueError, match="same length"):
+        m.fit(X, y, sample_weight=np.ones(119, np.float32))
+    with pytest.raises(ValueError, match="sample_weight_eval can only be used"):
+        m.fit(X, y, sample_weight_eval=np.ones(120, np.float32))
+    cases = [([1, 0], ValueError, "length must equal n_features"),
+             ([1, 2, 0], ValueError, "values must be one of"),
+             ({5: 1}, ValueError, "out of range"),
+             ({"0": 1}, TypeError, "integer feature indices")]
+    for mono, err, msg in cases:
+        with pytest.raises(err, match=msg):
+            WarpGBM(n_estimators=5, max_depth=2, monotonic_constraints=mono).fit(X, y)
+
+
 def test_tables_to_dicts_and_flatten_roundtrip():
     """Level-order node tables (as the batched trainer writes them) become
     the reference's nested dicts, whose flattening traverses identically."""
