@@ -65,7 +65,11 @@ def test_multiclass_heap_forest_matches_predict_proba():
     np.testing.assert_allclose(p, m.predict_proba(X), rtol=1e-5, atol=1e-6)
 
 
-def test_server_alone_equals_mixed(model_and_codes):
+@pytest.mark.parametrize("piece_rows", [None, 1024])
+def test_server_alone_equals_mixed(model_and_codes, piece_rows):
+    """Requests served one at a time equal the same requests sent by four
+    concurrent clients (launches shared), whole or cut into 1024-row
+    pieces spread over launches."""
     from warpgbm_amd.serving import HeapForest
     from warpgbm_amd.server import PredictServer
     m, codes = model_and_codes
@@ -76,7 +80,7 @@ def test_server_alone_equals_mixed(model_and_codes):
     starts = rng.integers(0, c.shape[0] - 3000, size=40)
     reqs = [c[s:s + k] for s, k in zip(starts, sizes)]
     alone = []
-    srv = PredictServer(hf, memory_budget=64 << 20, max_batch_rows=8192, slots=3)
+    srv = PredictServer(hf, memory_budget=64 << 20, max_batch_rows=8192, slots=3, piece_rows=piece_rows)
     try:
         for r in reqs:  # one at a time
             alone.append(srv.submit(r).result(timeout=60))
