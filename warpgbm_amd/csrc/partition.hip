@@ -84,8 +84,12 @@ k_part_count(PartArgs a) { part_count_block(a, blockIdx.x); }
 // One workgroup: per-segment left totals and chunk offsets, child segment
 // tables, smaller-child build list and sibling map.
 
-constexpr int kPsStage = 4096;  // values per staged scan pass (two arrays of them in LDS)
+constexpr int kPsStage = 16384;  // values per staged scan pass (two arrays of them in dynamic LDS)
 constexpr int kPsLoads = 8;     // items a thread loads together in the gather loops
+// Stage index of item j: one pad word after every 32, so that the threads of
+// a wave, each scanning its own run of consecutive items, hit distinct banks.
+__device__ __forceinline__ int ps_pad(int j) { return j + (j >> 5); }
+constexpr int kPsStagePad = kPsStage + kPsStage / 32;
 
 // Exclusive scan of two values per item over n items (every thread calls
 // it): passes of kPsStage items are loaded into LDS (kPsLoads loads in flight
@@ -108,14 +112,14 @@ __device__ __forceinline__ int2 scan2_staged(int n, Get get, Put put, int* s0, i
 #pragma unroll
       for (int u = 0; u < kPsLoads; ++u) {
         const int k = k0 + u * nt + t;
-        if (k < span) { s0[k] = v[u].x; s1[k] = v[u].y; }
+        if (k < span) { s0[ps_pad(k)] = v[u].x; s1[ps_pad(k)] = v[u].y; }
       }
     }
     __syncthreads();
     int a0 = 0, a1 = 0;
     for (int i = 0; i < cpt; ++i) {
       const int j = t * cpt + i;
-      if (j < span) { a0 += s0[j]; a1 += s1[j]; }
+      if (j < span) { a0 += s0[ps_pad(j)]; a1 += s1[ps_pad(j)]; }
     }
     int tot0, tot1;
     int r0 = carry.x + block_exscan(a0, &tot0);
@@ -123,13 +127,13 @@ __device__ __forceinline__ int2 scan2_staged(int n, Get get, Put put, int* s0, i
     for (int i = 0; i < cpt; ++i) {
       const int j = t * cpt + i;
       if (j < span) {
-        const int x0 = s0[j], x1 = s1[j];
-        s0[j] = r0; s1[j] = r1;
+        const int x0 = s0[ps_pad(j)], x1 = s1[ps_pad(j)];
+        s0[ps_pad(j)] = r0; s1[ps_pad(j)] = r1;
         r0 += x0; r1 += x1;
       }
     }
     __syncthreads();
-    for (int k = t; k < span; k += nt) put(base + k, s0[k], s1[k]);
+    for (int k = t; k < span; k += nt) put(base + k, s0[ps_pad(k)], s1[ps_pad(k)]);
     carry.x += tot0;
     carry.y += tot1;
     __syncthreads();  // the next pass reuses the stage
@@ -143,7 +147,9 @@ __device__ __forceinline__ int2 scan2_staged(int n, Get get, Put put, int* s0, i
 // LDS in passes; the gather loops issue kPsLoads items' loads together.
 __global__ void __launch_bounds__(1024)
 k_part_scan(PartArgs a) {
-  __shared__ int s_v0[kPsStage], s_v1[kPsStage];
+  extern __shared__ int s_stage[];  // [2][kPsStagePad]
+  int* s_v0 = s_stage;
+  int* s_v1 = s_stage + kPsStagePad;
   const int E = a.n_eras;
   const int nseg = a.n_nodes * E;
   const int t = threadIdx.x, nt = blockDim.x;
@@ -421,7 +427,18 @@ static int part_scan_launch(const PartArgs& a, int64_t max_chunks, hipStream_t s
     hipLaunchKernelGGL(k_part_scan_lds, dim3(1), dim3(1024), 0, st, a);
     WGBM_LAUNCHED("k_part_scan_lds");
   } else {
-    hipLaunchKernelGGL(k_part_scan, dim3(1), dim3(1024), 0, st, a);
+    // One workgroup; its two stage arrays (132 KB) take most of a CU's LDS,
+    // which needs the launch's dynamic LDS limit raised once per device.
+    constexpr size_t lds = 2 * kPsStagePad * sizeof(int);
+    static bool opted[64] = {};
+    int dev = 0;
+    WGBM_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64 || !opted[dev]) {
+      WGBM_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_part_scan),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+      if (dev >= 0 && dev < 64) opted[dev] = true;
+    }
+    hipLaunchKernelGGL(k_part_scan, dim3(1), dim3(1024), lds, st, a);
     WGBM_LAUNCHED("k_part_scan");
   }
   return kOk;
