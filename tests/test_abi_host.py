@@ -98,6 +98,23 @@ def test_monotone_normalisation():
         m._normalize_monotonic_constraints(3)
 
 
+def test_era_inverse_equals_numpy_unique():
+    """The counting era map gives np.unique's (sorted ids, inverse) pair for
+    constant, narrow, negative, gapped, wide (sorted fallback), empty and
+    non-integer era arrays."""
+    from warpgbm_amd.core import _era_inverse
+    rng = np.random.default_rng(0)
+    cases = [np.ones(1000, np.int32), np.zeros(7, np.int64), rng.integers(-50, 50, 5000),
+             rng.integers(0, 1 << 30, 3000), rng.choice([3, 17, 99, -4], 10000), np.array([5]),
+             np.array([], np.int32), rng.integers(0, 200, 100_000).astype(np.int64),
+             np.array([2**31 - 1, -2**31, 0], np.int64), rng.normal(size=100) * 10]
+    for c in cases:
+        u, inv = _era_inverse(c)
+        u_ref, inv_ref = np.unique(np.asarray(c).astype(np.int32), return_inverse=True)
+        assert u.dtype == u_ref.dtype and np.array_equal(u, u_ref)
+        assert np.array_equal(inv.reshape(-1), inv_ref.reshape(-1))
+
+
 def test_fit_argument_errors_before_any_device_work():
     """tests/test_sample_weight.py:121-139 and
     tests/test_monotonic_constraints.py:194-240: bad weights and bad

@@ -101,6 +101,27 @@ def _to_device(X_np, device, dtype=torch.float32):
     return out
 
 
+def _era_inverse(era_id):
+    """(sorted unique era ids, index of each row's id among them), as
+    np.unique(era_id.astype(int32), return_inverse=True) gives them
+    (warpgbm/core.py:1284 takes torch.unique's, the same pair).  Ids that
+    are all equal, or span fewer than 2^22 values, are mapped by counting
+    in O(n) instead of sorting (a million rows: ~1 ms against 4 ms sorted,
+    and ~90 ms for the first sort of a process)."""
+    e = np.asarray(era_id).astype(np.int32).reshape(-1)
+    if e.size == 0:
+        return np.unique(e, return_inverse=True)
+    lo, hi = int(e.min()), int(e.max())
+    if lo == hi:
+        return np.array([lo], np.int32), np.zeros(e.size, np.int64)
+    if hi - lo < (1 << 22):
+        off = (e - lo).astype(np.int64) if lo != 0 else e.astype(np.int64)
+        present = np.bincount(off, minlength=hi - lo + 1) > 0
+        rank = np.cumsum(present) - 1
+        return (np.flatnonzero(present) + lo).astype(np.int32), rank[off]
+    return np.unique(e, return_inverse=True)
+
+
 _PREORDER = {}
 
 
@@ -535,8 +556,7 @@ class WarpGBM(BaseEstimator, RegressorMixin):
         self._invalidate_prediction_cache()
         self.num_samples, self.num_features = N, F
         self.num_outputs = 1
-        era = np.zeros(N, np.int32) if era_id is None else np.unique(np.asarray(era_id).astype(np.int32),
-                                                                      return_inverse=True)[1].astype(np.int32)
+        era = np.zeros(N, np.int32) if era_id is None else _era_inverse(era_id)[1].astype(np.int32)
         self._era_np = era.reshape(-1)
         self.unique_eras = torch.arange(int(self._era_np.max()) + 1, device=self.device)
         self.num_eras = len(self.unique_eras)
@@ -709,7 +729,7 @@ class WarpGBM(BaseEstimator, RegressorMixin):
             N, F = X_np.shape
             self.num_samples, self.num_features = N, F
             Y_gpu = torch.from_numpy(np.ascontiguousarray(Y_np)).to(dtype=torch.float32, device=self.device)
-            uniq, inv = np.unique(np.asarray(era_id_np).astype(np.int32), return_inverse=True)
+            uniq, inv = _era_inverse(era_id_np)
             self._era_np = inv.astype(np.int32).reshape(-1)
             era_indices = torch.from_numpy(self._era_np).to(self.device)
             unique_eras = torch.from_numpy(uniq).to(self.device)
