@@ -549,6 +549,10 @@ static int enqueue_round(const wgbm_fit_config& c, const FitLayout& L, const wgb
     WGBM_LAUNCHED("k_root_stats_plan");
     for (int d = 0; d <= D; ++d) {
       const int n_nodes = K << d;
+      // One era and few nodes: the level's decisions fold into its tail
+      // (no decision launch, no leaf or partition plan).
+      const bool fused = level_tail_fusable(n_nodes, E, L.KN);
+      const bool fused_next = d < D && level_tail_fusable((int64_t)n_nodes * 2, E, L.KN);
       const FitLayout::Tables& cur = L.tab[d & 1];
       const FitLayout::Tables& nxt = L.tab[(d + 1) & 1];
       const int32_t* perm_d = d == 0 ? I(L.perm_root) : I(L.perm_pp[d & 1]);
@@ -561,8 +565,8 @@ static int enqueue_round(const wgbm_fit_config& c, const FitLayout& L, const wgb
       la.dec_feat = d < D ? I(L.dec_feat) : nullptr;
       la.node_start = I(cur.node_start);
       la.node_count = I(cur.node_count);
-      la.seg_first = I(L.l_seg_first);
-      la.nseg_dev = I(L.l_nseg);
+      la.seg_first = fused ? I(L.p_seg_first) : I(L.l_seg_first);
+      la.nseg_dev = fused ? I(L.p_nseg) : I(L.l_nseg);
       la.multi = I(L.l_multi);
       la.n_multi = I(L.l_n_multi);
       la.perm = perm_d;
@@ -689,9 +693,13 @@ static int enqueue_round(const wgbm_fit_config& c, const FitLayout& L, const wgb
         s.era_g = Fp(L.era_g);
         s.era_d = reinterpret_cast<int8_t*>(ws + L.era_d);
         s.gain_nodes = L.gain_nodes;
-        // decisions, then the leaf and partition plans, in one launch
-        const PlanJob leaf_job = leaf_plan_job(la), part_job = part_plan_job(p);
-        rc = level_scan_launch(s, st, &leaf_job, &part_job);
+        if (fused) {
+          rc = level_scan_launch(s, st, nullptr, nullptr, false);  // the tail picks
+        } else {
+          // decisions, then the leaf and partition plans, in one launch
+          const PlanJob leaf_job = leaf_plan_job(la), part_job = part_plan_job(p);
+          rc = level_scan_launch(s, st, &leaf_job, &part_job);
+        }
         if (rc != kOk) return rc;
       }
       RecordArgs r{};
@@ -713,7 +721,7 @@ static int enqueue_round(const wgbm_fit_config& c, const FitLayout& L, const wgb
       r.value = io->node_value;
       r.count = io->node_count;
       r.out_era_gain = io->node_era_gain;
-      if (d == 0 && D == 0) {  // no decisions ran: the leaf plan of the only level
+      if (d == 0 && D == 0 && !fused) {  // no decisions ran: the leaf plan of the only level
         const PlanJob j = leaf_plan_job(la);
         plan_launch(j.src, j.chunk_rows, j.seg_first, j.multi, j.n_multi, st);
         WGBM_LAUNCHED("k_hist_plan(leaf)");
@@ -736,7 +744,7 @@ static int enqueue_round(const wgbm_fit_config& c, const FitLayout& L, const wgb
         na.seg_first = I(L.h_seg_first);
         na.seg_pfirst = I(L.h_seg_pfirst);
         next = hist_plan_job(na, L.hc, I(L.h_multi), I(L.h_n_multi));
-      } else if (d + 1 == D) {
+      } else if (d + 1 == D && !fused_next) {
         LeafArgs ln = la;
         ln.n_nodes = n_nodes * 2;
         ln.node_active = I(nxt.node_active);
@@ -745,7 +753,24 @@ static int enqueue_round(const wgbm_fit_config& c, const FitLayout& L, const wgb
         ln.node_count = I(nxt.node_count);
         next = leaf_plan_job(ln);
       }
-      rc = level_tail_launch(d < D ? &p : nullptr, la, r, L.KN, st, d < D ? &next : nullptr);
+      if (fused) {
+        TailPick pk{};
+        pk.cand_agree = I(L.cand_agree);
+        pk.cand_crit = Fp(L.cand_crit);
+        pk.cand_idx = I(L.cand_idx);
+        pk.wg_per_node = (int)level_scan_groups(c.n_feat, E, c.n_bins);
+        pk.n_bins = c.n_bins;
+        pk.dec_feat = I(L.dec_feat);
+        pk.dec_bin = I(L.dec_bin);
+        pk.era_gain = Fp(L.era_gain);
+        TailMap m{I(L.p_seg_first), I(L.p_nseg), I(L.p_cseg)};
+        // before a fused last level `next` stays empty: no child statistics, no plan
+        const bool has_next = d + 1 < D || (d + 1 == D && !fused_next);
+        rc = level_tail_fused_launch(d < D ? &p : nullptr, la, r, d < D ? &pk : nullptr, m, L.KN, st,
+                                     d < D && has_next ? &next : nullptr);
+      } else {
+        rc = level_tail_launch(d < D ? &p : nullptr, la, r, L.KN, st, d < D ? &next : nullptr);
+      }
       if (rc != kOk) return rc;
     }
     if (io->train_loss) {  // the round's training loss, after its last score update

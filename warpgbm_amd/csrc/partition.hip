@@ -12,6 +12,7 @@
 #include "wgbm_level.h"
 #include "wgbm_stats.h"
 #include "wgbm_plan.h"
+#include "wgbm_cand.h"
 
 namespace wgbm {
 
@@ -41,16 +42,9 @@ __device__ __forceinline__ int chunk_segment(const PartArgs& a, int nseg, int ch
   return a.chunk_seg ? a.chunk_seg[chunk] : upper_index(a.seg_first, nseg, chunk);
 }
 
-__device__ __forceinline__ void part_count_block(const PartArgs& a, int chunk) {
-  const int nseg = *a.nseg_dev;
-  if (chunk >= a.seg_first[nseg]) return;
-  const int s = chunk_segment(a, nseg, chunk);
-  const int k = chunk - a.seg_first[s];
-  const int j = s / a.n_eras;
-  const int col = split_col(a, j);
-  const int bin = a.dec_bin[j];
-  const int c0 = a.seg_start[s] + k * a.chunk_rows;
-  const int c1 = min(c0 + a.chunk_rows, a.seg_start[s] + max(a.seg_count[s], 0));
+// Go-left flags of positions [c0, c1) of one split segment (column col,
+// threshold bin) and their count in lbase[chunk].
+__device__ __forceinline__ void part_count_rows(const PartArgs& a, int chunk, int c0, int c1, int col, int bin) {
   // U positions per thread at once: their row ids, then their codes, are
   // loaded together (two dependent round trips per U positions).
   constexpr int U = 4;
@@ -76,6 +70,17 @@ __device__ __forceinline__ void part_count_block(const PartArgs& a, int chunk) {
   if ((threadIdx.x & 63) == 0) s_c[threadIdx.x >> 6] = cnt;
   __syncthreads();
   if (threadIdx.x == 0) a.lbase[chunk] = s_c[0] + s_c[1] + s_c[2] + s_c[3];
+}
+
+__device__ __forceinline__ void part_count_block(const PartArgs& a, int chunk) {
+  const int nseg = *a.nseg_dev;
+  if (chunk >= a.seg_first[nseg]) return;
+  const int s = chunk_segment(a, nseg, chunk);
+  const int k = chunk - a.seg_first[s];
+  const int j = s / a.n_eras;
+  const int c0 = a.seg_start[s] + k * a.chunk_rows;
+  const int c1 = min(c0 + a.chunk_rows, a.seg_start[s] + max(a.seg_count[s], 0));
+  part_count_rows(a, chunk, c0, c1, split_col(a, j), a.dec_bin[j]);
 }
 
 __global__ void __launch_bounds__(256)
@@ -597,8 +602,10 @@ int partition_launch(const PartArgs& a, int64_t rows_max, hipStream_t st, bool p
 // Leaf sums over a node's positions: each workgroup of 256 threads takes
 // kLeafChunk consecutive positions; thread t adds positions t, t+256, ...
 // in float64, the workgroup combines thread sums by a fixed tree, and the
-// node adds its chunk sums in chunk order.
-constexpr int kLeafChunk = 4096;
+// node adds its chunk sums in chunk order.  The chunk is the partition's,
+// so that a one-era level tail (which maps chunks to nodes before knowing
+// which nodes are leaves) sums a leaf exactly as the leaf plan does.
+constexpr int kLeafChunk = kPartChunk;
 
 __device__ __forceinline__ void leaf_gh(const LeafArgs& a, int p, double& g, double& h) {
   if (a.gh) {
@@ -610,13 +617,8 @@ __device__ __forceinline__ void leaf_gh(const LeafArgs& a, int p, double& g, dou
   }
 }
 
-__device__ __forceinline__ void leaf_partial_block(const LeafArgs& a, int chunk) {
-  const int nseg = *a.nseg_dev;
-  if (chunk >= a.seg_first[nseg]) return;
-  const int j = upper_index(a.seg_first, nseg, chunk);
-  const int k = chunk - a.seg_first[j];
-  const int c0 = a.node_start[j] + k * kLeafChunk;
-  const int c1 = min(c0 + kLeafChunk, a.node_start[j] + a.node_count[j]);
+// Float64 sums of g and h over positions [c0, c1) into part[2 * chunk ..].
+__device__ __forceinline__ void leaf_partial_rows(const LeafArgs& a, int chunk, int c0, int c1) {
   double sg = 0.0, sh = 0.0;
   for (int p = c0 + threadIdx.x; p < c1; p += 256) {
     double g, h;
@@ -636,6 +638,16 @@ __device__ __forceinline__ void leaf_partial_block(const LeafArgs& a, int chunk)
   if (threadIdx.x == 0) { a.part[2 * (int64_t)chunk] = s_g[0]; a.part[2 * (int64_t)chunk + 1] = s_h[0]; }
 }
 
+__device__ __forceinline__ void leaf_partial_block(const LeafArgs& a, int chunk) {
+  const int nseg = *a.nseg_dev;
+  if (chunk >= a.seg_first[nseg]) return;
+  const int j = upper_index(a.seg_first, nseg, chunk);
+  const int k = chunk - a.seg_first[j];
+  const int c0 = a.node_start[j] + k * kLeafChunk;
+  const int c1 = min(c0 + kLeafChunk, a.node_start[j] + a.node_count[j]);
+  leaf_partial_rows(a, chunk, c0, c1);
+}
+
 __global__ void __launch_bounds__(256)
 k_leaf_partial(LeafArgs a) { leaf_partial_block(a, blockIdx.x); }
 
@@ -653,10 +665,9 @@ __device__ __forceinline__ float leaf_value(const LeafArgs& a, int j, double& sg
 
 // Per chunk: the leaf value, written once per node (chunk 0), and
 // pred[row*K + class] += f32(lr) * leaf for the chunk's rows.
-__device__ __forceinline__ void leaf_apply_block(const LeafArgs& a, int chunk) {
-  const int nseg = *a.nseg_dev;
-  if (chunk >= a.seg_first[nseg]) return;
-  const int j = upper_index(a.seg_first, nseg, chunk);
+// Chunk `chunk` of leaf j: the leaf value, written once per node (chunk 0),
+// and pred[row*K + class] += f32(lr) * leaf for the chunk's rows.
+__device__ __forceinline__ void leaf_apply_node(const LeafArgs& a, int chunk, int j) {
   const int k = chunk - a.seg_first[j];
   __shared__ float s_v;
   if (threadIdx.x == 0) {
@@ -680,6 +691,12 @@ __device__ __forceinline__ void leaf_apply_block(const LeafArgs& a, int chunk) {
     float* d = a.pred + (int64_t)r * a.pred_stride + cls;
     *d = *d + step;
   }
+}
+
+__device__ __forceinline__ void leaf_apply_block(const LeafArgs& a, int chunk) {
+  const int nseg = *a.nseg_dev;
+  if (chunk >= a.seg_first[nseg]) return;
+  leaf_apply_node(a, chunk, upper_index(a.seg_first, nseg, chunk));
 }
 
 __global__ void __launch_bounds__(256)
@@ -813,6 +830,159 @@ int level_tail_launch(const PartArgs* p, const LeafArgs& la, const RecordArgs& r
       hipLaunchKernelGGL(k_child_stats, dim3((unsigned)(2 * p->n_nodes)), dim3(256), 0, st, *p);
       WGBM_LAUNCHED("k_child_stats");
     }
+  }
+  return kOk;
+}
+
+// ----------------------------------------------------------------- one-era level tail, decisions folded in
+// A one-era level's chunks are mapped to nodes before anything is known
+// about the decisions: active node j owns max(1, ceil(count_j / kPartChunk))
+// chunks, in node order (a node without rows owns one empty chunk, so that
+// its decision and record are written).  Every workgroup of the first launch
+// computes that map itself (n_nodes <= kTailMaxNodes), reduces its node's
+// scan candidates to the decision (the same reduction pick_node runs) and
+// then counts go-left rows (split) or sums the leaf's rows (leaf): the
+// separate decision launch and the leaf and partition plans are gone.
+// Leaves use the partition's chunk, so their sums are formed exactly as the
+// leaf plan forms them.
+constexpr int kTailMaxNodes = kScanLdsSeg;
+
+bool level_tail_fusable(int64_t n_nodes, int n_eras, int64_t rows_max) {
+  const int64_t chunks = ceil_div(rows_max, kPartChunk) + n_nodes;
+  return n_eras == 1 && n_nodes >= 1 && n_nodes <= kTailMaxNodes && chunks + 1 <= kScanLdsChunks;
+}
+
+// The chunk map (every thread of a 256-thread workgroup calls it): returns
+// the node owning `chunk` (-1 past the last chunk) and its first chunk.
+// Workgroup 0 stores the first-chunk table and the node count for the later
+// launches of the level.
+__device__ __forceinline__ int tail_map(const LeafArgs& la, const TailMap& m, int chunk, int& first) {
+  __shared__ int s_first[kTailMaxNodes + 1];
+  __shared__ int s_node;
+  const int n = la.n_nodes, t = threadIdx.x;
+  auto chunks_of = [&](int j) -> int {
+    if (j >= n || !la.node_active[j]) return 0;
+    const int c = la.node_count[j];
+    return c <= 0 ? 1 : (c + kPartChunk - 1) / kPartChunk;
+  };
+  const int j0 = 2 * t, j1 = 2 * t + 1;
+  const int n0 = chunks_of(j0), n1 = chunks_of(j1);
+  if (t == 0) s_node = -1;
+  int tot;
+  const int pre = block_exscan(n0 + n1, &tot);  // its barriers order s_node's reset before the writes below
+  if (j0 < n) s_first[j0] = pre;
+  if (j1 < n) s_first[j1] = pre + n0;
+  if (t == 0) s_first[n] = tot;
+  if (n0 > 0 && chunk >= pre && chunk < pre + n0) s_node = j0;
+  if (n1 > 0 && chunk >= pre + n0 && chunk < pre + n0 + n1) s_node = j1;
+  __syncthreads();
+  if (blockIdx.x == 0) {
+    for (int j = t; j <= n; j += blockDim.x) m.seg_first[j] = s_first[j];
+    if (t == 0) *m.nseg = n;
+  }
+  const int j = s_node;
+  first = j >= 0 ? s_first[j] : 0;
+  return j;
+}
+
+// splits: 1 before the last level (decisions from the scan candidates, go-left
+// counts of split nodes), 0 at the last level (every active node a leaf).
+__global__ void __launch_bounds__(256)
+k_tail_count(PartArgs p, LeafArgs la, TailPick pk, TailMap m, int splits) {
+  const int chunk = blockIdx.x;
+  int first;
+  const int j = tail_map(la, m, chunk, first);
+  if (j < 0) return;
+  const int k = chunk - first;
+  if (threadIdx.x == 0) m.chunk_node[chunk] = j;
+  int fl = -1, bin = -1;
+  if (splits) {
+    __shared__ int s_fb[2];
+    if (threadIdx.x < kWave) {
+      const Cand b = node_best(pk.cand_agree, pk.cand_crit, pk.cand_idx, j, pk.wg_per_node, threadIdx.x);
+      int f_, b_;
+      cand_split(b, pk.n_bins, f_, b_);
+      if (threadIdx.x == 0) {
+        s_fb[0] = f_;
+        s_fb[1] = b_;
+        if (k == 0) {  // one workgroup per node writes the decision
+          pk.dec_feat[j] = f_;
+          pk.dec_bin[j] = b_;
+          if (pk.era_gain) pk.era_gain[j] = f_ >= 0 ? b.crit : 0.0f;  // one era: the winner's gain
+        }
+      }
+    }
+    __syncthreads();
+    fl = s_fb[0];
+    bin = s_fb[1];
+  }
+  const int c0 = la.node_start[j] + k * kPartChunk;
+  const int c1 = min(c0 + kPartChunk, la.node_start[j] + max(la.node_count[j], 0));
+  if (fl >= 0) {
+    part_count_rows(p, chunk, c0, c1, p.feat_idx ? p.feat_idx[fl] : fl, bin);
+  } else {
+    leaf_partial_rows(la, chunk, c0, c1);
+    if (splits && threadIdx.x == 0) p.lbase[chunk] = 0;  // a leaf's chunks move no rows
+  }
+}
+
+// Scatter of split nodes' chunks, leaf score updates, then the tree record
+// (workgroups n_chunks ...).
+__global__ void __launch_bounds__(256)
+k_tail_move(PartArgs p, LeafArgs la, RecordArgs r, TailMap m, int n_chunks, int splits) {
+  const int b = blockIdx.x;
+  if (b < n_chunks) {
+    const int j = m.chunk_node[b];
+    if (b >= m.seg_first[la.n_nodes]) return;
+    if (splits && p.dec_feat[j] >= 0) part_scatter_block(p, b);
+    else leaf_apply_node(la, b, j);
+  } else {
+    record_thread(r, la, (int64_t)(b - n_chunks) * 256 + threadIdx.x);
+  }
+}
+
+int level_tail_fused_launch(const PartArgs* p, const LeafArgs& la, const RecordArgs& r, const TailPick* pick,
+                            const TailMap& m, int64_t rows_max, hipStream_t st, const PlanJob* next) {
+  WGBM_REQUIRE(level_tail_fusable(la.n_nodes, r.E, rows_max), "level tail: not a one-era level of at most %d nodes",
+               kTailMaxNodes);
+  WGBM_REQUIRE(m.seg_first && m.nseg && m.chunk_node && la.seg_first == m.seg_first && la.nseg_dev == m.nseg,
+               "level tail: the leaf tables must be the chunk map's");
+  WGBM_REQUIRE(la.node_active && la.node_start && la.node_count && la.perm && (la.gh || (la.g && la.h)) &&
+                   la.leaf_val && la.part && r.active && r.node_count && r.round_ctr && r.status && r.feat &&
+                   r.bin && r.value && r.count,
+               "level tail: missing leaf or record pointer");
+  if (p) {
+    WGBM_REQUIRE(pick && pick->cand_agree && pick->cand_crit && pick->cand_idx && pick->wg_per_node >= 1 &&
+                     pick->dec_feat == p->dec_feat && pick->dec_bin == p->dec_bin && r.dec_feat == p->dec_feat &&
+                     r.dec_bin == p->dec_bin && la.dec_feat == p->dec_feat,
+                 "level tail: the decisions must go where the partition and the record read them");
+    WGBM_REQUIRE(p->n_eras == 1 && p->n_nodes == la.n_nodes && p->chunk_rows == kPartChunk &&
+                     p->seg_first == m.seg_first && p->nseg_dev == m.nseg && p->chunk_seg == m.chunk_node &&
+                     p->codes && p->perm_in && p->perm_out && p->flag && p->lbase && p->rbase &&
+                     p->c_seg_start && p->c_seg_count && p->c_active && p->c_node_start && p->c_node_count &&
+                     p->b_seg_start && p->b_seg_count && p->b_seg_slot && p->n_build && p->sib && p->n_split &&
+                     p->c_stats && p->stat_part,
+                 "level tail: missing partition table or scratch pointer");
+  }
+  const int64_t n_chunks = ceil_div(rows_max, kPartChunk) + la.n_nodes;
+  const int64_t n_rec = ceil_div((int64_t)r.n_nodes * r.E, 256);
+  hipLaunchKernelGGL(k_tail_count, dim3((unsigned)n_chunks), dim3(256), 0, st, p ? *p : PartArgs{}, la,
+                     pick ? *pick : TailPick{}, m, p ? 1 : 0);
+  WGBM_LAUNCHED("k_tail_count");
+  if (p) {
+    const int rc = part_scan_launch(*p, n_chunks, st);  // the one-workgroup LDS scan (level_tail_fusable)
+    if (rc != kOk) return rc;
+  }
+  hipLaunchKernelGGL(k_tail_move, dim3((unsigned)(n_chunks + n_rec)), dim3(256), 0, st, p ? *p : PartArgs{}, la, r,
+                     m, (int)n_chunks, p ? 1 : 0);
+  WGBM_LAUNCHED("k_tail_move");
+  // Child statistics and the next level's histogram plan; none before a
+  // last level that is itself fused (no histogram follows, and its chunk
+  // map needs no plan).
+  if (p && next) {
+    hipLaunchKernelGGL(k_child_stats_plan, dim3((unsigned)(2 * p->n_nodes + 1)), dim3(kPlanThreads), 0, st, *p,
+                       *next);
+    WGBM_LAUNCHED("k_child_stats_plan");
   }
   return kOk;
 }

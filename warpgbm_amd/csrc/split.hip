@@ -16,23 +16,10 @@
 #include "common.h"
 #include "wgbm_internal.h"
 #include "wgbm_plan.h"
+#include "wgbm_cand.h"
 #include <math.h>
 
 namespace wgbm {
-
-struct Cand {
-  int agree;    // |sum over eras of direction| (0 when E == 1)
-  float crit;   // -inf when the candidate is masked out
-  int idx;      // flat index f_local*(B-1) + b, INT_MAX for "none"
-};
-
-__device__ __forceinline__ bool cand_better(const Cand& a, const Cand& b) {
-  if (a.agree != b.agree) return a.agree > b.agree;
-  if (a.crit != b.crit) return a.crit > b.crit;
-  return a.idx < b.idx;
-}
-
-__device__ __forceinline__ Cand cand_none() { return Cand{-1, -INFINITY, 0x7fffffff}; }
 
 __device__ __forceinline__ void gain_dir(double GL, double HL, double Gt, double Ht, double mcw,
                                          double lam, float& gain, float& dir) {
@@ -48,17 +35,6 @@ __device__ __forceinline__ void gain_dir(double GL, double HL, double Gt, double
     gain = 0.0f;
     dir = 0.0f;
   }
-}
-
-__device__ __forceinline__ Cand wave_best(Cand c) {
-  for (int o = 32; o >= 1; o >>= 1) {
-    Cand d;
-    d.agree = __shfl_xor(c.agree, o);
-    d.crit = __shfl_xor(c.crit, o);
-    d.idx = __shfl_xor(c.idx, o);
-    if (cand_better(d, c)) c = d;
-  }
-  return c;
 }
 
 // Reference op: per-era gain and direction, thread per (era, feature).
@@ -226,28 +202,10 @@ k_level_scan(const LevelScanArgs a) {
 // Winner of one node, then (optionally) its per-era gains; run by one wave
 // (lane = 0..63).
 __device__ void pick_node(const LevelScanArgs& a, int node, int wg_per_node, int lane) {
-  Cand b = cand_none();
-  if (a.node_active[node]) {
-    // Four candidates' loads per lane in flight, then the comparisons.
-    constexpr int U = 4;
-    for (int w0 = lane; w0 < wg_per_node; w0 += 64 * U) {
-      Cand k[U];
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const int64_t o = (int64_t)node * wg_per_node + min(w0 + 64 * u, wg_per_node - 1);
-        k[u] = Cand{a.cand_agree[o], a.cand_crit[o], a.cand_idx[o]};
-      }
-#pragma unroll
-      for (int u = 0; u < U; ++u)
-        if (w0 + 64 * u < wg_per_node && cand_better(k[u], b)) b = k[u];
-    }
-  }
-  b = wave_best(b);
-  int fl = -1, bin = -1;
-  if (b.idx != 0x7fffffff && b.crit != -INFINITY) {
-    fl = b.idx / (a.n_bins - 1);
-    bin = b.idx % (a.n_bins - 1);
-  }
+  const Cand b = a.node_active[node] ? node_best(a.cand_agree, a.cand_crit, a.cand_idx, node, wg_per_node, lane)
+                                     : cand_none();
+  int fl, bin;
+  cand_split(b, a.n_bins, fl, bin);
   if (lane == 0) {
     a.dec_feat[node] = fl;
     a.dec_bin[node] = bin;
@@ -522,13 +480,14 @@ k_level_gainsel(const LevelScanArgs a, int groups) {
   }
 }
 
-int level_scan_launch(const LevelScanArgs& a, hipStream_t st, const PlanJob* leaf, const PlanJob* part) {
+int level_scan_launch(const LevelScanArgs& a, hipStream_t st, const PlanJob* leaf, const PlanJob* part, bool pick) {
   if (a.n_nodes == 0) return kOk;
   if (a.n_eras == 1) {
     const int wg_per_node = (int)level_scan_groups(a.n_feat, 1, a.n_bins);
     hipLaunchKernelGGL(k_level_scan, dim3((unsigned)(a.n_nodes * (int64_t)wg_per_node)), dim3(64 * kScanFeat), 0,
                        st, a);
     WGBM_LAUNCHED("k_level_scan");
+    if (!pick) return kOk;  // the level tail reduces the candidates (level_tail_fused_launch)
   } else if (a.n_bins >= 2 && a.n_bins <= 8 && a.n_nodes >= kGainSelMinNodes) {
     // many nodes: the fused selection; fewer: per-era gains, then selection
     // (200 eras x 400 features: 179-205 us against ~290 at 128 nodes, about

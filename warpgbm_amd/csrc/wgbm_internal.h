@@ -127,9 +127,10 @@ struct LevelScanArgs {
 struct PlanJob;
 // leaf / part: when both are given, the level's leaf and partition plans run
 // after the per-node decisions (in the same single-workgroup launch for one
-// era and few nodes, else in launches of their own).
+// era and few nodes, else in launches of their own).  pick == false (one era
+// only): launch the scan alone and leave the candidates for the level tail.
 int level_scan_launch(const LevelScanArgs& a, hipStream_t st, const PlanJob* leaf = nullptr,
-                      const PlanJob* part = nullptr);
+                      const PlanJob* part = nullptr, bool pick = true);
 int64_t level_scan_groups(int n_feat, int n_eras, int n_bins);  // candidate slots per node
 // Nodes per pass of the E > 1 scan for a gain buffer of at most cap bytes.
 int64_t level_scan_gain_nodes(int64_t n_nodes, int n_eras, int n_feat, int n_bins, size_t cap);

@@ -109,6 +109,34 @@ struct RecordArgs {
 int level_tail_launch(const PartArgs* p, const LeafArgs& la, const RecordArgs& r, int64_t rows_max, hipStream_t st,
                       const PlanJob* next);
 
+// One-era level tail with the decisions folded in (level_tail_fusable):
+// the scan's per-workgroup candidates and where the decisions go ...
+struct TailPick {
+  const int32_t* cand_agree;
+  const float* cand_crit;
+  const int32_t* cand_idx;
+  int32_t wg_per_node;          // candidates per node (level_scan_groups)
+  int32_t n_bins;
+  int32_t* dec_feat;            // [n_nodes] local feature or -1
+  int32_t* dec_bin;
+  float* era_gain;              // [n_nodes] the winner's gain, or nullptr
+};
+// ... and the level's chunk map, written by its first launch (the leaf
+// tables and, before the last level, the partition's plan tables must be
+// these arrays).
+struct TailMap {
+  int32_t* seg_first;           // [n_nodes + 1] first chunk of each node
+  int32_t* nseg;                // scalar: n_nodes
+  int32_t* chunk_node;          // [chunks] node of each chunk
+};
+bool level_tail_fusable(int64_t n_nodes, int n_eras, int64_t rows_max);
+// Three launches (decisions with counts or leaf sums; the partition scan;
+// scatter with leaf updates and the record), then child statistics with
+// `next` (the next level's histogram plan) when given.  p == nullptr at the
+// last level.
+int level_tail_fused_launch(const PartArgs* p, const LeafArgs& la, const RecordArgs& r, const TailPick* pick,
+                            const TailMap& m, int64_t rows_max, hipStream_t st, const PlanJob* next);
+
 // planned: the plan ran in an earlier fused launch (leaf_plan_job /
 // part_plan_job beside the level's decisions).  next_hist: run the next level's histogram
 // plan in the same launch as the child statistics.
