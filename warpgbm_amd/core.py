@@ -64,6 +64,31 @@ class _Workspace:
         return b
 
 
+# The batched engine's fit workspace (the column-major code copy, level
+# histograms, tables) is scratch for the duration of one wgbm_fit_rounds call:
+# the call returns when its rounds are done.  One buffer per (device, stream)
+# therefore serves every model in turn, so a model fitted while an earlier one
+# is still alive reuses the buffer instead of allocating (and first touching)
+# gigabytes of its own, and no model keeps it after its fit.  Fits on one
+# device and stream run one after another.
+_FIT_WORKSPACES: dict = {}
+
+
+def _fit_ws_key(device):
+    d = torch.device(device)
+    return (d.index if d.index is not None else torch.cuda.current_device(), _stream(d))
+
+
+def _fit_workspace(device, nbytes: int) -> torch.Tensor:
+    key = _fit_ws_key(device)
+    b = _FIT_WORKSPACES.get(key)
+    if b is None or b.numel() < nbytes:
+        _FIT_WORKSPACES.pop(key, None)  # the old buffer goes back to the allocator first
+        b = torch.empty(max(int(nbytes), 256), dtype=torch.uint8, device=device)
+        _FIT_WORKSPACES[key] = b
+    return b
+
+
 _STAGE_BYTES = 64 << 20  # pinned staging chunk for large host-to-device copies
 
 
@@ -823,6 +848,9 @@ class WarpGBM(BaseEstimator, RegressorMixin):
         dev = torch.device(self.device)
         if dev.type == "cuda" and torch.cuda.is_available():
             free, _ = torch.cuda.mem_get_info(dev)
+            # the device's shared fit workspace is reused (or replaced) by this fit
+            idx = dev.index if dev.index is not None else torch.cuda.current_device()
+            free += max((b.numel() for (i, _), b in _FIT_WORKSPACES.items() if i == idx), default=0)
             if need > self._BATCHED_MEM_SHARE * free:
                 print(f"Batched engine needs {need / 2**30:.1f} GiB of workspace "
                       f"({free / 2**30:.1f} GiB free): training per node instead.")
@@ -1080,7 +1108,7 @@ class WarpGBM(BaseEstimator, RegressorMixin):
         if self.autotune:
             from . import autotune
             self.hist_config_ = autotune.tune_fit(self, cfg, k)
-        ws = self._ws.get("fit", _lib.query("wgbm_fit_workspace_bytes", ctypes.byref(cfg)), self.device)
+        ws = _fit_workspace(self.device, _lib.query("wgbm_fit_workspace_bytes", ctypes.byref(cfg)))
         every = self.eval_every_n_trees
         if every is not None:
             # raw scores of the eval rows under the trees so far (warm start
@@ -1159,7 +1187,7 @@ class WarpGBM(BaseEstimator, RegressorMixin):
     def _trial_rounds_ms(self, cfg, n_rounds, feats_gen=None):
         """Histogram time (build + merge launches, ms) of n_rounds boosting
         rounds run on a copy of the scores: the model is left as it was."""
-        ws = self._ws.get("fit", _lib.query("wgbm_fit_workspace_bytes", ctypes.byref(cfg)), self.device)
+        ws = _fit_workspace(self.device, _lib.query("wgbm_fit_workspace_bytes", ctypes.byref(cfg)))
         feats = None
         if cfg.n_feat < self.num_features:
             feats = torch.stack([torch.randperm(self.num_features, generator=feats_gen, device=self.device,

@@ -328,14 +328,33 @@ k_hist_build(const HistArgs a) {
     constexpr int NG = kWave / LG;  // groups per 64-position block (even)
     const uint32_t colo = lane_on ? (uint32_t)fl0 : 0u;
     auto load_codes = [&](int row) -> TV {
-      const int8_t* rp = codes + (int64_t)max(row, 0) * stride;
+      // The row's address is uniform: formed in scalar registers (a 64-bit
+      // product on the scalar unit), so the load takes it as its scalar base
+      // and the lane's column offset as its vector offset -- no vector
+      // instruction per row for the address.
+      const uint64_t ra = (uint64_t)(uintptr_t)codes + (uint64_t)(uint32_t)max(row, 0) * (uint64_t)stride;
+      const int8_t* rp = reinterpret_cast<const int8_t*>(
+          ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(ra >> 32)) << 32) |
+          (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)ra));
       TV v;
       if constexpr (ID) {
-        v = *reinterpret_cast<const TV*>(rp + colo);
+        // global (not generic) address space: a global load with a scalar
+        // base, not a flat load (which would also count as an LDS access)
+        const uintptr_t a16 = reinterpret_cast<uintptr_t>(rp) + colo;
+        if constexpr (FPL == 16) {
+          using G4 = __attribute__((address_space(1))) const uint32_t;
+          G4* p4 = reinterpret_cast<G4*>(a16);
+          v = make_uint4(p4[0], p4[1], p4[2], p4[3]);
+        } else {
+          using GTV = __attribute__((address_space(1))) const TV;
+          v = *reinterpret_cast<GTV*>(a16);
+        }
       } else {
+        using G8 = __attribute__((address_space(1))) const uint8_t;
+        G8* gp = reinterpret_cast<G8*>(reinterpret_cast<uintptr_t>(rp));
         uint8_t tmp[FPL];
 #pragma unroll
-        for (int j = 0; j < FPL; ++j) tmp[j] = (uint8_t)rp[cols[j]];
+        for (int j = 0; j < FPL; ++j) tmp[j] = gp[cols[j]];
         __builtin_memcpy(&v, tmp, sizeof(TV));
       }
       return v;
@@ -412,10 +431,11 @@ k_hist_build(const HistArgs a) {
       const int cnt = min(kWave, w1 - base);
       auto add_group = [&](const TV* V, int g0) {
         if (g0 >= cnt) return;  // uniform: a partial last block
+        // No per-row check: a position past the wave's range carries a zero
+        // word (and row 0's codes), so its adds change nothing.
 #pragma unroll
         for (int q = 0; q < LG; ++q) {
           const int r = g0 + q;
-          if (r >= cnt) break;  // uniform: no adds for positions past the wave's range
           const unsigned long long qp =
               ((unsigned long long)(uint32_t)__builtin_amdgcn_readlane((int)(myqp >> 32), r) << 32) |
               (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)myqp, r);

@@ -652,19 +652,32 @@ __global__ void __launch_bounds__(256)
 k_leaf_partial(LeafArgs a) { leaf_partial_block(a, blockIdx.x); }
 
 // Leaf j's total from its chunk sums (chunk order, float64) and its value
-// f32(sum g / (sum h + lambda)).
+// f32(sum g / (sum h + lambda)).  The chunk sums are loaded U at a time (one
+// round trip per U chunks, not one per chunk), then added in chunk order.
 __device__ __forceinline__ float leaf_value(const LeafArgs& a, int j, double& sg, double& sh) {
   sg = 0.0;
   sh = 0.0;
-  for (int c = a.seg_first[j]; c < a.seg_first[j + 1]; ++c) {
-    sg += a.part[2 * (int64_t)c];
-    sh += a.part[2 * (int64_t)c + 1];
+  const int first = a.seg_first[j], last = a.seg_first[j + 1];
+  constexpr int U = 16;
+  for (int c0 = first; c0 < last; c0 += U) {
+    double vg[U], vh[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t c = min(c0 + u, last - 1);
+      vg[u] = a.part[2 * c];
+      vh[u] = a.part[2 * c + 1];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (c0 + u < last) {
+        sg += vg[u];
+        sh += vh[u];
+      }
+    }
   }
   return (float)(sg / (sh + a.lam));
 }
 
-// Per chunk: the leaf value, written once per node (chunk 0), and
-// pred[row*K + class] += f32(lr) * leaf for the chunk's rows.
 // Chunk `chunk` of leaf j: the leaf value, written once per node (chunk 0),
 // and pred[row*K + class] += f32(lr) * leaf for the chunk's rows.
 __device__ __forceinline__ void leaf_apply_node(const LeafArgs& a, int chunk, int j) {
