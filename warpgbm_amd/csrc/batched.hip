@@ -137,7 +137,7 @@ static FitLayout fit_layout(const wgbm_fit_config& c) {
   L.l_nseg = take(4);
   L.l_multi = take(4 * (L.leaf_chunks + 1));
   L.l_n_multi = take(4);
-  L.l_part = take(8 * 2 * L.leaf_chunks);
+  L.l_part = take(sizeof(I128) * 2 * L.leaf_chunks);
   L.l_val = take(4 * L.max_nodes);
   L.root_seg_start = take(4 * K * E);
   L.root_seg_count = take(4 * K * E);
@@ -579,7 +579,8 @@ static int enqueue_round(const wgbm_fit_config& c, const FitLayout& L, const wgb
       la.nodes_per_class = 1 << d;
       la.leaf_val = Fp(L.l_val);
       la.node_sum = nullptr;
-      la.part = reinterpret_cast<double*>(ws + L.l_part);
+      la.part = reinterpret_cast<I128*>(ws + L.l_part);
+      la.stats = I(cur.stats);
       PartArgs p{};
       if (d < D) {
         p.codes = io->codes;
@@ -764,7 +765,7 @@ static int enqueue_round(const wgbm_fit_config& c, const FitLayout& L, const wgb
         pk.dec_bin = I(L.dec_bin);
         pk.era_gain = Fp(L.era_gain);
         TailMap m{I(L.p_seg_first), I(L.p_nseg), I(L.p_cseg)};
-        // before a fused last level `next` stays empty: no child statistics, no plan
+        // before a fused last level `next` stays empty: child statistics only, no plan
         const bool has_next = d + 1 < D || (d + 1 == D && !fused_next);
         rc = level_tail_fused_launch(d < D ? &p : nullptr, la, r, d < D ? &pk : nullptr, m, L.KN, st,
                                      d < D && has_next ? &next : nullptr);

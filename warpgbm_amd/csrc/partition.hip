@@ -599,13 +599,26 @@ int partition_launch(const PartArgs& a, int64_t rows_max, hipStream_t st, bool p
 }
 
 // ----------------------------------------------------------------- leaves
-// Leaf sums over a node's positions: each workgroup of 256 threads takes
-// kLeafChunk consecutive positions; thread t adds positions t, t+256, ...
-// in float64, the workgroup combines thread sums by a fixed tree, and the
-// node adds its chunk sums in chunk order.  The chunk is the partition's,
-// so that a one-era level tail (which maps chunks to nodes before knowing
-// which nodes are leaves) sums a leaf exactly as the leaf plan does.
+// Leaf sums over a node's positions, exact and independent of the order of
+// the rows: each row's g and h become 62-bit fixed-point integers,
+// round(g * 2^sg) and round(h * 2^sh), with the shifts taken from the
+// leaf's max |g| and max |h| (its statistics), and the integers are summed
+// in 128 bits.  Each workgroup of 256 threads takes kLeafChunk consecutive
+// positions; the node adds its chunk sums, converts each total to float64
+// once and forms f32(G / (H + lambda)).  A row's rounding error is at most
+// 2^-63 of the leaf's largest |g| (|h|): below float64 summation's.
 constexpr int kLeafChunk = kPartChunk;
+constexpr int kLeafBits = 62;
+
+struct LeafShift {
+  int sg, sh;
+};
+
+__device__ __forceinline__ LeafShift leaf_shift(const LeafArgs& a, int j) {
+  const int32_t* st = a.stats + (int64_t)j * kStatWords;
+  return LeafShift{fixed_shift(__builtin_bit_cast(float, st[0]), kLeafBits),
+                   fixed_shift(__builtin_bit_cast(float, st[3]), kLeafBits)};
+}
 
 __device__ __forceinline__ void leaf_gh(const LeafArgs& a, int p, double& g, double& h) {
   if (a.gh) {
@@ -617,25 +630,44 @@ __device__ __forceinline__ void leaf_gh(const LeafArgs& a, int p, double& g, dou
   }
 }
 
-// Float64 sums of g and h over positions [c0, c1) into part[2 * chunk ..].
-__device__ __forceinline__ void leaf_partial_rows(const LeafArgs& a, int chunk, int c0, int c1) {
-  double sg = 0.0, sh = 0.0;
+// Exact fixed-point sums of g and h over positions [c0, c1) of leaf j into
+// part[2 * chunk ..].
+__device__ __forceinline__ void leaf_partial_rows(const LeafArgs& a, int chunk, int j, int c0, int c1) {
+  const LeafShift ls = leaf_shift(a, j);
+  I128 sg, sh;
   for (int p = c0 + threadIdx.x; p < c1; p += 256) {
     double g, h;
     leaf_gh(a, p, g, h);
-    sg += g; sh += h;
+    sg.add(to_fixed((float)g, ls.sg));
+    sh.add(to_fixed((float)h, ls.sh));
   }
-  __shared__ double s_g[256], s_h[256];
-  s_g[threadIdx.x] = sg; s_h[threadIdx.x] = sh;
+  // Fixed-tree workgroup reduction in LDS (halves of each 128-bit sum).
+  __shared__ unsigned long long s_lo[2][256];
+  __shared__ long long s_hi[2][256];
+  const int t = threadIdx.x;
+  s_lo[0][t] = sg.lo; s_hi[0][t] = sg.hi;
+  s_lo[1][t] = sh.lo; s_hi[1][t] = sh.hi;
   __syncthreads();
   for (int w = 128; w >= 1; w >>= 1) {
-    if ((int)threadIdx.x < w) {
-      s_g[threadIdx.x] += s_g[threadIdx.x + w];
-      s_h[threadIdx.x] += s_h[threadIdx.x + w];
+    if (t < w) {
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        I128 x, y;
+        x.lo = s_lo[q][t]; x.hi = s_hi[q][t];
+        y.lo = s_lo[q][t + w]; y.hi = s_hi[q][t + w];
+        x.add(y);
+        s_lo[q][t] = x.lo; s_hi[q][t] = x.hi;
+      }
     }
     __syncthreads();
   }
-  if (threadIdx.x == 0) { a.part[2 * (int64_t)chunk] = s_g[0]; a.part[2 * (int64_t)chunk + 1] = s_h[0]; }
+  if (t == 0) {
+    I128 g, h;
+    g.lo = s_lo[0][0]; g.hi = s_hi[0][0];
+    h.lo = s_lo[1][0]; h.hi = s_hi[1][0];
+    a.part[2 * (int64_t)chunk] = g;
+    a.part[2 * (int64_t)chunk + 1] = h;
+  }
 }
 
 __device__ __forceinline__ void leaf_partial_block(const LeafArgs& a, int chunk) {
@@ -645,22 +677,22 @@ __device__ __forceinline__ void leaf_partial_block(const LeafArgs& a, int chunk)
   const int k = chunk - a.seg_first[j];
   const int c0 = a.node_start[j] + k * kLeafChunk;
   const int c1 = min(c0 + kLeafChunk, a.node_start[j] + a.node_count[j]);
-  leaf_partial_rows(a, chunk, c0, c1);
+  leaf_partial_rows(a, chunk, j, c0, c1);
 }
 
 __global__ void __launch_bounds__(256)
 k_leaf_partial(LeafArgs a) { leaf_partial_block(a, blockIdx.x); }
 
-// Leaf j's total from its chunk sums (chunk order, float64) and its value
-// f32(sum g / (sum h + lambda)).  The chunk sums are loaded U at a time (one
-// round trip per U chunks, not one per chunk), then added in chunk order.
+// Leaf j's totals from its chunk sums (exact: any order gives the same
+// integers), each converted to float64 once, and its value
+// f32(G / (H + lambda)).  The chunk sums are loaded U at a time (one round
+// trip per U chunks).
 __device__ __forceinline__ float leaf_value(const LeafArgs& a, int j, double& sg, double& sh) {
-  sg = 0.0;
-  sh = 0.0;
   const int first = a.seg_first[j], last = a.seg_first[j + 1];
-  constexpr int U = 16;
+  I128 tg, th;
+  constexpr int U = 8;
   for (int c0 = first; c0 < last; c0 += U) {
-    double vg[U], vh[U];
+    I128 vg[U], vh[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int64_t c = min(c0 + u, last - 1);
@@ -670,11 +702,14 @@ __device__ __forceinline__ float leaf_value(const LeafArgs& a, int j, double& sg
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       if (c0 + u < last) {
-        sg += vg[u];
-        sh += vh[u];
+        tg.add(vg[u]);
+        th.add(vh[u]);
       }
     }
   }
+  const LeafShift ls = leaf_shift(a, j);
+  sg = ldexp(tg.to_double(), -ls.sg);
+  sh = ldexp(th.to_double(), -ls.sh);
   return (float)(sg / (sh + a.lam));
 }
 
@@ -733,7 +768,7 @@ PlanJob leaf_plan_job(const LeafArgs& a) {
 int leaf_launch(const LeafArgs& a, int64_t rows_max, hipStream_t st, bool planned) {
   if (a.n_nodes == 0) return kOk;
   WGBM_REQUIRE(a.node_active && a.node_start && a.node_count && a.seg_first && a.nseg_dev && a.multi && a.n_multi &&
-                   a.perm && (a.gh || (a.g && a.h)) && a.leaf_val && a.part,
+                   a.perm && (a.gh || (a.g && a.h)) && a.leaf_val && a.part && a.stats,
                "leaf: missing table or scratch pointer");
   if (!planned) {
     const PlanJob j = leaf_plan_job(a);
@@ -934,7 +969,7 @@ k_tail_count(PartArgs p, LeafArgs la, TailPick pk, TailMap m, int splits) {
   if (fl >= 0) {
     part_count_rows(p, chunk, c0, c1, p.feat_idx ? p.feat_idx[fl] : fl, bin);
   } else {
-    leaf_partial_rows(la, chunk, c0, c1);
+    leaf_partial_rows(la, chunk, j, c0, c1);
     if (splits && threadIdx.x == 0) p.lbase[chunk] = 0;  // a leaf's chunks move no rows
   }
 }
@@ -989,13 +1024,16 @@ int level_tail_fused_launch(const PartArgs* p, const LeafArgs& la, const RecordA
   hipLaunchKernelGGL(k_tail_move, dim3((unsigned)(n_chunks + n_rec)), dim3(256), 0, st, p ? *p : PartArgs{}, la, r,
                      m, (int)n_chunks, p ? 1 : 0);
   WGBM_LAUNCHED("k_tail_move");
-  // Child statistics and the next level's histogram plan; none before a
-  // last level that is itself fused (no histogram follows, and its chunk
-  // map needs no plan).
+  // Child statistics with the next level's histogram plan; before a last
+  // level that is itself fused, the statistics alone (they scale its leaves'
+  // sums; its chunk map needs no plan).
   if (p && next) {
     hipLaunchKernelGGL(k_child_stats_plan, dim3((unsigned)(2 * p->n_nodes + 1)), dim3(kPlanThreads), 0, st, *p,
                        *next);
     WGBM_LAUNCHED("k_child_stats_plan");
+  } else if (p) {
+    hipLaunchKernelGGL(k_child_stats, dim3((unsigned)(2 * p->n_nodes)), dim3(256), 0, st, *p);
+    WGBM_LAUNCHED("k_child_stats");
   }
   return kOk;
 }
@@ -1182,12 +1220,14 @@ extern "C" int wgbm_partition_rows(const int8_t* bin_indices, int64_t n_total_ro
 extern "C" size_t wgbm_leaf_workspace_bytes(int64_t n_samples) {
   if (n_samples < 0) return 0;
   const int64_t mc = leaf_max_chunks(n_samples, 1);
-  return (size_t)(64 * 4 + 8 * 2 * mc + 4 * mc) + 1024;
+  // scalars, chunk sums, plan scratch, statistics partials and the leaf's statistics
+  return (size_t)(64 * 4 + sizeof(I128) * 2 * mc + 4 * mc + 4 * kStatWords * (kStatBlocks + 1)) + 1024;
 }
 
 // Leaf value of one node (plain path): sum(g)/(sum(h)+lambda) over its rows
-// in float64 (warpgbm/core.py:1377-1380), written as float32 to *leaf_out,
-// and pred[row*stride + col] += f32(lr) * leaf for every row when pred is set.
+// (warpgbm/core.py:1377-1380; the sums exact in fixed point as the batched
+// engine forms them), written as float32 to *leaf_out, and
+// pred[row*stride + col] += f32(lr) * leaf for every row when pred is set.
 extern "C" int wgbm_leaf_value(const float* grads, const float* hess, const int32_t* rows, int64_t n,
                                int64_t n_total_rows, double lambda, float learning_rate, float* pred,
                                int32_t pred_stride, int32_t pred_col, float* leaf_out, void* workspace,
@@ -1215,12 +1255,22 @@ extern "C" int wgbm_leaf_value(const float* grads, const float* hess, const int3
   a.nodes_per_class = 1;
   a.leaf_val = leaf_out;
   a.node_sum = nullptr;
-  double* part = reinterpret_cast<double*>(reinterpret_cast<char*>(workspace) + 256);
+  I128* part = reinterpret_cast<I128*>(reinterpret_cast<char*>(workspace) + 256);
   a.part = part;
   a.multi = reinterpret_cast<int32_t*>(part + 2 * mc);
+  int32_t* stat_part = a.multi + mc;
+  int32_t* stats = stat_part + (int64_t)kStatWords * kStatBlocks;
+  a.stats = stats;
   if (pred) a.pred = pred + pred_col;
   hipLaunchKernelGGL(k_leaf_single_init, dim3(1), dim3(64), 0, stream, sc, (int)n);
   WGBM_LAUNCHED("k_leaf_single_init");
+  // the leaf's max |g| and max |h| set the scale of its exact sums
+  const int nsb = stats_blocks(n);
+  hipLaunchKernelGGL(k_list_stats, dim3((unsigned)nsb), dim3(256), 0, stream, rows, (int)n, n_total_rows, grads, hess,
+                     stat_part);
+  WGBM_LAUNCHED("k_list_stats");
+  hipLaunchKernelGGL(k_stats_reduce, dim3(1), dim3(256), 0, stream, stat_part, nsb, stats);
+  WGBM_LAUNCHED("k_stats_reduce");
   return leaf_launch(a, n, stream);
 }
 

@@ -1,5 +1,6 @@
 // Level-wise tree growth: partition and leaf argument blocks.
 #pragma once
+#include "wgbm_stats.h"
 #include "common.h"
 #include "wgbm_internal.h"
 
@@ -78,7 +79,8 @@ struct LeafArgs {
   int32_t nodes_per_class;
   float* leaf_val;              // [n_nodes]
   double* node_sum;             // [n_nodes*2] or nullptr
-  double* part;                 // [max chunks * 2]
+  const int32_t* stats;         // [n_nodes][kStatWords]: each leaf's max |g| and max |h| give its sum's scale
+  I128* part;                   // [max chunks * 2]: exact fixed-point sums of g, h per chunk
 };
 
 // Where the tree record of one level goes (flat node arrays of the caller,

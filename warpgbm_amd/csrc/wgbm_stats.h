@@ -169,6 +169,8 @@ struct I128 {
 constexpr int kStatBlocks = 1024;
 int stats_blocks(int64_t n);
 __global__ void k_gh_stats(const float2* __restrict__ gh, int64_t n, int32_t* __restrict__ part);
+__global__ void k_list_stats(const int32_t* __restrict__ rows, int n, int64_t n_rows, const float* __restrict__ g,
+                             const float* __restrict__ h, int32_t* __restrict__ part);
 __global__ void k_stats_reduce(const int32_t* __restrict__ part, int n_part, int32_t* __restrict__ out);
 
 }  // namespace wgbm
