@@ -95,14 +95,25 @@ int wgbm_partition_rows(const int8_t* bin_indices, int64_t n_total_rows, int64_t
                         const int32_t* rows, int64_t n, int32_t feature, int32_t bin, int32_t* out_rows,
                         int32_t* n_left_out, void* workspace, size_t workspace_bytes, hipStream_t stream);
 
-/* Leaf value sum(g)/(sum(h)+lambda) over a node's rows (float64 sums,
- * warpgbm/core.py:1377-1380) to *leaf_out; if pred is not NULL also
- * pred[row*pred_stride + pred_col] += f32(lr) * leaf for each row
- * (warpgbm/core.py:1381-1386). */
+/* Leaf value sum(g)/(sum(h)+lambda) over a node's rows (warpgbm/core.py:
+ * 1377-1380; the sums exact -- each row's g and h as a 62-bit fixed-point
+ * integer summed in 128 bits, so the order of the rows does not matter --
+ * scaled by scale_stats, the 4-int statistics wgbm_row_stats gives for the
+ * leaf's tree, or with NULL by the leaf's own rows) to *leaf_out; if pred is
+ * not NULL also pred[row*pred_stride + pred_col] += f32(lr) * leaf for each
+ * row (warpgbm/core.py:1381-1386). */
 size_t wgbm_leaf_workspace_bytes(int64_t n_samples);
 int wgbm_leaf_value(const float* grads, const float* hess, const int32_t* rows, int64_t n, int64_t n_total_rows,
                     double lambda, float learning_rate, float* pred, int32_t pred_stride, int32_t pred_col,
-                    float* leaf_out, void* workspace, size_t workspace_bytes, hipStream_t stream);
+                    float* leaf_out, const int32_t* scale_stats, void* workspace, size_t workspace_bytes,
+                    hipStream_t stream);
+/* Statistics of a row list -- max |g|, min h, max h, max |h| as
+ * order-preserving ints -- into out_stats (4 device ints); a tree's root rows
+ * give the scale of its leaves' sums (wgbm_leaf_value).  No counterpart in
+ * the reference: the exact leaf sums need it. */
+size_t wgbm_row_stats_workspace_bytes(void);
+int wgbm_row_stats(const float* grads, const float* hess, const int32_t* rows, int64_t n, int64_t n_total_rows,
+                   int32_t* out_stats, void* workspace, size_t workspace_bytes, hipStream_t stream);
 
 /* sibling = parent - child, float32 elementwise (warpgbm/core.py:1445-1455). */
 int wgbm_hist_subtract(const float* parent_g, const float* parent_h, const float* child_g, const float* child_h,

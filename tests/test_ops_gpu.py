@@ -301,7 +301,8 @@ def test_leaf_value_and_score_update(dev):
     pt = _t(pred, dev)
     gt, ht, rt = _t(g, dev), _t(h, dev), _t(rows, dev)
     _lib.call("wgbm_leaf_value", gt.data_ptr(), ht.data_ptr(), rt.data_ptr(), rows.size, N, 1e-6, 0.1,
-              pt.data_ptr(), 1, 0, out.data_ptr(), ws.data_ptr(), ws.numel(), torch.cuda.current_stream().cuda_stream)
+              pt.data_ptr(), 1, 0, out.data_ptr(), None, ws.data_ptr(), ws.numel(),
+              torch.cuda.current_stream().cuda_stream)
     torch.cuda.synchronize()
     ref = O.leaf_value(g, h, rows, 1e-6)
     v = float(out.item())

@@ -54,7 +54,9 @@ SIGNATURES = {
     "wgbm_partition_workspace_bytes": (sz, [i64]),
     "wgbm_partition_rows": (C.c_int, [vp, i64, i64, vp, i64, i32, i32, vp, vp, vp, sz, vp]),
     "wgbm_leaf_workspace_bytes": (sz, [i64]),
-    "wgbm_leaf_value": (C.c_int, [vp, vp, vp, i64, i64, f64, f32, vp, i32, i32, vp, vp, sz, vp]),
+    "wgbm_leaf_value": (C.c_int, [vp, vp, vp, i64, i64, f64, f32, vp, i32, i32, vp, vp, vp, sz, vp]),
+    "wgbm_row_stats_workspace_bytes": (sz, []),
+    "wgbm_row_stats": (C.c_int, [vp, vp, vp, i64, i64, vp, vp, sz, vp]),
     "wgbm_hist_subtract": (C.c_int, [vp, vp, vp, vp, vp, vp, i64, vp]),
     "wgbm_grad_regression": (C.c_int, [vp, vp, vp, i64, vp, vp, vp]),
     "wgbm_grad_softmax": (C.c_int, [vp, vp, vp, i64, i32, vp, vp, vp]),

@@ -886,6 +886,17 @@ class WarpGBM(BaseEstimator, RegressorMixin):
         f, b = out.cpu().tolist()
         return int(f), int(b)
 
+    def _tree_scale(self, grad, hess, out=None):
+        """Statistics of a tree's root rows (device int32[4]): the scale of
+        its leaves' exact sums, as the batched engine scales them."""
+        if out is None:
+            out = torch.empty(4, dtype=torch.int32, device=self.device)
+        ws = self._ws.get("row_stats", _lib.query("wgbm_row_stats_workspace_bytes"), self.device)
+        rows = self.root_node_indices
+        _lib.call("wgbm_row_stats", grad.data_ptr(), hess.data_ptr(), rows.data_ptr(), rows.numel(),
+                  self.num_samples, out.data_ptr(), ws.data_ptr(), ws.numel(), _stream(self.device))
+        return out
+
     def _plain_leaf(self, node_indices, class_k):
         n = node_indices.numel()
         ws = self._ws.get("leaf", _lib.query("wgbm_leaf_workspace_bytes", n), self.device)
@@ -897,12 +908,14 @@ class WarpGBM(BaseEstimator, RegressorMixin):
                 _lib.call("wgbm_leaf_value", self.grad_out[m].data_ptr(), self.hess_out[m].data_ptr(),
                           node_indices.data_ptr(), n, self.num_samples, float(self.L2_reg),
                           float(self.learning_rate), self.gradients.data_ptr(), M, m,
-                          self._leaf_out[m].data_ptr(), ws.data_ptr(), ws.numel(), _stream(self.device))
+                          self._leaf_out[m].data_ptr(), self._scale_out[m].data_ptr(), ws.data_ptr(), ws.numel(),
+                          _stream(self.device))
             return {"leaf_value": self._leaf_out[:M].cpu().tolist(), "samples": int(n)}
         K = self.num_classes if class_k is not None else 1
         _lib.call("wgbm_leaf_value", self.grad.data_ptr(), self.hess.data_ptr(), node_indices.data_ptr(), n,
                   self.num_samples, float(self.L2_reg), float(self.learning_rate), self.gradients.data_ptr(), K,
-                  class_k or 0, self._leaf_out.data_ptr(), ws.data_ptr(), ws.numel(), _stream(self.device))
+                  class_k or 0, self._leaf_out.data_ptr(), self._scale.data_ptr(), ws.data_ptr(), ws.numel(),
+                  _stream(self.device))
         return {"leaf_value": float(self._leaf_out.item()), "samples": int(n)}
 
     def grow_tree(self, gradient_histogram, hessian_histogram, node_indices, depth, class_k=None):
@@ -978,6 +991,7 @@ class WarpGBM(BaseEstimator, RegressorMixin):
                           self.sample_weight_gpu.data_ptr(), self.num_samples, self.grad.data_ptr(),
                           self.hess.data_ptr(), _stream(self.device))
                 self._tree_features(k)
+                self._scale = self._tree_scale(self.grad, self.hess)
                 G, H = self.compute_histograms(self.root_node_indices, self.feat_indices_tree, self.grad, self.hess)
                 self.forest[i] = self.grow_tree(G, H, self.root_node_indices, 0)
                 self._trees_trained = i + 1
@@ -1016,6 +1030,7 @@ class WarpGBM(BaseEstimator, RegressorMixin):
                 for c in range(K):
                     self.grad = g_all[c]
                     self.hess = h_all[c]
+                    self._scale = self._tree_scale(self.grad, self.hess)
                     G, H = self.compute_histograms(self.root_node_indices, self.feat_indices_tree, self.grad,
                                                    self.hess)
                     trees_k.append(self.grow_tree(G, H, self.root_node_indices, 0, class_k=c))
@@ -1054,11 +1069,14 @@ class WarpGBM(BaseEstimator, RegressorMixin):
         self.hess_out = torch.empty_like(self.grad_out)
         self.grad = torch.empty(N, dtype=torch.float32, device=self.device)
         self.hess = torch.empty_like(self.grad)
+        self._scale_out = torch.empty((M, 4), dtype=torch.int32, device=self.device)
         for i in range(start, self.n_estimators):
             _lib.call("wgbm_grad_multi_regression", self.Y_gpu.data_ptr(), self.gradients.data_ptr(),
                       self.sample_weight_gpu.data_ptr(), N, M, self.grad_out.data_ptr(), self.hess_out.data_ptr(),
                       self.grad.data_ptr(), self.hess.data_ptr(), _stream(self.device))
             self._tree_features(k)
+            for m in range(M):  # each output's leaves are scaled by that output's root rows
+                self._tree_scale(self.grad_out[m], self.hess_out[m], out=self._scale_out[m])
             G, H = self.compute_histograms(self.root_node_indices, self.feat_indices_tree, self.grad, self.hess)
             self.forest[i] = self.grow_tree(G, H, self.root_node_indices, 0)
             self._trees_trained = i + 1

@@ -23,6 +23,7 @@
 #include "wgbm_stats.h"
 #include "wgbm_plan.h"
 #include "../../include/warpgbm_amd.h"
+#include <algorithm>
 
 namespace wgbm {
 
@@ -40,6 +41,7 @@ struct FitLayout {
   size_t stat_part, round_ctr, fidx_cur;
   size_t p_flag, p_seg_first, p_nseg, p_multi, p_n_multi, p_lbase, p_rbase, p_cscan, p_segL, p_segP, codes_t, p_cseg;
   size_t l_seg_first, l_nseg, l_multi, l_n_multi, l_part, l_val;
+  size_t tree_stats, curs[2], child_part;
   size_t root_seg_start, root_seg_count, root_seg_slot, root_nseg, era_start;
   // sizes
   int64_t KN, max_nodes, node_elems, hist_nodes, part_chunks, leaf_chunks, hist_part_floats;
@@ -138,6 +140,10 @@ static FitLayout fit_layout(const wgbm_fit_config& c) {
   L.l_multi = take(4 * (L.leaf_chunks + 1));
   L.l_n_multi = take(4);
   L.l_part = take(sizeof(I128) * 2 * L.leaf_chunks);
+  L.tree_stats = take(4 * kStatWords * K);    // per class: its root's statistics (scale of the leaf sums)
+  // split cursors of the fused levels (at most kScanLdsSeg nodes), by level parity
+  for (int i = 0; i < 2; ++i) L.curs[i] = take(8 * kCurStride * (size_t)std::min<int64_t>(L.max_nodes, kScanLdsSegMax));
+  L.child_part = take(sizeof(I128) * 4 * L.part_chunks);  // last split level: children's sums per chunk
   L.l_val = take(4 * L.max_nodes);
   L.root_seg_start = take(4 * K * E);
   L.root_seg_count = take(4 * K * E);
@@ -383,9 +389,14 @@ k_loss_final(const double* __restrict__ part, int nb, const int32_t* __restrict_
 
 // The root statistics of every class (workgroups [0, K)) and the root
 // histogram plan (workgroup K) in one launch of 1024-thread workgroups.
+// The root statistics are also the round's tree statistics (the scale of
+// its leaf sums); the plan workgroup also clears level 0's split cursors.
 __global__ void __launch_bounds__(1024)
-k_root_stats_plan(const int32_t* __restrict__ part, int n_part, int32_t* __restrict__ out, int K, const PlanJob j) {
+k_root_stats_plan(const int32_t* __restrict__ part, int n_part, int32_t* __restrict__ out,
+                  int32_t* __restrict__ tree_out, int K, const PlanJob j, unsigned long long* __restrict__ cur0,
+                  int n_cur) {
   if ((int)blockIdx.x == K) {
+    for (int q = threadIdx.x; q < n_cur; q += blockDim.x) cur0[(int64_t)q * kCurStride] = 0ull;
     plan_body(j.src, j.chunk_rows, j.seg_first, j.seg_pfirst, j.multi, j.n_multi);
     return;
   }
@@ -393,7 +404,85 @@ k_root_stats_plan(const int32_t* __restrict__ part, int n_part, int32_t* __restr
   StatAcc acc;
   for (int i = threadIdx.x; i < n_part; i += blockDim.x) acc.load_merge(p + (int64_t)i * kStatWords);
   acc.block_reduce();
-  if (threadIdx.x == 0) acc.store(out + (int64_t)blockIdx.x * kStatWords);
+  if (threadIdx.x == 0) {
+    acc.store(out + (int64_t)blockIdx.x * kStatWords);
+    acc.store(tree_out + (int64_t)blockIdx.x * kStatWords);
+  }
+}
+
+// The round's trees added to the scores: thread q takes rows q, q + T, ...
+// (T threads per class, four rows walked together so that their code loads
+// are in flight at once), walks class c's tree of round round_ctr[0] from
+// the root through its split nodes (left when code <= bin, as the partition
+// sends rows) to its leaf, and adds f32(lr) * leaf (each row's one update of
+// the round, the value the per-node engine adds).  The tree's nodes are
+// staged in LDS when they fit.
+constexpr int kApplyRows = 4;
+constexpr int kApplyStage = 2047;  // nodes staged (trees of depth <= 10)
+__global__ void __launch_bounds__(256)
+k_apply_tree(const int32_t* __restrict__ status, const int32_t* __restrict__ feat, const int32_t* __restrict__ bin,
+             const float* __restrict__ value, const int32_t* __restrict__ round_ctr, int K, int D,
+             const int8_t* __restrict__ codes_t, int64_t N, float lr, float* __restrict__ scores, int blocks_per_class) {
+  const int c = blockIdx.x / blocks_per_class;
+  const int blk = blockIdx.x - c * blocks_per_class;
+  const int NH = (2 << D) - 1;
+  const int64_t o = ((int64_t)round_ctr[0] * K + c) * NH;
+  __shared__ int s_f[kApplyStage];
+  __shared__ int s_b[kApplyStage];
+  __shared__ float s_v[kApplyStage];
+  const bool staged = NH <= kApplyStage;
+  if (staged) {
+    for (int h = threadIdx.x; h < NH; h += blockDim.x) {
+      const int st = status[o + h];
+      s_f[h] = st == 1 ? feat[o + h] : (st == 2 ? -1 : -2);  // -1 leaf, -2 absent
+      s_b[h] = bin[o + h];
+      s_v[h] = value[o + h];
+    }
+    __syncthreads();
+  }
+  auto node_feat = [&](int h) -> int {
+    if (staged) return s_f[h];
+    const int st = status[o + h];
+    return st == 1 ? feat[o + h] : (st == 2 ? -1 : -2);
+  };
+  const int64_t T = (int64_t)blocks_per_class * blockDim.x;
+  for (int64_t r0 = (int64_t)blk * blockDim.x + threadIdx.x; r0 < N; r0 += T * kApplyRows) {
+    int64_t row[kApplyRows];
+    int h[kApplyRows];
+    bool done[kApplyRows];
+#pragma unroll
+    for (int u = 0; u < kApplyRows; ++u) {
+      row[u] = r0 + u * T;
+      h[u] = 0;
+      done[u] = row[u] >= N;
+    }
+    for (int d = 0; d < D; ++d) {
+      int8_t code[kApplyRows];
+      int f[kApplyRows];
+#pragma unroll
+      for (int u = 0; u < kApplyRows; ++u) {
+        f[u] = done[u] ? -1 : node_feat(h[u]);
+        code[u] = f[u] >= 0 ? codes_t[(int64_t)f[u] * N + row[u]] : (int8_t)0;
+      }
+#pragma unroll
+      for (int u = 0; u < kApplyRows; ++u) {
+        if (f[u] >= 0) {
+          const int b = staged ? s_b[h[u]] : bin[o + h[u]];
+          h[u] = 2 * h[u] + 1 + ((int)code[u] <= b ? 0 : 1);
+        } else {
+          done[u] = true;
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kApplyRows; ++u) {
+      if (row[u] < N && node_feat(h[u]) == -1) {
+        const float v = staged ? s_v[h[u]] : value[o + h[u]];
+        float* d = scores + row[u] * K + c;
+        *d = *d + lr * v;
+      }
+    }
+  }
 }
 
 }  // namespace wgbm
@@ -545,14 +634,16 @@ static int enqueue_round(const wgbm_fit_config& c, const FitLayout& L, const wgb
       root_plan = hist_plan_job(ra_h, L.hc, I(L.h_multi), I(L.h_n_multi));
     }
     hipLaunchKernelGGL(k_root_stats_plan, dim3((unsigned)(K + (D > 0 ? 1 : 0))), dim3(kPlanThreads), 0, st, I(L.stat_part), nsb,
-                       I(L.tab[0].stats), K, root_plan);
+                       I(L.tab[0].stats), I(L.tree_stats), K, root_plan,
+                       reinterpret_cast<unsigned long long*>(ws + L.curs[0]), std::min(K, kScanLdsSegMax));
     WGBM_LAUNCHED("k_root_stats_plan");
     for (int d = 0; d <= D; ++d) {
       const int n_nodes = K << d;
       // One era and few nodes: the level's decisions fold into its tail
       // (no decision launch, no leaf or partition plan).
-      const bool fused = level_tail_fusable(n_nodes, E, L.KN);
-      const bool fused_next = d < D && level_tail_fusable((int64_t)n_nodes * 2, E, L.KN);
+      const bool fused = d < D && level_tail_fusable(n_nodes, E, L.KN);
+      // after a fused last split level the leaves are recorded: nothing is left to do
+      if (d == D && d > 0 && level_tail_fusable(n_nodes / 2, E, L.KN)) break;
       const FitLayout::Tables& cur = L.tab[d & 1];
       const FitLayout::Tables& nxt = L.tab[(d + 1) & 1];
       const int32_t* perm_d = d == 0 ? I(L.perm_root) : I(L.perm_pp[d & 1]);
@@ -574,13 +665,13 @@ static int enqueue_round(const wgbm_fit_config& c, const FitLayout& L, const wgb
       la.n_rows = N;
       la.lam = c.lambda;
       la.lr = c.learning_rate;
-      la.pred = io->scores;
+      la.pred = nullptr;  // leaves are recorded; k_apply_tree adds the round's trees to the scores
       la.pred_stride = K;
       la.nodes_per_class = 1 << d;
       la.leaf_val = Fp(L.l_val);
       la.node_sum = nullptr;
       la.part = reinterpret_cast<I128*>(ws + L.l_part);
-      la.stats = I(cur.stats);
+      la.stats = I(L.tree_stats);
       PartArgs p{};
       if (d < D) {
         p.codes = io->codes;
@@ -745,7 +836,7 @@ static int enqueue_round(const wgbm_fit_config& c, const FitLayout& L, const wgb
         na.seg_first = I(L.h_seg_first);
         na.seg_pfirst = I(L.h_seg_pfirst);
         next = hist_plan_job(na, L.hc, I(L.h_multi), I(L.h_n_multi));
-      } else if (d + 1 == D && !fused_next) {
+      } else if (d + 1 == D && !fused) {
         LeafArgs ln = la;
         ln.n_nodes = n_nodes * 2;
         ln.node_active = I(nxt.node_active);
@@ -765,14 +856,22 @@ static int enqueue_round(const wgbm_fit_config& c, const FitLayout& L, const wgb
         pk.dec_bin = I(L.dec_bin);
         pk.era_gain = Fp(L.era_gain);
         TailMap m{I(L.p_seg_first), I(L.p_nseg), I(L.p_cseg)};
-        // before a fused last level `next` stays empty: child statistics only, no plan
-        const bool has_next = d + 1 < D || (d + 1 == D && !fused_next);
-        rc = level_tail_fused_launch(d < D ? &p : nullptr, la, r, d < D ? &pk : nullptr, m, L.KN, st,
-                                     d < D && has_next ? &next : nullptr);
+        TailCursors tc{reinterpret_cast<unsigned long long*>(ws + L.curs[d & 1]),
+                       reinterpret_cast<unsigned long long*>(ws + L.curs[(d + 1) & 1]),
+                       reinterpret_cast<I128*>(ws + L.child_part)};
+        const bool last = d + 1 == D;
+        rc = level_fused_launch(p, la, r, pk, m, tc, last, L.KN, st, last ? nullptr : &next);
       } else {
         rc = level_tail_launch(d < D ? &p : nullptr, la, r, L.KN, st, d < D ? &next : nullptr);
       }
       if (rc != kOk) return rc;
+    }
+    {  // the round's trees, now recorded, added to the scores
+      const int bpc = (int)std::min<int64_t>(ceil_div(N, 256 * kApplyRows), 4096);
+      hipLaunchKernelGGL(k_apply_tree, dim3((unsigned)(K * bpc)), dim3(256), 0, st, io->node_status, io->node_feat,
+                         io->node_bin, io->node_value, I(L.round_ctr), K, D,
+                         reinterpret_cast<const int8_t*>(ws + L.codes_t), N, c.learning_rate, io->scores, bpc);
+      WGBM_LAUNCHED("k_apply_tree");
     }
     if (io->train_loss) {  // the round's training loss, after its last score update
       hipLaunchKernelGGL(k_loss_part, dim3((unsigned)nsb), dim3(256), 0, st, io->y, c.objective == 1 ? io->labels : nullptr,

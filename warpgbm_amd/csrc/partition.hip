@@ -601,12 +601,12 @@ int partition_launch(const PartArgs& a, int64_t rows_max, hipStream_t st, bool p
 // ----------------------------------------------------------------- leaves
 // Leaf sums over a node's positions, exact and independent of the order of
 // the rows: each row's g and h become 62-bit fixed-point integers,
-// round(g * 2^sg) and round(h * 2^sh), with the shifts taken from the
-// leaf's max |g| and max |h| (its statistics), and the integers are summed
-// in 128 bits.  Each workgroup of 256 threads takes kLeafChunk consecutive
+// round(g * 2^sg) and round(h * 2^sh), with the shifts taken from the max |g|
+// and max |h| of the leaf's tree (its root's statistics), and the integers
+// are summed in 128 bits.  Each workgroup of 256 threads takes kLeafChunk consecutive
 // positions; the node adds its chunk sums, converts each total to float64
 // once and forms f32(G / (H + lambda)).  A row's rounding error is at most
-// 2^-63 of the leaf's largest |g| (|h|): below float64 summation's.
+// 2^-63 of the tree's largest |g| (|h|).
 constexpr int kLeafChunk = kPartChunk;
 constexpr int kLeafBits = 62;
 
@@ -614,8 +614,10 @@ struct LeafShift {
   int sg, sh;
 };
 
+// The scale of leaf j's sums: the statistics of its tree (class j /
+// nodes_per_class), the same for every leaf of the tree in both engines.
 __device__ __forceinline__ LeafShift leaf_shift(const LeafArgs& a, int j) {
-  const int32_t* st = a.stats + (int64_t)j * kStatWords;
+  const int32_t* st = a.stats + (int64_t)(j / a.nodes_per_class) * kStatWords;
   return LeafShift{fixed_shift(__builtin_bit_cast(float, st[0]), kLeafBits),
                    fixed_shift(__builtin_bit_cast(float, st[3]), kLeafBits)};
 }
@@ -894,6 +896,7 @@ int level_tail_launch(const PartArgs* p, const LeafArgs& la, const RecordArgs& r
 // Leaves use the partition's chunk, so their sums are formed exactly as the
 // leaf plan forms them.
 constexpr int kTailMaxNodes = kScanLdsSeg;
+static_assert(kTailMaxNodes == kScanLdsSegMax, "fused levels: the cursor arrays hold kScanLdsSegMax nodes");
 
 bool level_tail_fusable(int64_t n_nodes, int n_eras, int64_t rows_max) {
   const int64_t chunks = ceil_div(rows_max, kPartChunk) + n_nodes;
@@ -933,108 +936,347 @@ __device__ __forceinline__ int tail_map(const LeafArgs& la, const TailMap& m, in
   return j;
 }
 
-// splits: 1 before the last level (decisions from the scan candidates, go-left
-// counts of split nodes), 0 at the last level (every active node a leaf).
+// A chunk's rows in four passes of 256 positions: go-left flags, block
+// counts and in-block ranks.
+struct ChunkRows {
+  static constexpr int U = kPartChunk / 256;
+  int row[U];
+  float2 gh[U];
+  bool left[U], ok[U];
+  int lrank[U], rrank[U];  // rank among the chunk's left (right) rows, in position order
+  int L, R;                // the chunk's left and right rows
+};
+
+// The chunk's row ids and (g, h): issued before the node's decision is
+// known, so that they are in flight meanwhile.
+__device__ __forceinline__ void chunk_load(const PartArgs& a, int c0, int c1, ChunkRows& cr) {
+#pragma unroll
+  for (int u = 0; u < ChunkRows::U; ++u) {
+    const int p = c0 + u * 256 + (int)threadIdx.x;
+    cr.ok[u] = p < c1;
+    const int q = max(min(p, c1 - 1), 0);  // always a valid position (an empty chunk may start at 0)
+    cr.row[u] = a.perm_in[q];
+    cr.gh[u] = a.gh_in[q];
+  }
+}
+
+// Go-left flags, counts and ranks of the loaded rows (split on column col at bin).
+__device__ __forceinline__ void chunk_rows(const PartArgs& a, int col, int bin, ChunkRows& cr) {
+  constexpr int U = ChunkRows::U;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int u = 0; u < U; ++u) cr.left[u] = cr.ok[u] && goes_left(a, cr.row[u], col, bin);
+  __shared__ int s_n[U][4][2];  // [pass][wave] left and right counts
+  uint64_t lm[U], rm[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    lm[u] = __ballot(cr.left[u]);
+    rm[u] = __ballot(cr.ok[u] && !cr.left[u]);
+    if (lane == 0) { s_n[u][wv][0] = __popcll(lm[u]); s_n[u][wv][1] = __popcll(rm[u]); }
+  }
+  __syncthreads();
+  int lb = 0, rb = 0;  // rows of the passes and waves before this lane's
+  cr.L = 0;
+  cr.R = 0;
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+      const int nl = s_n[u][w][0], nr = s_n[u][w][1];
+      if (w == wv) {
+        cr.lrank[u] = lb + lanes_below(lm[u]);
+        cr.rrank[u] = rb + lanes_below(rm[u]);
+      }
+      lb += nl;
+      rb += nr;
+    }
+  }
+  cr.L = lb;
+  cr.R = rb;
+}
+
+// Node j's cursor: one 64-bit word, the left rows claimed in its high half and
+// the right rows in its low half, alone in its 256-byte line (the claims on
+// different nodes go to different cache lines and channels).
+__device__ __forceinline__ unsigned long long tail_claim(const TailCursors& tc, int j, int L, int R) {
+  return atomicAdd(&tc.cur[(int64_t)j * kCurStride], ((unsigned long long)(uint32_t)L << 32) | (uint32_t)R);
+}
+
+__device__ __forceinline__ int tail_count(const unsigned long long* cur, int j, int side) {
+  const unsigned long long v = cur[(int64_t)j * kCurStride];
+  return side ? (int)(uint32_t)v : (int)(v >> 32);
+}
+
+// Single-pass split of one chunk of node j: the chunk claims its place in
+// each child with one atomic add on the node's cursor (left rows fill the
+// node's range from its start, right rows from its end), then writes its
+// rows there in position order.  Which chunk comes first in a
+// child depends on scheduling; nothing downstream depends on the order of a
+// node's rows (histograms and leaf sums are exact integer sums).
+__device__ __forceinline__ void chunk_scatter(const PartArgs& a, const TailCursors& tc, int chunk, int j, int col,
+                                              int bin, ChunkRows& cr) {
+  chunk_rows(a, col, bin, cr);
+  __shared__ int s_base[2];
+  if (threadIdx.x == 0) {
+    const unsigned long long old = tail_claim(tc, j, cr.L, cr.R);
+    s_base[0] = (int)(old >> 32);
+    s_base[1] = (int)(uint32_t)old;
+  }
+  __syncthreads();
+  const int lo = a.node_start[j] + s_base[0];
+  const int hi = a.node_start[j] + a.node_count[j] - s_base[1] - cr.R;  // this chunk's right rows start here
+  StatAcc sl, sr;
+#pragma unroll
+  for (int u = 0; u < ChunkRows::U; ++u) {
+    if (!cr.ok[u]) continue;
+    const int dst = cr.left[u] ? lo + cr.lrank[u] : hi + cr.rrank[u];
+    a.perm_out[dst] = cr.row[u];
+    a.gh_out[dst] = cr.gh[u];
+    if (cr.left[u]) sl.add(cr.gh[u].x, cr.gh[u].y);
+    else sr.add(cr.gh[u].x, cr.gh[u].y);
+  }
+  sl.block_reduce();
+  sr.block_reduce();
+  if (threadIdx.x == 0) {  // one partial per chunk and side, folded per child by k_tail_next
+    sl.store(a.stat_part + ((int64_t)chunk * 2 + 0) * kStatWords);
+    sr.store(a.stat_part + ((int64_t)chunk * 2 + 1) * kStatWords);
+  }
+}
+
+// A chunk of a node whose children are leaves (the last split level): no
+// rows move; the chunk's exact sums of g and h per side (the children's
+// leaf sums, scaled as leaf_partial_rows scales them) and its row counts.
+__device__ __forceinline__ void chunk_child_sums(const PartArgs& a, const LeafArgs& la, const TailCursors& tc,
+                                                 int chunk, int j, int col, int bin, ChunkRows& cr) {
+  chunk_rows(a, col, bin, cr);
+  const LeafShift ls = leaf_shift(la, j);  // children belong to j's tree
+  I128 acc[4];  // left g, left h, right g, right h
+#pragma unroll
+  for (int u = 0; u < ChunkRows::U; ++u) {
+    if (!cr.ok[u]) continue;
+    const int side = cr.left[u] ? 0 : 2;
+    acc[side].add(to_fixed(cr.gh[u].x, ls.sg));
+    acc[side + 1].add(to_fixed(cr.gh[u].y, ls.sh));
+  }
+  __shared__ unsigned long long s_lo[4][256];
+  __shared__ long long s_hi[4][256];
+  const int t = threadIdx.x;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) { s_lo[q][t] = acc[q].lo; s_hi[q][t] = acc[q].hi; }
+  __syncthreads();
+  for (int w = 128; w >= 1; w >>= 1) {
+    if (t < w) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        I128 x, y;
+        x.lo = s_lo[q][t]; x.hi = s_hi[q][t];
+        y.lo = s_lo[q][t + w]; y.hi = s_hi[q][t + w];
+        x.add(y);
+        s_lo[q][t] = x.lo; s_hi[q][t] = x.hi;
+      }
+    }
+    __syncthreads();
+  }
+  if (t < 4) {
+    I128 x;
+    x.lo = s_lo[t][0]; x.hi = s_hi[t][0];
+    tc.child_part[4 * (int64_t)chunk + t] = x;
+  }
+  if (t == 0) tail_claim(tc, j, cr.L, cr.R);  // the children's row counts
+}
+
+// The level's one partition launch: decisions (as k_level_pick forms them),
+// then per chunk a single-pass split, the children's leaf sums (last split
+// level) or, for a leaf, its chunk sums.
 __global__ void __launch_bounds__(256)
-k_tail_count(PartArgs p, LeafArgs la, TailPick pk, TailMap m, int splits) {
+k_tail_part(PartArgs p, LeafArgs la, TailPick pk, TailMap m, TailCursors tc, int last) {
   const int chunk = blockIdx.x;
   int first;
   const int j = tail_map(la, m, chunk, first);
   if (j < 0) return;
   const int k = chunk - first;
   if (threadIdx.x == 0) m.chunk_node[chunk] = j;
-  int fl = -1, bin = -1;
-  if (splits) {
-    __shared__ int s_fb[2];
-    if (threadIdx.x < kWave) {
-      const Cand b = node_best(pk.cand_agree, pk.cand_crit, pk.cand_idx, j, pk.wg_per_node, threadIdx.x);
-      int f_, b_;
-      cand_split(b, pk.n_bins, f_, b_);
-      if (threadIdx.x == 0) {
-        s_fb[0] = f_;
-        s_fb[1] = b_;
-        if (k == 0) {  // one workgroup per node writes the decision
-          pk.dec_feat[j] = f_;
-          pk.dec_bin[j] = b_;
-          if (pk.era_gain) pk.era_gain[j] = f_ >= 0 ? b.crit : 0.0f;  // one era: the winner's gain
-        }
-      }
-    }
-    __syncthreads();
-    fl = s_fb[0];
-    bin = s_fb[1];
-  }
   const int c0 = la.node_start[j] + k * kPartChunk;
   const int c1 = min(c0 + kPartChunk, la.node_start[j] + max(la.node_count[j], 0));
-  if (fl >= 0) {
-    part_count_rows(p, chunk, c0, c1, p.feat_idx ? p.feat_idx[fl] : fl, bin);
-  } else {
+  ChunkRows cr;
+  chunk_load(p, c0, c1, cr);
+  __shared__ int s_fb[2];
+  if (threadIdx.x < kWave) {
+    const Cand b = node_best(pk.cand_agree, pk.cand_crit, pk.cand_idx, j, pk.wg_per_node, threadIdx.x);
+    int f_, b_;
+    cand_split(b, pk.n_bins, f_, b_);
+    if (threadIdx.x == 0) {
+      s_fb[0] = f_;
+      s_fb[1] = b_;
+      if (k == 0) {  // one workgroup per node writes the decision
+        pk.dec_feat[j] = f_;
+        pk.dec_bin[j] = b_;
+        if (pk.era_gain) pk.era_gain[j] = f_ >= 0 ? b.crit : 0.0f;  // one era: the winner's gain
+      }
+    }
+  }
+  __syncthreads();
+  const int fl = s_fb[0], bin = s_fb[1];
+  if (fl < 0) {
     leaf_partial_rows(la, chunk, j, c0, c1);
-    if (splits && threadIdx.x == 0) p.lbase[chunk] = 0;  // a leaf's chunks move no rows
+    return;
   }
+  const int col = p.feat_idx ? p.feat_idx[fl] : fl;
+  if (last) chunk_child_sums(p, la, tc, chunk, j, col, bin, cr);
+  else chunk_scatter(p, tc, chunk, j, col, bin, cr);
 }
 
-// Scatter of split nodes' chunks, leaf score updates, then the tree record
-// (workgroups n_chunks ...).
-__global__ void __launch_bounds__(256)
-k_tail_move(PartArgs p, LeafArgs la, RecordArgs r, TailMap m, int n_chunks, int splits) {
+// Sum of a 128-bit integer over the 64 lanes of a wave (every lane gets it).
+__device__ __forceinline__ I128 wave_sum128(I128 x) {
+  for (int o = 32; o >= 1; o >>= 1) {
+    I128 y;
+    y.lo = ((unsigned long long)(uint32_t)__shfl_xor((int)(uint32_t)(x.lo >> 32), o) << 32) |
+           (uint32_t)__shfl_xor((int)(uint32_t)x.lo, o);
+    y.hi = (long long)(((unsigned long long)(uint32_t)__shfl_xor((int)(uint32_t)((unsigned long long)x.hi >> 32), o)
+                        << 32) |
+                       (uint32_t)__shfl_xor((int)(uint32_t)(unsigned long long)x.hi, o));
+    x.add(y);
+  }
+  return x;
+}
+
+// Tree record of a child of the last split level (child c of node c / 2),
+// one wave per child: a leaf whose sums come from its parent's chunks (the
+// lanes take the chunks, the exact sums add in any order), or nothing.
+__device__ __forceinline__ void record_child(const RecordArgs& r, const PartArgs& p, const LeafArgs& la,
+                                             const TailMap& m, const TailCursors& tc, int c, int lane) {
+  const int j = c >> 1, side = c & 1;
+  const int NH = (2 << r.D) - 1;
+  const int d = r.d + 1;  // the children's level
+  const int cls = c >> d, cc = c & ((1 << d) - 1);
+  const int64_t o = ((int64_t)r.round_ctr[0] * r.K + cls) * NH + ((1 << d) - 1) + cc;
+  const bool leaf = p.node_active[j] && p.dec_feat[j] >= 0;
+  float v = 0.0f;
+  if (leaf) {
+    I128 tg, th;
+    for (int q = m.seg_first[j] + lane; q < m.seg_first[j + 1]; q += kWave) {
+      tg.add(tc.child_part[4 * (int64_t)q + 2 * side]);
+      th.add(tc.child_part[4 * (int64_t)q + 2 * side + 1]);
+    }
+    tg = wave_sum128(tg);
+    th = wave_sum128(th);
+    const LeafShift ls = leaf_shift(la, j);
+    const double sg = ldexp(tg.to_double(), -ls.sg), sh = ldexp(th.to_double(), -ls.sh);
+    v = (float)(sg / (sh + la.lam));
+  }
+  if (lane != 0) return;
+  r.status[o] = leaf ? 2 : 0;
+  r.feat[o] = -1;
+  r.bin[o] = -1;
+  r.value[o] = v;
+  r.count[o] = leaf ? tail_count(tc.cur, j, side) : 0;
+  if (r.out_era_gain) r.out_era_gain[o] = 0.0f;
+}
+
+// After the partition launch: workgroup 0 builds the child tables from the
+// cursors (left child at the node's start, right child after it), the
+// smaller-child build list (left on ties: warpgbm/core.py:1438) and sibling
+// map, clears the next level's cursors and runs the next level's histogram
+// plan; workgroups 1 .. 2n fold the children's statistics; the rest write
+// the level's tree record (and, after the last split level, the leaves').
+__global__ void __launch_bounds__(kPlanThreads)
+k_tail_next(PartArgs p, LeafArgs la, RecordArgs r, TailMap m, TailCursors tc, const PlanJob next, int has_plan,
+            int last) {
+  const int n = p.n_nodes, t = threadIdx.x;
   const int b = blockIdx.x;
-  if (b < n_chunks) {
-    const int j = m.chunk_node[b];
-    if (b >= m.seg_first[la.n_nodes]) return;
-    if (splits && p.dec_feat[j] >= 0) part_scatter_block(p, b);
-    else leaf_apply_node(la, b, j);
-  } else {
-    record_thread(r, la, (int64_t)(b - n_chunks) * 256 + threadIdx.x);
+  const int n_stat = last ? 0 : 2 * n;
+  if (b == 0) {
+    if (last) return;
+    __shared__ int s_cnt[2 * kTailMaxNodes];
+    for (int j = t; j < n; j += blockDim.x) {
+      const int l = 2 * j, rr = 2 * j + 1;
+      const bool sp = p.node_active[j] && p.dec_feat[j] >= 0;
+      const int start = p.node_start[j];
+      const int Lt = sp ? tail_count(tc.cur, j, 0) : 0, Rt = sp ? tail_count(tc.cur, j, 1) : 0;
+      p.c_active[l] = sp; p.c_active[rr] = sp;
+      p.c_node_start[l] = sp ? start : 0; p.c_node_count[l] = Lt;
+      p.c_node_start[rr] = sp ? start + Lt : 0; p.c_node_count[rr] = Rt;
+      p.c_seg_start[l] = p.c_node_start[l]; p.c_seg_count[l] = Lt;  // one era: a node is its segment
+      p.c_seg_start[rr] = p.c_node_start[rr]; p.c_seg_count[rr] = Rt;
+      s_cnt[l] = sp ? Lt : -1;
+      s_cnt[rr] = Rt;
+    }
+    // the next level's cursors (it uses them only if it is fused: at most kTailMaxNodes nodes)
+    for (int q = t; q < min(2 * n, kTailMaxNodes); q += blockDim.x) tc.cur_next[(int64_t)q * kCurStride] = 0ull;
+    __syncthreads();
+    const int sp = t < n && s_cnt[2 * t] >= 0;
+    int ns;
+    const int rank = block_exscan(sp, &ns);
+    if (sp) {
+      const int l = 2 * t, rr = 2 * t + 1;
+      const int built = s_cnt[l] <= s_cnt[rr] ? l : rr;
+      p.sib[3 * rank + 0] = t;
+      p.sib[3 * rank + 1] = built;
+      p.sib[3 * rank + 2] = built ^ 1;
+      p.b_seg_start[rank] = built == l ? p.node_start[t] : p.node_start[t] + s_cnt[l];
+      p.b_seg_count[rank] = s_cnt[built];
+      p.b_seg_slot[rank] = built;
+    }
+    if (t == 0) {
+      *p.n_split = ns;
+      *p.n_build = ns;
+    }
+    __syncthreads();  // the build list is complete (and visible to this workgroup) before its plan
+    if (has_plan) plan_body(next.src, next.chunk_rows, next.seg_first, next.seg_pfirst, next.multi, next.n_multi);
+    return;
+  }
+  if (b <= n_stat) {
+    const int c = b - 1;
+    const int j = c >> 1, side = c & 1;
+    StatAcc acc;
+    if (p.node_active[j] && p.dec_feat[j] >= 0)
+      for (int q = m.seg_first[j] + t; q < m.seg_first[j + 1]; q += blockDim.x)
+        acc.load_merge(p.stat_part + ((int64_t)q * 2 + side) * kStatWords);
+    acc.block_reduce();
+    if (t == 0) acc.store(p.c_stats + (int64_t)c * kStatWords);
+    return;
+  }
+  const int n_rec = (n + blockDim.x - 1) / blockDim.x;  // workgroups of the level's own records
+  const int rb = b - 1 - n_stat;
+  if (rb < n_rec) {
+    const int q = rb * blockDim.x + t;
+    if (q < n) record_thread(r, la, q);
+  } else if (last) {  // a wave per child
+    const int c = (rb - n_rec) * (blockDim.x / kWave) + (t >> 6);
+    if (c < 2 * n) record_child(r, p, la, m, tc, c, t & 63);
   }
 }
 
-int level_tail_fused_launch(const PartArgs* p, const LeafArgs& la, const RecordArgs& r, const TailPick* pick,
-                            const TailMap& m, int64_t rows_max, hipStream_t st, const PlanJob* next) {
-  WGBM_REQUIRE(level_tail_fusable(la.n_nodes, r.E, rows_max), "level tail: not a one-era level of at most %d nodes",
-               kTailMaxNodes);
-  WGBM_REQUIRE(m.seg_first && m.nseg && m.chunk_node && la.seg_first == m.seg_first && la.nseg_dev == m.nseg,
+int level_fused_launch(const PartArgs& p, const LeafArgs& la, const RecordArgs& r, const TailPick& pick,
+                       const TailMap& m, const TailCursors& tc, bool last, int64_t rows_max, hipStream_t st,
+                       const PlanJob* next) {
+  WGBM_REQUIRE(level_tail_fusable(la.n_nodes, r.E, rows_max) && p.n_eras == 1 && p.n_nodes == la.n_nodes &&
+                   r.n_nodes == la.n_nodes && r.d < r.D,
+               "level tail: not a one-era level of at most %d nodes before the last", kTailMaxNodes);
+  WGBM_REQUIRE(m.seg_first && m.nseg && m.chunk_node && la.seg_first == m.seg_first && la.nseg_dev == m.nseg &&
+                   p.chunk_seg == m.chunk_node && tc.cur && tc.cur_next && (!last || tc.child_part),
                "level tail: the leaf tables must be the chunk map's");
-  WGBM_REQUIRE(la.node_active && la.node_start && la.node_count && la.perm && (la.gh || (la.g && la.h)) &&
-                   la.leaf_val && la.part && r.active && r.node_count && r.round_ctr && r.status && r.feat &&
+  WGBM_REQUIRE(pick.cand_agree && pick.cand_crit && pick.cand_idx && pick.wg_per_node >= 1 &&
+                   pick.dec_feat == p.dec_feat && pick.dec_bin == p.dec_bin && r.dec_feat == p.dec_feat &&
+                   r.dec_bin == p.dec_bin && la.dec_feat == p.dec_feat,
+               "level tail: the decisions must go where the partition and the record read them");
+  WGBM_REQUIRE(la.node_active && la.node_start && la.node_count && la.perm && la.gh && la.stats && la.part &&
+                   la.pred == nullptr && p.codes && p.perm_in && p.gh_in && p.perm_out && p.gh_out &&
+                   p.c_seg_start && p.c_seg_count && p.c_active && p.c_node_start && p.c_node_count &&
+                   p.b_seg_start && p.b_seg_count && p.b_seg_slot && p.n_build && p.sib && p.n_split &&
+                   p.c_stats && p.stat_part && r.active && r.node_count && r.round_ctr && r.status && r.feat &&
                    r.bin && r.value && r.count,
-               "level tail: missing leaf or record pointer");
-  if (p) {
-    WGBM_REQUIRE(pick && pick->cand_agree && pick->cand_crit && pick->cand_idx && pick->wg_per_node >= 1 &&
-                     pick->dec_feat == p->dec_feat && pick->dec_bin == p->dec_bin && r.dec_feat == p->dec_feat &&
-                     r.dec_bin == p->dec_bin && la.dec_feat == p->dec_feat,
-                 "level tail: the decisions must go where the partition and the record read them");
-    WGBM_REQUIRE(p->n_eras == 1 && p->n_nodes == la.n_nodes && p->chunk_rows == kPartChunk &&
-                     p->seg_first == m.seg_first && p->nseg_dev == m.nseg && p->chunk_seg == m.chunk_node &&
-                     p->codes && p->perm_in && p->perm_out && p->flag && p->lbase && p->rbase &&
-                     p->c_seg_start && p->c_seg_count && p->c_active && p->c_node_start && p->c_node_count &&
-                     p->b_seg_start && p->b_seg_count && p->b_seg_slot && p->n_build && p->sib && p->n_split &&
-                     p->c_stats && p->stat_part,
-                 "level tail: missing partition table or scratch pointer");
-  }
+               "level tail: missing table or scratch pointer");
+  WGBM_REQUIRE(last || next, "level tail: the next level needs its histogram plan");
   const int64_t n_chunks = ceil_div(rows_max, kPartChunk) + la.n_nodes;
-  const int64_t n_rec = ceil_div((int64_t)r.n_nodes * r.E, 256);
-  hipLaunchKernelGGL(k_tail_count, dim3((unsigned)n_chunks), dim3(256), 0, st, p ? *p : PartArgs{}, la,
-                     pick ? *pick : TailPick{}, m, p ? 1 : 0);
-  WGBM_LAUNCHED("k_tail_count");
-  if (p) {
-    const int rc = part_scan_launch(*p, n_chunks, st);  // the one-workgroup LDS scan (level_tail_fusable)
-    if (rc != kOk) return rc;
-  }
-  hipLaunchKernelGGL(k_tail_move, dim3((unsigned)(n_chunks + n_rec)), dim3(256), 0, st, p ? *p : PartArgs{}, la, r,
-                     m, (int)n_chunks, p ? 1 : 0);
-  WGBM_LAUNCHED("k_tail_move");
-  // Child statistics with the next level's histogram plan; before a last
-  // level that is itself fused, the statistics alone (they scale its leaves'
-  // sums; its chunk map needs no plan).
-  if (p && next) {
-    hipLaunchKernelGGL(k_child_stats_plan, dim3((unsigned)(2 * p->n_nodes + 1)), dim3(kPlanThreads), 0, st, *p,
-                       *next);
-    WGBM_LAUNCHED("k_child_stats_plan");
-  } else if (p) {
-    hipLaunchKernelGGL(k_child_stats, dim3((unsigned)(2 * p->n_nodes)), dim3(256), 0, st, *p);
-    WGBM_LAUNCHED("k_child_stats");
-  }
+  hipLaunchKernelGGL(k_tail_part, dim3((unsigned)n_chunks), dim3(256), 0, st, p, la, pick, m, tc, last ? 1 : 0);
+  WGBM_LAUNCHED("k_tail_part");
+  const int64_t n_stat = last ? 0 : 2 * (int64_t)la.n_nodes;
+  const int64_t n_rec = ceil_div((int64_t)la.n_nodes, kPlanThreads) +
+                        (last ? ceil_div(2 * (int64_t)la.n_nodes, kPlanThreads / kWave) : 0);
+  hipLaunchKernelGGL(k_tail_next, dim3((unsigned)(1 + n_stat + n_rec)), dim3(kPlanThreads), 0, st, p, la, r, m, tc,
+                     next ? *next : PlanJob{}, next ? 1 : 0, last ? 1 : 0);
+  WGBM_LAUNCHED("k_tail_next");
   return kOk;
 }
 
@@ -1226,12 +1468,35 @@ extern "C" size_t wgbm_leaf_workspace_bytes(int64_t n_samples) {
 
 // Leaf value of one node (plain path): sum(g)/(sum(h)+lambda) over its rows
 // (warpgbm/core.py:1377-1380; the sums exact in fixed point as the batched
-// engine forms them), written as float32 to *leaf_out, and
+// engine forms them, scaled by scale_stats -- the statistics of the leaf's
+// tree, as the batched engine scales its leaves -- or when that is NULL by
+// the leaf's own), written as float32 to *leaf_out, and
 // pred[row*stride + col] += f32(lr) * leaf for every row when pred is set.
+extern "C" size_t wgbm_row_stats_workspace_bytes(void) { return 4 * kStatWords * (size_t)kStatBlocks; }
+
+// Statistics of a row list (max |g|, min h, max h, max |h| as order-preserving
+// integers) into out_stats (device, 4 ints): a tree's root rows give the
+// scale of its leaf sums.
+extern "C" int wgbm_row_stats(const float* grads, const float* hess, const int32_t* rows, int64_t n,
+                              int64_t n_total_rows, int32_t* out_stats, void* workspace, size_t workspace_bytes,
+                              hipStream_t stream) {
+  WGBM_REQUIRE(n >= 0 && n < 0x7fffffff, "row count out of range");
+  WGBM_REQUIRE(out_stats && workspace && workspace_bytes >= wgbm_row_stats_workspace_bytes(),
+               "row statistics: missing output or workspace too small");
+  int32_t* part = static_cast<int32_t*>(workspace);
+  const int nsb = stats_blocks(n);
+  hipLaunchKernelGGL(k_list_stats, dim3((unsigned)nsb), dim3(256), 0, stream, rows, (int)n, n_total_rows, grads, hess,
+                     part);
+  WGBM_LAUNCHED("k_list_stats");
+  hipLaunchKernelGGL(k_stats_reduce, dim3(1), dim3(256), 0, stream, part, nsb, out_stats);
+  WGBM_LAUNCHED("k_stats_reduce");
+  return kOk;
+}
+
 extern "C" int wgbm_leaf_value(const float* grads, const float* hess, const int32_t* rows, int64_t n,
                                int64_t n_total_rows, double lambda, float learning_rate, float* pred,
-                               int32_t pred_stride, int32_t pred_col, float* leaf_out, void* workspace,
-                               size_t workspace_bytes, hipStream_t stream) {
+                               int32_t pred_stride, int32_t pred_col, float* leaf_out, const int32_t* scale_stats,
+                               void* workspace, size_t workspace_bytes, hipStream_t stream) {
   WGBM_REQUIRE(n >= 0 && n < 0x7fffffff, "row count out of range");
   WGBM_REQUIRE(workspace_bytes >= wgbm_leaf_workspace_bytes(n), "leaf workspace too small");
   WGBM_REQUIRE(pred == nullptr || (pred_stride >= 1 && pred_col >= 0 && pred_col < pred_stride),
@@ -1260,17 +1525,18 @@ extern "C" int wgbm_leaf_value(const float* grads, const float* hess, const int3
   a.multi = reinterpret_cast<int32_t*>(part + 2 * mc);
   int32_t* stat_part = a.multi + mc;
   int32_t* stats = stat_part + (int64_t)kStatWords * kStatBlocks;
-  a.stats = stats;
+  a.stats = scale_stats ? scale_stats : stats;
   if (pred) a.pred = pred + pred_col;
   hipLaunchKernelGGL(k_leaf_single_init, dim3(1), dim3(64), 0, stream, sc, (int)n);
   WGBM_LAUNCHED("k_leaf_single_init");
-  // the leaf's max |g| and max |h| set the scale of its exact sums
-  const int nsb = stats_blocks(n);
-  hipLaunchKernelGGL(k_list_stats, dim3((unsigned)nsb), dim3(256), 0, stream, rows, (int)n, n_total_rows, grads, hess,
-                     stat_part);
-  WGBM_LAUNCHED("k_list_stats");
-  hipLaunchKernelGGL(k_stats_reduce, dim3(1), dim3(256), 0, stream, stat_part, nsb, stats);
-  WGBM_LAUNCHED("k_stats_reduce");
+  if (!scale_stats) {  // scaled by the leaf's own max |g| and max |h|
+    const int nsb = stats_blocks(n);
+    hipLaunchKernelGGL(k_list_stats, dim3((unsigned)nsb), dim3(256), 0, stream, rows, (int)n, n_total_rows, grads,
+                       hess, stat_part);
+    WGBM_LAUNCHED("k_list_stats");
+    hipLaunchKernelGGL(k_stats_reduce, dim3(1), dim3(256), 0, stream, stat_part, nsb, stats);
+    WGBM_LAUNCHED("k_stats_reduce");
+  }
   return leaf_launch(a, n, stream);
 }
 

@@ -124,20 +124,34 @@ struct TailPick {
   float* era_gain;              // [n_nodes] the winner's gain, or nullptr
 };
 // ... and the level's chunk map, written by its first launch (the leaf
-// tables and, before the last level, the partition's plan tables must be
-// these arrays).
+// tables and the partition's chunk-to-segment map must be these arrays).
 struct TailMap {
   int32_t* seg_first;           // [n_nodes + 1] first chunk of each node
   int32_t* nseg;                // scalar: n_nodes
   int32_t* chunk_node;          // [chunks] node of each chunk
 };
+// Per-node cursors of the single-pass split (node j's word at j *
+// kCurStride: rows claimed in its left child in the high 32 bits and in its
+// right child in the low 32, zero before the level) and the next level's
+// (cleared for it); after the last split level, the children's exact sums
+// per chunk ({left g, left h, right g, right h}).
+constexpr int kCurStride = 32;  // 64-bit words: one 256-byte line per node
+constexpr int kScanLdsSegMax = 512;  // most nodes of a fused level (kTailMaxNodes)
+struct TailCursors {
+  unsigned long long* cur;
+  unsigned long long* cur_next;
+  I128* child_part;
+};
 bool level_tail_fusable(int64_t n_nodes, int n_eras, int64_t rows_max);
-// Three launches (decisions with counts or leaf sums; the partition scan;
-// scatter with leaf updates and the record), then child statistics with
-// `next` (the next level's histogram plan) when given.  p == nullptr at the
-// last level.
-int level_tail_fused_launch(const PartArgs* p, const LeafArgs& la, const RecordArgs& r, const TailPick* pick,
-                            const TailMap& m, int64_t rows_max, hipStream_t st, const PlanJob* next);
+// A one-era level before the last in two launches: decisions with a
+// single-pass split (or, last == true -- the children are leaves -- the
+// children's sums without moving rows), then the child tables, statistics,
+// the next level's histogram plan and the tree record (of the children too
+// when last).  Leaves are recorded, not applied: the round's tree is applied
+// to the scores by walking it (k_apply_tree).
+int level_fused_launch(const PartArgs& p, const LeafArgs& la, const RecordArgs& r, const TailPick& pick,
+                       const TailMap& m, const TailCursors& tc, bool last, int64_t rows_max, hipStream_t st,
+                       const PlanJob* next);
 
 // planned: the plan ran in an earlier fused launch (leaf_plan_job /
 // part_plan_job beside the level's decisions).  next_hist: run the next level's histogram
