@@ -598,6 +598,20 @@ int partition_launch(const PartArgs& a, int64_t rows_max, hipStream_t st, bool p
   return kOk;
 }
 
+// Sum of a 128-bit integer over the 64 lanes of a wave (every lane gets it).
+__device__ __forceinline__ I128 wave_sum128(I128 x) {
+  for (int o = 32; o >= 1; o >>= 1) {
+    I128 y;
+    y.lo = ((unsigned long long)(uint32_t)__shfl_xor((int)(uint32_t)(x.lo >> 32), o) << 32) |
+           (uint32_t)__shfl_xor((int)(uint32_t)x.lo, o);
+    y.hi = (long long)(((unsigned long long)(uint32_t)__shfl_xor((int)(uint32_t)((unsigned long long)x.hi >> 32), o)
+                        << 32) |
+                       (uint32_t)__shfl_xor((int)(uint32_t)(unsigned long long)x.hi, o));
+    x.add(y);
+  }
+  return x;
+}
+
 // ----------------------------------------------------------------- leaves
 // Leaf sums over a node's positions, exact and independent of the order of
 // the rows: each row's g and h become 62-bit fixed-point integers,
@@ -633,42 +647,36 @@ __device__ __forceinline__ void leaf_gh(const LeafArgs& a, int p, double& g, dou
 }
 
 // Exact fixed-point sums of g and h over positions [c0, c1) of leaf j into
-// part[2 * chunk ..].
+// part[2 * chunk ..] (any workgroup of whole waves, at most 1024 threads).
 __device__ __forceinline__ void leaf_partial_rows(const LeafArgs& a, int chunk, int j, int c0, int c1) {
   const LeafShift ls = leaf_shift(a, j);
   I128 sg, sh;
-  for (int p = c0 + threadIdx.x; p < c1; p += 256) {
+  for (int p = c0 + threadIdx.x; p < c1; p += blockDim.x) {
     double g, h;
     leaf_gh(a, p, g, h);
     sg.add(to_fixed((float)g, ls.sg));
     sh.add(to_fixed((float)h, ls.sh));
   }
-  // Fixed-tree workgroup reduction in LDS (halves of each 128-bit sum).
-  __shared__ unsigned long long s_lo[2][256];
-  __shared__ long long s_hi[2][256];
+  // per wave (shuffles), then the waves' sums in LDS
+  sg = wave_sum128(sg);
+  sh = wave_sum128(sh);
+  __shared__ unsigned long long s_lo[2][16];
+  __shared__ long long s_hi[2][16];
   const int t = threadIdx.x;
-  s_lo[0][t] = sg.lo; s_hi[0][t] = sg.hi;
-  s_lo[1][t] = sh.lo; s_hi[1][t] = sh.hi;
-  __syncthreads();
-  for (int w = 128; w >= 1; w >>= 1) {
-    if (t < w) {
-#pragma unroll
-      for (int q = 0; q < 2; ++q) {
-        I128 x, y;
-        x.lo = s_lo[q][t]; x.hi = s_hi[q][t];
-        y.lo = s_lo[q][t + w]; y.hi = s_hi[q][t + w];
-        x.add(y);
-        s_lo[q][t] = x.lo; s_hi[q][t] = x.hi;
-      }
-    }
-    __syncthreads();
+  __syncthreads();  // the arrays may still be read by a previous call
+  if ((t & 63) == 0) {
+    s_lo[0][t >> 6] = sg.lo; s_hi[0][t >> 6] = sg.hi;
+    s_lo[1][t >> 6] = sh.lo; s_hi[1][t >> 6] = sh.hi;
   }
-  if (t == 0) {
-    I128 g, h;
-    g.lo = s_lo[0][0]; g.hi = s_hi[0][0];
-    h.lo = s_lo[1][0]; h.hi = s_hi[1][0];
-    a.part[2 * (int64_t)chunk] = g;
-    a.part[2 * (int64_t)chunk + 1] = h;
+  __syncthreads();
+  if (t < 2) {
+    I128 x;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
+      I128 y;
+      y.lo = s_lo[t][w]; y.hi = s_hi[t][w];
+      x.add(y);
+    }
+    a.part[2 * (int64_t)chunk + t] = x;
   }
 }
 
@@ -886,7 +894,7 @@ int level_tail_launch(const PartArgs* p, const LeafArgs& la, const RecordArgs& r
 
 // ----------------------------------------------------------------- one-era level tail, decisions folded in
 // A one-era level's chunks are mapped to nodes before anything is known
-// about the decisions: active node j owns max(1, ceil(count_j / kPartChunk))
+// about the decisions: active node j owns max(1, ceil(count_j / kTailChunk))
 // chunks, in node order (a node without rows owns one empty chunk, so that
 // its decision and record are written).  Every workgroup of the first launch
 // computes that map itself (n_nodes <= kTailMaxNodes), reduces its node's
@@ -898,12 +906,18 @@ int level_tail_launch(const PartArgs* p, const LeafArgs& la, const RecordArgs& r
 constexpr int kTailMaxNodes = kScanLdsSeg;
 static_assert(kTailMaxNodes == kScanLdsSegMax, "fused levels: the cursor arrays hold kScanLdsSegMax nodes");
 
+// Rows per chunk of a fused level and its workgroup: 4096 rows by 1024
+// threads (four passes), so that a node's chunks -- each claiming its place
+// with one atomic on the node's cursor, serialised at the memory -- are
+// fewer (the root of a million rows: 245 claims, not 977).
+constexpr int kTailChunk = 4096;
+constexpr int kTailThreads = 1024;
+
 bool level_tail_fusable(int64_t n_nodes, int n_eras, int64_t rows_max) {
-  const int64_t chunks = ceil_div(rows_max, kPartChunk) + n_nodes;
-  return n_eras == 1 && n_nodes >= 1 && n_nodes <= kTailMaxNodes && chunks + 1 <= kScanLdsChunks;
+  return n_eras == 1 && n_nodes >= 1 && n_nodes <= kTailMaxNodes && rows_max < 0x7fffffff;
 }
 
-// The chunk map (every thread of a 256-thread workgroup calls it): returns
+// The chunk map (every thread of the workgroup calls it): returns
 // the node owning `chunk` (-1 past the last chunk) and its first chunk.
 // Workgroup 0 stores the first-chunk table and the node count for the later
 // launches of the level.
@@ -914,7 +928,7 @@ __device__ __forceinline__ int tail_map(const LeafArgs& la, const TailMap& m, in
   auto chunks_of = [&](int j) -> int {
     if (j >= n || !la.node_active[j]) return 0;
     const int c = la.node_count[j];
-    return c <= 0 ? 1 : (c + kPartChunk - 1) / kPartChunk;
+    return c <= 0 ? 1 : (c + kTailChunk - 1) / kTailChunk;
   };
   const int j0 = 2 * t, j1 = 2 * t + 1;
   const int n0 = chunks_of(j0), n1 = chunks_of(j1);
@@ -936,10 +950,10 @@ __device__ __forceinline__ int tail_map(const LeafArgs& la, const TailMap& m, in
   return j;
 }
 
-// A chunk's rows in four passes of 256 positions: go-left flags, block
-// counts and in-block ranks.
+// A chunk's rows in four passes of kTailThreads positions: go-left flags,
+// block counts and in-block ranks.
 struct ChunkRows {
-  static constexpr int U = kPartChunk / 256;
+  static constexpr int U = kTailChunk / kTailThreads;
   int row[U];
   float2 gh[U];
   bool left[U], ok[U];
@@ -952,7 +966,7 @@ struct ChunkRows {
 __device__ __forceinline__ void chunk_load(const PartArgs& a, int c0, int c1, ChunkRows& cr) {
 #pragma unroll
   for (int u = 0; u < ChunkRows::U; ++u) {
-    const int p = c0 + u * 256 + (int)threadIdx.x;
+    const int p = c0 + u * kTailThreads + (int)threadIdx.x;
     cr.ok[u] = p < c1;
     const int q = max(min(p, c1 - 1), 0);  // always a valid position (an empty chunk may start at 0)
     cr.row[u] = a.perm_in[q];
@@ -966,31 +980,35 @@ __device__ __forceinline__ void chunk_rows(const PartArgs& a, int col, int bin, 
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
   for (int u = 0; u < U; ++u) cr.left[u] = cr.ok[u] && goes_left(a, cr.row[u], col, bin);
-  __shared__ int s_n[U][4][2];  // [pass][wave] left and right counts
+  constexpr int NW = kTailThreads / kWave;
+  static_assert(U * NW <= kWave, "one wave scans the (pass, wave) counts");
+  __shared__ int s_n[U * NW][2];  // left and right rows of (pass, wave), then their exclusive prefixes
+  __shared__ int s_tot[2];
   uint64_t lm[U], rm[U];
 #pragma unroll
   for (int u = 0; u < U; ++u) {
     lm[u] = __ballot(cr.left[u]);
     rm[u] = __ballot(cr.ok[u] && !cr.left[u]);
-    if (lane == 0) { s_n[u][wv][0] = __popcll(lm[u]); s_n[u][wv][1] = __popcll(rm[u]); }
+    if (lane == 0) { s_n[u * NW + wv][0] = __popcll(lm[u]); s_n[u * NW + wv][1] = __popcll(rm[u]); }
   }
   __syncthreads();
-  int lb = 0, rb = 0;  // rows of the passes and waves before this lane's
-  cr.L = 0;
-  cr.R = 0;
+  if (wv == 0) {  // exclusive scan over (pass, wave) in position order
+    int xl = lane < U * NW ? s_n[lane][0] : 0, xr = lane < U * NW ? s_n[lane][1] : 0;
+    const int vl = xl, vr = xr;
+    for (int o = 1; o < kWave; o <<= 1) {
+      const int yl = __shfl_up(xl, o), yr = __shfl_up(xr, o);
+      if (lane >= o) { xl += yl; xr += yr; }
+    }
+    if (lane < U * NW) { s_n[lane][0] = xl - vl; s_n[lane][1] = xr - vr; }
+    if (lane == kWave - 1) { s_tot[0] = xl; s_tot[1] = xr; }
+  }
+  __syncthreads();
 #pragma unroll
   for (int u = 0; u < U; ++u) {
-#pragma unroll
-    for (int w = 0; w < 4; ++w) {
-      const int nl = s_n[u][w][0], nr = s_n[u][w][1];
-      if (w == wv) {
-        cr.lrank[u] = lb + lanes_below(lm[u]);
-        cr.rrank[u] = rb + lanes_below(rm[u]);
-      }
-      lb += nl;
-      rb += nr;
-    }
+    cr.lrank[u] = s_n[u * NW + wv][0] + lanes_below(lm[u]);
+    cr.rrank[u] = s_n[u * NW + wv][1] + lanes_below(rm[u]);
   }
+  const int lb = s_tot[0], rb = s_tot[1];
   cr.L = lb;
   cr.R = rb;
 }
@@ -1058,28 +1076,24 @@ __device__ __forceinline__ void chunk_child_sums(const PartArgs& a, const LeafAr
     acc[side].add(to_fixed(cr.gh[u].x, ls.sg));
     acc[side + 1].add(to_fixed(cr.gh[u].y, ls.sh));
   }
-  __shared__ unsigned long long s_lo[4][256];
-  __shared__ long long s_hi[4][256];
+  // per wave (shuffles), then the waves' sums in LDS
+  constexpr int NW = kTailThreads / kWave;
+  __shared__ unsigned long long s_lo[4][NW];
+  __shared__ long long s_hi[4][NW];
   const int t = threadIdx.x;
 #pragma unroll
-  for (int q = 0; q < 4; ++q) { s_lo[q][t] = acc[q].lo; s_hi[q][t] = acc[q].hi; }
-  __syncthreads();
-  for (int w = 128; w >= 1; w >>= 1) {
-    if (t < w) {
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        I128 x, y;
-        x.lo = s_lo[q][t]; x.hi = s_hi[q][t];
-        y.lo = s_lo[q][t + w]; y.hi = s_hi[q][t + w];
-        x.add(y);
-        s_lo[q][t] = x.lo; s_hi[q][t] = x.hi;
-      }
-    }
-    __syncthreads();
+  for (int q = 0; q < 4; ++q) {
+    const I128 w = wave_sum128(acc[q]);
+    if ((t & 63) == 0) { s_lo[q][t >> 6] = w.lo; s_hi[q][t >> 6] = w.hi; }
   }
+  __syncthreads();
   if (t < 4) {
     I128 x;
-    x.lo = s_lo[t][0]; x.hi = s_hi[t][0];
+    for (int w = 0; w < NW; ++w) {
+      I128 y;
+      y.lo = s_lo[t][w]; y.hi = s_hi[t][w];
+      x.add(y);
+    }
     tc.child_part[4 * (int64_t)chunk + t] = x;
   }
   if (t == 0) tail_claim(tc, j, cr.L, cr.R);  // the children's row counts
@@ -1088,7 +1102,7 @@ __device__ __forceinline__ void chunk_child_sums(const PartArgs& a, const LeafAr
 // The level's one partition launch: decisions (as k_level_pick forms them),
 // then per chunk a single-pass split, the children's leaf sums (last split
 // level) or, for a leaf, its chunk sums.
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(kTailThreads)
 k_tail_part(PartArgs p, LeafArgs la, TailPick pk, TailMap m, TailCursors tc, int last) {
   const int chunk = blockIdx.x;
   int first;
@@ -1096,8 +1110,8 @@ k_tail_part(PartArgs p, LeafArgs la, TailPick pk, TailMap m, TailCursors tc, int
   if (j < 0) return;
   const int k = chunk - first;
   if (threadIdx.x == 0) m.chunk_node[chunk] = j;
-  const int c0 = la.node_start[j] + k * kPartChunk;
-  const int c1 = min(c0 + kPartChunk, la.node_start[j] + max(la.node_count[j], 0));
+  const int c0 = la.node_start[j] + k * kTailChunk;
+  const int c1 = min(c0 + kTailChunk, la.node_start[j] + max(la.node_count[j], 0));
   ChunkRows cr;
   chunk_load(p, c0, c1, cr);
   __shared__ int s_fb[2];
@@ -1124,20 +1138,6 @@ k_tail_part(PartArgs p, LeafArgs la, TailPick pk, TailMap m, TailCursors tc, int
   const int col = p.feat_idx ? p.feat_idx[fl] : fl;
   if (last) chunk_child_sums(p, la, tc, chunk, j, col, bin, cr);
   else chunk_scatter(p, tc, chunk, j, col, bin, cr);
-}
-
-// Sum of a 128-bit integer over the 64 lanes of a wave (every lane gets it).
-__device__ __forceinline__ I128 wave_sum128(I128 x) {
-  for (int o = 32; o >= 1; o >>= 1) {
-    I128 y;
-    y.lo = ((unsigned long long)(uint32_t)__shfl_xor((int)(uint32_t)(x.lo >> 32), o) << 32) |
-           (uint32_t)__shfl_xor((int)(uint32_t)x.lo, o);
-    y.hi = (long long)(((unsigned long long)(uint32_t)__shfl_xor((int)(uint32_t)((unsigned long long)x.hi >> 32), o)
-                        << 32) |
-                       (uint32_t)__shfl_xor((int)(uint32_t)(unsigned long long)x.hi, o));
-    x.add(y);
-  }
-  return x;
 }
 
 // Tree record of a child of the last split level (child c of node c / 2),
@@ -1268,8 +1268,9 @@ int level_fused_launch(const PartArgs& p, const LeafArgs& la, const RecordArgs& 
                    r.bin && r.value && r.count,
                "level tail: missing table or scratch pointer");
   WGBM_REQUIRE(last || next, "level tail: the next level needs its histogram plan");
-  const int64_t n_chunks = ceil_div(rows_max, kPartChunk) + la.n_nodes;
-  hipLaunchKernelGGL(k_tail_part, dim3((unsigned)n_chunks), dim3(256), 0, st, p, la, pick, m, tc, last ? 1 : 0);
+  const int64_t n_chunks = ceil_div(rows_max, kTailChunk) + la.n_nodes;
+  hipLaunchKernelGGL(k_tail_part, dim3((unsigned)n_chunks), dim3(kTailThreads), 0, st, p, la, pick, m, tc,
+                     last ? 1 : 0);
   WGBM_LAUNCHED("k_tail_part");
   const int64_t n_stat = last ? 0 : 2 * (int64_t)la.n_nodes;
   const int64_t n_rec = ceil_div((int64_t)la.n_nodes, kPlanThreads) +
