@@ -652,3 +652,62 @@ def test_high_dynamic_range_residuals_batched_equals_plain_and_oracle(weighted):
     rec = O.grow_tree(codes, g, ww.astype(np.float64), np.zeros(n, np.int32), 1, 16, 5, 5, 0.0, 1e-6)
     msg = _compare_dict_to_oracle(mb.forest[0], rec.tree, "root")
     assert msg is None, msg
+
+
+@pytest.mark.parametrize("depth,K", [(11, 1), (8, 5)])
+def test_fused_levels_then_stable_levels_batched_equals_plain(depth, K):
+    """One-era levels of at most 512 nodes take the fused single-pass split;
+    deeper levels (1024 nodes and more) the stable partition.  A tree that
+    crosses from one to the other (depth 11; 5 classes at depth 8: levels of
+    5 * 2^d nodes) grows the per-node engine's trees, leaf values and scores
+    bit for bit, and its first tree is the oracle's."""
+    from warpgbm_amd import WarpGBM
+    rng = np.random.default_rng(40 + depth)
+    n, f = 40000, 16
+    X = rng.standard_normal((n, f)).astype(np.float32)
+    if K == 1:
+        y = (np.sin(3 * X[:, 0]) + X[:, 1] * X[:, 2] + 0.1 * rng.standard_normal(n)).astype(np.float32)
+        kw = dict(objective="regression")
+    else:
+        y = np.argmax(X[:, :K] + 0.3 * rng.standard_normal((n, K)), axis=1)
+        kw = dict(objective="multiclass")
+    kw.update(num_bins=32, max_depth=depth, n_estimators=3, learning_rate=0.2)
+    mb = WarpGBM(engine="batched", **kw).fit(X, y)
+    mp = WarpGBM(engine="plain", **kw).fit(X, y)
+    for i in range(3):
+        ta = mb.forest[i] if K == 1 else mb.forest[i]
+        tb = mp.forest[i] if K == 1 else mp.forest[i]
+        if K == 1:
+            msg = _trees_equal(ta, tb, f"tree{i}:")
+            assert msg is None, msg
+        else:
+            for c in range(K):
+                msg = _trees_equal(ta[c], tb[c], f"round{i} class{c}:")
+                assert msg is None, msg
+    if K == 1:
+        assert np.array_equal(mb.predict(X), mp.predict(X))
+        codes, _ = O.bin_matrix(X, 32)
+        g = (y - np.float32(mb.base_prediction)).astype(np.float32)
+        rec = O.grow_tree(codes, g, np.ones_like(g), np.zeros(n, np.int32), 1, 32, depth, 20, 0.0, 1e-6)
+        msg = _compare_dict_to_oracle(mb.forest[0], rec.tree, "root")
+        assert msg is None, msg
+    else:
+        assert np.array_equal(mb.predict_proba(X), mp.predict_proba(X))
+
+
+def test_fused_split_is_deterministic_across_runs():
+    """The fused split writes a child's rows in the order its chunks claim
+    their places, which varies with scheduling; histograms and leaf sums are
+    exact, so repeated fits give the same node tables, leaves and training
+    scores bit for bit."""
+    from warpgbm_amd import WarpGBM
+    X, y = _data_cfg1(n=200000, f=40, seed=9)
+    kw = dict(objective="regression", num_bins=32, max_depth=6, n_estimators=6, learning_rate=0.3)
+    runs = []
+    for _ in range(3):
+        m = WarpGBM(engine="batched", **kw).fit(X, y)
+        runs.append((m._last_tables, m.predict(X)))
+    for t, p in runs[1:]:
+        for key in ("status", "feat", "bin", "value", "count"):
+            assert np.array_equal(t[key], runs[0][0][key]), key
+        assert np.array_equal(p, runs[0][1])
