@@ -41,7 +41,7 @@ struct FitLayout {
   size_t stat_part, round_ctr, fidx_cur;
   size_t p_flag, p_seg_first, p_nseg, p_multi, p_n_multi, p_lbase, p_rbase, p_cscan, p_segL, p_segP, codes_t, p_cseg;
   size_t l_seg_first, l_nseg, l_multi, l_n_multi, l_part, l_val;
-  size_t tree_stats, curs[2], child_part;
+  size_t tree_stats, curs[2], child_part, row_leaf;
   size_t root_seg_start, root_seg_count, root_seg_slot, root_nseg, era_start;
   // sizes
   int64_t KN, max_nodes, node_elems, hist_nodes, part_chunks, leaf_chunks, hist_part_floats;
@@ -144,6 +144,7 @@ static FitLayout fit_layout(const wgbm_fit_config& c) {
   // split cursors of the fused levels (at most kScanLdsSeg nodes), by level parity
   for (int i = 0; i < 2; ++i) L.curs[i] = take(8 * kCurStride * (size_t)std::min<int64_t>(L.max_nodes, kScanLdsSegMax));
   L.child_part = take(sizeof(I128) * 4 * L.part_chunks);  // last split level: children's sums per chunk
+  L.row_leaf = take(2 * (size_t)N);                        // each row's leaf in the round's tree (one class)
   L.l_val = take(4 * L.max_nodes);
   L.root_seg_start = take(4 * K * E);
   L.root_seg_count = take(4 * K * E);
@@ -163,14 +164,15 @@ struct RootArgs {
   int E, K, N;
   int32_t *era_start, *seg_start, *seg_count, *seg_slot, *nseg;
   int32_t *t_seg_start, *t_seg_count, *t_active, *t_node_start, *t_node_count;
-  int32_t* round_ctr;
+  const int32_t* round_ctr;
   const int32_t* feat_all;
   int n_feat;
   int32_t* fidx_cur;
 };
 
-// One workgroup: the round's root segment tables, level-0 node tables and
-// feature subset; advances the device round counter.
+// One workgroup: the root segment tables, level-0 node tables and feature
+// subset of round round_ctr[1] (the round about to be built; the counter
+// itself advances later in the round, in k_root_stats_plan).
 __device__ __forceinline__ void root_tables_block(const RootArgs& a) {
   const int E = a.E, K = a.K, N = a.N;
   const int r = a.round_ctr[1];
@@ -199,11 +201,7 @@ __device__ __forceinline__ void root_tables_block(const RootArgs& a) {
     a.t_node_start[c] = c * N;
     a.t_node_count[c] = run;
   }
-  if (threadIdx.x == 0) {
-    a.round_ctr[0] = r;
-    a.round_ctr[1] = r + 1;
-    *a.nseg = K * E;
-  }
+  if (threadIdx.x == 0) *a.nseg = K * E;
 }
 
 // codes_t[c][r] = codes[r][c], through 64 x 64 LDS tiles (row-contiguous
@@ -277,36 +275,173 @@ k_root_perm(const int32_t* __restrict__ order, int64_t N, int K, int32_t* __rest
   perm[i] = order ? order[p] : (int32_t)p;
 }
 
+// A recorded tree walked by rows: class c's tree of round round_ctr[0] in
+// the node tables (heap order, 2^(D+1) - 1 nodes).  A workgroup stages the
+// tree's nodes in LDS when they fit; rows go left when code <= bin, as the
+// partition sends them, and the walk stops at a leaf.
+struct TreeArgs {
+  const int32_t* status;
+  const int32_t* feat;
+  const int32_t* bin;
+  const float* value;
+  const int32_t* round_ctr;
+  int K, D;
+  const int8_t* codes_t;  // column-major codes [columns][N]
+  int64_t N;
+  float lr;
+  // One class only: each row's leaf (heap index) in the tree, written by
+  // the tree's partition; null to walk the tree from the codes.
+  const int16_t* row_leaf;
+};
+
+constexpr int kApplyRows = 4;      // rows a thread walks together (their code loads in flight at once)
+constexpr int kApplyStage = 2047;  // nodes staged (trees of depth <= 10)
+
+struct TreeStage {
+  int* f;    // per node: split column, -1 leaf, -2 absent
+  int* b;
+  float* v;
+  int64_t o;  // the tree's first node in the tables
+  bool staged;
+};
+
+// Every thread of the workgroup calls it (one barrier when staged).
+__device__ __forceinline__ TreeStage tree_stage(const TreeArgs& t, int c, int* sf, int* sb, float* sv) {
+  const int NH = (2 << t.D) - 1;
+  TreeStage ts{sf, sb, sv, ((int64_t)t.round_ctr[0] * t.K + c) * NH, NH <= kApplyStage};
+  if (ts.staged) {
+    for (int h = threadIdx.x; h < NH; h += blockDim.x) {
+      const int st = t.status[ts.o + h];
+      sf[h] = st == 1 ? t.feat[ts.o + h] : (st == 2 ? -1 : -2);
+      sb[h] = t.bin[ts.o + h];
+      sv[h] = t.value[ts.o + h];
+    }
+    __syncthreads();
+  }
+  return ts;
+}
+
+__device__ __forceinline__ int tree_feat(const TreeArgs& t, const TreeStage& ts, int h) {
+  if (ts.staged) return ts.f[h];
+  const int st = t.status[ts.o + h];
+  return st == 1 ? t.feat[ts.o + h] : (st == 2 ? -1 : -2);
+}
+
+// The leaf value (the raw leaf, before the learning rate) of each of U
+// rows, or 0 with ok false for rows past N or that reach no leaf.
+template <int U>
+__device__ __forceinline__ void tree_walk(const TreeArgs& t, const TreeStage& ts, const int64_t (&row)[U],
+                                          float (&val)[U], bool (&ok)[U]) {
+  if (t.row_leaf) {  // the leaves the partition recorded
+    const int NH = (2 << t.D) - 1;
+    int h[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) h[u] = row[u] < t.N ? (int)t.row_leaf[row[u]] : -1;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      ok[u] = h[u] >= 0 && h[u] < NH && tree_feat(t, ts, h[u]) == -1;
+      val[u] = ok[u] ? (ts.staged ? ts.v[h[u]] : t.value[ts.o + h[u]]) : 0.0f;
+    }
+    return;
+  }
+  int h[U];
+  bool done[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) { h[u] = 0; done[u] = row[u] >= t.N; }
+  for (int d = 0; d < t.D; ++d) {
+    int8_t code[U];
+    int f[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      f[u] = done[u] ? -1 : tree_feat(t, ts, h[u]);
+      code[u] = f[u] >= 0 ? t.codes_t[(int64_t)f[u] * t.N + row[u]] : (int8_t)0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (f[u] >= 0) {
+        const int b = ts.staged ? ts.b[h[u]] : t.bin[ts.o + h[u]];
+        h[u] = 2 * h[u] + 1 + ((int)code[u] <= b ? 0 : 1);
+      } else {
+        done[u] = true;
+      }
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    ok[u] = row[u] < t.N && tree_feat(t, ts, h[u]) == -1;
+    val[u] = ok[u] ? (ts.staged ? ts.v[h[u]] : t.value[ts.o + h[u]]) : 0.0f;
+  }
+}
+
 // Regression gradients in root position order, with one statistics partial
-// per block (grid-stride over positions).
-__device__ __forceinline__ void grad_regression_block(const float* __restrict__ y, const float* __restrict__ pred,
-                                                      const float* __restrict__ w, int64_t n,
-                                                      const int32_t* __restrict__ perm0, float2* __restrict__ gh,
-                                                      int32_t* __restrict__ part, int blk, int nblk) {
+// per block (workgroups [0, nblk), grid-stride over positions, kApplyRows at
+// a time), and the root tables (workgroup nblk).  With apply, the previous
+// round's tree is first added to each row's score (f32(lr) * leaf, the
+// update k_apply_tree makes) -- the walk and the gradient of a row in one
+// pass.
+__global__ void __launch_bounds__(256)
+k_round_start_regression(const float* __restrict__ y, float* __restrict__ pred, const float* __restrict__ w,
+                         int64_t n, const int32_t* __restrict__ perm0, float2* __restrict__ gh,
+                         int32_t* __restrict__ part, int nblk, const TreeArgs ta, int apply, const RootArgs ra) {
+  if ((int)blockIdx.x == nblk) {  // the last workgroup writes the round's root tables
+    root_tables_block(ra);
+    return;
+  }
+  __shared__ int s_f[kApplyStage], s_b[kApplyStage];
+  __shared__ float s_v[kApplyStage];
+  const int blk = blockIdx.x;
   StatAcc acc;
-  for (int64_t p = (int64_t)blk * blockDim.x + threadIdx.x; p < n; p += (int64_t)nblk * blockDim.x) {
-    const int64_t r = perm0[p];
-    const float h = w[r];
-    const float g = (y[r] - pred[r]) * h;
-    gh[p] = make_float2(g, h);
-    acc.add(g, h);
+  const int64_t T = (int64_t)nblk * blockDim.x;
+  int64_t p0 = (int64_t)blk * blockDim.x + threadIdx.x;
+  int64_t row[kApplyRows];
+  auto load_rows = [&]() {
+#pragma unroll
+    for (int u = 0; u < kApplyRows; ++u) {
+      const int64_t p = p0 + u * T;
+      row[u] = p < n ? (int64_t)perm0[p] : n;  // n: no row
+    }
+  };
+  load_rows();  // the first rows' ids are in flight while the tree is staged
+  TreeStage ts{};
+  if (apply) ts = tree_stage(ta, 0, s_f, s_b, s_v);
+  for (; p0 < n; p0 += T * kApplyRows, load_rows()) {
+    // the rows' score, target and weight, loaded before the walk so that
+    // they are in flight with its loads (a position past n reads row 0)
+    float f[kApplyRows], yv[kApplyRows], wv[kApplyRows];
+#pragma unroll
+    for (int u = 0; u < kApplyRows; ++u) {
+      const int64_t r = row[u] < n ? row[u] : 0;
+      f[u] = pred[r];
+      yv[u] = y[r];
+      wv[u] = w[r];
+    }
+    float lv[kApplyRows];
+    bool ok[kApplyRows];
+    if (apply) {
+      tree_walk<kApplyRows>(ta, ts, row, lv, ok);
+    } else {
+#pragma unroll
+      for (int u = 0; u < kApplyRows; ++u) { lv[u] = 0.0f; ok[u] = false; }
+    }
+#pragma unroll
+    for (int u = 0; u < kApplyRows; ++u) {
+      const int64_t p = p0 + u * T;
+      if (p >= n) continue;
+      if (ok[u]) {
+        f[u] = f[u] + ta.lr * lv[u];
+        pred[row[u]] = f[u];
+      }
+      const float g = (yv[u] - f[u]) * wv[u];
+      gh[p] = make_float2(g, wv[u]);
+      acc.add(g, wv[u]);
+    }
   }
   acc.block_reduce();
   if (threadIdx.x == 0) acc.store(part + (int64_t)blk * kStatWords);
 }
 
-// Round start, regression: gradient blocks [0, nblk) and the root tables
-// (last workgroup) in one launch.
-__global__ void __launch_bounds__(256)
-k_round_start_regression(const float* __restrict__ y, const float* __restrict__ pred, const float* __restrict__ w,
-                         int64_t n, const int32_t* __restrict__ perm0, float2* __restrict__ gh,
-                         int32_t* __restrict__ part, int nblk, const RootArgs ra) {
-  if ((int)blockIdx.x == nblk) root_tables_block(ra);
-  else grad_regression_block(y, pred, w, n, perm0, gh, part, blockIdx.x, nblk);
-}
-
-// Round start, multiclass: softmax gradients (blocks [0, nblk)) and the root
-// tables (last workgroup).
+// Round start, multiclass: softmax gradients (workgroups [0, nblk)) and the
+// root tables (workgroup nblk).
 __global__ void __launch_bounds__(256)
 k_round_start_softmax(const float* __restrict__ F, const int32_t* __restrict__ y, const float* __restrict__ w,
                       int64_t n, int K, const int32_t* __restrict__ perm0, float2* __restrict__ gh, int nblk,
@@ -390,15 +525,22 @@ k_loss_final(const double* __restrict__ part, int nb, const int32_t* __restrict_
 // The root statistics of every class (workgroups [0, K)) and the root
 // histogram plan (workgroup K) in one launch of 1024-thread workgroups.
 // The root statistics are also the round's tree statistics (the scale of
-// its leaf sums); the plan workgroup also clears level 0's split cursors.
+// its leaf sums); workgroup 0 advances the round counter (the round start
+// before it reads the previous round's index); workgroup K clears level 0's
+// split cursors and runs the root histogram plan.
 __global__ void __launch_bounds__(1024)
 k_root_stats_plan(const int32_t* __restrict__ part, int n_part, int32_t* __restrict__ out,
                   int32_t* __restrict__ tree_out, int K, const PlanJob j, unsigned long long* __restrict__ cur0,
-                  int n_cur) {
+                  int n_cur, int32_t* __restrict__ round_ctr) {
   if ((int)blockIdx.x == K) {
     for (int q = threadIdx.x; q < n_cur; q += blockDim.x) cur0[(int64_t)q * kCurStride] = 0ull;
     plan_body(j.src, j.chunk_rows, j.seg_first, j.seg_pfirst, j.multi, j.n_multi);
     return;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    const int r = round_ctr[1];
+    round_ctr[0] = r;
+    round_ctr[1] = r + 1;
   }
   const int32_t* p = part + (int64_t)blockIdx.x * n_part * kStatWords;
   StatAcc acc;
@@ -410,76 +552,30 @@ k_root_stats_plan(const int32_t* __restrict__ part, int n_part, int32_t* __restr
   }
 }
 
-// The round's trees added to the scores: thread q takes rows q, q + T, ...
-// (T threads per class, four rows walked together so that their code loads
-// are in flight at once), walks class c's tree of round round_ctr[0] from
-// the root through its split nodes (left when code <= bin, as the partition
-// sends rows) to its leaf, and adds f32(lr) * leaf (each row's one update of
-// the round, the value the per-node engine adds).  The tree's nodes are
-// staged in LDS when they fit.
-constexpr int kApplyRows = 4;
-constexpr int kApplyStage = 2047;  // nodes staged (trees of depth <= 10)
+// The round's trees added to the scores: thread q of class c takes rows
+// q, q + T, ... (T threads per class), kApplyRows walked together, and adds
+// f32(lr) * leaf (each row's one update of the round, the value the
+// per-node engine adds).
 __global__ void __launch_bounds__(256)
-k_apply_tree(const int32_t* __restrict__ status, const int32_t* __restrict__ feat, const int32_t* __restrict__ bin,
-             const float* __restrict__ value, const int32_t* __restrict__ round_ctr, int K, int D,
-             const int8_t* __restrict__ codes_t, int64_t N, float lr, float* __restrict__ scores, int blocks_per_class) {
+k_apply_tree(const TreeArgs ta, float* __restrict__ scores, int blocks_per_class) {
   const int c = blockIdx.x / blocks_per_class;
   const int blk = blockIdx.x - c * blocks_per_class;
-  const int NH = (2 << D) - 1;
-  const int64_t o = ((int64_t)round_ctr[0] * K + c) * NH;
-  __shared__ int s_f[kApplyStage];
-  __shared__ int s_b[kApplyStage];
+  __shared__ int s_f[kApplyStage], s_b[kApplyStage];
   __shared__ float s_v[kApplyStage];
-  const bool staged = NH <= kApplyStage;
-  if (staged) {
-    for (int h = threadIdx.x; h < NH; h += blockDim.x) {
-      const int st = status[o + h];
-      s_f[h] = st == 1 ? feat[o + h] : (st == 2 ? -1 : -2);  // -1 leaf, -2 absent
-      s_b[h] = bin[o + h];
-      s_v[h] = value[o + h];
-    }
-    __syncthreads();
-  }
-  auto node_feat = [&](int h) -> int {
-    if (staged) return s_f[h];
-    const int st = status[o + h];
-    return st == 1 ? feat[o + h] : (st == 2 ? -1 : -2);
-  };
+  const TreeStage ts = tree_stage(ta, c, s_f, s_b, s_v);
   const int64_t T = (int64_t)blocks_per_class * blockDim.x;
-  for (int64_t r0 = (int64_t)blk * blockDim.x + threadIdx.x; r0 < N; r0 += T * kApplyRows) {
+  for (int64_t r0 = (int64_t)blk * blockDim.x + threadIdx.x; r0 < ta.N; r0 += T * kApplyRows) {
     int64_t row[kApplyRows];
-    int h[kApplyRows];
-    bool done[kApplyRows];
+#pragma unroll
+    for (int u = 0; u < kApplyRows; ++u) row[u] = r0 + u * T;
+    float v[kApplyRows];
+    bool ok[kApplyRows];
+    tree_walk<kApplyRows>(ta, ts, row, v, ok);
 #pragma unroll
     for (int u = 0; u < kApplyRows; ++u) {
-      row[u] = r0 + u * T;
-      h[u] = 0;
-      done[u] = row[u] >= N;
-    }
-    for (int d = 0; d < D; ++d) {
-      int8_t code[kApplyRows];
-      int f[kApplyRows];
-#pragma unroll
-      for (int u = 0; u < kApplyRows; ++u) {
-        f[u] = done[u] ? -1 : node_feat(h[u]);
-        code[u] = f[u] >= 0 ? codes_t[(int64_t)f[u] * N + row[u]] : (int8_t)0;
-      }
-#pragma unroll
-      for (int u = 0; u < kApplyRows; ++u) {
-        if (f[u] >= 0) {
-          const int b = staged ? s_b[h[u]] : bin[o + h[u]];
-          h[u] = 2 * h[u] + 1 + ((int)code[u] <= b ? 0 : 1);
-        } else {
-          done[u] = true;
-        }
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < kApplyRows; ++u) {
-      if (row[u] < N && node_feat(h[u]) == -1) {
-        const float v = staged ? s_v[h[u]] : value[o + h[u]];
-        float* d = scores + row[u] * K + c;
-        *d = *d + lr * v;
+      if (ok[u]) {
+        float* d = scores + row[u] * ta.K + c;
+        *d = *d + ta.lr * v[u];
       }
     }
   }
@@ -571,8 +667,55 @@ extern "C" size_t wgbm_fit_workspace_bytes(const wgbm_fit_config* c) {
 // here depends on the round index on the host: the round counter, the
 // feature subset and the output offsets are read on the device, so the
 // launches of one round can be captured once and replayed.
+// The round's trees reach the scores by a tree walk: for regression without
+// a per-round loss, at the start of the next round, in the launch that
+// forms that round's gradients (apply_prev; the call's last round is walked
+// on its own after the rounds, tree_walk_launch); otherwise at the end of
+// the round itself.
+static bool defer_tree_walk(const wgbm_fit_config& c, const wgbm_fit_io* io) {
+  return c.objective == 0 && io->train_loss == nullptr;
+}
+
+// A regression tree whose levels all fuse into their tails: its partition
+// records every row's leaf (a leaf's rows when the leaf is decided, the last
+// split level's rows by side), so the deferred walk reads the leaves instead
+// of the codes.
+static bool row_leaves(const wgbm_fit_config& c, const wgbm_fit_io* io, const FitLayout& L) {
+  if (!defer_tree_walk(c, io) || c.n_classes != 1 || c.max_depth < 1) return false;
+  for (int d = 0; d < c.max_depth; ++d)
+    if (!level_tail_fusable((int64_t)1 << d, c.n_eras, L.KN)) return false;
+  return true;
+}
+
+static TreeArgs tree_args(const wgbm_fit_config& c, const FitLayout& L, const wgbm_fit_io* io, char* ws) {
+  TreeArgs t{};
+  t.status = io->node_status;
+  t.feat = io->node_feat;
+  t.bin = io->node_bin;
+  t.value = io->node_value;
+  t.round_ctr = reinterpret_cast<const int32_t*>(ws + L.round_ctr);
+  t.K = c.n_classes;
+  t.D = c.max_depth;
+  t.codes_t = reinterpret_cast<const int8_t*>(ws + L.codes_t);
+  t.N = c.n_rows;
+  t.lr = c.learning_rate;
+  t.row_leaf = row_leaves(c, io, L) ? reinterpret_cast<const int16_t*>(ws + L.row_leaf) : nullptr;
+  return t;
+}
+
+
+static int tree_walk_launch(const wgbm_fit_config& c, const FitLayout& L, const wgbm_fit_io* io, char* ws,
+                            hipStream_t st) {
+  const int64_t N = c.n_rows;
+  const int bpc = (int)std::min<int64_t>(ceil_div(N, 256 * kApplyRows), 4096);
+  hipLaunchKernelGGL(k_apply_tree, dim3((unsigned)(c.n_classes * bpc)), dim3(256), 0, st, tree_args(c, L, io, ws),
+                     io->scores, bpc);
+  WGBM_LAUNCHED("k_apply_tree");
+  return kOk;
+}
+
 static int enqueue_round(const wgbm_fit_config& c, const FitLayout& L, const wgbm_fit_io* io, char* ws,
-                         hipStream_t st) {
+                         hipStream_t st, bool apply_prev) {
   int rc = kOk;
   auto I = [&](size_t off) { return reinterpret_cast<int32_t*>(ws + off); };
   auto Fp = [&](size_t off) { return reinterpret_cast<float*>(ws + off); };
@@ -580,9 +723,10 @@ static int enqueue_round(const wgbm_fit_config& c, const FitLayout& L, const wgb
   const int K = c.n_classes, E = c.n_eras, D = c.max_depth;
   const int64_t N = c.n_rows;
   const int32_t* fidx = io->feat_idx ? I(L.fidx_cur) : nullptr;
-    // Round start in two launches: gradients with the level-0 tables
-    // (rewritten every round: partition overwrites table 0 at level 1),
-    // then the root statistics with the root histogram plan.
+    // Round start in two launches: gradients (after the previous round's
+    // tree walk when it is deferred), then the root statistics with the
+    // level-0 tables (rewritten every round: the partition overwrites table 0
+    // at level 1) and the root histogram plan.
     RootArgs ra{};
     ra.era_count = io->era_count;
     ra.E = E;
@@ -604,8 +748,9 @@ static int enqueue_round(const wgbm_fit_config& c, const FitLayout& L, const wgb
     ra.fidx_cur = I(L.fidx_cur);
     const int nsb = stats_blocks(N);
     if (c.objective == 0) {
-      hipLaunchKernelGGL(k_round_start_regression, dim3((unsigned)(nsb + 1)), dim3(256), 0, st, io->y, io->scores,
-                         io->weight, N, I(L.perm_root), G2(L.gh_root), I(L.stat_part), nsb, ra);
+      hipLaunchKernelGGL(k_round_start_regression, dim3((unsigned)(nsb + 1)), dim3(256), 0, st, io->y,
+                         io->scores, io->weight, N, I(L.perm_root), G2(L.gh_root), I(L.stat_part), nsb,
+                         tree_args(c, L, io, ws), apply_prev ? 1 : 0, ra);
       WGBM_LAUNCHED("k_round_start_regression");
     } else {
       const int gb = (int)ceil_div(N, 256);
@@ -635,7 +780,8 @@ static int enqueue_round(const wgbm_fit_config& c, const FitLayout& L, const wgb
     }
     hipLaunchKernelGGL(k_root_stats_plan, dim3((unsigned)(K + (D > 0 ? 1 : 0))), dim3(kPlanThreads), 0, st, I(L.stat_part), nsb,
                        I(L.tab[0].stats), I(L.tree_stats), K, root_plan,
-                       reinterpret_cast<unsigned long long*>(ws + L.curs[0]), std::min(K, kScanLdsSegMax));
+                       reinterpret_cast<unsigned long long*>(ws + L.curs[0]), std::min(K, kScanLdsSegMax),
+                       I(L.round_ctr));
     WGBM_LAUNCHED("k_root_stats_plan");
     for (int d = 0; d <= D; ++d) {
       const int n_nodes = K << d;
@@ -858,7 +1004,8 @@ static int enqueue_round(const wgbm_fit_config& c, const FitLayout& L, const wgb
         TailMap m{I(L.p_seg_first), I(L.p_nseg), I(L.p_cseg)};
         TailCursors tc{reinterpret_cast<unsigned long long*>(ws + L.curs[d & 1]),
                        reinterpret_cast<unsigned long long*>(ws + L.curs[(d + 1) & 1]),
-                       reinterpret_cast<I128*>(ws + L.child_part)};
+                       reinterpret_cast<I128*>(ws + L.child_part),
+                       row_leaves(c, io, L) ? reinterpret_cast<int16_t*>(ws + L.row_leaf) : nullptr, (1 << d) - 1};
         const bool last = d + 1 == D;
         rc = level_fused_launch(p, la, r, pk, m, tc, last, L.KN, st, last ? nullptr : &next);
       } else {
@@ -866,12 +1013,9 @@ static int enqueue_round(const wgbm_fit_config& c, const FitLayout& L, const wgb
       }
       if (rc != kOk) return rc;
     }
-    {  // the round's trees, now recorded, added to the scores
-      const int bpc = (int)std::min<int64_t>(ceil_div(N, 256 * kApplyRows), 4096);
-      hipLaunchKernelGGL(k_apply_tree, dim3((unsigned)(K * bpc)), dim3(256), 0, st, io->node_status, io->node_feat,
-                         io->node_bin, io->node_value, I(L.round_ctr), K, D,
-                         reinterpret_cast<const int8_t*>(ws + L.codes_t), N, c.learning_rate, io->scores, bpc);
-      WGBM_LAUNCHED("k_apply_tree");
+    if (!defer_tree_walk(c, io)) {  // the round's trees, now recorded, added to the scores
+      rc = tree_walk_launch(c, L, io, ws, st);
+      if (rc != kOk) return rc;
     }
     if (io->train_loss) {  // the round's training loss, after its last score update
       hipLaunchKernelGGL(k_loss_part, dim3((unsigned)nsb), dim3(256), 0, st, io->y, c.objective == 1 ? io->labels : nullptr,
@@ -945,16 +1089,18 @@ extern "C" int wgbm_fit_rounds(const wgbm_fit_config* cfg, const wgbm_fit_io* io
   g_graph_rounds = 0;
   hipLaunchKernelGGL(k_round_init, dim3(1), dim3(64), 0, st, I(L.round_ctr), 0);
   WGBM_LAUNCHED("k_round_init");
+  const bool defer = defer_tree_walk(c, io);
   if (n_rounds < kGraphMinRounds || hist_timer_on() || !g_graph_enabled) {
     for (int r = 0; r < n_rounds; ++r) {
-      rc = enqueue_round(c, L, io, ws, st);
+      rc = enqueue_round(c, L, io, ws, st, defer && r > 0);
       if (rc != kOk) return rc;
     }
-    return kOk;
+    return defer && n_rounds > 0 ? tree_walk_launch(c, L, io, ws, st) : kOk;  // the last round's trees
   }
   // Round 0 runs eagerly (it also sets kernel attributes); round 1 is
-  // captured into a graph on a side stream and replayed for rounds 1..n-1.
-  rc = enqueue_round(c, L, io, ws, st);
+  // captured into a graph on a side stream and replayed for rounds 1..n-1
+  // (each walking the previous round's trees first when that is deferred).
+  rc = enqueue_round(c, L, io, ws, st, false);
   if (rc != kOk) return rc;
   hipStream_t gs = graph_stream();
   WGBM_REQUIRE(gs != nullptr, "could not create the graph stream");
@@ -966,7 +1112,7 @@ extern "C" int wgbm_fit_rounds(const wgbm_fit_config* cfg, const wgbm_fit_io* io
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   if (hipStreamBeginCapture(gs, hipStreamCaptureModeThreadLocal) == hipSuccess) {
-    const int crc = enqueue_round(c, L, io, ws, gs);
+    const int crc = enqueue_round(c, L, io, ws, gs, defer);
     const hipError_t ce = hipStreamEndCapture(gs, &graph);
     if (crc == kOk && ce == hipSuccess && graph &&
         hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess)
@@ -977,7 +1123,7 @@ extern "C" int wgbm_fit_rounds(const wgbm_fit_config* cfg, const wgbm_fit_io* io
     // Capture is an optimisation: on any failure run the rounds eagerly.
     (void)hipGetLastError();
     for (int r = 1; r < n_rounds; ++r) {
-      rc = enqueue_round(c, L, io, ws, gs);
+      rc = enqueue_round(c, L, io, ws, gs, defer);
       if (rc != kOk) { (void)hipEventDestroy(ev_in); (void)hipEventDestroy(ev_out); return rc; }
     }
   }
@@ -985,6 +1131,15 @@ extern "C" int wgbm_fit_rounds(const wgbm_fit_config* cfg, const wgbm_fit_io* io
     const hipError_t le = hipGraphLaunch(exec, gs);
     if (le != hipSuccess) { (void)hipGraphExecDestroy(exec); WGBM_HIP(le); }
     ++g_graph_rounds;
+  }
+  if (defer) {  // the last round's trees
+    rc = tree_walk_launch(c, L, io, ws, gs);
+    if (rc != kOk) {
+      if (exec) (void)hipGraphExecDestroy(exec);
+      (void)hipEventDestroy(ev_in);
+      (void)hipEventDestroy(ev_out);
+      return rc;
+    }
   }
   WGBM_HIP(hipEventRecord(ev_out, gs));
   WGBM_HIP(hipStreamWaitEvent(st, ev_out, 0));

@@ -1067,6 +1067,12 @@ __device__ __forceinline__ void chunk_scatter(const PartArgs& a, const TailCurso
 __device__ __forceinline__ void chunk_child_sums(const PartArgs& a, const LeafArgs& la, const TailCursors& tc,
                                                  int chunk, int j, int col, int bin, ChunkRows& cr) {
   chunk_rows(a, col, bin, cr);
+  if (tc.row_leaf) {  // heap children 2h + 1 (left) and 2h + 2 (right)
+    const int h = 2 * (tc.node_base + j) + 1;
+#pragma unroll
+    for (int u = 0; u < ChunkRows::U; ++u)
+      if (cr.ok[u]) tc.row_leaf[cr.row[u]] = (int16_t)(h + (cr.left[u] ? 0 : 1));
+  }
   const LeafShift ls = leaf_shift(la, j);  // children belong to j's tree
   I128 acc[4];  // left g, left h, right g, right h
 #pragma unroll
@@ -1132,6 +1138,11 @@ k_tail_part(PartArgs p, LeafArgs la, TailPick pk, TailMap m, TailCursors tc, int
   __syncthreads();
   const int fl = s_fb[0], bin = s_fb[1];
   if (fl < 0) {
+    if (tc.row_leaf) {
+#pragma unroll
+      for (int u = 0; u < ChunkRows::U; ++u)
+        if (cr.ok[u]) tc.row_leaf[cr.row[u]] = (int16_t)(tc.node_base + j);
+    }
     leaf_partial_rows(la, chunk, j, c0, c1);
     return;
   }
@@ -1268,6 +1279,9 @@ int level_fused_launch(const PartArgs& p, const LeafArgs& la, const RecordArgs& 
                    r.bin && r.value && r.count,
                "level tail: missing table or scratch pointer");
   WGBM_REQUIRE(last || next, "level tail: the next level needs its histogram plan");
+  WGBM_REQUIRE(!tc.row_leaf || (la.n_nodes == la.nodes_per_class && tc.node_base == la.n_nodes - 1 &&
+                                4 * (int64_t)la.n_nodes - 2 <= 32767),
+               "level tail: row leaves are recorded for one class, heap indices below 2^15");
   const int64_t n_chunks = ceil_div(rows_max, kTailChunk) + la.n_nodes;
   hipLaunchKernelGGL(k_tail_part, dim3((unsigned)n_chunks), dim3(kTailThreads), 0, st, p, la, pick, m, tc,
                      last ? 1 : 0);

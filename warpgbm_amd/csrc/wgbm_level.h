@@ -141,6 +141,11 @@ struct TailCursors {
   unsigned long long* cur;
   unsigned long long* cur_next;
   I128* child_part;
+  // One class: every row of a node decided a leaf, and every row of the
+  // last split level by side, gets its leaf's heap index here (null: not
+  // recorded).  Node j of the level is heap node node_base + j.
+  int16_t* row_leaf;
+  int node_base;
 };
 bool level_tail_fusable(int64_t n_nodes, int n_eras, int64_t rows_max);
 // A one-era level before the last in two launches: decisions with a
