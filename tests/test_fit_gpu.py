@@ -711,3 +711,31 @@ def test_fused_split_is_deterministic_across_runs():
         for key in ("status", "feat", "bin", "value", "count"):
             assert np.array_equal(t[key], runs[0][0][key]), key
         assert np.array_equal(p, runs[0][1])
+
+
+@pytest.mark.parametrize("case", ["one_era", "eras", "deep", "eval_set"])
+def test_training_scores_after_fit_equal_plain(case):
+    """The scores a fit leaves behind (what a continued fit starts from) are
+    the per-node engine's bit for bit, whichever way the batched engine adds
+    its trees: from the leaves its partition records (one era), by walking
+    the codes (eras; a depth whose deepest levels do not fuse), or at the end
+    of each round (an eval set, whose rounds log the training loss)."""
+    from warpgbm_amd import WarpGBM
+    X, y = _data_cfg1(n=30000, f=40, seed=7)
+    kw = dict(objective="regression", num_bins=32, max_depth=5, n_estimators=9, learning_rate=0.2, random_state=0)
+    fk = {}
+    if case == "eras":
+        fk["era_id"] = np.arange(len(y)) % 3
+    elif case == "deep":
+        kw.update(max_depth=11, min_child_weight=1, n_estimators=3)
+    elif case == "eval_set":
+        fk.update(X_eval=X[:2000], y_eval=y[:2000], eval_every_n_trees=4)
+    mb = WarpGBM(engine="batched", **kw).fit(X, y, **fk)
+    mp = WarpGBM(engine="plain", **kw).fit(X, y, **fk)
+    for i in range(kw["n_estimators"]):
+        msg = _trees_equal(mb.forest[i], mp.forest[i], f"tree{i}:")
+        assert msg is None, msg
+    sb, sp = mb.gradients.cpu().numpy(), mp.gradients.cpu().numpy()
+    assert np.array_equal(sb, sp), float(np.abs(sb - sp).max())
+    # and they are the model's own raw predictions of the training rows
+    np.testing.assert_allclose(sb, mb.predict(X), rtol=1e-5, atol=1e-5)
