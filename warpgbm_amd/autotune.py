@@ -158,8 +158,8 @@ def tune_fit(model, cfg, n_feat, rounds: int = 2, repeats: int = 2, use_cache: b
     B, n_cols = int(cfg.n_bins), int(cfg.n_cols)
     dev = torch.device(model.device)
     key = f"fit2:{n_feat}x{B}x{n_cols}:{'one' if one_plane else 'two'}-plane@{torch.cuda.get_device_name(dev)}"
-    disk = _load_cache() if use_cache else {}
     hit = _MEM.get(key) if use_cache else None
+    disk = _load_cache() if use_cache and hit is None else {}
     if hit is None and use_cache and key in disk:
         hit = dict(disk[key], config=tuple(disk[key]["config"]))
     if hit is not None:

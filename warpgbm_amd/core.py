@@ -581,12 +581,16 @@ class WarpGBM(BaseEstimator, RegressorMixin):
         self._invalidate_prediction_cache()
         self.num_samples, self.num_features = N, F
         self.num_outputs = 1
-        era = np.zeros(N, np.int32) if era_id is None else _era_inverse(era_id)[1].astype(np.int32)
-        self._era_np = era.reshape(-1)
-        self.unique_eras = torch.arange(int(self._era_np.max()) + 1, device=self.device)
+        if era_id is None:  # one era: no host pass over the rows
+            self._era_np = np.zeros(N, np.int32)
+            self.unique_eras = torch.arange(1, device=self.device)
+            self.era_indices = torch.zeros(N, dtype=torch.int32, device=self.device)
+        else:
+            self._era_np = _era_inverse(era_id)[1].astype(np.int32).reshape(-1)
+            self.unique_eras = torch.arange(int(self._era_np.max()) + 1, device=self.device)
+            self.era_indices = torch.from_numpy(self._era_np).to(self.device)
         self.num_eras = len(self.unique_eras)
         self.bin_indices = bin_indices.contiguous()
-        self.era_indices = torch.from_numpy(self._era_np).to(self.device)
         self.Y_gpu = y.to(torch.float32).contiguous()
         w = torch.ones(N, dtype=torch.float32, device=self.device) if sample_weight is None else sample_weight
         self.sample_weight_gpu = w.to(torch.float32).contiguous()
@@ -600,8 +604,11 @@ class WarpGBM(BaseEstimator, RegressorMixin):
             self._root_order = torch.from_numpy(np.argsort(self._era_np, kind="stable").astype(np.int32)).to(self.device)
         else:
             self._root_order = None
-        self._era_count = torch.from_numpy(np.bincount(self._era_np, minlength=self.num_eras).astype(np.int32)).to(
-            self.device)
+        if era_id is None:
+            self._era_count = torch.full((1,), N, dtype=torch.int32, device=self.device)
+        else:
+            self._era_count = torch.from_numpy(np.bincount(self._era_np, minlength=self.num_eras).astype(np.int32)).to(
+                self.device)
         self.root_node_indices = (self._root_order if self._root_order is not None else
                                   torch.arange(N, dtype=torch.int32, device=self.device))
         self.base_prediction = float(self.Y_gpu.double().mean().item())
@@ -1169,16 +1176,15 @@ class WarpGBM(BaseEstimator, RegressorMixin):
         NH = 2 ** (self.max_depth + 1) - 1
         dev = self.device
         shape = (n, K, NH)
-        t = {
-            "status": torch.empty(shape, dtype=torch.int32, device=dev),
-            "feat": torch.empty(shape, dtype=torch.int32, device=dev),
-            "bin": torch.empty(shape, dtype=torch.int32, device=dev),
-            "value": torch.empty(shape, dtype=torch.float32, device=dev),
-            "count": torch.empty(shape, dtype=torch.int32, device=dev),
-            "era_gain": torch.empty(shape + (self.num_eras,), dtype=torch.float32, device=dev),
-        }
+        # All tables in one device buffer, read back with one copy.
+        m = n * K * NH
+        E = self.num_eras
+        buf = torch.empty(m * (5 + E) + (n if with_loss else 0), dtype=torch.int32, device=dev)
+        t = {key: buf[i * m:(i + 1) * m].view(shape) for i, key in enumerate(("status", "feat", "bin", "count"))}
+        t["value"] = buf[4 * m:5 * m].view(torch.float32).view(shape)
+        t["era_gain"] = buf[5 * m:(5 + E) * m].view(torch.float32).view(shape + (E,))
         if with_loss:
-            t["train_loss"] = torch.empty((n,), dtype=torch.float32, device=dev)
+            t["train_loss"] = buf[(5 + E) * m:].view(torch.float32)
         io = _lib.FitIO()
         io.codes = self.bin_indices.data_ptr()
         io.y = self.Y_gpu.data_ptr() if cfg.objective == 0 else None
@@ -1198,7 +1204,13 @@ class WarpGBM(BaseEstimator, RegressorMixin):
         io.train_loss = t["train_loss"].data_ptr() if with_loss else None
         _lib.call("wgbm_fit_rounds", ctypes.byref(cfg), ctypes.byref(io), n, ws.data_ptr(), ws.numel(),
                   _stream(dev))
-        tables = {key: v.cpu().numpy() for key, v in t.items()}
+        host = buf.cpu().numpy()
+        tables = {"status": host[:m].reshape(shape), "feat": host[m:2 * m].reshape(shape),
+                  "bin": host[2 * m:3 * m].reshape(shape), "count": host[3 * m:4 * m].reshape(shape),
+                  "value": host[4 * m:5 * m].view(np.float32).reshape(shape),
+                  "era_gain": host[5 * m:(5 + E) * m].view(np.float32).reshape(shape + (E,))}
+        if with_loss:
+            tables["train_loss"] = host[(5 + E) * m:].view(np.float32)
         self._last_tables = tables
         return tables
 

@@ -128,6 +128,7 @@ k_select_split(const float* __restrict__ gain, const float* __restrict__ dir, in
 // pick_node reduces them (cand_better is a strict total order, so the
 // reduction order is free).
 constexpr int kScanFeat = 4;   // waves per workgroup
+constexpr int kScanBins32 = 32;  // bin count of the vectorised path
 
 __host__ __device__ __forceinline__ int scan_fpw(int n_bins) { return n_bins <= 32 ? 2 : 1; }
 
@@ -159,8 +160,39 @@ k_level_scan(const LevelScanArgs a) {
   const int64_t off = ((int64_t)(node < a.n_nodes ? node : 0) * a.n_feat + fc) * B;
   const float* g = a.G + off;
   const float* h = a.H + off;
+  if (B == kScanBins32) {
+    // 32 bins (two features per wave, lane sl owns threshold sl): every lane
+    // loads its feature's 32 bins with 16-byte loads (the 32 lanes of a
+    // feature read the same lines) and forms the totals and its left sums
+    // itself, in bin order -- the sums the loop below forms, without a
+    // shuffle per bin.
+    const float4* g4 = reinterpret_cast<const float4*>(g);
+    const float4* h4 = reinterpret_cast<const float4*>(h);
+    float gv[kScanBins32], hv[kScanBins32];
+#pragma unroll
+    for (int k = 0; k < kScanBins32 / 4; ++k) {
+      const float4 x = g4[k], y = h4[k];
+      gv[4 * k] = x.x; gv[4 * k + 1] = x.y; gv[4 * k + 2] = x.z; gv[4 * k + 3] = x.w;
+      hv[4 * k] = y.x; hv[4 * k + 1] = y.y; hv[4 * k + 2] = y.z; hv[4 * k + 3] = y.w;
+    }
+    const int b = sl;
+    double Gt = 0.0, Ht = 0.0, GL = 0.0, HL = 0.0;
+#pragma unroll
+    for (int q = 0; q < kScanBins32; ++q) {
+      const double gi = (double)gv[q], hi = (double)hv[q];
+      Gt += gi;
+      Ht += hi;
+      if (q <= b) { GL += gi; HL += hi; }
+    }
+    float ga, di;
+    gain_dir(GL, HL, Gt, Ht, (double)a.mcw, (double)a.lam, ga, di);
+    if (active && f < a.n_feat && b < B - 1) {
+      const Cand k{0, (ga > a.msg && (req == 0 || di == (float)req)) ? ga : -INFINITY, f * (B - 1) + b};
+      if (cand_better(k, best)) best = k;
+    }
+  }
   const bool total_in_lane = B <= LPF;
-  for (int b0 = 0; b0 < B - 1; b0 += LPF) {
+  for (int b0 = 0; B != kScanBins32 && b0 < B - 1; b0 += LPF) {
     const int b = b0 + sl;  // this lane's threshold
     double Gt = 0.0, Ht = 0.0, GL = 0.0, HL = 0.0;
     for (int i0 = 0; i0 < B; i0 += LPF) {
