@@ -267,6 +267,58 @@ k_transpose_codes4(const int8_t* __restrict__ codes, int64_t n_rows, int64_t n_c
   }
 }
 
+// Same transpose over 128 x 128 tiles with 8-byte loads and 16-byte stores
+// (n_cols % 8 == 0, n_rows % 16 == 0, codes 8-byte and codes_t 16-byte
+// aligned): each thread has 64 bytes in flight each way, and eight
+// consecutive threads write one column's 128 rows, a whole line.  Thread
+// (w, g) gathers word column w of rows 16g .. 16g + 15 and writes the four
+// columns 4w .. 4w + 3 of those rows.  Tiles are numbered row tile major
+// and dealt to the XCDs in contiguous runs (workgroup b runs on XCD b % 8),
+// so the column tiles of a row tile -- whose rows share lines when a row
+// is not a whole number of lines -- are read one after another through
+// one L2.
+constexpr int kTrTile = 128;
+constexpr int kTrRowWords = kTrTile / 4 + 1;  // a tile row in LDS, padded by one word
+__global__ void __launch_bounds__(256)
+k_transpose_codes16(const int8_t* __restrict__ codes, int64_t n_rows, int64_t n_cols, int8_t* __restrict__ codes_t,
+                    int64_t n_ctiles, int64_t per_xcd) {
+  __shared__ uint32_t tile[kTrTile * kTrRowWords];
+  const int64_t tid = (int64_t)(blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
+  const int64_t rt = tid / n_ctiles;
+  if (rt * kTrTile >= n_rows) return;  // past the last tile (uniform)
+  const int64_t r0 = rt * kTrTile, c0 = (tid - rt * n_ctiles) * kTrTile;
+  const int t = threadIdx.x;
+  const int ch = t & 15;  // 8-byte chunk of a tile row
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const int r = (t >> 4) + 16 * k;
+    const int64_t rr = r0 + r, cc = c0 + 8 * ch;
+    uint2 v = make_uint2(0u, 0u);
+    if (rr < n_rows && cc < n_cols) v = *reinterpret_cast<const uint2*>(codes + rr * n_cols + cc);
+    tile[r * kTrRowWords + 2 * ch] = v.x;
+    tile[r * kTrRowWords + 2 * ch + 1] = v.y;
+  }
+  __syncthreads();
+  const int w = t >> 3, g = t & 7;
+  uint32_t x[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) x[i] = tile[(16 * g + i) * kTrRowWords + w];
+  const int64_t rr = r0 + 16 * g;
+  if (rr >= n_rows) return;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int64_t cc = c0 + 4 * w + j;
+    if (cc >= n_cols) break;
+    const int sh = 8 * j;
+    uint32_t o[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      o[q] = ((x[4 * q] >> sh) & 0xffu) | (((x[4 * q + 1] >> sh) & 0xffu) << 8) |
+             (((x[4 * q + 2] >> sh) & 0xffu) << 16) | (((x[4 * q + 3] >> sh) & 0xffu) << 24);
+    *reinterpret_cast<uint4*>(codes_t + cc * n_rows + rr) = make_uint4(o[0], o[1], o[2], o[3]);
+  }
+}
+
 __global__ void __launch_bounds__(256)
 k_root_perm(const int32_t* __restrict__ order, int64_t N, int K, int32_t* __restrict__ perm) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1073,7 +1125,15 @@ extern "C" int wgbm_fit_rounds(const wgbm_fit_config* cfg, const wgbm_fit_io* io
                      I(L.perm_root));
   WGBM_LAUNCHED("k_root_perm");
   const bool words = c.n_cols % 4 == 0 && N % 4 == 0 && (uintptr_t)io->codes % 4 == 0;
-  if (words) {
+  const bool wide = c.n_cols % 8 == 0 && N % 16 == 0 && (uintptr_t)io->codes % 8 == 0 &&
+                    (uintptr_t)(ws + L.codes_t) % 16 == 0;
+  if (wide) {
+    const int64_t n_ctiles = ceil_div(c.n_cols, kTrTile);
+    const int64_t per_xcd = ceil_div(ceil_div(N, kTrTile) * n_ctiles, 8);
+    hipLaunchKernelGGL(k_transpose_codes16, dim3((unsigned)(8 * per_xcd)), dim3(256), 0, st, io->codes, N, c.n_cols,
+                       reinterpret_cast<int8_t*>(ws + L.codes_t), n_ctiles, per_xcd);
+    WGBM_LAUNCHED("k_transpose_codes16");
+  } else if (words) {
     hipLaunchKernelGGL(k_transpose_codes4, dim3((unsigned)ceil_div(N, 64), (unsigned)ceil_div(c.n_cols, 64)),
                        dim3(256), 0, st, io->codes, N, c.n_cols, reinterpret_cast<int8_t*>(ws + L.codes_t));
     WGBM_LAUNCHED("k_transpose_codes4");

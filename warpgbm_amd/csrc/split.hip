@@ -176,13 +176,14 @@ k_level_scan(const LevelScanArgs a) {
       hv[4 * k] = y.x; hv[4 * k + 1] = y.y; hv[4 * k + 2] = y.z; hv[4 * k + 3] = y.w;
     }
     const int b = sl;
+    // Running sums over bins 0..q; the left sums are their values at q = b
+    // (the same additions in the same order as summing bins 0..b).
     double Gt = 0.0, Ht = 0.0, GL = 0.0, HL = 0.0;
 #pragma unroll
     for (int q = 0; q < kScanBins32; ++q) {
-      const double gi = (double)gv[q], hi = (double)hv[q];
-      Gt += gi;
-      Ht += hi;
-      if (q <= b) { GL += gi; HL += hi; }
+      Gt += (double)gv[q];
+      Ht += (double)hv[q];
+      if (q == b) { GL = Gt; HL = Ht; }
     }
     float ga, di;
     gain_dir(GL, HL, Gt, Ht, (double)a.mcw, (double)a.lam, ga, di);
