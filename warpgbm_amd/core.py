@@ -1262,25 +1262,19 @@ class WarpGBM(BaseEstimator, RegressorMixin):
             hh = pre[pp]
             np.add.at(self.per_era_feature_importance_.T, ft[rr, cc, hh], eg[rr, cc, hh])
         ids = self._feature_ids
-        views = {}  # 0-d views into the feature-id vector, one per feature
-        # Present slots of all n*K trees, flat index t = (r*K + c)*NH + h;
-        # walked backwards, so children (t + h + 1, t + h + 2) come first.
+        # Slots of all n*K trees by flat index t = (r*K + c)*NH + h: the
+        # leaves first, then the splits from the last slot back, so a split's
+        # children (t + h + 1, t + h + 2) are built before it.
         flat = st[:n].reshape(-1)
-        idx = np.flatnonzero(flat)
-        T, Hs, S = idx.tolist(), (idx % NH).tolist(), flat[idx].tolist()
-        Fs, Bs = ft[:n].reshape(-1)[idx].tolist(), bt[:n].reshape(-1)[idx].tolist()
-        Vs, Ns = vt[:n].reshape(-1)[idx].tolist(), ct[:n].reshape(-1)[idx].tolist()
-        nodes = {}
-        for i in range(len(T) - 1, -1, -1):
-            t, h = T[i], Hs[i]
-            if S[i] == 1:
-                f = Fs[i]
-                fv = views.get(f)
-                if fv is None:
-                    fv = views[f] = ids[f]
-                nodes[t] = {"feature": fv, "bin": Bs[i], "left": nodes[t + h + 1], "right": nodes[t + h + 2]}
-            else:
-                nodes[t] = {"leaf_value": Vs[i], "samples": Ns[i]}
+        nodes = [None] * flat.size
+        lf = np.flatnonzero(flat == 2)
+        for t, v, c in zip(lf.tolist(), vt[:n].reshape(-1)[lf].tolist(), ct[:n].reshape(-1)[lf].tolist()):
+            nodes[t] = {"leaf_value": v, "samples": c}
+        sp = np.flatnonzero(flat == 1)[::-1]
+        fs = ft[:n].reshape(-1)[sp]
+        views = {f: ids[f] for f in np.unique(fs).tolist()}  # 0-d views into the feature-id vector
+        for t, l, f, b in zip(sp.tolist(), (sp + sp % NH + 1).tolist(), fs.tolist(), bt[:n].reshape(-1)[sp].tolist()):
+            nodes[t] = {"feature": views[f], "bin": b, "left": nodes[l], "right": nodes[l + 1]}
         for r in range(n):
             trees = [nodes[(r * K + c) * NH] for c in range(K)]
             if self.objective == "regression":
