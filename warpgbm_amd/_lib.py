@@ -37,7 +37,7 @@ class FitIO(C.Structure):
     ]
 
 
-ABI_VERSION = 3  # wgbm_abi_version() of the library this binding matches
+ABI_VERSION = 4  # wgbm_abi_version() of the library this binding matches
 
 # name -> (restype, argtypes)
 SIGNATURES = {

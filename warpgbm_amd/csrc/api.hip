@@ -17,7 +17,7 @@ void set_error(const char* fmt, ...) {
 
 extern "C" const char* wgbm_last_error(void) { return wgbm::g_err; }
 
-extern "C" int wgbm_abi_version(void) { return 3; }
+extern "C" int wgbm_abi_version(void) { return 4; }
 
 // Name of the GPU architecture the library was built for.
 extern "C" const char* wgbm_build_arch(void) { return "gfx950"; }
