@@ -1030,16 +1030,23 @@ int64_t hist_max_multi(const HistConfig& c, int n_feat, int64_t rows_max) {
   return hist_chunk_cap(c, n_feat, rows_max) + 1;  // each holds more than C rows
 }
 
-size_t hist_partial_words(const HistConfig& c, int n_feat, int n_bins, int64_t rows_max) {
-  return hist_partial_flags(c, n_feat, rows_max) * kWave * c.fpl * n_bins;
+// Multi-chunk segments hold more than C rows each, so their chunks number at
+// most 2 * rows / C <= 2 * cap.
+static int64_t partial_chunks(const HistConfig& c, int n_feat, int64_t rows_max) {
+  return 2 * hist_chunk_cap(c, n_feat, rows_max) + 2;
 }
 
+// Words of the partials: per chunk the configuration's slabs of 64 * FPL
+// features (a launch with fewer features per lane pads less).
+size_t hist_partial_words(const HistConfig& c, int n_feat, int n_bins, int64_t rows_max) {
+  const int64_t padded = ceil_div(n_feat, kWave * c.fpl) * kWave * c.fpl;
+  return (size_t)(partial_chunks(c, n_feat, rows_max) * padded * n_bins);
+}
+
+// Flags of the partials, [partial chunk][slab], for any features per lane
+// (at most one slab per 64 features).
 size_t hist_partial_flags(const HistConfig& c, int n_feat, int64_t rows_max) {
-  const int nslab = (int)ceil_div(n_feat, kWave * c.fpl);
-  // Multi-chunk segments hold more than C rows each, so their chunks number
-  // at most 2 * rows / C <= 2 * cap.
-  const int64_t chunks = 2 * hist_chunk_cap(c, n_feat, rows_max) + 2;
-  return (size_t)chunks * nslab;
+  return (size_t)(partial_chunks(c, n_feat, rows_max) * ceil_div(n_feat > 0 ? n_feat : 1, kWave));
 }
 
 // ---------------------------------------------------------------------------
@@ -1184,6 +1191,41 @@ PlanJob hist_plan_job(const HistArgs& a, const HistConfig& c, int32_t* multi, in
   return j;
 }
 
+// Compute units of the current device (cached per device).
+static int device_cus() {
+  static int cus[64] = {};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+  if (cus[dev] == 0) {
+    int n = 0;
+    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) n = 256;
+    cus[dev] = n;
+  }
+  return cus[dev];
+}
+
+// The launch configuration of a build.  A launch of many small segments -- a
+// deep level of many eras: thousands of (node, era) segments of a few rows,
+// each its own work item -- is bound by each item's chain of dependent loads
+// and its clear and read-out, not by the LDS adds: it runs one wave per work
+// item over one feature per lane (the smallest LDS histogram) and as many
+// workgroups as the CUs hold at once, so that many more items are in flight.
+// The plan, and so every result, is the configuration's own; the partials'
+// words and flags are sized for any features per lane.
+constexpr int64_t kFragSegments = 1024;  // segments of a launch from which it may count as fragmented
+constexpr int64_t kFragRows = 256;       // ... with at most this many rows per segment on average
+static HistConfig fragmented_config(const HistArgs& a, const HistConfig& c, int64_t rows_max, int64_t nseg_max) {
+  if (nseg_max < kFragSegments || rows_max > kFragRows * nseg_max) return c;
+  HistConfig cl = c;
+  cl.fpl = 1;
+  cl.waves = 1;
+  cl.depth = 16;
+  const size_t lds = lds_bytes(cl, a.n_bins, a.hess_const != 0);
+  const int64_t per_cu = std::min<int64_t>((int64_t)((160 * 1024) / (lds + 1024)), 16);
+  cl.slots = (int32_t)std::max<int64_t>(((int64_t)device_cus() * per_cu) & ~(int64_t)7, 8);
+  return cl;
+}
+
 int hist_launch(HistArgs a, const HistConfig& c, int64_t rows_max, int64_t nseg_max,
                 int32_t* multi, int32_t* n_multi, hipStream_t st) {
   WGBM_REQUIRE(!a.sub_g || (a.sub_h && !a.accumulate && a.slots_per_node > 0),
@@ -1194,14 +1236,15 @@ int hist_launch(HistArgs a, const HistConfig& c, int64_t rows_max, int64_t nseg_
                    a.seg_first && a.seg_pfirst && a.nseg_dev && a.out_g && a.out_h && a.part && a.pflag &&
                    a.fine && a.stats && multi && n_multi,
                "histogram: missing table or scratch pointer");
+  const HistConfig cl = fragmented_config(a, c, rows_max, nseg_max);  // the launch's tiling
   a.chunk_rows = c.chunk_rows;
-  a.nslab = (int)ceil_div(a.n_feat, kWave * c.fpl);
+  a.nslab = (int)ceil_div(a.n_feat, kWave * cl.fpl);
   a.h_bits = hess_bits(a.n_rows);
   a.fine_x = fine_extra(a.n_rows);
-  a.wide_ok = (a.row_stride % c.fpl == 0) && ((uintptr_t)a.codes % (uintptr_t)c.fpl == 0);
-  a.lds_pitch = lds_pitch(c, a.n_bins);
-  if (lds_bytes(c, a.n_bins, a.hess_const != 0) > kLdsMax) {
-    set_error("histogram LDS budget exceeded (%d bins, %d waves, %d features per lane)", a.n_bins, c.waves, c.fpl);
+  a.wide_ok = (a.row_stride % cl.fpl == 0) && ((uintptr_t)a.codes % (uintptr_t)cl.fpl == 0);
+  a.lds_pitch = lds_pitch(cl, a.n_bins);
+  if (lds_bytes(cl, a.n_bins, a.hess_const != 0) > kLdsMax) {
+    set_error("histogram LDS budget exceeded (%d bins, %d waves, %d features per lane)", a.n_bins, cl.waves, cl.fpl);
     return kTooBig;
   }
   if (!a.planned) {
@@ -1212,9 +1255,9 @@ int hist_launch(HistArgs a, const HistConfig& c, int64_t rows_max, int64_t nseg_
   const int64_t max_multi = hist_max_multi(c, a.n_feat, rows_max);
   if (a.n_feat == 0) return kOk;
 #define WGBM_D2(FPL, D0, D1)                                                         \
-  return c.depth <= D0 ? launch_fpl<FPL, D0>(a, c, max_chunks, max_multi, multi, n_multi, st) \
-                       : launch_fpl<FPL, D1>(a, c, max_chunks, max_multi, multi, n_multi, st);
-  switch (c.fpl) {
+  return cl.depth <= D0 ? launch_fpl<FPL, D0>(a, cl, max_chunks, max_multi, multi, n_multi, st) \
+                        : launch_fpl<FPL, D1>(a, cl, max_chunks, max_multi, multi, n_multi, st);
+  switch (cl.fpl) {
     case 1: WGBM_D2(1, 16, 32)
     case 2: WGBM_D2(2, 8, 16)
     case 4: WGBM_D2(4, 8, 16)
@@ -1222,7 +1265,7 @@ int hist_launch(HistArgs a, const HistConfig& c, int64_t rows_max, int64_t nseg_
     case 16: WGBM_D2(16, 2, 4)
   }
 #undef WGBM_D2
-  set_error("unsupported features-per-lane %d", c.fpl);
+  set_error("unsupported features-per-lane %d", cl.fpl);
   return kBadArg;
 }
 
