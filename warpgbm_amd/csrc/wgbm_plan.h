@@ -263,11 +263,11 @@ __device__ __forceinline__ void plan_body(const PlanSrc& src, int32_t chunk_rows
   constexpr int nt = kPlanThreads;
   if (src.chunk_out) {
     // Rows per chunk for this launch: enough chunks to give every slot a
-    // work item (chunk_target of them), within [chunk_min, chunk_rows].  Each
-    // non-empty segment may end in a ragged chunk, so the rows are spread over
-    // chunk_target minus that many chunks (at least half the target): the
-    // launch then has at most chunk_target chunks and no slot takes a second
-    // work item after the others have finished.
+    // work item (chunk_target of them), within [chunk_min, chunk_rows].  With
+    // C >= rows / tgt, the live segments' chunks number sum ceil(count / C)
+    // < rows / C + live <= tgt + live, so tgt = chunk_target - live + 1 (at
+    // least half the target) keeps the launch within chunk_target chunks:
+    // no slot takes a second work item after the others have finished.
     __shared__ int s_rows, s_live;
     if (t == 0) { s_rows = 0; s_live = 0; }
     __syncthreads();
@@ -280,7 +280,7 @@ __device__ __forceinline__ void plan_body(const PlanSrc& src, int32_t chunk_rows
     for (int o = 32; o >= 1; o >>= 1) { mine += __shfl_xor(mine, o); live += __shfl_xor(live, o); }
     if (lane == 0) { atomicAdd(&s_rows, mine); atomicAdd(&s_live, live); }
     __syncthreads();
-    const int tgt = max(src.chunk_target - s_live, (src.chunk_target + 1) / 2);
+    const int tgt = max(src.chunk_target - s_live + 1, (src.chunk_target + 1) / 2);
     const int want = (s_rows + tgt - 1) / tgt;
     chunk_rows = min(max((want + 63) & ~63, src.chunk_min), chunk_rows);  // whole 64-row blocks
     if (t == 0) *src.chunk_out = chunk_rows;
