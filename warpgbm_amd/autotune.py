@@ -146,18 +146,23 @@ def tune(n_feat: int, n_bins: int, n_cols: int | None = None, rows: int = 262_14
     return best, times
 
 
-def tune_fit(model, cfg, n_feat, rounds: int = 2, repeats: int = 2, use_cache: bool = True, verbose=False):
+def tune_fit(model, cfg, n_feat, rounds: int = 3, repeats: int = 2, use_cache: bool = True, verbose=False,
+             warm_rounds: int = 8):
     """Pick the histogram configuration of a batched fit and install it for
     the fit's histogram shape.  ``model``: a WarpGBM whose training data are
-    set up on the device; ``cfg``: its wgbm_fit_config.  Each candidate runs
-    ``rounds`` rounds on a copy of the scores (``WarpGBM._trial_rounds_ms``),
-    timed with HIP events around every histogram build and merge; the best
-    of ``repeats`` runs counts.  Returns a dict describing the choice."""
+    set up on the device; ``cfg``: its wgbm_fit_config.  The scores are first
+    advanced ``warm_rounds`` rounds on a copy (fits of at least four times
+    that many trees), so that candidates are timed on trees like the fit's
+    typical ones rather than its first; each candidate then runs ``rounds``
+    rounds from there (``WarpGBM._trial_rounds_ms``), timed with HIP events
+    around every histogram build and merge; the best of ``repeats`` runs
+    counts.  The model's scores are left as they were.  Returns a dict
+    describing the choice."""
     import time
     one_plane = cfg.objective == 0 and cfg.const_weight != 0
     B, n_cols = int(cfg.n_bins), int(cfg.n_cols)
     dev = torch.device(model.device)
-    key = f"fit2:{n_feat}x{B}x{n_cols}:{'one' if one_plane else 'two'}-plane@{torch.cuda.get_device_name(dev)}"
+    key = f"fit3:{n_feat}x{B}x{n_cols}:{'one' if one_plane else 'two'}-plane@{torch.cuda.get_device_name(dev)}"
     hit = _MEM.get(key) if use_cache else None
     disk = _load_cache() if use_cache and hit is None else {}
     if hit is None and use_cache and key in disk:
@@ -171,16 +176,23 @@ def tune_fit(model, cfg, n_feat, rounds: int = 2, repeats: int = 2, use_cache: b
     gen.manual_seed(0)
     _lib.call("wgbm_set_hist_config_shape", n_feat, B, n_cols, int(one_plane), 0, 0, 0, 0, 0)
     auto = current(n_feat, B, n_cols, one_plane)
-    model._trial_rounds_ms(cfg, 1, gen)  # warm-up: first launches of the kernels
+    saved = (model.gradients, getattr(model, "_last_tables", None))
     times = {}
-    for c in [auto] + candidates(n_feat, B, n_cols, one_plane):
-        _lib.call("wgbm_set_hist_config_shape", n_feat, B, n_cols, int(one_plane), *c)
-        c = current(n_feat, B, n_cols, one_plane)  # the configuration in effect (depths are clamped)
-        if c in times:
-            continue
-        times[c] = min(model._trial_rounds_ms(cfg, rounds, gen) for _ in range(repeats)) / rounds
-        if verbose:
-            print(c, f"{times[c]:.3f} ms per round")
+    try:
+        if warm_rounds > 0 and model.n_estimators >= 4 * warm_rounds:
+            model.gradients = saved[0].clone()
+            model._advance_rounds(cfg, warm_rounds, gen)
+        model._trial_rounds_ms(cfg, 1, gen)  # warm-up: first launches of the kernels
+        for c in [auto] + candidates(n_feat, B, n_cols, one_plane):
+            _lib.call("wgbm_set_hist_config_shape", n_feat, B, n_cols, int(one_plane), *c)
+            c = current(n_feat, B, n_cols, one_plane)  # the configuration in effect (depths are clamped)
+            if c in times:
+                continue
+            times[c] = min(model._trial_rounds_ms(cfg, rounds, gen) for _ in range(repeats)) / rounds
+            if verbose:
+                print(c, f"{times[c]:.3f} ms per round")
+    finally:
+        model.gradients, model._last_tables = saved
     best = min(times, key=times.get)
     _lib.call("wgbm_set_hist_config_shape", n_feat, B, n_cols, int(one_plane), *best)
     res = {"config": tuple(best), "ms_per_round": times[best], "auto_config": tuple(auto),

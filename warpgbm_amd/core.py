@@ -1214,6 +1214,18 @@ class WarpGBM(BaseEstimator, RegressorMixin):
         self._last_tables = tables
         return tables
 
+    def _advance_rounds(self, cfg, n_rounds, feats_gen=None):
+        """Boost n_rounds on the current scores, discarding the trees: the
+        state from which the autotuner times its candidates (a fit's trees a
+        few rounds in, not its first)."""
+        ws = _fit_workspace(self.device, _lib.query("wgbm_fit_workspace_bytes", ctypes.byref(cfg)))
+        feats = None
+        if cfg.n_feat < self.num_features:
+            feats = torch.stack([torch.randperm(self.num_features, generator=feats_gen, device=self.device,
+                                                dtype=torch.int32)[:cfg.n_feat] for _ in range(n_rounds)])
+            feats = feats.contiguous()
+        self._run_rounds(cfg, ws, n_rounds, feats)
+
     def _trial_rounds_ms(self, cfg, n_rounds, feats_gen=None):
         """Histogram time (build + merge launches, ms) of n_rounds boosting
         rounds run on a copy of the scores: the model is left as it was."""
