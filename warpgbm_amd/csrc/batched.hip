@@ -605,9 +605,9 @@ k_root_stats_plan(const int32_t* __restrict__ part, int n_part, int32_t* __restr
 }
 
 // The round's trees added to the scores: thread q of class c takes rows
-// q, q + T, ... (T threads per class), kApplyRows walked together, and adds
-// f32(lr) * leaf (each row's one update of the round, the value the
-// per-node engine adds).
+// q, q + T, ... (T threads per class), kApplyRows walked together (or
+// their recorded leaves read), and adds f32(lr) * leaf (each row's one
+// update of the round, the value the per-node engine adds).
 __global__ void __launch_bounds__(256)
 k_apply_tree(const TreeArgs ta, float* __restrict__ scores, int blocks_per_class) {
   const int c = blockIdx.x / blocks_per_class;
@@ -863,7 +863,7 @@ static int enqueue_round(const wgbm_fit_config& c, const FitLayout& L, const wgb
       la.n_rows = N;
       la.lam = c.lambda;
       la.lr = c.learning_rate;
-      la.pred = nullptr;  // leaves are recorded; k_apply_tree adds the round's trees to the scores
+      la.pred = nullptr;  // leaves are recorded; the round's trees reach the scores by the tree walk
       la.pred_stride = K;
       la.nodes_per_class = 1 << d;
       la.leaf_val = Fp(L.l_val);

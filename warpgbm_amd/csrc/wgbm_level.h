@@ -152,8 +152,9 @@ bool level_tail_fusable(int64_t n_nodes, int n_eras, int64_t rows_max);
 // single-pass split (or, last == true -- the children are leaves -- the
 // children's sums without moving rows), then the child tables, statistics,
 // the next level's histogram plan and the tree record (of the children too
-// when last).  Leaves are recorded, not applied: the round's tree is applied
-// to the scores by walking it (k_apply_tree).
+// when last).  Leaves are recorded, not applied: the round's tree reaches the
+// scores afterwards, from the row leaves tc.row_leaf records (when set) or
+// by walking it over the codes.
 int level_fused_launch(const PartArgs& p, const LeafArgs& la, const RecordArgs& r, const TailPick& pick,
                        const TailMap& m, const TailCursors& tc, bool last, int64_t rows_max, hipStream_t st,
                        const PlanJob* next);
