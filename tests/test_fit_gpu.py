@@ -713,19 +713,25 @@ def test_fused_split_is_deterministic_across_runs():
         assert np.array_equal(p, runs[0][1])
 
 
-@pytest.mark.parametrize("case", ["one_era", "eras", "deep", "eval_set"])
+@pytest.mark.parametrize("case", ["one_era", "early_leaves", "depth10", "eras", "deep", "eval_set"])
 def test_training_scores_after_fit_equal_plain(case):
     """The scores a fit leaves behind (what a continued fit starts from) are
     the per-node engine's bit for bit, whichever way the batched engine adds
-    its trees: from the leaves its partition records (one era), by walking
-    the codes (eras; a depth whose deepest levels do not fuse), or at the end
-    of each round (an eval set, whose rounds log the training loss)."""
+    its trees: from the leaves its partition records (one era; with leaves
+    decided above the last level; at depth 10, the deepest whose levels all
+    fuse), by walking the codes (eras; a depth whose deepest levels do not
+    fuse), or at the end of each round (an eval set, whose rounds log the
+    training loss)."""
     from warpgbm_amd import WarpGBM
     X, y = _data_cfg1(n=30000, f=40, seed=7)
     kw = dict(objective="regression", num_bins=32, max_depth=5, n_estimators=9, learning_rate=0.2, random_state=0)
     fk = {}
     if case == "eras":
         fk["era_id"] = np.arange(len(y)) % 3
+    elif case == "early_leaves":
+        kw.update(max_depth=6, min_child_weight=2500)  # nodes under 5000 rows stop splitting
+    elif case == "depth10":
+        kw.update(max_depth=10, min_child_weight=1, n_estimators=4)
     elif case == "deep":
         kw.update(max_depth=11, min_child_weight=1, n_estimators=3)
     elif case == "eval_set":
@@ -735,6 +741,10 @@ def test_training_scores_after_fit_equal_plain(case):
     for i in range(kw["n_estimators"]):
         msg = _trees_equal(mb.forest[i], mp.forest[i], f"tree{i}:")
         assert msg is None, msg
+    if case == "early_leaves":  # the case exercises leaves recorded above the last split level
+        def leaf_depths(t, d=0):
+            return [d] if "leaf_value" in t else leaf_depths(t["left"], d + 1) + leaf_depths(t["right"], d + 1)
+        assert min(min(leaf_depths(t)) for t in mb.forest) < kw["max_depth"] - 1
     sb, sp = mb.gradients.cpu().numpy(), mp.gradients.cpu().numpy()
     assert np.array_equal(sb, sp), float(np.abs(sb - sp).max())
     # and they are the model's own raw predictions of the training rows
