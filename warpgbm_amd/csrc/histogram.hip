@@ -31,6 +31,7 @@
 #include "wgbm_internal.h"
 #include "wgbm_stats.h"
 #include "wgbm_plan.h"
+#include <hip/hip_ext.h>
 #include <algorithm>
 #include <mutex>
 #include <type_traits>
@@ -1051,7 +1052,9 @@ size_t hist_partial_flags(const HistConfig& c, int n_feat, int64_t rows_max) {
 
 // ---------------------------------------------------------------------------
 // Live kernel timing: when enabled, every histogram build and merge launch
-// is bracketed by HIP events on its own stream.
+// carries a pair of HIP events on its stream, recorded by the launch itself
+// (hipExtLaunchKernel: the kernel's own start and end, not the queue's
+// packets around it).
 struct KernelTimer {
   bool on = false;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> build, merge;
@@ -1060,19 +1063,20 @@ static KernelTimer g_timer;
 
 bool hist_timer_on() { return g_timer.on; }
 
-static void timer_mark(std::vector<std::pair<hipEvent_t, hipEvent_t>>& v, bool start, hipStream_t st) {
-  if (!g_timer.on) return;
-  if (start) {
-    hipEvent_t a = nullptr, b = nullptr;
-    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
-      g_timer.on = false;  // timing is best effort: stop rather than record garbage
-      return;
-    }
-    v.emplace_back(a, b);
-    (void)hipEventRecord(a, st);
-  } else if (!v.empty()) {
-    (void)hipEventRecord(v.back().second, st);
+// A new event pair for one timed launch (false: timing is off).
+static bool timer_pair(std::vector<std::pair<hipEvent_t, hipEvent_t>>& v, hipEvent_t& a, hipEvent_t& b) {
+  a = nullptr;
+  b = nullptr;
+  if (!g_timer.on) return false;
+  if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
+    if (a) (void)hipEventDestroy(a);
+    a = nullptr;
+    b = nullptr;
+    g_timer.on = false;  // timing is best effort: stop rather than record garbage
+    return false;
   }
+  v.emplace_back(a, b);
+  return true;
 }
 
 static void timer_clear() {
@@ -1097,15 +1101,22 @@ static int launch_waves(const HistArgs& a, const HistConfig& c, int64_t groups, 
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsMax));
     if (dev >= 0 && dev < 64) opted[dev] = kLdsMax;
   }
-  timer_mark(g_timer.build, true, st);
-  if (a.gh)
+  hipEvent_t e0, e1;
+  if (timer_pair(g_timer.build, e0, e1)) {
+    if (a.gh)
+      hipExtLaunchKernelGGL((k_hist_build<FPL, L, true, CNT, WIDE>), dim3((unsigned)groups), dim3(64 * c.waves), lds,
+                            st, e0, e1, 0, a);
+    else
+      hipExtLaunchKernelGGL((k_hist_build<FPL, L, false, CNT, WIDE>), dim3((unsigned)groups), dim3(64 * c.waves), lds,
+                            st, e0, e1, 0, a);
+  } else if (a.gh) {
     hipLaunchKernelGGL((k_hist_build<FPL, L, true, CNT, WIDE>), dim3((unsigned)groups), dim3(64 * c.waves), lds, st,
                        a);
-  else
+  } else {
     hipLaunchKernelGGL((k_hist_build<FPL, L, false, CNT, WIDE>), dim3((unsigned)groups), dim3(64 * c.waves), lds, st,
                        a);
+  }
   WGBM_LAUNCHED("k_hist_build");
-  timer_mark(g_timer.build, false, st);
   return kOk;
 }
 
@@ -1145,10 +1156,13 @@ static int launch_fpl(const HistArgs& a, const HistConfig& c, int64_t max_chunks
     const int SL = kWave * FPL * a.n_bins;
     const int64_t merge_items = max_multi * a.nslab * ceil_div(SL, kMergeThreads / kMergeMaxParts);
     const int64_t mg = std::min<int64_t>((merge_items + 7) & ~(int64_t)7, 2048);  // a multiple of 8
-    timer_mark(g_timer.merge, true, st);
-    hipLaunchKernelGGL((k_hist_merge<FPL>), dim3((unsigned)mg), dim3(kMergeThreads), 0, st, a, n_multi);
+    hipEvent_t e0, e1;
+    if (timer_pair(g_timer.merge, e0, e1))
+      hipExtLaunchKernelGGL((k_hist_merge<FPL>), dim3((unsigned)mg), dim3(kMergeThreads), 0, st, e0, e1, 0, a,
+                            n_multi);
+    else
+      hipLaunchKernelGGL((k_hist_merge<FPL>), dim3((unsigned)mg), dim3(kMergeThreads), 0, st, a, n_multi);
     WGBM_LAUNCHED("k_hist_merge");
-    timer_mark(g_timer.merge, false, st);
   }
   return kOk;
 }
