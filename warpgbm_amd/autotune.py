@@ -146,7 +146,7 @@ def tune(n_feat: int, n_bins: int, n_cols: int | None = None, rows: int = 262_14
     return best, times
 
 
-def tune_fit(model, cfg, n_feat, rounds: int = 3, repeats: int = 2, use_cache: bool = True, verbose=False,
+def tune_fit(model, cfg, n_feat, rounds: int = 2, repeats: int = 3, use_cache: bool = True, verbose=False,
              warm_rounds: int = 8):
     """Pick the histogram configuration of a batched fit and install it for
     the fit's histogram shape.  ``model``: a WarpGBM whose training data are
@@ -154,9 +154,11 @@ def tune_fit(model, cfg, n_feat, rounds: int = 3, repeats: int = 2, use_cache: b
     advanced ``warm_rounds`` rounds on a copy (fits of at least four times
     that many trees), so that candidates are timed on trees like the fit's
     typical ones rather than its first; each candidate then runs ``rounds``
-    rounds from there (``WarpGBM._trial_rounds_ms``), timed with HIP events
-    around every histogram build and merge; the best of ``repeats`` runs
-    counts.  The model's scores are left as they were.  Returns a dict
+    rounds from there in one call (``WarpGBM._trial_rounds_ms``; fewer than
+    three rounds launch eagerly, no graph to capture), timed on the device
+    as a whole: every launch of the call, not only the histogram ones, since
+    a tiling also moves the cost of the merges.  The best of ``repeats``
+    runs counts.  The model's scores are left as they were.  Returns a dict
     describing the choice."""
     import time
     one_plane = cfg.objective == 0 and cfg.const_weight != 0

@@ -1171,7 +1171,9 @@ class WarpGBM(BaseEstimator, RegressorMixin):
                         self.compute_eval_multiclass(i + r, train_loss=loss)
             i += n
 
-    def _run_rounds(self, cfg, ws, n, feats, with_loss=False):
+    def _run_rounds(self, cfg, ws, n, feats, with_loss=False, read_back=True):
+        """n boosting rounds in one wgbm_fit_rounds call; their node tables
+        (host arrays), or None without read_back."""
         K = cfg.n_classes
         NH = 2 ** (self.max_depth + 1) - 1
         dev = self.device
@@ -1204,6 +1206,8 @@ class WarpGBM(BaseEstimator, RegressorMixin):
         io.train_loss = t["train_loss"].data_ptr() if with_loss else None
         _lib.call("wgbm_fit_rounds", ctypes.byref(cfg), ctypes.byref(io), n, ws.data_ptr(), ws.numel(),
                   _stream(dev))
+        if not read_back:
+            return None
         host = buf.cpu().numpy()
         tables = {"status": host[:m].reshape(shape), "feat": host[m:2 * m].reshape(shape),
                   "bin": host[2 * m:3 * m].reshape(shape), "count": host[3 * m:4 * m].reshape(shape),
@@ -1227,8 +1231,9 @@ class WarpGBM(BaseEstimator, RegressorMixin):
         self._run_rounds(cfg, ws, n_rounds, feats)
 
     def _trial_rounds_ms(self, cfg, n_rounds, feats_gen=None):
-        """Histogram time (build + merge launches, ms) of n_rounds boosting
-        rounds run on a copy of the scores: the model is left as it was."""
+        """Device time (ms) of one wgbm_fit_rounds call of n_rounds boosting
+        rounds on a copy of the scores, between events on the fit's stream:
+        the model is left as it was."""
         ws = _fit_workspace(self.device, _lib.query("wgbm_fit_workspace_bytes", ctypes.byref(cfg)))
         feats = None
         if cfg.n_feat < self.num_features:
@@ -1237,14 +1242,16 @@ class WarpGBM(BaseEstimator, RegressorMixin):
             feats = feats.contiguous()
         saved, last = self.gradients, getattr(self, "_last_tables", None)
         self.gradients = saved.clone()
-        out = (ctypes.c_double * 4)()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
         try:
-            _lib.call("wgbm_profile_begin")
-            self._run_rounds(cfg, ws, n_rounds, feats)
+            e0.record()
+            self._run_rounds(cfg, ws, n_rounds, feats, read_back=False)
+            e1.record()
+            e1.synchronize()
         finally:
-            _lib.call("wgbm_profile_end", out)
             self.gradients, self._last_tables = saved, last
-        return out[0] + out[2]
+        return e0.elapsed_time(e1)
 
     def _eval_add_rounds(self, tables, n, K):
         """Add the n new rounds' trees to the device eval scores: the rounds'
