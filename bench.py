@@ -9,7 +9,7 @@ engine on bin codes already resident in HBM):
 Prints ONE JSON line: value = seconds per batched fit (lower is better),
 beside the plain per-node engine's fit time on the same data (and whether
 the two produce identical trees), the histogram kernel's achieved HBM
-bandwidth (HIP events around every histogram launch of one fit), the
+bandwidth (HIP events attached to every histogram launch of one fit), the
 predict latency of 10k-row requests on the resident compiled forest, and
 the oracle timed on the host cores for a bounded sample.
 """
@@ -560,7 +560,7 @@ def main():
     dt = (time.perf_counter() - t0) / args.steps
     log(f"batched fit: {dt:.3f} s per {args.trees}-tree fit")
 
-    # histogram kernel: HIP events around every build/merge launch of one fit
+    # histogram kernel: HIP events attached to every build/merge launch of one fit
     _lib.call("wgbm_profile_begin")
     mp_ = fit("batched")
     out4 = (__import__("ctypes").c_double * 4)()
